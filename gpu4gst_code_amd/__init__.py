@@ -1,0 +1,287 @@
+"""MI355X group Steiner tree (DPBF) solver: Python binding of the C-ABI in include/gst.h.
+
+The solver itself is native: libgst.so (host C++ + gfx950 HIP kernels).  This module only moves
+numpy arrays across the ABI for tests, benches and scripts.  Every GPU entry point raises when
+the library or the device is missing; there is no CPU fallback.
+
+    import gpu4gst_code_amd as gst
+    G = gst.gen_barabasi_albert(200_000, 5, 1, 1000, seed=1)
+    groups = gst.gen_groups(G.n, 6, 10, 200, seed=1000)
+    with gst.Solver() as s:
+        dg = s.upload(G)
+        r = s.solve(dg, groups)
+        print(r.cost, r.edges)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import (GST_MAX_GROUPS, GST_W_FLOAT32, GST_W_INT32, STATUS_NAMES)
+
+__all__ = ["HostGraph", "Groups", "Result", "Solver", "GstError", "gen_random_connected",
+           "gen_barabasi_albert", "gen_rmat", "gen_groups", "groups_from_lists", "validate_tree",
+           "GST_MAX_GROUPS", "GST_W_INT32", "GST_W_FLOAT32"]
+
+
+class GstError(RuntimeError):
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        self.name = STATUS_NAMES.get(status, f"status {status}")
+        super().__init__(f"{self.name}: {message}" if message else self.name)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data if a.size else 0
+
+
+@dataclass
+class HostGraph:
+    """Symmetric CSR on the host (row_ptr int64, col int32, w int32 or float32)."""
+    row_ptr: np.ndarray
+    col: np.ndarray
+    w: np.ndarray
+
+    def __post_init__(self):
+        self.row_ptr = np.ascontiguousarray(self.row_ptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(self.col, dtype=np.int32)
+        if self.w.dtype == np.float32:
+            self.w = np.ascontiguousarray(self.w, dtype=np.float32)
+        else:
+            self.w = np.ascontiguousarray(self.w, dtype=np.int32)
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptr.shape[0] - 1)
+
+    @property
+    def m(self) -> int:
+        return int(self.row_ptr[-1])
+
+    @property
+    def w_type(self) -> int:
+        return GST_W_FLOAT32 if self.w.dtype == np.float32 else GST_W_INT32
+
+    def as_c(self) -> _abi.gst_csr:
+        return _abi.gst_csr(self.n, self.m, _ptr(self.row_ptr), _ptr(self.col), _ptr(self.w), self.w_type)
+
+    @staticmethod
+    def from_edges(n: int, eu, ev, w) -> "HostGraph":
+        """Undirected edge list -> symmetric CSR (rows sorted by neighbour id)."""
+        eu = np.asarray(eu, dtype=np.int64)
+        ev = np.asarray(ev, dtype=np.int64)
+        w = np.asarray(w)
+        src = np.concatenate([eu, ev])
+        dst = np.concatenate([ev, eu])
+        ww = np.concatenate([w, w])
+        order = np.lexsort((dst, src))
+        src, dst, ww = src[order], dst[order], ww[order]
+        row = np.zeros(n + 1, dtype=np.int64)
+        np.add.at(row, src + 1, 1)
+        row = np.cumsum(row)
+        return HostGraph(row, dst.astype(np.int32), ww.astype(np.float32 if w.dtype.kind == "f" else np.int32))
+
+
+@dataclass
+class Groups:
+    """g groups as a CSR of member lists; group i is bit i of the DP masks."""
+    grp_ptr: np.ndarray
+    grp_vtx: np.ndarray
+
+    def __post_init__(self):
+        self.grp_ptr = np.ascontiguousarray(self.grp_ptr, dtype=np.int32)
+        self.grp_vtx = np.ascontiguousarray(self.grp_vtx, dtype=np.int32)
+
+    @property
+    def g(self) -> int:
+        return int(self.grp_ptr.shape[0] - 1)
+
+    def members(self, i: int) -> np.ndarray:
+        return self.grp_vtx[self.grp_ptr[i]:self.grp_ptr[i + 1]]
+
+    def as_c(self) -> _abi.gst_groups:
+        return _abi.gst_groups(self.g, _ptr(self.grp_ptr), _ptr(self.grp_vtx))
+
+    def masks(self, n: int) -> np.ndarray:
+        """Per-vertex group bitmask (uint32)."""
+        gm = np.zeros(n, dtype=np.uint32)
+        for i in range(self.g):
+            gm[self.members(i)] |= np.uint32(1 << i)
+        return gm
+
+
+def groups_from_lists(lists: Sequence[Sequence[int]]) -> Groups:
+    ptr = np.zeros(len(lists) + 1, dtype=np.int32)
+    for i, l in enumerate(lists):
+        ptr[i + 1] = ptr[i] + len(l)
+    vtx = np.concatenate([np.asarray(l, dtype=np.int32) for l in lists]) if lists else np.zeros(0, np.int32)
+    return Groups(ptr, vtx)
+
+
+@dataclass
+class Result:
+    cost: float
+    tree_cost: float
+    root: int
+    eu: np.ndarray
+    ev: np.ndarray
+    ew: np.ndarray
+    used_ub: bool
+    fell_back: bool
+    stats: dict = field(default_factory=dict)
+
+    @property
+    def edges(self):
+        return list(zip(self.eu.tolist(), self.ev.tolist(), self.ew.tolist()))
+
+
+def _check(lib, status: int, ctx=None, what: str = ""):
+    if status != _abi.GST_OK:
+        msg = lib.gst_last_error(ctx).decode() if ctx else what
+        raise GstError(status, msg)
+
+
+def _take_host_csr(lib, h: _abi.gst_host_csr) -> HostGraph:
+    n, m = h.n, h.m
+    row = np.ctypeslib.as_array(h.row_ptr, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(h.col, shape=(max(m, 1),))[:m].copy()
+    if h.w_type == GST_W_FLOAT32:
+        w = np.ctypeslib.as_array(C.cast(h.w, C.POINTER(C.c_float)), shape=(max(m, 1),))[:m].copy()
+    else:
+        w = np.ctypeslib.as_array(C.cast(h.w, C.POINTER(C.c_int32)), shape=(max(m, 1),))[:m].copy()
+    lib.gst_host_csr_free(C.byref(h))
+    return HostGraph(row, col, w)
+
+
+def gen_random_connected(n: int, n_edges: int, wlo: int, whi: int, seed: int) -> HostGraph:
+    """Random spanning tree plus uniform extra edges (no duplicates, no self-loops), int weights."""
+    lib = _abi.load()
+    h = _abi.gst_host_csr()
+    _check(lib, lib.gst_gen_random_connected(n, n_edges, wlo, whi, seed & (2**64 - 1), C.byref(h)),
+           what="gen_random_connected")
+    return _take_host_csr(lib, h)
+
+
+def gen_barabasi_albert(n: int, m: int, wlo: int, whi: int, seed: int) -> HostGraph:
+    lib = _abi.load()
+    h = _abi.gst_host_csr()
+    _check(lib, lib.gst_gen_barabasi_albert(n, m, wlo, whi, seed & (2**64 - 1), C.byref(h)),
+           what="gen_barabasi_albert")
+    return _take_host_csr(lib, h)
+
+
+def gen_rmat(scale: int, edge_factor: int, a=0.57, b=0.19, c=0.19, *, weights="int", wlo=1, whi=1000,
+             mu=0.0, sigma=1.0, seed: int = 0) -> HostGraph:
+    lib = _abi.load()
+    h = _abi.gst_host_csr()
+    wt = GST_W_FLOAT32 if weights == "lognormal" else GST_W_INT32
+    _check(lib, lib.gst_gen_rmat(scale, edge_factor, a, b, c, wt, wlo, whi, mu, sigma, seed & (2**64 - 1),
+                                 C.byref(h)), what="gen_rmat")
+    return _take_host_csr(lib, h)
+
+
+def gen_groups(n: int, g: int, smin: int, smax: int, seed: int) -> Groups:
+    lib = _abi.load()
+    ptr = np.zeros(g + 1, dtype=np.int32)
+    vtx = np.zeros(g * min(smax, n), dtype=np.int32)
+    _check(lib, lib.gst_gen_groups(n, g, smin, smax, seed & (2**64 - 1), _ptr(ptr), _ptr(vtx)), what="gen_groups")
+    return Groups(ptr, vtx[:ptr[-1]].copy())
+
+
+def validate_tree(graph: HostGraph, groups: Groups, eu, ev, root: int = -1) -> float:
+    """Raise GstError unless (eu, ev) is a tree of the graph touching every group; return its cost."""
+    lib = _abi.load()
+    eu = np.ascontiguousarray(eu, dtype=np.int32)
+    ev = np.ascontiguousarray(ev, dtype=np.int32)
+    cost = C.c_double(0.0)
+    csr, grp = graph.as_c(), groups.as_c()
+    st = lib.gst_validate_tree(C.byref(csr), C.byref(grp), int(eu.shape[0]), _ptr(eu), _ptr(ev), int(root),
+                               C.byref(cost))
+    _check(lib, st, what="validate_tree")
+    return cost.value
+
+
+class DeviceGraph:
+    def __init__(self, solver: "Solver", handle: int, host: HostGraph):
+        self.solver = solver
+        self.handle = handle
+        self.host = host
+
+    @property
+    def n(self) -> int:
+        return self.host.n
+
+    def free(self):
+        if self.handle:
+            self.solver.lib.gst_graph_free(self.handle)
+            self.handle = None
+
+
+class Solver:
+    """One gst_ctx on one device (default device 0)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        ctx = C.c_void_p()
+        st = self.lib.gst_ctx_create(device, C.byref(ctx))
+        if st != _abi.GST_OK:
+            msg = self.lib.gst_last_error(ctx).decode() if ctx.value else ""
+            if ctx.value:
+                self.lib.gst_ctx_destroy(ctx)
+            raise GstError(st, msg)
+        self.ctx = ctx
+        self._graphs = []
+
+    def close(self):
+        for g in self._graphs:
+            g.free()
+        self._graphs = []
+        if self.ctx:
+            self.lib.gst_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, graph: HostGraph) -> DeviceGraph:
+        h = C.c_void_p()
+        csr = graph.as_c()
+        _check(self.lib, self.lib.gst_graph_upload(self.ctx, C.byref(csr), C.byref(h)), self.ctx)
+        dg = DeviceGraph(self, h.value, graph)
+        self._graphs.append(dg)
+        return dg
+
+    def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
+              want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0) -> Result:
+        opts = _abi.gst_opts()
+        opts.ub = float(ub) if ub is not None else 0.0
+        opts.prune_lb = 1 if prune_lb else 0
+        opts.want_tree = 1 if want_tree else 0
+        opts.rounds_per_sync = rounds_per_sync
+        opts.chunk_arcs = chunk_arcs
+        res = _abi.gst_result()
+        grp = groups.as_c()
+        st = self.lib.gst_solve(self.ctx, graph.handle, C.byref(grp), C.byref(opts), C.byref(res))
+        try:
+            _check(self.lib, st, self.ctx)
+            ne = res.n_edges
+            if ne > 0:
+                eu = np.ctypeslib.as_array(res.eu, shape=(ne,)).copy()
+                ev = np.ctypeslib.as_array(res.ev, shape=(ne,)).copy()
+                ew = np.ctypeslib.as_array(res.ew, shape=(ne,)).copy()
+            else:
+                eu = np.zeros(0, np.int32)
+                ev = np.zeros(0, np.int32)
+                ew = np.zeros(0, np.float64)
+            stats = {name: getattr(res.stats, name) for name, _ in _abi.gst_stats._fields_ if name != "pad0"}
+            return Result(res.cost, res.tree_cost, res.root, eu, ev, ew, bool(res.used_ub), bool(res.fell_back),
+                          stats)
+        finally:
+            self.lib.gst_result_free(C.byref(res))

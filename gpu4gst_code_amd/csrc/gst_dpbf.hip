@@ -1,0 +1,1096 @@
+// DPBF group Steiner tree solve on one MI355X (gfx950): kernels and the per-query driver.
+//
+// Formulation (Ding et al. ICDE'07 DPBF, as restated from
+// rucgraph/graph_v_of_v_idealID_DPBF_only_ec.h:133-324 and the dense shape of src/DP.cu:102-215):
+//   T(v,X) = 0                                   if v belongs to every group in X   (leaf)
+//   T(v,X) = min_{X1 ⊂ X} T(v,X1) + T(v,X\X1)                                        (merge)
+//   T(v,X) = min_u T(u,X) + w(u,v)                                                    (grow)
+// Masks are processed level by level (level k = popcount k).  Level k: one merge kernel fills
+// T(.,X) for every X of the level from the final lower levels, then frontier grow rounds run the
+// multi-source shortest-path closure of all C(g,k) masks of the level together until no cost
+// drops.  The full mask needs no grow: min_v T(v,full) cannot drop by growing.  The tree is
+// traced on the device from the packed predecessors.
+//
+// HBM layout of the state table: one u64 word per (v,X); level k is a block [V][m_k] (m_k =
+// C(g,k) masks in increasing mask order), so the C(g,k) masks a grow relaxation touches at a
+// neighbour are one contiguous run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "gst_internal.h"
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return gst_set_error(ctx, GST_HIP_ERROR, "%s failed: %s (%s:%d)", #call,         \
+                                 hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int GROW_BLOCK = 256;              // 4 waves
+constexpr int GROW_WAVES = GROW_BLOCK / WAVE;
+constexpr int MERGE_BLOCK = 64;              // one wave per workgroup (its LDS row is private)
+constexpr int TRACE_STACK = 512;
+
+// device-side control block of the grow rounds of one level
+struct RoundCtl {
+    uint32_t round;
+    uint32_t blocks_done;
+    uint32_t cnt[3];        // items in list r%3
+    uint32_t level_done;
+    uint32_t err;           // bit0: int cost overflow, bit1: traceback failure
+    uint32_t pad;
+};
+
+enum StatIdx {
+    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_MVERT, ST_MPAIRS, ST_NSTAT
+};
+
+struct GrowArgs {
+    uint64_t* Tk;          // level-k block
+    const uint64_t* T1;    // level-1 block (group distances, for the lower bound)
+    const uint32_t* masks; // masks of level k by rank
+    const uint32_t* row;
+    const int32_t* col;
+    const uint32_t* w;
+    const uint32_t* rev;
+    uint64_t* list[3];
+    uint64_t* chg[3];
+    uint32_t* inq[3];
+    RoundCtl* ctl;
+    unsigned long long* stats;
+    uint32_t m;            // masks at this level
+    uint32_t W;            // 64-bit bitmap words per vertex
+    uint32_t g;
+    uint32_t chunk;        // arcs per work item
+    uint32_t ub;           // cost bits of the bound (GST_INF_COST: none)
+    uint32_t use_lb;
+};
+
+struct MergeArgs {
+    uint64_t* T;           // whole table
+    const uint64_t* pairs; // pairs of this level
+    const uint32_t* masks; // all masks by row position
+    const uint32_t* gm;    // group mask per vertex
+    const uint32_t* row;
+    uint64_t* list0;
+    uint64_t* chg0;
+    uint32_t* inq0;
+    RoundCtl* ctl;
+    unsigned long long* root;
+    unsigned long long* stats;
+    uint64_t lvl_base[GST_MAX_GROUPS + 2];  // word offset of each level block
+    uint32_t m[GST_MAX_GROUPS + 2];
+    uint32_t off[GST_MAX_GROUPS + 2];
+    uint32_t n;
+    uint32_t g;
+    uint32_t k;
+    uint32_t P;            // pairs at this level
+    uint32_t W;            // bitmap words at this level
+    uint32_t chunk;
+    uint32_t ub;
+    uint32_t use_lb;
+    uint32_t is_full;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+template <bool FLT>
+__device__ __forceinline__ uint32_t add_cost(uint32_t a, uint32_t b, uint32_t& ovf) {
+    if (a == GST_INF_COST || b == GST_INF_COST) return GST_INF_COST;
+    if (FLT) {
+        return __float_as_uint(__fadd_rn(__uint_as_float(a), __uint_as_float(b)));
+    } else {
+        const uint64_t s = (uint64_t)a + (uint64_t)b;
+        if (s >= (uint64_t)GST_INF_COST) { ovf = 1; return GST_INF_COST; }
+        return (uint32_t)s;
+    }
+}
+
+// a + b >= ub, exact for ints, in float arithmetic for floats
+template <bool FLT>
+__device__ __forceinline__ bool reaches_ub(uint32_t a, uint32_t b, uint32_t ub) {
+    if (ub == GST_INF_COST) return a == GST_INF_COST || b == GST_INF_COST;
+    if (a == GST_INF_COST || b == GST_INF_COST) return true;
+    if (FLT) return __fadd_rn(__uint_as_float(a), __uint_as_float(b)) >= __uint_as_float(ub);
+    return (uint64_t)a + (uint64_t)b >= (uint64_t)ub;
+}
+
+// position of the t-th (0-based) set bit of x; t < popcount(x)
+__device__ __forceinline__ uint32_t select_bit(uint64_t x, uint32_t t) {
+    uint32_t lo = 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const uint32_t mid = lo + s - 1;
+        const uint64_t msk = (mid >= 63) ? ~0ull : ((2ull << mid) - 1ull);
+        if ((uint32_t)__popcll(x & msk) <= t) lo += s;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) v |= __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_min64(unsigned long long v) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        unsigned long long o = __shfl_xor(v, d, WAVE);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) v += __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+// Wave-aggregated append of n work items (v, chunk 0..n-1) per lane: one atomicAdd per wave,
+// exclusive prefix sum over lanes.  Must be called by all lanes of the wave (convergent).
+__device__ __forceinline__ void wave_enqueue(uint64_t* list, uint32_t* cnt, uint32_t v,
+                                             uint32_t n, uint32_t& enq_stat) {
+    const uint32_t lane = lane_id();
+    uint32_t incl = n;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, WAVE);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+    if (total == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(cnt, total);
+    base = __shfl(base, 0, WAVE);
+    uint32_t pos = base + incl - n;
+    for (uint32_t c = 0; c < n; ++c) list[pos + c] = (uint64_t)v | ((uint64_t)c << 32);
+    enq_stat += n;
+}
+
+__device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, uint32_t chunk) {
+    const uint32_t d = row[v + 1] - row[v];
+    return (d + chunk - 1) / chunk;
+}
+
+// ---------------------------------------------------------------------------------------
+// small utility kernels (fills instead of hipMemset so that every step can be graph-captured)
+// ---------------------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t val, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t val, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+__global__ void k_set_group_bits(uint32_t* gm, const int32_t* grp_ptr, const int32_t* grp_vtx, int g) {
+    for (int i = 0; i < g; ++i) {
+        const int s = grp_ptr[i], e = grp_ptr[i + 1];
+        for (int j = s + blockIdx.x * blockDim.x + threadIdx.x; j < e; j += gridDim.x * blockDim.x)
+            atomicOr(&gm[grp_vtx[j]], 1u << i);
+    }
+}
+
+__global__ void k_ctl_reset(RoundCtl* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->round = 0; ctl->blocks_done = 0;
+        ctl->cnt[0] = 0; ctl->cnt[1] = 0; ctl->cnt[2] = 0;
+        ctl->level_done = 0;
+    }
+}
+
+// Level 1: T(v,{i}) = 0 for members of group i, inf elsewhere; members seed the frontier.
+__global__ void k_init_level1(uint64_t* T1, const uint32_t* gm, const uint32_t* row, uint32_t n,
+                              uint32_t g, uint32_t chunk, uint64_t* list0, uint64_t* chg0,
+                              uint32_t* inq0, RoundCtl* ctl, unsigned long long* stats,
+                              int seed_frontier) {
+    const uint32_t lane = lane_id();
+    uint32_t enq = 0;
+    const uint64_t tid0 = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = tid0; base < n; base += stride) {  // wave-uniform trip count
+        const uint64_t v = base + threadIdx.x;
+        uint32_t nitems = 0;
+        if (v < n) {
+            const uint32_t gmv = gm[v];
+            for (uint32_t i = 0; i < g; ++i)
+                T1[v * g + i] = ((gmv >> i) & 1u) ? (uint64_t)GST_PRED_LEAF : GST_INF_WORD;
+            if (gmv && seed_frontier) {
+                nitems = num_chunks(row, (uint32_t)v, chunk);
+                if (nitems) { chg0[v] = gmv; inq0[v] = 1; }
+            }
+        }
+        wave_enqueue(list0, &ctl->cnt[0], (uint32_t)v, nitems, enq);
+    }
+    (void)lane;
+    enq = wave_sum32(enq);
+    if (lane_id() == 0 && enq) atomicAdd(&stats[ST_ENQ], (unsigned long long)enq);
+}
+
+// ---------------------------------------------------------------------------------------
+// grow: one round of frontier relaxation for all masks of one level
+// ---------------------------------------------------------------------------------------
+template <bool FLT>
+__global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
+    RoundCtl* ctl = a.ctl;
+    if (ctl->level_done) return;
+    const uint32_t r = ctl->round;
+    const uint32_t cur = r % 3, nxt = (r + 1) % 3, prv = (r + 2) % 3;
+    const uint32_t n_cur = ctl->cnt[cur];
+    const uint32_t n_prv = ctl->cnt[prv];
+    const uint32_t lane = lane_id();
+    const uint32_t W = a.W, m = a.m;
+
+    // phase A: clear the change bitmaps / queue flags of round r-1's items
+    {
+        const uint64_t* lp = a.list[prv];
+        uint64_t* cp = a.chg[prv];
+        uint32_t* ip = a.inq[prv];
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_prv; i += gridDim.x * blockDim.x) {
+            const uint64_t it = lp[i];
+            if ((it >> 32) == 0) {
+                const uint32_t v = (uint32_t)it;
+                for (uint32_t w = 0; w < W; ++w) cp[(uint64_t)v * W + w] = 0;
+                ip[v] = 0;
+            }
+        }
+    }
+
+    // phase B: relax the items of round r
+    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0;
+    uint32_t ovf = 0;
+    const uint32_t gwave = blockIdx.x * GROW_WAVES + (threadIdx.x / WAVE);
+    const uint32_t nwaves = gridDim.x * GROW_WAVES;
+    const uint64_t* lcur = a.list[cur];
+    const uint64_t* ccur = a.chg[cur];
+    uint64_t* cnxt = a.chg[nxt];
+    uint32_t* inxt = a.inq[nxt];
+    uint64_t* lnxt = a.list[nxt];
+    uint32_t* cntn = &ctl->cnt[nxt];
+    const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
+
+    for (uint32_t i = gwave; i < n_cur; i += nwaves) {
+        const uint64_t it = lcur[i];
+        const uint32_t v = (uint32_t)it;
+        const uint32_t c = (uint32_t)(it >> 32);
+        const uint32_t rs = a.row[v], re = a.row[v + 1];
+        const uint32_t e0 = rs + c * a.chunk;
+        const uint32_t e1 = min(re, e0 + a.chunk);
+        st_items++;
+        st_arcs += e1 - e0;
+        // level-1 distances of v for the lower bound: every lane gets dist(v, G_i) for all i
+        uint32_t dall[GST_MAX_GROUPS];
+        if (prune_lb) {
+            const uint32_t d1 = lane < a.g ? (uint32_t)(a.T1[(uint64_t)v * a.g + lane] >> 32) : GST_INF_COST;
+#pragma unroll
+            for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi) dall[gi] = __shfl(d1, gi, WAVE);
+        }
+
+        for (uint32_t wd = 0; wd < W; ++wd) {
+            uint64_t bm = ccur[(uint64_t)v * W + wd];
+            if (bm == 0) continue;
+            // drop masks whose cost (plus lower bound) reaches the bound
+            {
+                const uint32_t na0 = (uint32_t)__popcll(bm);
+                uint64_t keep = 0;
+                if (lane < na0) {
+                    const uint32_t bit = select_bit(bm, lane);
+                    const uint32_t rk = wd * 64 + bit;
+                    const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
+                    uint32_t lb = 0;
+                    if (prune_lb) {
+                        const uint32_t X = a.masks[rk];
+#pragma unroll
+                        for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi) {
+                            if (gi < a.g && !((X >> gi) & 1u))
+                                lb = (dall[gi] == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, dall[gi]);
+                        }
+                    }
+                    if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
+                }
+                bm = wave_or64(keep);
+            }
+            const uint32_t na = (uint32_t)__popcll(bm);
+            if (na == 0) continue;
+            const uint32_t A = WAVE / na;           // arc slots per step
+            const uint32_t t = lane % na, s_arc = lane / na;
+            const bool lane_ok = s_arc < A;
+            const uint32_t rk = wd * 64 + select_bit(bm, t);
+            const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
+            const uint64_t bitv = 1ull << (rk & 63);
+            const uint32_t wsel = rk >> 6;
+            const uint32_t len = e1 - e0;
+            const uint32_t steps = (len + A - 1) / A;
+            for (uint32_t s = 0; s < steps; ++s) {
+                const uint32_t e = e0 + s * A + s_arc;
+                uint32_t enq_v = 0, enq_n = 0;
+                if (lane_ok && e < e1) {
+                    const uint32_t u = (uint32_t)a.col[e];
+                    const uint32_t wt = a.w[e];
+                    const uint32_t ra = a.rev[e];
+                    const uint32_t cand = add_cost<FLT>(cv, wt, ovf);
+                    st_relax++;
+                    if (cand != GST_INF_COST && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
+                        unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+                        unsigned long long curw = *addr;
+                        const unsigned long long nw = ((unsigned long long)cand << 32) | ra;
+                        bool improved = false;
+                        if (nw < curw) {
+                            if (cand == cv) {
+                                // effective zero-weight edge: only strict cost decreases, so
+                                // equal-cost predecessors can never close a cycle
+                                while (cand < (uint32_t)(curw >> 32)) {
+                                    const unsigned long long old = atomicCAS(addr, curw, nw);
+                                    if (old == curw) { improved = true; break; }
+                                    curw = old;
+                                }
+                            } else {
+                                const unsigned long long old = atomicMin(addr, nw);
+                                improved = cand < (uint32_t)(old >> 32);
+                            }
+                        }
+                        if (improved) {
+                            st_upd++;
+                            const unsigned long long ob =
+                                atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + wsel], (unsigned long long)bitv);
+                            if (!(ob & bitv)) {
+                                if (atomicExch(&inxt[u], 1u) == 0u) {
+                                    enq_v = u;
+                                    enq_n = num_chunks(a.row, u, a.chunk);
+                                }
+                            }
+                        }
+                    }
+                }
+                wave_enqueue(lnxt, cntn, enq_v, enq_n, st_enq);
+            }
+        }
+    }
+
+    // statistics: one atomic per wave
+    {
+        const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
+        const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
+        const uint32_t sr = wave_sum32(st_relax);
+        const uint32_t su = wave_sum32(st_upd);
+        const uint32_t se = wave_sum32(st_enq);
+        const uint32_t so = wave_sum32(ovf);
+        if (lane == 0) {
+            if (si) atomicAdd(&a.stats[ST_ITEMS], (unsigned long long)si);
+            if (sa) atomicAdd(&a.stats[ST_ARCS], (unsigned long long)sa);
+            if (sr) atomicAdd(&a.stats[ST_RELAX], (unsigned long long)sr);
+            if (su) atomicAdd(&a.stats[ST_UPD], (unsigned long long)su);
+            if (se) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)se);
+            if (so) atomicOr(&ctl->err, 1u);
+        }
+    }
+
+    // epilogue: the last workgroup to finish advances the round
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t prev = atomicAdd(&ctl->blocks_done, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            ctl->blocks_done = 0;
+            ctl->cnt[prv] = 0;
+            if (n_cur == 0) {
+                ctl->level_done = 1;
+            } else {
+                ctl->round = r + 1;
+            }
+            __threadfence();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// merge: T(v,X) for all X of level k from the lower levels of v, one wavefront per vertex
+// ---------------------------------------------------------------------------------------
+template <bool FLT>
+__global__ void __launch_bounds__(MERGE_BLOCK) k_merge(MergeArgs a) {
+    extern __shared__ unsigned long long lds[];  // [off_k] lower-level row + [m_k] results
+    const uint32_t k = a.k;
+    const uint32_t L = a.off[k];
+    const uint32_t mk = a.m[k];
+    unsigned long long* rowv = lds;
+    unsigned long long* res = lds + L;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = a.g;
+    const bool prune = a.ub != GST_INF_COST;
+    const bool prune_lb = prune && a.use_lb;
+    uint32_t ovf = 0, enq = 0;
+    uint32_t st_pairs = 0, st_vert = 0;
+    unsigned long long best_root = GST_INF_WORD;
+
+    for (uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
+        // lower levels of v into LDS
+        for (uint32_t j = 1; j < k; ++j) {
+            const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v * a.m[j];
+            for (uint32_t t = lane; t < a.m[j]; t += WAVE) rowv[a.off[j] + t] = src[t];
+        }
+        const uint32_t gmv = a.gm[v];
+        for (uint32_t r = lane; r < mk; r += WAVE) {
+            const uint32_t X = a.masks[a.off[k] + r];
+            res[r] = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
+        }
+        __syncthreads();
+        for (uint32_t p = lane; p < a.P; p += WAVE) {
+            const uint64_t e = a.pairs[p];
+            const uint32_t r = (uint32_t)(e & 0xFFFF);
+            const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
+            const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
+            const uint32_t X1 = (uint32_t)(e >> 48);
+            const uint32_t c1 = (uint32_t)(rowv[p1] >> 32);
+            const uint32_t c2 = (uint32_t)(rowv[p2] >> 32);
+            const uint32_t s = add_cost<FLT>(c1, c2, ovf);
+            if (s != GST_INF_COST) {
+                const unsigned long long wv = ((unsigned long long)s << 32) | (GST_PRED_MERGE | X1);
+                atomicMin(&res[r], wv);
+            }
+        }
+        st_pairs += (a.P + WAVE - 1 - lane) / WAVE;
+        __syncthreads();
+        // bound, store, frontier
+        uint64_t* dst = a.T + a.lvl_base[k] + (uint64_t)v * mk;
+        const bool has_arcs = a.row[v + 1] > a.row[v];
+        bool any = false;
+        for (uint32_t base = 0; base < mk; base += WAVE) {
+            const uint32_t r = base + lane;
+            unsigned long long wv = GST_INF_WORD;
+            if (r < mk) {
+                wv = res[r];
+                const uint32_t cst = (uint32_t)(wv >> 32);
+                if (cst != GST_INF_COST && prune) {
+                    uint32_t lb = 0;
+                    if (prune_lb && !a.is_full) {
+                        const uint32_t X = a.masks[a.off[k] + r];
+                        for (uint32_t gi = 0; gi < g; ++gi) {
+                            if ((X >> gi) & 1u) continue;
+                            const uint32_t di = (uint32_t)(rowv[gi] >> 32);  // level-1 row position gi = mask 1<<gi
+                            lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
+                        }
+                    }
+                    if (reaches_ub<FLT>(cst, lb, a.ub)) wv = GST_INF_WORD;
+                }
+                dst[r] = wv;
+                if (wv != GST_INF_WORD && a.is_full) {
+                    const unsigned long long rw = ((unsigned long long)(uint32_t)(wv >> 32) << 32) | v;
+                    best_root = rw < best_root ? rw : best_root;
+                }
+            }
+            const uint64_t word = __ballot(wv != GST_INF_WORD);
+            if (word) any = true;
+            if (!a.is_full && has_arcs && lane == 0 && word) a.chg0[(uint64_t)v * a.W + (base >> 6)] = word;
+        }
+        uint32_t nitems = 0;
+        if (!a.is_full) {
+            if (any && has_arcs && lane == 0) {
+                nitems = num_chunks(a.row, v, a.chunk);
+                a.inq0[v] = 1;
+            }
+            wave_enqueue(a.list0, &a.ctl->cnt[0], v, nitems, enq);
+        }
+        st_vert++;
+        __syncthreads();
+    }
+    if (a.is_full) {
+        best_root = wave_min64(best_root);
+        if (lane == 0 && best_root != GST_INF_WORD) atomicMin(a.root, best_root);
+    }
+    const uint32_t sp = wave_sum32(st_pairs);
+    const uint32_t so = wave_sum32(ovf);
+    if (lane == 0) {
+        atomicAdd(&a.stats[ST_MPAIRS], (unsigned long long)sp);
+        atomicAdd(&a.stats[ST_MVERT], (unsigned long long)st_vert);
+        if (enq) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)enq);
+        if (so) atomicOr(&a.ctl->err, 1u);
+    }
+}
+
+// g == 1: the full mask is level 1; its minimum is any member (cost 0)
+__global__ void k_root_scan_l1(const uint64_t* T1, uint32_t n, unsigned long long* root) {
+    unsigned long long best = GST_INF_WORD;
+    const uint64_t tid0 = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = tid0; base < n; base += stride) {
+        const uint64_t v = base + threadIdx.x;
+        if (v < n) {
+            const unsigned long long wv = T1[v];
+            if (wv != GST_INF_WORD) {
+                const unsigned long long rw = ((wv >> 32) << 32) | v;
+                best = rw < best ? rw : best;
+            }
+        }
+    }
+    best = wave_min64(best);
+    if (lane_id() == 0 && best != GST_INF_WORD) atomicMin(root, best);
+}
+
+// ---------------------------------------------------------------------------------------
+// traceback on the device: one wavefront walks the predecessor structure from (root, full)
+// ---------------------------------------------------------------------------------------
+struct TraceArgs {
+    const uint64_t* T;
+    const uint32_t* pos_of;   // mask -> row position
+    const int32_t* col;
+    const uint32_t* w;
+    const unsigned long long* root;
+    uint32_t* out_u;
+    uint32_t* out_v;
+    uint32_t* out_w;
+    uint32_t* out_n;
+    RoundCtl* ctl;
+    uint64_t lvl_base[GST_MAX_GROUPS + 2];
+    uint32_t m[GST_MAX_GROUPS + 2];
+    uint32_t off[GST_MAX_GROUPS + 2];
+    uint32_t full;
+    uint32_t cap;             // capacity of the edge output
+    uint32_t max_iter;
+};
+
+__global__ void __launch_bounds__(WAVE) k_traceback(TraceArgs a) {
+    __shared__ uint64_t stk[TRACE_STACK];
+    __shared__ uint32_t sp_s, nout_s, fail_s;
+    const uint32_t lane = threadIdx.x;
+    if (lane == 0) {
+        const unsigned long long rw = *a.root;
+        sp_s = 0; nout_s = 0; fail_s = 0;
+        if (rw != GST_INF_WORD) {
+            stk[0] = ((uint64_t)a.full << 32) | (uint32_t)(rw & 0xFFFFFFFFu);
+            sp_s = 1;
+        }
+    }
+    __syncthreads();
+    uint32_t iter = 0;
+    while (true) {
+        const uint32_t sp = sp_s;
+        if (sp == 0 || fail_s) break;
+        if (++iter > a.max_iter) { if (lane == 0) fail_s = 1; break; }
+        const uint32_t take = min(sp, (uint32_t)WAVE);
+        uint64_t item = 0;
+        const bool act = lane < take;
+        if (act) item = stk[sp - 1 - lane];
+        __syncthreads();
+        if (lane == 0) sp_s = sp - take;
+        __syncthreads();
+        uint32_t npush = 0;
+        uint64_t push0 = 0, push1 = 0;
+        uint32_t eu = 0, ev = 0, ew = 0;
+        bool emit = false;
+        if (act) {
+            const uint32_t v = (uint32_t)item;
+            const uint32_t X = (uint32_t)(item >> 32);
+            const uint32_t pos = a.pos_of[X];
+            const uint32_t k = (uint32_t)__popc(X);
+            const uint32_t rk = pos - a.off[k];
+            const uint64_t wv = a.T[a.lvl_base[k] + (uint64_t)v * a.m[k] + rk];
+            const uint32_t pred = (uint32_t)wv;
+            if (wv == GST_INF_WORD) {
+                fail_s = 1;
+            } else if (pred == GST_PRED_LEAF) {
+                // v covers X by itself
+            } else if (pred & 0x80000000u) {
+                const uint32_t X1 = pred & 0x7FFFFFFFu;
+                if (X1 == 0 || (X1 & ~X) || X1 == X) { fail_s = 1; }
+                else {
+                    push0 = ((uint64_t)X1 << 32) | v;
+                    push1 = ((uint64_t)(X ^ X1) << 32) | v;
+                    npush = 2;
+                }
+            } else {
+                const uint32_t u = (uint32_t)a.col[pred];
+                eu = v; ev = u; ew = a.w[pred];
+                emit = true;
+                push0 = ((uint64_t)X << 32) | u;
+                npush = 1;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            if (npush) {
+                const uint32_t p = atomicAdd(&sp_s, npush);
+                if (p + npush > TRACE_STACK) fail_s = 1;
+                else {
+                    stk[p] = push0;
+                    if (npush == 2) stk[p + 1] = push1;
+                }
+            }
+            if (emit) {
+                const uint32_t o = atomicAdd(&nout_s, 1u);
+                if (o < a.cap) { a.out_u[o] = eu; a.out_v[o] = ev; a.out_w[o] = ew; }
+                else fail_s = 1;
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (lane == 0) {
+        *a.out_n = nout_s;
+        if (fail_s) atomicOr(&a.ctl->err, 2u);
+    }
+}
+
+int grid_for(uint64_t work, int block, int max_blocks) {
+    uint64_t b = (work + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > (uint64_t)max_blocks) b = max_blocks;
+    return (int)b;
+}
+
+}  // namespace
+
+// =======================================================================================
+// per-query device buffers (grown on demand, reused across queries)
+// =======================================================================================
+struct gst_query_bufs {
+    uint64_t cap_T = 0, cap_V = 0, cap_items = 0, cap_W = 0, cap_grp = 0, cap_edges = 0;
+    uint64_t* T = nullptr;
+    uint32_t* gm = nullptr;
+    uint64_t* chg[3] = {nullptr, nullptr, nullptr};
+    uint32_t* inq[3] = {nullptr, nullptr, nullptr};
+    uint64_t* lists[3] = {nullptr, nullptr, nullptr};
+    RoundCtl* ctl = nullptr;
+    unsigned long long* root = nullptr;
+    unsigned long long* stats = nullptr;
+    uint32_t* edges = nullptr;   // 3 * cap_edges + 1 (count)
+    int32_t* grp = nullptr;      // grp_ptr (g+1) then members
+    // level tables for the current g
+    int tab_g = -1;
+    uint64_t* pairs = nullptr;
+    uint32_t* masks = nullptr;
+    uint32_t* pos_of = nullptr;
+    bool dirty = true;           // bitmaps / flags need clearing
+    RoundCtl* h_ctl = nullptr;   // pinned
+    std::vector<hipEvent_t> evs;
+};
+
+static void free_bufs(gst_query_bufs* b) {
+    if (!b) return;
+    hipFree(b->T); hipFree(b->gm);
+    for (int i = 0; i < 3; ++i) { hipFree(b->chg[i]); hipFree(b->inq[i]); hipFree(b->lists[i]); }
+    hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
+    hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
+    if (b->h_ctl) hipHostFree(b->h_ctl);
+    for (auto e : b->evs) hipEventDestroy(e);
+    delete b;
+}
+
+void gst_query_bufs_free(gst_ctx* ctx) {
+    free_bufs(ctx->qb);
+    ctx->qb = nullptr;
+}
+
+gst_status gst_ctx_device_init(gst_ctx* ctx) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return gst_set_error(ctx, GST_NO_DEVICE, "no HIP device available");
+    if (ctx->device < 0 || ctx->device >= ndev)
+        return gst_set_error(ctx, GST_INVALID_ARG, "device %d out of range (%d devices)", ctx->device, ndev);
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    ctx->stream = (void*)s;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->cu_count = prop.multiProcessorCount;
+    return GST_OK;
+}
+
+void gst_ctx_device_fini(gst_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    gst_query_bufs_free(ctx);
+    if (ctx->stream) hipStreamDestroy((hipStream_t)ctx->stream);
+    ctx->stream = nullptr;
+}
+
+gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)g->n, m = (size_t)g->m;
+    std::vector<uint32_t> row32(n + 1);
+    for (size_t i = 0; i <= n; ++i) row32[i] = (uint32_t)g->h_row[i];
+    std::vector<uint32_t> rev(std::max<size_t>(m, 1));
+    for (size_t v = 0; v < n; ++v) {
+        for (int64_t e = g->h_row[v]; e < g->h_row[v + 1]; ++e) {
+            const int32_t u = g->h_col[(size_t)e];
+            // binary search v in row u
+            int64_t lo = g->h_row[(size_t)u], hi = g->h_row[(size_t)u + 1];
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (g->h_col[(size_t)mid] < (int32_t)v) lo = mid + 1; else hi = mid;
+            }
+            rev[(size_t)e] = (uint32_t)lo;
+        }
+    }
+    const size_t mm = std::max<size_t>(m, 1);
+    if (hipMalloc(&g->d_row, (n + 1) * 4) != hipSuccess || hipMalloc(&g->d_col, mm * 4) != hipSuccess ||
+        hipMalloc(&g->d_w, mm * 4) != hipSuccess || hipMalloc(&g->d_rev, mm * 4) != hipSuccess) {
+        gst_graph_device_free(g);
+        return gst_set_error(ctx, GST_OOM, "graph upload: device allocation failed (%zu arcs)", m);
+    }
+    hipStream_t s = (hipStream_t)ctx->stream;
+    HIPCHK(hipMemcpyAsync(g->d_row, row32.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+    if (m) {
+        HIPCHK(hipMemcpyAsync(g->d_col, g->h_col.data(), m * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(g->d_w, g->h_w.data(), m * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(g->d_rev, rev.data(), m * 4, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return GST_OK;
+}
+
+void gst_graph_device_free(gst_graph* g) {
+    if (!g) return;
+    if (g->ctx) hipSetDevice(g->ctx->device);
+    hipFree(g->d_row); hipFree(g->d_col); hipFree(g->d_w); hipFree(g->d_rev);
+    g->d_row = nullptr; g->d_col = nullptr; g->d_w = nullptr; g->d_rev = nullptr;
+}
+
+namespace {
+
+template <class T>
+bool grow_alloc(T*& p, uint64_t& cap, uint64_t need, size_t elem) {
+    if (p && cap >= need) return true;
+    hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc((void**)&p, std::max<uint64_t>(need, 1) * elem) != hipSuccess) { p = nullptr; return false; }
+    cap = need;
+    return true;
+}
+
+uint32_t default_chunk(int m) {
+    // arcs per work item: enough steps to amortise the item setup, small enough that hub
+    // vertices split over many wavefronts
+    (void)m;
+    return 64;
+}
+
+}  // namespace
+
+gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* groups,
+                          const gst_opts* opts_in, gst_result* res) {
+    const auto t_start = std::chrono::steady_clock::now();
+    gst_opts opts{};
+    opts.ub = 0; opts.prune_lb = 1; opts.want_tree = 1;
+    if (opts_in) opts = *opts_in;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)ctx->stream;
+    const int g = groups->n_groups;
+    const uint32_t n = (uint32_t)G->n;
+    const gst_levels& L = gst_get_levels(ctx, g);
+    const bool flt = G->w_type == GST_W_FLOAT32;
+
+    // bound in cost bits
+    uint32_t ub_bits = GST_INF_COST;
+    const bool has_ub = std::isfinite(opts.ub) && opts.ub > 0;
+    if (has_ub) {
+        if (flt) {
+            // the DP sums fp32 costs in its own order; keep a relative slack of 1e-4 above the
+            // bound so that rounding never drops a structure whose exact cost is <= the bound
+            float fu = (float)(opts.ub * (1.0 + 1e-4));
+            fu = std::nextafter(fu, INFINITY);
+            std::memcpy(&ub_bits, &fu, 4);
+        } else {
+            const double c = std::floor(opts.ub) + 1.0;   // keep cost <= ub, drop cost >= floor(ub)+1
+            ub_bits = c >= 4294967295.0 ? GST_INF_COST : (uint32_t)c;
+        }
+    }
+
+    // ---- buffers ----
+    if (!ctx->qb) ctx->qb = new gst_query_bufs();
+    gst_query_bufs* B = ctx->qb;
+    const uint64_t words = (uint64_t)n * (uint64_t)L.full;
+    uint32_t Wmax = 1;
+    for (int k = 1; k <= g; ++k) Wmax = std::max<uint32_t>(Wmax, (uint32_t)((L.m[k] + 63) / 64));
+    const uint32_t chunk = opts.chunk_arcs > 0 ? (uint32_t)opts.chunk_arcs : default_chunk(0);
+    const uint64_t items_cap = (uint64_t)n + (uint64_t)G->m / chunk + 64;
+    const uint64_t edges_cap = (uint64_t)n + 2 * (uint64_t)g + 16;
+    const uint64_t grp_need = (uint64_t)(g + 1) + (uint64_t)groups->grp_ptr[g];
+    bool ok = true;
+    const uint64_t oldV = B->cap_V, oldW = B->cap_W;
+    ok = ok && grow_alloc(B->T, B->cap_T, words, 8);
+    uint64_t capv = B->cap_V, capw = B->cap_W, capi = B->cap_items;
+    if (!(B->gm && capv >= n && capw >= Wmax)) {
+        uint64_t dummy;
+        for (int i = 0; i < 3; ++i) {
+            dummy = 0; hipFree(B->chg[i]); B->chg[i] = nullptr;
+            ok = ok && grow_alloc(B->chg[i], dummy, (uint64_t)n * Wmax, 8);
+            dummy = 0; hipFree(B->inq[i]); B->inq[i] = nullptr;
+            ok = ok && grow_alloc(B->inq[i], dummy, (uint64_t)n, 4);
+        }
+        dummy = 0; hipFree(B->gm); B->gm = nullptr;
+        ok = ok && grow_alloc(B->gm, dummy, (uint64_t)n, 4);
+        B->cap_V = n; B->cap_W = Wmax;
+        B->dirty = true;
+    }
+    (void)oldV; (void)oldW;
+    if (!(B->lists[0] && capi >= items_cap)) {
+        for (int i = 0; i < 3; ++i) {
+            uint64_t dummy = 0; hipFree(B->lists[i]); B->lists[i] = nullptr;
+            ok = ok && grow_alloc(B->lists[i], dummy, items_cap, 8);
+        }
+        B->cap_items = items_cap;
+    }
+    if (!B->ctl) {
+        uint64_t d = 0;
+        ok = ok && grow_alloc(B->ctl, d, 1, sizeof(RoundCtl));
+        d = 0; ok = ok && grow_alloc(B->root, d, 1, 8);
+        d = 0; ok = ok && grow_alloc(B->stats, d, ST_NSTAT, 8);
+        if (hipHostMalloc((void**)&B->h_ctl, sizeof(RoundCtl)) != hipSuccess) ok = false;
+    }
+    ok = ok && grow_alloc(B->edges, B->cap_edges, 3 * edges_cap + 1, 4);
+    if (ok && B->cap_edges != 3 * edges_cap + 1) {
+        // capacity bookkeeping: cap_edges counts words; recover the per-array capacity
+    }
+    ok = ok && grow_alloc(B->grp, B->cap_grp, grp_need, 4);
+    if (!ok) {
+        gst_query_bufs_free(ctx);
+        return gst_set_error(ctx, GST_OOM, "solve: device allocation failed (V=%u, g=%d, table %.2f GB)",
+                             n, g, words * 8.0 / 1e9);
+    }
+    const uint32_t ecap = (uint32_t)((B->cap_edges - 1) / 3);
+    if (B->tab_g != g) {
+        hipFree(B->pairs); hipFree(B->masks); hipFree(B->pos_of);
+        B->pairs = nullptr; B->masks = nullptr; B->pos_of = nullptr;
+        uint64_t d = 0;
+        ok = grow_alloc(B->pairs, d, std::max<size_t>(L.pairs.size(), 1), 8);
+        d = 0; ok = ok && grow_alloc(B->masks, d, L.masks.size(), 4);
+        d = 0; ok = ok && grow_alloc(B->pos_of, d, L.pos_of.size(), 4);
+        if (!ok) { gst_query_bufs_free(ctx); return gst_set_error(ctx, GST_OOM, "solve: table allocation failed"); }
+        if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(B->pairs, L.pairs.data(), L.pairs.size() * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B->masks, L.masks.data(), L.masks.size() * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B->pos_of, L.pos_of.data(), L.pos_of.size() * 4, hipMemcpyHostToDevice, s));
+        B->tab_g = g;
+    }
+    const int fill_grid = std::max(1, ctx->cu_count * 8);
+    if (B->dirty) {
+        for (int i = 0; i < 3; ++i) {
+            k_fill_u64<<<fill_grid, 256, 0, s>>>(B->chg[i], 0ull, B->cap_V * B->cap_W);
+            k_fill_u32<<<fill_grid, 256, 0, s>>>(B->inq[i], 0u, B->cap_V);
+        }
+        B->dirty = false;
+    }
+
+    // ---- group bits ----
+    std::vector<int32_t> hgrp((size_t)grp_need);
+    for (int i = 0; i <= g; ++i) hgrp[(size_t)i] = groups->grp_ptr[i];
+    for (int32_t j = 0; j < groups->grp_ptr[g]; ++j) hgrp[(size_t)(g + 1 + j)] = groups->grp_vtx[j];
+    HIPCHK(hipMemcpyAsync(B->grp, hgrp.data(), grp_need * 4, hipMemcpyHostToDevice, s));
+    k_fill_u32<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->gm, 0u, n);
+    k_set_group_bits<<<64, 256, 0, s>>>(B->gm, B->grp, B->grp + g + 1, g);
+    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->stats, 0ull, ST_NSTAT);
+    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->root, GST_INF_WORD, 1);
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)B->ctl, 0u, sizeof(RoundCtl) / 4);
+
+    std::vector<uint64_t> lvl_base(GST_MAX_GROUPS + 2, 0);
+    for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
+
+    // events: pairs (start,end) per phase; kind 0 = grow, 1 = merge, 2 = trace
+    struct Ev { hipEvent_t a, b; int kind; };
+    std::vector<Ev> evs;
+    auto new_ev = [&](int kind) -> Ev* {
+        Ev e; e.kind = kind;
+        hipEventCreate(&e.a); hipEventCreate(&e.b);
+        evs.push_back(e);
+        return &evs.back();
+    };
+    auto cleanup_evs = [&]() { for (auto& e : evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); } };
+
+    const int grow_grid = std::max(1, ctx->cu_count * 4);   // 4 workgroups x 4 waves per CU
+    const int rounds_per_sync = opts.rounds_per_sync > 0 ? opts.rounds_per_sync : 8;
+    uint32_t total_rounds = 0, grow_launches = 0, merge_launches = 0;
+
+    auto run_grow = [&](int k) -> gst_status {
+        GrowArgs ga;
+        ga.Tk = B->T + lvl_base[k];
+        ga.T1 = B->T + lvl_base[1];
+        ga.masks = B->masks + L.off[k];
+        ga.row = G->d_row; ga.col = G->d_col; ga.w = G->d_w; ga.rev = G->d_rev;
+        for (int i = 0; i < 3; ++i) { ga.list[i] = B->lists[i]; ga.chg[i] = B->chg[i]; ga.inq[i] = B->inq[i]; }
+        ga.ctl = B->ctl; ga.stats = B->stats;
+        ga.m = (uint32_t)L.m[k]; ga.W = (uint32_t)((L.m[k] + 63) / 64); ga.g = (uint32_t)g;
+        ga.chunk = chunk; ga.ub = ub_bits; ga.use_lb = opts.prune_lb ? 1u : 0u;
+        Ev* e = new_ev(0);
+        hipEventRecord(e->a, s);
+        for (int it = 0; it < 100000; ++it) {
+            for (int rr = 0; rr < rounds_per_sync; ++rr) {
+                if (flt) k_grow_round<true><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
+                else k_grow_round<false><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
+                grow_launches++;
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (B->h_ctl->err & 1u) return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits");
+            if (B->h_ctl->level_done) break;
+        }
+        if (!B->h_ctl->level_done) { B->dirty = true; return gst_set_error(ctx, GST_INTERNAL, "grow did not converge"); }
+        total_rounds += B->h_ctl->round + 1;
+        hipEventRecord(e->b, s);
+        return GST_OK;
+    };
+
+    gst_status st = GST_OK;
+    // ---- level 1 ----
+    {
+        k_ctl_reset<<<1, 64, 0, s>>>(B->ctl);
+        Ev* e = new_ev(1);
+        hipEventRecord(e->a, s);
+        k_init_level1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], B->gm, G->d_row, n, (uint32_t)g,
+                                                                   chunk, B->lists[0], B->chg[0], B->inq[0], B->ctl,
+                                                                   B->stats, g > 1 ? 1 : 0);
+        hipEventRecord(e->b, s);
+        if (g > 1) st = run_grow(1);
+        else k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T, n, B->root);
+    }
+    // ---- levels 2..g ----
+    for (int k = 2; k <= g && st == GST_OK; ++k) {
+        const bool full = (k == g);
+        k_ctl_reset<<<1, 64, 0, s>>>(B->ctl);
+        MergeArgs ma;
+        ma.T = B->T; ma.pairs = B->pairs + L.pair_off[k]; ma.masks = B->masks; ma.gm = B->gm; ma.row = G->d_row;
+        ma.list0 = B->lists[0]; ma.chg0 = B->chg[0]; ma.inq0 = B->inq[0]; ma.ctl = B->ctl; ma.root = B->root;
+        ma.stats = B->stats;
+        for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ma.lvl_base[j] = lvl_base[j]; ma.m[j] = (uint32_t)L.m[j]; ma.off[j] = (uint32_t)L.off[j]; }
+        ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
+        ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ub_bits; ma.use_lb = opts.prune_lb ? 1u : 0u;
+        ma.is_full = full ? 1u : 0u;
+        const size_t lds = ((size_t)L.off[k] + (size_t)L.m[k]) * 8;
+        const int mgrid = std::max(1, std::min<int>((int)n, ctx->cu_count * 32));
+        Ev* e = new_ev(1);
+        hipEventRecord(e->a, s);
+        if (flt) k_merge<true><<<mgrid, MERGE_BLOCK, lds, s>>>(ma);
+        else k_merge<false><<<mgrid, MERGE_BLOCK, lds, s>>>(ma);
+        hipEventRecord(e->b, s);
+        merge_launches++;
+        HIPCHK(hipGetLastError());
+        if (!full) st = run_grow(k);
+    }
+    if (st != GST_OK) { B->dirty = true; cleanup_evs(); return st; }
+
+    // ---- root + traceback ----
+    unsigned long long h_root = GST_INF_WORD;
+    HIPCHK(hipMemcpyAsync(&h_root, B->root, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (B->h_ctl->err & 1u) { cleanup_evs(); return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits"); }
+
+    std::memset(res, 0, sizeof(*res));
+    res->n_groups = g;
+    res->root = -1;
+    res->used_ub = has_ub ? 1 : 0;
+    if (h_root == GST_INF_WORD) {
+        cleanup_evs();
+        if (has_ub)
+            return gst_set_error(ctx, GST_INFEASIBLE, "no tree of cost <= bound %.9g touches all %d groups", opts.ub, g);
+        return gst_set_error(ctx, GST_INFEASIBLE, "no tree touches all %d groups", g);
+    }
+    const uint32_t root_cost_bits = (uint32_t)(h_root >> 32);
+    const uint32_t root_v = (uint32_t)(h_root & 0xFFFFFFFFu);
+    if (flt) { float f; std::memcpy(&f, &root_cost_bits, 4); res->cost = (double)f; }
+    else res->cost = (double)root_cost_bits;
+    res->root = (int32_t)root_v;
+
+    if (opts.want_tree) {
+        TraceArgs ta;
+        ta.T = B->T; ta.pos_of = B->pos_of; ta.col = G->d_col; ta.w = G->d_w; ta.root = B->root;
+        ta.out_u = B->edges; ta.out_v = B->edges + ecap; ta.out_w = B->edges + 2 * ecap; ta.out_n = B->edges + 3 * ecap;
+        ta.ctl = B->ctl;
+        for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ta.lvl_base[j] = lvl_base[j]; ta.m[j] = (uint32_t)L.m[j]; ta.off[j] = (uint32_t)L.off[j]; }
+        ta.full = (uint32_t)L.full; ta.cap = ecap; ta.max_iter = n + 4u * (uint32_t)g + 64u;
+        Ev* e = new_ev(2);
+        hipEventRecord(e->a, s);
+        k_traceback<<<1, WAVE, 0, s>>>(ta);
+        hipEventRecord(e->b, s);
+        HIPCHK(hipGetLastError());
+        uint32_t nout = 0;
+        HIPCHK(hipMemcpyAsync(&nout, B->edges + 3 * ecap, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if ((B->h_ctl->err & 2u) || nout > ecap) {
+            cleanup_evs();
+            return gst_set_error(ctx, GST_INTERNAL, "device traceback failed (edges=%u)", nout);
+        }
+        std::vector<uint32_t> hu(nout), hv(nout), hw(nout);
+        if (nout) {
+            HIPCHK(hipMemcpyAsync(hu.data(), B->edges, nout * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hv.data(), B->edges + ecap, nout * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hw.data(), B->edges + 2 * ecap, nout * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        // deduplicate (a tie-free optimum is a tree; duplicates can only come from zero-cost ties)
+        std::vector<std::pair<uint64_t, uint32_t>> ed;
+        ed.reserve(nout);
+        for (uint32_t i = 0; i < nout; ++i) {
+            uint32_t x = hu[i], y = hv[i];
+            if (x > y) std::swap(x, y);
+            ed.push_back({((uint64_t)x << 32) | y, hw[i]});
+        }
+        std::sort(ed.begin(), ed.end());
+        ed.erase(std::unique(ed.begin(), ed.end(),
+                             [](const std::pair<uint64_t, uint32_t>& p, const std::pair<uint64_t, uint32_t>& q) {
+                                 return p.first == q.first;
+                             }),
+                 ed.end());
+        res->n_edges = (int32_t)ed.size();
+        res->eu = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
+        res->ev = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
+        res->ew = (double*)std::malloc(sizeof(double) * std::max<size_t>(ed.size(), 1));
+        double tc = 0;
+        for (size_t i = 0; i < ed.size(); ++i) {
+            res->eu[i] = (int32_t)(ed[i].first >> 32);
+            res->ev[i] = (int32_t)(ed[i].first & 0xFFFFFFFFu);
+            double wv;
+            if (flt) { float f; std::memcpy(&f, &ed[i].second, 4); wv = (double)f; }
+            else wv = (double)ed[i].second;
+            res->ew[i] = wv;
+            tc += wv;
+        }
+        res->tree_cost = tc;
+    }
+
+    // ---- stats ----
+    unsigned long long hs[ST_NSTAT];
+    HIPCHK(hipMemcpyAsync(hs, B->stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    gst_stats& S = res->stats;
+    S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
+    S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ];
+    S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
+    S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
+    for (auto& e : evs) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, e.a, e.b);
+        if (e.kind == 0) S.t_grow_ms += ms;
+        else if (e.kind == 1) S.t_merge_ms += ms;
+        else S.t_trace_ms += ms;
+    }
+    cleanup_evs();
+    // algorithmic byte model: see DESIGN (grow: item 8 + row 8 + bitmap 8W + own costs 8/relax-lane-group
+    // approximated by 12 B per scanned arc + 8 B read + 16 B RMW per update; merge: 8 B per word in/out)
+    S.grow_bytes = S.grow_items * (8ull + 8ull + 8ull) + S.grow_arcs * 12ull + S.grow_relax * 8ull + S.grow_updates * 16ull;
+    uint64_t mb = 0;
+    for (int k = 2; k <= g; ++k) mb += (uint64_t)(L.off[k] + L.m[k]) * 8ull;
+    S.merge_bytes = (uint64_t)n * mb;
+    S.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return GST_OK;
+}

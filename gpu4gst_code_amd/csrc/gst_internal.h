@@ -1,0 +1,81 @@
+// Internal declarations shared by the host driver (gst_api.cpp) and the HIP kernels
+// (gst_dpbf.hip).  Not part of the public ABI.
+#pragma once
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/gst.h"
+
+// ---------------------------------------------------------------------------------------
+// Packed DP state word: high 32 bits = cost, low 32 bits = predecessor.
+//   cost  : uint32 for integer weights, IEEE-754 bits of a non-negative float for float32
+//           weights (non-negative float bit patterns order like the floats); 0xFFFFFFFF = inf.
+//   pred  : < 0x80000000  -> grow: index of the arc (u -> src) in the row of this state's
+//                            vertex u; the tree edge is (u, col[pred]) with weight w[pred]
+//           0x80000000     -> leaf: v itself covers the mask
+//           0x80000000|X1  -> merge of (v,X1) and (v,X^X1), X1 holds the lowest bit of X
+// One 64-bit atomic-min keeps cost and predecessor consistent (the reference updates cost by
+// atomicMin and the predecessor by plain stores, src/DPQ.cu:267-272) and breaks cost ties by
+// the smaller predecessor.
+// ---------------------------------------------------------------------------------------
+#define GST_INF_COST   0xFFFFFFFFu
+#define GST_INF_WORD   0xFFFFFFFFFFFFFFFFull
+#define GST_PRED_LEAF  0x80000000u
+#define GST_PRED_MERGE 0x80000000u
+
+struct gst_levels {
+    int g = 0;
+    int full = 0;                       // 2^g - 1
+    int m[GST_MAX_GROUPS + 2] = {0};    // masks per level (popcount k), k = 1..g
+    int off[GST_MAX_GROUPS + 2] = {0};  // per-vertex row offset of level k: sum_{j<k} m_j
+    std::vector<uint32_t> masks;        // masks ordered by (popcount, value); row position = index
+    std::vector<uint32_t> pos_of;       // mask -> row position (index into masks)
+    std::vector<uint32_t> rank_of;      // mask -> rank within its level
+    // merge pairs per level k (k >= 2): packed (dst rank:16 | pos1:16 | pos2:16 | X1:16)
+    std::vector<uint64_t> pairs;
+    int pair_off[GST_MAX_GROUPS + 2] = {0};
+    int pair_cnt[GST_MAX_GROUPS + 2] = {0};
+    void build(int g);
+};
+
+struct gst_graph {
+    gst_ctx* ctx = nullptr;
+    int32_t n = 0;
+    int64_t m = 0;
+    int32_t w_type = 0;
+    uint32_t* d_row = nullptr;   // n+1 arc offsets (arcs < 2^31)
+    int32_t*  d_col = nullptr;   // m
+    uint32_t* d_w = nullptr;     // m (int32 or float bits)
+    uint32_t* d_rev = nullptr;   // m: index of the reverse arc
+    // host copy for validation / traceback weight lookups
+    std::vector<int64_t> h_row;
+    std::vector<int32_t> h_col;
+    std::vector<uint32_t> h_w;
+    int32_t max_deg = 0;
+};
+
+struct gst_query_bufs;  // device buffers of one query (defined in gst_dpbf.hip)
+
+struct gst_ctx {
+    int device = 0;
+    std::string last_error;
+    void* stream = nullptr;     // hipStream_t
+    gst_query_bufs* qb = nullptr;
+    gst_levels lv_cache[GST_MAX_GROUPS + 1];
+    bool lv_ready[GST_MAX_GROUPS + 1] = {false};
+    int cu_count = 0;
+};
+
+// implemented in gst_dpbf.hip
+gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
+                          const gst_opts* opts, gst_result* res);
+void gst_query_bufs_free(gst_ctx* ctx);
+gst_status gst_ctx_device_init(gst_ctx* ctx);
+void gst_ctx_device_fini(gst_ctx* ctx);
+gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g);
+void gst_graph_device_free(gst_graph* g);
+
+gst_status gst_set_error(gst_ctx* ctx, gst_status s, const char* fmt, ...);
+const gst_levels& gst_get_levels(gst_ctx* ctx, int g);

@@ -1,0 +1,191 @@
+/*
+ * gst.h — C-ABI of the MI355X group Steiner tree (DPBF) solver.
+ *
+ * The reference exposes its solver as plain C++ free functions:
+ *   DPBF_GPU(...)                              rucgraph/DPQ.cuh:57, src/DPQ.cu:501-768
+ *   graph_v_of_v_idealID_DPBF_only_ec(...)     rucgraph/graph_v_of_v_idealID_DPBF_only_ec.h:133-134
+ * with the input graph converted to CSR by toCSR (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:29-65)
+ * and groups given as "group vertices" whose adjacency lists hold the members
+ * (rucgraph/graph_hash_of_mixed_weighted_generate_random_groups_of_vertices.h:42-47).
+ *
+ * This header replaces that boundary with a C ABI: plain pointers and sizes, status codes, no
+ * exceptions, no C++ or torch types.  The C++ wrapper in gst.hpp rebuilds the reference's host
+ * API (graph_v_of_v_idealID in, tree + cost out) on top of it.
+ *
+ * Ownership: the caller owns every input array; arrays inside a gst_result are owned by the
+ * library and released by gst_result_free.  A gst_ctx is single-threaded (one host thread at a
+ * time); separate contexts may be used concurrently.
+ */
+#ifndef GST_H
+#define GST_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GST_MAX_GROUPS 12      /* compile-time maximum number of groups g (state table holds 2^g-1 words/vertex) */
+
+typedef enum {
+    GST_OK = 0,
+    GST_INFEASIBLE = 1,        /* no connected tree touches every group (reference: empty tree, src/DPQ.cu:620-624,723-726) */
+    GST_TOO_MANY_GROUPS = 2,   /* g > GST_MAX_GROUPS (reference: exit(1) for g>=20, ...DPBF_only_ec.h:139-142) */
+    GST_OOM = 3,               /* device allocation failed */
+    GST_INVALID_ARG = 4,       /* malformed CSR, negative / non-finite weight, vertex id out of range, ... */
+    GST_HIP_ERROR = 5,         /* a HIP runtime call or kernel failed */
+    GST_COST_OVERFLOW = 6,     /* an integer path cost exceeded the 32-bit cost field */
+    GST_NO_DEVICE = 7,         /* no usable GPU */
+    GST_INTERNAL = 8           /* internal consistency check failed (traceback did not close, ...) */
+} gst_status;
+
+typedef enum { GST_W_INT32 = 0, GST_W_FLOAT32 = 1 } gst_weight_type;
+
+/* Undirected graph as symmetric CSR (both arcs of every edge present).  Replaces CSR_graph /
+ * toCSR (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:11-65), which truncates weights to int;
+ * here int32 weights stay int32 and float32 weights stay float32. */
+typedef struct {
+    int32_t        n;          /* vertices 0..n-1 */
+    int64_t        m;          /* arcs = row_ptr[n] */
+    const int64_t* row_ptr;    /* n+1 */
+    const int32_t* col;        /* m */
+    const void*    w;          /* m weights, int32_t or float, >= 0 */
+    int32_t        w_type;     /* gst_weight_type */
+} gst_csr;
+
+/* g groups as a CSR of member lists.  Group i (bit i of the DP masks) holds
+ * grp_vtx[grp_ptr[i] .. grp_ptr[i+1]).  Members may repeat across groups. */
+typedef struct {
+    int32_t        n_groups;
+    const int32_t* grp_ptr;    /* n_groups+1 */
+    const int32_t* grp_vtx;
+} gst_groups;
+
+typedef struct {
+    double  ub;                /* known feasible-tree cost U: states whose cost (+ lower bound) exceeds U
+                                  are dropped, so the solve returns the optimum when it is <= U and
+                                  GST_INFEASIBLE otherwise.  <= 0 or not finite: no bound (plain DPBF) */
+    int32_t prune_lb;          /* with a bound: also drop (v,X) when cost + max_{i not in X} dist(v,G_i) >= ub */
+    int32_t want_tree;         /* 1: run the device traceback and return the edge list */
+    int32_t rounds_per_sync;   /* grow rounds launched per host check (0 = tuned default) */
+    int32_t chunk_arcs;        /* arcs per frontier work item (0 = tuned default) */
+    int32_t reserved[8];
+} gst_opts;
+
+typedef struct {
+    uint64_t grow_items;       /* frontier work items dequeued */
+    uint64_t grow_arcs;        /* arcs scanned by those items */
+    uint64_t grow_relax;       /* (arc, mask) relaxations attempted */
+    uint64_t grow_updates;     /* relaxations that lowered a cost */
+    uint64_t grow_enqueued;    /* work items enqueued */
+    uint64_t merge_vertices;   /* vertex rows merged */
+    uint64_t merge_pairs;      /* (submask, complement) pairs evaluated */
+    uint64_t grow_bytes;       /* algorithmic bytes of the grow kernel (model in DESIGN) */
+    uint64_t merge_bytes;      /* algorithmic bytes of the merge kernel */
+    uint32_t grow_rounds;
+    uint32_t grow_launches;
+    uint32_t merge_launches;
+    uint32_t pad0;
+    double   t_total_ms;       /* wall time of the solve, host clock */
+    double   t_grow_ms;        /* device time of grow kernels (HIP events) */
+    double   t_merge_ms;       /* device time of merge kernels (HIP events) */
+    double   t_trace_ms;       /* device time of the traceback */
+} gst_stats;
+
+typedef struct {
+    double   cost;             /* optimal DP cost (exact for int32, fp32 DP value for float32) */
+    double   tree_cost;        /* sum of returned edge weights, recomputed in double */
+    int32_t  root;             /* root vertex of the returned tree (-1 when none) */
+    int32_t  n_edges;
+    int32_t* eu;               /* n_edges, library-owned */
+    int32_t* ev;
+    double*  ew;
+    uint8_t  used_ub;          /* a finite bound was applied */
+    uint8_t  fell_back;        /* learned path: the bounded solve was replaced by plain DPBF */
+    uint8_t  pad[2];
+    int32_t  n_groups;
+    gst_stats stats;
+} gst_result;
+
+typedef struct gst_ctx   gst_ctx;
+typedef struct gst_graph gst_graph;
+
+/* --- context -------------------------------------------------------------------------- */
+/* Replaces the implicit device/global state of src/DPQ.cu:21 (global vectors) and the per-call
+ * allocations of src/DPQ.cu:525-542; one context per device and host thread. */
+gst_status  gst_ctx_create(int32_t device, gst_ctx** out);
+void        gst_ctx_destroy(gst_ctx* ctx);
+const char* gst_last_error(const gst_ctx* ctx);
+const char* gst_status_str(gst_status s);
+
+/* --- graph ---------------------------------------------------------------------------- */
+/* Device-resident CSR reused across queries.  Replaces toCSR
+ * (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:29-65): rows are sorted by neighbour id,
+ * parallel arcs collapse to their lightest weight, and each arc gets its reverse-arc index. */
+gst_status gst_graph_upload(gst_ctx* ctx, const gst_csr* csr, gst_graph** out);
+void       gst_graph_free(gst_graph* g);
+int32_t    gst_graph_num_vertices(const gst_graph* g);
+int64_t    gst_graph_num_arcs(const gst_graph* g);
+
+/* --- solve ---------------------------------------------------------------------------- */
+/* One exact GST query.  Replaces DPBF_GPU (rucgraph/DPQ.cuh:57, src/DPQ.cu:501-768): the
+ * returned edge list is the tree, result->cost is *real_cost, stats replace *pointer1 / *rt.
+ * opts may be NULL (plain DPBF, tree returned). */
+gst_status gst_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
+                     const gst_opts* opts, gst_result* result);
+void       gst_result_free(gst_result* r);
+
+/* Tree check used by the tests and the C++ wrapper; extends this_is_a_feasible_solution_gpu
+ * (rucgraph/test_gpu.h:45-77) and graph_hash_of_mixed_weighted_sum_of_ec
+ * (rucgraph/graph_hash_of_mixed_weighted_sum_of_nw_ec.h:58-87): the edge set must be a tree
+ * (connected, |E| = |V|-1, every edge present in the graph with that weight) touching every
+ * group.  *cost_out receives the double sum of the edge weights.  A single vertex touching
+ * every group is a valid tree of cost 0 (pass n_edges = 0 and root). */
+gst_status gst_validate_tree(const gst_csr* csr, const gst_groups* groups, int32_t n_edges,
+                             const int32_t* eu, const int32_t* ev, int32_t root,
+                             double* cost_out);
+
+/* --- synthetic instances (host, deterministic) ------------------------------------------ */
+/* Host CSR owned by the library.  The generators use the library's own seeded RNG
+ * (splitmix64 / xoshiro256**), so a seed names the same instance on every machine. */
+typedef struct {
+    int32_t  n;
+    int64_t  m;
+    int64_t* row_ptr;
+    int32_t* col;
+    void*    w;
+    int32_t  w_type;
+} gst_host_csr;
+
+void gst_host_csr_free(gst_host_csr* h);
+
+/* Random connected graph: random spanning tree (vertex i attaches to a uniform earlier vertex)
+ * plus uniform extra edges without duplicates or self-loops, int weights uniform in [wlo,whi].
+ * Same recipe as graph_v_of_v_idealID_generate_random_connected_graph
+ * (rucgraph/graph_v_of_v_idealID/random_graph/graph_v_of_v_idealID_generate_random_connected_graph.h:41-101). */
+gst_status gst_gen_random_connected(int32_t n, int64_t n_edges, int32_t wlo, int32_t whi,
+                                    uint64_t seed, gst_host_csr* out);
+/* Barabasi-Albert preferential attachment: an m-clique seed, then each new vertex links to m
+ * distinct earlier vertices chosen with probability proportional to degree.  Int weights
+ * uniform in [wlo,whi]. */
+gst_status gst_gen_barabasi_albert(int32_t n, int32_t m, int32_t wlo, int32_t whi,
+                                   uint64_t seed, gst_host_csr* out);
+/* R-MAT (a,b,c,d) with 2^scale vertices and edge_factor*2^scale generated edges, symmetrised,
+ * deduplicated, self-loops removed.  w_type GST_W_INT32: uniform [wlo,whi];
+ * GST_W_FLOAT32: lognormal(mu, sigma). */
+gst_status gst_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                        int32_t w_type, int32_t wlo, int32_t whi, double mu, double sigma,
+                        uint64_t seed, gst_host_csr* out);
+/* g groups with sizes uniform in [smin,smax] (clamped to n), members uniform without
+ * replacement within a group (groups may overlap), as in
+ * graph_hash_of_mixed_weighted_generate_random_groups_of_vertices
+ * (rucgraph/graph_hash_of_mixed_weighted_generate_random_groups_of_vertices.h:27-50).
+ * grp_ptr must hold g+1 entries; grp_vtx must hold g*min(smax,n) entries. */
+gst_status gst_gen_groups(int32_t n, int32_t g, int32_t smin, int32_t smax, uint64_t seed,
+                          int32_t* grp_ptr, int32_t* grp_vtx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GST_H */

@@ -1,0 +1,327 @@
+/*
+ * dpbf_oracle.c -- CPU oracle for the group Steiner tree solver.  TEST INFRASTRUCTURE ONLY:
+ * it is linked by tests/, by __graft_entry__.smoke() as the checker, and by bench.py's CPU
+ * baseline leg; the product path never calls it.
+ *
+ * Contents
+ *   orc_dpbf         label-setting DPBF (Ding et al. ICDE'07), restating
+ *                    rucgraph/graph_v_of_v_idealID_DPBF_only_ec.h:133-324:
+ *                      seeding  T(v,p)=0 for every non-empty p within v's groups      (:175-197)
+ *                      pop the cheapest state; the first full-mask pop is optimal      (:202-223)
+ *                      grow     T(u,p) <- T(v,p) + w(v,u)                              (:227-264)
+ *                      merge    T(v,p|p2) <- T(v,p) + T(v,p2), p2 disjoint from p      (:267-316)
+ *                      traceback from (root, full) through grow/merge records          (:90-129)
+ *                    Differences, on purpose: costs are doubles with +inf for an absent
+ *                    state (the reference adds INT_MAX for an absent T(v,p2) at :279-293,
+ *                    which overflows); weights are not truncated to int (:231 truncates);
+ *                    the heap is an indexed binary heap with ties broken by state id.
+ *   orc_dijkstra     multi-source Dijkstra with initial potentials (textbook), used for the
+ *                    single-mask levels and for the closed forms of g <= 4 in the tests.
+ *   orc_bruteforce   exhaustive search over connected vertex subsets touching every group,
+ *                    cost = minimum spanning tree of the induced subgraph (n <= 20).
+ *   orc_dpbf_batch   orc_dpbf over many group sets with a pool of threads (CPU baseline).
+ *
+ * Pinning (tests/test_oracle.py): brute force on tiny graphs, the g=1 (cost 0) and g=2
+ * (group-to-group shortest path) special cases, the g=3 spider and g=4 two-centre closed forms
+ * built from orc_dijkstra, and trees re-validated edge by edge.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_OK 0
+#define ORC_INFEASIBLE 1
+#define ORC_OOM 3
+#define ORC_INVALID 4
+
+/* ---------------------------------------------------------------- indexed binary heap */
+typedef struct {
+    int64_t* h;      /* heap array of state ids */
+    int32_t* pos;    /* state -> heap index, -1 absent, -2 popped */
+    const double* key;
+    int64_t n;
+} heap_t;
+
+static int heap_less(const heap_t* H, int64_t a, int64_t b) {
+    const double ka = H->key[a], kb = H->key[b];
+    if (ka != kb) return ka < kb;
+    return a < b;
+}
+
+static void heap_up(heap_t* H, int64_t i) {
+    const int64_t x = H->h[i];
+    while (i > 0) {
+        const int64_t p = (i - 1) / 2;
+        if (!heap_less(H, x, H->h[p])) break;
+        H->h[i] = H->h[p];
+        H->pos[H->h[i]] = (int32_t)i;
+        i = p;
+    }
+    H->h[i] = x;
+    H->pos[x] = (int32_t)i;
+}
+
+static void heap_down(heap_t* H, int64_t i) {
+    const int64_t x = H->h[i];
+    for (;;) {
+        int64_t c = 2 * i + 1;
+        if (c >= H->n) break;
+        if (c + 1 < H->n && heap_less(H, H->h[c + 1], H->h[c])) c++;
+        if (!heap_less(H, H->h[c], x)) break;
+        H->h[i] = H->h[c];
+        H->pos[H->h[i]] = (int32_t)i;
+        i = c;
+    }
+    H->h[i] = x;
+    H->pos[x] = (int32_t)i;
+}
+
+static void heap_push_or_decrease(heap_t* H, int64_t s) {
+    if (H->pos[s] >= 0) {
+        heap_up(H, H->pos[s]);
+    } else {
+        H->h[H->n] = s;
+        H->pos[s] = (int32_t)H->n;
+        H->n++;
+        heap_up(H, H->n - 1);
+    }
+}
+
+static int64_t heap_pop(heap_t* H) {
+    const int64_t top = H->h[0];
+    H->n--;
+    if (H->n > 0) {
+        H->h[0] = H->h[H->n];
+        H->pos[H->h[0]] = 0;
+        heap_down(H, 0);
+    }
+    H->pos[top] = -2;
+    return top;
+}
+
+/* ---------------------------------------------------------------- DPBF */
+/*
+ * n, row(n+1), col(m), w(m): symmetric CSR; g groups with per-vertex bitmask gm(n).
+ * Outputs: *cost (inf if infeasible), *root, edges (eu, ev, ew) up to cap, *n_edges;
+ * stats[0] = states popped, stats[1] = grow relaxations, stats[2] = merge evaluations.
+ */
+int orc_dpbf(int32_t n, const int64_t* row, const int32_t* col, const double* w, int32_t g,
+             const uint32_t* gm, double* cost_out, int32_t* root_out, int32_t* eu, int32_t* ev,
+             double* ew, int32_t cap, int32_t* n_edges, int64_t* stats) {
+    if (n < 1 || g < 1 || g > 20) return ORC_INVALID;
+    const int64_t S = (int64_t)1 << g;
+    const uint32_t full = (uint32_t)(S - 1);
+    const int64_t NS = (int64_t)n * S;
+    double* cost = (double*)malloc(sizeof(double) * (size_t)NS);
+    int8_t* ty = (int8_t*)malloc((size_t)NS);
+    int32_t* pa = (int32_t*)malloc(sizeof(int32_t) * (size_t)NS);
+    int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)NS);
+    int64_t* hp = (int64_t*)malloc(sizeof(int64_t) * (size_t)NS);
+    if (!cost || !ty || !pa || !pos || !hp) {
+        free(cost); free(ty); free(pa); free(pos); free(hp);
+        return ORC_OOM;
+    }
+    for (int64_t s = 0; s < NS; ++s) { cost[s] = INFINITY; ty[s] = -1; pos[s] = -1; }
+    heap_t H = {hp, pos, cost, 0};
+    int64_t pops = 0, relax = 0, merges = 0;
+
+    for (int32_t v = 0; v < n; ++v) {
+        const uint32_t gv = gm[v] & full;
+        if (!gv) continue;
+        /* every non-empty p inside v's group set is a single-vertex tree */
+        for (uint32_t p = gv; p; p = (p - 1) & gv) {
+            const int64_t s = (int64_t)v * S + p;
+            cost[s] = 0.0; ty[s] = 0;
+            heap_push_or_decrease(&H, s);
+        }
+    }
+
+    int32_t root = -1;
+    while (H.n > 0) {
+        const int64_t s = heap_pop(&H);
+        pops++;
+        const int32_t v = (int32_t)(s / S);
+        const uint32_t p = (uint32_t)(s % S);
+        const double cvp = cost[s];
+        if (p == full) { root = v; break; }
+        /* grow */
+        for (int64_t e = row[v]; e < row[v + 1]; ++e) {
+            const int32_t u = col[e];
+            const int64_t su = (int64_t)u * S + p;
+            if (pos[su] == -2) continue;
+            relax++;
+            const double c = cvp + w[e];
+            if (c < cost[su]) {
+                cost[su] = c; ty[su] = 1; pa[su] = (int32_t)e;   /* arc v->u in v's row */
+                heap_push_or_decrease(&H, su);
+            }
+        }
+        /* merge with every disjoint p2 that has a tree at v */
+        const uint32_t comp = full ^ p;
+        for (uint32_t p2 = comp; p2; p2 = (p2 - 1) & comp) {
+            const int64_t s2 = (int64_t)v * S + p2;
+            if (!(cost[s2] < INFINITY)) continue;
+            merges++;
+            const int64_t sx = (int64_t)v * S + (p | p2);
+            if (pos[sx] == -2) continue;
+            const double c = cvp + cost[s2];
+            if (c < cost[sx]) {
+                cost[sx] = c; ty[sx] = 2; pa[sx] = (int32_t)p;
+                heap_push_or_decrease(&H, sx);
+            }
+        }
+    }
+    if (stats) { stats[0] = pops; stats[1] = relax; stats[2] = merges; }
+    int rc = ORC_OK;
+    int32_t ne = 0;
+    if (root < 0) {
+        *cost_out = INFINITY;
+        rc = ORC_INFEASIBLE;
+    } else {
+        *cost_out = cost[(int64_t)root * S + full];
+        /* traceback with an explicit stack of states; reuse hp as the stack */
+        int64_t sp = 0;
+        hp[sp++] = (int64_t)root * S + full;
+        while (sp > 0) {
+            const int64_t s = hp[--sp];
+            const int32_t v = (int32_t)(s / S);
+            const uint32_t p = (uint32_t)(s % S);
+            if (ty[s] == 0) continue;
+            if (ty[s] == 1) {
+                const int64_t e = pa[s];
+                /* arc e lies in the row of the source vertex x with col[e] == v */
+                int32_t lo = 0, hi = n;  /* find x: row[x] <= e < row[x+1] */
+                while (hi - lo > 1) {
+                    const int32_t mid = (lo + hi) / 2;
+                    if (row[mid] <= e) lo = mid; else hi = mid;
+                }
+                const int32_t x = lo;
+                if (ne < cap && eu) { eu[ne] = x; ev[ne] = v; ew[ne] = w[e]; }
+                ne++;
+                hp[sp++] = (int64_t)x * S + p;
+            } else if (ty[s] == 2) {
+                const uint32_t p1 = (uint32_t)pa[s];
+                hp[sp++] = (int64_t)v * S + p1;
+                hp[sp++] = (int64_t)v * S + (p ^ p1);
+            } else {
+                rc = ORC_INVALID;
+                break;
+            }
+        }
+    }
+    if (root_out) *root_out = root;
+    if (n_edges) *n_edges = ne;
+    free(cost); free(ty); free(pa); free(pos); free(hp);
+    return rc;
+}
+
+/* ---------------------------------------------------------------- Dijkstra */
+/* dist[v] = min_s init[s] + d(s,v); init[v] = INFINITY marks a non-source */
+int orc_dijkstra(int32_t n, const int64_t* row, const int32_t* col, const double* w, const double* init,
+                 double* dist) {
+    int64_t* hp = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    if (!hp || !pos) { free(hp); free(pos); return ORC_OOM; }
+    heap_t H = {hp, pos, dist, 0};
+    for (int32_t v = 0; v < n; ++v) {
+        dist[v] = init[v];
+        pos[v] = -1;
+        if (init[v] < INFINITY) heap_push_or_decrease(&H, v);
+    }
+    while (H.n > 0) {
+        const int64_t v = heap_pop(&H);
+        for (int64_t e = row[v]; e < row[v + 1]; ++e) {
+            const int32_t u = col[e];
+            if (pos[u] == -2) continue;
+            const double c = dist[v] + w[e];
+            if (c < dist[u]) { dist[u] = c; heap_push_or_decrease(&H, u); }
+        }
+    }
+    free(hp); free(pos);
+    return ORC_OK;
+}
+
+/* ---------------------------------------------------------------- brute force */
+/* min over connected vertex subsets touching every group of the MST weight of the induced
+ * subgraph; n <= 20.  Returns INFINITY when infeasible. */
+double orc_bruteforce(int32_t n, const int64_t* row, const int32_t* col, const double* w, int32_t g,
+                      const uint32_t* gm) {
+    if (n < 1 || n > 20) return NAN;
+    const uint32_t full = (uint32_t)((1u << g) - 1);
+    double adj[20][20];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) adj[i][j] = INFINITY;
+    for (int v = 0; v < n; ++v)
+        for (int64_t e = row[v]; e < row[v + 1]; ++e)
+            if (w[e] < adj[v][col[e]]) adj[v][col[e]] = w[e];
+    double best = INFINITY;
+    for (uint32_t sub = 1; sub < (1u << n); ++sub) {
+        uint32_t cover = 0;
+        for (int v = 0; v < n; ++v)
+            if (sub >> v & 1u) cover |= gm[v];
+        if ((cover & full) != full) continue;
+        /* Prim on the induced subgraph */
+        int verts[20], k = 0;
+        for (int v = 0; v < n; ++v)
+            if (sub >> v & 1u) verts[k++] = v;
+        double key[20];
+        int in[20];
+        for (int i = 0; i < k; ++i) { key[i] = INFINITY; in[i] = 0; }
+        key[0] = 0.0;
+        double total = 0.0;
+        int ok = 1;
+        for (int it = 0; it < k; ++it) {
+            int bi = -1;
+            for (int i = 0; i < k; ++i)
+                if (!in[i] && (bi < 0 || key[i] < key[bi])) bi = i;
+            if (!(key[bi] < INFINITY)) { ok = 0; break; }
+            in[bi] = 1;
+            total += key[bi];
+            for (int i = 0; i < k; ++i)
+                if (!in[i] && adj[verts[bi]][verts[i]] < key[i]) key[i] = adj[verts[bi]][verts[i]];
+        }
+        if (ok && total < best) best = total;
+    }
+    return best;
+}
+
+/* ---------------------------------------------------------------- batch (CPU baseline) */
+typedef struct {
+    int32_t n; const int64_t* row; const int32_t* col; const double* w;
+    int32_t nq; const int32_t* gq; const uint32_t* gms;  /* gms: nq x n group masks */
+    double* costs;
+    int32_t next;
+    pthread_mutex_t mu;
+} batch_t;
+
+static void* batch_worker(void* arg) {
+    batch_t* B = (batch_t*)arg;
+    for (;;) {
+        pthread_mutex_lock(&B->mu);
+        const int32_t q = B->next++;
+        pthread_mutex_unlock(&B->mu);
+        if (q >= B->nq) break;
+        double c = INFINITY;
+        int32_t root = -1, ne = 0;
+        orc_dpbf(B->n, B->row, B->col, B->w, B->gq[q], B->gms + (int64_t)q * B->n, &c, &root, NULL, NULL, NULL, 0,
+                 &ne, NULL);
+        B->costs[q] = c;
+    }
+    return NULL;
+}
+
+int orc_dpbf_batch(int32_t nthreads, int32_t n, const int64_t* row, const int32_t* col, const double* w,
+                   int32_t nq, const int32_t* gq, const uint32_t* gms, double* costs) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    batch_t B;
+    B.n = n; B.row = row; B.col = col; B.w = w; B.nq = nq; B.gq = gq; B.gms = gms; B.costs = costs; B.next = 0;
+    pthread_mutex_init(&B.mu, NULL);
+    pthread_t th[256];
+    for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, batch_worker, &B);
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&B.mu);
+    return ORC_OK;
+}

@@ -1,0 +1,116 @@
+"""CPU: the C-ABI library loads, exports every declared symbol, and its host logic behaves."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gpu4gst_code_amd as gst
+from gpu4gst_code_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for hdr in os.listdir(os.path.join(ROOT, "include")):
+        if not hdr.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(gst_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names <= set(_abi.SIGNATURES), names - set(_abi.SIGNATURES)
+
+
+def test_status_strings():
+    lib = _abi.load()
+    for code, name in _abi.STATUS_NAMES.items():
+        assert lib.gst_status_str(code).decode() == name
+
+
+def test_no_device_fails_loudly():
+    """Without a GPU the solver refuses to run (there is no CPU fallback)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(gst.GstError) as e:
+        gst.Solver(0)
+    assert e.value.status == _abi.GST_NO_DEVICE
+
+
+def _digest(G):
+    return hash((G.row_ptr.tobytes(), G.col.tobytes(), G.w.tobytes()))
+
+
+def test_generators_deterministic_and_shaped():
+    a = gst.gen_random_connected(1000, 5000, 1, 100, seed=5)
+    b = gst.gen_random_connected(1000, 5000, 1, 100, seed=5)
+    c = gst.gen_random_connected(1000, 5000, 1, 100, seed=6)
+    assert _digest(a) == _digest(b) != _digest(c)
+    assert a.n == 1000 and a.m == 10000
+    assert a.w.min() >= 1 and a.w.max() <= 100
+    ba = gst.gen_barabasi_albert(2000, 5, 1, 1000, seed=1)
+    assert ba.m == 2 * (10 + (2000 - 5) * 5)
+    r = gst.gen_rmat(10, 8, weights="lognormal", seed=2)
+    assert r.n == 1024 and r.w.dtype == np.float32 and (r.w > 0).all()
+    g1 = gst.gen_groups(1000, 6, 10, 200, seed=9)
+    g2 = gst.gen_groups(1000, 6, 10, 200, seed=9)
+    assert np.array_equal(g1.grp_vtx, g2.grp_vtx) and g1.g == 6
+    for i in range(6):
+        m = g1.members(i)
+        assert 10 <= len(m) <= 200 and len(set(m.tolist())) == len(m)
+
+
+def _symmetric(G):
+    src = np.repeat(np.arange(G.n), np.diff(G.row_ptr))
+    fwd = set(zip(src.tolist(), G.col.tolist(), G.w.tolist()))
+    bwd = set(zip(G.col.tolist(), src.tolist(), G.w.tolist()))
+    return fwd == bwd and not np.any(src == G.col)
+
+
+def test_generated_graphs_are_symmetric_simple():
+    for G in (gst.gen_random_connected(300, 900, 1, 9, seed=1), gst.gen_barabasi_albert(500, 3, 1, 9, seed=2),
+              gst.gen_rmat(9, 8, seed=3)):
+        assert _symmetric(G)
+        for v in range(G.n):
+            row = G.col[G.row_ptr[v]:G.row_ptr[v + 1]]
+            assert np.all(np.diff(row) > 0)
+
+
+def test_validate_tree_rejects_bad_trees():
+    G = gst.HostGraph.from_edges(5, [0, 1, 2, 3, 0], [1, 2, 3, 4, 4], np.array([1, 2, 3, 4, 5], np.int32))
+    groups = gst.groups_from_lists([[0], [3]])
+    assert gst.validate_tree(G, groups, [0, 1, 2], [1, 2, 3]) == 6.0
+    assert gst.validate_tree(G, groups, [0, 4], [4, 3]) == 9.0
+    with pytest.raises(gst.GstError):  # cycle
+        gst.validate_tree(G, groups, [0, 1, 2, 3, 0], [1, 2, 3, 4, 4])
+    with pytest.raises(gst.GstError):  # disconnected
+        gst.validate_tree(G, groups, [0, 2], [1, 3])
+    with pytest.raises(gst.GstError):  # not an edge
+        gst.validate_tree(G, groups, [0], [3])
+    with pytest.raises(gst.GstError):  # group 1 untouched
+        gst.validate_tree(G, groups, [0], [1])
+    single = gst.groups_from_lists([[2], [2, 4]])
+    assert gst.validate_tree(G, single, [], [], root=2) == 0.0
+
+
+def test_gstp_roundtrip(tmp_path):
+    from gpu4gst_code_amd.gstp import read_gstp, write_gstp
+    G = gst.gen_random_connected(50, 120, 1, 9, seed=4)
+    groups = gst.gen_groups(50, 3, 2, 5, seed=5)
+    p = tmp_path / "x.stp"
+    write_gstp(str(p), G, groups)
+    G2, groups2, lam = read_gstp(str(p))
+    assert np.array_equal(G.row_ptr, G2.row_ptr) and np.array_equal(G.col, G2.col) and np.array_equal(G.w, G2.w)
+    for i in range(3):
+        assert sorted(groups.members(i).tolist()) == groups2.members(i).tolist()

@@ -1,0 +1,108 @@
+"""CPU: pin the oracle (label-setting DPBF restatement) against brute force, closed forms and goldens."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpu4gst_code_amd as gst
+import oracle
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+@pytest.mark.parametrize("s", range(0, 120, 7))
+def test_tiny_vs_bruteforce(s):
+    n = 10 + s % 5
+    g = 1 + s % 6
+    G = gst.gen_random_connected(n, n + 6, 1, 20, seed=5000 + s)
+    groups = gst.gen_groups(n, g, 1, 3, seed=6000 + s)
+    c = oracle.dpbf_cost(G, groups)
+    assert c == oracle.bruteforce(G, groups)
+    assert c == GOLD["tiny"]["costs"][str(s)]
+
+
+@pytest.mark.parametrize("s", range(40))
+def test_float_weights_vs_bruteforce(s):
+    """Float weights: oracle (double) equals brute force up to summation order."""
+    rng = np.random.default_rng(s)
+    n = 12
+    G0 = gst.gen_random_connected(n, 20, 1, 1, seed=700 + s)
+    w = rng.lognormal(0.0, 1.0, size=G0.m // 2).astype(np.float32)
+    src = np.repeat(np.arange(n), np.diff(G0.row_ptr))
+    keep = src < G0.col
+    G = gst.HostGraph.from_edges(n, src[keep], G0.col[keep], w)
+    groups = gst.gen_groups(n, 1 + s % 5, 1, 3, seed=800 + s)
+    c = oracle.dpbf_cost(G, groups)
+    b = oracle.bruteforce(G, groups)
+    assert abs(c - b) <= 1e-12 * max(1.0, b)
+
+
+@pytest.mark.parametrize("g", [1, 2, 3, 4])
+@pytest.mark.parametrize("s", range(4))
+def test_closed_forms(g, s):
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=40 + s)
+    groups = gst.gen_groups(G.n, g, 5, 20, seed=50 + s + 10 * g)
+    assert oracle.dpbf_cost(G, groups) == oracle.closed_form(G, groups)
+
+
+def test_closed_form_g4_ba():
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=3)
+    for q in range(2):
+        groups = gst.gen_groups(G.n, 4, 10, 200, seed=90 + q)
+        assert oracle.dpbf_cost(G, groups) == oracle.closed_form(G, groups)
+
+
+def test_c1_goldens():
+    for s in ("0", "1", "2", "17", "199"):
+        G = gst.gen_random_connected(1000, 5000, 1, 100, seed=int(s))
+        groups = gst.gen_groups(G.n, 3, 5, 20, seed=int(s) + 1)
+        assert oracle.dpbf_cost(G, groups) == GOLD["c1"]["costs"][s]
+
+
+def test_reference_fixture():
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "gstp_fixture.npz"))
+    G = gst.HostGraph(z["row_ptr"], z["col"], z["w"])
+    groups = gst.Groups(z["grp_ptr"], z["grp_vtx"])
+    c, root, eu, ev, ew, _ = oracle.dpbf(G, groups)
+    assert c == GOLD["gstp_fixture"]["cost"] == 16.0
+    assert oracle.closed_form(G, groups) == c
+    assert gst.validate_tree(G, groups, eu, ev, root) == c
+
+
+def test_oracle_tree_is_valid_and_costs_match():
+    for s in range(5):
+        G = gst.gen_random_connected(1000, 5000, 1, 100, seed=300 + s)
+        groups = gst.gen_groups(G.n, 5, 5, 20, seed=400 + s)
+        c, root, eu, ev, ew, _ = oracle.dpbf(G, groups)
+        assert gst.validate_tree(G, groups, eu, ev, root) == c
+        assert float(np.sum(ew)) == c
+
+
+def test_infeasible():
+    # two components: groups on different sides
+    G = gst.HostGraph.from_edges(6, [0, 1, 3, 4], [1, 2, 4, 5], np.array([1, 1, 1, 1], np.int32))
+    groups = gst.groups_from_lists([[0], [5]])
+    assert oracle.dpbf_cost(G, groups) == np.inf
+    assert oracle.bruteforce(G, groups) == np.inf
+
+
+def test_planted_mistakes_are_caught():
+    """A plausible oracle mistake must fail a pin: dropping a group bit (wrong member mapping)
+    or counting an edge twice changes the cost seen by the closed form / brute force."""
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=1)
+    groups = gst.gen_groups(G.n, 3, 5, 20, seed=2)
+    good = oracle.dpbf_cost(G, groups)
+    # mistake 1: group 2 loses its first member
+    lists = [list(groups.members(i)) for i in range(3)]
+    bad_groups = gst.groups_from_lists([lists[0], lists[1], lists[2][1:]])
+    # the pin recomputed on the true groups disagrees unless the dropped member was irrelevant
+    diffs = 0
+    for drop in range(len(lists[2])):
+        bg = gst.groups_from_lists([lists[0], lists[1], lists[2][:drop] + lists[2][drop + 1:]])
+        diffs += oracle.dpbf_cost(G, bg) != oracle.closed_form(G, groups)
+    assert diffs >= 1
+    # mistake 2: doubled weights on the traced tree break the cost identity
+    c, root, eu, ev, ew, _ = oracle.dpbf(G, groups)
+    assert 2 * float(np.sum(ew)) != good
+    _ = bad_groups
