@@ -259,13 +259,15 @@ class Solver:
         return dg
 
     def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
-              want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0) -> Result:
+              want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0,
+              pull_permille: int = 0) -> Result:
         opts = _abi.gst_opts()
         opts.ub = float(ub) if ub is not None else 0.0
         opts.prune_lb = 1 if prune_lb else 0
         opts.want_tree = 1 if want_tree else 0
         opts.rounds_per_sync = rounds_per_sync
         opts.chunk_arcs = chunk_arcs
+        opts.pull_permille = pull_permille
         res = _abi.gst_result()
         grp = groups.as_c()
         st = self.lib.gst_solve(self.ctx, graph.handle, C.byref(grp), C.byref(opts), C.byref(res))

@@ -36,7 +36,8 @@ class gst_groups(C.Structure):
 
 class gst_opts(C.Structure):
     _fields_ = [("ub", C.c_double), ("prune_lb", C.c_int32), ("want_tree", C.c_int32),
-                ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("reserved", C.c_int32 * 8)]
+                ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
 
 
 class gst_stats(C.Structure):

@@ -27,7 +27,7 @@ HIP_SOURCES = ["gst_dpbf.hip"]
 # on this platform); hipcc's host pass warns that it does not know the feature, which is harmless.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wno-unused-command-line-argument", "-I" + INCLUDE]
+             "-Wno-unused-command-line-argument", "-Wno-unused-value", "-Wno-unused-result", "-I" + INCLUDE]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + INCLUDE]
 
 
@@ -51,6 +51,9 @@ def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     shutil.move(tmp, LIB)
+    # the reference-shaped C++ caller (include/gst.hpp) used by the wrapper test
+    _run([CXX, "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(HERE, "examples", "dpbf_gpu_example.cpp"),
+          "-L" + HERE, "-lgst", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BUILD, "dpbf_gpu_example")])
     return LIB
 
 

@@ -39,7 +39,6 @@ namespace {
 constexpr int WAVE = 64;
 constexpr int GROW_BLOCK = 256;              // 4 waves
 constexpr int GROW_WAVES = GROW_BLOCK / WAVE;
-constexpr int MERGE_BLOCK = 64;              // one wave per workgroup (its LDS row is private)
 constexpr int TRACE_STACK = 512;
 
 // device-side control block of the grow rounds of one level
@@ -75,6 +74,9 @@ struct GrowArgs {
     uint32_t chunk;        // arcs per work item
     uint32_t ub;           // cost bits of the bound (GST_INF_COST: none)
     uint32_t use_lb;
+    const uint64_t* all_items;  // every (vertex, chunk) item of the graph, for pull rounds
+    uint32_t n_all;
+    uint32_t pull_thresh;  // a round with at least this many queued items pulls over all items
 };
 
 struct MergeArgs {
@@ -101,6 +103,8 @@ struct MergeArgs {
     uint32_t ub;
     uint32_t use_lb;
     uint32_t is_full;
+    uint32_t VB;           // vertices per workgroup tile
+    uint32_t pairs_in_lds;
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -159,10 +163,26 @@ __device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
     return v;
 }
 
-// Wave-aggregated append of n work items (v, chunk 0..n-1) per lane: one atomicAdd per wave,
-// exclusive prefix sum over lanes.  Must be called by all lanes of the wave (convergent).
-__device__ __forceinline__ void wave_enqueue(uint64_t* list, uint32_t* cnt, uint32_t v,
-                                             uint32_t n, uint32_t& enq_stat) {
+// Work-item queue of one workgroup in LDS.  Waves reserve slots with an LDS atomic (one per
+// wave and step), and the workgroup appends its items to the global list with a single global
+// atomicAdd at the end, so the global counter sees one atomic per workgroup instead of one per
+// wave-step.  A reservation that does not fit goes straight to the global list.
+constexpr uint32_t QCAP = 2048;
+struct BlockQueue {
+    uint64_t items[QCAP];
+    uint32_t n;       // slots reserved
+    uint32_t valid;   // items[0, min(n, valid)) hold queued items
+    uint32_t gbase;
+};
+
+__device__ __forceinline__ void bq_init(BlockQueue& q) {
+    if (threadIdx.x == 0) { q.n = 0; q.valid = QCAP; }
+}
+
+// Append n items (v, chunk 0..n-1) per lane.  Convergent: every lane of the wave calls it.
+__device__ __forceinline__ void bq_push(BlockQueue& q, uint64_t* list, uint32_t* cnt, uint32_t v,
+                                        uint32_t n, uint32_t& enq_stat) {
+    if (__ballot(n != 0) == 0) return;
     const uint32_t lane = lane_id();
     uint32_t incl = n;
 #pragma unroll
@@ -171,13 +191,35 @@ __device__ __forceinline__ void wave_enqueue(uint64_t* list, uint32_t* cnt, uint
         if (lane >= (uint32_t)d) incl += y;
     }
     const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-    if (total == 0) return;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(cnt, total);
-    base = __shfl(base, 0, WAVE);
-    uint32_t pos = base + incl - n;
-    for (uint32_t c = 0; c < n; ++c) list[pos + c] = (uint64_t)v | ((uint64_t)c << 32);
+    uint32_t b = 0, gb = 0xFFFFFFFFu;
+    if (lane == 0) {
+        b = atomicAdd(&q.n, total);
+        if (b + total > QCAP) {
+            atomicMin(&q.valid, b);
+            gb = atomicAdd(cnt, total);
+        }
+    }
+    b = __shfl(b, 0, WAVE);
+    gb = __shfl(gb, 0, WAVE);
+    const uint32_t off = incl - n;
+    for (uint32_t c = 0; c < n; ++c) {
+        const uint64_t item = (uint64_t)v | ((uint64_t)c << 32);
+        if (gb == 0xFFFFFFFFu) q.items[b + off + c] = item;
+        else list[gb + off + c] = item;
+    }
     enq_stat += n;
+}
+
+// Publish the workgroup's queued items (all threads of the block call it).
+__device__ __forceinline__ void bq_flush(BlockQueue& q, uint64_t* list, uint32_t* cnt) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t k = min(q.n, q.valid);
+        q.gbase = k ? atomicAdd(cnt, k) : 0u;
+        q.n = k;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < q.n; i += blockDim.x) list[q.gbase + i] = q.items[i];
 }
 
 __device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, uint32_t chunk) {
@@ -216,32 +258,65 @@ __global__ void k_ctl_reset(RoundCtl* ctl) {
     }
 }
 
-// Level 1: T(v,{i}) = 0 for members of group i, inf elsewhere; members seed the frontier.
+// Level 1: T(v,{i}) = 0 for members of group i, inf elsewhere; members with arcs are flagged as
+// the first frontier (bitmap = their groups).
 __global__ void k_init_level1(uint64_t* T1, const uint32_t* gm, const uint32_t* row, uint32_t n,
-                              uint32_t g, uint32_t chunk, uint64_t* list0, uint64_t* chg0,
-                              uint32_t* inq0, RoundCtl* ctl, unsigned long long* stats,
-                              int seed_frontier) {
-    const uint32_t lane = lane_id();
-    uint32_t enq = 0;
-    const uint64_t tid0 = (uint64_t)blockIdx.x * blockDim.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = tid0; base < n; base += stride) {  // wave-uniform trip count
-        const uint64_t v = base + threadIdx.x;
-        uint32_t nitems = 0;
-        if (v < n) {
-            const uint32_t gmv = gm[v];
-            for (uint32_t i = 0; i < g; ++i)
-                T1[v * g + i] = ((gmv >> i) & 1u) ? (uint64_t)GST_PRED_LEAF : GST_INF_WORD;
-            if (gmv && seed_frontier) {
-                nitems = num_chunks(row, (uint32_t)v, chunk);
-                if (nitems) { chg0[v] = gmv; inq0[v] = 1; }
-            }
-        }
-        wave_enqueue(list0, &ctl->cnt[0], (uint32_t)v, nitems, enq);
+                              uint32_t g, uint64_t* chg0, uint32_t* inq0, int seed_frontier) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t gmv = gm[v];
+        for (uint32_t i = 0; i < g; ++i)
+            T1[v * g + i] = ((gmv >> i) & 1u) ? (uint64_t)GST_PRED_LEAF : GST_INF_WORD;
+        if (gmv && seed_frontier && row[v + 1] > row[v]) { chg0[v] = gmv; inq0[v] = 1; }
     }
-    (void)lane;
-    enq = wave_sum32(enq);
-    if (lane_id() == 0 && enq) atomicAdd(&stats[ST_ENQ], (unsigned long long)enq);
+}
+
+// Work-item list from per-vertex flags (flags == nullptr: every vertex with arcs), one item per
+// chunk of the vertex's row; one global atomic per workgroup.
+__global__ void __launch_bounds__(256) k_compact(const uint32_t* flags, const uint32_t* row, uint32_t n,
+                                                 uint32_t chunk, uint64_t* list, uint32_t* cnt_out,
+                                                 unsigned long long* stats) {
+    __shared__ uint32_t s_wtot[4];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = base + threadIdx.x;
+        uint32_t k = 0;
+        if (v < n && (flags == nullptr || flags[v])) k = (row[v + 1] - row[v] + chunk - 1) / chunk;
+        uint32_t incl = k;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d, WAVE);
+            if (lane >= (uint32_t)d) incl += y;
+        }
+        if (lane == WAVE - 1) s_wtot[wid] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t tot = s_wtot[0] + s_wtot[1] + s_wtot[2] + s_wtot[3];
+            s_base = tot ? atomicAdd(cnt_out, tot) : 0u;
+            if (tot && stats) atomicAdd(&stats[ST_ENQ], (unsigned long long)tot);
+        }
+        __syncthreads();
+        uint32_t off = s_base;
+        for (uint32_t w = 0; w < wid; ++w) off += s_wtot[w];
+        off += incl - k;
+        for (uint32_t c = 0; c < k; ++c) list[off + c] = v | ((uint64_t)c << 32);
+        __syncthreads();
+    }
+}
+
+// End of a grow round: reset the counter of the list cleaned in this round and advance, or mark
+// the level finished when the round had no items.
+__global__ void k_round_end(RoundCtl* ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ctl->level_done) return;
+    const uint32_t r = ctl->round;
+    if (ctl->cnt[r % 3] == 0) {
+        ctl->level_done = 1;
+    } else {
+        ctl->cnt[(r + 2) % 3] = 0;
+        ctl->round = r + 1;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -249,6 +324,9 @@ __global__ void k_init_level1(uint64_t* T1, const uint32_t* gm, const uint32_t* 
 // ---------------------------------------------------------------------------------------
 template <bool FLT>
 __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
+    __shared__ BlockQueue q;
+    __shared__ unsigned long long s_stats[ST_NSTAT];
+    __shared__ uint32_t s_ovf;
     RoundCtl* ctl = a.ctl;
     if (ctl->level_done) return;
     const uint32_t r = ctl->round;
@@ -257,6 +335,9 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     const uint32_t n_prv = ctl->cnt[prv];
     const uint32_t lane = lane_id();
     const uint32_t W = a.W, m = a.m;
+    bq_init(q);
+    if (threadIdx.x < ST_NSTAT) s_stats[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_ovf = 0;
 
     // phase A: clear the change bitmaps / queue flags of round r-1's items
     {
@@ -272,6 +353,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             }
         }
     }
+    __syncthreads();
 
     // phase B: relax the items of round r
     uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0;
@@ -285,8 +367,102 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     uint64_t* lnxt = a.list[nxt];
     uint32_t* cntn = &ctl->cnt[nxt];
     const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
+    const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
 
-    for (uint32_t i = gwave; i < n_cur; i += nwaves) {
+    // ---- pull round: every vertex takes the best grow from its neighbours (dense frontier) ----
+    for (uint32_t i = gwave; pull && i < a.n_all; i += nwaves) {
+        const uint64_t it = a.all_items[i];
+        const uint32_t u = (uint32_t)it;
+        const uint32_t c = (uint32_t)(it >> 32);
+        const uint32_t rs = a.row[u], re = a.row[u + 1];
+        const uint32_t e0 = rs + c * a.chunk;
+        const uint32_t e1 = min(re, e0 + a.chunk);
+        st_items++;
+        st_arcs += e1 - e0;
+        uint32_t dall[GST_MAX_GROUPS];
+        if (prune_lb) {
+            const uint32_t d1 = lane < a.g ? (uint32_t)(a.T1[(uint64_t)u * a.g + lane] >> 32) : GST_INF_COST;
+#pragma unroll
+            for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi) dall[gi] = __shfl(d1, gi, WAVE);
+        }
+        for (uint32_t mb = 0; mb < m; mb += WAVE) {
+            const uint32_t na = min((uint32_t)WAVE, m - mb);
+            const uint32_t A = WAVE / na;
+            const uint32_t t = lane % na, s_arc = lane / na;
+            const bool lane_ok = s_arc < A;
+            const uint32_t rk = mb + t;
+            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+            const unsigned long long own = *addr;
+            const uint32_t ocost = (uint32_t)(own >> 32);
+            uint32_t lb = 0;
+            if (prune_lb) {
+                const uint32_t X = a.masks[rk];
+#pragma unroll
+                for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi)
+                    if (gi < a.g && !((X >> gi) & 1u))
+                        lb = (dall[gi] == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, dall[gi]);
+            }
+            unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;  // best via positive / zero-effect edges
+            const uint32_t len = e1 - e0;
+            const uint32_t steps = (len + A - 1) / A;
+            for (uint32_t s = 0; s < steps; ++s) {
+                const uint32_t e = e0 + s * A + s_arc;
+                if (lane_ok && e < e1) {
+                    const uint32_t nb = (uint32_t)a.col[e];
+                    const uint32_t cn = (uint32_t)(a.Tk[(uint64_t)nb * m + rk] >> 32);
+                    st_relax++;
+                    if (cn < ocost) {  // only a cheaper neighbour can lower (or tie) our cost
+                        const uint32_t cand = add_cost<FLT>(cn, a.w[e], ovf);
+                        if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
+                            const unsigned long long nw = ((unsigned long long)cand << 32) | e;
+                            if (cand > cn) bs = nw < bs ? nw : bs;
+                            else if (cand < ocost) bz = nw < bz ? nw : bz;
+                        }
+                    }
+                }
+            }
+            for (uint32_t k2 = 1; k2 < A; k2 <<= 1) {
+                const unsigned long long y1 = __shfl_down(bs, k2 * na, WAVE);
+                const unsigned long long y2 = __shfl_down(bz, k2 * na, WAVE);
+                if ((s_arc % (2 * k2)) == 0 && s_arc + k2 < A) {
+                    bs = y1 < bs ? y1 : bs;
+                    bz = y2 < bz ? y2 : bz;
+                }
+            }
+            uint32_t enq_v = 0, enq_n = 0;
+            if (s_arc == 0) {
+                bool improved = false;
+                if (bz != GST_INF_WORD && (uint32_t)(bz >> 32) < (uint32_t)(bs >> 32)) {
+                    // a zero-effect edge wins: accept only a strict cost decrease
+                    const uint32_t cz = (uint32_t)(bz >> 32);
+                    unsigned long long curw = own;
+                    while (cz < (uint32_t)(curw >> 32)) {
+                        const unsigned long long old = atomicCAS(addr, curw, bz);
+                        if (old == curw) { improved = true; break; }
+                        curw = old;
+                    }
+                } else if (bs < own) {
+                    const unsigned long long old = atomicMin(addr, bs);
+                    improved = (uint32_t)(bs >> 32) < (uint32_t)(old >> 32);
+                }
+                if (improved) {
+                    st_upd++;
+                    const uint64_t bitv = 1ull << (rk & 63);
+                    unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + (rk >> 6)];
+                    const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
+                    const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
+                    if (first) {
+                        enq_v = u;
+                        enq_n = num_chunks(a.row, u, a.chunk);
+                    }
+                }
+            }
+            bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
+        }
+    }
+
+    // ---- push round: queued vertices relax their changed masks into their neighbours ----
+    for (uint32_t i = gwave; !pull && i < n_cur; i += nwaves) {
         const uint64_t it = lcur[i];
         const uint32_t v = (uint32_t)it;
         const uint32_t c = (uint32_t)(it >> 32);
@@ -307,7 +483,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             uint64_t bm = ccur[(uint64_t)v * W + wd];
             if (bm == 0) continue;
             // drop masks whose cost (plus lower bound) reaches the bound
-            {
+            if (a.ub != GST_INF_COST) {
                 const uint32_t na0 = (uint32_t)__popcll(bm);
                 uint64_t keep = 0;
                 if (lane < na0) {
@@ -341,50 +517,55 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             for (uint32_t s = 0; s < steps; ++s) {
                 const uint32_t e = e0 + s * A + s_arc;
                 uint32_t enq_v = 0, enq_n = 0;
-                if (lane_ok && e < e1) {
+                if (lane_ok && e < e1 && cv != GST_INF_COST) {
                     const uint32_t u = (uint32_t)a.col[e];
                     const uint32_t wt = a.w[e];
-                    const uint32_t ra = a.rev[e];
                     const uint32_t cand = add_cost<FLT>(cv, wt, ovf);
                     st_relax++;
                     if (cand != GST_INF_COST && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
                         unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
                         unsigned long long curw = *addr;
-                        const unsigned long long nw = ((unsigned long long)cand << 32) | ra;
-                        bool improved = false;
-                        if (nw < curw) {
-                            if (cand == cv) {
-                                // effective zero-weight edge: only strict cost decreases, so
-                                // equal-cost predecessors can never close a cycle
-                                while (cand < (uint32_t)(curw >> 32)) {
-                                    const unsigned long long old = atomicCAS(addr, curw, nw);
-                                    if (old == curw) { improved = true; break; }
-                                    curw = old;
+                        if (cand <= (uint32_t)(curw >> 32)) {
+                            const unsigned long long nw = ((unsigned long long)cand << 32) | a.rev[e];
+                            bool improved = false;
+                            if (nw < curw) {
+                                if (cand == cv) {
+                                    // effective zero-weight edge: only strict cost decreases, so
+                                    // equal-cost predecessors can never close a cycle
+                                    while (cand < (uint32_t)(curw >> 32)) {
+                                        const unsigned long long old = atomicCAS(addr, curw, nw);
+                                        if (old == curw) { improved = true; break; }
+                                        curw = old;
+                                    }
+                                } else {
+                                    const unsigned long long old = atomicMin(addr, nw);
+                                    improved = cand < (uint32_t)(old >> 32);
                                 }
-                            } else {
-                                const unsigned long long old = atomicMin(addr, nw);
-                                improved = cand < (uint32_t)(old >> 32);
                             }
-                        }
-                        if (improved) {
-                            st_upd++;
-                            const unsigned long long ob =
-                                atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + wsel], (unsigned long long)bitv);
-                            if (!(ob & bitv)) {
-                                if (atomicExch(&inxt[u], 1u) == 0u) {
-                                    enq_v = u;
-                                    enq_n = num_chunks(a.row, u, a.chunk);
+                            if (improved) {
+                                st_upd++;
+                                unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
+                                if (!(*cw & bitv)) {
+                                    const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
+                                    // first change of u in this round: queue it (one word per
+                                    // vertex: the word itself was empty; several words: flag)
+                                    const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
+                                    if (first) {
+                                        enq_v = u;
+                                        enq_n = num_chunks(a.row, u, a.chunk);
+                                    }
                                 }
                             }
                         }
                     }
                 }
-                wave_enqueue(lnxt, cntn, enq_v, enq_n, st_enq);
+                bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
             }
         }
     }
+    bq_flush(q, lnxt, cntn);
 
-    // statistics: one atomic per wave
+    // statistics: LDS reduction, one global atomic per workgroup and counter
     {
         const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
         const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
@@ -393,136 +574,136 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
         const uint32_t se = wave_sum32(st_enq);
         const uint32_t so = wave_sum32(ovf);
         if (lane == 0) {
-            if (si) atomicAdd(&a.stats[ST_ITEMS], (unsigned long long)si);
-            if (sa) atomicAdd(&a.stats[ST_ARCS], (unsigned long long)sa);
-            if (sr) atomicAdd(&a.stats[ST_RELAX], (unsigned long long)sr);
-            if (su) atomicAdd(&a.stats[ST_UPD], (unsigned long long)su);
-            if (se) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)se);
-            if (so) atomicOr(&ctl->err, 1u);
+            if (si) atomicAdd(&s_stats[ST_ITEMS], (unsigned long long)si);
+            if (sa) atomicAdd(&s_stats[ST_ARCS], (unsigned long long)sa);
+            if (sr) atomicAdd(&s_stats[ST_RELAX], (unsigned long long)sr);
+            if (su) atomicAdd(&s_stats[ST_UPD], (unsigned long long)su);
+            if (se) atomicAdd(&s_stats[ST_ENQ], (unsigned long long)se);
+            if (so) s_ovf = 1;
         }
-    }
-
-    // epilogue: the last workgroup to finish advances the round
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t prev = atomicAdd(&ctl->blocks_done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            ctl->blocks_done = 0;
-            ctl->cnt[prv] = 0;
-            if (n_cur == 0) {
-                ctl->level_done = 1;
-            } else {
-                ctl->round = r + 1;
-            }
-            __threadfence();
-        }
+        __syncthreads();
+        if (threadIdx.x < ST_MVERT && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+        if (threadIdx.x == 0 && s_ovf) atomicOr(&ctl->err, 1u);
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// merge: T(v,X) for all X of level k from the lower levels of v, one wavefront per vertex
+// merge: T(v,X) for all X of level k from the final lower levels of v.
+// A workgroup takes a tile of VB consecutive vertices: their lower-level rows (contiguous runs of
+// the level blocks) are staged in LDS with coalesced loads, the split table of the level sits in
+// LDS, and each thread owns (vertex, mask) outputs: min over the 2^(k-1)-1 splits in registers,
+// then a coalesced store of the level-k rows.  No atomics except the LDS bitmap of finite masks.
 // ---------------------------------------------------------------------------------------
-template <bool FLT>
-__global__ void __launch_bounds__(MERGE_BLOCK) k_merge(MergeArgs a) {
-    extern __shared__ unsigned long long lds[];  // [off_k] lower-level row + [m_k] results
-    const uint32_t k = a.k;
-    const uint32_t L = a.off[k];
-    const uint32_t mk = a.m[k];
-    unsigned long long* rowv = lds;
-    unsigned long long* res = lds + L;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t g = a.g;
-    const bool prune = a.ub != GST_INF_COST;
-    const bool prune_lb = prune && a.use_lb;
-    uint32_t ovf = 0, enq = 0;
-    uint32_t st_pairs = 0, st_vert = 0;
-    unsigned long long best_root = GST_INF_WORD;
+constexpr int MERGE_THREADS = 256;
 
-    for (uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
-        // lower levels of v into LDS
+template <bool FLT>
+__global__ void __launch_bounds__(MERGE_THREADS) k_merge(MergeArgs a) {
+    extern __shared__ unsigned long long lds[];
+    const uint32_t k = a.k;
+    const uint32_t L = a.off[k];             // lower-level words per vertex
+    const uint32_t mk = a.m[k];
+    const uint32_t S = (1u << (k - 1)) - 1;  // splits per mask
+    const uint32_t VB = a.VB;
+    const uint32_t W = a.W;
+    const uint32_t g = a.g;
+    unsigned long long* rows = lds;                          // VB * L
+    unsigned long long* bm = rows + (size_t)VB * L;          // VB * W
+    const unsigned long long* pt = (const unsigned long long*)a.pairs;
+    if (a.pairs_in_lds) {
+        unsigned long long* pl = bm + (size_t)VB * W;
+        for (uint32_t i = threadIdx.x; i < a.P; i += blockDim.x) pl[i] = a.pairs[i];
+        pt = pl;
+    }
+    const bool prune = a.ub != GST_INF_COST;
+    const bool prune_lb = prune && a.use_lb && !a.is_full;
+    uint32_t ovf = 0;
+    unsigned long long best_root = GST_INF_WORD;
+    uint64_t st_pairs = 0, st_vert = 0;
+    const uint64_t* Tk_base = a.T + a.lvl_base[k];
+
+    for (uint32_t v0 = blockIdx.x * VB; v0 < a.n; v0 += gridDim.x * VB) {
+        const uint32_t nv = min(VB, a.n - v0);
+        __syncthreads();
         for (uint32_t j = 1; j < k; ++j) {
-            const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v * a.m[j];
-            for (uint32_t t = lane; t < a.m[j]; t += WAVE) rowv[a.off[j] + t] = src[t];
+            const uint32_t mj = a.m[j];
+            const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v0 * mj;
+            const uint32_t cnt = nv * mj;
+            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+                const uint32_t b = i / mj, t = i - b * mj;
+                rows[b * L + a.off[j] + t] = src[i];
+            }
         }
-        const uint32_t gmv = a.gm[v];
-        for (uint32_t r = lane; r < mk; r += WAVE) {
+        for (uint32_t i = threadIdx.x; i < nv * W; i += blockDim.x) bm[i] = 0;
+        __syncthreads();
+        const uint32_t items = nv * mk;
+        for (uint32_t i = threadIdx.x; i < items; i += blockDim.x) {
+            const uint32_t b = i / mk, r = i - b * mk;
+            const uint32_t v = v0 + b;
             const uint32_t X = a.masks[a.off[k] + r];
-            res[r] = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
-        }
-        __syncthreads();
-        for (uint32_t p = lane; p < a.P; p += WAVE) {
-            const uint64_t e = a.pairs[p];
-            const uint32_t r = (uint32_t)(e & 0xFFFF);
-            const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
-            const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
-            const uint32_t X1 = (uint32_t)(e >> 48);
-            const uint32_t c1 = (uint32_t)(rowv[p1] >> 32);
-            const uint32_t c2 = (uint32_t)(rowv[p2] >> 32);
-            const uint32_t s = add_cost<FLT>(c1, c2, ovf);
-            if (s != GST_INF_COST) {
-                const unsigned long long wv = ((unsigned long long)s << 32) | (GST_PRED_MERGE | X1);
-                atomicMin(&res[r], wv);
+            const uint32_t gmv = a.gm[v];
+            const unsigned long long* rv = rows + (size_t)b * L;
+            unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
+            const unsigned long long* pr = pt + (size_t)r * S;
+            for (uint32_t s2 = 0; s2 < S; ++s2) {
+                const unsigned long long e = pr[s2];
+                const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
+                const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
+                const uint32_t c1 = (uint32_t)(rv[p1] >> 32);
+                const uint32_t c2 = (uint32_t)(rv[p2] >> 32);
+                const uint32_t sum = add_cost<FLT>(c1, c2, ovf);
+                if (sum != GST_INF_COST) {
+                    const unsigned long long wv = ((unsigned long long)sum << 32) | (GST_PRED_MERGE | (uint32_t)(e >> 48));
+                    best = wv < best ? wv : best;
+                }
             }
-        }
-        st_pairs += (a.P + WAVE - 1 - lane) / WAVE;
-        __syncthreads();
-        // bound, store, frontier
-        uint64_t* dst = a.T + a.lvl_base[k] + (uint64_t)v * mk;
-        const bool has_arcs = a.row[v + 1] > a.row[v];
-        bool any = false;
-        for (uint32_t base = 0; base < mk; base += WAVE) {
-            const uint32_t r = base + lane;
-            unsigned long long wv = GST_INF_WORD;
-            if (r < mk) {
-                wv = res[r];
-                const uint32_t cst = (uint32_t)(wv >> 32);
-                if (cst != GST_INF_COST && prune) {
-                    uint32_t lb = 0;
-                    if (prune_lb && !a.is_full) {
-                        const uint32_t X = a.masks[a.off[k] + r];
-                        for (uint32_t gi = 0; gi < g; ++gi) {
-                            if ((X >> gi) & 1u) continue;
-                            const uint32_t di = (uint32_t)(rowv[gi] >> 32);  // level-1 row position gi = mask 1<<gi
-                            lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
-                        }
+            if (best != GST_INF_WORD && prune) {
+                uint32_t lb = 0;
+                if (prune_lb) {
+                    for (uint32_t gi = 0; gi < g; ++gi) {
+                        if ((X >> gi) & 1u) continue;
+                        const uint32_t di = (uint32_t)(rv[gi] >> 32);  // level-1 position gi = mask 1<<gi
+                        lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
                     }
-                    if (reaches_ub<FLT>(cst, lb, a.ub)) wv = GST_INF_WORD;
                 }
-                dst[r] = wv;
-                if (wv != GST_INF_WORD && a.is_full) {
-                    const unsigned long long rw = ((unsigned long long)(uint32_t)(wv >> 32) << 32) | v;
+                if (reaches_ub<FLT>((uint32_t)(best >> 32), lb, a.ub)) best = GST_INF_WORD;
+            }
+            ((uint64_t*)Tk_base)[(uint64_t)v0 * mk + i] = best;
+            if (best != GST_INF_WORD) {
+                if (a.is_full) {
+                    const unsigned long long rw = ((best >> 32) << 32) | v;
                     best_root = rw < best_root ? rw : best_root;
+                } else {
+                    atomicOr(&bm[b * W + (r >> 6)], 1ull << (r & 63));
                 }
             }
-            const uint64_t word = __ballot(wv != GST_INF_WORD);
-            if (word) any = true;
-            if (!a.is_full && has_arcs && lane == 0 && word) a.chg0[(uint64_t)v * a.W + (base >> 6)] = word;
         }
-        uint32_t nitems = 0;
-        if (!a.is_full) {
-            if (any && has_arcs && lane == 0) {
-                nitems = num_chunks(a.row, v, a.chunk);
-                a.inq0[v] = 1;
-            }
-            wave_enqueue(a.list0, &a.ctl->cnt[0], v, nitems, enq);
-        }
-        st_vert++;
+        st_pairs += (uint64_t)items * S;
+        st_vert += nv;
         __syncthreads();
+        if (!a.is_full) {
+            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
+                const uint32_t v = v0 + b;
+                if (a.row[v + 1] == a.row[v]) continue;
+                unsigned long long any = 0;
+                for (uint32_t w = 0; w < W; ++w) {
+                    const unsigned long long x = bm[b * W + w];
+                    any |= x;
+                    if (x) a.chg0[(uint64_t)v * W + w] = x;
+                }
+                if (any) a.inq0[v] = 1;
+            }
+        }
     }
     if (a.is_full) {
         best_root = wave_min64(best_root);
-        if (lane == 0 && best_root != GST_INF_WORD) atomicMin(a.root, best_root);
+        if (lane_id() == 0 && best_root != GST_INF_WORD) atomicMin(a.root, best_root);
     }
-    const uint32_t sp = wave_sum32(st_pairs);
     const uint32_t so = wave_sum32(ovf);
-    if (lane == 0) {
-        atomicAdd(&a.stats[ST_MPAIRS], (unsigned long long)sp);
+    if (threadIdx.x == 0) {
+        atomicAdd(&a.stats[ST_MPAIRS], (unsigned long long)st_pairs);
         atomicAdd(&a.stats[ST_MVERT], (unsigned long long)st_vert);
-        if (enq) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)enq);
-        if (so) atomicOr(&a.ctl->err, 1u);
     }
+    if (lane_id() == 0 && so) atomicOr(&a.ctl->err, 1u);
 }
 
 // g == 1: the full mask is level 1; its minimum is any member (cost 0)
@@ -678,6 +859,14 @@ struct gst_query_bufs {
     uint32_t* masks = nullptr;
     uint32_t* pos_of = nullptr;
     bool dirty = true;           // bitmaps / flags need clearing
+    // every (vertex, chunk) item of the current graph, for pull rounds
+    uint64_t* all_items = nullptr;
+    uint64_t cap_all = 0;
+    uint32_t n_all = 0;
+    uint32_t* all_cnt = nullptr;
+    const gst_graph* all_g = nullptr;
+    int64_t all_m = -1;
+    uint32_t all_chunk = 0;
     RoundCtl* h_ctl = nullptr;   // pinned
     std::vector<hipEvent_t> evs;
 };
@@ -688,6 +877,7 @@ static void free_bufs(gst_query_bufs* b) {
     for (int i = 0; i < 3; ++i) { hipFree(b->chg[i]); hipFree(b->inq[i]); hipFree(b->lists[i]); }
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
+    hipFree(b->all_items); hipFree(b->all_cnt);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -890,6 +1080,24 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         B->dirty = false;
     }
 
+    // ---- all (vertex, chunk) items of the graph, rebuilt when the graph or chunk changes ----
+    if (B->all_g != G || B->all_m != G->m || B->all_chunk != chunk || !B->all_items) {
+        if (!grow_alloc(B->all_items, B->cap_all, items_cap, 8)) {
+            gst_query_bufs_free(ctx);
+            return gst_set_error(ctx, GST_OOM, "solve: item list allocation failed");
+        }
+        if (!B->all_cnt) {
+            uint64_t d = 0;
+            if (!grow_alloc(B->all_cnt, d, 1, 4)) { gst_query_bufs_free(ctx); return gst_set_error(ctx, GST_OOM, "solve: allocation failed"); }
+        }
+        k_fill_u32<<<1, 64, 0, s>>>(B->all_cnt, 0u, 1);
+        k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(nullptr, G->d_row, n, chunk, B->all_items, B->all_cnt, nullptr);
+        uint32_t na = 0;
+        HIPCHK(hipMemcpyAsync(&na, B->all_cnt, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        B->n_all = na; B->all_g = G; B->all_m = G->m; B->all_chunk = chunk;
+    }
+
     // ---- group bits ----
     std::vector<int32_t> hgrp((size_t)grp_need);
     for (int i = 0; i <= g; ++i) hgrp[(size_t)i] = groups->grp_ptr[i];
@@ -928,15 +1136,25 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         for (int i = 0; i < 3; ++i) { ga.list[i] = B->lists[i]; ga.chg[i] = B->chg[i]; ga.inq[i] = B->inq[i]; }
         ga.ctl = B->ctl; ga.stats = B->stats;
         ga.m = (uint32_t)L.m[k]; ga.W = (uint32_t)((L.m[k] + 63) / 64); ga.g = (uint32_t)g;
-        ga.chunk = chunk; ga.ub = ub_bits; ga.use_lb = opts.prune_lb ? 1u : 0u;
-        Ev* e = new_ev(0);
-        hipEventRecord(e->a, s);
+        ga.chunk = chunk; ga.ub = ub_bits;
+        // the one-label lower bound reads the group distances of level 1, which are final only
+        // after level 1 has converged: never prune with it inside level 1
+        ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;
+        ga.all_items = B->all_items; ga.n_all = B->n_all;
+        {
+            const int pm = opts.pull_permille == 0 ? 300 : opts.pull_permille;
+            ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu : (uint32_t)std::max<uint64_t>(1, (uint64_t)B->n_all * (uint64_t)pm / 1000);
+        }
         for (int it = 0; it < 100000; ++it) {
+            Ev* e = new_ev(0);
+            hipEventRecord(e->a, s);
             for (int rr = 0; rr < rounds_per_sync; ++rr) {
                 if (flt) k_grow_round<true><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
                 else k_grow_round<false><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
+                k_round_end<<<1, 64, 0, s>>>(B->ctl);
                 grow_launches++;
             }
+            hipEventRecord(e->b, s);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -945,7 +1163,6 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         }
         if (!B->h_ctl->level_done) { B->dirty = true; return gst_set_error(ctx, GST_INTERNAL, "grow did not converge"); }
         total_rounds += B->h_ctl->round + 1;
-        hipEventRecord(e->b, s);
         return GST_OK;
     };
 
@@ -956,8 +1173,10 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         Ev* e = new_ev(1);
         hipEventRecord(e->a, s);
         k_init_level1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], B->gm, G->d_row, n, (uint32_t)g,
-                                                                   chunk, B->lists[0], B->chg[0], B->inq[0], B->ctl,
-                                                                   B->stats, g > 1 ? 1 : 0);
+                                                                   B->chg[0], B->inq[0], g > 1 ? 1 : 0);
+        if (g > 1)
+            k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->inq[0], G->d_row, n, chunk, B->lists[0],
+                                                                  &B->ctl->cnt[0], B->stats);
         hipEventRecord(e->b, s);
         if (g > 1) st = run_grow(1);
         else k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T, n, B->root);
@@ -974,13 +1193,24 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ub_bits; ma.use_lb = opts.prune_lb ? 1u : 0u;
         ma.is_full = full ? 1u : 0u;
-        const size_t lds = ((size_t)L.off[k] + (size_t)L.m[k]) * 8;
-        const int mgrid = std::max(1, std::min<int>((int)n, ctx->cu_count * 32));
+        const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
+        const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;
+        const bool pin = pair_bytes <= 16384;
+        const size_t budget = 40 * 1024;
+        uint32_t VB = (uint32_t)(budget / (((size_t)Lk + Wk) * 8));
+        VB = std::max<uint32_t>(1, std::min<uint32_t>(VB, 256));
+        ma.VB = VB;
+        ma.pairs_in_lds = pin ? 1u : 0u;
+        const size_t lds = (size_t)VB * (Lk + Wk) * 8 + (pin ? pair_bytes : 0);
+        const int mgrid = std::max(1, std::min<int>((int)((n + VB - 1) / VB), ctx->cu_count * 8));
         Ev* e = new_ev(1);
         hipEventRecord(e->a, s);
-        if (flt) k_merge<true><<<mgrid, MERGE_BLOCK, lds, s>>>(ma);
-        else k_merge<false><<<mgrid, MERGE_BLOCK, lds, s>>>(ma);
+        if (flt) k_merge<true><<<mgrid, MERGE_THREADS, lds, s>>>(ma);
+        else k_merge<false><<<mgrid, MERGE_THREADS, lds, s>>>(ma);
         hipEventRecord(e->b, s);
+        if (!full)
+            k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->inq[0], G->d_row, n, chunk, B->lists[0],
+                                                                  &B->ctl->cnt[0], B->stats);
         merge_launches++;
         HIPCHK(hipGetLastError());
         if (!full) st = run_grow(k);

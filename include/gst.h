@@ -70,7 +70,10 @@ typedef struct {
     int32_t want_tree;         /* 1: run the device traceback and return the edge list */
     int32_t rounds_per_sync;   /* grow rounds launched per host check (0 = tuned default) */
     int32_t chunk_arcs;        /* arcs per frontier work item (0 = tuned default) */
-    int32_t reserved[8];
+    int32_t pull_permille;     /* a grow round whose queue holds at least this many per mille of all work
+                                  items pulls over every vertex instead of pushing (0 = tuned default,
+                                  < 0 = never pull) */
+    int32_t reserved[7];
 } gst_opts;
 
 typedef struct {

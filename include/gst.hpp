@@ -1,0 +1,213 @@
+// gst.hpp -- header-only C++ host API over the C-ABI in gst.h, shaped like the reference's
+// host API so that a caller of the reference can switch:
+//
+//   reference                                                       here
+//   graph_v_of_v_idealID (rucgraph/graph_v_of_v_idealID/           gst::graph_v_of_v_idealID (same type)
+//     graph_v_of_v_idealID.h:50)
+//   toCSR(graph) (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:29)  gst::device_graph(ctx, graph)
+//   DPBF_GPU(..., cumpulsory_group_vertices, group_graph,           gst::DPBF_GPU(ctx, dgraph, cumpulsory_group_vertices,
+//            input_graph, pointer1, real_cost, ..., rt)                           group_graph, &real_cost, &rt)
+//     (rucgraph/DPQ.cuh:57)
+//   graph_hash_of_mixed_weighted (returned tree,                    gst::tree (add_vertex / add_edge /
+//     rucgraph/graph_hash_of_mixed_weighted/                           edge_weight / hash_of_vectors)
+//     graph_hash_of_mixed_weighted.h:20-126)
+//
+// Group convention as in the reference: every id in cumpulsory_group_vertices is a group vertex
+// whose group_graph row lists its members (src/DPQ.cu:27-31).  Group bit i is the i-th id in the
+// set's iteration order, as in set_max_ID (src/DPQ.cu:22-34); the optimal cost does not depend on
+// the order.  Weights: a graph whose weights are all integral is solved with exact int32
+// weights, otherwise with float32 weights (the reference truncates every weight to int,
+// csr_graph.hpp:20,46).  Errors are reported by gst::error exceptions in this header only; no
+// exception crosses the C ABI.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+#include "gst.h"
+
+namespace gst {
+
+typedef std::vector<std::vector<std::pair<int, double>>> graph_v_of_v_idealID;
+
+class error : public std::runtime_error {
+public:
+    error(gst_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+    gst_status status;
+};
+
+// The part of graph_hash_of_mixed_weighted that callers of DPBF_GPU use.
+struct tree {
+    std::unordered_map<int, std::vector<std::pair<int, double>>> hash_of_vectors;
+    void add_vertex(int v) { hash_of_vectors[v]; }
+    void add_edge(int u, int v, double w) {
+        hash_of_vectors[u].push_back({v, w});
+        hash_of_vectors[v].push_back({u, w});
+    }
+    double edge_weight(int u, int v) const {
+        auto it = hash_of_vectors.find(u);
+        if (it == hash_of_vectors.end()) return INFINITY;
+        for (auto& p : it->second)
+            if (p.first == v) return p.second;
+        return INFINITY;
+    }
+    size_t num_vertices() const { return hash_of_vectors.size(); }
+    size_t num_edges() const {
+        size_t s = 0;
+        for (auto& kv : hash_of_vectors) s += kv.second.size();
+        return s / 2;
+    }
+    double cost() const {
+        double c = 0;
+        for (auto& kv : hash_of_vectors)
+            for (auto& p : kv.second)
+                if (kv.first < p.first) c += p.second;
+        return c;
+    }
+};
+
+class context {
+public:
+    explicit context(int device = 0) {
+        gst_status s = gst_ctx_create(device, &ctx_);
+        if (s != GST_OK) {
+            std::string m = ctx_ ? gst_last_error(ctx_) : gst_status_str(s);
+            gst_ctx_destroy(ctx_);
+            ctx_ = nullptr;
+            throw error(s, m);
+        }
+    }
+    ~context() { gst_ctx_destroy(ctx_); }
+    context(const context&) = delete;
+    context& operator=(const context&) = delete;
+    gst_ctx* get() const { return ctx_; }
+    void check(gst_status s) const {
+        if (s != GST_OK) throw error(s, std::string(gst_status_str(s)) + ": " + gst_last_error(ctx_));
+    }
+
+private:
+    gst_ctx* ctx_ = nullptr;
+};
+
+// Device-resident CSR of a graph_v_of_v_idealID (the toCSR step, done once per graph).
+class device_graph {
+public:
+    device_graph(context& ctx, const graph_v_of_v_idealID& g) : ctx_(ctx) {
+        const int32_t n = (int32_t)g.size();
+        row_.assign((size_t)n + 1, 0);
+        bool integral = true;
+        for (int32_t v = 0; v < n; ++v) {
+            row_[(size_t)v + 1] = row_[(size_t)v] + (int64_t)g[(size_t)v].size();
+            for (auto& p : g[(size_t)v])
+                if (p.second != std::floor(p.second) || p.second > 2147483647.0) integral = false;
+        }
+        col_.reserve((size_t)row_.back());
+        for (int32_t v = 0; v < n; ++v)
+            for (auto& p : g[(size_t)v]) col_.push_back(p.first);
+        w_type_ = integral ? GST_W_INT32 : GST_W_FLOAT32;
+        if (integral) {
+            wi_.reserve(col_.size());
+            for (auto& r : g) for (auto& p : r) wi_.push_back((int32_t)p.second);
+        } else {
+            wf_.reserve(col_.size());
+            for (auto& r : g) for (auto& p : r) wf_.push_back((float)p.second);
+        }
+        gst_csr c = csr();
+        ctx_.check(gst_graph_upload(ctx_.get(), &c, &g_));
+    }
+    ~device_graph() { gst_graph_free(g_); }
+    device_graph(const device_graph&) = delete;
+    device_graph& operator=(const device_graph&) = delete;
+    const gst_graph* get() const { return g_; }
+    gst_csr csr() const {
+        gst_csr c;
+        c.n = (int32_t)row_.size() - 1;
+        c.m = row_.back();
+        c.row_ptr = row_.data();
+        c.col = col_.data();
+        c.w = w_type_ == GST_W_INT32 ? (const void*)wi_.data() : (const void*)wf_.data();
+        c.w_type = w_type_;
+        return c;
+    }
+    context& ctx() const { return ctx_; }
+
+private:
+    context& ctx_;
+    gst_graph* g_ = nullptr;
+    std::vector<int64_t> row_;
+    std::vector<int32_t> col_;
+    std::vector<int32_t> wi_;
+    std::vector<float> wf_;
+    int32_t w_type_ = GST_W_INT32;
+};
+
+// Solve with explicit member lists (group i = bit i).
+inline tree solve(device_graph& dg, const std::vector<std::vector<int>>& groups, double* cost,
+                  const gst_opts* opts = nullptr, gst_stats* stats = nullptr) {
+    std::vector<int32_t> ptr(groups.size() + 1, 0), vtx;
+    for (size_t i = 0; i < groups.size(); ++i) {
+        ptr[i + 1] = ptr[i] + (int32_t)groups[i].size();
+        for (int v : groups[i]) vtx.push_back(v);
+    }
+    gst_groups gg{(int32_t)groups.size(), ptr.data(), vtx.data()};
+    gst_result r;
+    dg.ctx().check(gst_solve(dg.ctx().get(), dg.get(), &gg, opts, &r));
+    tree t;
+    if (r.n_edges == 0 && r.root >= 0) t.add_vertex(r.root);
+    for (int32_t i = 0; i < r.n_edges; ++i) t.add_edge(r.eu[i], r.ev[i], r.ew[i]);
+    if (cost) *cost = r.cost;
+    if (stats) *stats = r.stats;
+    gst_result_free(&r);
+    return t;
+}
+
+// DPBF_GPU (rucgraph/DPQ.cuh:57): tree of minimum cost touching every group.
+// real_cost <- optimal cost (reference: *real_cost), rt <- seconds of device work (reference: *rt),
+// pointer1 += work items processed (reference: dequeued states).
+inline tree DPBF_GPU(device_graph& dg, const std::unordered_set<int>& cumpulsory_group_vertices,
+                     const graph_v_of_v_idealID& group_graph, double* real_cost, double* rt = nullptr,
+                     long long* pointer1 = nullptr) {
+    std::vector<std::vector<int>> groups;
+    for (int gv : cumpulsory_group_vertices) {
+        std::vector<int> mem;
+        if (gv >= 0 && (size_t)gv < group_graph.size())
+            for (auto& p : group_graph[(size_t)gv]) mem.push_back(p.first);
+        groups.push_back(std::move(mem));
+    }
+    gst_stats st;
+    tree t = solve(dg, groups, real_cost, nullptr, &st);
+    if (rt) *rt = (st.t_grow_ms + st.t_merge_ms + st.t_trace_ms) / 1e3;
+    if (pointer1) *pointer1 += (long long)st.grow_items;
+    return t;
+}
+
+// this_is_a_feasible_solution_gpu (rucgraph/test_gpu.h:45-77), strengthened: the edge set must be
+// a tree of the graph touching every group; returns its cost or throws.
+inline double check_tree(device_graph& dg, const std::vector<std::vector<int>>& groups, const tree& t) {
+    std::vector<int32_t> eu, ev;
+    int32_t root = -1;
+    for (auto& kv : t.hash_of_vectors) {
+        root = kv.first;
+        for (auto& p : kv.second)
+            if (kv.first < p.first) { eu.push_back(kv.first); ev.push_back(p.first); }
+    }
+    std::vector<int32_t> ptr(groups.size() + 1, 0), vtx;
+    for (size_t i = 0; i < groups.size(); ++i) {
+        ptr[i + 1] = ptr[i] + (int32_t)groups[i].size();
+        for (int v : groups[i]) vtx.push_back(v);
+    }
+    gst_groups gg{(int32_t)groups.size(), ptr.data(), vtx.data()};
+    gst_csr c = dg.csr();
+    double cost = 0;
+    gst_status s = gst_validate_tree(&c, &gg, (int32_t)eu.size(), eu.data(), ev.data(), root, &cost);
+    if (s != GST_OK) throw error(s, "not a feasible tree");
+    return cost;
+}
+
+}  // namespace gst
