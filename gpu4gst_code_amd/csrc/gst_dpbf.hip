@@ -48,7 +48,8 @@ struct RoundCtl {
     uint32_t cnt[3];        // items in list r%3
     uint32_t level_done;
     uint32_t err;           // bit0: int cost overflow, bit1: traceback failure
-    uint32_t pad;
+    uint32_t top;           // arrival counter over block groups
+    uint32_t sub[64];       // arrival counters of groups of 32 workgroups
 };
 
 enum StatIdx {
@@ -74,8 +75,9 @@ struct GrowArgs {
     uint32_t chunk;        // arcs per work item
     uint32_t ub;           // cost bits of the bound (GST_INF_COST: none)
     uint32_t use_lb;
-    const uint64_t* all_items;  // every (vertex, chunk) item of the graph, for pull rounds
+    const uint64_t* all_items;  // (vertex, chunk) items of the hub rows, for pull rounds
     uint32_t n_all;
+    uint32_t n;            // vertices
     uint32_t pull_thresh;  // a round with at least this many queued items pulls over all items
 };
 
@@ -227,6 +229,25 @@ __device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, 
     return (d + chunk - 1) / chunk;
 }
 
+// One-label lower bound of state (v, X): max over groups i not in X of dist(v, G_i), read from the
+// (final) level-1 row of v; inf when some missing group is unreachable within the bound.
+__device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, uint32_t v, uint32_t g, uint32_t X) {
+    uint32_t lb = 0;
+    const uint64_t* r1 = T1 + (uint64_t)v * g;
+    for (uint32_t gi = 0; gi < g; ++gi) {
+        if ((X >> gi) & 1u) continue;
+        const uint32_t di = (uint32_t)(r1[gi] >> 32);
+        if (di == GST_INF_COST) return GST_INF_COST;
+        lb = max(lb, di);
+    }
+    return lb;
+}
+
+constexpr uint32_t UNR = 4;   // arc steps issued together (independent gathers in flight per wave)
+// Rows longer than this are "hubs": pull rounds split them into chunk items combined with atomics;
+// shorter rows are pulled whole by one lane group with plain stores.
+constexpr uint32_t HUB_DEG = 128;
+
 // ---------------------------------------------------------------------------------------
 // small utility kernels (fills instead of hipMemset so that every step can be graph-captured)
 // ---------------------------------------------------------------------------------------
@@ -275,14 +296,15 @@ __global__ void k_init_level1(uint64_t* T1, const uint32_t* gm, const uint32_t* 
 // chunk of the vertex's row; one global atomic per workgroup.
 __global__ void __launch_bounds__(256) k_compact(const uint32_t* flags, const uint32_t* row, uint32_t n,
                                                  uint32_t chunk, uint64_t* list, uint32_t* cnt_out,
-                                                 unsigned long long* stats) {
+                                                 unsigned long long* stats, uint32_t min_deg) {
     __shared__ uint32_t s_wtot[4];
     __shared__ uint32_t s_base;
     const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t v = base + threadIdx.x;
         uint32_t k = 0;
-        if (v < n && (flags == nullptr || flags[v])) k = (row[v + 1] - row[v] + chunk - 1) / chunk;
+        if (v < n && (flags == nullptr || flags[v]) && row[v + 1] - row[v] > min_deg)
+            k = (row[v + 1] - row[v] + chunk - 1) / chunk;
         uint32_t incl = k;
 #pragma unroll
         for (int d = 1; d < WAVE; d <<= 1) {
@@ -302,20 +324,6 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* flags, const ui
         off += incl - k;
         for (uint32_t c = 0; c < k; ++c) list[off + c] = v | ((uint64_t)c << 32);
         __syncthreads();
-    }
-}
-
-// End of a grow round: reset the counter of the list cleaned in this round and advance, or mark
-// the level finished when the round had no items.
-__global__ void k_round_end(RoundCtl* ctl) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (ctl->level_done) return;
-    const uint32_t r = ctl->round;
-    if (ctl->cnt[r % 3] == 0) {
-        ctl->level_done = 1;
-    } else {
-        ctl->cnt[(r + 2) % 3] = 0;
-        ctl->round = r + 1;
     }
 }
 
@@ -369,7 +377,87 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
     const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
 
-    // ---- pull round: every vertex takes the best grow from its neighbours (dense frontier) ----
+    // ---- pull round, light rows: each lane group owns one row and each lane one mask of it ----
+    // (tropical SpMM row-split: the lane scans the whole row, so no cross-lane reduction, and the
+    // row has a single writer, so plain stores and a ballot-built change word replace atomics)
+    if (pull) {
+        const uint32_t mchunk = m < (uint32_t)WAVE ? m : (uint32_t)WAVE;
+        const uint32_t G = WAVE / mchunk;                 // rows per wave
+        const uint32_t gslot = lane / mchunk, t0 = lane % mchunk;
+        const bool slot_ok = gslot < G;
+        for (uint32_t base = gwave * G; base < a.n; base += nwaves * G) {
+            const uint32_t u = base + gslot;
+            const bool row_ok = slot_ok && u < a.n;
+            uint32_t rs = 0, re = 0;
+            if (row_ok) { rs = a.row[u]; re = a.row[u + 1]; }
+            const bool light = row_ok && re > rs && re - rs <= HUB_DEG;
+            const uint32_t deg = light ? re - rs : 0u;
+            st_items += wave_sum32(light && t0 == 0 ? 1u : 0u);
+            st_arcs += wave_sum32(light && t0 == 0 ? deg : 0u);
+            uint32_t maxdeg = deg;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor(maxdeg, d, WAVE));
+            for (uint32_t mb = 0; mb < m; mb += mchunk) {
+                const uint32_t rk = mb + t0;
+                const bool lane_ok = light && rk < m;
+                unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+                const unsigned long long own = lane_ok ? *addr : 0ull;
+                const uint32_t ocost = (uint32_t)(own >> 32);
+                const uint32_t lb = (prune_lb && lane_ok) ? label_lb(a.T1, u, a.g, a.masks[rk]) : 0u;
+                unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;
+                for (uint32_t s = 0; s < maxdeg; s += UNR) {
+                    uint32_t nb[UNR], wt[UNR], cn[UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        const uint32_t e = rs + s + j;
+                        const bool ok = lane_ok && s + j < deg;
+                        nb[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+                        wt[j] = ok ? a.w[e] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j)
+                        cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        if (nb[j] == 0xFFFFFFFFu) continue;
+                        st_relax++;
+                        if (cn[j] < ocost) {
+                            const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
+                            if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
+                                const unsigned long long nw = ((unsigned long long)cand << 32) | (rs + s + j);
+                                if (cand > cn[j]) bs = nw < bs ? nw : bs;
+                                else if (cand < ocost) bz = nw < bz ? nw : bz;
+                            }
+                        }
+                    }
+                }
+                // single writer: zero-effect candidates only on a strict decrease, strict ones by min
+                unsigned long long nwv = own;
+                if (lane_ok) {
+                    if (bz != GST_INF_WORD && (uint32_t)(bz >> 32) < (uint32_t)(bs >> 32)) nwv = bz;
+                    else if (bs < own) nwv = bs;
+                }
+                const bool improved = lane_ok && (uint32_t)(nwv >> 32) < ocost;
+                if (lane_ok && nwv != own) *addr = nwv;
+                const uint64_t imp = __ballot(improved);
+                uint32_t enq_v = 0, enq_n = 0;
+                if (lane_ok && t0 == 0) {
+                    const uint64_t word = (imp >> (gslot * mchunk)) & (mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1));
+                    if (word) {
+                        st_upd += (uint32_t)__popcll(word);
+                        // single writer of u's change word in a pull round; the flag marks the
+                        // first masks block that changed
+                        cnxt[(uint64_t)u * W + (mb >> 6)] = word;
+                        const bool first = (W == 1) ? true : (atomicExch(&inxt[u], 1u) == 0u);
+                        if (first) { enq_v = u; enq_n = (deg + a.chunk - 1) / a.chunk; }
+                    }
+                }
+                bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
+            }
+        }
+    }
+
+    // ---- pull round, hub rows: (row, chunk) items, lanes = (arc slot, mask), combined by atomics ----
     for (uint32_t i = gwave; pull && i < a.n_all; i += nwaves) {
         const uint64_t it = a.all_items[i];
         const uint32_t u = (uint32_t)it;
@@ -379,12 +467,6 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
         const uint32_t e1 = min(re, e0 + a.chunk);
         st_items++;
         st_arcs += e1 - e0;
-        uint32_t dall[GST_MAX_GROUPS];
-        if (prune_lb) {
-            const uint32_t d1 = lane < a.g ? (uint32_t)(a.T1[(uint64_t)u * a.g + lane] >> 32) : GST_INF_COST;
-#pragma unroll
-            for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi) dall[gi] = __shfl(d1, gi, WAVE);
-        }
         for (uint32_t mb = 0; mb < m; mb += WAVE) {
             const uint32_t na = min((uint32_t)WAVE, m - mb);
             const uint32_t A = WAVE / na;
@@ -394,28 +476,32 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
             const unsigned long long own = *addr;
             const uint32_t ocost = (uint32_t)(own >> 32);
-            uint32_t lb = 0;
-            if (prune_lb) {
-                const uint32_t X = a.masks[rk];
-#pragma unroll
-                for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi)
-                    if (gi < a.g && !((X >> gi) & 1u))
-                        lb = (dall[gi] == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, dall[gi]);
-            }
+            const uint32_t lb = prune_lb ? label_lb(a.T1, u, a.g, a.masks[rk]) : 0u;
             unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;  // best via positive / zero-effect edges
             const uint32_t len = e1 - e0;
             const uint32_t steps = (len + A - 1) / A;
-            for (uint32_t s = 0; s < steps; ++s) {
-                const uint32_t e = e0 + s * A + s_arc;
-                if (lane_ok && e < e1) {
-                    const uint32_t nb = (uint32_t)a.col[e];
-                    const uint32_t cn = (uint32_t)(a.Tk[(uint64_t)nb * m + rk] >> 32);
+            for (uint32_t s = 0; s < steps; s += UNR) {
+                uint32_t nb[UNR], wt[UNR], cn[UNR];
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j) {
+                    const uint32_t e = e0 + (s + j) * A + s_arc;
+                    const bool ok = lane_ok && e < e1;
+                    nb[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+                    wt[j] = ok ? a.w[e] : 0u;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j)
+                    cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j) {
+                    if (nb[j] == 0xFFFFFFFFu) continue;
                     st_relax++;
-                    if (cn < ocost) {  // only a cheaper neighbour can lower (or tie) our cost
-                        const uint32_t cand = add_cost<FLT>(cn, a.w[e], ovf);
+                    if (cn[j] < ocost) {  // only a cheaper neighbour can lower (or tie) our cost
+                        const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
                         if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
+                            const uint32_t e = e0 + (s + j) * A + s_arc;
                             const unsigned long long nw = ((unsigned long long)cand << 32) | e;
-                            if (cand > cn) bs = nw < bs ? nw : bs;
+                            if (cand > cn[j]) bs = nw < bs ? nw : bs;
                             else if (cand < ocost) bz = nw < bz ? nw : bz;
                         }
                     }
@@ -471,14 +557,6 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
         const uint32_t e1 = min(re, e0 + a.chunk);
         st_items++;
         st_arcs += e1 - e0;
-        // level-1 distances of v for the lower bound: every lane gets dist(v, G_i) for all i
-        uint32_t dall[GST_MAX_GROUPS];
-        if (prune_lb) {
-            const uint32_t d1 = lane < a.g ? (uint32_t)(a.T1[(uint64_t)v * a.g + lane] >> 32) : GST_INF_COST;
-#pragma unroll
-            for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi) dall[gi] = __shfl(d1, gi, WAVE);
-        }
-
         for (uint32_t wd = 0; wd < W; ++wd) {
             uint64_t bm = ccur[(uint64_t)v * W + wd];
             if (bm == 0) continue;
@@ -490,15 +568,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
                     const uint32_t bit = select_bit(bm, lane);
                     const uint32_t rk = wd * 64 + bit;
                     const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
-                    uint32_t lb = 0;
-                    if (prune_lb) {
-                        const uint32_t X = a.masks[rk];
-#pragma unroll
-                        for (uint32_t gi = 0; gi < GST_MAX_GROUPS; ++gi) {
-                            if (gi < a.g && !((X >> gi) & 1u))
-                                lb = (dall[gi] == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, dall[gi]);
-                        }
-                    }
+                    const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
                     if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
                 }
                 bm = wave_or64(keep);
@@ -514,19 +584,32 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             const uint32_t wsel = rk >> 6;
             const uint32_t len = e1 - e0;
             const uint32_t steps = (len + A - 1) / A;
-            for (uint32_t s = 0; s < steps; ++s) {
-                const uint32_t e = e0 + s * A + s_arc;
-                uint32_t enq_v = 0, enq_n = 0;
-                if (lane_ok && e < e1 && cv != GST_INF_COST) {
-                    const uint32_t u = (uint32_t)a.col[e];
-                    const uint32_t wt = a.w[e];
-                    const uint32_t cand = add_cost<FLT>(cv, wt, ovf);
-                    st_relax++;
-                    if (cand != GST_INF_COST && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
-                        unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-                        unsigned long long curw = *addr;
-                        if (cand <= (uint32_t)(curw >> 32)) {
-                            const unsigned long long nw = ((unsigned long long)cand << 32) | a.rev[e];
+            for (uint32_t s = 0; s < steps; s += UNR) {
+                uint32_t uu[UNR], wt[UNR], ra[UNR];
+                unsigned long long cw0[UNR];
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j) {
+                    const uint32_t e = e0 + (s + j) * A + s_arc;
+                    const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
+                    uu[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+                    wt[j] = ok ? a.w[e] : 0u;
+                    ra[j] = ok ? a.rev[e] : 0u;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j)
+                    cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j) {
+                    uint32_t enq_v = 0, enq_n = 0;
+                    const uint32_t u = uu[j];
+                    if (u != 0xFFFFFFFFu) {
+                        st_relax++;
+                        const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
+                        unsigned long long curw = cw0[j];
+                        if (cand != GST_INF_COST && cand <= (uint32_t)(curw >> 32) &&
+                            (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
+                            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+                            const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
                             bool improved = false;
                             if (nw < curw) {
                                 if (cand == cv) {
@@ -544,9 +627,9 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
                             }
                             if (improved) {
                                 st_upd++;
-                                unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
-                                if (!(*cw & bitv)) {
-                                    const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
+                                unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
+                                if (!(*cwp & bitv)) {
+                                    const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
                                     // first change of u in this round: queue it (one word per
                                     // vertex: the word itself was empty; several words: flag)
                                     const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
@@ -558,8 +641,8 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
                             }
                         }
                     }
+                    bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
                 }
-                bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
             }
         }
     }
@@ -584,6 +667,29 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
         __syncthreads();
         if (threadIdx.x < ST_MVERT && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
         if (threadIdx.x == 0 && s_ovf) atomicOr(&ctl->err, 1u);
+    }
+
+    // end of round: the last workgroup to arrive (two-level count: groups of 32, then groups)
+    // resets the counter of the list cleaned in phase A and advances the round, or marks the level
+    // finished when this round had no items.  The decision uses only values every workgroup read
+    // at kernel start, and the next launch sees these stores at the kernel boundary, so no
+    // device-scope fence (an L2 write-back on this chip) is needed here.
+    if (threadIdx.x == 0) {
+        const uint32_t grp = blockIdx.x >> 5;
+        const uint32_t gsize = min(32u, gridDim.x - (grp << 5));
+        if (atomicAdd(&ctl->sub[grp], 1u) == gsize - 1) {
+            ctl->sub[grp] = 0;
+            const uint32_t ngrp = (gridDim.x + 31) >> 5;
+            if (atomicAdd(&ctl->top, 1u) == ngrp - 1) {
+                ctl->top = 0;
+                if (n_cur == 0) {
+                    ctl->level_done = 1;
+                } else {
+                    ctl->cnt[prv] = 0;
+                    ctl->round = r + 1;
+                }
+            }
+        }
     }
 }
 
@@ -859,10 +965,12 @@ struct gst_query_bufs {
     uint32_t* masks = nullptr;
     uint32_t* pos_of = nullptr;
     bool dirty = true;           // bitmaps / flags need clearing
+    int level_rounds[GST_MAX_GROUPS + 2] = {0};   // grow rounds per level in the last solve
     // every (vertex, chunk) item of the current graph, for pull rounds
     uint64_t* all_items = nullptr;
     uint64_t cap_all = 0;
-    uint32_t n_all = 0;
+    uint32_t n_all = 0;          // hub items (rows longer than HUB_DEG)
+    uint64_t n_items_total = 0;  // work items of a full frontier
     uint32_t* all_cnt = nullptr;
     const gst_graph* all_g = nullptr;
     int64_t all_m = -1;
@@ -968,10 +1076,10 @@ bool grow_alloc(T*& p, uint64_t& cap, uint64_t need, size_t elem) {
 }
 
 uint32_t default_chunk(int m) {
-    // arcs per work item: enough steps to amortise the item setup, small enough that hub
-    // vertices split over many wavefronts
+    // arcs per work item: small enough that hub rows split over many wavefronts, large enough
+    // to amortise the item setup (swept on BA V=200k: 32 beats 64 and 128)
     (void)m;
-    return 64;
+    return 32;
 }
 
 }  // namespace
@@ -1091,11 +1199,15 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             if (!grow_alloc(B->all_cnt, d, 1, 4)) { gst_query_bufs_free(ctx); return gst_set_error(ctx, GST_OOM, "solve: allocation failed"); }
         }
         k_fill_u32<<<1, 64, 0, s>>>(B->all_cnt, 0u, 1);
-        k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(nullptr, G->d_row, n, chunk, B->all_items, B->all_cnt, nullptr);
+        k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(nullptr, G->d_row, n, chunk, B->all_items, B->all_cnt, nullptr,
+                                                              HUB_DEG);
         uint32_t na = 0;
         HIPCHK(hipMemcpyAsync(&na, B->all_cnt, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         B->n_all = na; B->all_g = G; B->all_m = G->m; B->all_chunk = chunk;
+        uint64_t tot = 0;
+        for (uint32_t v = 0; v < n; ++v) tot += (uint64_t)((G->h_row[v + 1] - G->h_row[v] + chunk - 1) / chunk);
+        B->n_items_total = tot;
     }
 
     // ---- group bits ----
@@ -1123,9 +1235,11 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     };
     auto cleanup_evs = [&]() { for (auto& e : evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); } };
 
-    const int grow_grid = std::max(1, ctx->cu_count * 4);   // 4 workgroups x 4 waves per CU
+    // 5 workgroups x 4 waves per CU (occupancy 5); the arrival counters hold up to 64 x 32 groups
+    const int grow_grid = std::max(1, std::min(ctx->cu_count * 5, 64 * 32));
     const int rounds_per_sync = opts.rounds_per_sync > 0 ? opts.rounds_per_sync : 8;
     uint32_t total_rounds = 0, grow_launches = 0, merge_launches = 0;
+    int* ctx_level_rounds = B->level_rounds;
 
     auto run_grow = [&](int k) -> gst_status {
         GrowArgs ga;
@@ -1140,18 +1254,20 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         // the one-label lower bound reads the group distances of level 1, which are final only
         // after level 1 has converged: never prune with it inside level 1
         ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;
-        ga.all_items = B->all_items; ga.n_all = B->n_all;
+        ga.all_items = B->all_items; ga.n_all = B->n_all; ga.n = n;
         {
-            const int pm = opts.pull_permille == 0 ? 300 : opts.pull_permille;
-            ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu : (uint32_t)std::max<uint64_t>(1, (uint64_t)B->n_all * (uint64_t)pm / 1000);
+            const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
+            ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu
+                                    : (uint32_t)std::max<uint64_t>(1, B->n_items_total * (uint64_t)pm / 1000);
         }
+        // first batch sized from the rounds this level took last time, then short batches
+        int batch = opts.rounds_per_sync > 0 ? rounds_per_sync : std::max(8, ctx_level_rounds[k] + 2);
         for (int it = 0; it < 100000; ++it) {
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
-            for (int rr = 0; rr < rounds_per_sync; ++rr) {
+            for (int rr = 0; rr < batch; ++rr) {
                 if (flt) k_grow_round<true><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
                 else k_grow_round<false><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
-                k_round_end<<<1, 64, 0, s>>>(B->ctl);
                 grow_launches++;
             }
             hipEventRecord(e->b, s);
@@ -1160,7 +1276,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             HIPCHK(hipStreamSynchronize(s));
             if (B->h_ctl->err & 1u) return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits");
             if (B->h_ctl->level_done) break;
+            batch = opts.rounds_per_sync > 0 ? rounds_per_sync : 4;
         }
+        ctx_level_rounds[k] = (int)B->h_ctl->round + 1;
         if (!B->h_ctl->level_done) { B->dirty = true; return gst_set_error(ctx, GST_INTERNAL, "grow did not converge"); }
         total_rounds += B->h_ctl->round + 1;
         return GST_OK;
@@ -1176,7 +1294,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
                                                                    B->chg[0], B->inq[0], g > 1 ? 1 : 0);
         if (g > 1)
             k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->inq[0], G->d_row, n, chunk, B->lists[0],
-                                                                  &B->ctl->cnt[0], B->stats);
+                                                                  &B->ctl->cnt[0], B->stats, 0u);
         hipEventRecord(e->b, s);
         if (g > 1) st = run_grow(1);
         else k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T, n, B->root);
@@ -1210,7 +1328,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         hipEventRecord(e->b, s);
         if (!full)
             k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->inq[0], G->d_row, n, chunk, B->lists[0],
-                                                                  &B->ctl->cnt[0], B->stats);
+                                                                  &B->ctl->cnt[0], B->stats, 0u);
         merge_launches++;
         HIPCHK(hipGetLastError());
         if (!full) st = run_grow(k);
