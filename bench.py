@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--groups", type=int, default=6)
     ap.add_argument("--cpu-sample", type=int, default=16, help="queries in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied by one plain device copy before the run (counter calibration)")
     return ap.parse_args()
 
 
@@ -51,16 +53,20 @@ def main():
     solver = gst.Solver(0)
     dg = solver.upload(G)
 
+    if args.calibrate_copy > 0:
+        import ctypes
+        calib = ctypes.CDLL(os.path.join(ROOT, "gpu4gst_code_amd", "libgst_calib.so"))
+        calib.gst_calib_copy.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
+        assert calib.gst_calib_copy(0, args.calibrate_copy << 20, 1) == 0
+
     for i in range(args.warmup):
         solver.solve(dg, queries[i % 64])
 
-    import torch
-    torch.cuda.synchronize()
+    # every solve ends with a stream synchronisation, so the host clock brackets device work
     t0 = time.perf_counter()
     results = []
     for i in range(args.steps):
         results.append(solver.solve(dg, queries[i % 64]))
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
     qps = args.steps / wall
@@ -92,6 +98,11 @@ def main():
                "sample": f"{len(sample)} of the 64 queries, label-setting DPBF (oracle/dpbf_oracle.c), "
                          f"{nth} threads, {cw:.1f} s wall"}
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "dpbf_plain_ba200k_g6_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic_bytes_per_dispatch"]
+
     line = {
         "metric": "exact-cost GST queries/s (plain DPBF, BA V=200k g=6)",
         "value": round(qps, 3),
@@ -109,7 +120,7 @@ def main():
                                "64 queries one at a time", "mode": "plain DPBF"},
         "state_relax_per_s": round(relax / (grow_ms * 1e-3), 1) if grow_ms > 0 else None,
         "roofline": {"bound": "hbm", "achieved": round(grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(grow_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(grow_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_grow_round", "launches": grow_launches,
                      "bytes_per_launch": round(grow_bytes / max(grow_launches, 1), 1),
                      "ms_per_launch": round(grow_ms / max(grow_launches, 1), 5)},

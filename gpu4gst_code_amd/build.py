@@ -51,6 +51,10 @@ def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     shutil.move(tmp, LIB)
+    # counter-calibration helper: a plain copy kernel of known size (bench.py --calibrate-copy)
+    calib_obj = os.path.join(BUILD, "gst_calib.hip.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gst_calib.hip"), "-o", calib_obj])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", os.path.join(HERE, "libgst_calib.so"), calib_obj])
     # the reference-shaped C++ caller (include/gst.hpp) used by the wrapper test
     _run([CXX, "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(HERE, "examples", "dpbf_gpu_example.cpp"),
           "-L" + HERE, "-lgst", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(BUILD, "dpbf_gpu_example")])

@@ -1,0 +1,25 @@
+// Measurement helper (not part of the solver ABI): a plain device copy of known size, used to
+// calibrate the FETCH_SIZE / WRITE_SIZE counters against the algorithmic byte counts.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+__global__ void __launch_bounds__(256) gst_calib_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                             uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Copies `bytes` (rounded down to 16) device-to-device `iters` times; returns 0 on success.
+extern "C" int gst_calib_copy(int device, uint64_t bytes, int iters) {
+    if (hipSetDevice(device) != hipSuccess) return 1;
+    const uint64_t n = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return 2;
+    if (hipMalloc(&b, n * 16) != hipSuccess) { (void)hipFree(a); return 2; }
+    (void)hipMemset(a, 1, n * 16);
+    for (int i = 0; i < iters; ++i) gst_calib_copy_kernel<<<2048, 256>>>(a, b, n);
+    const hipError_t e = hipDeviceSynchronize();
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return e == hipSuccess ? 0 : 3;
+}

@@ -40,9 +40,8 @@ def test_status_strings():
 
 def test_no_device_fails_loudly():
     """Without a GPU the solver refuses to run (there is no CPU fallback)."""
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present")
     with pytest.raises(gst.GstError) as e:
         gst.Solver(0)
     assert e.value.status == _abi.GST_NO_DEVICE
