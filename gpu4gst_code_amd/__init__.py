@@ -260,7 +260,7 @@ class Solver:
 
     def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
               want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0,
-              pull_permille: int = 0) -> Result:
+              pull_permille: int = 0, grow_mode: int = 0) -> Result:
         opts = _abi.gst_opts()
         opts.ub = float(ub) if ub is not None else 0.0
         opts.prune_lb = 1 if prune_lb else 0
@@ -268,6 +268,7 @@ class Solver:
         opts.rounds_per_sync = rounds_per_sync
         opts.chunk_arcs = chunk_arcs
         opts.pull_permille = pull_permille
+        opts.grow_mode = grow_mode
         res = _abi.gst_result()
         grp = groups.as_c()
         st = self.lib.gst_solve(self.ctx, graph.handle, C.byref(grp), C.byref(opts), C.byref(res))
@@ -282,7 +283,7 @@ class Solver:
                 eu = np.zeros(0, np.int32)
                 ev = np.zeros(0, np.int32)
                 ew = np.zeros(0, np.float64)
-            stats = {name: getattr(res.stats, name) for name, _ in _abi.gst_stats._fields_ if name != "pad0"}
+            stats = {name: getattr(res.stats, name) for name, _ in _abi.gst_stats._fields_}
             return Result(res.cost, res.tree_cost, res.root, eu, ev, ew, bool(res.used_ub), bool(res.fell_back),
                           stats)
         finally:

@@ -37,7 +37,7 @@ class gst_groups(C.Structure):
 class gst_opts(C.Structure):
     _fields_ = [("ub", C.c_double), ("prune_lb", C.c_int32), ("want_tree", C.c_int32),
                 ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("grow_mode", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class gst_stats(C.Structure):
@@ -45,7 +45,7 @@ class gst_stats(C.Structure):
                 ("grow_updates", C.c_uint64), ("grow_enqueued", C.c_uint64), ("merge_vertices", C.c_uint64),
                 ("merge_pairs", C.c_uint64), ("grow_bytes", C.c_uint64), ("merge_bytes", C.c_uint64),
                 ("grow_rounds", C.c_uint32), ("grow_launches", C.c_uint32), ("merge_launches", C.c_uint32),
-                ("pad0", C.c_uint32), ("t_total_ms", C.c_double), ("t_grow_ms", C.c_double),
+                ("async_launches", C.c_uint32), ("t_total_ms", C.c_double), ("t_grow_ms", C.c_double),
                 ("t_merge_ms", C.c_double), ("t_trace_ms", C.c_double)]
 
 

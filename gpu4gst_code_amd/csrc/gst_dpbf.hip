@@ -79,6 +79,19 @@ struct GrowArgs {
     uint32_t n_all;
     uint32_t n;            // vertices
     uint32_t pull_thresh;  // a round with at least this many queued items pulls over all items
+    // asynchronous (work-queue) mode
+    unsigned long long* ring;   // 4 words per task slot
+    uint32_t ring_mask;         // slots - 1 (power of two)
+    struct AsyncCtl* actl;
+};
+
+// device-side control of an asynchronous grow launch
+struct AsyncCtl {
+    uint32_t head;      // next task index to claim
+    uint32_t tail;      // next task index to reserve
+    uint32_t pending;   // tasks reserved (or initial) and not finished
+    uint32_t err;       // bit0: int cost overflow, bit2: wait bound exceeded, bit3: ring overflow
+    uint32_t pad[4];
 };
 
 struct MergeArgs {
@@ -343,6 +356,15 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     const uint32_t n_prv = ctl->cnt[prv];
     const uint32_t lane = lane_id();
     const uint32_t W = a.W, m = a.m;
+    const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
+    // only as many workgroups as the round needs take part (a sparse round costs a few blocks)
+    uint32_t part = gridDim.x;
+    if (!pull) {
+        const uint32_t need_b = (n_cur + GROW_WAVES - 1) / GROW_WAVES;
+        const uint32_t need_a = (n_prv + GROW_BLOCK - 1) / GROW_BLOCK;
+        part = max(1u, min(gridDim.x, max(need_b, need_a)));
+    }
+    if (blockIdx.x >= part) return;
     bq_init(q);
     if (threadIdx.x < ST_NSTAT) s_stats[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_ovf = 0;
@@ -352,7 +374,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
         const uint64_t* lp = a.list[prv];
         uint64_t* cp = a.chg[prv];
         uint32_t* ip = a.inq[prv];
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_prv; i += gridDim.x * blockDim.x) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_prv; i += part * blockDim.x) {
             const uint64_t it = lp[i];
             if ((it >> 32) == 0) {
                 const uint32_t v = (uint32_t)it;
@@ -367,7 +389,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0;
     uint32_t ovf = 0;
     const uint32_t gwave = blockIdx.x * GROW_WAVES + (threadIdx.x / WAVE);
-    const uint32_t nwaves = gridDim.x * GROW_WAVES;
+    const uint32_t nwaves = part * GROW_WAVES;
     const uint64_t* lcur = a.list[cur];
     const uint64_t* ccur = a.chg[cur];
     uint64_t* cnxt = a.chg[nxt];
@@ -375,7 +397,6 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     uint64_t* lnxt = a.list[nxt];
     uint32_t* cntn = &ctl->cnt[nxt];
     const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
-    const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
 
     // ---- pull round, light rows: each lane group owns one row and each lane one mask of it ----
     // (tropical SpMM row-split: the lane scans the whole row, so no cross-lane reduction, and the
@@ -676,10 +697,10 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     // device-scope fence (an L2 write-back on this chip) is needed here.
     if (threadIdx.x == 0) {
         const uint32_t grp = blockIdx.x >> 5;
-        const uint32_t gsize = min(32u, gridDim.x - (grp << 5));
+        const uint32_t gsize = min(32u, part - (grp << 5));
         if (atomicAdd(&ctl->sub[grp], 1u) == gsize - 1) {
             ctl->sub[grp] = 0;
-            const uint32_t ngrp = (gridDim.x + 31) >> 5;
+            const uint32_t ngrp = (part + 31) >> 5;
             if (atomicAdd(&ctl->top, 1u) == ngrp - 1) {
                 ctl->top = 0;
                 if (n_cur == 0) {
@@ -691,6 +712,320 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// grow, asynchronous: one persistent launch finishes a level from its current frontier with a
+// device work queue instead of rounds.  Tasks:
+//   VERTEX v          take v's change bitmap (atomic exchange) and relax its arcs for those masks,
+//                     or, for a row longer than one chunk, split it into CHUNK tasks
+//   CHUNK v,c,wd,bm   relax arcs [c*chunk, ...) of v for the masks in word wd of bm
+// Task index i < n0 is item i of the current round list (chunk-0 items are VERTEX tasks, the rest
+// are no-ops: the vertex task covers the row); later tasks live in a ring of 4-word slots, every
+// word tagged with the task index, so a reader knows a slot is complete without any fence.
+// pending counts tasks reserved and not finished: it is raised before a task becomes visible
+// and lowered after the task's own enqueues, so pending == 0 means the level is finished.
+// Every wait is bounded: past the bound the launch reports an error instead of spinning on.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t TASK_VERTEX = 0, TASK_CHUNK = 1;
+constexpr uint32_t ASYNC_BATCH = 4;           // tasks claimed per wave and claim
+constexpr uint32_t ASYNC_MAX_WAIT = 1u << 21; // polls of one slot before giving up (~seconds)
+
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// write one task into the ring (caller reserved index idx >= n0)
+__device__ __forceinline__ void task_write(const GrowArgs& a, uint32_t n0, uint32_t idx, uint32_t kind, uint32_t v,
+                                           uint32_t c, uint32_t wd, uint64_t bm) {
+    unsigned long long* sl = a.ring + 4ull * ((idx - n0) & a.ring_mask);
+    const unsigned long long tag = (unsigned long long)idx << 32;
+    st_agent(sl + 0, tag | v);
+    st_agent(sl + 1, tag | ((unsigned long long)c << 9) | ((unsigned long long)wd << 1) | kind);
+    if (kind == TASK_CHUNK) {
+        st_agent(sl + 2, tag | (bm & 0xFFFFFFFFull));
+        st_agent(sl + 3, tag | (bm >> 32));
+    }
+}
+
+// relax arcs [e0,e1) of v for the masks set in word wd of bm; queue improved neighbours.
+// Convergent: all lanes of the wave call it with the same arguments.
+template <bool FLT>
+__device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t e0, uint32_t e1, uint32_t wd,
+                           uint64_t bm, uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq, uint32_t& ovf) {
+    const uint32_t lane = lane_id();
+    const uint32_t m = a.m, W = a.W;
+    uint64_t* chg = a.chg[a.ctl->round % 3];
+    uint32_t* inq = a.inq[a.ctl->round % 3];
+    const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
+    // drop masks at or above the bound
+    if (a.ub != GST_INF_COST) {
+        const uint32_t na0 = (uint32_t)__popcll(bm);
+        uint64_t keep = 0;
+        if (lane < na0) {
+            const uint32_t bit = select_bit(bm, lane);
+            const uint32_t rk = wd * 64 + bit;
+            const uint32_t cv = (uint32_t)(ld_agent((const unsigned long long*)(a.Tk + (uint64_t)v * m + rk)) >> 32);
+            const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
+            if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
+        }
+        bm = wave_or64(keep);
+    }
+    const uint32_t na = (uint32_t)__popcll(bm);
+    if (na == 0) return;
+    const uint32_t A = WAVE / na;
+    const uint32_t t = lane % na, s_arc = lane / na;
+    const bool lane_ok = s_arc < A;
+    const uint32_t rk = wd * 64 + select_bit(bm, t);
+    // own cost, read at the coherence point (it may have just been lowered on another XCD)
+    const uint32_t cv = (uint32_t)(ld_agent((const unsigned long long*)(a.Tk + (uint64_t)v * m + rk)) >> 32);
+    const uint64_t bitv = 1ull << (rk & 63);
+    const uint32_t wsel = rk >> 6;
+    const uint32_t steps = (e1 - e0 + A - 1) / A;
+    for (uint32_t s = 0; s < steps; s += UNR) {
+        uint32_t uu[UNR], wt[UNR], ra[UNR];
+        unsigned long long cw0[UNR];
+#pragma unroll
+        for (uint32_t j = 0; j < UNR; ++j) {
+            const uint32_t e = e0 + (s + j) * A + s_arc;
+            const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
+            uu[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+            wt[j] = ok ? a.w[e] : 0u;
+            ra[j] = ok ? a.rev[e] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < UNR; ++j)
+            cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
+        uint32_t enq_mask = 0;
+        uint32_t enq_u[UNR];
+#pragma unroll
+        for (uint32_t j = 0; j < UNR; ++j) {
+            enq_u[j] = 0;
+            const uint32_t u = uu[j];
+            if (u == 0xFFFFFFFFu) continue;
+            st_relax++;
+            const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
+            unsigned long long curw = cw0[j];
+            if (cand == GST_INF_COST || cand > (uint32_t)(curw >> 32)) continue;
+            if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand, 0u, a.ub)) continue;
+            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+            const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
+            bool improved = false;
+            if (nw < curw) {
+                if (cand == cv) {
+                    while (cand < (uint32_t)(curw >> 32)) {
+                        const unsigned long long old = atomicCAS(addr, curw, nw);
+                        if (old == curw) { improved = true; break; }
+                        curw = old;
+                    }
+                } else {
+                    const unsigned long long old = atomicMin(addr, nw);
+                    improved = cand < (uint32_t)(old >> 32);
+                }
+            }
+            if (!improved) continue;
+            st_upd++;
+            unsigned long long* cwp = (unsigned long long*)&chg[(uint64_t)u * W + wsel];
+            const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
+            const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inq[u], 1u) == 0u);
+            if (first) { enq_mask |= 1u << j; enq_u[j] = u; }
+        }
+        // queue the vertices whose first change this was: one reservation per wave and step
+#pragma unroll
+        for (uint32_t j = 0; j < UNR; ++j) {
+            const bool want = (enq_mask >> j) & 1u;
+            const uint64_t bal = __ballot(want);
+            if (bal == 0) continue;
+            const uint32_t cntw = (uint32_t)__popcll(bal);
+            uint32_t base = 0;
+            if (lane == 0) {
+                const uint32_t p = atomicAdd(&a.actl->pending, cntw);
+                if (p + cntw > a.ring_mask + 1) atomicOr(&a.actl->err, 8u);
+                base = atomicAdd(&a.actl->tail, cntw);
+            }
+            base = __shfl(base, 0, WAVE);
+            if (want) {
+                const uint32_t pos = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                task_write(a, n0, pos, TASK_VERTEX, enq_u[j], 0, 0, 0);
+            }
+            st_enq += want ? 1u : 0u;
+        }
+    }
+}
+
+template <bool FLT>
+__global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
+    RoundCtl* ctl = a.ctl;
+    AsyncCtl* ac = a.actl;
+    if (ctl->level_done) return;
+    const uint32_t r = ctl->round;
+    const uint32_t cur = r % 3, prv = (r + 2) % 3;
+    const uint32_t n0 = ctl->cnt[cur];
+    const uint32_t lane = lane_id();
+    const uint32_t W = a.W;
+    // clear the previous round's bitmaps and flags (the round scheme's phase A)
+    {
+        const uint32_t n_prv = ctl->cnt[prv];
+        const uint64_t* lp = a.list[prv];
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_prv; i += gridDim.x * blockDim.x) {
+            const uint64_t it = lp[i];
+            if ((it >> 32) == 0) {
+                const uint32_t v = (uint32_t)it;
+                for (uint32_t w = 0; w < W; ++w) a.chg[prv][(uint64_t)v * W + w] = 0;
+                a.inq[prv][v] = 0;
+            }
+        }
+    }
+    uint64_t* chg = a.chg[cur];
+    uint32_t* inq = a.inq[cur];
+    const uint64_t* lcur = a.list[cur];
+    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, ovf = 0;
+    uint32_t done_local = 0;
+    bool stop = false;
+    while (!stop) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ac->head, ASYNC_BATCH);
+        base = __shfl(base, 0, WAVE);
+        for (uint32_t j = 0; j < ASYNC_BATCH && !stop; ++j) {
+            const uint32_t idx = base + j;
+            uint32_t kind, v, c = 0, wd = 0;
+            uint64_t bm = 0;
+            if (idx < n0) {
+                const uint64_t it = lcur[idx];
+                v = (uint32_t)it;
+                c = (uint32_t)(it >> 32);
+                kind = c == 0 ? TASK_VERTEX : 0xFFu;  // later chunks of an initial row: no-op
+            } else {
+                // wait for the slot of task idx (lane 0 polls, then broadcasts)
+                unsigned long long w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+                uint32_t outcome = 0;  // 1 = task ready, 2 = level finished, 3 = gave up
+                if (lane == 0) {
+                    if (done_local) { atomicSub(&ac->pending, done_local); done_local = 0; }
+                    const unsigned long long* sl = a.ring + 4ull * ((idx - n0) & a.ring_mask);
+                    for (uint32_t it = 0; it < ASYNC_MAX_WAIT; ++it) {
+                        w0 = ld_agent(sl + 0);
+                        w1 = ld_agent(sl + 1);
+                        if ((uint32_t)(w0 >> 32) == idx && (uint32_t)(w1 >> 32) == idx) {
+                            if ((w1 & 1ull) == TASK_CHUNK) {
+                                w2 = ld_agent(sl + 2);
+                                w3 = ld_agent(sl + 3);
+                                if ((uint32_t)(w2 >> 32) != idx || (uint32_t)(w3 >> 32) != idx) continue;
+                            }
+                            outcome = 1;
+                            // invalidate the slot: tags restart at 0 in every launch
+                            const unsigned long long dead = 0xFFFFFFFF00000000ull;
+                            unsigned long long* sw = a.ring + 4ull * ((idx - n0) & a.ring_mask);
+                            st_agent(sw + 0, dead);
+                            st_agent(sw + 1, dead);
+                            if ((w1 & 1ull) == TASK_CHUNK) { st_agent(sw + 2, dead); st_agent(sw + 3, dead); }
+                            break;
+                        }
+                        if (ld_agent32(&ac->pending) == 0) { outcome = 2; break; }
+                        // exponential back-off: idle waves must not flood the coherence point
+                        if (it < 2) continue;
+                        else if (it < 8) __builtin_amdgcn_s_sleep(8);
+                        else if (it < 32) __builtin_amdgcn_s_sleep(32);
+                        else __builtin_amdgcn_s_sleep(127);
+                    }
+                    if (outcome == 0) { outcome = 3; atomicOr(&ac->err, 4u); }
+                }
+                outcome = __shfl(outcome, 0, WAVE);
+                if (outcome != 1) { stop = true; break; }
+                w0 = __shfl(w0, 0, WAVE); w1 = __shfl(w1, 0, WAVE);
+                w2 = __shfl(w2, 0, WAVE); w3 = __shfl(w3, 0, WAVE);
+                v = (uint32_t)w0;
+                kind = (uint32_t)(w1 & 1ull);
+                wd = (uint32_t)((w1 >> 1) & 0xFFull);
+                c = (uint32_t)((w1 & 0xFFFFFFFFull) >> 9);
+                bm = (w2 & 0xFFFFFFFFull) | ((w3 & 0xFFFFFFFFull) << 32);
+            }
+            if (kind == TASK_VERTEX) {
+                // take the change bitmap of v
+                uint64_t bms[16];
+                uint32_t nz = 0;
+                if (lane == 0) atomicExch(&inq[v], 0u);
+                for (uint32_t w = 0; w < W && w < 16; ++w) {
+                    uint64_t x = 0;
+                    if (lane == 0) x = atomicExch((unsigned long long*)&chg[(uint64_t)v * W + w], 0ull);
+                    bms[w] = __shfl(x, 0, WAVE);
+                    nz += bms[w] != 0;
+                }
+                const uint32_t rs = a.row[v], re = a.row[v + 1];
+                const uint32_t nc = (re - rs + a.chunk - 1) / a.chunk;
+                if (nz && nc == 1) {
+                    st_items++;
+                    st_arcs += re - rs;
+                    for (uint32_t w = 0; w < W && w < 16; ++w)
+                        if (bms[w]) async_relax<FLT>(a, n0, v, rs, re, w, bms[w], st_relax, st_upd, st_enq, ovf);
+                } else if (nz && nc > 1) {
+                    // split the row: one CHUNK task per (chunk, non-empty word)
+                    const uint32_t ntask = nc * nz;
+                    uint32_t tb = 0;
+                    if (lane == 0) {
+                        const uint32_t p = atomicAdd(&ac->pending, ntask);
+                        if (p + ntask > a.ring_mask + 1) atomicOr(&ac->err, 8u);
+                        tb = atomicAdd(&ac->tail, ntask);
+                    }
+                    tb = __shfl(tb, 0, WAVE);
+                    uint32_t k2 = 0;
+                    for (uint32_t w = 0; w < W && w < 16; ++w) {
+                        if (!bms[w]) continue;
+                        for (uint32_t cc = lane; cc < nc; cc += WAVE)
+                            task_write(a, n0, tb + k2 * nc + cc, TASK_CHUNK, v, cc, w, bms[w]);
+                        k2++;
+                    }
+                    st_enq += ntask;
+                }
+            } else if (kind == TASK_CHUNK) {
+                const uint32_t rs = a.row[v], re = a.row[v + 1];
+                const uint32_t e0 = rs + c * a.chunk;
+                const uint32_t e1 = min(re, e0 + a.chunk);
+                st_items++;
+                st_arcs += e1 - e0;
+                async_relax<FLT>(a, n0, v, e0, e1, wd, bm, st_relax, st_upd, st_enq, ovf);
+            }
+            done_local++;
+        }
+        if (lane == 0 && done_local) { atomicSub(&ac->pending, done_local); done_local = 0; }
+    }
+    if (lane == 0 && done_local) atomicSub(&ac->pending, done_local);
+    const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
+    const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
+    const uint32_t sr = wave_sum32(st_relax);
+    const uint32_t su = wave_sum32(st_upd);
+    const uint32_t se = wave_sum32(st_enq);
+    const uint32_t so = wave_sum32(ovf);
+    if (lane == 0) {
+        if (si) atomicAdd(&a.stats[ST_ITEMS], (unsigned long long)si);
+        if (sa) atomicAdd(&a.stats[ST_ARCS], (unsigned long long)sa);
+        if (sr) atomicAdd(&a.stats[ST_RELAX], (unsigned long long)sr);
+        if (su) atomicAdd(&a.stats[ST_UPD], (unsigned long long)su);
+        if (se) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)se);
+        if (so) atomicOr(&ac->err, 1u);
+    }
+}
+
+// after an asynchronous launch: the level is finished (counters of the round scheme reset)
+__global__ void k_async_end(RoundCtl* ctl, AsyncCtl* ac) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ac->err & 1u) ctl->err |= 1u;
+    if (ac->err & 12u) ctl->err |= 4u;   // lost work: the host re-runs the query on the round path
+    ctl->level_done = 1;
+}
+
+// prepare an asynchronous launch from the current round's list
+__global__ void k_async_begin(const RoundCtl* ctl, AsyncCtl* ac) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t n0 = ctl->cnt[ctl->round % 3];
+    ac->head = 0;
+    ac->tail = n0;
+    ac->pending = n0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -965,6 +1300,10 @@ struct gst_query_bufs {
     uint32_t* masks = nullptr;
     uint32_t* pos_of = nullptr;
     bool dirty = true;           // bitmaps / flags need clearing
+    unsigned long long* ring = nullptr;   // async task slots (4 words each)
+    uint64_t cap_ring = 0;                // slots (power of two)
+    AsyncCtl* actl = nullptr;
+    bool ring_dirty = true;               // slots need their tags invalidated
     int level_rounds[GST_MAX_GROUPS + 2] = {0};   // grow rounds per level in the last solve
     // every (vertex, chunk) item of the current graph, for pull rounds
     uint64_t* all_items = nullptr;
@@ -986,6 +1325,7 @@ static void free_bufs(gst_query_bufs* b) {
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
     hipFree(b->all_items); hipFree(b->all_cnt);
+    hipFree(b->ring); hipFree(b->actl);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -1148,6 +1488,21 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         }
         B->cap_items = items_cap;
     }
+    {
+        uint64_t need = 1;
+        while (need < 2 * items_cap) need <<= 1;
+        if (!B->ring || B->cap_ring < need) {
+            hipFree(B->ring);
+            B->ring = nullptr;
+            B->cap_ring = 0;
+            if (hipMalloc((void**)&B->ring, need * 32) != hipSuccess) ok = false;
+            else { B->cap_ring = need; B->ring_dirty = true; }
+        }
+        if (!B->actl) {
+            uint64_t d = 0;
+            ok = ok && grow_alloc(B->actl, d, 1, sizeof(AsyncCtl));
+        }
+    }
     if (!B->ctl) {
         uint64_t d = 0;
         ok = ok && grow_alloc(B->ctl, d, 1, sizeof(RoundCtl));
@@ -1180,6 +1535,10 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         B->tab_g = g;
     }
     const int fill_grid = std::max(1, ctx->cu_count * 8);
+    if (B->ring_dirty) {
+        k_fill_u64<<<fill_grid, 256, 0, s>>>((uint64_t*)B->ring, ~0ull, B->cap_ring * 4);
+        B->ring_dirty = false;
+    }
     if (B->dirty) {
         for (int i = 0; i < 3; ++i) {
             k_fill_u64<<<fill_grid, 256, 0, s>>>(B->chg[i], 0ull, B->cap_V * B->cap_W);
@@ -1220,6 +1579,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->stats, 0ull, ST_NSTAT);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->root, GST_INF_WORD, 1);
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)B->ctl, 0u, sizeof(RoundCtl) / 4);
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)B->actl, 0u, sizeof(AsyncCtl) / 4);
 
     std::vector<uint64_t> lvl_base(GST_MAX_GROUPS + 2, 0);
     for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
@@ -1238,7 +1598,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     // 5 workgroups x 4 waves per CU (occupancy 5); the arrival counters hold up to 64 x 32 groups
     const int grow_grid = std::max(1, std::min(ctx->cu_count * 5, 64 * 32));
     const int rounds_per_sync = opts.rounds_per_sync > 0 ? opts.rounds_per_sync : 8;
-    uint32_t total_rounds = 0, grow_launches = 0, merge_launches = 0;
+    uint32_t total_rounds = 0, grow_launches = 0, merge_launches = 0, async_launches = 0;
+    const int mode = opts.grow_mode;   // 0 tuned, 1 rounds only, 2 work queue only
+    const int async_grid = std::max(1, ctx->cu_count * 5);
     int* ctx_level_rounds = B->level_rounds;
 
     auto run_grow = [&](int k) -> gst_status {
@@ -1260,8 +1622,30 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu
                                     : (uint32_t)std::max<uint64_t>(1, B->n_items_total * (uint64_t)pm / 1000);
         }
-        // first batch sized from the rounds this level took last time, then short batches
-        int batch = opts.rounds_per_sync > 0 ? rounds_per_sync : std::max(8, ctx_level_rounds[k] + 2);
+        ga.ring = B->ring; ga.ring_mask = (uint32_t)(B->cap_ring - 1); ga.actl = B->actl;
+        auto launch_async = [&]() {
+            Ev* e = new_ev(0);
+            hipEventRecord(e->a, s);
+            k_async_begin<<<1, 64, 0, s>>>(B->ctl, B->actl);
+            if (flt) k_grow_async<true><<<async_grid, GROW_BLOCK, 0, s>>>(ga);
+            else k_grow_async<false><<<async_grid, GROW_BLOCK, 0, s>>>(ga);
+            k_async_end<<<1, 64, 0, s>>>(B->ctl, B->actl);
+            hipEventRecord(e->b, s);
+            grow_launches++;
+            async_launches++;
+        };
+        // work queue for the whole level: bounded solves (sparse frontiers) and level 1 (the
+        // frontier starts at the group members); no host round trip
+        if (mode == 2) {
+            launch_async();
+            HIPCHK(hipGetLastError());
+            return GST_OK;
+        }
+        // dense phase in rounds (pull while the queue holds >= the pull threshold), then the
+        // work queue finishes the level once the frontier is sparse
+        int batch = opts.rounds_per_sync > 0 ? rounds_per_sync
+                                             : (mode != 3 ? std::max(8, ctx_level_rounds[k] + 2)
+                                                          : std::max(2, ctx_level_rounds[k]));
         for (int it = 0; it < 100000; ++it) {
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
@@ -1276,7 +1660,14 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             HIPCHK(hipStreamSynchronize(s));
             if (B->h_ctl->err & 1u) return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits");
             if (B->h_ctl->level_done) break;
-            batch = opts.rounds_per_sync > 0 ? rounds_per_sync : 4;
+            if (mode == 3 && B->h_ctl->cnt[B->h_ctl->round % 3] < ga.pull_thresh) {
+                ctx_level_rounds[k] = (int)B->h_ctl->round;
+                total_rounds += B->h_ctl->round;
+                launch_async();
+                HIPCHK(hipGetLastError());
+                return GST_OK;
+            }
+            batch = opts.rounds_per_sync > 0 ? rounds_per_sync : (mode != 3 ? 4 : 1);
         }
         ctx_level_rounds[k] = (int)B->h_ctl->round + 1;
         if (!B->h_ctl->level_done) { B->dirty = true; return gst_set_error(ctx, GST_INTERNAL, "grow did not converge"); }
@@ -1341,6 +1732,15 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (B->h_ctl->err & 1u) { cleanup_evs(); return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits"); }
+    if (B->h_ctl->err & 4u) {
+        // the work queue gave up (wait bound or ring capacity): redo the query on the round path
+        cleanup_evs();
+        B->dirty = true;
+        B->ring_dirty = true;
+        gst_opts o2 = opts;
+        o2.grow_mode = 1;
+        return gst_dpbf_solve(ctx, G, groups, &o2, res);
+    }
 
     std::memset(res, 0, sizeof(*res));
     res->n_groups = g;
@@ -1399,6 +1799,32 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
                                  return p.first == q.first;
                              }),
                  ed.end());
+        // Every edge on a cycle of the traced union has weight zero (dropping it would give a
+        // feasible structure cheaper than the optimum), so any spanning tree of the union has the
+        // DP cost: keep the edges that join two components.
+        {
+            std::vector<uint32_t> vs;
+            vs.reserve(ed.size() * 2);
+            for (auto& p2 : ed) { vs.push_back((uint32_t)(p2.first >> 32)); vs.push_back((uint32_t)p2.first); }
+            std::sort(vs.begin(), vs.end());
+            vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+            std::vector<uint32_t> par(vs.size());
+            for (size_t i = 0; i < par.size(); ++i) par[i] = (uint32_t)i;
+            auto fnd = [&](uint32_t x) {
+                while (par[x] != x) { par[x] = par[par[x]]; x = par[x]; }
+                return x;
+            };
+            auto idx = [&](uint32_t v) { return (uint32_t)(std::lower_bound(vs.begin(), vs.end(), v) - vs.begin()); };
+            std::vector<std::pair<uint64_t, uint32_t>> kept;
+            kept.reserve(ed.size());
+            for (auto& p2 : ed) {
+                const uint32_t a2 = fnd(idx((uint32_t)(p2.first >> 32))), b2 = fnd(idx((uint32_t)p2.first));
+                if (a2 == b2) continue;
+                par[a2] = b2;
+                kept.push_back(p2);
+            }
+            ed.swap(kept);
+        }
         res->n_edges = (int32_t)ed.size();
         res->eu = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
         res->ev = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
@@ -1425,6 +1851,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ];
     S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
+    S.async_launches = async_launches;
     for (auto& e : evs) {
         float ms = 0;
         hipEventElapsedTime(&ms, e.a, e.b);

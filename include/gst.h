@@ -73,7 +73,10 @@ typedef struct {
     int32_t pull_permille;     /* a grow round whose queue holds at least this many per mille of all work
                                   items pulls over every vertex instead of pushing (0 = tuned default,
                                   < 0 = never pull) */
-    int32_t reserved[7];
+    int32_t grow_mode;         /* 0 = tuned (pull/push rounds), 1 = rounds only, 2 = asynchronous device work
+                                  queue for every level, 3 = rounds while the frontier is dense, then the
+                                  work queue */
+    int32_t reserved[6];
 } gst_opts;
 
 typedef struct {
@@ -89,7 +92,7 @@ typedef struct {
     uint32_t grow_rounds;
     uint32_t grow_launches;
     uint32_t merge_launches;
-    uint32_t pad0;
+    uint32_t async_launches;   /* grow levels finished by the asynchronous work queue */
     double   t_total_ms;       /* wall time of the solve, host clock */
     double   t_grow_ms;        /* device time of grow kernels (HIP events) */
     double   t_merge_ms;       /* device time of merge kernels (HIP events) */

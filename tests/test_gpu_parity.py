@@ -174,3 +174,39 @@ def test_cpp_wrapper_example():
         G = gst.gen_random_connected(1000, 5000, 1, 100, seed=seed)
         groups = gst.gen_groups(G.n, 3, 5, 20, seed=seed + 1)
         assert cost == oracle.dpbf_cost(G, groups)
+
+
+@pytest.mark.parametrize("grow_mode,chunk", [(2, 0), (2, 4), (1, 0), (0, 3)])
+def test_work_queue_modes(solver, grow_mode, chunk):
+    """Asynchronous work-queue grow (alone, with rows split into many chunk tasks), rounds only,
+    and the tuned mix give the oracle cost; g=8 exercises a level wider than one bitmap word."""
+    G = gst.gen_barabasi_albert(4000, 4, 1, 60, seed=51)
+    dg = solver.upload(G)
+    for q, g in enumerate((3, 6, 8)):
+        groups = gst.gen_groups(G.n, g, 2, 30, seed=60 + q)
+        r = solver.solve(dg, groups, grow_mode=grow_mode, chunk_arcs=chunk)
+        want = oracle.dpbf_cost(G, groups)
+        print(f"mode {grow_mode} chunk {chunk} g {g}: gpu {r.cost} oracle {want} async {r.stats['async_launches']}",
+              flush=True)
+        _check_tree(G, groups, r, want)
+        rb = solver.solve(dg, groups, ub=want * 1.2, grow_mode=grow_mode, chunk_arcs=chunk)
+        _check_tree(G, groups, rb, want)
+    dg.free()
+
+
+def test_work_queue_float_and_zero_weights(solver):
+    G = gst.gen_rmat(11, 8, weights="lognormal", seed=23)
+    dg = solver.upload(G)
+    for q in range(3):
+        groups = gst.gen_groups(G.n, 5, 5, 40, seed=90 + q)
+        want = oracle.dpbf_cost(G, groups)
+        r = solver.solve(dg, groups, grow_mode=2)
+        assert abs(r.cost - want) <= 1e-6 * want
+    dg.free()
+    G0 = gst.gen_random_connected(500, 2000, 0, 2, seed=77)
+    dg = solver.upload(G0)
+    for q in range(3):
+        groups = gst.gen_groups(G0.n, 6, 2, 10, seed=95 + q)
+        r = solver.solve(dg, groups, grow_mode=2)
+        _check_tree(G0, groups, r, oracle.dpbf_cost(G0, groups))
+    dg.free()
