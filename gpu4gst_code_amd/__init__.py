@@ -288,3 +288,87 @@ class Solver:
                           stats)
         finally:
             self.lib.gst_result_free(C.byref(res))
+
+
+class Model:
+    """Learned-bound network on one Solver's device (see include/gst.h, gst_model_*)."""
+
+    def __init__(self, solver: "Solver", seed: int = 0, params: Optional[np.ndarray] = None):
+        self.solver = solver
+        self.lib = solver.lib
+        h = C.c_void_p()
+        _check(self.lib, self.lib.gst_model_create(solver.ctx, seed & (2**64 - 1), C.byref(h)), solver.ctx)
+        self.handle = h.value
+        self.num_params = int(self.lib.gst_model_num_params())
+        if params is not None:
+            self.set_params(params)
+
+    def free(self):
+        if self.handle:
+            self.lib.gst_model_free(self.handle)
+            self.handle = None
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        _check(self.lib, self.lib.gst_model_get_params(self.handle, _ptr(out)), self.solver.ctx)
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.num_params,)
+        _check(self.lib, self.lib.gst_model_set_params(self.handle, _ptr(p)), self.solver.ctx)
+
+    def scores(self, graph: "DeviceGraph", groups: Groups) -> np.ndarray:
+        out = np.zeros(graph.n, np.float32)
+        grp = groups.as_c()
+        _check(self.lib, self.lib.gst_model_scores(self.handle, graph.handle, C.byref(grp), _ptr(out)), self.solver.ctx)
+        return out
+
+    def loss_grad(self, graph: "DeviceGraph", groups: Groups, labels, pos_weight: float = 1.0, want_grad=False):
+        labels = np.ascontiguousarray(labels, dtype=np.uint8)
+        grad = np.zeros(self.num_params, np.float32) if want_grad else None
+        loss = C.c_float(0)
+        grp = groups.as_c()
+        _check(self.lib, self.lib.gst_model_loss_grad(self.handle, graph.handle, C.byref(grp), _ptr(labels),
+                                                      float(pos_weight), C.byref(loss),
+                                                      _ptr(grad) if grad is not None else None), self.solver.ctx)
+        return loss.value, grad
+
+    def adam_step(self, lr: float) -> None:
+        _check(self.lib, self.lib.gst_model_adam_step(self.handle, float(lr)), self.solver.ctx)
+
+    def save(self, path: str) -> None:
+        np.save(path, self.get_params())
+
+    def load(self, path: str) -> None:
+        self.set_params(np.load(path, allow_pickle=False))
+
+
+def _solve_learned(self, graph: DeviceGraph, groups: Groups, model: Model, topk_frac: float = 0.05, *,
+                   prune_lb: bool = True, want_tree: bool = True) -> Result:
+    opts = _abi.gst_opts()
+    opts.prune_lb = 1 if prune_lb else 0
+    opts.want_tree = 1 if want_tree else 0
+    res = _abi.gst_result()
+    ls = _abi.gst_learned_stats()
+    grp = groups.as_c()
+    st = self.lib.gst_solve_learned(self.ctx, graph.handle, C.byref(grp), model.handle, float(topk_frac),
+                                    C.byref(opts), C.byref(res), C.byref(ls))
+    try:
+        _check(self.lib, st, self.ctx)
+        ne = res.n_edges
+        if ne > 0:
+            eu = np.ctypeslib.as_array(res.eu, shape=(ne,)).copy()
+            ev = np.ctypeslib.as_array(res.ev, shape=(ne,)).copy()
+            ew = np.ctypeslib.as_array(res.ew, shape=(ne,)).copy()
+        else:
+            eu, ev, ew = np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float64)
+        stats = {name: getattr(res.stats, name) for name, _ in _abi.gst_stats._fields_}
+        stats.update({name: getattr(ls, name) for name, _ in _abi.gst_learned_stats._fields_})
+        return Result(res.cost, res.tree_cost, res.root, eu, ev, ew, bool(res.used_ub), bool(res.fell_back), stats)
+    finally:
+        self.lib.gst_result_free(C.byref(res))
+
+
+Solver.solve_learned = _solve_learned
+__all__ += ["Model"]

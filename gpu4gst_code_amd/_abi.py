@@ -56,6 +56,11 @@ class gst_result(C.Structure):
                 ("n_groups", C.c_int32), ("stats", gst_stats)]
 
 
+class gst_learned_stats(C.Structure):
+    _fields_ = [("sub_vertices", C.c_uint32), ("sub_arcs", C.c_uint32), ("ub", C.c_double),
+                ("t_score_select_ms", C.c_double), ("t_sub_solve_ms", C.c_double), ("t_full_solve_ms", C.c_double)]
+
+
 class gst_host_csr(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int64), ("row_ptr", C.POINTER(C.c_int64)),
                 ("col", C.POINTER(C.c_int32)), ("w", C.c_void_p), ("w_type", C.c_int32)]
@@ -76,6 +81,17 @@ SIGNATURES = {
     "gst_result_free": (None, [C.POINTER(gst_result)]),
     "gst_validate_tree": (C.c_int, [C.POINTER(gst_csr), C.POINTER(gst_groups), C.c_int32, C.c_void_p,
                                     C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "gst_model_num_params": (C.c_int64, []),
+    "gst_model_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gst_model_free": (None, [C.c_void_p]),
+    "gst_model_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gst_model_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gst_model_scores": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.c_void_p]),
+    "gst_model_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.c_void_p, C.c_float,
+                                      C.POINTER(C.c_float), C.c_void_p]),
+    "gst_model_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "gst_solve_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.c_void_p, C.c_double,
+                                    C.POINTER(gst_opts), C.POINTER(gst_result), C.POINTER(gst_learned_stats)]),
     "gst_host_csr_free": (None, [C.POINTER(gst_host_csr)]),
     "gst_gen_random_connected": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64,
                                            C.POINTER(gst_host_csr)]),

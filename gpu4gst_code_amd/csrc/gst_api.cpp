@@ -6,11 +6,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <new>
 #include <numeric>
 #include <vector>
 
 #include "gst_internal.h"
+#include "gst_mpnn.h"
 
 gst_status gst_set_error(gst_ctx* ctx, gst_status s, const char* fmt, ...) {
     if (ctx) {
@@ -67,6 +69,11 @@ void gst_levels::build(int g_) {
         }
         pair_cnt[k] = (int)pairs.size() - pair_off[k];
     }
+}
+
+uint64_t gst_next_graph_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
 }
 
 const gst_levels& gst_get_levels(gst_ctx* ctx, int g) {
@@ -177,6 +184,7 @@ gst_status gst_graph_upload(gst_ctx* ctx, const gst_csr* c, gst_graph** out) {
     gst_graph* g = new (std::nothrow) gst_graph();
     if (!g) return gst_set_error(ctx, GST_OOM, "graph: host allocation failed");
     g->ctx = ctx;
+    g->uid = gst_next_graph_uid();
     g->n = c->n;
     g->w_type = c->w_type;
     const uint32_t* w = (const uint32_t*)c->w;
@@ -256,6 +264,34 @@ gst_status gst_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
         if (groups->grp_vtx[j] < 0 || groups->grp_vtx[j] >= g->n)
             return gst_set_error(ctx, GST_INVALID_ARG, "groups: member %d outside [0,%d)", groups->grp_vtx[j], g->n);
     return gst_dpbf_solve(ctx, g, groups, opts, result);
+}
+
+gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, gst_model* m,
+                             double topk_frac, const gst_opts* opts, gst_result* result, gst_learned_stats* stats) {
+    if (!m) return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "null model") : GST_INVALID_ARG;
+    if (!(topk_frac > 0.0 && topk_frac <= 1.0))
+        return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "topk_frac must be in (0, 1]") : GST_INVALID_ARG;
+    // same argument checks as gst_solve (runs them and the plain solve path's validation)
+    if (!ctx || !g || !groups || !result) return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "null argument") : GST_INVALID_ARG;
+    std::memset(result, 0, sizeof(*result));
+    result->root = -1;
+    if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
+    if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
+    const int ng = groups->n_groups;
+    if (ng < 1) return gst_set_error(ctx, GST_INVALID_ARG, "need at least one group");
+    if (ng > GST_MAX_GROUPS)
+        return gst_set_error(ctx, GST_TOO_MANY_GROUPS, "%d groups exceed the maximum of %d", ng, GST_MAX_GROUPS);
+    if (!groups->grp_ptr || (groups->grp_ptr[ng] > 0 && !groups->grp_vtx) || groups->grp_ptr[0] != 0)
+        return gst_set_error(ctx, GST_INVALID_ARG, "groups: malformed grp_ptr");
+    for (int i = 0; i < ng; ++i) {
+        if (groups->grp_ptr[i + 1] < groups->grp_ptr[i])
+            return gst_set_error(ctx, GST_INVALID_ARG, "groups: grp_ptr decreases at group %d", i);
+        if (groups->grp_ptr[i + 1] == groups->grp_ptr[i]) return gst_set_error(ctx, GST_INFEASIBLE, "group %d is empty", i);
+    }
+    for (int32_t j = 0; j < groups->grp_ptr[ng]; ++j)
+        if (groups->grp_vtx[j] < 0 || groups->grp_vtx[j] >= g->n)
+            return gst_set_error(ctx, GST_INVALID_ARG, "groups: member %d outside [0,%d)", groups->grp_vtx[j], g->n);
+    return gst_learned_solve(ctx, g, groups, m, topk_frac, opts, result, stats);
 }
 
 void gst_result_free(gst_result* r) {

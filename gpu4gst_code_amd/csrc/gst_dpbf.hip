@@ -40,6 +40,7 @@ constexpr int WAVE = 64;
 constexpr int GROW_BLOCK = 256;              // 4 waves
 constexpr int GROW_WAVES = GROW_BLOCK / WAVE;
 constexpr int TRACE_STACK = 512;
+constexpr int NCTL = GST_MAX_GROUPS + 2;     // one round-control block per level (entry 0: query flags)
 
 // device-side control block of the grow rounds of one level
 struct RoundCtl {
@@ -1304,14 +1305,19 @@ struct gst_query_bufs {
     uint64_t cap_ring = 0;                // slots (power of two)
     AsyncCtl* actl = nullptr;
     bool ring_dirty = true;               // slots need their tags invalidated
-    int level_rounds[GST_MAX_GROUPS + 2] = {0};   // grow rounds per level in the last solve
+    int level_rounds[GST_MAX_GROUPS + 2] = {0};   // grow rounds per level of the current solve
+    bool budget_valid = false;                   // unused by chained lookups (see budgets)
+    // round budgets learned per (graph, group count, bounded?): hints only, every chained solve
+    // checks convergence and falls back to host-checked rounds
+    struct Budget { uint64_t uid; int ng; bool ub; int rounds[GST_MAX_GROUPS + 2]; };
+    std::vector<Budget> budgets;
     // every (vertex, chunk) item of the current graph, for pull rounds
     uint64_t* all_items = nullptr;
     uint64_t cap_all = 0;
     uint32_t n_all = 0;          // hub items (rows longer than HUB_DEG)
     uint64_t n_items_total = 0;  // work items of a full frontier
     uint32_t* all_cnt = nullptr;
-    const gst_graph* all_g = nullptr;
+    uint64_t all_uid = 0;
     int64_t all_m = -1;
     uint32_t all_chunk = 0;
     RoundCtl* h_ctl = nullptr;   // pinned
@@ -1505,10 +1511,10 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     }
     if (!B->ctl) {
         uint64_t d = 0;
-        ok = ok && grow_alloc(B->ctl, d, 1, sizeof(RoundCtl));
+        ok = ok && grow_alloc(B->ctl, d, NCTL, sizeof(RoundCtl));
         d = 0; ok = ok && grow_alloc(B->root, d, 1, 8);
         d = 0; ok = ok && grow_alloc(B->stats, d, ST_NSTAT, 8);
-        if (hipHostMalloc((void**)&B->h_ctl, sizeof(RoundCtl)) != hipSuccess) ok = false;
+        if (hipHostMalloc((void**)&B->h_ctl, sizeof(RoundCtl) * NCTL) != hipSuccess) ok = false;
     }
     ok = ok && grow_alloc(B->edges, B->cap_edges, 3 * edges_cap + 1, 4);
     if (ok && B->cap_edges != 3 * edges_cap + 1) {
@@ -1548,7 +1554,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     }
 
     // ---- all (vertex, chunk) items of the graph, rebuilt when the graph or chunk changes ----
-    if (B->all_g != G || B->all_m != G->m || B->all_chunk != chunk || !B->all_items) {
+    if (B->all_uid != G->uid || B->all_chunk != chunk || !B->all_items) {
         if (!grow_alloc(B->all_items, B->cap_all, items_cap, 8)) {
             gst_query_bufs_free(ctx);
             return gst_set_error(ctx, GST_OOM, "solve: item list allocation failed");
@@ -1563,7 +1569,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         uint32_t na = 0;
         HIPCHK(hipMemcpyAsync(&na, B->all_cnt, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        B->n_all = na; B->all_g = G; B->all_m = G->m; B->all_chunk = chunk;
+        B->n_all = na; B->all_uid = G->uid; B->all_chunk = chunk;
         uint64_t tot = 0;
         for (uint32_t v = 0; v < n; ++v) tot += (uint64_t)((G->h_row[v + 1] - G->h_row[v] + chunk - 1) / chunk);
         B->n_items_total = tot;
@@ -1578,7 +1584,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     k_set_group_bits<<<64, 256, 0, s>>>(B->gm, B->grp, B->grp + g + 1, g);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->stats, 0ull, ST_NSTAT);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->root, GST_INF_WORD, 1);
-    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)B->ctl, 0u, sizeof(RoundCtl) / 4);
+    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)B->ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)B->actl, 0u, sizeof(AsyncCtl) / 4);
 
     std::vector<uint64_t> lvl_base(GST_MAX_GROUPS + 2, 0);
@@ -1599,7 +1605,14 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     const int grow_grid = std::max(1, std::min(ctx->cu_count * 5, 64 * 32));
     const int rounds_per_sync = opts.rounds_per_sync > 0 ? opts.rounds_per_sync : 8;
     uint32_t total_rounds = 0, grow_launches = 0, merge_launches = 0, async_launches = 0;
-    const int mode = opts.grow_mode;   // 0 tuned, 1 rounds only, 2 work queue only
+    const int mode = opts.grow_mode;   // 0 tuned, 1 rounds only, 2 work queue only, 3 rounds then queue
+    // chained rounds once every level's round count is known for this graph and group count
+    gst_query_bufs::Budget* bud = nullptr;
+    for (auto& b : B->budgets)
+        if (b.uid == (G->budget_key ? G->budget_key : G->uid) && b.ng == g && b.ub == has_ub) bud = &b;
+    const bool chained = (mode == 0 || mode == 1) && opts.rounds_per_sync == 0 && bud != nullptr &&
+                         !opts.reserved[0];
+    for (int k = 0; k < GST_MAX_GROUPS + 2; ++k) B->level_rounds[k] = chained ? bud->rounds[k] : 0;
     const int async_grid = std::max(1, ctx->cu_count * 5);
     int* ctx_level_rounds = B->level_rounds;
 
@@ -1610,7 +1623,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ga.masks = B->masks + L.off[k];
         ga.row = G->d_row; ga.col = G->d_col; ga.w = G->d_w; ga.rev = G->d_rev;
         for (int i = 0; i < 3; ++i) { ga.list[i] = B->lists[i]; ga.chg[i] = B->chg[i]; ga.inq[i] = B->inq[i]; }
-        ga.ctl = B->ctl; ga.stats = B->stats;
+        ga.ctl = B->ctl + k; ga.stats = B->stats;
         ga.m = (uint32_t)L.m[k]; ga.W = (uint32_t)((L.m[k] + 63) / 64); ga.g = (uint32_t)g;
         ga.chunk = chunk; ga.ub = ub_bits;
         // the one-label lower bound reads the group distances of level 1, which are final only
@@ -1626,10 +1639,10 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         auto launch_async = [&]() {
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
-            k_async_begin<<<1, 64, 0, s>>>(B->ctl, B->actl);
+            k_async_begin<<<1, 64, 0, s>>>(B->ctl + k, B->actl);
             if (flt) k_grow_async<true><<<async_grid, GROW_BLOCK, 0, s>>>(ga);
             else k_grow_async<false><<<async_grid, GROW_BLOCK, 0, s>>>(ga);
-            k_async_end<<<1, 64, 0, s>>>(B->ctl, B->actl);
+            k_async_end<<<1, 64, 0, s>>>(B->ctl + k, B->actl);
             hipEventRecord(e->b, s);
             grow_launches++;
             async_launches++;
@@ -1638,6 +1651,21 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         // frontier starts at the group members); no host round trip
         if (mode == 2) {
             launch_async();
+            HIPCHK(hipGetLastError());
+            return GST_OK;
+        }
+        // chained: a round budget learned from earlier solves, no host round trip; convergence
+        // of every level is checked once at the end of the query
+        if (chained) {
+            const int budget = ctx_level_rounds[k] + std::max(3, ctx_level_rounds[k] / 3);
+            Ev* e = new_ev(0);
+            hipEventRecord(e->a, s);
+            for (int rr = 0; rr < budget; ++rr) {
+                if (flt) k_grow_round<true><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
+                else k_grow_round<false><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
+                grow_launches++;
+            }
+            hipEventRecord(e->b, s);
             HIPCHK(hipGetLastError());
             return GST_OK;
         }
@@ -1656,7 +1684,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             }
             hipEventRecord(e->b, s);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl + k, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (B->h_ctl->err & 1u) return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits");
             if (B->h_ctl->level_done) break;
@@ -1678,14 +1706,13 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     gst_status st = GST_OK;
     // ---- level 1 ----
     {
-        k_ctl_reset<<<1, 64, 0, s>>>(B->ctl);
         Ev* e = new_ev(1);
         hipEventRecord(e->a, s);
         k_init_level1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], B->gm, G->d_row, n, (uint32_t)g,
                                                                    B->chg[0], B->inq[0], g > 1 ? 1 : 0);
         if (g > 1)
             k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->inq[0], G->d_row, n, chunk, B->lists[0],
-                                                                  &B->ctl->cnt[0], B->stats, 0u);
+                                                                  &B->ctl[1].cnt[0], B->stats, 0u);
         hipEventRecord(e->b, s);
         if (g > 1) st = run_grow(1);
         else k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T, n, B->root);
@@ -1693,10 +1720,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     // ---- levels 2..g ----
     for (int k = 2; k <= g && st == GST_OK; ++k) {
         const bool full = (k == g);
-        k_ctl_reset<<<1, 64, 0, s>>>(B->ctl);
         MergeArgs ma;
         ma.T = B->T; ma.pairs = B->pairs + L.pair_off[k]; ma.masks = B->masks; ma.gm = B->gm; ma.row = G->d_row;
-        ma.list0 = B->lists[0]; ma.chg0 = B->chg[0]; ma.inq0 = B->inq[0]; ma.ctl = B->ctl; ma.root = B->root;
+        ma.list0 = B->lists[0]; ma.chg0 = B->chg[0]; ma.inq0 = B->inq[0]; ma.ctl = B->ctl + k; ma.root = B->root;
         ma.stats = B->stats;
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ma.lvl_base[j] = lvl_base[j]; ma.m[j] = (uint32_t)L.m[j]; ma.off[j] = (uint32_t)L.off[j]; }
         ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
@@ -1719,7 +1745,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         hipEventRecord(e->b, s);
         if (!full)
             k_compact<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->inq[0], G->d_row, n, chunk, B->lists[0],
-                                                                  &B->ctl->cnt[0], B->stats, 0u);
+                                                                  &B->ctl[k].cnt[0], B->stats, 0u);
         merge_launches++;
         HIPCHK(hipGetLastError());
         if (!full) st = run_grow(k);
@@ -1729,10 +1755,40 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     // ---- root + traceback ----
     unsigned long long h_root = GST_INF_WORD;
     HIPCHK(hipMemcpyAsync(&h_root, B->root, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl) * NCTL, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (B->h_ctl->err & 1u) { cleanup_evs(); return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits"); }
-    if (B->h_ctl->err & 4u) {
+    uint32_t all_err = 0;
+    bool unconverged = false;
+    for (int k = 0; k <= g; ++k) {
+        all_err |= B->h_ctl[k].err;
+        if (k >= 1 && k < g && g > 1 && !B->h_ctl[k].level_done) unconverged = true;
+    }
+    if (all_err & 1u) { cleanup_evs(); return gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits"); }
+    // learn the round budget of every level from this solve
+    {
+        if (!bud) {
+            if (B->budgets.size() >= 16) B->budgets.erase(B->budgets.begin());
+            B->budgets.push_back(gst_query_bufs::Budget{G->budget_key ? G->budget_key : G->uid, g, has_ub, {0}});
+            bud = &B->budgets.back();
+        }
+        for (int k = 1; k < g; ++k) {
+            const int used = (int)B->h_ctl[k].round + 1;
+            bud->rounds[k] = chained ? std::max(bud->rounds[k], used) : used;
+        }
+    }
+    if (chained) {
+        total_rounds = 0;
+        for (int k = 1; k < g; ++k) total_rounds += B->h_ctl[k].round + 1;
+        if (unconverged) {
+            // a level ran out of its round budget: redo the query with a host check per batch
+            cleanup_evs();
+            B->dirty = true;
+            gst_opts o2 = opts;
+            o2.reserved[0] = 1;   // internal: no chaining for the redo
+            return gst_dpbf_solve(ctx, G, groups, &o2, res);
+        }
+    }
+    if (all_err & 4u) {
         // the work queue gave up (wait bound or ring capacity): redo the query on the round path
         cleanup_evs();
         B->dirty = true;
@@ -1762,7 +1818,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         TraceArgs ta;
         ta.T = B->T; ta.pos_of = B->pos_of; ta.col = G->d_col; ta.w = G->d_w; ta.root = B->root;
         ta.out_u = B->edges; ta.out_v = B->edges + ecap; ta.out_w = B->edges + 2 * ecap; ta.out_n = B->edges + 3 * ecap;
-        ta.ctl = B->ctl;
+        ta.ctl = B->ctl;   // entry 0 carries query-level flags
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ta.lvl_base[j] = lvl_base[j]; ta.m[j] = (uint32_t)L.m[j]; ta.off[j] = (uint32_t)L.off[j]; }
         ta.full = (uint32_t)L.full; ta.cap = ecap; ta.max_iter = n + 4u * (uint32_t)g + 64u;
         Ev* e = new_ev(2);
@@ -1774,7 +1830,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         HIPCHK(hipMemcpyAsync(&nout, B->edges + 3 * ecap, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(B->h_ctl, B->ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if ((B->h_ctl->err & 2u) || nout > ecap) {
+        if ((B->h_ctl[0].err & 2u) || nout > ecap) {
             cleanup_evs();
             return gst_set_error(ctx, GST_INTERNAL, "device traceback failed (edges=%u)", nout);
         }

@@ -42,6 +42,8 @@ struct gst_levels {
 
 struct gst_graph {
     gst_ctx* ctx = nullptr;
+    uint64_t uid = 0;            // unique per upload: caches key on it, never on the address
+    uint64_t budget_key = 0;     // key of the learned round budgets (0: uid)
     int32_t n = 0;
     int64_t m = 0;
     int32_t w_type = 0;
@@ -78,4 +80,5 @@ gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g);
 void gst_graph_device_free(gst_graph* g);
 
 gst_status gst_set_error(gst_ctx* ctx, gst_status s, const char* fmt, ...);
+uint64_t gst_next_graph_uid();
 const gst_levels& gst_get_levels(gst_ctx* ctx, int g);

@@ -1,0 +1,1017 @@
+// Learned upper bound for the DPBF solve: a two-layer mean-aggregation message-passing network
+// (hidden 64, BF16 weights and activations, FP32 accumulation) scores vertices; the DPBF optimum
+// of the subgraph induced by the top-scored vertices (plus every group member) is a feasible
+// tree, whose cost bounds the full solve.  The network never decides the answer: the bounded
+// full DPBF returns the exact optimum, and without a feasible subgraph tree the plain DPBF runs.
+//
+//   x_v   = [log1p(deg)/8, log1p(min incident weight)/8, member of group 0..7, 0 x 6]   (16, bf16)
+//   h1_v  = relu(W1 [x_v ; mean_{u in N(v)} x_u] + b1)                                    (64)
+//   h2_v  = relu(W2 [h1_v ; mean_{u in N(v)} h1_u] + b2)                                   (64)
+//   s_v   = w3 . h2_v + b3
+// Dense layers run on the gfx950 MFMA unit (v_mfma_f32_16x16x32_bf16); aggregation is an
+// HBM-bound gather.  Training: weighted logistic loss on "vertex is in the optimal tree" labels
+// produced by the GPU DPBF itself, gradients by hand-written backward kernels, Adam in FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gst_internal.h"
+#include "gst_mpnn.h"
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return gst_set_error(ctx, GST_HIP_ERROR, "%s failed: %s (%s:%d)", #call,         \
+                                 hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int FIN = 16;    // input features (10 used)
+constexpr int HID = 64;
+constexpr int K1 = 2 * FIN;   // 32
+constexpr int K2 = 2 * HID;   // 128
+constexpr int AGG_HUB = 512;  // rows longer than this are aggregated in chunks
+
+// parameter offsets (floats)
+constexpr int OFF_W1 = 0;                    // [HID][K1]
+constexpr int OFF_B1 = OFF_W1 + HID * K1;    // [HID]
+constexpr int OFF_W2 = OFF_B1 + HID;         // [HID][K2]
+constexpr int OFF_B2 = OFF_W2 + HID * K2;    // [HID]
+constexpr int OFF_W3 = OFF_B2 + HID;         // [HID]
+constexpr int OFF_B3 = OFF_W3 + HID;         // [1]
+constexpr int NPARAM = OFF_B3 + 1;
+
+__device__ __forceinline__ float bf2f(bf16 x) { return (float)x; }
+__device__ __forceinline__ bf16 f2bf(float x) { return (bf16)x; }  // round to nearest even
+
+// ---------------------------------------------------------------------------------------
+// features
+// ---------------------------------------------------------------------------------------
+__global__ void k_features(const uint32_t* row, const uint32_t* w, int w_float, const uint32_t* gm, uint32_t n,
+                           bf16* x) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t rs = row[v], re = row[v + 1];
+        float minw = 0.f;
+        if (re > rs) {
+            minw = INFINITY;
+            for (uint32_t e = rs; e < re; ++e) {
+                const float wf = w_float ? __uint_as_float(w[e]) : (float)w[e];
+                minw = fminf(minw, wf);
+            }
+        }
+        const uint32_t gmv = gm[v];
+        bf16* xv = x + (uint64_t)v * FIN;
+        xv[0] = f2bf(log1pf((float)(re - rs)) * 0.125f);
+        xv[1] = f2bf(log1pf(minw) * 0.125f);
+        for (int i = 0; i < 8; ++i) xv[2 + i] = f2bf(((gmv >> i) & 1u) ? 1.f : 0.f);
+        for (int i = 10; i < FIN; ++i) xv[i] = f2bf(0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// aggregation (HBM-bound gathers)
+//   forward : out[u] = mean_{nb in N(u)} in[nb]             (bf16 in/out, fp32 sums)
+//   backward: out[u] += sum_{nb in N(u)} in[nb] / deg(nb)   (fp32; transpose of the mean, the
+//             graph being symmetric)
+// One wave per light row: lanes = (neighbour slot, feature pair).  Rows longer than AGG_HUB are
+// split into AGG_HUB-arc chunks whose partial sums are added atomically into fp32 scratch.
+// ---------------------------------------------------------------------------------------
+template <int D, bool BWD>
+__global__ void __launch_bounds__(256) k_aggregate(const uint32_t* row, const int32_t* col, uint32_t n, const void* in_,
+                                                   void* out_, float* hub_acc, const uint32_t* hub_rows,
+                                                   uint32_t n_hub_items) {
+    constexpr int FP = D / 2;            // feature pairs
+    constexpr int SLOTS = 64 / FP;       // neighbour slots per wave
+    const uint32_t lane = __lane_id();
+    const uint32_t fp = lane % FP, slot = lane / FP;
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    // light rows
+    for (uint32_t u = gwave; u < n; u += nwaves) {
+        const uint32_t rs = row[u], re = row[u + 1];
+        const uint32_t deg = re - rs;
+        if (deg > AGG_HUB) continue;
+        float s0 = 0.f, s1 = 0.f;
+        for (uint32_t e = rs + slot; e < re; e += SLOTS) {
+            const uint32_t nb = (uint32_t)col[e];
+            if (BWD) {
+                const float* in = (const float*)in_;
+                const float sc = 1.f / (float)(row[nb + 1] - row[nb]);
+                s0 += in[(uint64_t)nb * D + 2 * fp] * sc;
+                s1 += in[(uint64_t)nb * D + 2 * fp + 1] * sc;
+            } else {
+                const bf16* in = (const bf16*)in_;
+                s0 += bf2f(in[(uint64_t)nb * D + 2 * fp]);
+                s1 += bf2f(in[(uint64_t)nb * D + 2 * fp + 1]);
+            }
+        }
+#pragma unroll
+        for (int o = FP; o < 64; o <<= 1) {
+            s0 += __shfl_xor(s0, o, 64);
+            s1 += __shfl_xor(s1, o, 64);
+        }
+        if (slot == 0) {
+            if (BWD) {
+                float* out = (float*)out_;
+                out[(uint64_t)u * D + 2 * fp] += s0;
+                out[(uint64_t)u * D + 2 * fp + 1] += s1;
+            } else {
+                bf16* out = (bf16*)out_;
+                const float inv = deg ? 1.f / (float)deg : 0.f;
+                out[(uint64_t)u * D + 2 * fp] = f2bf(s0 * inv);
+                out[(uint64_t)u * D + 2 * fp + 1] = f2bf(s1 * inv);
+            }
+        }
+    }
+    // hub chunks: item i -> (hub index h, chunk c) packed as h | c << 32 in hub_rows pairs
+    for (uint32_t i = gwave; i < n_hub_items; i += nwaves) {
+        const uint32_t u = hub_rows[2 * i];
+        const uint32_t h = hub_rows[2 * i + 1] & 0xFFFFFu;
+        const uint32_t c = hub_rows[2 * i + 1] >> 20;
+        const uint32_t rs = row[u], re = row[u + 1];
+        const uint32_t e0 = rs + c * AGG_HUB, e1 = min(re, e0 + AGG_HUB);
+        float s0 = 0.f, s1 = 0.f;
+        for (uint32_t e = e0 + slot; e < e1; e += SLOTS) {
+            const uint32_t nb = (uint32_t)col[e];
+            if (BWD) {
+                const float* in = (const float*)in_;
+                const float sc = 1.f / (float)(row[nb + 1] - row[nb]);
+                s0 += in[(uint64_t)nb * D + 2 * fp] * sc;
+                s1 += in[(uint64_t)nb * D + 2 * fp + 1] * sc;
+            } else {
+                const bf16* in = (const bf16*)in_;
+                s0 += bf2f(in[(uint64_t)nb * D + 2 * fp]);
+                s1 += bf2f(in[(uint64_t)nb * D + 2 * fp + 1]);
+            }
+        }
+#pragma unroll
+        for (int o = FP; o < 64; o <<= 1) {
+            s0 += __shfl_xor(s0, o, 64);
+            s1 += __shfl_xor(s1, o, 64);
+        }
+        if (slot == 0) {
+            atomicAdd(&hub_acc[(uint64_t)h * D + 2 * fp], s0);
+            atomicAdd(&hub_acc[(uint64_t)h * D + 2 * fp + 1], s1);
+        }
+    }
+}
+
+// finish hub rows: forward writes mean (bf16), backward adds the sums
+template <int D, bool BWD>
+__global__ void k_aggregate_hub_finish(const uint32_t* row, const uint32_t* hub_list, uint32_t n_hub, const float* hub_acc,
+                                       void* out_) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_hub * D; i += gridDim.x * blockDim.x) {
+        const uint32_t h = i / D, f = i % D;
+        const uint32_t u = hub_list[h];
+        if (BWD) {
+            ((float*)out_)[(uint64_t)u * D + f] += hub_acc[i];
+        } else {
+            const float deg = (float)(row[u + 1] - row[u]);
+            ((bf16*)out_)[(uint64_t)u * D + f] = f2bf(hub_acc[i] / deg);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// dense layers on MFMA (v_mfma_f32_16x16x32_bf16).  Lane l of a 16x16x32 step holds
+//   A[l%16][8(l/16)+j], B[8(l/16)+j][l%16] (j<8) and D[4(l/16)+i][l%16] (i<4).
+// ---------------------------------------------------------------------------------------
+
+// out[v][n] = relu(sum_k X[v][k] Wt[n][k] + b[n]), X = [self | nbr] (D each), K = 2D, N = 64.
+// One wave per 16-row tile.
+template <int D>
+__global__ void __launch_bounds__(256) k_dense_fwd(const bf16* self, const bf16* nbr, const bf16* Wt, const float* b,
+                                                   uint32_t n, bf16* out) {
+    constexpr int K = 2 * D;
+    const uint32_t lane = __lane_id();
+    const uint32_t r = lane % 16, q = lane / 16;
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    const uint32_t ntiles = (n + 15) / 16;
+    for (uint32_t t = gwave; t < ntiles; t += nwaves) {
+        const uint32_t v0 = t * 16;
+        floatx4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+        const uint32_t va = v0 + r;
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const int kk = k0 + 8 * (int)q;   // this lane's 8 consecutive k
+            bf16x8 a;
+            if (va < n) {
+                const bf16* src = kk < D ? self + (uint64_t)va * D + kk : nbr + (uint64_t)va * D + (kk - D);
+                a = *(const bf16x8*)src;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] = f2bf(0.f);
+            }
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const bf16x8 bv = *(const bf16x8*)(Wt + (uint64_t)(nb * 16 + r) * K + kk);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[nb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const uint32_t col_n = nb * 16 + r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t v = v0 + 4 * q + i;
+                if (v < n) out[(uint64_t)v * HID + col_n] = f2bf(fmaxf(acc[nb][i] + b[col_n], 0.f));
+            }
+        }
+    }
+}
+
+// dX[v][k] = sum_n dZ[v][n] W[n][k]  (dZ bf16 [n x 64], Wkn bf16 [K][64] = W transposed, dX fp32)
+template <int K>
+__global__ void __launch_bounds__(256) k_dense_bwd_x(const bf16* dZ, const bf16* Wkn, uint32_t n, float* dX) {
+    const uint32_t lane = __lane_id();
+    const uint32_t r = lane % 16, q = lane / 16;
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    const uint32_t ntiles = (n + 15) / 16;
+    for (uint32_t t = gwave; t < ntiles; t += nwaves) {
+        const uint32_t v0 = t * 16;
+        const uint32_t va = v0 + r;
+        bf16x8 a0, a1;   // dZ row chunk: n in [8q, 8q+8) and [32+8q, 32+8q+8)
+        if (va < n) {
+            a0 = *(const bf16x8*)(dZ + (uint64_t)va * HID + 8 * q);
+            a1 = *(const bf16x8*)(dZ + (uint64_t)va * HID + 32 + 8 * q);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) { a0[j] = f2bf(0.f); a1[j] = f2bf(0.f); }
+        }
+#pragma unroll
+        for (int kb = 0; kb < K / 16; ++kb) {
+            floatx4 acc = {0, 0, 0, 0};
+            // B[n][k] = W[n][k] = Wkn[k][n]: lane needs n = 8q..8q+7 at k = kb*16 + r
+            const bf16x8 b0 = *(const bf16x8*)(Wkn + (uint64_t)(kb * 16 + r) * HID + 8 * q);
+            const bf16x8 b1 = *(const bf16x8*)(Wkn + (uint64_t)(kb * 16 + r) * HID + 32 + 8 * q);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t v = v0 + 4 * q + i;
+                if (v < n) dX[(uint64_t)v * K + kb * 16 + r] = acc[i];
+            }
+        }
+    }
+}
+
+// dW[n][k] += sum_v dZ[v][n] X[v][k], X = [self | nbr]; reduction over v in 32-row chunks staged
+// in LDS (transposed operands), partial sums of a workgroup added atomically.
+template <int D>
+__global__ void __launch_bounds__(256) k_dense_bwd_w(const bf16* dZ, const bf16* self, const bf16* nbr, uint32_t n,
+                                                     float* dW) {
+    constexpr int K = 2 * D;
+    constexpr int NT = 4 * (K / 16);           // output tiles (n-block, k-block)
+    constexpr int TPW = (NT + 3) / 4;          // tiles per wave
+    __shared__ bf16 sZ[32][HID + 8];
+    __shared__ bf16 sX[32][K + 8];
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+    const uint32_t r = lane % 16, q = lane / 16;
+    floatx4 acc[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = {0, 0, 0, 0};
+    const uint32_t nchunks = (n + 31) / 32;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint32_t v0 = ch * 32;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 32 * HID; i += blockDim.x) {
+            const uint32_t vv = i / HID, f = i % HID;
+            sZ[vv][f] = (v0 + vv < n) ? dZ[(uint64_t)(v0 + vv) * HID + f] : f2bf(0.f);
+        }
+        for (uint32_t i = threadIdx.x; i < 32 * K; i += blockDim.x) {
+            const uint32_t vv = i / K, k = i % K;
+            bf16 x = f2bf(0.f);
+            if (v0 + vv < n) x = k < (uint32_t)D ? self[(uint64_t)(v0 + vv) * D + k] : nbr[(uint64_t)(v0 + vv) * D + (k - D)];
+            sX[vv][k] = x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            const int tile = (int)wid * TPW + t;
+            if (tile >= NT) break;
+            const int nb = tile % 4, kb = tile / 4;
+            bf16x8 a, bv;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                a[j] = sZ[8 * q + j][nb * 16 + r];     // A[n][vv] = dZ[vv][n]
+                bv[j] = sX[8 * q + j][kb * 16 + r];    // B[vv][k] = X[vv][k]
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int tile = (int)wid * TPW + t;
+        if (tile >= NT) break;
+        const int nb = tile % 4, kb = tile / 4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) atomicAdd(&dW[(uint64_t)(nb * 16 + 4 * q + i) * K + kb * 16 + r], acc[t][i]);
+    }
+}
+
+// s_v = w3 . h2_v + b3
+__global__ void k_head(const bf16* h2, const float* w3, const float* b3, uint32_t n, float* s) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        float acc = b3[0];
+        const bf16* hv = h2 + (uint64_t)v * HID;
+#pragma unroll 8
+        for (int f = 0; f < HID; ++f) acc += w3[f] * bf2f(hv[f]);
+        s[v] = acc;
+    }
+}
+
+// weighted logistic loss: L = sum_v wt_v * (softplus(s) - y s); dS = wt_v * (sigmoid(s) - y)
+// wt_v = pos_weight for y=1, 1 otherwise; everything divided by n.
+__global__ void k_loss(const float* s, const uint8_t* y, uint32_t n, float pos_weight, float* dS, float* loss) {
+    float l = 0.f;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const float sv = s[v];
+        const float yv = y[v] ? 1.f : 0.f;
+        const float wt = (y[v] ? pos_weight : 1.f) / (float)n;
+        const float sp = sv > 0 ? sv + log1pf(expf(-sv)) : log1pf(expf(sv));
+        l += wt * (sp - yv * sv);
+        const float sig = 1.f / (1.f + expf(-sv));
+        dS[v] = wt * (sig - yv);
+    }
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+    if (__lane_id() == 0) atomicAdd(loss, l);
+}
+
+// head backward: grad w3 += sum dS h2, grad b3 += sum dS, dZ2 = dS w3 (relu mask), as bf16
+__global__ void k_head_bwd(const float* dS, const bf16* h2, const float* w3, uint32_t n, bf16* dZ2, float* gw3, float* gb3) {
+    __shared__ float sw[HID + 1];
+    if (threadIdx.x <= HID) sw[threadIdx.x] = 0.f;
+    __syncthreads();
+    const uint32_t f = threadIdx.x % HID;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * HID;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)(i / HID);
+        const float h = bf2f(h2[i]);
+        const float d = dS[v];
+        atomicAdd(&sw[f], d * h);
+        if (f == 0) atomicAdd(&sw[HID], d);
+        dZ2[i] = f2bf(h > 0.f ? d * w3[f] : 0.f);
+    }
+    __syncthreads();
+    if (threadIdx.x < HID) atomicAdd(&gw3[threadIdx.x], sw[threadIdx.x]);
+    if (threadIdx.x == HID) atomicAdd(gb3, sw[HID]);
+}
+
+// layer-2 input gradient split: dH1 = dX2[:, :64] (self part); dA1 = dX2[:, 64:] goes through the
+// transposed aggregation; then dZ1 = dH1 (relu mask of h1), bias grads
+__global__ void k_split_self(const float* dX2, uint32_t n, float* dH1, float* dA1) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * HID;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = i / HID, f = i % HID;
+        dH1[i] = dX2[v * K2 + f];
+        dA1[i] = dX2[v * K2 + HID + f];
+    }
+}
+
+__global__ void k_relu_mask_bias(const float* dH, const bf16* h, uint32_t n, bf16* dZ, float* gb) {
+    __shared__ float sb[HID];
+    if (threadIdx.x < HID) sb[threadIdx.x] = 0.f;
+    __syncthreads();
+    const uint32_t f = threadIdx.x % HID;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * HID;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float d = bf2f(h[i]) > 0.f ? dH[i] : 0.f;
+        const bf16 db = f2bf(d);
+        dZ[i] = db;
+        atomicAdd(&sb[f], bf2f(db));
+    }
+    __syncthreads();
+    if (threadIdx.x < HID) atomicAdd(&gb[threadIdx.x], sb[threadIdx.x]);
+}
+
+// layer-2 bias gradient from dZ2 (bf16)
+__global__ void k_bias_grad(const bf16* dZ, uint32_t n, float* gb) {
+    __shared__ float sb[HID];
+    if (threadIdx.x < HID) sb[threadIdx.x] = 0.f;
+    __syncthreads();
+    const uint32_t f = threadIdx.x % HID;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)n * HID;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        atomicAdd(&sb[f], bf2f(dZ[i]));
+    __syncthreads();
+    if (threadIdx.x < HID) atomicAdd(&gb[threadIdx.x], sb[threadIdx.x]);
+}
+
+// Adam on fp32 master parameters; refresh the bf16 copies (Wt row-major and transposed)
+__global__ void k_adam(float* p, float* g, float* m, float* v, int np, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2, float inv_count, bf16* W1t, bf16* W2t, bf16* W1kn, bf16* W2kn) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+        const float gi = g[i] * inv_count;
+        m[i] = b1 * m[i] + (1.f - b1) * gi;
+        v[i] = b2 * v[i] + (1.f - b2) * gi * gi;
+        const float mh = m[i] / bc1, vh = v[i] / bc2;
+        p[i] -= lr * mh / (sqrtf(vh) + eps);
+        g[i] = 0.f;
+    }
+}
+
+__global__ void k_refresh_bf16(const float* p, bf16* W1t, bf16* W2t, bf16* W1kn, bf16* W2kn) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < HID * K2; i += gridDim.x * blockDim.x) {
+        if (i < HID * K1) {
+            const int nn = i / K1, k = i % K1;
+            W1t[i] = f2bf(p[OFF_W1 + i]);
+            W1kn[k * HID + nn] = f2bf(p[OFF_W1 + i]);
+        }
+        const int nn = i / K2, k = i % K2;
+        W2t[i] = f2bf(p[OFF_W2 + i]);
+        W2kn[k * HID + nn] = f2bf(p[OFF_W2 + i]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// top-k selection and induced subgraph
+// ---------------------------------------------------------------------------------------
+// order-preserving key of a float score (larger score -> larger key)
+__device__ __forceinline__ uint32_t score_key(float s) {
+    const uint32_t b = __float_as_uint(s);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// histogram of the top 16 key bits (pass 0) or of the low 16 bits among keys whose high 16 bits
+// equal `prefix` (pass 1)
+__global__ void k_key_hist(const float* s, uint32_t n, int pass, uint32_t prefix, uint32_t* hist) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t k = score_key(s[v]);
+        if (pass == 0) atomicAdd(&hist[k >> 16], 1u);
+        else if ((k >> 16) == prefix) atomicAdd(&hist[k & 0xFFFFu], 1u);
+    }
+}
+
+// select v when key >= thr (ties at thr resolved by a count budget), plus every group member
+__global__ void k_select(const float* s, const uint32_t* gm, uint32_t n, uint32_t thr, int* tie_budget,
+                         uint32_t* sel) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t k = score_key(s[v]);
+        uint32_t x = (k > thr) ? 1u : 0u;
+        if (k == thr && atomicSub(tie_budget, 1) > 0) x = 1u;   // signed: stays <= 0 once spent
+        if (gm[v]) x = 1u;
+        sel[v] = x;
+    }
+}
+
+// per-block exclusive scan of a u32 array (block = 1024 elements), block sums out
+__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* bsum) {
+    __shared__ uint32_t sw[4];
+    const uint32_t base = blockIdx.x * 1024;
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+    uint32_t vals[4], tot = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = base + threadIdx.x * 4 + j;
+        vals[j] = i < n ? in[i] : 0u;
+        tot += vals[j];
+    }
+    uint32_t incl = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    if (lane == 63) sw[wid] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t w2 = 0; w2 < wid; ++w2) off += sw[w2];
+    uint32_t run = off + incl - tot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = base + threadIdx.x * 4 + j;
+        if (i < n) out[i] = run;
+        run += vals[j];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
+}
+
+// exclusive scan of the block sums (single workgroup, sequential chunks), total in bsum[nb]
+__global__ void k_scan_sums(uint32_t* bsum, uint32_t nb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < nb; ++i) { const uint32_t x = bsum[i]; bsum[i] = run; run += x; }
+    bsum[nb] = run;
+}
+
+__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* bsum) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] += bsum[i / 1024];
+}
+
+// degree of v inside the selection
+__global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_t* sel, uint32_t n, uint32_t* deg) {
+    const uint32_t lane = __lane_id();
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    for (uint32_t v = gwave; v < n; v += nwaves) {
+        uint32_t c = 0;
+        if (sel[v]) {
+            for (uint32_t e = row[v] + lane; e < row[v + 1]; e += 64) c += sel[col[e]];
+        }
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) deg[v] = sel[v] ? c : 0u;
+    }
+}
+
+// fill the subgraph CSR: new row r' = newid[v]; arcs in neighbour order (ids stay sorted)
+__global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32_t* w, const uint32_t* sel,
+                           const uint32_t* newid, const uint32_t* arc_off, uint32_t n, uint32_t* srow, int32_t* scol,
+                           uint32_t* sw, uint32_t* old_of) {
+    const uint32_t lane = __lane_id();
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    for (uint32_t v = gwave; v < n; v += nwaves) {
+        if (!sel[v]) continue;
+        const uint32_t nv = newid[v];
+        uint32_t pos = arc_off[v];
+        if (lane == 0) { srow[nv] = pos; old_of[nv] = v; }
+        for (uint32_t e0 = row[v]; e0 < row[v + 1]; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const bool ok = e < row[v + 1] && sel[col[e]];
+            const uint64_t bal = __ballot(ok);
+            if (ok) {
+                const uint32_t p = pos + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                scol[p] = (int32_t)newid[col[e]];
+                sw[p] = w[e];
+            }
+            pos += (uint32_t)__popcll(bal);
+        }
+    }
+}
+
+__global__ void k_sub_rev(const uint32_t* srow, const int32_t* scol, uint32_t ns, uint32_t* srev) {
+    const uint32_t lane = __lane_id();
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    for (uint32_t v = gwave; v < ns; v += nwaves) {
+        for (uint32_t e = srow[v] + lane; e < srow[v + 1]; e += 64) {
+            const uint32_t u = (uint32_t)scol[e];
+            uint32_t lo = srow[u], hi = srow[u + 1];
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                if ((uint32_t)scol[mid] < v) lo = mid + 1; else hi = mid;
+            }
+            srev[e] = lo;
+        }
+    }
+}
+
+__global__ void k_sub_groups(const int32_t* gv, uint32_t cnt, const uint32_t* newid, int32_t* out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        out[i] = (int32_t)newid[gv[i]];
+}
+
+__global__ void k_set_bits(uint32_t* gm, const int32_t* ptr, const int32_t* vtx, int g) {
+    for (int i = 0; i < g; ++i)
+        for (int j = ptr[i] + blockIdx.x * blockDim.x + threadIdx.x; j < ptr[i + 1]; j += gridDim.x * blockDim.x)
+            atomicOr(&gm[vtx[j]], 1u << i);
+}
+
+__global__ void k_fill32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+int gridn(uint64_t work, int block, int cap) {
+    uint64_t b = (work + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > (uint64_t)cap) b = cap;
+    return (int)b;
+}
+
+template <class T>
+bool dalloc(T*& p, size_t elems) {
+    hipFree(p);
+    p = nullptr;
+    return hipMalloc((void**)&p, std::max<size_t>(elems, 1) * sizeof(T)) == hipSuccess;
+}
+
+}  // namespace
+
+// =======================================================================================
+struct gst_model {
+    gst_ctx* ctx = nullptr;
+    float* p = nullptr;   // NPARAM fp32 master
+    float* g = nullptr;   // gradient accumulator
+    float* m = nullptr;
+    float* v = nullptr;
+    bf16* W1t = nullptr;  // [HID][K1]
+    bf16* W2t = nullptr;  // [HID][K2]
+    bf16* W1kn = nullptr; // [K1][HID]
+    bf16* W2kn = nullptr; // [K2][HID]
+    int steps = 0;
+    int grad_count = 0;
+    // activations for one graph (grown on demand)
+    uint32_t cap_n = 0, cap_hub = 0;
+    bf16 *x = nullptr, *ax = nullptr, *h1 = nullptr, *ah1 = nullptr, *h2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
+    float *s = nullptr, *dS = nullptr, *dX2 = nullptr, *dH1 = nullptr, *dA1 = nullptr, *dX1 = nullptr, *loss = nullptr;
+    float* hub_acc = nullptr;
+    uint32_t* gm = nullptr;
+    uint8_t* y = nullptr;
+    int32_t* grp = nullptr;
+    uint32_t cap_grp = 0;
+    // hub rows of the last graph
+    uint64_t hub_uid = 0;
+    uint32_t* sel_scratch = nullptr;   // top-k / subgraph scratch (u32)
+    uint64_t cap_scratch = 0;
+    uint32_t* hub_list = nullptr;   // hub row ids
+    uint32_t* hub_items = nullptr;  // pairs (row, h | chunk << 20)
+    uint32_t n_hub = 0, n_hub_items = 0;
+};
+
+namespace {
+
+gst_status model_bufs(gst_model* M, const gst_graph* G) {
+    gst_ctx* ctx = M->ctx;
+    const uint32_t n = (uint32_t)G->n;
+    if (M->cap_n < n) {
+        bool ok = dalloc(M->x, (size_t)n * FIN) && dalloc(M->ax, (size_t)n * FIN) && dalloc(M->h1, (size_t)n * HID) &&
+                  dalloc(M->ah1, (size_t)n * HID) && dalloc(M->h2, (size_t)n * HID) && dalloc(M->dZ2, (size_t)n * HID) &&
+                  dalloc(M->dZ1, (size_t)n * HID) && dalloc(M->s, n) && dalloc(M->dS, n) &&
+                  dalloc(M->dX2, (size_t)n * K2) && dalloc(M->dH1, (size_t)n * HID) && dalloc(M->dA1, (size_t)n * HID) &&
+                  dalloc(M->gm, n) && dalloc(M->y, n);
+        if (!ok) { M->cap_n = 0; return gst_set_error(ctx, GST_OOM, "model: activation allocation failed (n=%u)", n); }
+        M->cap_n = n;
+    }
+    if (M->hub_uid != G->uid) {
+        // hub rows (host side, from the graph's host CSR)
+        std::vector<uint32_t> hl, hi;
+        for (uint32_t v = 0; v < n; ++v) {
+            const int64_t d = G->h_row[v + 1] - G->h_row[v];
+            if (d > AGG_HUB) {
+                const uint32_t h = (uint32_t)hl.size();
+                hl.push_back(v);
+                for (int64_t c = 0; c * AGG_HUB < d; ++c) { hi.push_back(v); hi.push_back(h | ((uint32_t)c << 20)); }
+            }
+        }
+        if (hl.size() >= (1u << 20)) return gst_set_error(ctx, GST_INVALID_ARG, "model: too many hub rows");
+        if (!dalloc(M->hub_list, hl.size()) || !dalloc(M->hub_items, hi.size()) ||
+            !dalloc(M->hub_acc, std::max<size_t>(hl.size(), 1) * HID))
+            return gst_set_error(ctx, GST_OOM, "model: hub allocation failed");
+        hipStream_t st = (hipStream_t)ctx->stream;
+        if (!hl.empty()) {
+            HIPCHK(hipMemcpyAsync(M->hub_list, hl.data(), hl.size() * 4, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(M->hub_items, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        M->n_hub = (uint32_t)hl.size();
+        M->n_hub_items = (uint32_t)(hi.size() / 2);
+        M->hub_uid = G->uid;
+    }
+    return GST_OK;
+}
+
+gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* groups) {
+    gst_ctx* ctx = M->ctx;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    const int g = groups->n_groups;
+    const uint32_t cnt = (uint32_t)(g + 1 + groups->grp_ptr[g]);
+    if (M->cap_grp < cnt) {
+        if (!dalloc(M->grp, cnt)) return gst_set_error(ctx, GST_OOM, "model: group allocation failed");
+        M->cap_grp = cnt;
+    }
+    std::vector<int32_t> h(cnt);
+    for (int i = 0; i <= g; ++i) h[(size_t)i] = groups->grp_ptr[i];
+    for (int32_t j = 0; j < groups->grp_ptr[g]; ++j) h[(size_t)(g + 1 + j)] = groups->grp_vtx[j];
+    HIPCHK(hipMemcpyAsync(M->grp, h.data(), cnt * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));   // h is a local buffer
+    const int fg = std::max(1, ctx->cu_count * 8);
+    k_fill32<<<gridn(G->n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)G->n);
+    k_set_bits<<<64, 256, 0, st>>>(M->gm, M->grp, M->grp + g + 1, g);   // features use bits 0..7
+    return GST_OK;
+}
+
+template <int D, bool BWD>
+void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipStream_t st) {
+    const int fg = std::max(1, M->ctx->cu_count * 8);
+    if (M->n_hub) k_fill32<<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>((uint32_t*)M->hub_acc, 0u,
+                                                                               (uint64_t)M->n_hub * D);
+    k_aggregate<D, BWD><<<std::max(1, M->ctx->cu_count * 16), 256, 0, st>>>(G->d_row, G->d_col, (uint32_t)G->n, in, out,
+                                                                          M->hub_acc, M->hub_items, M->n_hub_items);
+    if (M->n_hub)
+        k_aggregate_hub_finish<D, BWD><<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>(G->d_row, M->hub_list,
+                                                                                        M->n_hub, M->hub_acc, out);
+}
+
+gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups) {
+    gst_ctx* ctx = M->ctx;
+    gst_status sst = model_bufs(M, G);
+    if (sst != GST_OK) return sst;
+    sst = upload_groups(M, G, groups);
+    if (sst != GST_OK) return sst;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    const uint32_t n = (uint32_t)G->n;
+    const int fg = std::max(1, ctx->cu_count * 8);
+    const int dg = std::max(1, ctx->cu_count * 8);
+    k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, M->gm, n, M->x);
+    aggregate<FIN, false>(M, G, M->x, M->ax, st);
+    k_dense_fwd<FIN><<<dg, 256, 0, st>>>(M->x, M->ax, M->W1t, M->p + OFF_B1, n, M->h1);
+    aggregate<HID, false>(M, G, M->h1, M->ah1, st);
+    k_dense_fwd<HID><<<dg, 256, 0, st>>>(M->h1, M->ah1, M->W2t, M->p + OFF_B2, n, M->h2);
+    k_head<<<gridn(n, 256, fg), 256, 0, st>>>(M->h2, M->p + OFF_W3, M->p + OFF_B3, n, M->s);
+    HIPCHK(hipGetLastError());
+    return GST_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+int64_t gst_model_num_params(void) { return NPARAM; }
+
+gst_status gst_model_create(gst_ctx* ctx, uint64_t seed, gst_model** out) {
+    if (!ctx || !out) return GST_INVALID_ARG;
+    *out = nullptr;
+    if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
+    HIPCHK(hipSetDevice(ctx->device));
+    gst_model* M = new gst_model();
+    M->ctx = ctx;
+    bool ok = dalloc(M->p, NPARAM) && dalloc(M->g, NPARAM) && dalloc(M->m, NPARAM) && dalloc(M->v, NPARAM) &&
+              dalloc(M->W1t, HID * K1) && dalloc(M->W2t, HID * K2) && dalloc(M->W1kn, HID * K1) &&
+              dalloc(M->W2kn, HID * K2) && dalloc(M->loss, 1);
+    if (!ok) { gst_model_free(M); return gst_set_error(ctx, GST_OOM, "model: parameter allocation failed"); }
+    // He-style init from the library's own generator (splitmix64 -> uniform)
+    std::vector<float> h(NPARAM, 0.f);
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto unif = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    for (int i = 0; i < HID * K1; ++i) h[OFF_W1 + i] = (float)(unif() * std::sqrt(6.0 / 20.0));
+    for (int i = 0; i < HID * K2; ++i) h[OFF_W2 + i] = (float)(unif() * std::sqrt(6.0 / K2));
+    for (int i = 0; i < HID; ++i) h[OFF_W3 + i] = (float)(unif() * std::sqrt(6.0 / HID));
+    gst_status s = gst_model_set_params(M, h.data());
+    if (s != GST_OK) { gst_model_free(M); return s; }
+    *out = M;
+    return GST_OK;
+}
+
+void gst_model_free(gst_model* M) {
+    if (!M) return;
+    hipFree(M->p); hipFree(M->g); hipFree(M->m); hipFree(M->v);
+    hipFree(M->W1t); hipFree(M->W2t); hipFree(M->W1kn); hipFree(M->W2kn); hipFree(M->loss);
+    hipFree(M->x); hipFree(M->ax); hipFree(M->h1); hipFree(M->ah1); hipFree(M->h2); hipFree(M->dZ2); hipFree(M->dZ1);
+    hipFree(M->s); hipFree(M->dS); hipFree(M->dX2); hipFree(M->dH1); hipFree(M->dA1); hipFree(M->hub_acc);
+    hipFree(M->gm); hipFree(M->y); hipFree(M->grp); hipFree(M->hub_list); hipFree(M->hub_items);
+    hipFree(M->sel_scratch);
+    delete M;
+}
+
+gst_status gst_model_set_params(gst_model* M, const float* in) {
+    if (!M || !in) return GST_INVALID_ARG;
+    gst_ctx* ctx = M->ctx;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    HIPCHK(hipMemcpyAsync(M->p, in, NPARAM * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(M->g, 0, NPARAM * 4, st));
+    HIPCHK(hipMemsetAsync(M->m, 0, NPARAM * 4, st));
+    HIPCHK(hipMemsetAsync(M->v, 0, NPARAM * 4, st));
+    k_refresh_bf16<<<64, 256, 0, st>>>(M->p, M->W1t, M->W2t, M->W1kn, M->W2kn);
+    HIPCHK(hipStreamSynchronize(st));
+    M->steps = 0;
+    M->grad_count = 0;
+    return GST_OK;
+}
+
+gst_status gst_model_get_params(gst_model* M, float* out) {
+    if (!M || !out) return GST_INVALID_ARG;
+    gst_ctx* ctx = M->ctx;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    HIPCHK(hipMemcpyAsync(out, M->p, NPARAM * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return GST_OK;
+}
+
+gst_status gst_model_scores(gst_model* M, const gst_graph* G, const gst_groups* groups, float* scores) {
+    if (!M || !G || !groups || !scores) return GST_INVALID_ARG;
+    gst_ctx* ctx = M->ctx;
+    gst_status s = forward(M, G, groups);
+    if (s != GST_OK) return s;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    HIPCHK(hipMemcpyAsync(scores, M->s, (size_t)G->n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return GST_OK;
+}
+
+gst_status gst_model_loss_grad(gst_model* M, const gst_graph* G, const gst_groups* groups, const uint8_t* labels,
+                               float pos_weight, float* loss_out, float* grad_out) {
+    if (!M || !G || !groups || !labels) return GST_INVALID_ARG;
+    gst_ctx* ctx = M->ctx;
+    gst_status sst = forward(M, G, groups);
+    if (sst != GST_OK) return sst;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    const uint32_t n = (uint32_t)G->n;
+    const int fg = std::max(1, ctx->cu_count * 8);
+    const int dg = std::max(1, ctx->cu_count * 8);
+    HIPCHK(hipMemcpyAsync(M->y, labels, n, hipMemcpyHostToDevice, st));
+    k_fill32<<<1, 64, 0, st>>>((uint32_t*)M->loss, 0u, 1);
+    if (grad_out) k_fill32<<<gridn(NPARAM, 256, fg), 256, 0, st>>>((uint32_t*)M->g, 0u, NPARAM);
+    k_loss<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, M->y, n, pos_weight, M->dS, M->loss);
+    k_head_bwd<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dS, M->h2, M->p + OFF_W3, n, M->dZ2,
+                                                                   M->g + OFF_W3, M->g + OFF_B3);
+    k_bias_grad<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dZ2, n, M->g + OFF_B2);
+    k_dense_bwd_w<HID><<<std::max(1, ctx->cu_count * 2), 256, 0, st>>>(M->dZ2, M->h1, M->ah1, n, M->g + OFF_W2);
+    k_dense_bwd_x<K2><<<dg, 256, 0, st>>>(M->dZ2, M->W2kn, n, M->dX2);
+    k_split_self<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dX2, n, M->dH1, M->dA1);
+    aggregate<HID, true>(M, G, M->dA1, M->dH1, st);
+    k_relu_mask_bias<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dH1, M->h1, n, M->dZ1, M->g + OFF_B1);
+    k_dense_bwd_w<FIN><<<std::max(1, ctx->cu_count * 2), 256, 0, st>>>(M->dZ1, M->x, M->ax, n, M->g + OFF_W1);
+    HIPCHK(hipGetLastError());
+    float l = 0;
+    HIPCHK(hipMemcpyAsync(&l, M->loss, 4, hipMemcpyDeviceToHost, st));
+    if (grad_out) HIPCHK(hipMemcpyAsync(grad_out, M->g, NPARAM * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (loss_out) *loss_out = l;
+    M->grad_count++;
+    return GST_OK;
+}
+
+gst_status gst_model_adam_step(gst_model* M, float lr) {
+    if (!M) return GST_INVALID_ARG;
+    gst_ctx* ctx = M->ctx;
+    if (M->grad_count == 0) return GST_OK;
+    hipStream_t st = (hipStream_t)ctx->stream;
+    M->steps++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = 1.f - std::pow(b1, (float)M->steps), bc2 = 1.f - std::pow(b2, (float)M->steps);
+    k_adam<<<gridn(NPARAM, 256, 64), 256, 0, st>>>(M->p, M->g, M->m, M->v, NPARAM, lr, b1, b2, 1e-8f, bc1, bc2,
+                                                   1.f / (float)M->grad_count, M->W1t, M->W2t, M->W1kn, M->W2kn);
+    k_refresh_bf16<<<64, 256, 0, st>>>(M->p, M->W1t, M->W2t, M->W1kn, M->W2kn);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    M->grad_count = 0;
+    return GST_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// learned-bound solve
+// ---------------------------------------------------------------------------------------
+gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* groups, gst_model* M, double frac,
+                             const gst_opts* opts_in, gst_result* res, gst_learned_stats* ls) {
+    hipStream_t st = (hipStream_t)ctx->stream;
+    const uint32_t n = (uint32_t)G->n;
+    const int fg = std::max(1, ctx->cu_count * 8);
+    gst_learned_stats L{};
+    hipEvent_t e0, e1, e2, e3;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2)); HIPCHK(hipEventCreate(&e3));
+    HIPCHK(hipEventRecord(e0, st));
+    gst_status sst = forward(M, G, groups);
+    if (sst != GST_OK) return sst;
+    // ---- top-k threshold by a two-pass 16-bit radix select over order-preserving keys ----
+    uint32_t k_target = (uint32_t)std::min<double>(n, std::ceil(frac * n));
+    if (k_target < 1) k_target = 1;
+    static thread_local std::vector<uint32_t> hh;
+    hh.assign(65536, 0);
+    // scratch: histogram (65536) | sel (n) | newid (n) | deg (n) | arc offsets (n) | block sums
+    const uint32_t nbk = (n + 1023) / 1024;
+    const uint64_t need = 65536ull + 4ull * n + 2ull * (nbk + 2) + 64;
+    if (M->cap_scratch < need) {
+        if (!dalloc(M->sel_scratch, need)) { M->cap_scratch = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
+        M->cap_scratch = need;
+    }
+    uint32_t* d_hist = M->sel_scratch;
+    uint32_t* d_sel = d_hist + 65536;
+    uint32_t* d_newid = d_sel + n;
+    uint32_t* d_deg = d_newid + n;
+    uint32_t* d_arcoff = d_deg + n;
+    uint32_t* d_bsum = d_arcoff + n;     // 2 * (nbk + 2)
+    k_fill32<<<64, 256, 0, st>>>(d_hist, 0u, 65536);
+    k_key_hist<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, n, 0, 0, d_hist);
+    HIPCHK(hipMemcpyAsync(hh.data(), d_hist, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint32_t acc = 0, hi = 0;
+    for (int b = 65535; b >= 0; --b) {
+        if (acc + hh[(size_t)b] >= k_target) { hi = (uint32_t)b; break; }
+        acc += hh[(size_t)b];
+    }
+    const uint32_t need_hi = k_target - acc;   // keys with high bits == hi still needed
+    k_fill32<<<64, 256, 0, st>>>(d_hist, 0u, 65536);
+    k_key_hist<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, n, 1, hi, d_hist);
+    HIPCHK(hipMemcpyAsync(hh.data(), d_hist, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint32_t acc2 = 0, lo = 0;
+    for (int b = 65535; b >= 0; --b) {
+        if (acc2 + hh[(size_t)b] >= need_hi) { lo = (uint32_t)b; break; }
+        acc2 += hh[(size_t)b];
+    }
+    const uint32_t thr = (hi << 16) | lo;
+    const uint32_t ties = need_hi - acc2;
+    int* d_tie = (int*)d_hist;   // the histogram is no longer needed
+    const int ties_i = (int)ties;
+    HIPCHK(hipMemcpyAsync(d_tie, &ties_i, 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));   // ties_i lives on this stack frame
+    k_select<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, M->gm, n, thr, d_tie, d_sel);
+    // ---- induced subgraph ----
+    const uint32_t nb = nbk;
+    k_scan_blocks<<<nb, 256, 0, st>>>(d_sel, n, d_newid, d_bsum);
+    k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nb);
+    k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_newid, n, d_bsum);
+    k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, n, d_deg);
+    k_scan_blocks<<<nb, 256, 0, st>>>(d_deg, n, d_arcoff, d_bsum + nb + 2);
+    k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nb + 2, nb);
+    k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_arcoff, n, d_bsum + nb + 2);
+    uint32_t tots[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&tots[0], d_bsum + nb, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&tots[1], d_bsum + nb + 2 + nb, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint32_t ns = tots[0], ms = tots[1];
+    L.sub_vertices = ns;
+    L.sub_arcs = ms;
+    gst_graph* S = new gst_graph();
+    S->ctx = ctx; S->uid = gst_next_graph_uid(); S->n = (int32_t)ns;
+    S->budget_key = G->uid | (1ull << 63);   // learned subgraphs of one graph share round budgets S->m = ms; S->w_type = G->w_type;
+    uint32_t* d_old = nullptr;
+    bool ok = hipMalloc(&S->d_row, (ns + 1) * 4ull) == hipSuccess && hipMalloc(&S->d_col, std::max(ms, 1u) * 4ull) == hipSuccess &&
+              hipMalloc(&S->d_w, std::max(ms, 1u) * 4ull) == hipSuccess && hipMalloc(&S->d_rev, std::max(ms, 1u) * 4ull) == hipSuccess &&
+              hipMalloc(&d_old, std::max(ns, 1u) * 4ull) == hipSuccess;
+    if (!ok) { gst_graph_device_free(S); delete S; hipFree(d_old); return gst_set_error(ctx, GST_OOM, "learned solve: subgraph allocation failed"); }
+    k_sub_fill<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, n,
+                                                             S->d_row, S->d_col, S->d_w, d_old);
+    HIPCHK(hipMemcpyAsync(S->d_row + ns, &ms, 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));   // ms lives on this stack frame
+    k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S->d_row, S->d_col, ns, S->d_rev);
+    // host row offsets of the subgraph (the solver reads degrees for its work-item count)
+    S->h_row.resize((size_t)ns + 1);
+    {
+        std::vector<uint32_t> r32((size_t)ns + 1);
+        HIPCHK(hipMemcpyAsync(r32.data(), S->d_row, (ns + 1) * 4ull, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t i = 0; i <= ns; ++i) S->h_row[i] = r32[i];
+    }
+    // groups in subgraph ids
+    const int g = groups->n_groups;
+    const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
+    std::vector<int32_t> sub_vtx(gcnt);
+    {
+        int32_t* d_tmp = nullptr;
+        if (hipMalloc(&d_tmp, std::max(gcnt, 1u) * 4ull) != hipSuccess) { gst_graph_device_free(S); delete S; hipFree(d_old); return gst_set_error(ctx, GST_OOM, "learned solve: allocation failed"); }
+        k_sub_groups<<<gridn(gcnt, 256, 64), 256, 0, st>>>(M->grp + g + 1, gcnt, d_newid, d_tmp);
+        HIPCHK(hipMemcpyAsync(sub_vtx.data(), d_tmp, gcnt * 4ull, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        hipFree(d_tmp);
+    }
+    HIPCHK(hipEventRecord(e1, st));
+    gst_groups sg{g, groups->grp_ptr, sub_vtx.data()};
+    gst_opts so{};
+    so.prune_lb = 1;
+    so.want_tree = 1;
+    gst_result sr;
+    std::memset(&sr, 0, sizeof(sr));
+    double ub = 0.0;
+    bool have_ub = false;
+    if (ns > 0) {
+        gst_status ss = gst_dpbf_solve(ctx, S, &sg, &so, &sr);
+        if (ss == GST_OK) { ub = sr.cost; have_ub = true; }
+        gst_result_free(&sr);
+    }
+    HIPCHK(hipEventRecord(e2, st));
+    gst_graph_device_free(S);
+    delete S;
+    hipFree(d_old);
+    // ---- full solve, bounded by the subgraph tree when there is one ----
+    gst_opts fo{};
+    if (opts_in) fo = *opts_in;
+    else { fo.prune_lb = 1; fo.want_tree = 1; }
+    if (have_ub) fo.ub = ub;
+    gst_status fs = gst_dpbf_solve(ctx, G, groups, &fo, res);
+    if (fs == GST_INFEASIBLE && have_ub) {
+        // the bound cannot be below the optimum; reaching here means float rounding in the bound:
+        // solve without it
+        fo.ub = 0;
+        fs = gst_dpbf_solve(ctx, G, groups, &fo, res);
+        if (fs == GST_OK) res->fell_back = 1;
+    } else if (fs == GST_OK && !have_ub) {
+        res->fell_back = 1;
+    }
+    HIPCHK(hipEventRecord(e3, st));
+    HIPCHK(hipEventSynchronize(e3));
+    float a = 0, b = 0, c = 0;
+    hipEventElapsedTime(&a, e0, e1);
+    hipEventElapsedTime(&b, e1, e2);
+    hipEventElapsedTime(&c, e2, e3);
+    L.t_score_select_ms = a;
+    L.t_sub_solve_ms = b;
+    L.t_full_solve_ms = c;
+    L.ub = have_ub ? ub : -1.0;
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
+    if (ls) *ls = L;
+    return fs;
+}

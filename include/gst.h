@@ -152,6 +152,42 @@ gst_status gst_validate_tree(const gst_csr* csr, const gst_groups* groups, int32
                              const int32_t* eu, const int32_t* ev, int32_t root,
                              double* cost_out);
 
+/* --- learned upper bound ----------------------------------------------------------------- */
+/* No counterpart in the reference.  A two-layer mean-aggregation message-passing network (hidden
+ * 64, BF16 weights, FP32 accumulation, BF16 MFMA kernels) scores vertices; the DPBF optimum of the
+ * subgraph induced by the top-scored vertices plus every group member is a feasible tree whose
+ * cost bounds the full solve (cost + lower bound >= bound prunes states, as PrunedDP++ does with
+ * its bounds, rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:716-730, 795-957).  The bounded
+ * full solve returns the exact optimum; without a feasible subgraph tree the plain solve runs
+ * (result->fell_back = 1).  Parameters are NPARAM fp32 values (gst_model_num_params). */
+typedef struct gst_model gst_model;
+typedef struct {
+    uint32_t sub_vertices;     /* vertices of the induced subgraph */
+    uint32_t sub_arcs;
+    double   ub;               /* subgraph optimum used as the bound (-1: none) */
+    double   t_score_select_ms;/* network forward + top-k + subgraph build */
+    double   t_sub_solve_ms;   /* DPBF on the subgraph */
+    double   t_full_solve_ms;  /* bounded DPBF on the full graph */
+} gst_learned_stats;
+
+int64_t    gst_model_num_params(void);
+gst_status gst_model_create(gst_ctx* ctx, uint64_t seed, gst_model** out);
+void       gst_model_free(gst_model* m);
+gst_status gst_model_set_params(gst_model* m, const float* params);
+gst_status gst_model_get_params(gst_model* m, float* params);
+/* scores[n] (host) for one query */
+gst_status gst_model_scores(gst_model* m, const gst_graph* g, const gst_groups* groups, float* scores);
+/* forward + backward on one query with labels[n] in {0,1} (1 = vertex of the optimal tree):
+ * weighted logistic loss (weight pos_weight on positives, mean over vertices).  Gradients add
+ * into the model's accumulator; grad (NPARAM floats, may be NULL) receives this query's gradient. */
+gst_status gst_model_loss_grad(gst_model* m, const gst_graph* g, const gst_groups* groups, const uint8_t* labels,
+                               float pos_weight, float* loss, float* grad);
+/* Adam step with the mean of the accumulated gradients, then clears the accumulator */
+gst_status gst_model_adam_step(gst_model* m, float lr);
+/* exact solve with the learned bound; topk_frac = fraction of vertices kept (e.g. 0.05) */
+gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, gst_model* m,
+                             double topk_frac, const gst_opts* opts, gst_result* result, gst_learned_stats* stats);
+
 /* --- synthetic instances (host, deterministic) ------------------------------------------ */
 /* Host CSR owned by the library.  The generators use the library's own seeded RNG
  * (splitmix64 / xoshiro256**), so a seed names the same instance on every machine. */

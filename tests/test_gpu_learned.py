@@ -1,0 +1,125 @@
+"""GPU: the learned-bound network kernels against the CPU reference, and the learned solve's exactness."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpu4gst_code_amd as gst
+import oracle
+from oracle import mpnn
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.3, mpnn.NPARAM).astype(np.float32)
+    p[mpnn.OFF_B1:mpnn.OFF_W2] += 0.2
+    p[mpnn.OFF_B2:mpnn.OFF_W3] += 0.2
+    return p
+
+
+def _graph_with_hub(seed):
+    """BA graph plus one vertex joined to 700 others: exercises the chunked hub aggregation."""
+    G = gst.gen_barabasi_albert(3000, 3, 1, 40, seed=seed)
+    src = np.repeat(np.arange(G.n), np.diff(G.row_ptr))
+    keep = src < G.col
+    eu, ev, w = list(src[keep]), list(G.col[keep]), list(G.w[keep])
+    hub = 0
+    have = set(G.col[G.row_ptr[hub]:G.row_ptr[hub + 1]].tolist())
+    for v in range(1, 701):
+        if v not in have:
+            eu.append(hub); ev.append(v); w.append(7)
+    return gst.HostGraph.from_edges(G.n, eu, ev, np.array(w, np.int32))
+
+
+@pytest.mark.parametrize("with_hub", [False, True])
+def test_scores_and_gradients_match_cpu_reference(solver, with_hub):
+    G = _graph_with_hub(5) if with_hub else gst.gen_barabasi_albert(3000, 3, 1, 40, seed=5)
+    groups = gst.gen_groups(G.n, 6, 3, 20, seed=6)
+    gm = groups.masks(G.n)
+    rng = np.random.default_rng(7)
+    y = (rng.random(G.n) < 0.05).astype(np.uint8)
+    p = _params(8)
+    dg = solver.upload(G)
+    model = gst.Model(solver, params=p)
+    s_gpu = model.scores(dg, groups)
+    g_ref, A = mpnn.grads(p, G.row_ptr, G.col, G.w, gm, y, 10.0, rnd=True)
+    s_ref = A["s"]
+    err = np.max(np.abs(s_gpu - s_ref)) / (np.max(np.abs(s_ref)) + 1e-6)
+    print(f"hub={with_hub}: score max rel err {err:.3e}", flush=True)
+    assert err < 2e-2
+    loss, g_gpu = model.loss_grad(dg, groups, y, 10.0, want_grad=True)
+    g32, _ = mpnn.grads(p, G.row_ptr, G.col, G.w, gm, y, 10.0, rnd=False)
+    for name, a, b in (("W1", mpnn.OFF_W1, mpnn.OFF_B1), ("b1", mpnn.OFF_B1, mpnn.OFF_W2),
+                       ("W2", mpnn.OFF_W2, mpnn.OFF_B2), ("b2", mpnn.OFF_B2, mpnn.OFF_W3),
+                       ("w3", mpnn.OFF_W3, mpnn.OFF_B3), ("b3", mpnn.OFF_B3, mpnn.NPARAM)):
+        gg, gr, gf = g_gpu[a:b], g_ref[a:b], g32[a:b]
+        rel = np.linalg.norm(gg - gr) / (np.linalg.norm(gr) + 1e-12)
+        rel32 = np.linalg.norm(gg - gf) / (np.linalg.norm(gf) + 1e-12)
+        cos = float(np.dot(gg, gf) / (np.linalg.norm(gg) * np.linalg.norm(gf) + 1e-30))
+        print(f"  grad {name}: rel err vs bf16 reference {rel:.2e}, vs fp32 {rel32:.2e}, cosine {cos:.6f}", flush=True)
+        assert rel < 3e-2, name
+        assert cos > 0.99, name
+    model.free()
+    dg.free()
+
+
+def test_training_steps_reduce_loss(solver):
+    G = gst.gen_barabasi_albert(2000, 3, 1, 40, seed=9)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=3)
+    qs = [gst.gen_groups(G.n, 4, 3, 10, seed=20 + i) for i in range(4)]
+    labels = []
+    for q in qs:
+        r = solver.solve(dg, q)
+        y = np.zeros(G.n, np.uint8)
+        y[r.eu] = 1
+        y[r.ev] = 1
+        y[r.root] = 1
+        labels.append(y)
+    first = sum(model.loss_grad(dg, q, y, 20.0)[0] for q, y in zip(qs, labels))
+    model.adam_step(0.0)   # discard the accumulated gradients of the measurement
+    for _ in range(60):
+        for q, y in zip(qs, labels):
+            model.loss_grad(dg, q, y, 20.0)
+        model.adam_step(3e-3)
+    last = sum(model.loss_grad(dg, q, y, 20.0)[0] for q, y in zip(qs, labels))
+    print(f"loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.7 * first
+    model.free()
+    dg.free()
+
+
+@pytest.mark.parametrize("frac", [0.05, 0.2, 0.0005])
+def test_learned_solve_is_exact(solver, frac):
+    """Any network (here untrained) only bounds the search: the cost equals the oracle's."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=1)
+    for q in range(3):
+        groups = gst.gen_groups(G.n, 6, 10, 200, seed=1000 + q)
+        r = solver.solve_learned(dg, groups, model, frac)
+        want = GOLD["ba20k"]["costs"][str(q)]
+        print(f"frac {frac} q{q}: cost {r.cost} want {want} ub {r.stats['ub']} sub {r.stats['sub_vertices']} "
+              f"fell_back {r.fell_back}", flush=True)
+        assert r.cost == want
+        assert gst.validate_tree(G, groups, r.eu, r.ev, r.root) == want
+        if r.stats["ub"] > 0:
+            assert r.stats["ub"] >= want
+    model.free()
+    dg.free()
+
+
+def test_learned_solve_float_weights(solver):
+    G = gst.gen_rmat(12, 8, weights="lognormal", seed=21)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=2)
+    for q, want in GOLD["rmat_f"]["costs"].items():
+        groups = gst.gen_groups(G.n, 5, 5, 50, seed=3000 + int(q))
+        r = solver.solve_learned(dg, groups, model, 0.1)
+        assert abs(r.cost - want) <= 1e-6 * want
+    model.free()
+    dg.free()
