@@ -188,25 +188,34 @@ gst_status gst_graph_upload(gst_ctx* ctx, const gst_csr* c, gst_graph** out) {
     g->n = c->n;
     g->w_type = c->w_type;
     const uint32_t* w = (const uint32_t*)c->w;
-    // sorted, deduplicated rows without self-loops
+    // deduplicated rows without self-loops (the lightest parallel arc is kept), each row in
+    // increasing (weight, neighbour) order: a relaxation that walks a row can stop at the first
+    // arc whose weight alone already rules it out.  -0.0f weights are stored as +0.0f so that
+    // weight bit patterns order like the weights.
     g->h_row.assign((size_t)c->n + 1, 0);
     g->h_col.reserve((size_t)c->m);
     g->h_w.reserve((size_t)c->m);
     std::vector<std::pair<int32_t, uint32_t>> tmp;
+    std::vector<uint64_t> keyed;
     for (int32_t v = 0; v < c->n; ++v) {
         tmp.clear();
         for (int64_t e = c->row_ptr[v]; e < c->row_ptr[v + 1]; ++e)
-            if (c->col[e] != v) tmp.push_back({c->col[e], w[e]});
+            if (c->col[e] != v) tmp.push_back({c->col[e], w[e] == 0x80000000u && c->w_type == GST_W_FLOAT32 ? 0u : w[e]});
         std::sort(tmp.begin(), tmp.end(), [&](const std::pair<int32_t, uint32_t>& a, const std::pair<int32_t, uint32_t>& b) {
             if (a.first != b.first) return a.first < b.first;
             return w_less(a.second, b.second, c->w_type);
         });
+        keyed.clear();
         int32_t last = -1;
         for (auto& p : tmp) {
             if (p.first == last) continue;  // parallel arc: the lightest one was kept
-            g->h_col.push_back(p.first);
-            g->h_w.push_back(p.second);
+            keyed.push_back(arc_key(p.second, p.first));
             last = p.first;
+        }
+        std::sort(keyed.begin(), keyed.end());
+        for (uint64_t k : keyed) {
+            g->h_col.push_back((int32_t)(uint32_t)k);
+            g->h_w.push_back((uint32_t)(k >> 32));
         }
         g->h_row[(size_t)v + 1] = (int64_t)g->h_col.size();
         g->max_deg = std::max<int32_t>(g->max_deg, (int32_t)(g->h_row[(size_t)v + 1] - g->h_row[(size_t)v]));
@@ -216,10 +225,7 @@ gst_status gst_graph_upload(gst_ctx* ctx, const gst_csr* c, gst_graph** out) {
     for (int32_t v = 0; v < g->n; ++v) {
         for (int64_t e = g->h_row[(size_t)v]; e < g->h_row[(size_t)v + 1]; ++e) {
             const int32_t u = g->h_col[(size_t)e];
-            const auto b = g->h_col.begin() + g->h_row[(size_t)u];
-            const auto en = g->h_col.begin() + g->h_row[(size_t)u + 1];
-            const auto it = std::lower_bound(b, en, v);
-            if (it == en || *it != v || g->h_w[(size_t)(it - g->h_col.begin())] != g->h_w[(size_t)e]) {
+            if (gst_find_arc(g->h_row.data(), g->h_col.data(), g->h_w.data(), u, v, g->h_w[(size_t)e]) < 0) {
                 delete g;
                 return gst_set_error(ctx, GST_INVALID_ARG,
                                      "csr: arc %d->%d has no reverse arc of the same weight (graph must be undirected)", v, u);

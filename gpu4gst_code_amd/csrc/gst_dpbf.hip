@@ -44,8 +44,13 @@ constexpr int NCTL = GST_MAX_GROUPS + 2;     // one round-control block per leve
 
 // device-side control block of the grow rounds of one level
 struct RoundCtl {
-    uint32_t round;
-    uint32_t blocks_done;
+    union {
+        struct {
+            uint32_t round;
+            uint32_t seq;   // launch number of the last finished round
+        };
+        unsigned long long rsq;   // (seq << 32) | round, read and written as one word
+    };
     uint32_t cnt[3];        // items in list r%3
     uint32_t level_done;
     uint32_t err;           // bit0: int cost overflow, bit1: traceback failure
@@ -84,7 +89,13 @@ struct GrowArgs {
     unsigned long long* ring;   // 4 words per task slot
     uint32_t ring_mask;         // slots - 1 (power of two)
     struct AsyncCtl* actl;
+    uint32_t* rlog;             // optional round log (RLOG_ROUNDS x 4 words), nullptr: off
+    uint32_t seq;               // launch number within the level (1, 2, ...), set per launch
 };
+
+// round log (GST_TRACE_ROUNDS=1): per round {queued items | pull<<31, workgroups taking part,
+// wall clock at start (block 0), wall clock when the last workgroup finished}
+constexpr uint32_t RLOG_ROUNDS = 512;
 
 // device-side control of an asynchronous grow launch
 struct AsyncCtl {
@@ -144,6 +155,15 @@ __device__ __forceinline__ bool reaches_ub(uint32_t a, uint32_t b, uint32_t ub) 
     if (a == GST_INF_COST || b == GST_INF_COST) return true;
     if (FLT) return __fadd_rn(__uint_as_float(a), __uint_as_float(b)) >= __uint_as_float(ub);
     return (uint64_t)a + (uint64_t)b >= (uint64_t)ub;
+}
+
+// Pull: can an arc of weight w still lower (or tie with a smaller predecessor) a state of cost
+// ocost and lower bound lb under the bound ub?  Every candidate through the arc costs at least w,
+// so a false answer holds for every heavier arc of the row as well.
+template <bool FLT>
+__device__ __forceinline__ bool pull_arc_live(uint32_t w, uint32_t ocost, uint32_t lb, uint32_t ub) {
+    if (w > ocost) return false;
+    return ub == GST_INF_COST || !reaches_ub<FLT>(w, lb, ub);
 }
 
 // position of the t-th (0-based) set bit of x; t < popcount(x)
@@ -287,7 +307,7 @@ __global__ void k_set_group_bits(uint32_t* gm, const int32_t* grp_ptr, const int
 
 __global__ void k_ctl_reset(RoundCtl* ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ctl->round = 0; ctl->blocks_done = 0;
+        ctl->round = 0; ctl->seq = 0;
         ctl->cnt[0] = 0; ctl->cnt[1] = 0; ctl->cnt[2] = 0;
         ctl->level_done = 0;
     }
@@ -351,7 +371,12 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
     __shared__ uint32_t s_ovf;
     RoundCtl* ctl = a.ctl;
     if (ctl->level_done) return;
-    const uint32_t r = ctl->round;
+    // (round, seq) is one 8-byte word: a workgroup that starts only after this launch's round has
+    // ended (its participants all arrived and advanced the round) sees seq == a.seq and stays out
+    const unsigned long long rsq =
+        __hip_atomic_load(&ctl->rsq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(rsq >> 32) == a.seq) return;
+    const uint32_t r = (uint32_t)rsq;
     const uint32_t cur = r % 3, nxt = (r + 1) % 3, prv = (r + 2) % 3;
     const uint32_t n_cur = ctl->cnt[cur];
     const uint32_t n_prv = ctl->cnt[prv];
@@ -364,6 +389,11 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
         const uint32_t need_b = (n_cur + GROW_WAVES - 1) / GROW_WAVES;
         const uint32_t need_a = (n_prv + GROW_BLOCK - 1) / GROW_BLOCK;
         part = max(1u, min(gridDim.x, max(need_b, need_a)));
+    }
+    if (a.rlog && blockIdx.x == 0 && threadIdx.x == 0 && r < RLOG_ROUNDS) {
+        a.rlog[4 * r + 0] = n_cur | (pull ? 0x80000000u : 0u);
+        a.rlog[4 * r + 1] = part;
+        a.rlog[4 * r + 2] = (uint32_t)wall_clock64();
     }
     if (blockIdx.x >= part) return;
     bq_init(q);
@@ -429,13 +459,18 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
                 unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;
                 for (uint32_t s = 0; s < maxdeg; s += UNR) {
                     uint32_t nb[UNR], wt[UNR], cn[UNR];
+                    bool live[UNR];
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j) {
-                        const uint32_t e = rs + s + j;
                         const bool ok = lane_ok && s + j < deg;
-                        nb[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
-                        wt[j] = ok ? a.w[e] : 0u;
+                        wt[j] = ok ? a.w[rs + s + j] : 0u;
+                        live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
                     }
+                    // rows run in increasing weight: when the step's first arc is dead for every
+                    // lane, the rest of every row is dead too
+                    if (__ballot(live[0]) == 0) break;
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) nb[j] = live[j] ? (uint32_t)a.col[rs + s + j] : 0xFFFFFFFFu;
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j)
                         cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
@@ -504,13 +539,18 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             const uint32_t steps = (len + A - 1) / A;
             for (uint32_t s = 0; s < steps; s += UNR) {
                 uint32_t nb[UNR], wt[UNR], cn[UNR];
+                bool live[UNR];
 #pragma unroll
                 for (uint32_t j = 0; j < UNR; ++j) {
                     const uint32_t e = e0 + (s + j) * A + s_arc;
                     const bool ok = lane_ok && e < e1;
-                    nb[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
                     wt[j] = ok ? a.w[e] : 0u;
+                    live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
                 }
+                if (__ballot(live[0]) == 0) break;   // each lane's arcs run in increasing weight
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j)
+                    nb[j] = live[j] ? (uint32_t)a.col[e0 + (s + j) * A + s_arc] : 0xFFFFFFFFu;
 #pragma unroll
                 for (uint32_t j = 0; j < UNR; ++j)
                     cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
@@ -609,13 +649,21 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             for (uint32_t s = 0; s < steps; s += UNR) {
                 uint32_t uu[UNR], wt[UNR], ra[UNR];
                 unsigned long long cw0[UNR];
+                bool live[UNR];
 #pragma unroll
                 for (uint32_t j = 0; j < UNR; ++j) {
                     const uint32_t e = e0 + (s + j) * A + s_arc;
                     const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-                    uu[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
                     wt[j] = ok ? a.w[e] : 0u;
-                    ra[j] = ok ? a.rev[e] : 0u;
+                    // under a bound, cv + w >= ub holds for every heavier arc of the row as well
+                    live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
+                }
+                if (__ballot(live[0]) == 0) break;
+#pragma unroll
+                for (uint32_t j = 0; j < UNR; ++j) {
+                    const uint32_t e = e0 + (s + j) * A + s_arc;
+                    uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+                    ra[j] = live[j] ? a.rev[e] : 0u;
                 }
 #pragma unroll
                 for (uint32_t j = 0; j < UNR; ++j)
@@ -704,11 +752,15 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
             const uint32_t ngrp = (part + 31) >> 5;
             if (atomicAdd(&ctl->top, 1u) == ngrp - 1) {
                 ctl->top = 0;
+                if (a.rlog && r < RLOG_ROUNDS) a.rlog[4 * r + 3] = (uint32_t)wall_clock64();
                 if (n_cur == 0) {
                     ctl->level_done = 1;
+                    __hip_atomic_store(&ctl->rsq, ((unsigned long long)a.seq << 32) | r,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
                     ctl->cnt[prv] = 0;
-                    ctl->round = r + 1;
+                    __hip_atomic_store(&ctl->rsq, ((unsigned long long)a.seq << 32) | (r + 1),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
         }
@@ -792,13 +844,20 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
     for (uint32_t s = 0; s < steps; s += UNR) {
         uint32_t uu[UNR], wt[UNR], ra[UNR];
         unsigned long long cw0[UNR];
+        bool live[UNR];
 #pragma unroll
         for (uint32_t j = 0; j < UNR; ++j) {
             const uint32_t e = e0 + (s + j) * A + s_arc;
             const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-            uu[j] = ok ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
             wt[j] = ok ? a.w[e] : 0u;
-            ra[j] = ok ? a.rev[e] : 0u;
+            live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
+        }
+        if (__ballot(live[0]) == 0) break;   // arcs run in increasing weight
+#pragma unroll
+        for (uint32_t j = 0; j < UNR; ++j) {
+            const uint32_t e = e0 + (s + j) * A + s_arc;
+            uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+            ra[j] = live[j] ? a.rev[e] : 0u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < UNR; ++j)
@@ -1322,6 +1381,7 @@ struct gst_query_bufs {
     uint32_t all_chunk = 0;
     RoundCtl* h_ctl = nullptr;   // pinned
     std::vector<hipEvent_t> evs;
+    uint32_t* rlog = nullptr;    // round log of every level (GST_TRACE_ROUNDS), NCTL * RLOG_ROUNDS * 4 words
 };
 
 static void free_bufs(gst_query_bufs* b) {
@@ -1331,7 +1391,7 @@ static void free_bufs(gst_query_bufs* b) {
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
     hipFree(b->all_items); hipFree(b->all_cnt);
-    hipFree(b->ring); hipFree(b->actl);
+    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -1375,13 +1435,8 @@ gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {
     for (size_t v = 0; v < n; ++v) {
         for (int64_t e = g->h_row[v]; e < g->h_row[v + 1]; ++e) {
             const int32_t u = g->h_col[(size_t)e];
-            // binary search v in row u
-            int64_t lo = g->h_row[(size_t)u], hi = g->h_row[(size_t)u + 1];
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) / 2;
-                if (g->h_col[(size_t)mid] < (int32_t)v) lo = mid + 1; else hi = mid;
-            }
-            rev[(size_t)e] = (uint32_t)lo;
+            rev[(size_t)e] = (uint32_t)gst_find_arc(g->h_row.data(), g->h_col.data(), g->h_w.data(), u, (int32_t)v,
+                                                    g->h_w[(size_t)e]);
         }
     }
     const size_t mm = std::max<size_t>(m, 1);
@@ -1616,6 +1671,12 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     const int async_grid = std::max(1, ctx->cu_count * 5);
     int* ctx_level_rounds = B->level_rounds;
 
+    static const bool trace_rounds = std::getenv("GST_TRACE_ROUNDS") != nullptr;
+    if (trace_rounds) {
+        if (!B->rlog && hipMalloc(&B->rlog, sizeof(uint32_t) * NCTL * RLOG_ROUNDS * 4) != hipSuccess)
+            return gst_set_error(ctx, GST_OOM, "round log allocation failed");
+        k_fill_u32<<<64, 256, 0, s>>>(B->rlog, 0xFFFFFFFFu, (uint64_t)NCTL * RLOG_ROUNDS * 4);
+    }
     auto run_grow = [&](int k) -> gst_status {
         GrowArgs ga;
         ga.Tk = B->T + lvl_base[k];
@@ -1636,6 +1697,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
                                     : (uint32_t)std::max<uint64_t>(1, B->n_items_total * (uint64_t)pm / 1000);
         }
         ga.ring = B->ring; ga.ring_mask = (uint32_t)(B->cap_ring - 1); ga.actl = B->actl;
+        ga.rlog = trace_rounds ? B->rlog + (size_t)k * RLOG_ROUNDS * 4 : nullptr;
+        uint32_t launch_seq = 0;   // the level's control block starts with seq 0
+        ga.seq = 0;
         auto launch_async = [&]() {
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
@@ -1661,6 +1725,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
             for (int rr = 0; rr < budget; ++rr) {
+                ga.seq = ++launch_seq;
                 if (flt) k_grow_round<true><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
                 else k_grow_round<false><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
                 grow_launches++;
@@ -1678,6 +1743,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
             for (int rr = 0; rr < batch; ++rr) {
+                ga.seq = ++launch_seq;
                 if (flt) k_grow_round<true><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
                 else k_grow_round<false><<<grow_grid, GROW_BLOCK, 0, s>>>(ga);
                 grow_launches++;
@@ -1898,6 +1964,22 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         res->tree_cost = tc;
     }
 
+    if (trace_rounds) {
+        std::vector<uint32_t> lg((size_t)NCTL * RLOG_ROUNDS * 4);
+        HIPCHK(hipMemcpyAsync(lg.data(), B->rlog, lg.size() * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 1; k < g; ++k) {
+            std::fprintf(stderr, "[rounds] level %d (masks %d):", k, L.m[k]);
+            for (uint32_t r = 0; r < RLOG_ROUNDS; ++r) {
+                const uint32_t* e = &lg[((size_t)k * RLOG_ROUNDS + r) * 4];
+                if (e[0] == 0xFFFFFFFFu) break;
+                // wall clock runs at 100 MHz: 10 ns per tick
+                const double us = e[3] == 0xFFFFFFFFu ? -1.0 : (double)(uint32_t)(e[3] - e[2]) * 0.01;
+                std::fprintf(stderr, " %u%s/%u:%.1f", e[0] & 0x7FFFFFFFu, (e[0] >> 31) ? "P" : "", e[1], us);
+            }
+            std::fprintf(stderr, "\n");
+        }
+    }
     // ---- stats ----
     unsigned long long hs[ST_NSTAT];
     HIPCHK(hipMemcpyAsync(hs, B->stats, sizeof(hs), hipMemcpyDeviceToHost, s));

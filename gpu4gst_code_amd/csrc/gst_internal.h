@@ -58,6 +58,22 @@ struct gst_graph {
     int32_t max_deg = 0;
 };
 
+// Rows of an uploaded graph are ordered by (weight bits, neighbour); weights are non-negative
+// int32 or non-negative float32 (never -0.0f), so the bit patterns order like the weights.
+inline uint64_t arc_key(uint32_t w, int32_t col) { return ((uint64_t)w << 32) | (uint32_t)col; }
+
+// index of arc u->v of weight w in a (weight, neighbour)-ordered row, -1 when absent
+inline int64_t gst_find_arc(const int64_t* row, const int32_t* col, const uint32_t* w, int32_t u, int32_t v,
+                            uint32_t wv) {
+    int64_t lo = row[u], hi = row[u + 1];
+    const uint64_t key = arc_key(wv, v);
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (arc_key(w[mid], col[mid]) < key) lo = mid + 1; else hi = mid;
+    }
+    return (lo < row[u + 1] && col[lo] == v && w[lo] == wv) ? lo : -1;
+}
+
 struct gst_query_bufs;  // device buffers of one query (defined in gst_dpbf.hip)
 
 struct gst_ctx {

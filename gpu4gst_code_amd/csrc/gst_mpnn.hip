@@ -525,7 +525,8 @@ __global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_
     }
 }
 
-// fill the subgraph CSR: new row r' = newid[v]; arcs in neighbour order (ids stay sorted)
+// fill the subgraph CSR: new row r' = newid[v]; arcs keep their order (new ids are monotone in the
+// old ones, so rows stay in (weight, neighbour) order)
 __global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32_t* w, const uint32_t* sel,
                            const uint32_t* newid, const uint32_t* arc_off, uint32_t n, uint32_t* srow, int32_t* scol,
                            uint32_t* sw, uint32_t* old_of) {
@@ -551,17 +552,19 @@ __global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32
     }
 }
 
-__global__ void k_sub_rev(const uint32_t* srow, const int32_t* scol, uint32_t ns, uint32_t* srev) {
+__global__ void k_sub_rev(const uint32_t* srow, const int32_t* scol, const uint32_t* sw, uint32_t ns, uint32_t* srev) {
     const uint32_t lane = __lane_id();
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * blockDim.x / 64;
     for (uint32_t v = gwave; v < ns; v += nwaves) {
         for (uint32_t e = srow[v] + lane; e < srow[v + 1]; e += 64) {
+            // rows are ordered by (weight, neighbour): the reverse arc has the same weight
             const uint32_t u = (uint32_t)scol[e];
+            const uint64_t key = ((uint64_t)sw[e] << 32) | v;
             uint32_t lo = srow[u], hi = srow[u + 1];
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) / 2;
-                if ((uint32_t)scol[mid] < v) lo = mid + 1; else hi = mid;
+                if ((((uint64_t)sw[mid] << 32) | (uint32_t)scol[mid]) < key) lo = mid + 1; else hi = mid;
             }
             srev[e] = lo;
         }
@@ -937,7 +940,9 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
     L.sub_arcs = ms;
     gst_graph* S = new gst_graph();
     S->ctx = ctx; S->uid = gst_next_graph_uid(); S->n = (int32_t)ns;
-    S->budget_key = G->uid | (1ull << 63);   // learned subgraphs of one graph share round budgets S->m = ms; S->w_type = G->w_type;
+    S->budget_key = G->uid | (1ull << 63);   // learned subgraphs of one graph share round budgets
+    S->m = ms;
+    S->w_type = G->w_type;
     uint32_t* d_old = nullptr;
     bool ok = hipMalloc(&S->d_row, (ns + 1) * 4ull) == hipSuccess && hipMalloc(&S->d_col, std::max(ms, 1u) * 4ull) == hipSuccess &&
               hipMalloc(&S->d_w, std::max(ms, 1u) * 4ull) == hipSuccess && hipMalloc(&S->d_rev, std::max(ms, 1u) * 4ull) == hipSuccess &&
@@ -947,7 +952,7 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
                                                              S->d_row, S->d_col, S->d_w, d_old);
     HIPCHK(hipMemcpyAsync(S->d_row + ns, &ms, 4, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));   // ms lives on this stack frame
-    k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S->d_row, S->d_col, ns, S->d_rev);
+    k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S->d_row, S->d_col, S->d_w, ns, S->d_rev);
     // host row offsets of the subgraph (the solver reads degrees for its work-item count)
     S->h_row.resize((size_t)ns + 1);
     {
