@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -288,6 +288,57 @@ class Solver:
                           stats)
         finally:
             self.lib.gst_result_free(C.byref(res))
+
+    def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
+                    prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
+                    use_graph: bool = True, raise_on_error: bool = True):
+        """Solve many queries on one graph with the batched engine (gst_solve_batch).
+
+        Returns (results, stats): results[i] is a Result, or a GstError for a query that failed
+        when raise_on_error is False."""
+        nq = len(queries)
+        grps = (_abi.gst_groups * max(nq, 1))()
+        keep = [q.as_c() for q in queries]
+        for i, c in enumerate(keep):
+            grps[i] = c
+        opts = (_abi.gst_opts * max(nq, 1))()
+        for i in range(nq):
+            ub = ubs[i] if ubs is not None else None
+            opts[i].ub = float(ub) if ub is not None else 0.0
+            opts[i].prune_lb = 1 if prune_lb else 0
+            opts[i].want_tree = 1 if want_tree else 0
+        bo = _abi.gst_batch_opts()
+        bo.slots = slots
+        bo.rounds_per_step = rounds_per_step
+        bo.use_graph = 1 if use_graph else 0
+        res = (_abi.gst_result * max(nq, 1))()
+        status = (C.c_int * max(nq, 1))()
+        bs = _abi.gst_batch_stats()
+        st = self.lib.gst_solve_batch(self.ctx, graph.handle, nq, grps, opts, C.byref(bo), res, status, C.byref(bs))
+        out = []
+        try:
+            for i in range(nq):
+                if status[i] != _abi.GST_OK:
+                    out.append(GstError(status[i], "query %d failed" % i))
+                    continue
+                r = res[i]
+                ne = r.n_edges
+                if ne > 0:
+                    eu = np.ctypeslib.as_array(r.eu, shape=(ne,)).copy()
+                    ev = np.ctypeslib.as_array(r.ev, shape=(ne,)).copy()
+                    ew = np.ctypeslib.as_array(r.ew, shape=(ne,)).copy()
+                else:
+                    eu, ev, ew = np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float64)
+                stats = {name: getattr(r.stats, name) for name, _ in _abi.gst_stats._fields_}
+                out.append(Result(r.cost, r.tree_cost, r.root, eu, ev, ew, bool(r.used_ub), bool(r.fell_back), stats))
+        finally:
+            for i in range(nq):
+                self.lib.gst_result_free(C.byref(res[i]))
+        stats = {name: getattr(bs, name) for name, _ in _abi.gst_batch_stats._fields_ if name != "reserved"}
+        if st not in (_abi.GST_OK,) and raise_on_error:
+            bad = next((e for e in out if isinstance(e, GstError)), None)
+            raise GstError(st, self.lib.gst_last_error(self.ctx).decode()) if bad is None else bad
+        return out, stats
 
 
 class Model:

@@ -61,6 +61,19 @@ class gst_learned_stats(C.Structure):
                 ("t_score_select_ms", C.c_double), ("t_sub_solve_ms", C.c_double), ("t_full_solve_ms", C.c_double)]
 
 
+class gst_batch_opts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("rounds_per_step", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+class gst_batch_stats(C.Structure):
+    _fields_ = [("t_total_ms", C.c_double), ("queries_per_s", C.c_double), ("latency_p50_ms", C.c_double),
+                ("latency_p99_ms", C.c_double), ("steps", C.c_uint64), ("grow_launches", C.c_uint64),
+                ("grow_relax", C.c_uint64), ("grow_bytes", C.c_uint64), ("merge_bytes", C.c_uint64),
+                ("queries", C.c_int64), ("slots", C.c_int32), ("rounds_per_step", C.c_int32),
+                ("used_graph", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
 class gst_host_csr(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int64), ("row_ptr", C.POINTER(C.c_int64)),
                 ("col", C.POINTER(C.c_int32)), ("w", C.c_void_p), ("w_type", C.c_int32)]
@@ -79,6 +92,9 @@ SIGNATURES = {
     "gst_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.POINTER(gst_opts),
                             C.POINTER(gst_result)]),
     "gst_result_free": (None, [C.POINTER(gst_result)]),
+    "gst_solve_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(gst_groups), C.POINTER(gst_opts),
+                                  C.POINTER(gst_batch_opts), C.POINTER(gst_result), C.POINTER(C.c_int),
+                                  C.POINTER(gst_batch_stats)]),
     "gst_validate_tree": (C.c_int, [C.POINTER(gst_csr), C.POINTER(gst_groups), C.c_int32, C.c_void_p,
                                     C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "gst_model_num_params": (C.c_int64, []),

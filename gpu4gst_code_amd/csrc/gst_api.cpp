@@ -11,6 +11,7 @@
 #include <numeric>
 #include <vector>
 
+#include "gst_batch.h"
 #include "gst_internal.h"
 #include "gst_mpnn.h"
 
@@ -129,6 +130,26 @@ gst_status check_csr(gst_ctx* ctx, const gst_csr* c) {
         if (!weight_ok(w[e], c->w_type))
             return gst_set_error(ctx, GST_INVALID_ARG, "csr: arc %lld has a negative or non-finite weight", (long long)e);
     }
+    return GST_OK;
+}
+
+// group-list checks shared by every solve entry point
+gst_status check_query(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups) {
+    const int ng = groups->n_groups;
+    if (ng < 1) return gst_set_error(ctx, GST_INVALID_ARG, "need at least one group");
+    if (ng > GST_MAX_GROUPS)
+        return gst_set_error(ctx, GST_TOO_MANY_GROUPS, "%d groups exceed the maximum of %d", ng, GST_MAX_GROUPS);
+    if (!groups->grp_ptr || (groups->grp_ptr[ng] > 0 && !groups->grp_vtx) || groups->grp_ptr[0] != 0)
+        return gst_set_error(ctx, GST_INVALID_ARG, "groups: malformed grp_ptr");
+    for (int i = 0; i < ng; ++i) {
+        if (groups->grp_ptr[i + 1] < groups->grp_ptr[i])
+            return gst_set_error(ctx, GST_INVALID_ARG, "groups: grp_ptr decreases at group %d", i);
+        if (groups->grp_ptr[i + 1] == groups->grp_ptr[i])
+            return gst_set_error(ctx, GST_INFEASIBLE, "group %d is empty", i);
+    }
+    for (int32_t j = 0; j < groups->grp_ptr[ng]; ++j)
+        if (groups->grp_vtx[j] < 0 || groups->grp_vtx[j] >= g->n)
+            return gst_set_error(ctx, GST_INVALID_ARG, "groups: member %d outside [0,%d)", groups->grp_vtx[j], g->n);
     return GST_OK;
 }
 
@@ -254,23 +275,50 @@ gst_status gst_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
     result->root = -1;
     if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
     if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
-    const int ng = groups->n_groups;
-    if (ng < 1) return gst_set_error(ctx, GST_INVALID_ARG, "need at least one group");
-    if (ng > GST_MAX_GROUPS)
-        return gst_set_error(ctx, GST_TOO_MANY_GROUPS, "%d groups exceed the maximum of %d", ng, GST_MAX_GROUPS);
-    if (!groups->grp_ptr || (groups->grp_ptr[ng] > 0 && !groups->grp_vtx) || groups->grp_ptr[0] != 0)
-        return gst_set_error(ctx, GST_INVALID_ARG, "groups: malformed grp_ptr");
-    for (int i = 0; i < ng; ++i) {
-        if (groups->grp_ptr[i + 1] < groups->grp_ptr[i])
-            return gst_set_error(ctx, GST_INVALID_ARG, "groups: grp_ptr decreases at group %d", i);
-        if (groups->grp_ptr[i + 1] == groups->grp_ptr[i])
-            return gst_set_error(ctx, GST_INFEASIBLE, "group %d is empty", i);
-    }
-    for (int32_t j = 0; j < groups->grp_ptr[ng]; ++j)
-        if (groups->grp_vtx[j] < 0 || groups->grp_vtx[j] >= g->n)
-            return gst_set_error(ctx, GST_INVALID_ARG, "groups: member %d outside [0,%d)", groups->grp_vtx[j], g->n);
+    const gst_status cs = check_query(ctx, g, groups);
+    if (cs != GST_OK) return cs;
     return gst_dpbf_solve(ctx, g, groups, opts, result);
 }
+
+gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* groups,
+                           const gst_opts* opts, const gst_batch_opts* bopts, gst_result* results,
+                           gst_status* status, gst_batch_stats* stats) {
+    if (!ctx || !g || nq < 0 || (nq > 0 && (!groups || !results || !status)))
+        return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "null argument") : GST_INVALID_ARG;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
+    if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
+    if (bopts && (bopts->slots < 0 || bopts->rounds_per_step < 0))
+        return gst_set_error(ctx, GST_INVALID_ARG, "batch options: negative slots or rounds_per_step");
+    gst_status first = GST_OK;
+    int gmax = 1;
+    for (int32_t q = 0; q < nq; ++q) {
+        std::memset(&results[q], 0, sizeof(results[q]));
+        results[q].root = -1;
+        status[q] = check_query(ctx, g, &groups[q]);
+        if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
+        else if (first == GST_OK) first = status[q];
+    }
+    int32_t next = 0;
+    auto next_job = [&](gst_batch_job& j) -> bool {
+        while (next < nq && status[next] != GST_OK) ++next;
+        if (next >= nq) return false;
+        j = gst_batch_job();
+        j.G = g;
+        j.groups = &groups[next];
+        if (opts) j.opts = opts[next];
+        else { j.opts = gst_opts(); j.opts.want_tree = 1; }
+        j.res = &results[next];
+        j.tag = next;
+        ++next;
+        return true;
+    };
+    auto job_done = [&](gst_batch_job& j) { status[j.tag] = j.status; };
+    const gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)g->n, gmax, g->m, bopts, stats);
+    if (first == GST_OK) first = rs;
+    return first;
+}
+
 
 gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, gst_model* m,
                              double topk_frac, const gst_opts* opts, gst_result* result, gst_learned_stats* stats) {
@@ -283,20 +331,8 @@ gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups*
     result->root = -1;
     if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
     if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
-    const int ng = groups->n_groups;
-    if (ng < 1) return gst_set_error(ctx, GST_INVALID_ARG, "need at least one group");
-    if (ng > GST_MAX_GROUPS)
-        return gst_set_error(ctx, GST_TOO_MANY_GROUPS, "%d groups exceed the maximum of %d", ng, GST_MAX_GROUPS);
-    if (!groups->grp_ptr || (groups->grp_ptr[ng] > 0 && !groups->grp_vtx) || groups->grp_ptr[0] != 0)
-        return gst_set_error(ctx, GST_INVALID_ARG, "groups: malformed grp_ptr");
-    for (int i = 0; i < ng; ++i) {
-        if (groups->grp_ptr[i + 1] < groups->grp_ptr[i])
-            return gst_set_error(ctx, GST_INVALID_ARG, "groups: grp_ptr decreases at group %d", i);
-        if (groups->grp_ptr[i + 1] == groups->grp_ptr[i]) return gst_set_error(ctx, GST_INFEASIBLE, "group %d is empty", i);
-    }
-    for (int32_t j = 0; j < groups->grp_ptr[ng]; ++j)
-        if (groups->grp_vtx[j] < 0 || groups->grp_vtx[j] >= g->n)
-            return gst_set_error(ctx, GST_INVALID_ARG, "groups: member %d outside [0,%d)", groups->grp_vtx[j], g->n);
+    const gst_status cs = check_query(ctx, g, groups);
+    if (cs != GST_OK) return cs;
     return gst_learned_solve(ctx, g, groups, m, topk_frac, opts, result, stats);
 }
 

@@ -1,0 +1,30 @@
+// Batched engine interface shared by the C API (gst_api.cpp), the learned driver (gst_mpnn.hip)
+// and the engine (gst_batch.hip).  Not part of the public ABI.
+#pragma once
+
+#include <functional>
+
+#include "gst_internal.h"
+
+// One DPBF solve run by the engine.  The groups pointer must stay valid until job_done sees it.
+struct gst_batch_job {
+    const gst_graph* G = nullptr;
+    const gst_groups* groups = nullptr;
+    gst_opts opts{};             // ub, prune_lb and want_tree are honoured
+    bool temp_graph = false;     // graph made for this job only: push rounds only (no hub item list)
+    gst_result* res = nullptr;   // filled when the job finishes
+    gst_status status = GST_OK;
+    double t_fill = 0.0;         // host ms when the job entered a slot
+    double t_done = 0.0;         // host ms when its result was read
+    int64_t tag = 0;             // caller's bookkeeping
+    int kind = 0;
+};
+
+// Runs jobs until next_job returns false and every slot is idle.  next_job is asked for a job
+// whenever a slot is free; job_done receives each finished job (status and result set) and may
+// make more jobs available to next_job.  n_cap / g_cap / m_cap bound every job's graph size and
+// group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
+gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
+                         const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
+                         int64_t m_cap, const gst_batch_opts* bopts, gst_batch_stats* stats);
+void gst_batch_free(gst_ctx* ctx);

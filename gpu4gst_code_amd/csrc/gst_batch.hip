@@ -1,0 +1,785 @@
+// Batched DPBF engine: many exact queries in flight on one GPU.
+//
+// The one-query driver (gst_dpbf.hip) is bound by round latency on sparse frontiers: a grow round
+// over a few thousand items still costs a launch and a chain of dependent memory accesses.  Here up
+// to MAX_SLOTS queries ("slots") hold their own state tables, frontiers and round-control blocks,
+// and every launch serves all of them:
+//   k_plan_step   one wave: per slot, a finished grow level becomes a merge of the next level
+//                 (or of the full mask), a merged full mask becomes a traceback; lists the work
+//   k_batch_merge the merges of every listed slot, the grid split evenly among them
+//   k_batch_compact the new frontier of every slot merged below the full mask
+//   k_batch_trace one wave per slot whose full mask was merged one step earlier
+//   k_plan_grow   one wave: per growing slot, the workgroups its round needs (as the one-query
+//                 round sizes itself), scaled down to the grid when the sum exceeds it
+//   k_batch_grow  one grow round of every growing slot, each slot on its own slice of the grid
+// A step is k_plan_step, the three batch kernels, then rounds_per_step (k_plan_grow, k_batch_grow)
+// pairs; it is captured once as a HIP graph and replayed.  The host reads the slot phases after
+// each step, collects finished queries and refills their slots.  All kernel bodies are the
+// one-query bodies (gst_dpbf_dev.h) run on a slot's slice of the grid, so each query computes
+// exactly what gst_solve computes for it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "gst_batch.h"
+#include "gst_dpbf_dev.h"
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return gst_set_error(ctx, GST_HIP_ERROR, "%s failed: %s (%s:%d)", #call,         \
+                                 hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t MAX_SLOTS = 64;
+// merge LDS of a batched launch: vertex tiles up to 40 KB (the one-query budget) plus the split
+// table when it fits in 16 KB, for every (g, level) alike
+constexpr size_t MERGE_LDS_TILES = 40 * 1024;
+constexpr size_t MERGE_LDS_PAIRS = 16 * 1024;
+enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3 };
+
+struct SlotDev {
+    uint32_t phase;
+    uint32_t k;     // level being grown (PH_GROW)
+    uint32_t g;
+    uint32_t pad;
+};
+
+struct BatchPlan {
+    uint32_t seq;                          // grow launch number (the round-control seq)
+    uint32_t n_grow, n_merge, n_compact, n_trace;
+    uint32_t grow_slot[MAX_SLOTS];
+    uint32_t grow_level[MAX_SLOTS];
+    uint32_t grow_start[MAX_SLOTS + 1];    // first workgroup of each growing slot; [n_grow] = total
+    uint32_t merge_slot[MAX_SLOTS];
+    uint32_t merge_level[MAX_SLOTS];
+    uint32_t compact_slot[MAX_SLOTS];
+    uint32_t compact_level[MAX_SLOTS];
+    uint32_t trace_slot[MAX_SLOTS];
+};
+
+struct BatchArgs {
+    SlotDev* slots;
+    const GrowArgs* gargs;    // [slot * NCTL + level]
+    const MergeArgs* margs;   // [slot * NCTL + level]
+    const TraceArgs* targs;   // [slot]
+    BatchPlan* plan;
+    uint32_t S;               // slots (<= MAX_SLOTS)
+    uint32_t grid;            // workgroups a grow round may use in total
+};
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+__global__ void __launch_bounds__(WAVE) k_plan_step(BatchArgs b) {
+    const uint32_t s = threadIdx.x;
+    bool dm = false, dc = false, dt = false;
+    uint32_t lvl = 0;
+    if (s < b.S) {
+        SlotDev* sd = &b.slots[s];
+        const uint32_t ph = sd->phase;
+        if (ph == PH_TRACE) {
+            dt = true;
+            sd->phase = PH_DONE;
+        } else if (ph == PH_GROW && b.gargs[s * NCTL + sd->k].ctl->level_done) {
+            lvl = sd->k + 1;
+            dm = true;
+            if (lvl == sd->g) {
+                sd->phase = PH_TRACE;     // traced in the next step, after this merge found the root
+            } else {
+                dc = true;                // the merge flags the new frontier; compaction lists it
+                sd->k = lvl;
+            }
+        }
+    }
+    BatchPlan* p = b.plan;
+    const uint64_t mm = __ballot(dm), mc = __ballot(dc), mt = __ballot(dt);
+    if (dm) { const uint32_t i = lanes_below(mm); p->merge_slot[i] = s; p->merge_level[i] = lvl; }
+    if (dc) { const uint32_t i = lanes_below(mc); p->compact_slot[i] = s; p->compact_level[i] = lvl; }
+    if (dt) p->trace_slot[lanes_below(mt)] = s;
+    if (s == 0) {
+        p->n_merge = (uint32_t)__popcll(mm);
+        p->n_compact = (uint32_t)__popcll(mc);
+        p->n_trace = (uint32_t)__popcll(mt);
+    }
+}
+
+template <bool FLT>
+__global__ void __launch_bounds__(MERGE_THREADS) k_batch_merge(BatchArgs b) {
+    extern __shared__ unsigned long long lds[];
+    const BatchPlan* p = b.plan;
+    const uint32_t nm = p->n_merge;
+    if (nm == 0) return;
+    const uint32_t per = max(1u, gridDim.x / nm);   // gridDim.x >= MAX_SLOTS, so per >= 1 covers all
+    const uint32_t i = blockIdx.x / per;
+    if (i >= nm) return;
+    merge_body<FLT>(b.margs[p->merge_slot[i] * NCTL + p->merge_level[i]], lds, blockIdx.x - i * per, per);
+}
+
+__global__ void __launch_bounds__(256) k_batch_compact(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    const uint32_t nc = p->n_compact;
+    if (nc == 0) return;
+    const uint32_t per = max(1u, gridDim.x / nc);
+    const uint32_t i = blockIdx.x / per;
+    if (i >= nc) return;
+    const GrowArgs& a = b.gargs[p->compact_slot[i] * NCTL + p->compact_level[i]];
+    compact_body(a.inq[0], a.row, a.n, a.chunk, a.list[0], &a.ctl->cnt[0], a.stats, 0u, blockIdx.x - i * per, per);
+}
+
+__global__ void __launch_bounds__(WAVE) k_batch_trace(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    if (blockIdx.x >= p->n_trace) return;
+    trace_body(b.targs[p->trace_slot[blockIdx.x]]);
+}
+
+__global__ void __launch_bounds__(WAVE) k_plan_grow(BatchArgs b) {
+    const uint32_t s = threadIdx.x;
+    uint32_t need = 0, lvl = 0;
+    if (s < b.S) {
+        const SlotDev sd = b.slots[s];
+        if (sd.phase == PH_GROW) {
+            const GrowArgs& a = b.gargs[s * NCTL + sd.k];
+            const RoundCtl* c = a.ctl;
+            if (!c->level_done) {
+                const uint32_t r = c->round;
+                const uint32_t n_cur = c->cnt[r % 3], n_prv = c->cnt[(r + 2) % 3];
+                const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
+                // the same sizing as a one-query round: a pull round takes every workgroup, a push
+                // round one wave per item (and one thread per item of the list it clears)
+                need = pull ? b.grid
+                            : max(1u, min(b.grid, max((n_cur + GROW_WAVES - 1) / GROW_WAVES,
+                                                      (n_prv + GROW_BLOCK - 1) / GROW_BLOCK)));
+                lvl = sd.k;
+            }
+        }
+    }
+    const uint32_t total = wave_sum32(need);
+    uint32_t alloc = need;
+    if (total > b.grid && need) alloc = max(1u, (uint32_t)((uint64_t)need * b.grid / total));
+    uint32_t incl = alloc;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, WAVE);
+        if (lane_id() >= (uint32_t)d) incl += y;
+    }
+    BatchPlan* p = b.plan;
+    const uint64_t act = __ballot(need != 0);
+    if (need) {
+        const uint32_t i = lanes_below(act);
+        p->grow_slot[i] = s;
+        p->grow_level[i] = lvl;
+        p->grow_start[i] = incl - alloc;
+    }
+    if (s == WAVE - 1) {
+        p->n_grow = (uint32_t)__popcll(act);
+        p->grow_start[__popcll(act)] = incl;
+        p->seq = p->seq + 1;
+    }
+}
+
+// one grow round of every growing slot; the grid is b.grid + MAX_SLOTS workgroups (the plan's
+// per-slot minimum of one workgroup can exceed b.grid by at most one per slot)
+template <bool FLT>
+__global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_batch_grow(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    const uint32_t ng = p->n_grow;
+    const uint32_t bid = blockIdx.x;
+    if (ng == 0 || bid >= p->grow_start[ng]) return;
+    uint32_t lo = 0, hi = ng;   // grow_start[lo] <= bid < grow_start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p->grow_start[mid] <= bid) lo = mid; else hi = mid;
+    }
+    const uint32_t st = p->grow_start[lo];
+    grow_round_body<FLT>(b.gargs[p->grow_slot[lo] * NCTL + p->grow_level[lo]], bid - st, p->grow_start[lo + 1] - st,
+                         p->seq);
+}
+
+template <class T>
+bool dalloc(T*& p, size_t count) {
+    p = nullptr;
+    return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// engine state (one per context)
+// ---------------------------------------------------------------------------------------
+struct gst_batch_engine {
+    struct Slot {
+        uint64_t* T = nullptr;
+        uint32_t* gm = nullptr;
+        uint64_t* chg[3] = {nullptr, nullptr, nullptr};
+        uint32_t* inq[3] = {nullptr, nullptr, nullptr};
+        uint64_t* lists[3] = {nullptr, nullptr, nullptr};
+        RoundCtl* ctl = nullptr;
+        unsigned long long* root = nullptr;
+        unsigned long long* stats = nullptr;
+        uint32_t* edges = nullptr;          // 3 * ecap + 1
+        int32_t* grp = nullptr;
+        size_t grp_cap = 0;
+        // pinned host staging (args written at fill) and readback
+        int32_t* h_grp = nullptr;
+        GrowArgs* h_gargs = nullptr;        // NCTL
+        MergeArgs* h_margs = nullptr;       // NCTL
+        TraceArgs* h_targs = nullptr;
+        SlotDev* h_sd = nullptr;
+        unsigned char* h_back = nullptr;    // root | ctl[NCTL] | stats | out_n
+        uint32_t* h_edges = nullptr;        // 3 * ecap
+        // host bookkeeping
+        bool busy = false;
+        gst_batch_job job;
+        int g = 0;
+        bool flt = false;
+    };
+    int S = 0;
+    uint32_t n_cap = 0;
+    int g_cap = 0;
+    uint64_t items_cap = 0;
+    uint32_t W_cap = 1;
+    uint32_t ecap = 0;
+    std::vector<Slot> slots;
+    SlotDev* d_slots = nullptr;
+    GrowArgs* d_gargs = nullptr;
+    MergeArgs* d_margs = nullptr;
+    TraceArgs* d_targs = nullptr;
+    BatchPlan* d_plan = nullptr;
+    SlotDev* h_slots = nullptr;             // pinned phase snapshot
+    // level tables per group count
+    uint64_t* pairs[GST_MAX_GROUPS + 1] = {nullptr};
+    uint32_t* masks[GST_MAX_GROUPS + 1] = {nullptr};
+    uint32_t* pos_of[GST_MAX_GROUPS + 1] = {nullptr};
+    // hub work items of the last persistent graph (pull rounds)
+    uint64_t gi_uid = 0;
+    uint32_t gi_chunk = 0;
+    uint64_t* all_items = nullptr;
+    uint64_t all_cap = 0;
+    uint32_t n_all = 0;
+    uint64_t n_items_total = 0;
+    uint32_t* all_cnt = nullptr;
+    // captured step
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int g_S = -1, g_R = -1, g_flt = -1;
+};
+
+static size_t back_bytes() { return 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT + 8; }
+
+static void free_slot(gst_batch_engine::Slot& sl) {
+    hipFree(sl.T); hipFree(sl.gm);
+    for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
+    hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp);
+    if (sl.h_grp) hipHostFree(sl.h_grp);
+    if (sl.h_gargs) hipHostFree(sl.h_gargs);
+    if (sl.h_margs) hipHostFree(sl.h_margs);
+    if (sl.h_targs) hipHostFree(sl.h_targs);
+    if (sl.h_sd) hipHostFree(sl.h_sd);
+    if (sl.h_back) hipHostFree(sl.h_back);
+    if (sl.h_edges) hipHostFree(sl.h_edges);
+    sl = gst_batch_engine::Slot();
+}
+
+static void free_engine_slots(gst_batch_engine* E) {
+    for (auto& sl : E->slots) free_slot(sl);
+    E->slots.clear();
+    hipFree(E->d_slots); hipFree(E->d_gargs); hipFree(E->d_margs); hipFree(E->d_targs); hipFree(E->d_plan);
+    E->d_slots = nullptr; E->d_gargs = nullptr; E->d_margs = nullptr; E->d_targs = nullptr; E->d_plan = nullptr;
+    if (E->h_slots) hipHostFree(E->h_slots);
+    E->h_slots = nullptr;
+    if (E->gexec) hipGraphExecDestroy(E->gexec);
+    if (E->graph) hipGraphDestroy(E->graph);
+    E->gexec = nullptr; E->graph = nullptr; E->g_S = -1;
+    E->S = 0; E->n_cap = 0; E->g_cap = 0;
+}
+
+void gst_batch_free(gst_ctx* ctx) {
+    gst_batch_engine* E = ctx->be;
+    if (!E) return;
+    hipSetDevice(ctx->device);
+    free_engine_slots(E);
+    for (int g = 0; g <= GST_MAX_GROUPS; ++g) { hipFree(E->pairs[g]); hipFree(E->masks[g]); hipFree(E->pos_of[g]); }
+    hipFree(E->all_items); hipFree(E->all_cnt);
+    delete E;
+    ctx->be = nullptr;
+}
+
+// bytes of device memory one slot needs
+static size_t slot_bytes(uint32_t n, int g, uint64_t items_cap, uint32_t W, uint32_t ecap) {
+    const size_t words = (size_t)n * (((size_t)1 << g) - 1);
+    return words * 8 + (size_t)n * 4 + 3 * ((size_t)n * W * 8 + (size_t)n * 4 + items_cap * 8) +
+           sizeof(RoundCtl) * NCTL + 8 + 8 * ST_NSTAT + (3 * (size_t)ecap + 1) * 4 + 64 * 1024;
+}
+
+static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_t n, int g, int64_t m, uint32_t chunk) {
+    uint32_t W = 1;
+    const gst_levels& L = gst_get_levels(ctx, g);
+    for (int k = 1; k <= g; ++k) W = std::max<uint32_t>(W, (uint32_t)((L.m[k] + 63) / 64));
+    const uint64_t items_cap = (uint64_t)n + (uint64_t)m / chunk + 64;
+    const uint32_t ecap = n + 2u * (uint32_t)g + 16u;
+    if (E->S >= S && E->n_cap >= n && E->g_cap >= g && E->items_cap >= items_cap && E->W_cap >= W && !E->slots.empty())
+        return GST_OK;
+    hipStream_t s = (hipStream_t)ctx->stream;
+    HIPCHK(hipStreamSynchronize(s));
+    free_engine_slots(E);
+    E->n_cap = n; E->g_cap = g; E->items_cap = items_cap; E->W_cap = W; E->ecap = ecap;
+    E->slots.resize((size_t)S);
+    bool ok = dalloc(E->d_slots, MAX_SLOTS) && dalloc(E->d_gargs, (size_t)MAX_SLOTS * NCTL) &&
+              dalloc(E->d_margs, (size_t)MAX_SLOTS * NCTL) && dalloc(E->d_targs, MAX_SLOTS) && dalloc(E->d_plan, 1) &&
+              hipHostMalloc((void**)&E->h_slots, sizeof(SlotDev) * MAX_SLOTS) == hipSuccess;
+    const size_t words = (size_t)n * (((size_t)1 << g) - 1);
+    for (int i = 0; ok && i < S; ++i) {
+        auto& sl = E->slots[(size_t)i];
+        ok = ok && dalloc(sl.T, words) && dalloc(sl.gm, n);
+        for (int j = 0; ok && j < 3; ++j)
+            ok = dalloc(sl.chg[j], (size_t)n * W) && dalloc(sl.inq[j], n) && dalloc(sl.lists[j], items_cap);
+        ok = ok && dalloc(sl.ctl, NCTL) && dalloc(sl.root, 1) && dalloc(sl.stats, ST_NSTAT) &&
+             dalloc(sl.edges, 3 * (size_t)ecap + 1);
+        ok = ok && hipHostMalloc((void**)&sl.h_gargs, sizeof(GrowArgs) * NCTL) == hipSuccess &&
+             hipHostMalloc((void**)&sl.h_margs, sizeof(MergeArgs) * NCTL) == hipSuccess &&
+             hipHostMalloc((void**)&sl.h_targs, sizeof(TraceArgs)) == hipSuccess &&
+             hipHostMalloc((void**)&sl.h_sd, sizeof(SlotDev)) == hipSuccess &&
+             hipHostMalloc((void**)&sl.h_back, back_bytes()) == hipSuccess &&
+             hipHostMalloc((void**)&sl.h_edges, sizeof(uint32_t) * 3 * (size_t)ecap) == hipSuccess;
+        if (ok) {
+            const int fg = std::max(1, ctx->cu_count * 4);
+            for (int j = 0; j < 3; ++j) {
+                k_fill_u64<<<fg, 256, 0, s>>>(sl.chg[j], 0ull, (uint64_t)n * W);
+                k_fill_u32<<<fg, 256, 0, s>>>(sl.inq[j], 0u, n);
+            }
+        }
+    }
+    if (!ok) {
+        free_engine_slots(E);
+        return gst_set_error(ctx, GST_OOM, "batch: device allocation of %d slots failed (V=%u, g=%d)", S, n, g);
+    }
+    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)E->d_slots, 0u, sizeof(SlotDev) * MAX_SLOTS / 4);
+    k_fill_u32<<<4, 256, 0, s>>>((uint32_t*)E->d_plan, 0u, sizeof(BatchPlan) / 4);
+    E->S = S;
+    HIPCHK(hipGetLastError());
+    return GST_OK;
+}
+
+static gst_status ensure_tables(gst_ctx* ctx, gst_batch_engine* E, int g) {
+    if (E->masks[g]) return GST_OK;
+    const gst_levels& L = gst_get_levels(ctx, g);
+    hipStream_t s = (hipStream_t)ctx->stream;
+    if (!dalloc(E->pairs[g], std::max<size_t>(L.pairs.size(), 1)) || !dalloc(E->masks[g], L.masks.size()) ||
+        !dalloc(E->pos_of[g], L.pos_of.size()))
+        return gst_set_error(ctx, GST_OOM, "batch: level table allocation failed");
+    if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(E->pairs[g], L.pairs.data(), L.pairs.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E->masks[g], L.masks.data(), L.masks.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E->pos_of[g], L.pos_of.data(), L.pos_of.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));   // the level vectors are host memory
+    return GST_OK;
+}
+
+// hub items of a persistent graph, for pull rounds (rebuilt when the graph or chunk changes)
+static gst_status ensure_hub_items(gst_ctx* ctx, gst_batch_engine* E, const gst_graph* G, uint32_t chunk) {
+    if (E->gi_uid == G->uid && E->gi_chunk == chunk && E->all_items) return GST_OK;
+    hipStream_t s = (hipStream_t)ctx->stream;
+    const uint64_t cap = (uint64_t)G->n + (uint64_t)G->m / chunk + 64;
+    if (E->all_cap < cap) {
+        hipFree(E->all_items);
+        E->all_items = nullptr;
+        E->all_cap = 0;
+        if (!dalloc(E->all_items, cap)) return gst_set_error(ctx, GST_OOM, "batch: item list allocation failed");
+        E->all_cap = cap;
+    }
+    if (!E->all_cnt && !dalloc(E->all_cnt, 1)) return gst_set_error(ctx, GST_OOM, "batch: allocation failed");
+    k_fill_u32<<<1, 64, 0, s>>>(E->all_cnt, 0u, 1);
+    const uint32_t n = (uint32_t)G->n;
+    k_compact<<<std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8)), 256, 0, s>>>(
+        nullptr, G->d_row, n, chunk, E->all_items, E->all_cnt, nullptr, HUB_DEG);
+    uint32_t na = 0;
+    HIPCHK(hipMemcpyAsync(&na, E->all_cnt, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    uint64_t tot = 0;
+    for (uint32_t v = 0; v < n; ++v) tot += (uint64_t)((G->h_row[v + 1] - G->h_row[v] + chunk - 1) / chunk);
+    E->n_all = na; E->n_items_total = tot; E->gi_uid = G->uid; E->gi_chunk = chunk;
+    return GST_OK;
+}
+
+// bound in cost bits (same rule as the one-query driver)
+static uint32_t ub_bits_of(double ub, bool flt) {
+    if (!(std::isfinite(ub) && ub > 0)) return GST_INF_COST;
+    if (flt) {
+        float fu = (float)(ub * (1.0 + 1e-4));
+        fu = std::nextafter(fu, INFINITY);
+        uint32_t b;
+        std::memcpy(&b, &fu, 4);
+        return b;
+    }
+    const double c = std::floor(ub) + 1.0;
+    return c >= 4294967295.0 ? GST_INF_COST : (uint32_t)c;
+}
+
+// Enqueue the set-up of one query in slot `si`: group bits, level 1, its frontier, and the
+// argument blocks of every level.
+static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch_job& job, uint32_t chunk) {
+    hipStream_t s = (hipStream_t)ctx->stream;
+    auto& sl = E->slots[(size_t)si];
+    const gst_graph* G = job.G;
+    const gst_groups* gr = job.groups;
+    const int g = gr->n_groups;
+    const uint32_t n = (uint32_t)G->n;
+    const bool flt = G->w_type == GST_W_FLOAT32;
+    const gst_levels& L = gst_get_levels(ctx, g);
+    gst_status st = ensure_tables(ctx, E, g);
+    if (st != GST_OK) return st;
+    const bool persistent = !job.temp_graph;
+    if (persistent) {
+        st = ensure_hub_items(ctx, E, G, chunk);
+        if (st != GST_OK) return st;
+    }
+    // groups
+    const size_t gneed = (size_t)(g + 1) + (size_t)gr->grp_ptr[g];
+    if (sl.grp_cap < gneed) {
+        hipFree(sl.grp);
+        if (sl.h_grp) hipHostFree(sl.h_grp);
+        sl.grp = nullptr; sl.h_grp = nullptr; sl.grp_cap = 0;
+        const size_t cap = std::max<size_t>(gneed, 4096);
+        if (!dalloc(sl.grp, cap) || hipHostMalloc((void**)&sl.h_grp, cap * 4) != hipSuccess)
+            return gst_set_error(ctx, GST_OOM, "batch: group buffer allocation failed");
+        sl.grp_cap = cap;
+    }
+    for (int i = 0; i <= g; ++i) sl.h_grp[i] = gr->grp_ptr[i];
+    std::memcpy(sl.h_grp + g + 1, gr->grp_vtx, sizeof(int32_t) * (size_t)gr->grp_ptr[g]);
+    HIPCHK(hipMemcpyAsync(sl.grp, sl.h_grp, gneed * 4, hipMemcpyHostToDevice, s));
+
+    uint64_t lvl_base[GST_MAX_GROUPS + 2] = {0};
+    for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
+    const uint32_t ubb = ub_bits_of(job.opts.ub, flt);
+    const int fg = std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8));
+    k_fill_u32<<<fg, 256, 0, s>>>(sl.gm, 0u, n);
+    k_set_group_bits<<<64, 256, 0, s>>>(sl.gm, sl.grp, sl.grp + g + 1, g);
+    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)sl.stats, 0ull, ST_NSTAT);
+    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)sl.root, GST_INF_WORD, 1);
+    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)sl.ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
+    k_init_level1<<<fg, 256, 0, s>>>(sl.T + lvl_base[1], sl.gm, G->d_row, n, (uint32_t)g, sl.chg[0], sl.inq[0],
+                                    g > 1 ? 1 : 0);
+    if (g > 1)
+        k_compact<<<fg, 256, 0, s>>>(sl.inq[0], G->d_row, n, chunk, sl.lists[0], &sl.ctl[1].cnt[0], sl.stats, 0u);
+    else
+        k_root_scan_l1<<<fg, 256, 0, s>>>(sl.T, n, sl.root);
+
+    // argument blocks
+    for (int k = 1; k < g; ++k) {
+        GrowArgs& a = sl.h_gargs[k];
+        std::memset(&a, 0, sizeof(a));
+        a.Tk = sl.T + lvl_base[k];
+        a.T1 = sl.T + lvl_base[1];
+        a.masks = E->masks[g] + L.off[k];
+        a.row = G->d_row; a.col = G->d_col; a.w = G->d_w; a.rev = G->d_rev;
+        for (int i = 0; i < 3; ++i) { a.list[i] = sl.lists[i]; a.chg[i] = sl.chg[i]; a.inq[i] = sl.inq[i]; }
+        a.ctl = sl.ctl + k;
+        a.stats = sl.stats;
+        a.m = (uint32_t)L.m[k];
+        a.W = (uint32_t)((L.m[k] + 63) / 64);
+        a.g = (uint32_t)g;
+        a.chunk = chunk;
+        a.ub = ubb;
+        a.use_lb = (job.opts.prune_lb && k >= 2) ? 1u : 0u;
+        a.all_items = persistent ? E->all_items : nullptr;
+        a.n_all = persistent ? E->n_all : 0u;
+        a.n = n;
+        a.pull_thresh = persistent ? (uint32_t)std::max<uint64_t>(1, E->n_items_total * 450ull / 1000ull) : 0xFFFFFFFFu;
+        a.rlog = nullptr;
+        a.seq = 0;
+    }
+    for (int k = 2; k <= g; ++k) {
+        MergeArgs& ma = sl.h_margs[k];
+        std::memset(&ma, 0, sizeof(ma));
+        ma.T = sl.T; ma.pairs = E->pairs[g] + L.pair_off[k]; ma.masks = E->masks[g]; ma.gm = sl.gm; ma.row = G->d_row;
+        ma.list0 = sl.lists[0]; ma.chg0 = sl.chg[0]; ma.inq0 = sl.inq[0]; ma.ctl = sl.ctl + k; ma.root = sl.root;
+        ma.stats = sl.stats;
+        for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ma.lvl_base[j] = lvl_base[j]; ma.m[j] = (uint32_t)L.m[j]; ma.off[j] = (uint32_t)L.off[j]; }
+        ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
+        ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ubb; ma.use_lb = job.opts.prune_lb ? 1u : 0u;
+        ma.is_full = (k == g) ? 1u : 0u;
+        const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
+        uint32_t VB = (uint32_t)(MERGE_LDS_TILES / (((size_t)Lk + Wk) * 8));
+        ma.VB = std::max<uint32_t>(1, std::min<uint32_t>(VB, 256));
+        ma.pairs_in_lds = ((size_t)L.pair_cnt[k] * 8 <= MERGE_LDS_PAIRS) ? 1u : 0u;
+    }
+    {
+        TraceArgs& ta = *sl.h_targs;
+        std::memset(&ta, 0, sizeof(ta));
+        ta.T = sl.T; ta.pos_of = E->pos_of[g]; ta.col = G->d_col; ta.w = G->d_w; ta.root = sl.root;
+        ta.out_u = sl.edges; ta.out_v = sl.edges + E->ecap; ta.out_w = sl.edges + 2 * (size_t)E->ecap;
+        ta.out_n = sl.edges + 3 * (size_t)E->ecap;
+        ta.ctl = sl.ctl;
+        for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ta.lvl_base[j] = lvl_base[j]; ta.m[j] = (uint32_t)L.m[j]; ta.off[j] = (uint32_t)L.off[j]; }
+        ta.full = (uint32_t)L.full; ta.cap = E->ecap; ta.max_iter = n + 4u * (uint32_t)g + 64u;
+    }
+    sl.h_sd->phase = g > 1 ? PH_GROW : PH_TRACE;
+    sl.h_sd->k = 1;
+    sl.h_sd->g = (uint32_t)g;
+    sl.h_sd->pad = 0;
+    HIPCHK(hipMemcpyAsync(E->d_gargs + (size_t)si * NCTL, sl.h_gargs, sizeof(GrowArgs) * NCTL, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E->d_margs + (size_t)si * NCTL, sl.h_margs, sizeof(MergeArgs) * NCTL, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E->d_targs + si, sl.h_targs, sizeof(TraceArgs), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(E->d_slots + si, sl.h_sd, sizeof(SlotDev), hipMemcpyHostToDevice, s));
+    HIPCHK(hipGetLastError());
+    sl.busy = true;
+    sl.g = g;
+    sl.flt = flt;
+    return GST_OK;
+}
+
+// enqueue one step (plan/merge/compact/trace + R grow rounds)
+static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba, int R, bool flt) {
+    hipStream_t s = (hipStream_t)ctx->stream;
+    const int mgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 8);
+    const size_t mlds = MERGE_LDS_TILES + MERGE_LDS_PAIRS;
+    k_plan_step<<<1, WAVE, 0, s>>>(ba);
+    if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
+    else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
+    k_batch_compact<<<mgrid, 256, 0, s>>>(ba);
+    k_batch_trace<<<MAX_SLOTS, WAVE, 0, s>>>(ba);
+    for (int r = 0; r < R; ++r) {
+        k_plan_grow<<<1, WAVE, 0, s>>>(ba);
+        if (flt) k_batch_grow<true><<<ba.grid + MAX_SLOTS, GROW_BLOCK, 0, s>>>(ba);
+        else k_batch_grow<false><<<ba.grid + MAX_SLOTS, GROW_BLOCK, 0, s>>>(ba);
+    }
+}
+
+gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
+                         const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
+                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst) {
+    gst_batch_opts bo{};
+    if (bo_in) bo = *bo_in;
+    if (!ctx->be) ctx->be = new gst_batch_engine();
+    gst_batch_engine* E = ctx->be;
+    hipStream_t s = (hipStream_t)ctx->stream;
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t chunk = 32;
+    const double t0 = now_ms();
+    // slots: as asked, else as many as fit in half of the free device memory, at most 32
+    int S = bo.slots > 0 ? std::min<int>(bo.slots, (int)MAX_SLOTS) : 0;
+    {
+        uint32_t W = 1;
+        const gst_levels& L = gst_get_levels(ctx, g_cap);
+        for (int k = 1; k <= g_cap; ++k) W = std::max<uint32_t>(W, (uint32_t)((L.m[k] + 63) / 64));
+        const size_t per = slot_bytes(n_cap, g_cap, (uint64_t)n_cap + (uint64_t)m_cap / chunk + 64, W,
+                                      n_cap + 2u * (uint32_t)g_cap + 16u);
+        size_t fre = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fre, &tot));
+        if (E->S > 0) fre += (size_t)E->S * per;   // the engine's own slots would be reused
+        const int fit = (int)std::min<size_t>(MAX_SLOTS, fre / 2 / std::max<size_t>(per, 1));
+        if (S == 0) S = std::min(32, fit);
+        if (S < 1) return gst_set_error(ctx, GST_OOM, "batch: one slot needs %.2f GB of device memory", per / 1e9);
+    }
+    gst_status st = ensure_slots(ctx, E, S, n_cap, g_cap, m_cap, chunk);
+    if (st != GST_OK) return st;   // an engine with more slots is reused; its extra slots stay idle
+    const int R = bo.rounds_per_step > 0 ? bo.rounds_per_step : 4;
+    const bool use_graph = bo.use_graph != 0;
+
+    BatchArgs ba;
+    ba.slots = E->d_slots; ba.gargs = E->d_gargs; ba.margs = E->d_margs; ba.targs = E->d_targs; ba.plan = E->d_plan;
+    ba.S = (uint32_t)S;
+    ba.grid = (uint32_t)std::max(1, std::min(ctx->cu_count * 5, 64 * 32 - (int)MAX_SLOTS));
+    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)E->d_slots, 0u, sizeof(SlotDev) * MAX_SLOTS / 4);
+
+    std::vector<double> lat;
+    uint64_t steps = 0, n_done = 0, relax = 0, gbytes = 0, mbytes = 0;
+    gst_status first_err = GST_OK;
+    bool flt_set = false, flt = false;
+    bool source_open = true;
+    int busy = 0;
+    auto try_fill = [&](int si) -> gst_status {
+        while (source_open) {
+            gst_batch_job job;
+            if (!next_job(job)) { source_open = false; break; }
+            const bool jf = job.G->w_type == GST_W_FLOAT32;
+            if (!flt_set) { flt = jf; flt_set = true; }
+            if (jf != flt) {
+                job.status = gst_set_error(ctx, GST_INVALID_ARG, "batch: int and float graphs cannot share a batch");
+                job_done(job);
+                continue;
+            }
+            if ((uint32_t)job.G->n > E->n_cap || job.groups->n_groups > E->g_cap || job.groups->n_groups < 1) {
+                job.status = gst_set_error(ctx, GST_INVALID_ARG, "batch: query exceeds the engine's capacity");
+                job_done(job);
+                continue;
+            }
+            auto& sl = E->slots[(size_t)si];
+            sl.job = job;
+            sl.job.t_fill = now_ms();
+            gst_status fs = fill_slot(ctx, E, si, sl.job, chunk);
+            if (fs != GST_OK) return fs;
+            busy++;
+            return GST_OK;
+        }
+        return GST_OK;
+    };
+    for (int i = 0; i < S; ++i) {
+        st = try_fill(i);
+        if (st != GST_OK) return st;
+    }
+    bool graph_ready = false;
+    if (use_graph && busy > 0) {
+        if (E->gexec && E->g_S == S && E->g_R == R && E->g_flt == (int)flt) {
+            graph_ready = true;
+        } else {
+            if (E->gexec) { hipGraphExecDestroy(E->gexec); E->gexec = nullptr; }
+            if (E->graph) { hipGraphDestroy(E->graph); E->graph = nullptr; }
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueue_step(ctx, E, ba, R, flt);
+            HIPCHK(hipStreamEndCapture(s, &E->graph));
+            HIPCHK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
+            E->g_S = S; E->g_R = R; E->g_flt = (int)flt;
+            graph_ready = true;
+        }
+    }
+    // a step runs rounds_per_step grow rounds of every query in flight; every level converges, so
+    // this bound (minutes of work) only turns an engine defect into an error instead of a spin
+    const uint64_t max_steps = 2ull * 1000 * 1000;
+    while (busy > 0) {
+        if (graph_ready) HIPCHK(hipGraphLaunch(E->gexec, s));
+        else enqueue_step(ctx, E, ba, R, flt);
+        HIPCHK(hipGetLastError());
+        steps++;
+        HIPCHK(hipMemcpyAsync(E->h_slots, E->d_slots, sizeof(SlotDev) * (size_t)S, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (steps > max_steps) return gst_set_error(ctx, GST_INTERNAL, "batch: queries did not finish");
+        // finished queries: read root, control blocks, statistics and edge count, then the edges
+        std::vector<int> fin;
+        for (int i = 0; i < S; ++i)
+            if (E->slots[(size_t)i].busy && E->h_slots[i].phase == PH_DONE) fin.push_back(i);
+        if (fin.empty()) continue;
+        for (int i : fin) {
+            auto& sl = E->slots[(size_t)i];
+            unsigned char* hb = sl.h_back;
+            HIPCHK(hipMemcpyAsync(hb, sl.root, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hb + 8, sl.ctl, sizeof(RoundCtl) * NCTL, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hb + 8 + sizeof(RoundCtl) * NCTL, sl.stats, 8 * ST_NSTAT, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hb + 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT, sl.edges + 3 * (size_t)E->ecap, 4,
+                                  hipMemcpyDeviceToHost, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        for (int i : fin) {
+            auto& sl = E->slots[(size_t)i];
+            const unsigned char* hb = sl.h_back;
+            uint32_t nout = 0;
+            std::memcpy(&nout, hb + 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT, 4);
+            if (sl.job.opts.want_tree && nout > 0 && nout <= E->ecap) {
+                HIPCHK(hipMemcpyAsync(sl.h_edges, sl.edges, nout * 4ull, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipMemcpyAsync(sl.h_edges + E->ecap, sl.edges + E->ecap, nout * 4ull, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipMemcpyAsync(sl.h_edges + 2 * (size_t)E->ecap, sl.edges + 2 * (size_t)E->ecap, nout * 4ull,
+                                      hipMemcpyDeviceToHost, s));
+            }
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        for (int i : fin) {
+            auto& sl = E->slots[(size_t)i];
+            gst_batch_job& job = sl.job;
+            const unsigned char* hb = sl.h_back;
+            unsigned long long rw;
+            std::memcpy(&rw, hb, 8);
+            const RoundCtl* c = (const RoundCtl*)(hb + 8);
+            unsigned long long hs[ST_NSTAT];
+            std::memcpy(hs, hb + 8 + sizeof(RoundCtl) * NCTL, sizeof(hs));
+            uint32_t nout = 0;
+            std::memcpy(&nout, hb + 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT, 4);
+            uint32_t err = 0, rounds = 0;
+            for (int k = 0; k <= sl.g; ++k) err |= c[k].err;
+            for (int k = 1; k < sl.g; ++k) rounds += c[k].round + 1;
+            gst_result* res = job.res;
+            std::memset(res, 0, sizeof(*res));
+            res->root = -1;
+            res->n_groups = sl.g;
+            const bool has_ub = ub_bits_of(job.opts.ub, sl.flt) != GST_INF_COST;
+            res->used_ub = has_ub ? 1 : 0;
+            gst_stats& S2 = res->stats;
+            S2.grow_items = hs[ST_ITEMS]; S2.grow_arcs = hs[ST_ARCS]; S2.grow_relax = hs[ST_RELAX];
+            S2.grow_updates = hs[ST_UPD]; S2.grow_enqueued = hs[ST_ENQ];
+            S2.merge_vertices = hs[ST_MVERT]; S2.merge_pairs = hs[ST_MPAIRS];
+            S2.grow_rounds = rounds;
+            S2.grow_bytes = S2.grow_items * 24ull + S2.grow_arcs * 12ull + S2.grow_relax * 8ull + S2.grow_updates * 16ull;
+            {
+                const gst_levels& L = gst_get_levels(ctx, sl.g);
+                uint64_t mb = 0;
+                for (int k = 2; k <= sl.g; ++k) mb += (uint64_t)(L.off[k] + L.m[k]) * 8ull;
+                S2.merge_bytes = (uint64_t)job.G->n * mb;
+            }
+            relax += S2.grow_relax; gbytes += S2.grow_bytes; mbytes += S2.merge_bytes;
+            if (err & 1u) {
+                job.status = gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits");
+            } else if (rw == GST_INF_WORD) {
+                job.status = gst_set_error(ctx, GST_INFEASIBLE, has_ub ? "no tree of cost <= bound touches all groups"
+                                                                        : "no tree touches all groups");
+            } else if ((err & 2u) || (job.opts.want_tree && nout > E->ecap)) {
+                job.status = gst_set_error(ctx, GST_INTERNAL, "device traceback failed (edges=%u)", nout);
+            } else {
+                const uint32_t cb = (uint32_t)(rw >> 32);
+                if (sl.flt) { float f; std::memcpy(&f, &cb, 4); res->cost = (double)f; }
+                else res->cost = (double)cb;
+                res->root = (int32_t)(rw & 0xFFFFFFFFu);
+                if (job.opts.want_tree)
+                    gst_finish_tree(sl.flt, nout, sl.h_edges, sl.h_edges + E->ecap, sl.h_edges + 2 * (size_t)E->ecap, res);
+                job.status = GST_OK;
+            }
+            if (job.status != GST_OK && first_err == GST_OK) first_err = job.status;
+            if (job.status != GST_OK && (err & 1u)) {
+                // an aborted level can leave change flags behind: clear the slot's frontier state
+                const int fg = std::max(1, ctx->cu_count * 4);
+                for (int j = 0; j < 3; ++j) {
+                    k_fill_u64<<<fg, 256, 0, s>>>(sl.chg[j], 0ull, (uint64_t)E->n_cap * E->W_cap);
+                    k_fill_u32<<<fg, 256, 0, s>>>(sl.inq[j], 0u, E->n_cap);
+                }
+            }
+            job.t_done = now_ms();
+            lat.push_back(job.t_done - job.t_fill);
+            n_done++;
+            sl.busy = false;
+            busy--;
+            job_done(job);
+            // refill the slot, or park it (its phase block is written by exactly one of the two)
+            st = try_fill(i);
+            if (st != GST_OK) return st;
+            if (!sl.busy) {
+                sl.h_sd->phase = PH_EMPTY;
+                HIPCHK(hipMemcpyAsync(E->d_slots + i, sl.h_sd, sizeof(SlotDev), hipMemcpyHostToDevice, s));
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (bst) {
+        std::memset(bst, 0, sizeof(*bst));
+        bst->t_total_ms = now_ms() - t0;
+        bst->queries_per_s = bst->t_total_ms > 0 ? n_done * 1000.0 / bst->t_total_ms : 0.0;
+        if (!lat.empty()) {
+            std::sort(lat.begin(), lat.end());
+            auto pct = [&](double q) { return lat[std::min(lat.size() - 1, (size_t)std::floor(q * (lat.size() - 1) + 0.5))]; };
+            bst->latency_p50_ms = pct(0.50);
+            bst->latency_p99_ms = pct(0.99);
+        }
+        bst->steps = steps;
+        bst->grow_launches = steps * (uint64_t)R;
+        bst->grow_relax = relax;
+        bst->grow_bytes = gbytes;
+        bst->merge_bytes = mbytes;
+        bst->slots = S;
+        bst->rounds_per_step = R;
+        bst->used_graph = graph_ready ? 1 : 0;
+        bst->queries = (int64_t)n_done;
+    }
+    return first_err;
+}

@@ -34,1310 +34,9 @@
                                  hipGetErrorString(e_), __FILE__, __LINE__);                 \
     } while (0)
 
-namespace {
+#include "gst_batch.h"
+#include "gst_dpbf_dev.h"
 
-constexpr int WAVE = 64;
-constexpr int GROW_BLOCK = 256;              // 4 waves
-constexpr int GROW_WAVES = GROW_BLOCK / WAVE;
-constexpr int TRACE_STACK = 512;
-constexpr int NCTL = GST_MAX_GROUPS + 2;     // one round-control block per level (entry 0: query flags)
-
-// device-side control block of the grow rounds of one level
-struct RoundCtl {
-    union {
-        struct {
-            uint32_t round;
-            uint32_t seq;   // launch number of the last finished round
-        };
-        unsigned long long rsq;   // (seq << 32) | round, read and written as one word
-    };
-    uint32_t cnt[3];        // items in list r%3
-    uint32_t level_done;
-    uint32_t err;           // bit0: int cost overflow, bit1: traceback failure
-    uint32_t top;           // arrival counter over block groups
-    uint32_t sub[64];       // arrival counters of groups of 32 workgroups
-};
-
-enum StatIdx {
-    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_MVERT, ST_MPAIRS, ST_NSTAT
-};
-
-struct GrowArgs {
-    uint64_t* Tk;          // level-k block
-    const uint64_t* T1;    // level-1 block (group distances, for the lower bound)
-    const uint32_t* masks; // masks of level k by rank
-    const uint32_t* row;
-    const int32_t* col;
-    const uint32_t* w;
-    const uint32_t* rev;
-    uint64_t* list[3];
-    uint64_t* chg[3];
-    uint32_t* inq[3];
-    RoundCtl* ctl;
-    unsigned long long* stats;
-    uint32_t m;            // masks at this level
-    uint32_t W;            // 64-bit bitmap words per vertex
-    uint32_t g;
-    uint32_t chunk;        // arcs per work item
-    uint32_t ub;           // cost bits of the bound (GST_INF_COST: none)
-    uint32_t use_lb;
-    const uint64_t* all_items;  // (vertex, chunk) items of the hub rows, for pull rounds
-    uint32_t n_all;
-    uint32_t n;            // vertices
-    uint32_t pull_thresh;  // a round with at least this many queued items pulls over all items
-    // asynchronous (work-queue) mode
-    unsigned long long* ring;   // 4 words per task slot
-    uint32_t ring_mask;         // slots - 1 (power of two)
-    struct AsyncCtl* actl;
-    uint32_t* rlog;             // optional round log (RLOG_ROUNDS x 4 words), nullptr: off
-    uint32_t seq;               // launch number within the level (1, 2, ...), set per launch
-};
-
-// round log (GST_TRACE_ROUNDS=1): per round {queued items | pull<<31, workgroups taking part,
-// wall clock at start (block 0), wall clock when the last workgroup finished}
-constexpr uint32_t RLOG_ROUNDS = 512;
-
-// device-side control of an asynchronous grow launch
-struct AsyncCtl {
-    uint32_t head;      // next task index to claim
-    uint32_t tail;      // next task index to reserve
-    uint32_t pending;   // tasks reserved (or initial) and not finished
-    uint32_t err;       // bit0: int cost overflow, bit2: wait bound exceeded, bit3: ring overflow
-    uint32_t pad[4];
-};
-
-struct MergeArgs {
-    uint64_t* T;           // whole table
-    const uint64_t* pairs; // pairs of this level
-    const uint32_t* masks; // all masks by row position
-    const uint32_t* gm;    // group mask per vertex
-    const uint32_t* row;
-    uint64_t* list0;
-    uint64_t* chg0;
-    uint32_t* inq0;
-    RoundCtl* ctl;
-    unsigned long long* root;
-    unsigned long long* stats;
-    uint64_t lvl_base[GST_MAX_GROUPS + 2];  // word offset of each level block
-    uint32_t m[GST_MAX_GROUPS + 2];
-    uint32_t off[GST_MAX_GROUPS + 2];
-    uint32_t n;
-    uint32_t g;
-    uint32_t k;
-    uint32_t P;            // pairs at this level
-    uint32_t W;            // bitmap words at this level
-    uint32_t chunk;
-    uint32_t ub;
-    uint32_t use_lb;
-    uint32_t is_full;
-    uint32_t VB;           // vertices per workgroup tile
-    uint32_t pairs_in_lds;
-};
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-template <bool FLT>
-__device__ __forceinline__ uint32_t add_cost(uint32_t a, uint32_t b, uint32_t& ovf) {
-    if (a == GST_INF_COST || b == GST_INF_COST) return GST_INF_COST;
-    if (FLT) {
-        return __float_as_uint(__fadd_rn(__uint_as_float(a), __uint_as_float(b)));
-    } else {
-        const uint64_t s = (uint64_t)a + (uint64_t)b;
-        if (s >= (uint64_t)GST_INF_COST) { ovf = 1; return GST_INF_COST; }
-        return (uint32_t)s;
-    }
-}
-
-// a + b >= ub, exact for ints, in float arithmetic for floats
-template <bool FLT>
-__device__ __forceinline__ bool reaches_ub(uint32_t a, uint32_t b, uint32_t ub) {
-    if (ub == GST_INF_COST) return a == GST_INF_COST || b == GST_INF_COST;
-    if (a == GST_INF_COST || b == GST_INF_COST) return true;
-    if (FLT) return __fadd_rn(__uint_as_float(a), __uint_as_float(b)) >= __uint_as_float(ub);
-    return (uint64_t)a + (uint64_t)b >= (uint64_t)ub;
-}
-
-// Pull: can an arc of weight w still lower (or tie with a smaller predecessor) a state of cost
-// ocost and lower bound lb under the bound ub?  Every candidate through the arc costs at least w,
-// so a false answer holds for every heavier arc of the row as well.
-template <bool FLT>
-__device__ __forceinline__ bool pull_arc_live(uint32_t w, uint32_t ocost, uint32_t lb, uint32_t ub) {
-    if (w > ocost) return false;
-    return ub == GST_INF_COST || !reaches_ub<FLT>(w, lb, ub);
-}
-
-// position of the t-th (0-based) set bit of x; t < popcount(x)
-__device__ __forceinline__ uint32_t select_bit(uint64_t x, uint32_t t) {
-    uint32_t lo = 0;
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
-        const uint32_t mid = lo + s - 1;
-        const uint64_t msk = (mid >= 63) ? ~0ull : ((2ull << mid) - 1ull);
-        if ((uint32_t)__popcll(x & msk) <= t) lo += s;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) v |= __shfl_xor(v, d, WAVE);
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long wave_min64(unsigned long long v) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        unsigned long long o = __shfl_xor(v, d, WAVE);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) v += __shfl_xor(v, d, WAVE);
-    return v;
-}
-
-// Work-item queue of one workgroup in LDS.  Waves reserve slots with an LDS atomic (one per
-// wave and step), and the workgroup appends its items to the global list with a single global
-// atomicAdd at the end, so the global counter sees one atomic per workgroup instead of one per
-// wave-step.  A reservation that does not fit goes straight to the global list.
-constexpr uint32_t QCAP = 2048;
-struct BlockQueue {
-    uint64_t items[QCAP];
-    uint32_t n;       // slots reserved
-    uint32_t valid;   // items[0, min(n, valid)) hold queued items
-    uint32_t gbase;
-};
-
-__device__ __forceinline__ void bq_init(BlockQueue& q) {
-    if (threadIdx.x == 0) { q.n = 0; q.valid = QCAP; }
-}
-
-// Append n items (v, chunk 0..n-1) per lane.  Convergent: every lane of the wave calls it.
-__device__ __forceinline__ void bq_push(BlockQueue& q, uint64_t* list, uint32_t* cnt, uint32_t v,
-                                        uint32_t n, uint32_t& enq_stat) {
-    if (__ballot(n != 0) == 0) return;
-    const uint32_t lane = lane_id();
-    uint32_t incl = n;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const uint32_t y = __shfl_up(incl, d, WAVE);
-        if (lane >= (uint32_t)d) incl += y;
-    }
-    const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-    uint32_t b = 0, gb = 0xFFFFFFFFu;
-    if (lane == 0) {
-        b = atomicAdd(&q.n, total);
-        if (b + total > QCAP) {
-            atomicMin(&q.valid, b);
-            gb = atomicAdd(cnt, total);
-        }
-    }
-    b = __shfl(b, 0, WAVE);
-    gb = __shfl(gb, 0, WAVE);
-    const uint32_t off = incl - n;
-    for (uint32_t c = 0; c < n; ++c) {
-        const uint64_t item = (uint64_t)v | ((uint64_t)c << 32);
-        if (gb == 0xFFFFFFFFu) q.items[b + off + c] = item;
-        else list[gb + off + c] = item;
-    }
-    enq_stat += n;
-}
-
-// Publish the workgroup's queued items (all threads of the block call it).
-__device__ __forceinline__ void bq_flush(BlockQueue& q, uint64_t* list, uint32_t* cnt) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t k = min(q.n, q.valid);
-        q.gbase = k ? atomicAdd(cnt, k) : 0u;
-        q.n = k;
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < q.n; i += blockDim.x) list[q.gbase + i] = q.items[i];
-}
-
-__device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, uint32_t chunk) {
-    const uint32_t d = row[v + 1] - row[v];
-    return (d + chunk - 1) / chunk;
-}
-
-// One-label lower bound of state (v, X): max over groups i not in X of dist(v, G_i), read from the
-// (final) level-1 row of v; inf when some missing group is unreachable within the bound.
-__device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, uint32_t v, uint32_t g, uint32_t X) {
-    uint32_t lb = 0;
-    const uint64_t* r1 = T1 + (uint64_t)v * g;
-    for (uint32_t gi = 0; gi < g; ++gi) {
-        if ((X >> gi) & 1u) continue;
-        const uint32_t di = (uint32_t)(r1[gi] >> 32);
-        if (di == GST_INF_COST) return GST_INF_COST;
-        lb = max(lb, di);
-    }
-    return lb;
-}
-
-constexpr uint32_t UNR = 4;   // arc steps issued together (independent gathers in flight per wave)
-// Rows longer than this are "hubs": pull rounds split them into chunk items combined with atomics;
-// shorter rows are pulled whole by one lane group with plain stores.
-constexpr uint32_t HUB_DEG = 128;
-
-// ---------------------------------------------------------------------------------------
-// small utility kernels (fills instead of hipMemset so that every step can be graph-captured)
-// ---------------------------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t* p, uint32_t val, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = val;
-}
-
-__global__ void k_fill_u64(uint64_t* p, uint64_t val, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = val;
-}
-
-__global__ void k_set_group_bits(uint32_t* gm, const int32_t* grp_ptr, const int32_t* grp_vtx, int g) {
-    for (int i = 0; i < g; ++i) {
-        const int s = grp_ptr[i], e = grp_ptr[i + 1];
-        for (int j = s + blockIdx.x * blockDim.x + threadIdx.x; j < e; j += gridDim.x * blockDim.x)
-            atomicOr(&gm[grp_vtx[j]], 1u << i);
-    }
-}
-
-__global__ void k_ctl_reset(RoundCtl* ctl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        ctl->round = 0; ctl->seq = 0;
-        ctl->cnt[0] = 0; ctl->cnt[1] = 0; ctl->cnt[2] = 0;
-        ctl->level_done = 0;
-    }
-}
-
-// Level 1: T(v,{i}) = 0 for members of group i, inf elsewhere; members with arcs are flagged as
-// the first frontier (bitmap = their groups).
-__global__ void k_init_level1(uint64_t* T1, const uint32_t* gm, const uint32_t* row, uint32_t n,
-                              uint32_t g, uint64_t* chg0, uint32_t* inq0, int seed_frontier) {
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
-         v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t gmv = gm[v];
-        for (uint32_t i = 0; i < g; ++i)
-            T1[v * g + i] = ((gmv >> i) & 1u) ? (uint64_t)GST_PRED_LEAF : GST_INF_WORD;
-        if (gmv && seed_frontier && row[v + 1] > row[v]) { chg0[v] = gmv; inq0[v] = 1; }
-    }
-}
-
-// Work-item list from per-vertex flags (flags == nullptr: every vertex with arcs), one item per
-// chunk of the vertex's row; one global atomic per workgroup.
-__global__ void __launch_bounds__(256) k_compact(const uint32_t* flags, const uint32_t* row, uint32_t n,
-                                                 uint32_t chunk, uint64_t* list, uint32_t* cnt_out,
-                                                 unsigned long long* stats, uint32_t min_deg) {
-    __shared__ uint32_t s_wtot[4];
-    __shared__ uint32_t s_base;
-    const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t v = base + threadIdx.x;
-        uint32_t k = 0;
-        if (v < n && (flags == nullptr || flags[v]) && row[v + 1] - row[v] > min_deg)
-            k = (row[v + 1] - row[v] + chunk - 1) / chunk;
-        uint32_t incl = k;
-#pragma unroll
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const uint32_t y = __shfl_up(incl, d, WAVE);
-            if (lane >= (uint32_t)d) incl += y;
-        }
-        if (lane == WAVE - 1) s_wtot[wid] = incl;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t tot = s_wtot[0] + s_wtot[1] + s_wtot[2] + s_wtot[3];
-            s_base = tot ? atomicAdd(cnt_out, tot) : 0u;
-            if (tot && stats) atomicAdd(&stats[ST_ENQ], (unsigned long long)tot);
-        }
-        __syncthreads();
-        uint32_t off = s_base;
-        for (uint32_t w = 0; w < wid; ++w) off += s_wtot[w];
-        off += incl - k;
-        for (uint32_t c = 0; c < k; ++c) list[off + c] = v | ((uint64_t)c << 32);
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// grow: one round of frontier relaxation for all masks of one level
-// ---------------------------------------------------------------------------------------
-template <bool FLT>
-__global__ void __launch_bounds__(GROW_BLOCK) k_grow_round(GrowArgs a) {
-    __shared__ BlockQueue q;
-    __shared__ unsigned long long s_stats[ST_NSTAT];
-    __shared__ uint32_t s_ovf;
-    RoundCtl* ctl = a.ctl;
-    if (ctl->level_done) return;
-    // (round, seq) is one 8-byte word: a workgroup that starts only after this launch's round has
-    // ended (its participants all arrived and advanced the round) sees seq == a.seq and stays out
-    const unsigned long long rsq =
-        __hip_atomic_load(&ctl->rsq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)(rsq >> 32) == a.seq) return;
-    const uint32_t r = (uint32_t)rsq;
-    const uint32_t cur = r % 3, nxt = (r + 1) % 3, prv = (r + 2) % 3;
-    const uint32_t n_cur = ctl->cnt[cur];
-    const uint32_t n_prv = ctl->cnt[prv];
-    const uint32_t lane = lane_id();
-    const uint32_t W = a.W, m = a.m;
-    const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
-    // only as many workgroups as the round needs take part (a sparse round costs a few blocks)
-    uint32_t part = gridDim.x;
-    if (!pull) {
-        const uint32_t need_b = (n_cur + GROW_WAVES - 1) / GROW_WAVES;
-        const uint32_t need_a = (n_prv + GROW_BLOCK - 1) / GROW_BLOCK;
-        part = max(1u, min(gridDim.x, max(need_b, need_a)));
-    }
-    if (a.rlog && blockIdx.x == 0 && threadIdx.x == 0 && r < RLOG_ROUNDS) {
-        a.rlog[4 * r + 0] = n_cur | (pull ? 0x80000000u : 0u);
-        a.rlog[4 * r + 1] = part;
-        a.rlog[4 * r + 2] = (uint32_t)wall_clock64();
-    }
-    if (blockIdx.x >= part) return;
-    bq_init(q);
-    if (threadIdx.x < ST_NSTAT) s_stats[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_ovf = 0;
-
-    // phase A: clear the change bitmaps / queue flags of round r-1's items
-    {
-        const uint64_t* lp = a.list[prv];
-        uint64_t* cp = a.chg[prv];
-        uint32_t* ip = a.inq[prv];
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_prv; i += part * blockDim.x) {
-            const uint64_t it = lp[i];
-            if ((it >> 32) == 0) {
-                const uint32_t v = (uint32_t)it;
-                for (uint32_t w = 0; w < W; ++w) cp[(uint64_t)v * W + w] = 0;
-                ip[v] = 0;
-            }
-        }
-    }
-    __syncthreads();
-
-    // phase B: relax the items of round r
-    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0;
-    uint32_t ovf = 0;
-    const uint32_t gwave = blockIdx.x * GROW_WAVES + (threadIdx.x / WAVE);
-    const uint32_t nwaves = part * GROW_WAVES;
-    const uint64_t* lcur = a.list[cur];
-    const uint64_t* ccur = a.chg[cur];
-    uint64_t* cnxt = a.chg[nxt];
-    uint32_t* inxt = a.inq[nxt];
-    uint64_t* lnxt = a.list[nxt];
-    uint32_t* cntn = &ctl->cnt[nxt];
-    const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
-
-    // ---- pull round, light rows: each lane group owns one row and each lane one mask of it ----
-    // (tropical SpMM row-split: the lane scans the whole row, so no cross-lane reduction, and the
-    // row has a single writer, so plain stores and a ballot-built change word replace atomics)
-    if (pull) {
-        const uint32_t mchunk = m < (uint32_t)WAVE ? m : (uint32_t)WAVE;
-        const uint32_t G = WAVE / mchunk;                 // rows per wave
-        const uint32_t gslot = lane / mchunk, t0 = lane % mchunk;
-        const bool slot_ok = gslot < G;
-        for (uint32_t base = gwave * G; base < a.n; base += nwaves * G) {
-            const uint32_t u = base + gslot;
-            const bool row_ok = slot_ok && u < a.n;
-            uint32_t rs = 0, re = 0;
-            if (row_ok) { rs = a.row[u]; re = a.row[u + 1]; }
-            const bool light = row_ok && re > rs && re - rs <= HUB_DEG;
-            const uint32_t deg = light ? re - rs : 0u;
-            st_items += wave_sum32(light && t0 == 0 ? 1u : 0u);
-            st_arcs += wave_sum32(light && t0 == 0 ? deg : 0u);
-            uint32_t maxdeg = deg;
-#pragma unroll
-            for (int d = 1; d < WAVE; d <<= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor(maxdeg, d, WAVE));
-            for (uint32_t mb = 0; mb < m; mb += mchunk) {
-                const uint32_t rk = mb + t0;
-                const bool lane_ok = light && rk < m;
-                unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-                const unsigned long long own = lane_ok ? *addr : 0ull;
-                const uint32_t ocost = (uint32_t)(own >> 32);
-                const uint32_t lb = (prune_lb && lane_ok) ? label_lb(a.T1, u, a.g, a.masks[rk]) : 0u;
-                unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;
-                for (uint32_t s = 0; s < maxdeg; s += UNR) {
-                    uint32_t nb[UNR], wt[UNR], cn[UNR];
-                    bool live[UNR];
-#pragma unroll
-                    for (uint32_t j = 0; j < UNR; ++j) {
-                        const bool ok = lane_ok && s + j < deg;
-                        wt[j] = ok ? a.w[rs + s + j] : 0u;
-                        live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
-                    }
-                    // rows run in increasing weight: when the step's first arc is dead for every
-                    // lane, the rest of every row is dead too
-                    if (__ballot(live[0]) == 0) break;
-#pragma unroll
-                    for (uint32_t j = 0; j < UNR; ++j) nb[j] = live[j] ? (uint32_t)a.col[rs + s + j] : 0xFFFFFFFFu;
-#pragma unroll
-                    for (uint32_t j = 0; j < UNR; ++j)
-                        cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
-#pragma unroll
-                    for (uint32_t j = 0; j < UNR; ++j) {
-                        if (nb[j] == 0xFFFFFFFFu) continue;
-                        st_relax++;
-                        if (cn[j] < ocost) {
-                            const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
-                            if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
-                                const unsigned long long nw = ((unsigned long long)cand << 32) | (rs + s + j);
-                                if (cand > cn[j]) bs = nw < bs ? nw : bs;
-                                else if (cand < ocost) bz = nw < bz ? nw : bz;
-                            }
-                        }
-                    }
-                }
-                // single writer: zero-effect candidates only on a strict decrease, strict ones by min
-                unsigned long long nwv = own;
-                if (lane_ok) {
-                    if (bz != GST_INF_WORD && (uint32_t)(bz >> 32) < (uint32_t)(bs >> 32)) nwv = bz;
-                    else if (bs < own) nwv = bs;
-                }
-                const bool improved = lane_ok && (uint32_t)(nwv >> 32) < ocost;
-                if (lane_ok && nwv != own) *addr = nwv;
-                const uint64_t imp = __ballot(improved);
-                uint32_t enq_v = 0, enq_n = 0;
-                if (lane_ok && t0 == 0) {
-                    const uint64_t word = (imp >> (gslot * mchunk)) & (mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1));
-                    if (word) {
-                        st_upd += (uint32_t)__popcll(word);
-                        // single writer of u's change word in a pull round; the flag marks the
-                        // first masks block that changed
-                        cnxt[(uint64_t)u * W + (mb >> 6)] = word;
-                        const bool first = (W == 1) ? true : (atomicExch(&inxt[u], 1u) == 0u);
-                        if (first) { enq_v = u; enq_n = (deg + a.chunk - 1) / a.chunk; }
-                    }
-                }
-                bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
-            }
-        }
-    }
-
-    // ---- pull round, hub rows: (row, chunk) items, lanes = (arc slot, mask), combined by atomics ----
-    for (uint32_t i = gwave; pull && i < a.n_all; i += nwaves) {
-        const uint64_t it = a.all_items[i];
-        const uint32_t u = (uint32_t)it;
-        const uint32_t c = (uint32_t)(it >> 32);
-        const uint32_t rs = a.row[u], re = a.row[u + 1];
-        const uint32_t e0 = rs + c * a.chunk;
-        const uint32_t e1 = min(re, e0 + a.chunk);
-        st_items++;
-        st_arcs += e1 - e0;
-        for (uint32_t mb = 0; mb < m; mb += WAVE) {
-            const uint32_t na = min((uint32_t)WAVE, m - mb);
-            const uint32_t A = WAVE / na;
-            const uint32_t t = lane % na, s_arc = lane / na;
-            const bool lane_ok = s_arc < A;
-            const uint32_t rk = mb + t;
-            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-            const unsigned long long own = *addr;
-            const uint32_t ocost = (uint32_t)(own >> 32);
-            const uint32_t lb = prune_lb ? label_lb(a.T1, u, a.g, a.masks[rk]) : 0u;
-            unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;  // best via positive / zero-effect edges
-            const uint32_t len = e1 - e0;
-            const uint32_t steps = (len + A - 1) / A;
-            for (uint32_t s = 0; s < steps; s += UNR) {
-                uint32_t nb[UNR], wt[UNR], cn[UNR];
-                bool live[UNR];
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    const uint32_t e = e0 + (s + j) * A + s_arc;
-                    const bool ok = lane_ok && e < e1;
-                    wt[j] = ok ? a.w[e] : 0u;
-                    live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
-                }
-                if (__ballot(live[0]) == 0) break;   // each lane's arcs run in increasing weight
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j)
-                    nb[j] = live[j] ? (uint32_t)a.col[e0 + (s + j) * A + s_arc] : 0xFFFFFFFFu;
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j)
-                    cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    if (nb[j] == 0xFFFFFFFFu) continue;
-                    st_relax++;
-                    if (cn[j] < ocost) {  // only a cheaper neighbour can lower (or tie) our cost
-                        const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
-                        if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
-                            const uint32_t e = e0 + (s + j) * A + s_arc;
-                            const unsigned long long nw = ((unsigned long long)cand << 32) | e;
-                            if (cand > cn[j]) bs = nw < bs ? nw : bs;
-                            else if (cand < ocost) bz = nw < bz ? nw : bz;
-                        }
-                    }
-                }
-            }
-            for (uint32_t k2 = 1; k2 < A; k2 <<= 1) {
-                const unsigned long long y1 = __shfl_down(bs, k2 * na, WAVE);
-                const unsigned long long y2 = __shfl_down(bz, k2 * na, WAVE);
-                if ((s_arc % (2 * k2)) == 0 && s_arc + k2 < A) {
-                    bs = y1 < bs ? y1 : bs;
-                    bz = y2 < bz ? y2 : bz;
-                }
-            }
-            uint32_t enq_v = 0, enq_n = 0;
-            if (s_arc == 0) {
-                bool improved = false;
-                if (bz != GST_INF_WORD && (uint32_t)(bz >> 32) < (uint32_t)(bs >> 32)) {
-                    // a zero-effect edge wins: accept only a strict cost decrease
-                    const uint32_t cz = (uint32_t)(bz >> 32);
-                    unsigned long long curw = own;
-                    while (cz < (uint32_t)(curw >> 32)) {
-                        const unsigned long long old = atomicCAS(addr, curw, bz);
-                        if (old == curw) { improved = true; break; }
-                        curw = old;
-                    }
-                } else if (bs < own) {
-                    const unsigned long long old = atomicMin(addr, bs);
-                    improved = (uint32_t)(bs >> 32) < (uint32_t)(old >> 32);
-                }
-                if (improved) {
-                    st_upd++;
-                    const uint64_t bitv = 1ull << (rk & 63);
-                    unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + (rk >> 6)];
-                    const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
-                    const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
-                    if (first) {
-                        enq_v = u;
-                        enq_n = num_chunks(a.row, u, a.chunk);
-                    }
-                }
-            }
-            bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
-        }
-    }
-
-    // ---- push round: queued vertices relax their changed masks into their neighbours ----
-    for (uint32_t i = gwave; !pull && i < n_cur; i += nwaves) {
-        const uint64_t it = lcur[i];
-        const uint32_t v = (uint32_t)it;
-        const uint32_t c = (uint32_t)(it >> 32);
-        const uint32_t rs = a.row[v], re = a.row[v + 1];
-        const uint32_t e0 = rs + c * a.chunk;
-        const uint32_t e1 = min(re, e0 + a.chunk);
-        st_items++;
-        st_arcs += e1 - e0;
-        for (uint32_t wd = 0; wd < W; ++wd) {
-            uint64_t bm = ccur[(uint64_t)v * W + wd];
-            if (bm == 0) continue;
-            // drop masks whose cost (plus lower bound) reaches the bound
-            if (a.ub != GST_INF_COST) {
-                const uint32_t na0 = (uint32_t)__popcll(bm);
-                uint64_t keep = 0;
-                if (lane < na0) {
-                    const uint32_t bit = select_bit(bm, lane);
-                    const uint32_t rk = wd * 64 + bit;
-                    const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
-                    const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
-                    if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
-                }
-                bm = wave_or64(keep);
-            }
-            const uint32_t na = (uint32_t)__popcll(bm);
-            if (na == 0) continue;
-            const uint32_t A = WAVE / na;           // arc slots per step
-            const uint32_t t = lane % na, s_arc = lane / na;
-            const bool lane_ok = s_arc < A;
-            const uint32_t rk = wd * 64 + select_bit(bm, t);
-            const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
-            const uint64_t bitv = 1ull << (rk & 63);
-            const uint32_t wsel = rk >> 6;
-            const uint32_t len = e1 - e0;
-            const uint32_t steps = (len + A - 1) / A;
-            for (uint32_t s = 0; s < steps; s += UNR) {
-                uint32_t uu[UNR], wt[UNR], ra[UNR];
-                unsigned long long cw0[UNR];
-                bool live[UNR];
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    const uint32_t e = e0 + (s + j) * A + s_arc;
-                    const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-                    wt[j] = ok ? a.w[e] : 0u;
-                    // under a bound, cv + w >= ub holds for every heavier arc of the row as well
-                    live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
-                }
-                if (__ballot(live[0]) == 0) break;
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    const uint32_t e = e0 + (s + j) * A + s_arc;
-                    uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
-                    ra[j] = live[j] ? a.rev[e] : 0u;
-                }
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j)
-                    cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    uint32_t enq_v = 0, enq_n = 0;
-                    const uint32_t u = uu[j];
-                    if (u != 0xFFFFFFFFu) {
-                        st_relax++;
-                        const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
-                        unsigned long long curw = cw0[j];
-                        if (cand != GST_INF_COST && cand <= (uint32_t)(curw >> 32) &&
-                            (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
-                            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-                            const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
-                            bool improved = false;
-                            if (nw < curw) {
-                                if (cand == cv) {
-                                    // effective zero-weight edge: only strict cost decreases, so
-                                    // equal-cost predecessors can never close a cycle
-                                    while (cand < (uint32_t)(curw >> 32)) {
-                                        const unsigned long long old = atomicCAS(addr, curw, nw);
-                                        if (old == curw) { improved = true; break; }
-                                        curw = old;
-                                    }
-                                } else {
-                                    const unsigned long long old = atomicMin(addr, nw);
-                                    improved = cand < (uint32_t)(old >> 32);
-                                }
-                            }
-                            if (improved) {
-                                st_upd++;
-                                unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
-                                if (!(*cwp & bitv)) {
-                                    const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
-                                    // first change of u in this round: queue it (one word per
-                                    // vertex: the word itself was empty; several words: flag)
-                                    const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
-                                    if (first) {
-                                        enq_v = u;
-                                        enq_n = num_chunks(a.row, u, a.chunk);
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
-                }
-            }
-        }
-    }
-    bq_flush(q, lnxt, cntn);
-
-    // statistics: LDS reduction, one global atomic per workgroup and counter
-    {
-        const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
-        const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
-        const uint32_t sr = wave_sum32(st_relax);
-        const uint32_t su = wave_sum32(st_upd);
-        const uint32_t se = wave_sum32(st_enq);
-        const uint32_t so = wave_sum32(ovf);
-        if (lane == 0) {
-            if (si) atomicAdd(&s_stats[ST_ITEMS], (unsigned long long)si);
-            if (sa) atomicAdd(&s_stats[ST_ARCS], (unsigned long long)sa);
-            if (sr) atomicAdd(&s_stats[ST_RELAX], (unsigned long long)sr);
-            if (su) atomicAdd(&s_stats[ST_UPD], (unsigned long long)su);
-            if (se) atomicAdd(&s_stats[ST_ENQ], (unsigned long long)se);
-            if (so) s_ovf = 1;
-        }
-        __syncthreads();
-        if (threadIdx.x < ST_MVERT && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
-        if (threadIdx.x == 0 && s_ovf) atomicOr(&ctl->err, 1u);
-    }
-
-    // end of round: the last workgroup to arrive (two-level count: groups of 32, then groups)
-    // resets the counter of the list cleaned in phase A and advances the round, or marks the level
-    // finished when this round had no items.  The decision uses only values every workgroup read
-    // at kernel start, and the next launch sees these stores at the kernel boundary, so no
-    // device-scope fence (an L2 write-back on this chip) is needed here.
-    if (threadIdx.x == 0) {
-        const uint32_t grp = blockIdx.x >> 5;
-        const uint32_t gsize = min(32u, part - (grp << 5));
-        if (atomicAdd(&ctl->sub[grp], 1u) == gsize - 1) {
-            ctl->sub[grp] = 0;
-            const uint32_t ngrp = (part + 31) >> 5;
-            if (atomicAdd(&ctl->top, 1u) == ngrp - 1) {
-                ctl->top = 0;
-                if (a.rlog && r < RLOG_ROUNDS) a.rlog[4 * r + 3] = (uint32_t)wall_clock64();
-                if (n_cur == 0) {
-                    ctl->level_done = 1;
-                    __hip_atomic_store(&ctl->rsq, ((unsigned long long)a.seq << 32) | r,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                } else {
-                    ctl->cnt[prv] = 0;
-                    __hip_atomic_store(&ctl->rsq, ((unsigned long long)a.seq << 32) | (r + 1),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// grow, asynchronous: one persistent launch finishes a level from its current frontier with a
-// device work queue instead of rounds.  Tasks:
-//   VERTEX v          take v's change bitmap (atomic exchange) and relax its arcs for those masks,
-//                     or, for a row longer than one chunk, split it into CHUNK tasks
-//   CHUNK v,c,wd,bm   relax arcs [c*chunk, ...) of v for the masks in word wd of bm
-// Task index i < n0 is item i of the current round list (chunk-0 items are VERTEX tasks, the rest
-// are no-ops: the vertex task covers the row); later tasks live in a ring of 4-word slots, every
-// word tagged with the task index, so a reader knows a slot is complete without any fence.
-// pending counts tasks reserved and not finished: it is raised before a task becomes visible
-// and lowered after the task's own enqueues, so pending == 0 means the level is finished.
-// Every wait is bounded: past the bound the launch reports an error instead of spinning on.
-// ---------------------------------------------------------------------------------------
-constexpr uint32_t TASK_VERTEX = 0, TASK_CHUNK = 1;
-constexpr uint32_t ASYNC_BATCH = 4;           // tasks claimed per wave and claim
-constexpr uint32_t ASYNC_MAX_WAIT = 1u << 21; // polls of one slot before giving up (~seconds)
-
-__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld_agent32(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// write one task into the ring (caller reserved index idx >= n0)
-__device__ __forceinline__ void task_write(const GrowArgs& a, uint32_t n0, uint32_t idx, uint32_t kind, uint32_t v,
-                                           uint32_t c, uint32_t wd, uint64_t bm) {
-    unsigned long long* sl = a.ring + 4ull * ((idx - n0) & a.ring_mask);
-    const unsigned long long tag = (unsigned long long)idx << 32;
-    st_agent(sl + 0, tag | v);
-    st_agent(sl + 1, tag | ((unsigned long long)c << 9) | ((unsigned long long)wd << 1) | kind);
-    if (kind == TASK_CHUNK) {
-        st_agent(sl + 2, tag | (bm & 0xFFFFFFFFull));
-        st_agent(sl + 3, tag | (bm >> 32));
-    }
-}
-
-// relax arcs [e0,e1) of v for the masks set in word wd of bm; queue improved neighbours.
-// Convergent: all lanes of the wave call it with the same arguments.
-template <bool FLT>
-__device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t e0, uint32_t e1, uint32_t wd,
-                           uint64_t bm, uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq, uint32_t& ovf) {
-    const uint32_t lane = lane_id();
-    const uint32_t m = a.m, W = a.W;
-    uint64_t* chg = a.chg[a.ctl->round % 3];
-    uint32_t* inq = a.inq[a.ctl->round % 3];
-    const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
-    // drop masks at or above the bound
-    if (a.ub != GST_INF_COST) {
-        const uint32_t na0 = (uint32_t)__popcll(bm);
-        uint64_t keep = 0;
-        if (lane < na0) {
-            const uint32_t bit = select_bit(bm, lane);
-            const uint32_t rk = wd * 64 + bit;
-            const uint32_t cv = (uint32_t)(ld_agent((const unsigned long long*)(a.Tk + (uint64_t)v * m + rk)) >> 32);
-            const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
-            if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
-        }
-        bm = wave_or64(keep);
-    }
-    const uint32_t na = (uint32_t)__popcll(bm);
-    if (na == 0) return;
-    const uint32_t A = WAVE / na;
-    const uint32_t t = lane % na, s_arc = lane / na;
-    const bool lane_ok = s_arc < A;
-    const uint32_t rk = wd * 64 + select_bit(bm, t);
-    // own cost, read at the coherence point (it may have just been lowered on another XCD)
-    const uint32_t cv = (uint32_t)(ld_agent((const unsigned long long*)(a.Tk + (uint64_t)v * m + rk)) >> 32);
-    const uint64_t bitv = 1ull << (rk & 63);
-    const uint32_t wsel = rk >> 6;
-    const uint32_t steps = (e1 - e0 + A - 1) / A;
-    for (uint32_t s = 0; s < steps; s += UNR) {
-        uint32_t uu[UNR], wt[UNR], ra[UNR];
-        unsigned long long cw0[UNR];
-        bool live[UNR];
-#pragma unroll
-        for (uint32_t j = 0; j < UNR; ++j) {
-            const uint32_t e = e0 + (s + j) * A + s_arc;
-            const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-            wt[j] = ok ? a.w[e] : 0u;
-            live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
-        }
-        if (__ballot(live[0]) == 0) break;   // arcs run in increasing weight
-#pragma unroll
-        for (uint32_t j = 0; j < UNR; ++j) {
-            const uint32_t e = e0 + (s + j) * A + s_arc;
-            uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
-            ra[j] = live[j] ? a.rev[e] : 0u;
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < UNR; ++j)
-            cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
-        uint32_t enq_mask = 0;
-        uint32_t enq_u[UNR];
-#pragma unroll
-        for (uint32_t j = 0; j < UNR; ++j) {
-            enq_u[j] = 0;
-            const uint32_t u = uu[j];
-            if (u == 0xFFFFFFFFu) continue;
-            st_relax++;
-            const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
-            unsigned long long curw = cw0[j];
-            if (cand == GST_INF_COST || cand > (uint32_t)(curw >> 32)) continue;
-            if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand, 0u, a.ub)) continue;
-            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-            const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
-            bool improved = false;
-            if (nw < curw) {
-                if (cand == cv) {
-                    while (cand < (uint32_t)(curw >> 32)) {
-                        const unsigned long long old = atomicCAS(addr, curw, nw);
-                        if (old == curw) { improved = true; break; }
-                        curw = old;
-                    }
-                } else {
-                    const unsigned long long old = atomicMin(addr, nw);
-                    improved = cand < (uint32_t)(old >> 32);
-                }
-            }
-            if (!improved) continue;
-            st_upd++;
-            unsigned long long* cwp = (unsigned long long*)&chg[(uint64_t)u * W + wsel];
-            const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
-            const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inq[u], 1u) == 0u);
-            if (first) { enq_mask |= 1u << j; enq_u[j] = u; }
-        }
-        // queue the vertices whose first change this was: one reservation per wave and step
-#pragma unroll
-        for (uint32_t j = 0; j < UNR; ++j) {
-            const bool want = (enq_mask >> j) & 1u;
-            const uint64_t bal = __ballot(want);
-            if (bal == 0) continue;
-            const uint32_t cntw = (uint32_t)__popcll(bal);
-            uint32_t base = 0;
-            if (lane == 0) {
-                const uint32_t p = atomicAdd(&a.actl->pending, cntw);
-                if (p + cntw > a.ring_mask + 1) atomicOr(&a.actl->err, 8u);
-                base = atomicAdd(&a.actl->tail, cntw);
-            }
-            base = __shfl(base, 0, WAVE);
-            if (want) {
-                const uint32_t pos = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-                task_write(a, n0, pos, TASK_VERTEX, enq_u[j], 0, 0, 0);
-            }
-            st_enq += want ? 1u : 0u;
-        }
-    }
-}
-
-template <bool FLT>
-__global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
-    RoundCtl* ctl = a.ctl;
-    AsyncCtl* ac = a.actl;
-    if (ctl->level_done) return;
-    const uint32_t r = ctl->round;
-    const uint32_t cur = r % 3, prv = (r + 2) % 3;
-    const uint32_t n0 = ctl->cnt[cur];
-    const uint32_t lane = lane_id();
-    const uint32_t W = a.W;
-    // clear the previous round's bitmaps and flags (the round scheme's phase A)
-    {
-        const uint32_t n_prv = ctl->cnt[prv];
-        const uint64_t* lp = a.list[prv];
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_prv; i += gridDim.x * blockDim.x) {
-            const uint64_t it = lp[i];
-            if ((it >> 32) == 0) {
-                const uint32_t v = (uint32_t)it;
-                for (uint32_t w = 0; w < W; ++w) a.chg[prv][(uint64_t)v * W + w] = 0;
-                a.inq[prv][v] = 0;
-            }
-        }
-    }
-    uint64_t* chg = a.chg[cur];
-    uint32_t* inq = a.inq[cur];
-    const uint64_t* lcur = a.list[cur];
-    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, ovf = 0;
-    uint32_t done_local = 0;
-    bool stop = false;
-    while (!stop) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ac->head, ASYNC_BATCH);
-        base = __shfl(base, 0, WAVE);
-        for (uint32_t j = 0; j < ASYNC_BATCH && !stop; ++j) {
-            const uint32_t idx = base + j;
-            uint32_t kind, v, c = 0, wd = 0;
-            uint64_t bm = 0;
-            if (idx < n0) {
-                const uint64_t it = lcur[idx];
-                v = (uint32_t)it;
-                c = (uint32_t)(it >> 32);
-                kind = c == 0 ? TASK_VERTEX : 0xFFu;  // later chunks of an initial row: no-op
-            } else {
-                // wait for the slot of task idx (lane 0 polls, then broadcasts)
-                unsigned long long w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-                uint32_t outcome = 0;  // 1 = task ready, 2 = level finished, 3 = gave up
-                if (lane == 0) {
-                    if (done_local) { atomicSub(&ac->pending, done_local); done_local = 0; }
-                    const unsigned long long* sl = a.ring + 4ull * ((idx - n0) & a.ring_mask);
-                    for (uint32_t it = 0; it < ASYNC_MAX_WAIT; ++it) {
-                        w0 = ld_agent(sl + 0);
-                        w1 = ld_agent(sl + 1);
-                        if ((uint32_t)(w0 >> 32) == idx && (uint32_t)(w1 >> 32) == idx) {
-                            if ((w1 & 1ull) == TASK_CHUNK) {
-                                w2 = ld_agent(sl + 2);
-                                w3 = ld_agent(sl + 3);
-                                if ((uint32_t)(w2 >> 32) != idx || (uint32_t)(w3 >> 32) != idx) continue;
-                            }
-                            outcome = 1;
-                            // invalidate the slot: tags restart at 0 in every launch
-                            const unsigned long long dead = 0xFFFFFFFF00000000ull;
-                            unsigned long long* sw = a.ring + 4ull * ((idx - n0) & a.ring_mask);
-                            st_agent(sw + 0, dead);
-                            st_agent(sw + 1, dead);
-                            if ((w1 & 1ull) == TASK_CHUNK) { st_agent(sw + 2, dead); st_agent(sw + 3, dead); }
-                            break;
-                        }
-                        if (ld_agent32(&ac->pending) == 0) { outcome = 2; break; }
-                        // exponential back-off: idle waves must not flood the coherence point
-                        if (it < 2) continue;
-                        else if (it < 8) __builtin_amdgcn_s_sleep(8);
-                        else if (it < 32) __builtin_amdgcn_s_sleep(32);
-                        else __builtin_amdgcn_s_sleep(127);
-                    }
-                    if (outcome == 0) { outcome = 3; atomicOr(&ac->err, 4u); }
-                }
-                outcome = __shfl(outcome, 0, WAVE);
-                if (outcome != 1) { stop = true; break; }
-                w0 = __shfl(w0, 0, WAVE); w1 = __shfl(w1, 0, WAVE);
-                w2 = __shfl(w2, 0, WAVE); w3 = __shfl(w3, 0, WAVE);
-                v = (uint32_t)w0;
-                kind = (uint32_t)(w1 & 1ull);
-                wd = (uint32_t)((w1 >> 1) & 0xFFull);
-                c = (uint32_t)((w1 & 0xFFFFFFFFull) >> 9);
-                bm = (w2 & 0xFFFFFFFFull) | ((w3 & 0xFFFFFFFFull) << 32);
-            }
-            if (kind == TASK_VERTEX) {
-                // take the change bitmap of v
-                uint64_t bms[16];
-                uint32_t nz = 0;
-                if (lane == 0) atomicExch(&inq[v], 0u);
-                for (uint32_t w = 0; w < W && w < 16; ++w) {
-                    uint64_t x = 0;
-                    if (lane == 0) x = atomicExch((unsigned long long*)&chg[(uint64_t)v * W + w], 0ull);
-                    bms[w] = __shfl(x, 0, WAVE);
-                    nz += bms[w] != 0;
-                }
-                const uint32_t rs = a.row[v], re = a.row[v + 1];
-                const uint32_t nc = (re - rs + a.chunk - 1) / a.chunk;
-                if (nz && nc == 1) {
-                    st_items++;
-                    st_arcs += re - rs;
-                    for (uint32_t w = 0; w < W && w < 16; ++w)
-                        if (bms[w]) async_relax<FLT>(a, n0, v, rs, re, w, bms[w], st_relax, st_upd, st_enq, ovf);
-                } else if (nz && nc > 1) {
-                    // split the row: one CHUNK task per (chunk, non-empty word)
-                    const uint32_t ntask = nc * nz;
-                    uint32_t tb = 0;
-                    if (lane == 0) {
-                        const uint32_t p = atomicAdd(&ac->pending, ntask);
-                        if (p + ntask > a.ring_mask + 1) atomicOr(&ac->err, 8u);
-                        tb = atomicAdd(&ac->tail, ntask);
-                    }
-                    tb = __shfl(tb, 0, WAVE);
-                    uint32_t k2 = 0;
-                    for (uint32_t w = 0; w < W && w < 16; ++w) {
-                        if (!bms[w]) continue;
-                        for (uint32_t cc = lane; cc < nc; cc += WAVE)
-                            task_write(a, n0, tb + k2 * nc + cc, TASK_CHUNK, v, cc, w, bms[w]);
-                        k2++;
-                    }
-                    st_enq += ntask;
-                }
-            } else if (kind == TASK_CHUNK) {
-                const uint32_t rs = a.row[v], re = a.row[v + 1];
-                const uint32_t e0 = rs + c * a.chunk;
-                const uint32_t e1 = min(re, e0 + a.chunk);
-                st_items++;
-                st_arcs += e1 - e0;
-                async_relax<FLT>(a, n0, v, e0, e1, wd, bm, st_relax, st_upd, st_enq, ovf);
-            }
-            done_local++;
-        }
-        if (lane == 0 && done_local) { atomicSub(&ac->pending, done_local); done_local = 0; }
-    }
-    if (lane == 0 && done_local) atomicSub(&ac->pending, done_local);
-    const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
-    const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
-    const uint32_t sr = wave_sum32(st_relax);
-    const uint32_t su = wave_sum32(st_upd);
-    const uint32_t se = wave_sum32(st_enq);
-    const uint32_t so = wave_sum32(ovf);
-    if (lane == 0) {
-        if (si) atomicAdd(&a.stats[ST_ITEMS], (unsigned long long)si);
-        if (sa) atomicAdd(&a.stats[ST_ARCS], (unsigned long long)sa);
-        if (sr) atomicAdd(&a.stats[ST_RELAX], (unsigned long long)sr);
-        if (su) atomicAdd(&a.stats[ST_UPD], (unsigned long long)su);
-        if (se) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)se);
-        if (so) atomicOr(&ac->err, 1u);
-    }
-}
-
-// after an asynchronous launch: the level is finished (counters of the round scheme reset)
-__global__ void k_async_end(RoundCtl* ctl, AsyncCtl* ac) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (ac->err & 1u) ctl->err |= 1u;
-    if (ac->err & 12u) ctl->err |= 4u;   // lost work: the host re-runs the query on the round path
-    ctl->level_done = 1;
-}
-
-// prepare an asynchronous launch from the current round's list
-__global__ void k_async_begin(const RoundCtl* ctl, AsyncCtl* ac) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t n0 = ctl->cnt[ctl->round % 3];
-    ac->head = 0;
-    ac->tail = n0;
-    ac->pending = n0;
-}
-
-// ---------------------------------------------------------------------------------------
-// merge: T(v,X) for all X of level k from the final lower levels of v.
-// A workgroup takes a tile of VB consecutive vertices: their lower-level rows (contiguous runs of
-// the level blocks) are staged in LDS with coalesced loads, the split table of the level sits in
-// LDS, and each thread owns (vertex, mask) outputs: min over the 2^(k-1)-1 splits in registers,
-// then a coalesced store of the level-k rows.  No atomics except the LDS bitmap of finite masks.
-// ---------------------------------------------------------------------------------------
-constexpr int MERGE_THREADS = 256;
-
-template <bool FLT>
-__global__ void __launch_bounds__(MERGE_THREADS) k_merge(MergeArgs a) {
-    extern __shared__ unsigned long long lds[];
-    const uint32_t k = a.k;
-    const uint32_t L = a.off[k];             // lower-level words per vertex
-    const uint32_t mk = a.m[k];
-    const uint32_t S = (1u << (k - 1)) - 1;  // splits per mask
-    const uint32_t VB = a.VB;
-    const uint32_t W = a.W;
-    const uint32_t g = a.g;
-    unsigned long long* rows = lds;                          // VB * L
-    unsigned long long* bm = rows + (size_t)VB * L;          // VB * W
-    const unsigned long long* pt = (const unsigned long long*)a.pairs;
-    if (a.pairs_in_lds) {
-        unsigned long long* pl = bm + (size_t)VB * W;
-        for (uint32_t i = threadIdx.x; i < a.P; i += blockDim.x) pl[i] = a.pairs[i];
-        pt = pl;
-    }
-    const bool prune = a.ub != GST_INF_COST;
-    const bool prune_lb = prune && a.use_lb && !a.is_full;
-    uint32_t ovf = 0;
-    unsigned long long best_root = GST_INF_WORD;
-    uint64_t st_pairs = 0, st_vert = 0;
-    const uint64_t* Tk_base = a.T + a.lvl_base[k];
-
-    for (uint32_t v0 = blockIdx.x * VB; v0 < a.n; v0 += gridDim.x * VB) {
-        const uint32_t nv = min(VB, a.n - v0);
-        __syncthreads();
-        for (uint32_t j = 1; j < k; ++j) {
-            const uint32_t mj = a.m[j];
-            const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v0 * mj;
-            const uint32_t cnt = nv * mj;
-            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-                const uint32_t b = i / mj, t = i - b * mj;
-                rows[b * L + a.off[j] + t] = src[i];
-            }
-        }
-        for (uint32_t i = threadIdx.x; i < nv * W; i += blockDim.x) bm[i] = 0;
-        __syncthreads();
-        const uint32_t items = nv * mk;
-        for (uint32_t i = threadIdx.x; i < items; i += blockDim.x) {
-            const uint32_t b = i / mk, r = i - b * mk;
-            const uint32_t v = v0 + b;
-            const uint32_t X = a.masks[a.off[k] + r];
-            const uint32_t gmv = a.gm[v];
-            const unsigned long long* rv = rows + (size_t)b * L;
-            unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
-            const unsigned long long* pr = pt + (size_t)r * S;
-            for (uint32_t s2 = 0; s2 < S; ++s2) {
-                const unsigned long long e = pr[s2];
-                const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
-                const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
-                const uint32_t c1 = (uint32_t)(rv[p1] >> 32);
-                const uint32_t c2 = (uint32_t)(rv[p2] >> 32);
-                const uint32_t sum = add_cost<FLT>(c1, c2, ovf);
-                if (sum != GST_INF_COST) {
-                    const unsigned long long wv = ((unsigned long long)sum << 32) | (GST_PRED_MERGE | (uint32_t)(e >> 48));
-                    best = wv < best ? wv : best;
-                }
-            }
-            if (best != GST_INF_WORD && prune) {
-                uint32_t lb = 0;
-                if (prune_lb) {
-                    for (uint32_t gi = 0; gi < g; ++gi) {
-                        if ((X >> gi) & 1u) continue;
-                        const uint32_t di = (uint32_t)(rv[gi] >> 32);  // level-1 position gi = mask 1<<gi
-                        lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
-                    }
-                }
-                if (reaches_ub<FLT>((uint32_t)(best >> 32), lb, a.ub)) best = GST_INF_WORD;
-            }
-            ((uint64_t*)Tk_base)[(uint64_t)v0 * mk + i] = best;
-            if (best != GST_INF_WORD) {
-                if (a.is_full) {
-                    const unsigned long long rw = ((best >> 32) << 32) | v;
-                    best_root = rw < best_root ? rw : best_root;
-                } else {
-                    atomicOr(&bm[b * W + (r >> 6)], 1ull << (r & 63));
-                }
-            }
-        }
-        st_pairs += (uint64_t)items * S;
-        st_vert += nv;
-        __syncthreads();
-        if (!a.is_full) {
-            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
-                const uint32_t v = v0 + b;
-                if (a.row[v + 1] == a.row[v]) continue;
-                unsigned long long any = 0;
-                for (uint32_t w = 0; w < W; ++w) {
-                    const unsigned long long x = bm[b * W + w];
-                    any |= x;
-                    if (x) a.chg0[(uint64_t)v * W + w] = x;
-                }
-                if (any) a.inq0[v] = 1;
-            }
-        }
-    }
-    if (a.is_full) {
-        best_root = wave_min64(best_root);
-        if (lane_id() == 0 && best_root != GST_INF_WORD) atomicMin(a.root, best_root);
-    }
-    const uint32_t so = wave_sum32(ovf);
-    if (threadIdx.x == 0) {
-        atomicAdd(&a.stats[ST_MPAIRS], (unsigned long long)st_pairs);
-        atomicAdd(&a.stats[ST_MVERT], (unsigned long long)st_vert);
-    }
-    if (lane_id() == 0 && so) atomicOr(&a.ctl->err, 1u);
-}
-
-// g == 1: the full mask is level 1; its minimum is any member (cost 0)
-__global__ void k_root_scan_l1(const uint64_t* T1, uint32_t n, unsigned long long* root) {
-    unsigned long long best = GST_INF_WORD;
-    const uint64_t tid0 = (uint64_t)blockIdx.x * blockDim.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = tid0; base < n; base += stride) {
-        const uint64_t v = base + threadIdx.x;
-        if (v < n) {
-            const unsigned long long wv = T1[v];
-            if (wv != GST_INF_WORD) {
-                const unsigned long long rw = ((wv >> 32) << 32) | v;
-                best = rw < best ? rw : best;
-            }
-        }
-    }
-    best = wave_min64(best);
-    if (lane_id() == 0 && best != GST_INF_WORD) atomicMin(root, best);
-}
-
-// ---------------------------------------------------------------------------------------
-// traceback on the device: one wavefront walks the predecessor structure from (root, full)
-// ---------------------------------------------------------------------------------------
-struct TraceArgs {
-    const uint64_t* T;
-    const uint32_t* pos_of;   // mask -> row position
-    const int32_t* col;
-    const uint32_t* w;
-    const unsigned long long* root;
-    uint32_t* out_u;
-    uint32_t* out_v;
-    uint32_t* out_w;
-    uint32_t* out_n;
-    RoundCtl* ctl;
-    uint64_t lvl_base[GST_MAX_GROUPS + 2];
-    uint32_t m[GST_MAX_GROUPS + 2];
-    uint32_t off[GST_MAX_GROUPS + 2];
-    uint32_t full;
-    uint32_t cap;             // capacity of the edge output
-    uint32_t max_iter;
-};
-
-__global__ void __launch_bounds__(WAVE) k_traceback(TraceArgs a) {
-    __shared__ uint64_t stk[TRACE_STACK];
-    __shared__ uint32_t sp_s, nout_s, fail_s;
-    const uint32_t lane = threadIdx.x;
-    if (lane == 0) {
-        const unsigned long long rw = *a.root;
-        sp_s = 0; nout_s = 0; fail_s = 0;
-        if (rw != GST_INF_WORD) {
-            stk[0] = ((uint64_t)a.full << 32) | (uint32_t)(rw & 0xFFFFFFFFu);
-            sp_s = 1;
-        }
-    }
-    __syncthreads();
-    uint32_t iter = 0;
-    while (true) {
-        const uint32_t sp = sp_s;
-        if (sp == 0 || fail_s) break;
-        if (++iter > a.max_iter) { if (lane == 0) fail_s = 1; break; }
-        const uint32_t take = min(sp, (uint32_t)WAVE);
-        uint64_t item = 0;
-        const bool act = lane < take;
-        if (act) item = stk[sp - 1 - lane];
-        __syncthreads();
-        if (lane == 0) sp_s = sp - take;
-        __syncthreads();
-        uint32_t npush = 0;
-        uint64_t push0 = 0, push1 = 0;
-        uint32_t eu = 0, ev = 0, ew = 0;
-        bool emit = false;
-        if (act) {
-            const uint32_t v = (uint32_t)item;
-            const uint32_t X = (uint32_t)(item >> 32);
-            const uint32_t pos = a.pos_of[X];
-            const uint32_t k = (uint32_t)__popc(X);
-            const uint32_t rk = pos - a.off[k];
-            const uint64_t wv = a.T[a.lvl_base[k] + (uint64_t)v * a.m[k] + rk];
-            const uint32_t pred = (uint32_t)wv;
-            if (wv == GST_INF_WORD) {
-                fail_s = 1;
-            } else if (pred == GST_PRED_LEAF) {
-                // v covers X by itself
-            } else if (pred & 0x80000000u) {
-                const uint32_t X1 = pred & 0x7FFFFFFFu;
-                if (X1 == 0 || (X1 & ~X) || X1 == X) { fail_s = 1; }
-                else {
-                    push0 = ((uint64_t)X1 << 32) | v;
-                    push1 = ((uint64_t)(X ^ X1) << 32) | v;
-                    npush = 2;
-                }
-            } else {
-                const uint32_t u = (uint32_t)a.col[pred];
-                eu = v; ev = u; ew = a.w[pred];
-                emit = true;
-                push0 = ((uint64_t)X << 32) | u;
-                npush = 1;
-            }
-        }
-        __syncthreads();
-        if (act) {
-            if (npush) {
-                const uint32_t p = atomicAdd(&sp_s, npush);
-                if (p + npush > TRACE_STACK) fail_s = 1;
-                else {
-                    stk[p] = push0;
-                    if (npush == 2) stk[p + 1] = push1;
-                }
-            }
-            if (emit) {
-                const uint32_t o = atomicAdd(&nout_s, 1u);
-                if (o < a.cap) { a.out_u[o] = eu; a.out_v[o] = ev; a.out_w[o] = ew; }
-                else fail_s = 1;
-            }
-        }
-        __syncthreads();
-    }
-    __syncthreads();
-    if (lane == 0) {
-        *a.out_n = nout_s;
-        if (fail_s) atomicOr(&a.ctl->err, 2u);
-    }
-}
-
-int grid_for(uint64_t work, int block, int max_blocks) {
-    uint64_t b = (work + block - 1) / block;
-    if (b < 1) b = 1;
-    if (b > (uint64_t)max_blocks) b = max_blocks;
-    return (int)b;
-}
-
-}  // namespace
 
 // =======================================================================================
 // per-query device buffers (grown on demand, reused across queries)
@@ -1420,6 +119,7 @@ gst_status gst_ctx_device_init(gst_ctx* ctx) {
 
 void gst_ctx_device_fini(gst_ctx* ctx) {
     if (!ctx) return;
+    gst_batch_free(ctx);
     hipSetDevice(ctx->device);
     gst_query_bufs_free(ctx);
     if (ctx->stream) hipStreamDestroy((hipStream_t)ctx->stream);
@@ -1484,6 +184,67 @@ uint32_t default_chunk(int m) {
 }
 
 }  // namespace
+
+// Tree of a traced edge list: duplicates removed, then a spanning forest of the union (every
+// edge on a cycle of the union has weight zero, so any spanning tree of it has the DP cost).
+void gst_finish_tree(bool flt, uint32_t nout, const uint32_t* hu, const uint32_t* hv, const uint32_t* hw,
+                     gst_result* res) {
+    // deduplicate (a tie-free optimum is a tree; duplicates can only come from zero-cost ties)
+    std::vector<std::pair<uint64_t, uint32_t>> ed;
+    ed.reserve(nout);
+    for (uint32_t i = 0; i < nout; ++i) {
+        uint32_t x = hu[i], y = hv[i];
+        if (x > y) std::swap(x, y);
+        ed.push_back({((uint64_t)x << 32) | y, hw[i]});
+    }
+    std::sort(ed.begin(), ed.end());
+    ed.erase(std::unique(ed.begin(), ed.end(),
+                         [](const std::pair<uint64_t, uint32_t>& p, const std::pair<uint64_t, uint32_t>& q) {
+                             return p.first == q.first;
+                         }),
+             ed.end());
+    // Every edge on a cycle of the traced union has weight zero (dropping it would give a
+    // feasible structure cheaper than the optimum), so any spanning tree of the union has the
+    // DP cost: keep the edges that join two components.
+    {
+        std::vector<uint32_t> vs;
+        vs.reserve(ed.size() * 2);
+        for (auto& p2 : ed) { vs.push_back((uint32_t)(p2.first >> 32)); vs.push_back((uint32_t)p2.first); }
+        std::sort(vs.begin(), vs.end());
+        vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+        std::vector<uint32_t> par(vs.size());
+        for (size_t i = 0; i < par.size(); ++i) par[i] = (uint32_t)i;
+        auto fnd = [&](uint32_t x) {
+            while (par[x] != x) { par[x] = par[par[x]]; x = par[x]; }
+            return x;
+        };
+        auto idx = [&](uint32_t v) { return (uint32_t)(std::lower_bound(vs.begin(), vs.end(), v) - vs.begin()); };
+        std::vector<std::pair<uint64_t, uint32_t>> kept;
+        kept.reserve(ed.size());
+        for (auto& p2 : ed) {
+            const uint32_t a2 = fnd(idx((uint32_t)(p2.first >> 32))), b2 = fnd(idx((uint32_t)p2.first));
+            if (a2 == b2) continue;
+            par[a2] = b2;
+            kept.push_back(p2);
+        }
+        ed.swap(kept);
+    }
+    res->n_edges = (int32_t)ed.size();
+    res->eu = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
+    res->ev = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
+    res->ew = (double*)std::malloc(sizeof(double) * std::max<size_t>(ed.size(), 1));
+    double tc = 0;
+    for (size_t i = 0; i < ed.size(); ++i) {
+        res->eu[i] = (int32_t)(ed[i].first >> 32);
+        res->ev[i] = (int32_t)(ed[i].first & 0xFFFFFFFFu);
+        double wv;
+        if (flt) { float f; std::memcpy(&f, &ed[i].second, 4); wv = (double)f; }
+        else wv = (double)ed[i].second;
+        res->ew[i] = wv;
+        tc += wv;
+    }
+    res->tree_cost = tc;
+}
 
 gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* groups,
                           const gst_opts* opts_in, gst_result* res) {
@@ -1907,61 +668,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             HIPCHK(hipMemcpyAsync(hw.data(), B->edges + 2 * ecap, nout * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
         }
-        // deduplicate (a tie-free optimum is a tree; duplicates can only come from zero-cost ties)
-        std::vector<std::pair<uint64_t, uint32_t>> ed;
-        ed.reserve(nout);
-        for (uint32_t i = 0; i < nout; ++i) {
-            uint32_t x = hu[i], y = hv[i];
-            if (x > y) std::swap(x, y);
-            ed.push_back({((uint64_t)x << 32) | y, hw[i]});
-        }
-        std::sort(ed.begin(), ed.end());
-        ed.erase(std::unique(ed.begin(), ed.end(),
-                             [](const std::pair<uint64_t, uint32_t>& p, const std::pair<uint64_t, uint32_t>& q) {
-                                 return p.first == q.first;
-                             }),
-                 ed.end());
-        // Every edge on a cycle of the traced union has weight zero (dropping it would give a
-        // feasible structure cheaper than the optimum), so any spanning tree of the union has the
-        // DP cost: keep the edges that join two components.
-        {
-            std::vector<uint32_t> vs;
-            vs.reserve(ed.size() * 2);
-            for (auto& p2 : ed) { vs.push_back((uint32_t)(p2.first >> 32)); vs.push_back((uint32_t)p2.first); }
-            std::sort(vs.begin(), vs.end());
-            vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
-            std::vector<uint32_t> par(vs.size());
-            for (size_t i = 0; i < par.size(); ++i) par[i] = (uint32_t)i;
-            auto fnd = [&](uint32_t x) {
-                while (par[x] != x) { par[x] = par[par[x]]; x = par[x]; }
-                return x;
-            };
-            auto idx = [&](uint32_t v) { return (uint32_t)(std::lower_bound(vs.begin(), vs.end(), v) - vs.begin()); };
-            std::vector<std::pair<uint64_t, uint32_t>> kept;
-            kept.reserve(ed.size());
-            for (auto& p2 : ed) {
-                const uint32_t a2 = fnd(idx((uint32_t)(p2.first >> 32))), b2 = fnd(idx((uint32_t)p2.first));
-                if (a2 == b2) continue;
-                par[a2] = b2;
-                kept.push_back(p2);
-            }
-            ed.swap(kept);
-        }
-        res->n_edges = (int32_t)ed.size();
-        res->eu = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
-        res->ev = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(ed.size(), 1));
-        res->ew = (double*)std::malloc(sizeof(double) * std::max<size_t>(ed.size(), 1));
-        double tc = 0;
-        for (size_t i = 0; i < ed.size(); ++i) {
-            res->eu[i] = (int32_t)(ed[i].first >> 32);
-            res->ev[i] = (int32_t)(ed[i].first & 0xFFFFFFFFu);
-            double wv;
-            if (flt) { float f; std::memcpy(&f, &ed[i].second, 4); wv = (double)f; }
-            else wv = (double)ed[i].second;
-            res->ew[i] = wv;
-            tc += wv;
-        }
-        res->tree_cost = tc;
+        gst_finish_tree(flt, nout, hu.data(), hv.data(), hw.data(), res);
     }
 
     if (trace_rounds) {

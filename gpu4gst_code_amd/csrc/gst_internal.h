@@ -81,6 +81,7 @@ struct gst_ctx {
     std::string last_error;
     void* stream = nullptr;     // hipStream_t
     gst_query_bufs* qb = nullptr;
+    struct gst_batch_engine* be = nullptr;   // batched engine (gst_batch.hip), made on first use
     gst_levels lv_cache[GST_MAX_GROUPS + 1];
     bool lv_ready[GST_MAX_GROUPS + 1] = {false};
     int cu_count = 0;
@@ -90,6 +91,9 @@ struct gst_ctx {
 gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
                           const gst_opts* opts, gst_result* res);
 void gst_query_bufs_free(gst_ctx* ctx);
+// tree of a traced edge list (duplicates removed, spanning forest of the union) into res
+void gst_finish_tree(bool flt, uint32_t nout, const uint32_t* hu, const uint32_t* hv, const uint32_t* hw,
+                     gst_result* res);
 gst_status gst_ctx_device_init(gst_ctx* ctx);
 void gst_ctx_device_fini(gst_ctx* ctx);
 gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g);

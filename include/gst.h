@@ -127,8 +127,9 @@ const char* gst_status_str(gst_status s);
 
 /* --- graph ---------------------------------------------------------------------------- */
 /* Device-resident CSR reused across queries.  Replaces toCSR
- * (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:29-65): rows are sorted by neighbour id,
- * parallel arcs collapse to their lightest weight, and each arc gets its reverse-arc index. */
+ * (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:29-65): each row is ordered by (weight, neighbour id)
+ * so that relaxations can stop at the first arc that is too heavy, parallel arcs collapse to their
+ * lightest weight, self-loops are dropped, and each arc gets its reverse-arc index. */
 gst_status gst_graph_upload(gst_ctx* ctx, const gst_csr* csr, gst_graph** out);
 void       gst_graph_free(gst_graph* g);
 int32_t    gst_graph_num_vertices(const gst_graph* g);
@@ -151,6 +152,47 @@ void       gst_result_free(gst_result* r);
 gst_status gst_validate_tree(const gst_csr* csr, const gst_groups* groups, int32_t n_edges,
                              const int32_t* eu, const int32_t* ev, int32_t root,
                              double* cost_out);
+
+/* --- batched queries --------------------------------------------------------------------- */
+/* Many exact queries on one graph in flight together.  Replaces calling DPBF_GPU once per query
+ * (rucgraph/test_gpu.h:188-283 loops over queries; src/DPQ.cu:501-768 syncs with the host every
+ * round).  Up to `slots` queries hold their own state tables; one launch per grow round relaxes
+ * the current round of every query in flight, the grid split over the queries by their frontier
+ * sizes; the merges, frontier compactions and tracebacks of all queries that reached them run in
+ * one launch each; a step (those launches plus rounds_per_step grow rounds) is replayed as a HIP
+ * graph.  Each query gets what gst_solve returns for it: the same exact cost (the tree may be
+ * another tree of that cost). */
+typedef struct {
+    int32_t slots;             /* queries in flight (0: tuned, at most 64, limited by device memory) */
+    int32_t rounds_per_step;   /* grow rounds per step between host checks (0: tuned) */
+    int32_t use_graph;         /* 1: replay each step as a HIP graph (default), 0: launch kernel by kernel */
+    int32_t reserved[5];
+} gst_batch_opts;
+
+typedef struct {
+    double   t_total_ms;       /* wall time of the call, host clock */
+    double   queries_per_s;
+    double   latency_p50_ms;   /* per query: from entering a slot to its result being read */
+    double   latency_p99_ms;
+    uint64_t steps;
+    uint64_t grow_launches;
+    uint64_t grow_relax;       /* (arc, mask) relaxations over all queries */
+    uint64_t grow_bytes;       /* algorithmic bytes (same model as gst_stats) */
+    uint64_t merge_bytes;
+    int64_t  queries;          /* queries finished (solved or failed) */
+    int32_t  slots;
+    int32_t  rounds_per_step;
+    int32_t  used_graph;
+    int32_t  reserved[5];
+} gst_batch_stats;
+
+/* groups[nq]; opts: NULL (plain DPBF, trees returned) or nq entries, of which ub, prune_lb and
+ * want_tree are used; bopts may be NULL.  results[nq] and status[nq] receive each query's outcome:
+ * a failed query does not stop the others, and the call returns the first failing status (or
+ * GST_OK).  stats may be NULL. */
+gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* groups,
+                           const gst_opts* opts, const gst_batch_opts* bopts, gst_result* results,
+                           gst_status* status, gst_batch_stats* stats);
 
 /* --- learned upper bound ----------------------------------------------------------------- */
 /* No counterpart in the reference.  A two-layer mean-aggregation message-passing network (hidden
