@@ -260,7 +260,7 @@ class Solver:
 
     def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
               want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0,
-              pull_permille: int = 0, grow_mode: int = 0) -> Result:
+              pull_permille: int = 0, grow_mode: int = 0, push_lanes: int = 0) -> Result:
         opts = _abi.gst_opts()
         opts.ub = float(ub) if ub is not None else 0.0
         opts.prune_lb = 1 if prune_lb else 0
@@ -269,6 +269,7 @@ class Solver:
         opts.chunk_arcs = chunk_arcs
         opts.pull_permille = pull_permille
         opts.grow_mode = grow_mode
+        opts.push_lanes = push_lanes
         res = _abi.gst_result()
         grp = groups.as_c()
         st = self.lib.gst_solve(self.ctx, graph.handle, C.byref(grp), C.byref(opts), C.byref(res))
@@ -291,7 +292,7 @@ class Solver:
 
     def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
-                    use_graph: bool = True, raise_on_error: bool = True):
+                    use_graph: bool = True, push_lanes: int = 0, raise_on_error: bool = True):
         """Solve many queries on one graph with the batched engine (gst_solve_batch).
 
         Returns (results, stats): results[i] is a Result, or a GstError for a query that failed
@@ -307,6 +308,7 @@ class Solver:
             opts[i].ub = float(ub) if ub is not None else 0.0
             opts[i].prune_lb = 1 if prune_lb else 0
             opts[i].want_tree = 1 if want_tree else 0
+            opts[i].push_lanes = push_lanes
         bo = _abi.gst_batch_opts()
         bo.slots = slots
         bo.rounds_per_step = rounds_per_step

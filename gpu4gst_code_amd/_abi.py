@@ -37,7 +37,7 @@ class gst_groups(C.Structure):
 class gst_opts(C.Structure):
     _fields_ = [("ub", C.c_double), ("prune_lb", C.c_int32), ("want_tree", C.c_int32),
                 ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32),
-                ("grow_mode", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("grow_mode", C.c_int32), ("push_lanes", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class gst_stats(C.Structure):

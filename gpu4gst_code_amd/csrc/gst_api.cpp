@@ -296,6 +296,8 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
         std::memset(&results[q], 0, sizeof(results[q]));
         results[q].root = -1;
         status[q] = check_query(ctx, g, &groups[q]);
+        if (status[q] == GST_OK && opts && !gst_push_lanes_ok(opts[q].push_lanes))
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
         if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
         else if (first == GST_OK) first = status[q];
     }

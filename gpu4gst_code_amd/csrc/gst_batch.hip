@@ -42,10 +42,6 @@
 namespace {
 
 constexpr uint32_t MAX_SLOTS = 64;
-// merge LDS of a batched launch: vertex tiles up to 40 KB (the one-query budget) plus the split
-// table when it fits in 16 KB, for every (g, level) alike
-constexpr size_t MERGE_LDS_TILES = 40 * 1024;
-constexpr size_t MERGE_LDS_PAIRS = 16 * 1024;
 enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3 };
 
 struct SlotDev {
@@ -502,6 +498,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.pull_thresh = persistent ? (uint32_t)std::max<uint64_t>(1, E->n_items_total * 450ull / 1000ull) : 0xFFFFFFFFu;
         a.rlog = nullptr;
         a.seq = 0;
+        a.push_lanes = job.opts.push_lanes > 0 ? (uint32_t)job.opts.push_lanes : default_push_lanes(true);
     }
     for (int k = 2; k <= g; ++k) {
         MergeArgs& ma = sl.h_margs[k];
@@ -514,9 +511,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ubb; ma.use_lb = job.opts.prune_lb ? 1u : 0u;
         ma.is_full = (k == g) ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
-        uint32_t VB = (uint32_t)(MERGE_LDS_TILES / (((size_t)Lk + Wk) * 8));
-        ma.VB = std::max<uint32_t>(1, std::min<uint32_t>(VB, 256));
-        ma.pairs_in_lds = ((size_t)L.pair_cnt[k] * 8 <= MERGE_LDS_PAIRS) ? 1u : 0u;
+        ma.VB = merge_tile_vertices(Lk, Wk);
+        ma.pairs_in_lds = ((size_t)L.pair_cnt[k] * 8 <= MERGE_PAIRS_LDS) ? 1u : 0u;
     }
     {
         TraceArgs& ta = *sl.h_targs;
@@ -547,7 +543,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
 static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba, int R, bool flt) {
     hipStream_t s = (hipStream_t)ctx->stream;
     const int mgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 8);
-    const size_t mlds = MERGE_LDS_TILES + MERGE_LDS_PAIRS;
+    // one LDS size for every (g, level) a slot may merge: tile rows, bitmaps and split table
+    const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + MERGE_PAIRS_LDS;
     k_plan_step<<<1, WAVE, 0, s>>>(ba);
     if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
     else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);

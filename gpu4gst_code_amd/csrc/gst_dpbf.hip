@@ -252,6 +252,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     gst_opts opts{};
     opts.ub = 0; opts.prune_lb = 1; opts.want_tree = 1;
     if (opts_in) opts = *opts_in;
+    if (!gst_push_lanes_ok(opts.push_lanes))
+        return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)ctx->stream;
     const int g = groups->n_groups;
@@ -461,6 +463,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ga.rlog = trace_rounds ? B->rlog + (size_t)k * RLOG_ROUNDS * 4 : nullptr;
         uint32_t launch_seq = 0;   // the level's control block starts with seq 0
         ga.seq = 0;
+        ga.push_lanes = opts.push_lanes > 0 ? (uint32_t)opts.push_lanes : default_push_lanes(false);
         auto launch_async = [&]() {
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);
@@ -557,13 +560,11 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;
-        const bool pin = pair_bytes <= 16384;
-        const size_t budget = 40 * 1024;
-        uint32_t VB = (uint32_t)(budget / (((size_t)Lk + Wk) * 8));
-        VB = std::max<uint32_t>(1, std::min<uint32_t>(VB, 256));
+        const bool pin = pair_bytes <= MERGE_PAIRS_LDS;
+        const uint32_t VB = merge_tile_vertices(Lk, Wk);
         ma.VB = VB;
         ma.pairs_in_lds = pin ? 1u : 0u;
-        const size_t lds = (size_t)VB * (Lk + Wk) * 8 + (pin ? pair_bytes : 0);
+        const size_t lds = merge_lds_bytes(VB, Lk, Wk, pair_bytes, pin);
         const int mgrid = std::max(1, std::min<int>((int)((n + VB - 1) / VB), ctx->cu_count * 8));
         Ev* e = new_ev(1);
         hipEventRecord(e->a, s);

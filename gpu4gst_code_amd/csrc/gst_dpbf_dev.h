@@ -65,7 +65,13 @@ struct GrowArgs {
     struct AsyncCtl* actl;
     uint32_t* rlog;             // optional round log (RLOG_ROUNDS x 4 words), nullptr: off
     uint32_t seq;               // launch number within the level (1, 2, ...), set per launch
+    uint32_t push_lanes;        // lanes per work item in push rounds (8, 16, 32 or 64)
 };
+
+// Lanes per push item unless the caller chose (measured on BA V=200k, g=6): one query alone is
+// latency-bound and fastest with a whole wave per item; a batch of queries is throughput-bound and
+// fastest with four items per wave, whose chains of dependent loads then overlap.
+__host__ __device__ __forceinline__ uint32_t default_push_lanes(bool batched) { return batched ? 16u : 64u; }
 
 // round log (GST_TRACE_ROUNDS=1): per round {queued items | pull<<31, workgroups taking part,
 // wall clock at start (block 0), wall clock when the last workgroup finished}
@@ -150,6 +156,18 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t x, uint32_t t) {
         if ((uint32_t)__popcll(x & msk) <= t) lo += s;
     }
     return lo;
+}
+
+// OR over aligned groups of pl lanes (pl a power of two dividing the wave)
+__device__ __forceinline__ uint64_t group_or64(uint64_t v, uint32_t pl) {
+    for (uint32_t d = 1; d < pl; d <<= 1) v |= __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max32(uint32_t v) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) v = max(v, (uint32_t)__shfl_xor(v, d, WAVE));
+    return v;
 }
 
 __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
@@ -589,109 +607,121 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
         }
     }
 
-    // ---- push round: queued vertices relax their changed masks into their neighbours ----
-    for (uint32_t i = gwave; !pull && i < n_cur; i += nwaves) {
-        const uint64_t it = lcur[i];
-        const uint32_t v = (uint32_t)it;
-        const uint32_t c = (uint32_t)(it >> 32);
-        const uint32_t rs = a.row[v], re = a.row[v + 1];
-        const uint32_t e0 = rs + c * a.chunk;
-        const uint32_t e1 = min(re, e0 + a.chunk);
-        st_items++;
-        st_arcs += e1 - e0;
+    // ---- push round: queued (vertex, chunk) items relax their changed masks into their
+    // neighbours.  A wave serves PG items at once, one group of PL lanes each, so that the chains
+    // of dependent loads of several items are in flight together; inside a group the lanes are
+    // (mask, arc slot) pairs over blocks of up to PL masks. ----
+    const uint32_t PL = a.push_lanes, PG = WAVE / PL;   // PL: 8, 16, 32 or 64
+    const uint32_t pgrp = lane / PL, pl = lane % PL;
+    for (uint32_t base = gwave * PG; !pull && base < n_cur; base += nwaves * PG) {
+        const uint32_t i = base + pgrp;
+        const bool has = i < n_cur;
+        uint32_t v = 0, e0 = 0, e1 = 0;
+        if (has) {
+            const uint64_t it = lcur[i];
+            v = (uint32_t)it;
+            const uint32_t c = (uint32_t)(it >> 32);
+            const uint32_t rs = a.row[v], re = a.row[v + 1];
+            e0 = rs + c * a.chunk;
+            e1 = min(re, e0 + a.chunk);
+        }
+        st_items += wave_sum32(has && pl == 0 ? 1u : 0u);
+        st_arcs += wave_sum32(has && pl == 0 ? e1 - e0 : 0u);
         for (uint32_t wd = 0; wd < W; ++wd) {
-            uint64_t bm = ccur[(uint64_t)v * W + wd];
-            if (bm == 0) continue;
+            uint64_t bm = has ? ccur[(uint64_t)v * W + wd] : 0ull;
             // drop masks whose cost (plus lower bound) reaches the bound
             if (a.ub != GST_INF_COST) {
                 const uint32_t na0 = (uint32_t)__popcll(bm);
                 uint64_t keep = 0;
-                if (lane < na0) {
-                    const uint32_t bit = select_bit(bm, lane);
+                for (uint32_t t0 = pl; t0 < na0; t0 += PL) {
+                    const uint32_t bit = select_bit(bm, t0);
                     const uint32_t rk = wd * 64 + bit;
                     const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
                     const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
-                    if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
+                    if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep |= 1ull << bit;
                 }
-                bm = wave_or64(keep);
+                bm = group_or64(keep, PL);
             }
             const uint32_t na = (uint32_t)__popcll(bm);
-            if (na == 0) continue;
-            const uint32_t A = WAVE / na;           // arc slots per step
-            const uint32_t t = lane % na, s_arc = lane / na;
-            const bool lane_ok = s_arc < A;
-            const uint32_t rk = wd * 64 + select_bit(bm, t);
-            const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
-            const uint64_t bitv = 1ull << (rk & 63);
-            const uint32_t wsel = rk >> 6;
-            const uint32_t len = e1 - e0;
-            const uint32_t steps = (len + A - 1) / A;
-            for (uint32_t s = 0; s < steps; s += UNR) {
-                uint32_t uu[UNR], wt[UNR], ra[UNR];
-                unsigned long long cw0[UNR];
-                bool live[UNR];
+            const uint32_t na_max = wave_max32(na);
+            for (uint32_t mb = 0; mb < na_max; mb += PL) {
+                const uint32_t nm = na > mb ? min(PL, na - mb) : 0u;
+                const uint32_t A = nm ? PL / nm : 0u;   // arc slots per step
+                const uint32_t t = nm ? pl % nm : 0u, s_arc = nm ? pl / nm : 0u;
+                const bool lane_ok = nm != 0 && s_arc < A;
+                const uint32_t rk = wd * 64 + (lane_ok ? select_bit(bm, mb + t) : 0u);
+                const uint32_t cv = lane_ok ? (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32) : GST_INF_COST;
+                const uint64_t bitv = 1ull << (rk & 63);
+                const uint32_t wsel = rk >> 6;
+                const uint32_t len = e1 - e0;
+                const uint32_t steps = wave_max32(lane_ok ? (len + A - 1) / A : 0u);
+                for (uint32_t s = 0; s < steps; s += UNR) {
+                    uint32_t uu[UNR], wt[UNR], ra[UNR];
+                    unsigned long long cw0[UNR];
+                    bool live[UNR];
 #pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    const uint32_t e = e0 + (s + j) * A + s_arc;
-                    const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-                    wt[j] = ok ? a.w[e] : 0u;
-                    // under a bound, cv + w >= ub holds for every heavier arc of the row as well
-                    live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
-                }
-                if (__ballot(live[0]) == 0) break;
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        const uint32_t e = e0 + (s + j) * A + s_arc;
+                        const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
+                        wt[j] = ok ? a.w[e] : 0u;
+                        // under a bound, cv + w >= ub holds for every heavier arc of the row as well
+                        live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
+                    }
+                    if (__ballot(live[0]) == 0) break;
 #pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    const uint32_t e = e0 + (s + j) * A + s_arc;
-                    uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
-                    ra[j] = live[j] ? a.rev[e] : 0u;
-                }
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        const uint32_t e = e0 + (s + j) * A + s_arc;
+                        uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
+                        ra[j] = live[j] ? a.rev[e] : 0u;
+                    }
 #pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j)
-                    cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
+                    for (uint32_t j = 0; j < UNR; ++j)
+                        cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
 #pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j) {
-                    uint32_t enq_v = 0, enq_n = 0;
-                    const uint32_t u = uu[j];
-                    if (u != 0xFFFFFFFFu) {
-                        st_relax++;
-                        const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
-                        unsigned long long curw = cw0[j];
-                        if (cand != GST_INF_COST && cand <= (uint32_t)(curw >> 32) &&
-                            (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
-                            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-                            const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
-                            bool improved = false;
-                            if (nw < curw) {
-                                if (cand == cv) {
-                                    // effective zero-weight edge: only strict cost decreases, so
-                                    // equal-cost predecessors can never close a cycle
-                                    while (cand < (uint32_t)(curw >> 32)) {
-                                        const unsigned long long old = atomicCAS(addr, curw, nw);
-                                        if (old == curw) { improved = true; break; }
-                                        curw = old;
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        uint32_t enq_v = 0, enq_n = 0;
+                        const uint32_t u = uu[j];
+                        if (u != 0xFFFFFFFFu) {
+                            st_relax++;
+                            const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
+                            unsigned long long curw = cw0[j];
+                            if (cand != GST_INF_COST && cand <= (uint32_t)(curw >> 32) &&
+                                (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
+                                unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+                                const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
+                                bool improved = false;
+                                if (nw < curw) {
+                                    if (cand == cv) {
+                                        // effective zero-weight edge: only strict cost decreases, so
+                                        // equal-cost predecessors can never close a cycle
+                                        while (cand < (uint32_t)(curw >> 32)) {
+                                            const unsigned long long old = atomicCAS(addr, curw, nw);
+                                            if (old == curw) { improved = true; break; }
+                                            curw = old;
+                                        }
+                                    } else {
+                                        const unsigned long long old = atomicMin(addr, nw);
+                                        improved = cand < (uint32_t)(old >> 32);
                                     }
-                                } else {
-                                    const unsigned long long old = atomicMin(addr, nw);
-                                    improved = cand < (uint32_t)(old >> 32);
                                 }
-                            }
-                            if (improved) {
-                                st_upd++;
-                                unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
-                                if (!(*cwp & bitv)) {
-                                    const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
-                                    // first change of u in this round: queue it (one word per
-                                    // vertex: the word itself was empty; several words: flag)
-                                    const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
-                                    if (first) {
-                                        enq_v = u;
-                                        enq_n = num_chunks(a.row, u, a.chunk);
+                                if (improved) {
+                                    st_upd++;
+                                    unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
+                                    if (!(*cwp & bitv)) {
+                                        const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
+                                        // first change of u in this round: queue it (one word per
+                                        // vertex: the word itself was empty; several words: flag)
+                                        const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
+                                        if (first) {
+                                            enq_v = u;
+                                            enq_n = num_chunks(a.row, u, a.chunk);
+                                        }
                                     }
                                 }
                             }
                         }
+                        bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
                     }
-                    bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
                 }
             }
         }
@@ -1082,6 +1112,28 @@ __global__ void k_async_begin(const RoundCtl* ctl, AsyncCtl* ac) {
 // then a coalesced store of the level-k rows.  No atomics except the LDS bitmap of finite masks.
 // ---------------------------------------------------------------------------------------
 constexpr int MERGE_THREADS = 256;
+// LDS budget of a merge tile's cost rows: small enough for several workgroups per CU
+constexpr size_t MERGE_TILE_BYTES = 16 * 1024;
+// split tables up to this size are staged in LDS
+constexpr size_t MERGE_PAIRS_LDS = 16 * 1024;
+
+// 8-byte words taken by VB rows of L 4-byte costs
+__host__ __device__ __forceinline__ size_t merge_rows_words(uint32_t VB, uint32_t L) {
+    return ((size_t)VB * L + 1) / 2;
+}
+// vertices per merge tile for L lower-level costs and W bitmap words per vertex
+__host__ __device__ __forceinline__ uint32_t merge_tile_vertices(uint32_t L, uint32_t W) {
+    const size_t per = (size_t)L * 4 + (size_t)W * 8;
+    size_t vb = MERGE_TILE_BYTES / per;
+    if (vb < 1) vb = 1;
+    if (vb > 256) vb = 256;
+    return (uint32_t)vb;
+}
+// dynamic LDS bytes of a merge launch
+__host__ __device__ __forceinline__ size_t merge_lds_bytes(uint32_t VB, uint32_t L, uint32_t W, size_t pair_bytes,
+                                                         bool pin) {
+    return merge_rows_words(VB, L) * 8 + (size_t)VB * W * 8 + (pin ? pair_bytes : 0);
+}
 
 template <bool FLT>
 __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long long* lds, uint32_t bid, uint32_t nblk) {
@@ -1092,8 +1144,10 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     const uint32_t VB = a.VB;
     const uint32_t W = a.W;
     const uint32_t g = a.g;
-    unsigned long long* rows = lds;                          // VB * L
-    unsigned long long* bm = rows + (size_t)VB * L;          // VB * W
+    // LDS: the tile's lower-level costs (the merge reads no predecessors), then the bitmaps of
+    // finite outputs, then the split table
+    uint32_t* rows = (uint32_t*)lds;                                            // VB * L costs
+    unsigned long long* bm = lds + merge_rows_words(VB, L);                     // VB * W
     const unsigned long long* pt = (const unsigned long long*)a.pairs;
     if (a.pairs_in_lds) {
         unsigned long long* pl = bm + (size_t)VB * W;
@@ -1116,7 +1170,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             const uint32_t cnt = nv * mj;
             for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
                 const uint32_t b = i / mj, t = i - b * mj;
-                rows[b * L + a.off[j] + t] = src[i];
+                rows[b * L + a.off[j] + t] = (uint32_t)(src[i] >> 32);
             }
         }
         for (uint32_t i = threadIdx.x; i < nv * W; i += blockDim.x) bm[i] = 0;
@@ -1127,15 +1181,15 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             const uint32_t v = v0 + b;
             const uint32_t X = a.masks[a.off[k] + r];
             const uint32_t gmv = a.gm[v];
-            const unsigned long long* rv = rows + (size_t)b * L;
+            const uint32_t* rv = rows + (size_t)b * L;
             unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
             const unsigned long long* pr = pt + (size_t)r * S;
             for (uint32_t s2 = 0; s2 < S; ++s2) {
                 const unsigned long long e = pr[s2];
                 const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
                 const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
-                const uint32_t c1 = (uint32_t)(rv[p1] >> 32);
-                const uint32_t c2 = (uint32_t)(rv[p2] >> 32);
+                const uint32_t c1 = rv[p1];
+                const uint32_t c2 = rv[p2];
                 const uint32_t sum = add_cost<FLT>(c1, c2, ovf);
                 if (sum != GST_INF_COST) {
                     const unsigned long long wv = ((unsigned long long)sum << 32) | (GST_PRED_MERGE | (uint32_t)(e >> 48));
@@ -1147,7 +1201,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                 if (prune_lb) {
                     for (uint32_t gi = 0; gi < g; ++gi) {
                         if ((X >> gi) & 1u) continue;
-                        const uint32_t di = (uint32_t)(rv[gi] >> 32);  // level-1 position gi = mask 1<<gi
+                        const uint32_t di = rv[gi];  // level-1 position gi = mask 1<<gi
                         lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
                     }
                 }

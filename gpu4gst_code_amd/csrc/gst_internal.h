@@ -74,6 +74,8 @@ inline int64_t gst_find_arc(const int64_t* row, const int32_t* col, const uint32
     return (lo < row[u + 1] && col[lo] == v && w[lo] == wv) ? lo : -1;
 }
 
+inline bool gst_push_lanes_ok(int32_t p) { return p == 0 || p == 8 || p == 16 || p == 32 || p == 64; }
+
 struct gst_query_bufs;  // device buffers of one query (defined in gst_dpbf.hip)
 
 struct gst_ctx {

@@ -76,7 +76,8 @@ typedef struct {
     int32_t grow_mode;         /* 0 = tuned (pull/push rounds), 1 = rounds only, 2 = asynchronous device work
                                   queue for every level, 3 = rounds while the frontier is dense, then the
                                   work queue */
-    int32_t reserved[6];
+    int32_t push_lanes;        /* lanes per frontier item in push rounds: 8, 16, 32 or 64 (0 = tuned) */
+    int32_t reserved[5];
 } gst_opts;
 
 typedef struct {
