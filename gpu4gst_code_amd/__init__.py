@@ -260,7 +260,7 @@ class Solver:
 
     def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
               want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0,
-              pull_permille: int = 0, grow_mode: int = 0, push_lanes: int = 0) -> Result:
+              pull_permille: int = 0, grow_mode: int = 0, push_lanes: int = 0, half_bound: bool = True) -> Result:
         opts = _abi.gst_opts()
         opts.ub = float(ub) if ub is not None else 0.0
         opts.prune_lb = 1 if prune_lb else 0
@@ -270,6 +270,7 @@ class Solver:
         opts.pull_permille = pull_permille
         opts.grow_mode = grow_mode
         opts.push_lanes = push_lanes
+        opts.half_bound = 0 if half_bound else -1
         res = _abi.gst_result()
         grp = groups.as_c()
         st = self.lib.gst_solve(self.ctx, graph.handle, C.byref(grp), C.byref(opts), C.byref(res))
@@ -292,7 +293,8 @@ class Solver:
 
     def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
-                    use_graph: bool = True, push_lanes: int = 0, raise_on_error: bool = True):
+                    use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
+                    raise_on_error: bool = True):
         """Solve many queries on one graph with the batched engine (gst_solve_batch).
 
         Returns (results, stats): results[i] is a Result, or a GstError for a query that failed
@@ -309,6 +311,7 @@ class Solver:
             opts[i].prune_lb = 1 if prune_lb else 0
             opts[i].want_tree = 1 if want_tree else 0
             opts[i].push_lanes = push_lanes
+            opts[i].half_bound = 0 if half_bound else -1
         bo = _abi.gst_batch_opts()
         bo.slots = slots
         bo.rounds_per_step = rounds_per_step
@@ -423,5 +426,61 @@ def _solve_learned(self, graph: DeviceGraph, groups: Groups, model: Model, topk_
         self.lib.gst_result_free(C.byref(res))
 
 
+def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model: Model, topk_frac: float = 0.05, *,
+                         prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
+                         use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
+                         raise_on_error: bool = True):
+    """Exact solves of many queries with the learned bound, pipelined through the batched engine
+    (gst_solve_batch_learned).  Returns (results, batch stats); each Result's stats also carry the
+    query's learned-bound fields (sub_vertices, ub, t_sub_solve_ms, ...)."""
+    nq = len(queries)
+    grps = (_abi.gst_groups * max(nq, 1))()
+    keep = [q.as_c() for q in queries]
+    for i, c in enumerate(keep):
+        grps[i] = c
+    opts = (_abi.gst_opts * max(nq, 1))()
+    for i in range(nq):
+        opts[i].prune_lb = 1 if prune_lb else 0
+        opts[i].want_tree = 1 if want_tree else 0
+        opts[i].push_lanes = push_lanes
+        opts[i].half_bound = 0 if half_bound else -1
+    bo = _abi.gst_batch_opts()
+    bo.slots = slots
+    bo.rounds_per_step = rounds_per_step
+    bo.use_graph = 1 if use_graph else 0
+    res = (_abi.gst_result * max(nq, 1))()
+    status = (C.c_int * max(nq, 1))()
+    ls = (_abi.gst_learned_stats * max(nq, 1))()
+    bs = _abi.gst_batch_stats()
+    st = self.lib.gst_solve_batch_learned(self.ctx, graph.handle, nq, grps, model.handle, float(topk_frac), opts,
+                                          C.byref(bo), res, status, ls, C.byref(bs))
+    out = []
+    try:
+        for i in range(nq):
+            if status[i] != _abi.GST_OK:
+                out.append(GstError(status[i], "query %d failed" % i))
+                continue
+            r = res[i]
+            ne = r.n_edges
+            if ne > 0:
+                eu = np.ctypeslib.as_array(r.eu, shape=(ne,)).copy()
+                ev = np.ctypeslib.as_array(r.ev, shape=(ne,)).copy()
+                ew = np.ctypeslib.as_array(r.ew, shape=(ne,)).copy()
+            else:
+                eu, ev, ew = np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float64)
+            stats = {name: getattr(r.stats, name) for name, _ in _abi.gst_stats._fields_}
+            stats.update({name: getattr(ls[i], name) for name, _ in _abi.gst_learned_stats._fields_})
+            out.append(Result(r.cost, r.tree_cost, r.root, eu, ev, ew, bool(r.used_ub), bool(r.fell_back), stats))
+    finally:
+        for i in range(nq):
+            self.lib.gst_result_free(C.byref(res[i]))
+    stats = {name: getattr(bs, name) for name, _ in _abi.gst_batch_stats._fields_ if name != "reserved"}
+    if st != _abi.GST_OK and raise_on_error:
+        bad = next((e for e in out if isinstance(e, GstError)), None)
+        raise GstError(st, self.lib.gst_last_error(self.ctx).decode()) if bad is None else bad
+    return out, stats
+
+
 Solver.solve_learned = _solve_learned
+Solver.solve_batch_learned = _solve_batch_learned
 __all__ += ["Model"]

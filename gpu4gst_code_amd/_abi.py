@@ -37,7 +37,8 @@ class gst_groups(C.Structure):
 class gst_opts(C.Structure):
     _fields_ = [("ub", C.c_double), ("prune_lb", C.c_int32), ("want_tree", C.c_int32),
                 ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32),
-                ("grow_mode", C.c_int32), ("push_lanes", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("grow_mode", C.c_int32), ("push_lanes", C.c_int32), ("half_bound", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class gst_stats(C.Structure):
@@ -108,6 +109,10 @@ SIGNATURES = {
     "gst_model_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
     "gst_solve_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.c_void_p, C.c_double,
                                     C.POINTER(gst_opts), C.POINTER(gst_result), C.POINTER(gst_learned_stats)]),
+    "gst_solve_batch_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(gst_groups), C.c_void_p,
+                                          C.c_double, C.POINTER(gst_opts), C.POINTER(gst_batch_opts),
+                                          C.POINTER(gst_result), C.POINTER(C.c_int), C.POINTER(gst_learned_stats),
+                                          C.POINTER(gst_batch_stats)]),
     "gst_host_csr_free": (None, [C.POINTER(gst_host_csr)]),
     "gst_gen_random_connected": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64,
                                            C.POINTER(gst_host_csr)]),

@@ -298,6 +298,8 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
         status[q] = check_query(ctx, g, &groups[q]);
         if (status[q] == GST_OK && opts && !gst_push_lanes_ok(opts[q].push_lanes))
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
+        if (status[q] == GST_OK && opts && opts[q].half_bound != 0 && opts[q].half_bound != -1)
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1");
         if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
         else if (first == GST_OK) first = status[q];
     }
@@ -336,6 +338,31 @@ gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups*
     const gst_status cs = check_query(ctx, g, groups);
     if (cs != GST_OK) return cs;
     return gst_learned_solve(ctx, g, groups, m, topk_frac, opts, result, stats);
+}
+
+gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* groups,
+                                   gst_model* m, double topk_frac, const gst_opts* opts, const gst_batch_opts* bopts,
+                                   gst_result* results, gst_status* status, gst_learned_stats* lstats,
+                                   gst_batch_stats* stats) {
+    if (!ctx || !g || !m || nq < 0 || (nq > 0 && (!groups || !results || !status)))
+        return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "null argument") : GST_INVALID_ARG;
+    if (!(topk_frac > 0.0 && topk_frac <= 1.0)) return gst_set_error(ctx, GST_INVALID_ARG, "topk_frac must be in (0, 1]");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
+    if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
+    if (bopts && (bopts->slots < 0 || bopts->rounds_per_step < 0))
+        return gst_set_error(ctx, GST_INVALID_ARG, "batch options: negative slots or rounds_per_step");
+    for (int32_t q = 0; q < nq; ++q) {
+        std::memset(&results[q], 0, sizeof(results[q]));
+        results[q].root = -1;
+        if (lstats) { std::memset(&lstats[q], 0, sizeof(lstats[q])); lstats[q].ub = -1.0; }
+        status[q] = check_query(ctx, g, &groups[q]);
+        if (status[q] == GST_OK && opts && !gst_push_lanes_ok(opts[q].push_lanes))
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
+        if (status[q] == GST_OK && opts && opts[q].half_bound != 0 && opts[q].half_bound != -1)
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1");
+    }
+    return gst_learned_solve_batch(ctx, g, nq, groups, m, topk_frac, opts, bopts, results, status, lstats, stats);
 }
 
 void gst_result_free(gst_result* r) {

@@ -20,9 +20,9 @@ struct gst_batch_job {
     int kind = 0;
 };
 
-// Runs jobs until next_job returns false and every slot is idle.  next_job is asked for a job
-// whenever a slot is free; job_done receives each finished job (status and result set) and may
-// make more jobs available to next_job.  n_cap / g_cap / m_cap bound every job's graph size and
+// Runs jobs until every slot is idle and next_job has no job.  next_job is asked for a job whenever
+// a slot is free (also after it once returned false); job_done receives each finished job (status
+// and result set) and may make more jobs available to next_job, whose slot is refilled right after.  n_cap / g_cap / m_cap bound every job's graph size and
 // group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,

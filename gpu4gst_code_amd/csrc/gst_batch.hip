@@ -412,20 +412,6 @@ static gst_status ensure_hub_items(gst_ctx* ctx, gst_batch_engine* E, const gst_
     return GST_OK;
 }
 
-// bound in cost bits (same rule as the one-query driver)
-static uint32_t ub_bits_of(double ub, bool flt) {
-    if (!(std::isfinite(ub) && ub > 0)) return GST_INF_COST;
-    if (flt) {
-        float fu = (float)(ub * (1.0 + 1e-4));
-        fu = std::nextafter(fu, INFINITY);
-        uint32_t b;
-        std::memcpy(&b, &fu, 4);
-        return b;
-    }
-    const double c = std::floor(ub) + 1.0;
-    return c >= 4294967295.0 ? GST_INF_COST : (uint32_t)c;
-}
-
 // Enqueue the set-up of one query in slot `si`: group bits, level 1, its frontier, and the
 // argument blocks of every level.
 static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch_job& job, uint32_t chunk) {
@@ -461,7 +447,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
 
     uint64_t lvl_base[GST_MAX_GROUPS + 2] = {0};
     for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
-    const uint32_t ubb = ub_bits_of(job.opts.ub, flt);
+    const gst_bound_bits bb = gst_make_bound(job.opts.ub, flt, job.opts.half_bound);
+    const uint32_t ubb = bb.ub;
     const int fg = std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8));
     k_fill_u32<<<fg, 256, 0, s>>>(sl.gm, 0u, n);
     k_set_group_bits<<<64, 256, 0, s>>>(sl.gm, sl.grp, sl.grp + g + 1, g);
@@ -491,6 +478,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.g = (uint32_t)g;
         a.chunk = chunk;
         a.ub = ubb;
+        a.gmax = bb.gmax;
+        a.lbcap = bb.lbcap;
         a.use_lb = (job.opts.prune_lb && k >= 2) ? 1u : 0u;
         a.all_items = persistent ? E->all_items : nullptr;
         a.n_all = persistent ? E->n_all : 0u;
@@ -509,6 +498,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ma.lvl_base[j] = lvl_base[j]; ma.m[j] = (uint32_t)L.m[j]; ma.off[j] = (uint32_t)L.off[j]; }
         ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ubb; ma.use_lb = job.opts.prune_lb ? 1u : 0u;
+        ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
         ma.is_full = (k == g) ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
         ma.VB = merge_tile_vertices(Lk, Wk);
@@ -598,12 +588,13 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     uint64_t steps = 0, n_done = 0, relax = 0, gbytes = 0, mbytes = 0;
     gst_status first_err = GST_OK;
     bool flt_set = false, flt = false;
-    bool source_open = true;
     int busy = 0;
+    // next_job may have nothing now and more later (a finished job can make new work), so it is
+    // asked again whenever a slot frees; the run ends when no slot is busy
     auto try_fill = [&](int si) -> gst_status {
-        while (source_open) {
+        for (;;) {
             gst_batch_job job;
-            if (!next_job(job)) { source_open = false; break; }
+            if (!next_job(job)) break;
             const bool jf = job.G->w_type == GST_W_FLOAT32;
             if (!flt_set) { flt = jf; flt_set = true; }
             if (jf != flt) {
@@ -702,7 +693,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
             std::memset(res, 0, sizeof(*res));
             res->root = -1;
             res->n_groups = sl.g;
-            const bool has_ub = ub_bits_of(job.opts.ub, sl.flt) != GST_INF_COST;
+            const bool has_ub = gst_make_bound(job.opts.ub, sl.flt, 0).ub != GST_INF_COST;
             res->used_ub = has_ub ? 1 : 0;
             gst_stats& S2 = res->stats;
             S2.grow_items = hs[ST_ITEMS]; S2.grow_arcs = hs[ST_ARCS]; S2.grow_relax = hs[ST_RELAX];

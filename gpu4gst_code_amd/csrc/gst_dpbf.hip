@@ -254,6 +254,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     if (opts_in) opts = *opts_in;
     if (!gst_push_lanes_ok(opts.push_lanes))
         return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
+    if (opts.half_bound != 0 && opts.half_bound != -1)
+        return gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1 (got %d)", opts.half_bound);
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)ctx->stream;
     const int g = groups->n_groups;
@@ -262,20 +264,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     const bool flt = G->w_type == GST_W_FLOAT32;
 
     // bound in cost bits
-    uint32_t ub_bits = GST_INF_COST;
     const bool has_ub = std::isfinite(opts.ub) && opts.ub > 0;
-    if (has_ub) {
-        if (flt) {
-            // the DP sums fp32 costs in its own order; keep a relative slack of 1e-4 above the
-            // bound so that rounding never drops a structure whose exact cost is <= the bound
-            float fu = (float)(opts.ub * (1.0 + 1e-4));
-            fu = std::nextafter(fu, INFINITY);
-            std::memcpy(&ub_bits, &fu, 4);
-        } else {
-            const double c = std::floor(opts.ub) + 1.0;   // keep cost <= ub, drop cost >= floor(ub)+1
-            ub_bits = c >= 4294967295.0 ? GST_INF_COST : (uint32_t)c;
-        }
-    }
+    const gst_bound_bits bb = gst_make_bound(opts.ub, flt, opts.half_bound);
+    const uint32_t ub_bits = bb.ub;
 
     // ---- buffers ----
     if (!ctx->qb) ctx->qb = new gst_query_bufs();
@@ -449,7 +440,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         for (int i = 0; i < 3; ++i) { ga.list[i] = B->lists[i]; ga.chg[i] = B->chg[i]; ga.inq[i] = B->inq[i]; }
         ga.ctl = B->ctl + k; ga.stats = B->stats;
         ga.m = (uint32_t)L.m[k]; ga.W = (uint32_t)((L.m[k] + 63) / 64); ga.g = (uint32_t)g;
-        ga.chunk = chunk; ga.ub = ub_bits;
+        ga.chunk = chunk; ga.ub = ub_bits; ga.gmax = bb.gmax; ga.lbcap = bb.lbcap;
         // the one-label lower bound reads the group distances of level 1, which are final only
         // after level 1 has converged: never prune with it inside level 1
         ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;
@@ -557,6 +548,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ma.lvl_base[j] = lvl_base[j]; ma.m[j] = (uint32_t)L.m[j]; ma.off[j] = (uint32_t)L.off[j]; }
         ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ub_bits; ma.use_lb = opts.prune_lb ? 1u : 0u;
+        ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;

@@ -66,6 +66,9 @@ struct GrowArgs {
     uint32_t* rlog;             // optional round log (RLOG_ROUNDS x 4 words), nullptr: off
     uint32_t seq;               // launch number within the level (1, 2, ...), set per launch
     uint32_t push_lanes;        // lanes per work item in push rounds (8, 16, 32 or 64)
+    uint32_t gmax;              // a state is grown only while its cost is <= gmax (half the bound)
+    uint32_t lbcap;             // level-1 distances above gmax are only known to exceed it: the lower
+                                // bound reads min(distance, lbcap) (GST_INF_COST: no cap)
 };
 
 // Lanes per push item unless the caller chose (measured on BA V=200k, g=6): one query alone is
@@ -112,6 +115,8 @@ struct MergeArgs {
     uint32_t is_full;
     uint32_t VB;           // vertices per workgroup tile
     uint32_t pairs_in_lds;
+    uint32_t gmax;         // merged states above this cost are not put on the frontier
+    uint32_t lbcap;        // cap on level-1 distances in the lower bound (see GrowArgs)
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -256,13 +261,16 @@ __device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, 
 }
 
 // One-label lower bound of state (v, X): max over groups i not in X of dist(v, G_i), read from the
-// (final) level-1 row of v; inf when some missing group is unreachable within the bound.
-__device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, uint32_t v, uint32_t g, uint32_t X) {
+// (final) level-1 row of v; inf when some missing group is unreachable within the bound.  When level
+// 1 grew only states of cost <= gmax, a distance above gmax (or inf) says only that the true distance
+// exceeds gmax: each is capped at lbcap = gmax (bit patterns of non-negative floats order like the
+// floats, so the cap is a u32 min for both weight types).
+__device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, uint32_t v, uint32_t g, uint32_t X, uint32_t lbcap) {
     uint32_t lb = 0;
     const uint64_t* r1 = T1 + (uint64_t)v * g;
     for (uint32_t gi = 0; gi < g; ++gi) {
         if ((X >> gi) & 1u) continue;
-        const uint32_t di = (uint32_t)(r1[gi] >> 32);
+        const uint32_t di = min((uint32_t)(r1[gi] >> 32), lbcap);
         if (di == GST_INF_COST) return GST_INF_COST;
         lb = max(lb, di);
     }
@@ -453,7 +461,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
                 const unsigned long long own = lane_ok ? *addr : 0ull;
                 const uint32_t ocost = (uint32_t)(own >> 32);
-                const uint32_t lb = (prune_lb && lane_ok) ? label_lb(a.T1, u, a.g, a.masks[rk]) : 0u;
+                const uint32_t lb = (prune_lb && lane_ok) ? label_lb(a.T1, u, a.g, a.masks[rk], a.lbcap) : 0u;
                 unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;
                 for (uint32_t s = 0; s < maxdeg; s += UNR) {
                     uint32_t nb[UNR], wt[UNR], cn[UNR];
@@ -476,7 +484,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     for (uint32_t j = 0; j < UNR; ++j) {
                         if (nb[j] == 0xFFFFFFFFu) continue;
                         st_relax++;
-                        if (cn[j] < ocost) {
+                        if (cn[j] < ocost && cn[j] <= a.gmax) {
                             const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
                             if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
                                 const unsigned long long nw = ((unsigned long long)cand << 32) | (rs + s + j);
@@ -494,12 +502,15 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 }
                 const bool improved = lane_ok && (uint32_t)(nwv >> 32) < ocost;
                 if (lane_ok && nwv != own) *addr = nwv;
+                const uint64_t lmask = mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1);
                 const uint64_t imp = __ballot(improved);
+                // only states that may still grow go on the frontier
+                const uint64_t fl = __ballot(improved && (uint32_t)(nwv >> 32) <= a.gmax);
                 uint32_t enq_v = 0, enq_n = 0;
                 if (lane_ok && t0 == 0) {
-                    const uint64_t word = (imp >> (gslot * mchunk)) & (mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1));
+                    st_upd += (uint32_t)__popcll((imp >> (gslot * mchunk)) & lmask);
+                    const uint64_t word = (fl >> (gslot * mchunk)) & lmask;
                     if (word) {
-                        st_upd += (uint32_t)__popcll(word);
                         // single writer of u's change word in a pull round; the flag marks the
                         // first masks block that changed
                         cnxt[(uint64_t)u * W + (mb >> 6)] = word;
@@ -531,7 +542,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
             const unsigned long long own = *addr;
             const uint32_t ocost = (uint32_t)(own >> 32);
-            const uint32_t lb = prune_lb ? label_lb(a.T1, u, a.g, a.masks[rk]) : 0u;
+            const uint32_t lb = prune_lb ? label_lb(a.T1, u, a.g, a.masks[rk], a.lbcap) : 0u;
             unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;  // best via positive / zero-effect edges
             const uint32_t len = e1 - e0;
             const uint32_t steps = (len + A - 1) / A;
@@ -556,7 +567,8 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 for (uint32_t j = 0; j < UNR; ++j) {
                     if (nb[j] == 0xFFFFFFFFu) continue;
                     st_relax++;
-                    if (cn[j] < ocost) {  // only a cheaper neighbour can lower (or tie) our cost
+                    // only a cheaper neighbour can lower (or tie) our cost, and only one still grown
+                    if (cn[j] < ocost && cn[j] <= a.gmax) {
                         const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
                         if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
                             const uint32_t e = e0 + (s + j) * A + s_arc;
@@ -578,6 +590,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             uint32_t enq_v = 0, enq_n = 0;
             if (s_arc == 0) {
                 bool improved = false;
+                uint32_t ncost = GST_INF_COST;
                 if (bz != GST_INF_WORD && (uint32_t)(bz >> 32) < (uint32_t)(bs >> 32)) {
                     // a zero-effect edge wins: accept only a strict cost decrease
                     const uint32_t cz = (uint32_t)(bz >> 32);
@@ -587,12 +600,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         if (old == curw) { improved = true; break; }
                         curw = old;
                     }
+                    ncost = cz;
                 } else if (bs < own) {
                     const unsigned long long old = atomicMin(addr, bs);
                     improved = (uint32_t)(bs >> 32) < (uint32_t)(old >> 32);
+                    ncost = (uint32_t)(bs >> 32);
                 }
-                if (improved) {
-                    st_upd++;
+                if (improved) st_upd++;
+                if (improved && ncost <= a.gmax) {
                     const uint64_t bitv = 1ull << (rk & 63);
                     unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + (rk >> 6)];
                     const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
@@ -637,8 +652,8 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const uint32_t bit = select_bit(bm, t0);
                     const uint32_t rk = wd * 64 + bit;
                     const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
-                    const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
-                    if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep |= 1ull << bit;
+                    const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk], a.lbcap) : 0u;
+                    if (cv <= a.gmax && !reaches_ub<FLT>(cv, lb, a.ub)) keep |= 1ull << bit;
                 }
                 bm = group_or64(keep, PL);
             }
@@ -704,8 +719,8 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                         improved = cand < (uint32_t)(old >> 32);
                                     }
                                 }
-                                if (improved) {
-                                    st_upd++;
+                                if (improved) st_upd++;
+                                if (improved && cand <= a.gmax) {
                                     unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
                                     if (!(*cwp & bitv)) {
                                         const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
@@ -841,8 +856,8 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
             const uint32_t bit = select_bit(bm, lane);
             const uint32_t rk = wd * 64 + bit;
             const uint32_t cv = (uint32_t)(ld_agent((const unsigned long long*)(a.Tk + (uint64_t)v * m + rk)) >> 32);
-            const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk]) : 0u;
-            if (cv != GST_INF_COST && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
+            const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk], a.lbcap) : 0u;
+            if (cv <= a.gmax && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
         }
         bm = wave_or64(keep);
     }
@@ -907,6 +922,7 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
             }
             if (!improved) continue;
             st_upd++;
+            if (cand > a.gmax) continue;   // above half the bound: kept, never grown
             unsigned long long* cwp = (unsigned long long*)&chg[(uint64_t)u * W + wsel];
             const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
             const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inq[u], 1u) == 0u);
@@ -1201,7 +1217,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                 if (prune_lb) {
                     for (uint32_t gi = 0; gi < g; ++gi) {
                         if ((X >> gi) & 1u) continue;
-                        const uint32_t di = rv[gi];  // level-1 position gi = mask 1<<gi
+                        const uint32_t di = min(rv[gi], a.lbcap);  // level-1 position gi = mask 1<<gi
                         lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
                     }
                 }
@@ -1212,7 +1228,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                 if (a.is_full) {
                     const unsigned long long rw = ((best >> 32) << 32) | v;
                     best_root = rw < best_root ? rw : best_root;
-                } else {
+                } else if ((uint32_t)(best >> 32) <= a.gmax) {
                     atomicOr(&bm[b * W + (r >> 6)], 1ull << (r & 63));
                 }
             }

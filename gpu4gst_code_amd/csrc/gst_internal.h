@@ -3,6 +3,8 @@
 #pragma once
 
 #include <stdint.h>
+#include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -72,6 +74,36 @@ inline int64_t gst_find_arc(const int64_t* row, const int32_t* col, const uint32
         if (arc_key(w[mid], col[mid]) < key) lo = mid + 1; else hi = mid;
     }
     return (lo < row[u + 1] && col[lo] == v && w[lo] == wv) ? lo : -1;
+}
+
+// Cost bits of a bound U for the kernels:
+//   ub    : states of cost >= ub are dropped (ints: floor(U)+1, so cost <= U stays; floats: U with a
+//           relative slack of 1e-4, because the DP sums fp32 costs in its own order and rounding must
+//           never drop a structure whose exact cost is <= U)
+//   gmax  : with half_bound == 0, only states of cost <= U/2 are grown (ints: floor(floor(U)/2);
+//           floats: half the slackened bound); otherwise every finite state below ub
+//   lbcap : cap on level-1 distances in the lower bound: gmax when level 1 is cut at gmax, else inf
+struct gst_bound_bits {
+    uint32_t ub = 0xFFFFFFFFu, gmax = 0xFFFFFFFEu, lbcap = 0xFFFFFFFFu;
+};
+inline gst_bound_bits gst_make_bound(double U, bool flt, int32_t half_bound) {
+    gst_bound_bits b;
+    if (!(std::isfinite(U) && U > 0)) return b;
+    if (flt) {
+        float fu = (float)(U * (1.0 + 1e-4));
+        fu = std::nextafter(fu, INFINITY);
+        std::memcpy(&b.ub, &fu, 4);
+        if (half_bound == 0) {
+            float fh = std::nextafter(fu * 0.5f, INFINITY);
+            std::memcpy(&b.gmax, &fh, 4);
+        }
+    } else {
+        const double c = std::floor(U) + 1.0;
+        b.ub = c >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)c;
+        if (half_bound == 0 && b.ub != 0xFFFFFFFFu) b.gmax = (b.ub - 1u) / 2u;
+    }
+    if (half_bound == 0 && b.gmax != 0xFFFFFFFEu) b.lbcap = b.gmax;
+    return b;
 }
 
 inline bool gst_push_lanes_ok(int32_t p) { return p == 0 || p == 8 || p == 16 || p == 32 || p == 64; }

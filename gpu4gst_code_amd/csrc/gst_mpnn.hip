@@ -14,10 +14,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
+#include "gst_batch.h"
 #include "gst_internal.h"
 #include "gst_mpnn.h"
 
@@ -56,24 +58,27 @@ __device__ __forceinline__ bf16 f2bf(float x) { return (bf16)x; }  // round to n
 // ---------------------------------------------------------------------------------------
 // features
 // ---------------------------------------------------------------------------------------
+// Rows of an uploaded graph run in increasing weight, so the lightest incident arc is the first.
+// Two 16-byte stores per vertex.
 __global__ void k_features(const uint32_t* row, const uint32_t* w, int w_float, const uint32_t* gm, uint32_t n,
                            bf16* x) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const uint32_t rs = row[v], re = row[v + 1];
         float minw = 0.f;
-        if (re > rs) {
-            minw = INFINITY;
-            for (uint32_t e = rs; e < re; ++e) {
-                const float wf = w_float ? __uint_as_float(w[e]) : (float)w[e];
-                minw = fminf(minw, wf);
-            }
-        }
+        if (re > rs) minw = w_float ? __uint_as_float(w[rs]) : (float)w[rs];
         const uint32_t gmv = gm[v];
-        bf16* xv = x + (uint64_t)v * FIN;
-        xv[0] = f2bf(log1pf((float)(re - rs)) * 0.125f);
-        xv[1] = f2bf(log1pf(minw) * 0.125f);
-        for (int i = 0; i < 8; ++i) xv[2 + i] = f2bf(((gmv >> i) & 1u) ? 1.f : 0.f);
-        for (int i = 10; i < FIN; ++i) xv[i] = f2bf(0.f);
+        bf16x8 a, b;
+        a[0] = f2bf(log1pf((float)(re - rs)) * 0.125f);
+        a[1] = f2bf(log1pf(minw) * 0.125f);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a[2 + i] = f2bf(((gmv >> i) & 1u) ? 1.f : 0.f);
+        b[0] = f2bf(((gmv >> 6) & 1u) ? 1.f : 0.f);
+        b[1] = f2bf(((gmv >> 7) & 1u) ? 1.f : 0.f);
+#pragma unroll
+        for (int i = 2; i < 8; ++i) b[i] = f2bf(0.f);
+        bf16x8* xv = (bf16x8*)(x + (uint64_t)v * FIN);
+        xv[0] = a;
+        xv[1] = b;
     }
 }
 
@@ -85,8 +90,72 @@ __global__ void k_features(const uint32_t* row, const uint32_t* w, int w_float, 
 // One wave per light row: lanes = (neighbour slot, feature pair).  Rows longer than AGG_HUB are
 // split into AGG_HUB-arc chunks whose partial sums are added atomically into fp32 scratch.
 // ---------------------------------------------------------------------------------------
+// forward: lanes = (neighbour slot, 8-feature chunk); each lane gathers 16 bytes (8 bf16) of a
+// neighbour row per step, two steps in flight, and the slots are summed by shuffles
+template <int D>
+__global__ void __launch_bounds__(256) k_aggregate_fwd(const uint32_t* row, const int32_t* col, uint32_t n,
+                                                       const bf16* in, bf16* out, float* hub_acc,
+                                                       const uint32_t* hub_rows, uint32_t n_hub_items) {
+    constexpr int CH = D / 8;           // 16-byte chunks per row
+    constexpr int SLOTS = 64 / CH;      // neighbours per step
+    const uint32_t lane = __lane_id();
+    const uint32_t ch = lane % CH, slot = lane / CH;
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    const uint32_t total = n + n_hub_items;
+    for (uint32_t it = gwave; it < total; it += nwaves) {
+        uint32_t u, e0, e1, h = 0;
+        const bool hub = it >= n;
+        if (!hub) {
+            u = it;
+            e0 = row[u];
+            e1 = row[u + 1];
+            if (e1 - e0 > AGG_HUB) continue;
+        } else {
+            const uint32_t i = it - n;
+            u = hub_rows[2 * i];
+            h = hub_rows[2 * i + 1] & 0xFFFFFu;
+            const uint32_t c = hub_rows[2 * i + 1] >> 20;
+            const uint32_t rs = row[u], re = row[u + 1];
+            e0 = rs + c * AGG_HUB;
+            e1 = min(re, e0 + AGG_HUB);
+        }
+        float acc[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] = 0.f;
+        for (uint32_t e = e0 + slot; e < e1; e += 2 * SLOTS) {
+            const uint32_t nb0 = (uint32_t)col[e];
+            const bool two = e + SLOTS < e1;
+            const uint32_t nb1 = two ? (uint32_t)col[e + SLOTS] : 0u;
+            const bf16x8 x0 = *(const bf16x8*)(in + (uint64_t)nb0 * D + 8 * ch);
+            bf16x8 x1;
+            if (two) x1 = *(const bf16x8*)(in + (uint64_t)nb1 * D + 8 * ch);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += bf2f(x0[j]) + (two ? bf2f(x1[j]) : 0.f);
+        }
+#pragma unroll
+        for (int o = CH; o < 64; o <<= 1) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += __shfl_xor(acc[j], o, 64);
+        }
+        if (slot == 0) {
+            if (!hub) {
+                const uint32_t deg = e1 - e0;
+                const float inv = deg ? 1.f / (float)deg : 0.f;
+                bf16x8 o;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) o[j] = f2bf(acc[j] * inv);
+                *(bf16x8*)(out + (uint64_t)u * D + 8 * ch) = o;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) atomicAdd(&hub_acc[(uint64_t)h * D + 8 * ch + j], acc[j]);
+            }
+        }
+    }
+}
+
 template <int D, bool BWD>
-__global__ void __launch_bounds__(256) k_aggregate(const uint32_t* row, const int32_t* col, uint32_t n, const void* in_,
+__global__ void __launch_bounds__(256) k_aggregate_bwd(const uint32_t* row, const int32_t* col, uint32_t n, const void* in_,
                                                    void* out_, float* hub_acc, const uint32_t* hub_rows,
                                                    uint32_t n_hub_items) {
     constexpr int FP = D / 2;            // feature pairs
@@ -443,19 +512,70 @@ __device__ __forceinline__ uint32_t score_key(float s) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-// histogram of the top 16 key bits (pass 0) or of the low 16 bits among keys whose high 16 bits
-// equal `prefix` (pass 1)
-__global__ void k_key_hist(const float* s, uint32_t n, int pass, uint32_t prefix, uint32_t* hist) {
+// Top-k threshold by a three-pass radix select (11, 11 and 10 bits, high to low) over
+// order-preserving keys, with no host round trip.  sel_state: [0] keys still wanted (k at the
+// start, the tie budget at the end), [1] the digits of the threshold key found so far.
+__device__ __forceinline__ void radix_digit(int pass, uint32_t& lo, uint32_t& nb) {
+    lo = pass == 0 ? 21u : (pass == 1 ? 10u : 0u);
+    nb = pass == 2 ? 10u : 11u;
+}
+
+// histogram of this pass's digit over the keys whose higher bits equal the digits found so far,
+// privatised in LDS per workgroup (only non-empty bins reach the global histogram)
+__global__ void __launch_bounds__(256) k_key_hist(const float* s, uint32_t n, int pass, const uint32_t* sel_state,
+                                                  uint32_t* hist) {
+    __shared__ uint32_t lh[2048];
+    uint32_t lo, nb;
+    radix_digit(pass, lo, nb);
+    const uint32_t hi = lo + nb;
+    const uint32_t prefix = sel_state[1];
+    const uint32_t nbins = 1u << nb;
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const uint32_t k = score_key(s[v]);
-        if (pass == 0) atomicAdd(&hist[k >> 16], 1u);
-        else if ((k >> 16) == prefix) atomicAdd(&hist[k & 0xFFFFu], 1u);
+        if (hi >= 32u || (k >> hi) == prefix) atomicAdd(&lh[(k >> lo) & (nbins - 1u)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+// the digit whose bin, counted from the top, first reaches the wanted number (1 <= wanted <= keys
+// left): thread t sums nbins/256 bins from the top, a prefix over the 256 partial sums finds the
+// thread whose range holds the crossing, and that thread walks its bins and appends the digit
+__global__ void __launch_bounds__(256) k_pick_bin(const uint32_t* hist, uint32_t* sel_state, int pass) {
+    __shared__ uint32_t part[256];
+    uint32_t lo, nb;
+    radix_digit(pass, lo, nb);
+    const uint32_t nbins = 1u << nb, per = nbins / 256u;
+    const uint32_t t = threadIdx.x;
+    const uint32_t need = sel_state[0];
+    const uint32_t top = nbins - 1u - per * t;
+    uint32_t sum = 0;
+    for (uint32_t j = 0; j < per; ++j) sum += hist[top - j];
+    part[t] = sum;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t j = 0; j < t; ++j) before += part[j];
+    if (before < need && before + sum >= need) {
+        uint32_t acc = before;
+        for (uint32_t j = 0; j < per; ++j) {
+            const uint32_t h = hist[top - j];
+            if (acc + h >= need) {
+                sel_state[1] = (sel_state[1] << nb) | (top - j);
+                sel_state[0] = need - acc;
+                break;
+            }
+            acc += h;
+        }
     }
 }
 
-// select v when key >= thr (ties at thr resolved by a count budget), plus every group member
-__global__ void k_select(const float* s, const uint32_t* gm, uint32_t n, uint32_t thr, int* tie_budget,
-                         uint32_t* sel) {
+// select v when key > threshold or key == threshold within the tie budget, plus every group member
+__global__ void k_select(const float* s, const uint32_t* gm, uint32_t n, uint32_t* sel_state, uint32_t* sel) {
+    const uint32_t thr = sel_state[1];
+    int* tie_budget = (int*)&sel_state[0];
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const uint32_t k = score_key(s[v]);
         uint32_t x = (k > thr) ? 1u : 0u;
@@ -497,12 +617,24 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* in, uint32_
     if (threadIdx.x == 0) bsum[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
 }
 
-// exclusive scan of the block sums (single workgroup, sequential chunks), total in bsum[nb]
-__global__ void k_scan_sums(uint32_t* bsum, uint32_t nb) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// exclusive scan of the block sums by one wave (64 at a time with a running offset), total in bsum[nb]
+__global__ void __launch_bounds__(64) k_scan_sums(uint32_t* bsum, uint32_t nb) {
+    if (blockIdx.x != 0) return;
+    const uint32_t lane = threadIdx.x;
     uint32_t run = 0;
-    for (uint32_t i = 0; i < nb; ++i) { const uint32_t x = bsum[i]; bsum[i] = run; run += x; }
-    bsum[nb] = run;
+    for (uint32_t base = 0; base < nb; base += 64) {
+        const uint32_t i = base + lane;
+        const uint32_t x = i < nb ? bsum[i] : 0u;
+        uint32_t incl = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d, 64);
+            if (lane >= (uint32_t)d) incl += y;
+        }
+        if (i < nb) bsum[i] = run + incl - x;
+        run += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) bsum[nb] = run;
 }
 
 __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* bsum) {
@@ -571,6 +703,10 @@ __global__ void k_sub_rev(const uint32_t* srow, const int32_t* scol, const uint3
     }
 }
 
+__global__ void k_set_u32(uint32_t* p, const uint32_t* val) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = *val;
+}
+
 __global__ void k_sub_groups(const int32_t* gv, uint32_t cnt, const uint32_t* newid, int32_t* out) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
         out[i] = (int32_t)newid[gv[i]];
@@ -628,6 +764,8 @@ struct gst_model {
     uint64_t hub_uid = 0;
     uint32_t* sel_scratch = nullptr;   // top-k / subgraph scratch (u32)
     uint64_t cap_scratch = 0;
+    uint32_t* h_sel = nullptr;         // pinned readback of a selection (sizes, group ids)
+    uint32_t cap_h_sel = 0;
     uint32_t* hub_list = nullptr;   // hub row ids
     uint32_t* hub_items = nullptr;  // pairs (row, h | chunk << 20)
     uint32_t n_hub = 0, n_hub_items = 0;
@@ -700,8 +838,13 @@ void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipS
     const int fg = std::max(1, M->ctx->cu_count * 8);
     if (M->n_hub) k_fill32<<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>((uint32_t*)M->hub_acc, 0u,
                                                                                (uint64_t)M->n_hub * D);
-    k_aggregate<D, BWD><<<std::max(1, M->ctx->cu_count * 16), 256, 0, st>>>(G->d_row, G->d_col, (uint32_t)G->n, in, out,
-                                                                          M->hub_acc, M->hub_items, M->n_hub_items);
+    if (BWD)
+        k_aggregate_bwd<D, BWD><<<std::max(1, M->ctx->cu_count * 16), 256, 0, st>>>(G->d_row, G->d_col, (uint32_t)G->n, in,
+                                                                                  out, M->hub_acc, M->hub_items, M->n_hub_items);
+    else
+        k_aggregate_fwd<D><<<std::max(1, M->ctx->cu_count * 16), 256, 0, st>>>(G->d_row, G->d_col, (uint32_t)G->n,
+                                                                             (const bf16*)in, (bf16*)out, M->hub_acc,
+                                                                             M->hub_items, M->n_hub_items);
     if (M->n_hub)
         k_aggregate_hub_finish<D, BWD><<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>(G->d_row, M->hub_list,
                                                                                         M->n_hub, M->hub_acc, out);
@@ -774,6 +917,7 @@ void gst_model_free(gst_model* M) {
     hipFree(M->s); hipFree(M->dS); hipFree(M->dX2); hipFree(M->dH1); hipFree(M->dA1); hipFree(M->hub_acc);
     hipFree(M->gm); hipFree(M->y); hipFree(M->grp); hipFree(M->hub_list); hipFree(M->hub_items);
     hipFree(M->sel_scratch);
+    if (M->h_sel) hipHostFree(M->h_sel);
     delete M;
 }
 
@@ -867,130 +1011,164 @@ gst_status gst_model_adam_step(gst_model* M, float lr) {
 // ---------------------------------------------------------------------------------------
 // learned-bound solve
 // ---------------------------------------------------------------------------------------
-gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* groups, gst_model* M, double frac,
-                             const gst_opts* opts_in, gst_result* res, gst_learned_stats* ls) {
+namespace {
+
+// device CSR of one induced subgraph (reused across queries while large enough)
+struct SubBuf {
+    uint32_t* row = nullptr;
+    int32_t* col = nullptr;
+    uint32_t* w = nullptr;
+    uint32_t* rev = nullptr;
+    uint32_t cap_n = 0;
+    uint64_t cap_m = 0;
+};
+
+void sub_free(SubBuf& b) {
+    hipFree(b.row); hipFree(b.col); hipFree(b.w); hipFree(b.rev);
+    b = SubBuf();
+}
+
+bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms) {
+    if (b.cap_n >= ns + 1 && b.cap_m >= std::max(ms, 1u)) return true;
+    sub_free(b);
+    const uint32_t cn = ns + 1 + ns / 4 + 64;
+    const uint64_t cm = (uint64_t)std::max(ms, 1u) + ms / 4 + 256;
+    if (hipMalloc(&b.row, cn * 4ull) != hipSuccess || hipMalloc(&b.col, cm * 4) != hipSuccess ||
+        hipMalloc(&b.w, cm * 4) != hipSuccess || hipMalloc(&b.rev, cm * 4) != hipSuccess) {
+        sub_free(b);
+        return false;
+    }
+    b.cap_n = cn;
+    b.cap_m = cm;
+    return true;
+}
+
+// Scores, top-k selection and the subgraph induced by the top-k vertices plus every group member,
+// for one query, all on the context's stream with one host synchronisation (for the subgraph's
+// sizes and its group lists).  S is filled as a view of buf (never freed as a graph); with
+// want_host_rows the host row offsets are copied too (the one-query solver reads them).
+gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SubBuf& buf,
+                            gst_graph& S, std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool want_host_rows) {
+    gst_ctx* ctx = M->ctx;
     hipStream_t st = (hipStream_t)ctx->stream;
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
-    gst_learned_stats L{};
-    hipEvent_t e0, e1, e2, e3;
-    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2)); HIPCHK(hipEventCreate(&e3));
-    HIPCHK(hipEventRecord(e0, st));
     gst_status sst = forward(M, G, groups);
     if (sst != GST_OK) return sst;
-    // ---- top-k threshold by a two-pass 16-bit radix select over order-preserving keys ----
     uint32_t k_target = (uint32_t)std::min<double>(n, std::ceil(frac * n));
     if (k_target < 1) k_target = 1;
-    static thread_local std::vector<uint32_t> hh;
-    hh.assign(65536, 0);
-    // scratch: histogram (65536) | sel (n) | newid (n) | deg (n) | arc offsets (n) | block sums
+    const int g = groups->n_groups;
+    const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
+    // scratch: histogram (2048) | select state (8) | sel (n) | newid (n) | deg (n) | arc offsets (n) |
+    // block sums 2 (nbk + 2) | sub group ids (gcnt)
     const uint32_t nbk = (n + 1023) / 1024;
-    const uint64_t need = 65536ull + 4ull * n + 2ull * (nbk + 2) + 64;
+    const uint64_t need = 2048ull + 8 + 4ull * n + 2ull * (nbk + 2) + gcnt + 64;
     if (M->cap_scratch < need) {
         if (!dalloc(M->sel_scratch, need)) { M->cap_scratch = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
         M->cap_scratch = need;
     }
     uint32_t* d_hist = M->sel_scratch;
-    uint32_t* d_sel = d_hist + 65536;
+    uint32_t* d_state = d_hist + 2048;
+    uint32_t* d_sel = d_state + 8;
     uint32_t* d_newid = d_sel + n;
     uint32_t* d_deg = d_newid + n;
     uint32_t* d_arcoff = d_deg + n;
     uint32_t* d_bsum = d_arcoff + n;     // 2 * (nbk + 2)
-    k_fill32<<<64, 256, 0, st>>>(d_hist, 0u, 65536);
-    k_key_hist<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, n, 0, 0, d_hist);
-    HIPCHK(hipMemcpyAsync(hh.data(), d_hist, 65536 * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    uint32_t acc = 0, hi = 0;
-    for (int b = 65535; b >= 0; --b) {
-        if (acc + hh[(size_t)b] >= k_target) { hi = (uint32_t)b; break; }
-        acc += hh[(size_t)b];
+    int32_t* d_subgrp = (int32_t*)(d_bsum + 2 * (nbk + 2));
+    // pinned readback: ns, ms, then the subgraph ids of the group members
+    if (M->cap_h_sel < gcnt + 2) {
+        if (M->h_sel) hipHostFree(M->h_sel);
+        M->h_sel = nullptr;
+        M->cap_h_sel = 0;
+        if (hipHostMalloc((void**)&M->h_sel, (gcnt + 2 + 1024) * 4ull) != hipSuccess)
+            return gst_set_error(ctx, GST_OOM, "learned solve: pinned allocation failed");
+        M->cap_h_sel = gcnt + 2 + 1024;
     }
-    const uint32_t need_hi = k_target - acc;   // keys with high bits == hi still needed
-    k_fill32<<<64, 256, 0, st>>>(d_hist, 0u, 65536);
-    k_key_hist<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, n, 1, hi, d_hist);
-    HIPCHK(hipMemcpyAsync(hh.data(), d_hist, 65536 * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    uint32_t acc2 = 0, lo = 0;
-    for (int b = 65535; b >= 0; --b) {
-        if (acc2 + hh[(size_t)b] >= need_hi) { lo = (uint32_t)b; break; }
-        acc2 += hh[(size_t)b];
+    // ---- top-k threshold on the device ----
+    k_fill32<<<1, 64, 0, st>>>(d_state, 0u, 8);
+    k_fill32<<<1, 64, 0, st>>>(d_state, k_target, 1);
+    const int hg = std::max(1, std::min(ctx->cu_count, (int)((n + 4095) / 4096)));
+    for (int pass = 0; pass < 3; ++pass) {
+        k_fill32<<<8, 256, 0, st>>>(d_hist, 0u, 2048);
+        k_key_hist<<<hg, 256, 0, st>>>(M->s, n, pass, d_state, d_hist);
+        k_pick_bin<<<1, 256, 0, st>>>(d_hist, d_state, pass);
     }
-    const uint32_t thr = (hi << 16) | lo;
-    const uint32_t ties = need_hi - acc2;
-    int* d_tie = (int*)d_hist;   // the histogram is no longer needed
-    const int ties_i = (int)ties;
-    HIPCHK(hipMemcpyAsync(d_tie, &ties_i, 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));   // ties_i lives on this stack frame
-    k_select<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, M->gm, n, thr, d_tie, d_sel);
-    // ---- induced subgraph ----
-    const uint32_t nb = nbk;
-    k_scan_blocks<<<nb, 256, 0, st>>>(d_sel, n, d_newid, d_bsum);
-    k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nb);
+    k_select<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, M->gm, n, d_state, d_sel);
+    // ---- induced subgraph: new ids, arc offsets, group lists in new ids ----
+    k_scan_blocks<<<nbk, 256, 0, st>>>(d_sel, n, d_newid, d_bsum);
+    k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nbk);
     k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_newid, n, d_bsum);
     k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, n, d_deg);
-    k_scan_blocks<<<nb, 256, 0, st>>>(d_deg, n, d_arcoff, d_bsum + nb + 2);
-    k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nb + 2, nb);
-    k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_arcoff, n, d_bsum + nb + 2);
-    uint32_t tots[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&tots[0], d_bsum + nb, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&tots[1], d_bsum + nb + 2 + nb, 4, hipMemcpyDeviceToHost, st));
+    k_scan_blocks<<<nbk, 256, 0, st>>>(d_deg, n, d_arcoff, d_bsum + nbk + 2);
+    k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nbk + 2, nbk);
+    k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_arcoff, n, d_bsum + nbk + 2);
+    if (gcnt) k_sub_groups<<<gridn(gcnt, 256, 64), 256, 0, st>>>(M->grp + g + 1, gcnt, d_newid, d_subgrp);
+    HIPCHK(hipMemcpyAsync(&M->h_sel[0], d_bsum + nbk, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&M->h_sel[1], d_bsum + nbk + 2 + nbk, 4, hipMemcpyDeviceToHost, st));
+    if (gcnt) HIPCHK(hipMemcpyAsync(&M->h_sel[2], d_subgrp, gcnt * 4ull, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const uint32_t ns = tots[0], ms = tots[1];
+    const uint32_t ns = M->h_sel[0], ms = M->h_sel[1];
+    sub_vtx.assign((const int32_t*)&M->h_sel[2], (const int32_t*)&M->h_sel[2] + gcnt);
     L.sub_vertices = ns;
     L.sub_arcs = ms;
-    gst_graph* S = new gst_graph();
-    S->ctx = ctx; S->uid = gst_next_graph_uid(); S->n = (int32_t)ns;
-    S->budget_key = G->uid | (1ull << 63);   // learned subgraphs of one graph share round budgets
-    S->m = ms;
-    S->w_type = G->w_type;
-    uint32_t* d_old = nullptr;
-    bool ok = hipMalloc(&S->d_row, (ns + 1) * 4ull) == hipSuccess && hipMalloc(&S->d_col, std::max(ms, 1u) * 4ull) == hipSuccess &&
-              hipMalloc(&S->d_w, std::max(ms, 1u) * 4ull) == hipSuccess && hipMalloc(&S->d_rev, std::max(ms, 1u) * 4ull) == hipSuccess &&
-              hipMalloc(&d_old, std::max(ns, 1u) * 4ull) == hipSuccess;
-    if (!ok) { gst_graph_device_free(S); delete S; hipFree(d_old); return gst_set_error(ctx, GST_OOM, "learned solve: subgraph allocation failed"); }
+    if (!sub_reserve(buf, ns, ms)) return gst_set_error(ctx, GST_OOM, "learned solve: subgraph allocation failed");
+    S.ctx = ctx;
+    S.uid = gst_next_graph_uid();
+    S.budget_key = G->uid | (1ull << 63);   // learned subgraphs of one graph share round budgets
+    S.n = (int32_t)ns;
+    S.m = ms;
+    S.w_type = G->w_type;
+    S.d_row = buf.row; S.d_col = buf.col; S.d_w = buf.w; S.d_rev = buf.rev;
+    S.max_deg = 0;
+    if (ns == 0) return GST_OK;
+    // old ids of the subgraph vertices are not needed: the bound is a cost only
     k_sub_fill<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, n,
-                                                             S->d_row, S->d_col, S->d_w, d_old);
-    HIPCHK(hipMemcpyAsync(S->d_row + ns, &ms, 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));   // ms lives on this stack frame
-    k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S->d_row, S->d_col, S->d_w, ns, S->d_rev);
-    // host row offsets of the subgraph (the solver reads degrees for its work-item count)
-    S->h_row.resize((size_t)ns + 1);
-    {
+                                                             S.d_row, S.d_col, S.d_w, d_deg /* scratch for old ids */);
+    k_set_u32<<<1, 1, 0, st>>>(S.d_row + ns, d_bsum + nbk + 2 + nbk);
+    k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S.d_row, S.d_col, S.d_w, ns, S.d_rev);
+    HIPCHK(hipGetLastError());
+    if (want_host_rows) {
         std::vector<uint32_t> r32((size_t)ns + 1);
-        HIPCHK(hipMemcpyAsync(r32.data(), S->d_row, (ns + 1) * 4ull, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(r32.data(), S.d_row, (ns + 1) * 4ull, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        for (size_t i = 0; i <= ns; ++i) S->h_row[i] = r32[i];
+        S.h_row.assign(r32.begin(), r32.end());
     }
-    // groups in subgraph ids
-    const int g = groups->n_groups;
-    const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
-    std::vector<int32_t> sub_vtx(gcnt);
-    {
-        int32_t* d_tmp = nullptr;
-        if (hipMalloc(&d_tmp, std::max(gcnt, 1u) * 4ull) != hipSuccess) { gst_graph_device_free(S); delete S; hipFree(d_old); return gst_set_error(ctx, GST_OOM, "learned solve: allocation failed"); }
-        k_sub_groups<<<gridn(gcnt, 256, 64), 256, 0, st>>>(M->grp + g + 1, gcnt, d_newid, d_tmp);
-        HIPCHK(hipMemcpyAsync(sub_vtx.data(), d_tmp, gcnt * 4ull, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        hipFree(d_tmp);
-    }
+    return GST_OK;
+}
+
+}  // namespace
+
+gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* groups, gst_model* M, double frac,
+                             const gst_opts* opts_in, gst_result* res, gst_learned_stats* ls) {
+    hipStream_t st = (hipStream_t)ctx->stream;
+    gst_learned_stats L{};
+    hipEvent_t e0, e1, e2, e3;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2)); HIPCHK(hipEventCreate(&e3));
+    HIPCHK(hipEventRecord(e0, st));
+    SubBuf buf;
+    gst_graph S;
+    std::vector<int32_t> sub_vtx;
+    gst_status sst = learned_subgraph(M, G, groups, frac, buf, S, sub_vtx, L, true);
+    if (sst != GST_OK) { sub_free(buf); return sst; }
+    const uint32_t ns = (uint32_t)S.n;
     HIPCHK(hipEventRecord(e1, st));
+    const int g = groups->n_groups;
     gst_groups sg{g, groups->grp_ptr, sub_vtx.data()};
     gst_opts so{};
     so.prune_lb = 1;
-    so.want_tree = 1;
+    so.want_tree = 0;   // the bound is a cost
     gst_result sr;
     std::memset(&sr, 0, sizeof(sr));
     double ub = 0.0;
     bool have_ub = false;
     if (ns > 0) {
-        gst_status ss = gst_dpbf_solve(ctx, S, &sg, &so, &sr);
+        gst_status ss = gst_dpbf_solve(ctx, &S, &sg, &so, &sr);
         if (ss == GST_OK) { ub = sr.cost; have_ub = true; }
         gst_result_free(&sr);
     }
     HIPCHK(hipEventRecord(e2, st));
-    gst_graph_device_free(S);
-    delete S;
-    hipFree(d_old);
+    HIPCHK(hipStreamSynchronize(st));
+    sub_free(buf);
     // ---- full solve, bounded by the subgraph tree when there is one ----
     gst_opts fo{};
     if (opts_in) fo = *opts_in;
@@ -1019,4 +1197,123 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
     hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
     if (ls) *ls = L;
     return fs;
+}
+
+// ---------------------------------------------------------------------------------------
+// batched learned-bound solve: per query a subgraph job and then a bounded full job, both run by
+// the batched engine, so the subgraph solves and full solves of many queries share launches
+// ---------------------------------------------------------------------------------------
+gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq, const gst_groups* groups,
+                                   gst_model* M, double frac, const gst_opts* opts, const gst_batch_opts* bopts,
+                                   gst_result* results, gst_status* status, gst_learned_stats* lstats,
+                                   gst_batch_stats* bstats) {
+    struct Q {
+        gst_graph S;
+        int buf = -1;                   // subgraph buffer in use (index into bufs)
+        std::vector<int32_t> sub_vtx;
+        gst_groups sg{};
+        gst_result sub_res{};
+        gst_learned_stats L{};
+        double ub = 0.0;
+        bool have_ub = false, retried = false;
+        double t_sub_fill = 0.0, t_full_fill = 0.0;
+    };
+    std::vector<Q> qs((size_t)std::max(nq, 0));
+    std::vector<SubBuf> bufs;
+    std::vector<int> free_bufs;
+    std::vector<int32_t> pending_full;   // queries whose full solve may start
+    int32_t next = 0;
+    int gmax = 1;
+    for (int32_t q = 0; q < nq; ++q)
+        if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
+    gst_status setup_err = GST_OK;
+    auto full_job = [&](int32_t q, gst_batch_job& j) {
+        Q& x = qs[(size_t)q];
+        j = gst_batch_job();
+        j.G = G;
+        j.groups = &groups[q];
+        if (opts) j.opts = opts[q];
+        else { j.opts = gst_opts(); j.opts.prune_lb = 1; j.opts.want_tree = 1; }
+        j.opts.ub = x.have_ub && !x.retried ? x.ub : 0.0;
+        j.res = &results[q];
+        j.tag = q;
+        j.kind = 1;
+    };
+    auto next_job = [&](gst_batch_job& j) -> bool {
+        if (!pending_full.empty()) {
+            const int32_t q = pending_full.back();
+            pending_full.pop_back();
+            full_job(q, j);
+            return true;
+        }
+        while (next < nq && setup_err == GST_OK) {
+            const int32_t q = next++;
+            if (status[q] != GST_OK) continue;
+            Q& x = qs[(size_t)q];
+            if (free_bufs.empty()) { bufs.emplace_back(); free_bufs.push_back((int)bufs.size() - 1); }
+            x.buf = free_bufs.back();
+            free_bufs.pop_back();
+            const double t0 = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+            gst_status ss = learned_subgraph(M, G, &groups[q], frac, bufs[(size_t)x.buf], x.S, x.sub_vtx, x.L, false);
+            x.L.t_score_select_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+            if (ss != GST_OK && ss != GST_OOM) { setup_err = ss; status[q] = ss; free_bufs.push_back(x.buf); x.buf = -1; return false; }
+            if (ss != GST_OK || x.S.n == 0) {
+                // no subgraph: the plain solve runs (a fallback)
+                free_bufs.push_back(x.buf);
+                x.buf = -1;
+                full_job(q, j);
+                return true;
+            }
+            x.sg = gst_groups{groups[q].n_groups, groups[q].grp_ptr, x.sub_vtx.data()};
+            j = gst_batch_job();
+            j.G = &x.S;
+            j.groups = &x.sg;
+            j.opts = gst_opts();
+            j.opts.prune_lb = 1;
+            j.opts.want_tree = 0;
+            j.temp_graph = true;
+            j.res = &x.sub_res;
+            j.tag = q;
+            j.kind = 0;
+            return true;
+        }
+        return false;
+    };
+    std::vector<gst_status> job_fail;   // statuses of failed jobs (the engine returns the first of them)
+    auto job_done = [&](gst_batch_job& j) {
+        const int32_t q = (int32_t)j.tag;
+        Q& x = qs[(size_t)q];
+        if (j.status != GST_OK) job_fail.push_back(j.status);
+        if (j.kind == 0) {
+            x.L.t_sub_solve_ms = j.t_done - j.t_fill;
+            if (j.status == GST_OK) { x.have_ub = true; x.ub = x.sub_res.cost; }
+            gst_result_free(&x.sub_res);
+            if (x.buf >= 0) { free_bufs.push_back(x.buf); x.buf = -1; }
+            pending_full.push_back(q);
+            return;
+        }
+        x.L.t_full_solve_ms += j.t_done - j.t_fill;
+        if (j.status == GST_INFEASIBLE && x.have_ub && !x.retried) {
+            // the bound cannot be below the optimum; only float rounding gets here: run unbounded
+            x.retried = true;
+            gst_result_free(&results[q]);
+            pending_full.push_back(q);
+            return;
+        }
+        status[q] = j.status;
+        if (j.status == GST_OK) results[q].fell_back = (!x.have_ub || x.retried) ? 1 : 0;
+        x.L.ub = x.have_ub ? x.ub : -1.0;
+        if (lstats) lstats[q] = x.L;
+    };
+    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats);
+    for (auto& b : bufs) sub_free(b);
+    for (auto& x : qs) gst_result_free(&x.sub_res);
+    if (setup_err != GST_OK) return setup_err;
+    // a failed subgraph solve (no tree inside the subgraph) is no failure of its query: the
+    // engine's own errors are returned, job outcomes are read from status[]
+    if (rs != GST_OK && std::find(job_fail.begin(), job_fail.end(), rs) == job_fail.end()) return rs;
+    gst_status first = GST_OK;
+    for (int32_t q = 0; q < nq && first == GST_OK; ++q)
+        if (status[q] != GST_OK) first = status[q];
+    return first;
 }

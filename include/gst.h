@@ -77,7 +77,13 @@ typedef struct {
                                   queue for every level, 3 = rounds while the frontier is dense, then the
                                   work queue */
     int32_t push_lanes;        /* lanes per frontier item in push rounds: 8, 16, 32 or 64 (0 = tuned) */
-    int32_t reserved[5];
+    int32_t half_bound;        /* with a bound U: 0 = grow only states of cost <= U/2 (exact: every optimal
+                                  tree splits at a point of its weighted centre into parts of weight
+                                  <= OPT/2, whose states are then all exact, and merges plus one last
+                                  grow across the centre edge rebuild it; the tree-decomposition rule
+                                  of PrunedDP, rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:795,
+                                  src/DPQ.cu:255-258); -1 = grow every state below U */
+    int32_t reserved[4];
 } gst_opts;
 
 typedef struct {
@@ -230,6 +236,20 @@ gst_status gst_model_adam_step(gst_model* m, float lr);
 /* exact solve with the learned bound; topk_frac = fraction of vertices kept (e.g. 0.05) */
 gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, gst_model* m,
                              double topk_frac, const gst_opts* opts, gst_result* result, gst_learned_stats* stats);
+
+/* Many exact queries with the learned bound, pipelined through the batched engine (see
+ * gst_solve_batch): for each query the network scores, the top-k selection and the induced subgraph
+ * are computed on the device; the subgraph DPBF runs as one engine job and its optimum bounds the
+ * full solve, which runs as a second engine job (plain DPBF when there is no subgraph tree).  So the
+ * subgraph solves and bounded full solves of many queries share launches.  Each result is the
+ * exact optimum, as from gst_solve.  opts (NULL or nq entries): prune_lb, want_tree, push_lanes and
+ * half_bound are used (ub is set from the subgraph).  lstats (NULL or nq entries) receives each
+ * query's subgraph size, bound and times (t_sub_solve_ms / t_full_solve_ms: job latency in the
+ * engine).  No counterpart in the reference. */
+gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* groups,
+                                   gst_model* m, double topk_frac, const gst_opts* opts,
+                                   const gst_batch_opts* bopts, gst_result* results, gst_status* status,
+                                   gst_learned_stats* lstats, gst_batch_stats* stats);
 
 /* --- synthetic instances (host, deterministic) ------------------------------------------ */
 /* Host CSR owned by the library.  The generators use the library's own seeded RNG

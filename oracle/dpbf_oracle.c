@@ -20,6 +20,12 @@
  *   orc_bruteforce   exhaustive search over connected vertex subsets touching every group,
  *                    cost = minimum spanning tree of the induced subgraph (n <= 20).
  *   orc_dpbf_batch   orc_dpbf over many group sets with a pool of threads (CPU baseline).
+ *   orc_dpbf_pruned  orc_dpbf under a known feasible cost U, with the bounds of PrunedDP++
+ *                    (rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:497-964):
+ *                      a state is kept only while cost + LB(v,p) <= U, LB the one-label bound
+ *                      max over groups i not in p of dist(v, G_i)                      (:471-488)
+ *                      a popped state is grown only while cost <= U/2 (optional)       (:795)
+ *                    it must return the same optimum as orc_dpbf whenever U >= OPT.
  *
  * Pinning (tests/test_oracle.py): brute force on tiny graphs, the g=1 (cost 0) and g=2
  * (group-to-group shortest path) special cases, the g=3 spider and g=4 two-centre closed forms
@@ -99,6 +105,14 @@ static int64_t heap_pop(heap_t* H) {
     }
     H->pos[top] = -2;
     return top;
+}
+
+/* max over groups i not in p of dist[i][v] (dist: g x n) */
+static double orc_label_lb(const double* dist, int32_t n, int32_t g, int32_t v, uint32_t p) {
+    double lb = 0.0;
+    for (int32_t i = 0; i < g; ++i)
+        if (!((p >> i) & 1u) && dist[(int64_t)i * n + v] > lb) lb = dist[(int64_t)i * n + v];
+    return lb;
 }
 
 /* ---------------------------------------------------------------- DPBF */
@@ -215,6 +229,84 @@ int orc_dpbf(int32_t n, const int64_t* row, const int32_t* col, const double* w,
     if (n_edges) *n_edges = ne;
     free(cost); free(ty); free(pa); free(pos); free(hp);
     return rc;
+}
+
+/* ---------------------------------------------------------------- pruned DPBF */
+/*
+ * As orc_dpbf (cost only), under a bound U >= OPT: states with cost + lb > U are never stored,
+ * and with grow_frac > 0 only states with cost <= grow_frac * U are grown (PrunedDP++: 1/2).  dist (g x n, may be NULL when
+ * use_lb == 0) holds the exact group distances for the one-label lower bound.
+ * stats[0] = states popped, stats[1] = grow relaxations, stats[2] = merge evaluations.
+ */
+int orc_dpbf_pruned(int32_t n, const int64_t* row, const int32_t* col, const double* w, int32_t g,
+                    const uint32_t* gm, double U, double grow_frac, int32_t use_lb, const double* dist,
+                    double* cost_out, int64_t* stats) {
+    if (n < 1 || g < 1 || g > 20 || (use_lb && !dist)) return ORC_INVALID;
+    const int64_t S = (int64_t)1 << g;
+    const uint32_t full = (uint32_t)(S - 1);
+    const int64_t NS = (int64_t)n * S;
+    double* cost = (double*)malloc(sizeof(double) * (size_t)NS);
+    int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)NS);
+    int64_t* hp = (int64_t*)malloc(sizeof(int64_t) * (size_t)NS);
+    if (!cost || !pos || !hp) {
+        free(cost); free(pos); free(hp);
+        return ORC_OOM;
+    }
+    for (int64_t s = 0; s < NS; ++s) { cost[s] = INFINITY; pos[s] = -1; }
+    heap_t H = {hp, pos, cost, 0};
+    int64_t pops = 0, relax = 0, merges = 0;
+    /* one-label bound of (v, p) */
+#define ORC_LB(v, p) (use_lb ? orc_label_lb(dist, n, g, (v), (p)) : 0.0)
+    for (int32_t v = 0; v < n; ++v) {
+        const uint32_t gv = gm[v] & full;
+        if (!gv) continue;
+        for (uint32_t p = gv; p; p = (p - 1) & gv) {
+            const int64_t s = (int64_t)v * S + p;
+            if (ORC_LB(v, p) > U) continue;
+            cost[s] = 0.0;
+            heap_push_or_decrease(&H, s);
+        }
+    }
+    double best = INFINITY;
+    while (H.n > 0) {
+        const int64_t s = heap_pop(&H);
+        pops++;
+        const int32_t v = (int32_t)(s / S);
+        const uint32_t p = (uint32_t)(s % S);
+        const double cvp = cost[s];
+        if (p == full) { best = cvp; break; }
+        if (!(grow_frac > 0.0) || cvp <= grow_frac * U) {
+            for (int64_t e = row[v]; e < row[v + 1]; ++e) {
+                const int32_t u = col[e];
+                const int64_t su = (int64_t)u * S + p;
+                if (pos[su] == -2) continue;
+                relax++;
+                const double c = cvp + w[e];
+                if (c < cost[su] && c + ORC_LB(u, p) <= U) {
+                    cost[su] = c;
+                    heap_push_or_decrease(&H, su);
+                }
+            }
+        }
+        const uint32_t comp = full ^ p;
+        for (uint32_t p2 = comp; p2; p2 = (p2 - 1) & comp) {
+            const int64_t s2 = (int64_t)v * S + p2;
+            if (!(cost[s2] < INFINITY) || pos[s2] != -2) continue;   /* only settled partners */
+            merges++;
+            const int64_t sx = (int64_t)v * S + (p | p2);
+            if (pos[sx] == -2) continue;
+            const double c = cvp + cost[s2];
+            if (c < cost[sx] && c + ORC_LB(v, p | p2) <= U) {
+                cost[sx] = c;
+                heap_push_or_decrease(&H, sx);
+            }
+        }
+    }
+#undef ORC_LB
+    if (stats) { stats[0] = pops; stats[1] = relax; stats[2] = merges; }
+    *cost_out = best;
+    free(cost); free(pos); free(hp);
+    return best < INFINITY ? ORC_OK : ORC_INFEASIBLE;
 }
 
 /* ---------------------------------------------------------------- Dijkstra */

@@ -1,0 +1,102 @@
+"""GPU: bounded solves (a feasible cost U >= OPT) keep the oracle's optimum, with and without the
+half-bound grow rule (only states of cost <= U/2 are grown), in every grow mode and in the batch
+engine, on int, zero-weight and float32 graphs."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpu4gst_code_amd as gst
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+def _exact(G, groups, r, want):
+    assert r.cost == want
+    assert gst.validate_tree(G, groups, r.eu, r.ev, r.root) == want
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_c1_bounded_half_rule(solver, seed):
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=seed)
+    dg = solver.upload(G)
+    for g in (2, 3, 5, 7):
+        groups = gst.gen_groups(G.n, g, 5, 20, seed=seed + 10 * g)
+        want = oracle.dpbf_cost(G, groups)
+        for U in (want, want + 0.5, want * 1.3, want * 2.5):
+            for half in (True, False):
+                r = solver.solve(dg, groups, ub=U, half_bound=half)
+                print(f"seed {seed} g {g} U {U} half {half}: {r.cost} want {want}", flush=True)
+                _exact(G, groups, r, want)
+    dg.free()
+
+
+@pytest.mark.parametrize("grow_mode", [0, 1, 2, 3])
+def test_grow_modes_and_zero_weights(solver, grow_mode):
+    for s in range(3):
+        G = gst.gen_random_connected(500, 2000, 0, 3, seed=170 + s)
+        dg = solver.upload(G)
+        groups = gst.gen_groups(G.n, 6, 2, 10, seed=180 + s)
+        want = oracle.dpbf_cost(G, groups)
+        for U in (want, want + 1, 2 * want + 1):
+            r = solver.solve(dg, groups, ub=U, grow_mode=grow_mode)
+            _exact(G, groups, r, want)
+        dg.free()
+    G = gst.gen_barabasi_albert(4000, 4, 1, 60, seed=51)
+    dg = solver.upload(G)
+    for q, g in enumerate((3, 6, 8)):
+        groups = gst.gen_groups(G.n, g, 2, 30, seed=60 + q)
+        want = oracle.dpbf_cost(G, groups)
+        r = solver.solve(dg, groups, ub=want, grow_mode=grow_mode)
+        print(f"mode {grow_mode} g {g}: {r.cost} want {want}", flush=True)
+        _exact(G, groups, r, want)
+    dg.free()
+
+
+def test_float_bounded_half_rule(solver):
+    G = gst.gen_rmat(12, 8, weights="lognormal", seed=21)
+    dg = solver.upload(G)
+    for q, want in GOLD["rmat_f"]["costs"].items():
+        groups = gst.gen_groups(G.n, 5, 5, 50, seed=3000 + int(q))
+        for U in (want, want * 1.5):
+            r = solver.solve(dg, groups, ub=U)
+            assert abs(r.cost - want) <= 1e-6 * want
+            tc = gst.validate_tree(G, groups, r.eu, r.ev, r.root)
+            assert abs(tc - want) <= 1e-6 * want
+    dg.free()
+
+
+def test_half_rule_prunes_ba20k(solver):
+    """On the BA V=20k goldens the half rule relaxes fewer states than the plain bound."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    for q in range(3):
+        groups = gst.gen_groups(G.n, 6, 10, 200, seed=1000 + q)
+        want = GOLD["ba20k"]["costs"][str(q)]
+        full = solver.solve(dg, groups, ub=want * 1.05, half_bound=False)
+        half = solver.solve(dg, groups, ub=want * 1.05)
+        print(f"q{q}: relax half {half.stats['grow_relax']} vs full bound {full.stats['grow_relax']}", flush=True)
+        _exact(G, groups, half, want)
+        _exact(G, groups, full, want)
+        assert half.stats["grow_relax"] < full.stats["grow_relax"]
+    dg.free()
+
+
+def test_batch_bounded_half_rule(solver):
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=3)
+    dg = solver.upload(G)
+    rng = np.random.default_rng(13)
+    qs, want = [], []
+    for i in range(24):
+        g = int(rng.integers(2, 9))
+        qs.append(gst.gen_groups(G.n, g, 1, int(rng.integers(2, 40)), seed=2000 + i))
+        want.append(oracle.dpbf_cost(G, qs[-1]))
+    ubs = [w * f for w, f in zip(want, np.resize([1.0, 1.2, 2.0], len(want)))]
+    res, st = solver.solve_batch(dg, qs, ubs=ubs, slots=8)
+    for q, r, w in zip(qs, res, want):
+        _exact(G, q, r, w)
+        assert r.used_ub == (w > 0)   # a bound of 0 means "no bound"
+    dg.free()

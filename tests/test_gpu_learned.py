@@ -123,3 +123,109 @@ def test_learned_solve_float_weights(solver):
         assert abs(r.cost - want) <= 1e-6 * want
     model.free()
     dg.free()
+
+
+@pytest.mark.parametrize("frac,slots", [(0.05, 4), (0.0005, 3), (0.2, 8)])
+def test_batch_learned_is_exact(solver, frac, slots):
+    """The pipelined learned solve (subgraph jobs then bounded full jobs in the batched engine)
+    returns every golden optimum and a valid tree, whatever the network (untrained here)."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=1)
+    keys = sorted(GOLD["ba20k"]["costs"], key=int)
+    qs = [gst.gen_groups(G.n, 6, 10, 200, seed=1000 + int(k)) for k in keys]
+    res, st = solver.solve_batch_learned(dg, qs, model, frac, slots=slots)
+    assert st["queries"] >= len(qs)
+    for k, q, r in zip(keys, qs, res):
+        want = GOLD["ba20k"]["costs"][k]
+        print(f"frac {frac} q{k}: cost {r.cost} want {want} ub {r.stats['ub']} sub {r.stats['sub_vertices']} "
+              f"fell_back {r.fell_back}", flush=True)
+        assert r.cost == want
+        assert gst.validate_tree(G, q, r.eu, r.ev, r.root) == want
+        if r.stats["ub"] > 0:
+            assert r.stats["ub"] >= want
+            assert not r.fell_back
+    model.free()
+    dg.free()
+
+
+def test_batch_learned_float_and_mixed_groups(solver):
+    G = gst.gen_rmat(12, 8, weights="lognormal", seed=21)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=2)
+    items = sorted(GOLD["rmat_f"]["costs"].items(), key=lambda kv: int(kv[0]))
+    qs = [gst.gen_groups(G.n, 5, 5, 50, seed=3000 + int(q)) for q, _ in items]
+    res, _ = solver.solve_batch_learned(dg, qs, model, 0.1, slots=2)
+    for (q, want), r in zip(items, res):
+        assert abs(r.cost - want) <= 1e-6 * want
+    dg.free()
+    # mixed group counts on a small int graph, against the oracle
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=3)
+    dg = solver.upload(G)
+    rng = np.random.default_rng(17)
+    qs = [gst.gen_groups(G.n, int(rng.integers(1, 9)), 1, int(rng.integers(2, 40)), seed=4000 + i) for i in range(20)]
+    res, _ = solver.solve_batch_learned(dg, qs, model, 0.1, slots=5)
+    for q, r in zip(qs, res):
+        want = oracle.dpbf_cost(G, q)
+        assert r.cost == want
+        assert gst.validate_tree(G, q, r.eu, r.ev, r.root) == want
+    model.free()
+    dg.free()
+
+
+def test_trained_network_gain(solver):
+    """What the trained network gains over the same method without training: on BA V=50k queries
+    (g=6, groups of 10-200) the trained scores give a feasible subgraph tree far more often and a
+    tighter bound, so the bounded full solve relaxes fewer states; both stay exact."""
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "gpu4gst_code_amd", "models", "mpnn_ba50k.npy")
+    G = gst.gen_barabasi_albert(50000, 5, 1, 1000, seed=123457)
+    dg = solver.upload(G)
+    trained = gst.Model(solver)
+    trained.load(path)
+    untrained = gst.Model(solver, seed=7)
+    qs = [gst.gen_groups(G.n, 6, 10, 200, seed=777 + q) for q in range(12)]
+    plain = [solver.solve(dg, q) for q in qs]
+    out = {}
+    for name, m in (("trained", trained), ("untrained", untrained)):
+        res, _ = solver.solve_batch_learned(dg, qs, m, 0.05, slots=6)
+        for r, p in zip(res, plain):
+            assert r.cost == p.cost
+        fb = sum(r.fell_back for r in res) / len(res)
+        ratios = [r.stats["ub"] / p.cost for r, p in zip(res, plain) if r.stats["ub"] > 0]
+        relax = sum(r.stats["grow_relax"] for r in res) / sum(p.stats["grow_relax"] for p in plain)
+        out[name] = (fb, float(np.mean(ratios)) if ratios else float("inf"), relax)
+        print(f"{name}: fallback rate {fb:.2f}, mean ub/opt {out[name][1]:.4f}, relaxations vs plain {relax:.4f}",
+              flush=True)
+    assert out["trained"][0] <= 0.25
+    assert out["trained"][0] < out["untrained"][0] or out["trained"][1] < out["untrained"][1]
+    assert out["trained"][2] < out["untrained"][2]
+    assert out["trained"][2] < 0.5
+    trained.free()
+    untrained.free()
+    dg.free()
+
+
+@pytest.mark.parametrize("frac", [0.01, 0.05, 0.3, 1.0])
+def test_topk_selection_size(solver, frac):
+    """The device radix select keeps exactly ceil(frac n) top-scored vertices: the subgraph holds
+    them plus the group members outside them (checked where no score ties the threshold)."""
+    G = gst.gen_barabasi_albert(30000, 4, 1, 500, seed=19)
+    dg = solver.upload(G)
+    model = gst.Model(solver, params=_params(4))
+    for q in range(3):
+        groups = gst.gen_groups(G.n, 5, 5, 60, seed=600 + q)
+        sc = model.scores(dg, groups)
+        k = min(G.n, int(np.ceil(frac * G.n)))
+        order = np.argsort(-sc, kind="stable")
+        thr = sc[order[k - 1]]
+        members = np.unique(np.concatenate([groups.members(i) for i in range(groups.g)]))
+        r = solver.solve_learned(dg, groups, model, frac)
+        if k < G.n and np.sum(sc == thr) > 1:
+            continue   # ties at the threshold: any of them may be kept
+        top = np.zeros(G.n, bool)
+        top[order[:k]] = True
+        want = k + int(np.sum(~top[members]))
+        print(f"frac {frac} q{q}: sub_vertices {r.stats['sub_vertices']} want {want}", flush=True)
+        assert r.stats["sub_vertices"] == want
+    model.free()
+    dg.free()

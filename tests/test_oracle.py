@@ -106,3 +106,54 @@ def test_planted_mistakes_are_caught():
     c, root, eu, ev, ew, _ = oracle.dpbf(G, groups)
     assert 2 * float(np.sum(ew)) != good
     _ = bad_groups
+
+
+def _zero_weight_graph(n, extra, whi, seed):
+    return gst.gen_random_connected(n, n + extra, 0, whi, seed=seed)
+
+
+@pytest.mark.parametrize("s", range(0, 60, 3))
+def test_pruned_half_bound_vs_bruteforce(s):
+    """Bounds of PrunedDP++ (keep cost + LB <= U, grow only cost <= U/2) keep the optimum for any
+    U >= OPT: tiny graphs against brute force, with zero-weight edges on every third seed."""
+    n = 10 + s % 5
+    g = 1 + s % 6
+    G = _zero_weight_graph(n, 8, 4, 9000 + s) if s % 3 == 0 else gst.gen_random_connected(n, n + 6, 1, 20, seed=5000 + s)
+    groups = gst.gen_groups(n, g, 1, 3, seed=6000 + s)
+    b = oracle.bruteforce(G, groups)
+    for U in (b, b + 1, 2 * b + 3):
+        for lb in (True, False):
+            c, _ = oracle.dpbf_pruned(G, groups, U, 0.5, lb)
+            assert c == b, (U, lb, c, b)
+
+
+@pytest.mark.parametrize("s", range(6))
+def test_pruned_half_bound_c1_and_float(s):
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=s)
+    groups = gst.gen_groups(G.n, 3 + s % 4, 5, 20, seed=s + 1)
+    c0, st0 = oracle.dpbf(G, groups)[0], oracle.dpbf(G, groups)[5]
+    for U in (c0, c0 * 1.1, c0 * 3):
+        c, st = oracle.dpbf_pruned(G, groups, U)
+        assert c == c0
+        # label-setting stops at OPT, so a loose bound (U >= 2 OPT) may prune nothing
+        assert st["relax"] <= st0["relax"]
+        if U == c0:
+            assert st["relax"] < oracle.dpbf_pruned(G, groups, U, 0.0)[1]["relax"]
+    # a bound below the optimum leaves no tree
+    assert oracle.dpbf_pruned(G, groups, c0 - 1)[0] == np.inf
+    Gf = gst.gen_rmat(10, 8, weights="lognormal", seed=40 + s)
+    gf = gst.gen_groups(Gf.n, 4, 3, 20, seed=50 + s)
+    cf = oracle.dpbf_cost(Gf, gf)
+    assert abs(oracle.dpbf_pruned(Gf, gf, cf * (1 + 1e-9))[0] - cf) <= 1e-12 * cf
+
+
+def test_pruned_grow_limit_below_half_is_caught():
+    """Planted mistake: growing only states of cost <= U/3 loses the optimum on some instance,
+    so the tests above would catch a wrong grow limit."""
+    lost = 0
+    for s in range(40):
+        G = gst.gen_random_connected(12, 18, 1, 20, seed=7000 + s)
+        groups = gst.gen_groups(12, 2 + s % 4, 1, 2, seed=7100 + s)
+        b = oracle.bruteforce(G, groups)
+        lost += oracle.dpbf_pruned(G, groups, b, 1.0 / 3.0)[0] != b
+    assert lost >= 1
