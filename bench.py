@@ -1,14 +1,20 @@
-"""Exact group Steiner tree queries/s on one MI355X.
+"""Exact group Steiner tree queries/s on one MI355X: learned upper bound vs plain DPBF.
 
-Workload (one step = one exact GST query, solved through the C-ABI with inputs resident in HBM):
+Workload (one step = one exact GST query, solved through the C-ABI with the graph resident in HBM):
   Barabasi-Albert V=200,000, m=5 (~1.0M undirected edges), int32 weights uniform in [1,1000],
   g=6 groups of sizes uniform in [10,200]; query q uses group seed 1000+q (64 distinct queries,
-  cycled).  Plain DPBF (no bound) is the headline `value`; the same queries are also reported with
-  the learned upper bound when a trained model file is present.
+  cycled to --steps).
 
-Printed: one JSON line with the metric, the grow-kernel roofline (algorithmic bytes / kernel time
-vs 8 TB/s), the merge-kernel roofline, state relaxations/s, and a CPU baseline (the oracle's
-label-setting DPBF, run on the host cores over a bounded sample of the same queries).
+Headline `value`: queries/s of the batched learned-bound solve (gst_solve_batch_learned): the
+trained network (gpu4gst_code_amd/models/mpnn_ba50k.npy) scores vertices, the DPBF optimum of
+the top-3% induced subgraph bounds the full DPBF, subgraph and bounded full solves of up to 64
+queries share the batched engine's launches.  Every cost is checked against the plain DPBF of the
+same query, run one query at a time right after (reported as `plain_dpbf`).
+
+Printed: one JSON line with the metric, the speedup over plain DPBF, the roofline of the dominant
+kernel of the timed run (batched grow: algorithmic bytes / HIP-event kernel time vs 8 TB/s) and of
+the batched merge, the plain path's grow/merge rooflines, state relaxations/s, and a CPU baseline
+(the oracle's label-setting DPBF on the host cores over a bounded sample of the same queries).
 """
 from __future__ import annotations
 
@@ -29,15 +35,23 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=256, help="queries in the timed run (the 64 queries cycled)")
+    ap.add_argument("--warmup", type=int, default=16, help="queries solved before the timed run")
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--groups", type=int, default=6)
+    ap.add_argument("--frac", type=float, default=0.03, help="top-scored fraction of vertices kept in the subgraph")
+    ap.add_argument("--slots", type=int, default=64, help="queries in flight in the batched engine")
+    ap.add_argument("--model", default=os.path.join(ROOT, "gpu4gst_code_amd", "models", "mpnn_ba50k.npy"))
+    ap.add_argument("--autotune", type=int, default=1, help="1: pick launch knobs with gst_autotune before timing")
     ap.add_argument("--cpu-sample", type=int, default=16, help="queries in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied by one plain device copy before the run (counter calibration)")
     return ap.parse_args()
+
+
+def gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
 
 
 def main():
@@ -50,8 +64,13 @@ def main():
 
     G = gst.gen_barabasi_albert(args.n, 5, 1, 1000, seed=1)
     queries = [gst.gen_groups(G.n, args.groups, 10, 200, seed=1000 + q) for q in range(64)]
+    timed_q = [queries[i % 64] for i in range(args.steps)]
     solver = gst.Solver(0)
     dg = solver.upload(G)
+    if not os.path.exists(args.model):
+        raise SystemExit(f"bench.py: trained model {args.model} is missing")
+    model = gst.Model(solver)
+    model.load(args.model)
 
     if args.calibrate_copy > 0:
         import ctypes
@@ -59,52 +78,76 @@ def main():
         calib.gst_calib_copy.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
         assert calib.gst_calib_copy(0, args.calibrate_copy << 20, 1) == 0
 
-    for i in range(args.warmup):
-        solver.solve(dg, queries[i % 64])
+    # launch knobs for this graph (bounded one-query solves and the batched engine), picked by the
+    # project's autotuner on 16 of the queries; plain solves keep the built-in defaults
+    tune = None
+    if args.autotune:
+        t_tune = time.perf_counter()
+        tune = solver.autotune(dg, queries[:16], bounded=True, batch=True)
+        tune["seconds"] = round(time.perf_counter() - t_tune, 2)
 
-    # every solve ends with a stream synchronisation, so the host clock brackets device work
+    # warm-up: engine slots, level tables, captured steps, the model's buffers, the plain path
+    if args.warmup > 0:
+        solver.solve_batch_learned(dg, [queries[i % 64] for i in range(args.warmup)], model, args.frac,
+                                   slots=args.slots, time_kernels=True)
+        for i in range(min(args.warmup, 4)):
+            solver.solve(dg, queries[i % 64])
+
+    # ---- timed: batched learned-bound solve (each call ends with a stream synchronisation) ----
     t0 = time.perf_counter()
-    results = []
-    for i in range(args.steps):
-        results.append(solver.solve(dg, queries[i % 64]))
+    res, bst = solver.solve_batch_learned(dg, timed_q, model, args.frac, slots=args.slots, time_kernels=True)
     wall = time.perf_counter() - t0
-
     qps = args.steps / wall
-    grow_ms = sum(r.stats["t_grow_ms"] for r in results)
-    merge_ms = sum(r.stats["t_merge_ms"] for r in results)
-    grow_bytes = sum(r.stats["grow_bytes"] for r in results)
-    merge_bytes = sum(r.stats["merge_bytes"] for r in results)
-    relax = sum(r.stats["grow_relax"] for r in results)
-    grow_launches = sum(r.stats["grow_launches"] for r in results)
-    grow_gbs = grow_bytes / (grow_ms * 1e-3) / 1e9 if grow_ms > 0 else 0.0
-    merge_gbs = merge_bytes / (merge_ms * 1e-3) / 1e9 if merge_ms > 0 else 0.0
 
-    # every tree is checked (cheap, host side)
-    for i, r in enumerate(results[: min(len(results), 64)]):
-        tc = gst.validate_tree(G, queries[i % 64], r.eu, r.ev, r.root)
-        assert tc == r.cost, (i, tc, r.cost)
+    # ---- plain DPBF, one query at a time, on the 64 distinct queries ----
+    t1 = time.perf_counter()
+    plain = [solver.solve(dg, q) for q in queries]
+    plain_wall = time.perf_counter() - t1
+    plain_qps = len(queries) / plain_wall
+
+    # exactness: every learned-bound cost equals plain DPBF; every tree is valid at that cost
+    for i, r in enumerate(res):
+        want = plain[i % 64].cost
+        assert r.cost == want, (i, r.cost, want)
+        if i < 64:
+            tc = gst.validate_tree(G, timed_q[i], r.eu, r.ev, r.root)
+            assert tc == want, (i, tc, want)
+    fallback = sum(r.fell_back for r in res) / len(res)
+    ratios = [r.stats["ub"] / plain[i % 64].cost for i, r in enumerate(res) if r.stats["ub"] > 0 and plain[i % 64].cost > 0]
+
+    # rooflines: batched engine (timed run) and plain path
+    b_grow_gbs = gbs(bst["grow_bytes"], bst["t_grow_ms"])
+    b_merge_gbs = gbs(bst["merge_bytes"], bst["t_merge_ms"])
+    p_grow_ms = sum(r.stats["t_grow_ms"] for r in plain)
+    p_merge_ms = sum(r.stats["t_merge_ms"] for r in plain)
+    p_grow_bytes = sum(r.stats["grow_bytes"] for r in plain)
+    p_merge_bytes = sum(r.stats["merge_bytes"] for r in plain)
+    p_relax = sum(r.stats["grow_relax"] for r in plain)
+    p_launches = sum(r.stats["grow_launches"] for r in plain)
 
     cpu = None
     if args.cpu_sample > 0:
         import oracle
         sample = queries[: args.cpu_sample]
         nth = min(args.cpu_threads, len(sample), os.cpu_count() or 1)
-        t1 = time.perf_counter()
+        t2 = time.perf_counter()
         costs = oracle.dpbf_batch(G, sample, nthreads=nth)
-        cw = time.perf_counter() - t1
+        cw = time.perf_counter() - t2
         for i, c in enumerate(costs):
-            assert c == results[i % len(results)].cost or i >= len(results), (i, c)
+            assert c == plain[i].cost, (i, c, plain[i].cost)
         cpu = {"value": round(len(sample) / cw, 4), "unit": "queries/s", "cores": nth, "kind": "port",
                "sample": f"{len(sample)} of the 64 queries, label-setting DPBF (oracle/dpbf_oracle.c), "
                          f"{nth} threads, {cw:.1f} s wall"}
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "dpbf_plain_ba200k_g6_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "dpbf_learned_batch_ba200k_g6_traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath))["traffic_bytes_per_dispatch"]
+        traffic = json.load(open(tpath)).get("k_batch_grow", {}).get("traffic_bytes_per_dispatch")
 
+    glaunch = max(int(bst["grow_launches"]), 1)
+    mlaunch = max(int(bst["merge_launches"]), 1)
     line = {
-        "metric": "exact-cost GST queries/s (plain DPBF, BA V=200k g=6)",
+        "metric": "exact-cost GST queries/s (learned-UB batched DPBF vs plain DPBF, BA V=200k g=6)",
         "value": round(qps, 3),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -117,18 +160,35 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded BA graph and groups)",
         "config": {"workload": f"BA V={G.n} m=5 E={G.m // 2} int32 w[1,1000], g={args.groups} sizes 10-200, "
-                               "64 queries one at a time", "mode": "plain DPBF"},
-        "state_relax_per_s": round(relax / (grow_ms * 1e-3), 1) if grow_ms > 0 else None,
-        "roofline": {"bound": "hbm", "achieved": round(grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(grow_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_grow_round", "launches": grow_launches,
-                     "bytes_per_launch": round(grow_bytes / max(grow_launches, 1), 1),
-                     "ms_per_launch": round(grow_ms / max(grow_launches, 1), 5)},
-        "merge_roofline": {"bound": "hbm", "achieved": round(merge_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(merge_gbs / HBM_PEAK_GBS, 5), "kernel": "k_merge"},
+                               f"{args.steps} queries (64 distinct), learned bound top {args.frac:.0%} subgraph, "
+                               f"batched engine {args.slots} slots",
+                   "mode": "learned-UB batched DPBF"},
+        "plain_dpbf": {"value": round(plain_qps, 3), "unit": "queries/s", "mode": "plain DPBF, one query at a time"},
+        "speedup_vs_plain": round(qps / plain_qps, 3),
+        "learned": {"frac": args.frac, "fallback_rate": round(fallback, 4),
+                    "ub_over_opt_mean": round(float(np.mean(ratios)), 5) if ratios else None,
+                    "relax_vs_plain": round(sum(r.stats["grow_relax"] for r in res[:64]) / max(p_relax, 1), 5)},
+        "latency_ms": {"p50": round(bst["latency_p50_ms"], 3), "p99": round(bst["latency_p99_ms"], 3)},
+        "autotune": tune,
+        "state_relax_per_s": round(p_relax / (p_grow_ms * 1e-3), 1) if p_grow_ms > 0 else None,
+        "roofline": {"bound": "hbm", "achieved": round(b_grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(b_grow_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "k_batch_grow", "launches": glaunch,
+                     "bytes_per_launch": round(bst["grow_bytes"] / glaunch, 1),
+                     "ms_per_launch": round(bst["t_grow_ms"] / glaunch, 5)},
+        "merge_roofline": {"bound": "hbm", "achieved": round(b_merge_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(b_merge_gbs / HBM_PEAK_GBS, 5), "kernel": "k_batch_merge",
+                           "launches": mlaunch, "ms_per_launch": round(bst["t_merge_ms"] / mlaunch, 5)},
+        "plain_roofline": {"grow": {"kernel": "k_grow_round", "achieved": round(gbs(p_grow_bytes, p_grow_ms), 2),
+                                    "frac": round(gbs(p_grow_bytes, p_grow_ms) / HBM_PEAK_GBS, 5),
+                                    "launches": p_launches,
+                                    "ms_per_launch": round(p_grow_ms / max(p_launches, 1), 5)},
+                           "merge": {"kernel": "k_merge", "achieved": round(gbs(p_merge_bytes, p_merge_ms), 2),
+                                     "frac": round(gbs(p_merge_bytes, p_merge_ms) / HBM_PEAK_GBS, 5)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    model.free()
     solver.close()
 
 

@@ -260,7 +260,8 @@ class Solver:
 
     def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
               want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0,
-              pull_permille: int = 0, grow_mode: int = 0, push_lanes: int = 0, half_bound: bool = True) -> Result:
+              pull_permille: int = 0, grow_mode: int = 0, push_lanes: int = 0, half_bound: bool = True,
+              auto_bound: bool = False, merge_tile_kb: int = 0) -> Result:
         opts = _abi.gst_opts()
         opts.ub = float(ub) if ub is not None else 0.0
         opts.prune_lb = 1 if prune_lb else 0
@@ -271,6 +272,8 @@ class Solver:
         opts.grow_mode = grow_mode
         opts.push_lanes = push_lanes
         opts.half_bound = 0 if half_bound else -1
+        opts.auto_bound = 1 if auto_bound else 0
+        opts.merge_tile_kb = merge_tile_kb
         res = _abi.gst_result()
         grp = groups.as_c()
         st = self.lib.gst_solve(self.ctx, graph.handle, C.byref(grp), C.byref(opts), C.byref(res))
@@ -294,7 +297,7 @@ class Solver:
     def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                     use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
-                    raise_on_error: bool = True):
+                    auto_bound: bool = False, time_kernels: bool = False, raise_on_error: bool = True):
         """Solve many queries on one graph with the batched engine (gst_solve_batch).
 
         Returns (results, stats): results[i] is a Result, or a GstError for a query that failed
@@ -312,10 +315,12 @@ class Solver:
             opts[i].want_tree = 1 if want_tree else 0
             opts[i].push_lanes = push_lanes
             opts[i].half_bound = 0 if half_bound else -1
+            opts[i].auto_bound = 1 if auto_bound else 0
         bo = _abi.gst_batch_opts()
         bo.slots = slots
         bo.rounds_per_step = rounds_per_step
         bo.use_graph = 1 if use_graph else 0
+        bo.time_kernels = 1 if time_kernels else 0
         res = (_abi.gst_result * max(nq, 1))()
         status = (C.c_int * max(nq, 1))()
         bs = _abi.gst_batch_stats()
@@ -429,7 +434,7 @@ def _solve_learned(self, graph: DeviceGraph, groups: Groups, model: Model, topk_
 def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model: Model, topk_frac: float = 0.05, *,
                          prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                          use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
-                         raise_on_error: bool = True):
+                         time_kernels: bool = False, raise_on_error: bool = True):
     """Exact solves of many queries with the learned bound, pipelined through the batched engine
     (gst_solve_batch_learned).  Returns (results, batch stats); each Result's stats also carry the
     query's learned-bound fields (sub_vertices, ub, t_sub_solve_ms, ...)."""
@@ -448,6 +453,7 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
     bo.slots = slots
     bo.rounds_per_step = rounds_per_step
     bo.use_graph = 1 if use_graph else 0
+    bo.time_kernels = 1 if time_kernels else 0
     res = (_abi.gst_result * max(nq, 1))()
     status = (C.c_int * max(nq, 1))()
     ls = (_abi.gst_learned_stats * max(nq, 1))()
@@ -481,6 +487,21 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
     return out, stats
 
 
+def _autotune(self, graph: DeviceGraph, sample: List[Groups], *, bounded: bool = False, batch: bool = False) -> dict:
+    """Pick the launch knobs for this graph from a sample of queries (gst_autotune); later solves
+    that leave a knob at 0 use the choice.  Returns the tuning report as a dict."""
+    nq = len(sample)
+    grps = (_abi.gst_groups * max(nq, 1))()
+    keep = [q.as_c() for q in sample]
+    for i, c in enumerate(keep):
+        grps[i] = c
+    flags = (_abi.GST_TUNE_BOUNDED if bounded else 0) | (_abi.GST_TUNE_BATCH if batch else 0)
+    rep = _abi.gst_tune_report()
+    _check(self.lib, self.lib.gst_autotune(self.ctx, graph.handle, nq, grps, flags, C.byref(rep)), self.ctx)
+    return {name: getattr(rep, name) for name, _ in _abi.gst_tune_report._fields_ if name != "pad"}
+
+
 Solver.solve_learned = _solve_learned
+Solver.autotune = _autotune
 Solver.solve_batch_learned = _solve_batch_learned
 __all__ += ["Model"]

@@ -38,7 +38,7 @@ class gst_opts(C.Structure):
     _fields_ = [("ub", C.c_double), ("prune_lb", C.c_int32), ("want_tree", C.c_int32),
                 ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32),
                 ("grow_mode", C.c_int32), ("push_lanes", C.c_int32), ("half_bound", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("merge_tile_kb", C.c_int32), ("auto_bound", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class gst_stats(C.Structure):
@@ -64,7 +64,7 @@ class gst_learned_stats(C.Structure):
 
 class gst_batch_opts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("rounds_per_step", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("time_kernels", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class gst_batch_stats(C.Structure):
@@ -72,7 +72,19 @@ class gst_batch_stats(C.Structure):
                 ("latency_p99_ms", C.c_double), ("steps", C.c_uint64), ("grow_launches", C.c_uint64),
                 ("grow_relax", C.c_uint64), ("grow_bytes", C.c_uint64), ("merge_bytes", C.c_uint64),
                 ("queries", C.c_int64), ("slots", C.c_int32), ("rounds_per_step", C.c_int32),
-                ("used_graph", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("used_graph", C.c_int32), ("timed_kernels", C.c_int32), ("t_grow_ms", C.c_double),
+                ("t_merge_ms", C.c_double), ("merge_launches", C.c_uint64), ("reserved", C.c_int32 * 4)]
+
+
+class gst_tune_report(C.Structure):
+    _fields_ = [("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32), ("push_lanes", C.c_int32),
+                ("merge_tile_kb", C.c_int32), ("batch_rounds_per_step", C.c_int32), ("batch_push_lanes", C.c_int32),
+                ("configs_tried", C.c_int32), ("pad", C.c_int32), ("default_ms", C.c_double), ("tuned_ms", C.c_double),
+                ("batch_default_qps", C.c_double), ("batch_tuned_qps", C.c_double)]
+
+
+GST_TUNE_BOUNDED = 1
+GST_TUNE_BATCH = 2
 
 
 class gst_host_csr(C.Structure):
@@ -113,6 +125,8 @@ SIGNATURES = {
                                           C.c_double, C.POINTER(gst_opts), C.POINTER(gst_batch_opts),
                                           C.POINTER(gst_result), C.POINTER(C.c_int), C.POINTER(gst_learned_stats),
                                           C.POINTER(gst_batch_stats)]),
+    "gst_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(gst_groups), C.c_int32,
+                               C.POINTER(gst_tune_report)]),
     "gst_host_csr_free": (None, [C.POINTER(gst_host_csr)]),
     "gst_gen_random_connected": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64,
                                            C.POINTER(gst_host_csr)]),

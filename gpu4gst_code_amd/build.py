@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("GST_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["gst_api.cpp", "gst_gen.cpp"]
+HOST_SOURCES = ["gst_api.cpp", "gst_gen.cpp", "gst_tune.cpp"]
 HIP_SOURCES = ["gst_dpbf.hip", "gst_batch.hip", "gst_mpnn.hip"]
 
 # Packed-FP32 instructions are switched off for all device code (they have returned wrong results

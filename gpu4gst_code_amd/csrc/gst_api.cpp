@@ -318,7 +318,7 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
         return true;
     };
     auto job_done = [&](gst_batch_job& j) { status[j.tag] = j.status; };
-    const gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)g->n, gmax, g->m, bopts, stats);
+    const gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)g->n, gmax, g->m, bopts, stats, g->uid);
     if (first == GST_OK) first = rs;
     return first;
 }

@@ -123,6 +123,35 @@ __global__ void __launch_bounds__(MERGE_THREADS) k_batch_merge(BatchArgs b) {
     merge_body<FLT>(b.margs[p->merge_slot[i] * NCTL + p->merge_level[i]], lds, blockIdx.x - i * per, per);
 }
 
+// after the merges: a slot whose level-2 merge found a spider (automatic bound) tightens the bound
+// of its later levels in the argument blocks (read by every later launch)
+template <bool FLT>
+__global__ void __launch_bounds__(WAVE) k_batch_bound(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    const uint32_t i = threadIdx.x;
+    if (i >= p->n_merge || p->merge_level[i] != 2) return;
+    const uint32_t s = p->merge_slot[i];
+    const MergeArgs& m2 = b.margs[s * NCTL + 2];
+    if (!m2.spider) return;
+    const uint32_t sp = *m2.spider;
+    if (sp == GST_INF_COST) return;
+    uint32_t ub, gmax;
+    spider_bound<FLT>(sp, ub, gmax);
+    const uint32_t g = m2.g;
+    for (uint32_t k = 2; k <= g; ++k) {
+        if (k < g) {
+            GrowArgs* ga = const_cast<GrowArgs*>(&b.gargs[s * NCTL + k]);
+            if (ub < ga->ub) ga->ub = ub;
+            if (gmax < ga->gmax) ga->gmax = gmax;
+        }
+        if (k >= 3) {
+            MergeArgs* ma = const_cast<MergeArgs*>(&b.margs[s * NCTL + k]);
+            if (ub < ma->ub) ma->ub = ub;
+            if (gmax < ma->gmax) ma->gmax = gmax;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_batch_compact(BatchArgs b) {
     const BatchPlan* p = b.plan;
     const uint32_t nc = p->n_compact;
@@ -229,6 +258,7 @@ struct gst_batch_engine {
         unsigned long long* root = nullptr;
         unsigned long long* stats = nullptr;
         uint32_t* edges = nullptr;          // 3 * ecap + 1
+        uint32_t* spider = nullptr;         // cheapest spider of the level-2 merge (automatic bound)
         int32_t* grp = nullptr;
         size_t grp_cap = 0;
         // pinned host staging (args written at fill) and readback
@@ -281,7 +311,7 @@ static size_t back_bytes() { return 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT +
 static void free_slot(gst_batch_engine::Slot& sl) {
     hipFree(sl.T); hipFree(sl.gm);
     for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
-    hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp);
+    hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp); hipFree(sl.spider);
     if (sl.h_grp) hipHostFree(sl.h_grp);
     if (sl.h_gargs) hipHostFree(sl.h_gargs);
     if (sl.h_margs) hipHostFree(sl.h_margs);
@@ -346,7 +376,7 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
         for (int j = 0; ok && j < 3; ++j)
             ok = dalloc(sl.chg[j], (size_t)n * W) && dalloc(sl.inq[j], n) && dalloc(sl.lists[j], items_cap);
         ok = ok && dalloc(sl.ctl, NCTL) && dalloc(sl.root, 1) && dalloc(sl.stats, ST_NSTAT) &&
-             dalloc(sl.edges, 3 * (size_t)ecap + 1);
+             dalloc(sl.edges, 3 * (size_t)ecap + 1) && dalloc(sl.spider, 1);
         ok = ok && hipHostMalloc((void**)&sl.h_gargs, sizeof(GrowArgs) * NCTL) == hipSuccess &&
              hipHostMalloc((void**)&sl.h_margs, sizeof(MergeArgs) * NCTL) == hipSuccess &&
              hipHostMalloc((void**)&sl.h_targs, sizeof(TraceArgs)) == hipSuccess &&
@@ -414,7 +444,8 @@ static gst_status ensure_hub_items(gst_ctx* ctx, gst_batch_engine* E, const gst_
 
 // Enqueue the set-up of one query in slot `si`: group bits, level 1, its frontier, and the
 // argument blocks of every level.
-static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch_job& job, uint32_t chunk) {
+static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch_job& job, uint32_t chunk,
+                            uint32_t tuned_lanes) {
     hipStream_t s = (hipStream_t)ctx->stream;
     auto& sl = E->slots[(size_t)si];
     const gst_graph* G = job.G;
@@ -454,6 +485,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     k_set_group_bits<<<64, 256, 0, s>>>(sl.gm, sl.grp, sl.grp + g + 1, g);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)sl.stats, 0ull, ST_NSTAT);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)sl.root, GST_INF_WORD, 1);
+    k_fill_u32<<<1, 64, 0, s>>>(sl.spider, GST_INF_COST, 1);
     k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)sl.ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
     k_init_level1<<<fg, 256, 0, s>>>(sl.T + lvl_base[1], sl.gm, G->d_row, n, (uint32_t)g, sl.chg[0], sl.inq[0],
                                     g > 1 ? 1 : 0);
@@ -487,7 +519,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.pull_thresh = persistent ? (uint32_t)std::max<uint64_t>(1, E->n_items_total * 450ull / 1000ull) : 0xFFFFFFFFu;
         a.rlog = nullptr;
         a.seq = 0;
-        a.push_lanes = job.opts.push_lanes > 0 ? (uint32_t)job.opts.push_lanes : default_push_lanes(true);
+        a.push_lanes = job.opts.push_lanes > 0 ? (uint32_t)job.opts.push_lanes
+                                               : (tuned_lanes ? tuned_lanes : default_push_lanes(true));
     }
     for (int k = 2; k <= g; ++k) {
         MergeArgs& ma = sl.h_margs[k];
@@ -499,6 +532,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ubb; ma.use_lb = job.opts.prune_lb ? 1u : 0u;
         ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
+        ma.spider = (job.opts.auto_bound && k == 2 && g >= 3) ? sl.spider : nullptr;
         ma.is_full = (k == g) ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
         ma.VB = merge_tile_vertices(Lk, Wk);
@@ -529,27 +563,35 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     return GST_OK;
 }
 
-// enqueue one step (plan/merge/compact/trace + R grow rounds)
-static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba, int R, bool flt) {
+// enqueue one step (plan/merge/compact/trace + R grow rounds); ev (2 + 2R events, or nullptr)
+// brackets the merge launch and every grow launch
+static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba, int R, bool flt,
+                         const hipEvent_t* ev = nullptr) {
     hipStream_t s = (hipStream_t)ctx->stream;
     const int mgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 8);
     // one LDS size for every (g, level) a slot may merge: tile rows, bitmaps and split table
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + MERGE_PAIRS_LDS;
     k_plan_step<<<1, WAVE, 0, s>>>(ba);
+    if (ev) hipEventRecord(ev[0], s);
     if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
     else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
+    if (ev) hipEventRecord(ev[1], s);
+    if (flt) k_batch_bound<true><<<1, WAVE, 0, s>>>(ba);
+    else k_batch_bound<false><<<1, WAVE, 0, s>>>(ba);
     k_batch_compact<<<mgrid, 256, 0, s>>>(ba);
     k_batch_trace<<<MAX_SLOTS, WAVE, 0, s>>>(ba);
     for (int r = 0; r < R; ++r) {
         k_plan_grow<<<1, WAVE, 0, s>>>(ba);
+        if (ev) hipEventRecord(ev[2 + 2 * r], s);
         if (flt) k_batch_grow<true><<<ba.grid + MAX_SLOTS, GROW_BLOCK, 0, s>>>(ba);
         else k_batch_grow<false><<<ba.grid + MAX_SLOTS, GROW_BLOCK, 0, s>>>(ba);
+        if (ev) hipEventRecord(ev[3 + 2 * r], s);
     }
 }
 
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
-                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst) {
+                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid) {
     gst_batch_opts bo{};
     if (bo_in) bo = *bo_in;
     if (!ctx->be) ctx->be = new gst_batch_engine();
@@ -575,8 +617,22 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     }
     gst_status st = ensure_slots(ctx, E, S, n_cap, g_cap, m_cap, chunk);
     if (st != GST_OK) return st;   // an engine with more slots is reused; its extra slots stay idle
-    const int R = bo.rounds_per_step > 0 ? bo.rounds_per_step : 4;
-    const bool use_graph = bo.use_graph != 0;
+    const gst_tuned_batch* tb = tune_uid ? gst_find_tuned_batch(ctx, tune_uid) : nullptr;
+    const int R = bo.rounds_per_step > 0 ? bo.rounds_per_step : (tb && tb->rounds_per_step > 0 ? tb->rounds_per_step : 4);
+    const uint32_t tuned_lanes = tb ? (uint32_t)tb->push_lanes : 0u;
+    const bool timed = bo.time_kernels != 0;
+    const bool use_graph = bo.use_graph != 0 && !timed;
+    std::vector<hipEvent_t> tev;
+    double t_grow = 0.0, t_merge = 0.0;
+    uint64_t n_merge_launch = 0;
+    if (timed) {
+        tev.resize(2 + 2 * (size_t)R);
+        for (auto& e : tev) HIPCHK(hipEventCreate(&e));
+    }
+    struct EvFree {
+        std::vector<hipEvent_t>& v;
+        ~EvFree() { for (auto& e : v) hipEventDestroy(e); }
+    } ev_free{tev};
 
     BatchArgs ba;
     ba.slots = E->d_slots; ba.gargs = E->d_gargs; ba.margs = E->d_margs; ba.targs = E->d_targs; ba.plan = E->d_plan;
@@ -610,7 +666,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
             auto& sl = E->slots[(size_t)si];
             sl.job = job;
             sl.job.t_fill = now_ms();
-            gst_status fs = fill_slot(ctx, E, si, sl.job, chunk);
+            gst_status fs = fill_slot(ctx, E, si, sl.job, chunk, tuned_lanes);
             if (fs != GST_OK) return fs;
             busy++;
             return GST_OK;
@@ -641,11 +697,21 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     const uint64_t max_steps = 2ull * 1000 * 1000;
     while (busy > 0) {
         if (graph_ready) HIPCHK(hipGraphLaunch(E->gexec, s));
-        else enqueue_step(ctx, E, ba, R, flt);
+        else enqueue_step(ctx, E, ba, R, flt, timed ? tev.data() : nullptr);
         HIPCHK(hipGetLastError());
         steps++;
         HIPCHK(hipMemcpyAsync(E->h_slots, E->d_slots, sizeof(SlotDev) * (size_t)S, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        if (timed) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, tev[0], tev[1]));
+            t_merge += ms;
+            n_merge_launch++;
+            for (int r = 0; r < R; ++r) {
+                HIPCHK(hipEventElapsedTime(&ms, tev[2 + 2 * (size_t)r], tev[3 + 2 * (size_t)r]));
+                t_grow += ms;
+            }
+        }
         if (steps > max_steps) return gst_set_error(ctx, GST_INTERNAL, "batch: queries did not finish");
         // finished queries: read root, control blocks, statistics and edge count, then the edges
         std::vector<int> fin;
@@ -767,6 +833,10 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         bst->slots = S;
         bst->rounds_per_step = R;
         bst->used_graph = graph_ready ? 1 : 0;
+        bst->timed_kernels = timed ? 1 : 0;
+        bst->t_grow_ms = t_grow;
+        bst->t_merge_ms = t_merge;
+        bst->merge_launches = n_merge_launch;
         bst->queries = (int64_t)n_done;
     }
     return first_err;

@@ -51,6 +51,7 @@ struct gst_query_bufs {
     RoundCtl* ctl = nullptr;
     unsigned long long* root = nullptr;
     unsigned long long* stats = nullptr;
+    uint32_t* spider = nullptr;  // cheapest spider (automatic bound)
     uint32_t* edges = nullptr;   // 3 * cap_edges + 1 (count)
     int32_t* grp = nullptr;      // grp_ptr (g+1) then members
     // level tables for the current g
@@ -90,7 +91,7 @@ static void free_bufs(gst_query_bufs* b) {
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
     hipFree(b->all_items); hipFree(b->all_cnt);
-    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog);
+    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -256,6 +257,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
     if (opts.half_bound != 0 && opts.half_bound != -1)
         return gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1 (got %d)", opts.half_bound);
+    if (opts.merge_tile_kb < 0 || opts.merge_tile_kb > 48)
+        return gst_set_error(ctx, GST_INVALID_ARG, "merge_tile_kb must be 0..48 (got %d)", opts.merge_tile_kb);
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)ctx->stream;
     const int g = groups->n_groups;
@@ -265,8 +268,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
 
     // bound in cost bits
     const bool has_ub = std::isfinite(opts.ub) && opts.ub > 0;
-    const gst_bound_bits bb = gst_make_bound(opts.ub, flt, opts.half_bound);
-    const uint32_t ub_bits = bb.ub;
+    gst_apply_tuned(ctx, G, g, has_ub, opts);
+    gst_bound_bits bb = gst_make_bound(opts.ub, flt, opts.half_bound);
+    uint32_t ub_bits = bb.ub;
 
     // ---- buffers ----
     if (!ctx->qb) ctx->qb = new gst_query_bufs();
@@ -538,6 +542,25 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         if (g > 1) st = run_grow(1);
         else k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T, n, B->root);
     }
+    // ---- automatic bound: the cheapest spider once the group distances are final ----
+    if (st == GST_OK && opts.auto_bound && g >= 3) {
+        if (!B->spider && hipMalloc(&B->spider, 8) != hipSuccess) return gst_set_error(ctx, GST_OOM, "solve: allocation failed");
+        k_fill_u32<<<1, 64, 0, s>>>(B->spider, GST_INF_COST, 1);
+        if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], n, (uint32_t)g, B->spider);
+        else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], n, (uint32_t)g, B->spider);
+        uint32_t sp = GST_INF_COST;
+        HIPCHK(hipMemcpyAsync(&sp, B->spider, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (sp != GST_INF_COST) {
+            double U;
+            if (flt) { float f; std::memcpy(&f, &sp, 4); U = (double)f; } else U = (double)sp;
+            const gst_bound_bits sb = gst_make_bound(U, flt, opts.half_bound);
+            // level 1 keeps the cap it was grown under (bb.lbcap)
+            if (sb.ub < ub_bits) ub_bits = sb.ub;
+            if (sb.gmax < bb.gmax) bb.gmax = sb.gmax;
+            bb.ub = ub_bits;
+        }
+    }
     // ---- levels 2..g ----
     for (int k = 2; k <= g && st == GST_OK; ++k) {
         const bool full = (k == g);
@@ -549,11 +572,12 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ma.n = n; ma.g = (uint32_t)g; ma.k = (uint32_t)k; ma.P = (uint32_t)L.pair_cnt[k];
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ub_bits; ma.use_lb = opts.prune_lb ? 1u : 0u;
         ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
+        ma.spider = nullptr;
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;
         const bool pin = pair_bytes <= MERGE_PAIRS_LDS;
-        const uint32_t VB = merge_tile_vertices(Lk, Wk);
+        const uint32_t VB = merge_tile_vertices(Lk, Wk, (size_t)opts.merge_tile_kb * 1024);
         ma.VB = VB;
         ma.pairs_in_lds = pin ? 1u : 0u;
         const size_t lds = merge_lds_bytes(VB, Lk, Wk, pair_bytes, pin);

@@ -117,6 +117,7 @@ struct MergeArgs {
     uint32_t pairs_in_lds;
     uint32_t gmax;         // merged states above this cost are not put on the frontier
     uint32_t lbcap;        // cap on level-1 distances in the lower bound (see GrowArgs)
+    uint32_t* spider;      // k == 2 with an automatic bound: min over the tile's vertices of sum_i T(v,{i})
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -188,6 +189,27 @@ __device__ __forceinline__ unsigned long long wave_min64(unsigned long long v) {
         v = o < v ? o : v;
     }
     return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min32(uint32_t v) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) v = min(v, (uint32_t)__shfl_xor(v, d, WAVE));
+    return v;
+}
+
+// Bound bits of a spider cost sp (a feasible tree's cost), by the rule of the host's bound set-up:
+// ints keep cost <= sp and grow cost <= sp/2; floats get a relative slack of about 1e-4 and one ulp
+// (bit patterns of positive floats step by one ulp per unit).
+template <bool FLT>
+__device__ __forceinline__ void spider_bound(uint32_t sp, uint32_t& ub, uint32_t& gmax) {
+    if (FLT) {
+        const float u = __fmul_rn(__uint_as_float(sp), 1.0001f);
+        ub = __float_as_uint(u) + 1u;
+        gmax = __float_as_uint(__fmul_rn(__uint_as_float(ub), 0.5f)) + 1u;
+    } else {
+        ub = sp + 1u;
+        gmax = sp / 2u;
+    }
 }
 
 __device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
@@ -1128,7 +1150,7 @@ __global__ void k_async_begin(const RoundCtl* ctl, AsyncCtl* ac) {
 // then a coalesced store of the level-k rows.  No atomics except the LDS bitmap of finite masks.
 // ---------------------------------------------------------------------------------------
 constexpr int MERGE_THREADS = 256;
-// LDS budget of a merge tile's cost rows: small enough for several workgroups per CU
+// default LDS budget of a merge tile's cost rows: small enough for several workgroups per CU
 constexpr size_t MERGE_TILE_BYTES = 16 * 1024;
 // split tables up to this size are staged in LDS
 constexpr size_t MERGE_PAIRS_LDS = 16 * 1024;
@@ -1137,18 +1159,25 @@ constexpr size_t MERGE_PAIRS_LDS = 16 * 1024;
 __host__ __device__ __forceinline__ size_t merge_rows_words(uint32_t VB, uint32_t L) {
     return ((size_t)VB * L + 1) / 2;
 }
-// vertices per merge tile for L lower-level costs and W bitmap words per vertex
-__host__ __device__ __forceinline__ uint32_t merge_tile_vertices(uint32_t L, uint32_t W) {
-    const size_t per = (size_t)L * 4 + (size_t)W * 8;
-    size_t vb = MERGE_TILE_BYTES / per;
+// vertices per merge tile for L lower-level costs and W bitmap words per vertex within a budget of
+// tile_bytes (0: the default)
+__host__ __device__ __forceinline__ uint32_t merge_tile_vertices(uint32_t L, uint32_t W, size_t tile_bytes = 0) {
+    const size_t per = (size_t)L * 4 + (size_t)W * 8 + 4;
+    size_t vb = (tile_bytes ? tile_bytes : MERGE_TILE_BYTES) / per;
     if (vb < 1) vb = 1;
     if (vb > 256) vb = 256;
     return (uint32_t)vb;
 }
-// dynamic LDS bytes of a merge launch
+// dynamic LDS bytes of a merge launch: rows, bitmaps, group masks of the tile, split table
 __host__ __device__ __forceinline__ size_t merge_lds_bytes(uint32_t VB, uint32_t L, uint32_t W, size_t pair_bytes,
                                                          bool pin) {
-    return merge_rows_words(VB, L) * 8 + (size_t)VB * W * 8 + (pin ? pair_bytes : 0);
+    return merge_rows_words(VB, L) * 8 + (size_t)VB * W * 8 + ((size_t)VB + 1) / 2 * 8 + (pin ? pair_bytes : 0);
+}
+
+// i / d for i < 2^20 and 1 <= d < 2^12 without an integer division: (i + 0.5) / d in float is
+// off by less than 2^-3 / d, while its fractional part stays at least 0.5 / d away from an integer
+__device__ __forceinline__ uint32_t div_small(uint32_t i, float inv_d) {
+    return (uint32_t)(((float)i + 0.5f) * inv_d);
 }
 
 template <bool FLT>
@@ -1160,43 +1189,75 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     const uint32_t VB = a.VB;
     const uint32_t W = a.W;
     const uint32_t g = a.g;
-    // LDS: the tile's lower-level costs (the merge reads no predecessors), then the bitmaps of
-    // finite outputs, then the split table
+    // LDS: the tile's lower-level costs (the merge reads no predecessors), the bitmaps of finite
+    // outputs, the group masks of the tile's vertices, then the split table
     uint32_t* rows = (uint32_t*)lds;                                            // VB * L costs
     unsigned long long* bm = lds + merge_rows_words(VB, L);                     // VB * W
+    uint32_t* gms = (uint32_t*)(bm + (size_t)VB * W);                           // VB
     const unsigned long long* pt = (const unsigned long long*)a.pairs;
     if (a.pairs_in_lds) {
-        unsigned long long* pl = bm + (size_t)VB * W;
+        unsigned long long* pl = (unsigned long long*)(gms + ((VB + 1) & ~1u));
         for (uint32_t i = threadIdx.x; i < a.P; i += blockDim.x) pl[i] = a.pairs[i];
         pt = pl;
     }
     const bool prune = a.ub != GST_INF_COST;
     const bool prune_lb = prune && a.use_lb && !a.is_full;
+    const float inv_mk = 1.0f / (float)mk;
     uint32_t ovf = 0;
     unsigned long long best_root = GST_INF_WORD;
     uint64_t st_pairs = 0, st_vert = 0;
     const uint64_t* Tk_base = a.T + a.lvl_base[k];
+    const uint32_t* mk_masks = a.masks + a.off[k];
 
     for (uint32_t v0 = bid * VB; v0 < a.n; v0 += nblk * VB) {
         const uint32_t nv = min(VB, a.n - v0);
         __syncthreads();
+        // stage the cost halves of the lower-level runs (each level's run of the tile is one
+        // contiguous block of nv * m_j words), four loads in flight per thread
         for (uint32_t j = 1; j < k; ++j) {
             const uint32_t mj = a.m[j];
+            const float inv_mj = 1.0f / (float)mj;
             const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v0 * mj;
             const uint32_t cnt = nv * mj;
-            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-                const uint32_t b = i / mj, t = i - b * mj;
-                rows[b * L + a.off[j] + t] = (uint32_t)(src[i] >> 32);
+            const uint32_t oj = a.off[j];
+            for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += 4 * blockDim.x) {
+                uint64_t x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t i = i0 + u * blockDim.x;
+                    x[u] = i < cnt ? src[i] : 0ull;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t i = i0 + u * blockDim.x;
+                    if (i < cnt) {
+                        const uint32_t b = div_small(i, inv_mj), t = i - b * mj;
+                        rows[b * L + oj + t] = (uint32_t)(x[u] >> 32);
+                    }
+                }
             }
         }
         for (uint32_t i = threadIdx.x; i < nv * W; i += blockDim.x) bm[i] = 0;
+        for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x) gms[i] = a.gm[v0 + i];
         __syncthreads();
+        if (a.spider) {
+            // cheapest spider of the tile: sum of the group distances of one vertex (positions
+            // 0..g-1 of the staged rows are the level-1 masks 1<<i)
+            uint32_t sm = GST_INF_COST;
+            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
+                uint32_t acc = 0, o2 = 0;
+                for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, rows[b * L + gi], o2);
+                sm = min(sm, o2 ? GST_INF_COST : acc);
+            }
+            sm = wave_min32(sm);
+            if (lane_id() == 0 && sm != GST_INF_COST) atomicMin(a.spider, sm);
+        }
         const uint32_t items = nv * mk;
         for (uint32_t i = threadIdx.x; i < items; i += blockDim.x) {
-            const uint32_t b = i / mk, r = i - b * mk;
+            const uint32_t b = div_small(i, inv_mk), r = i - b * mk;
             const uint32_t v = v0 + b;
-            const uint32_t X = a.masks[a.off[k] + r];
-            const uint32_t gmv = a.gm[v];
+            const uint32_t X = mk_masks[r];
+            const uint32_t gmv = gms[b];
             const uint32_t* rv = rows + (size_t)b * L;
             unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
             const unsigned long long* pr = pt + (size_t)r * S;
@@ -1266,6 +1327,20 @@ template <bool FLT>
 __global__ void __launch_bounds__(MERGE_THREADS) k_merge(MergeArgs a) {
     extern __shared__ unsigned long long lds[];
     merge_body<FLT>(a, lds, blockIdx.x, gridDim.x);
+}
+
+// cheapest spider: min over v of sum_i T(v,{i}) (the union of v's shortest paths to every group
+// holds a tree of at most that cost)
+template <bool FLT>
+__global__ void k_spider_scan(const uint64_t* T1, uint32_t n, uint32_t g, uint32_t* out) {
+    uint32_t sm = GST_INF_COST;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t acc = 0, o2 = 0;
+        for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, (uint32_t)(T1[v * g + gi] >> 32), o2);
+        sm = min(sm, o2 ? GST_INF_COST : acc);
+    }
+    sm = wave_min32(sm);
+    if (lane_id() == 0 && sm != GST_INF_COST) atomicMin(out, sm);
 }
 
 // g == 1: the full mask is level 1; its minimum is any member (cost 0)

@@ -110,7 +110,22 @@ inline bool gst_push_lanes_ok(int32_t p) { return p == 0 || p == 8 || p == 16 ||
 
 struct gst_query_bufs;  // device buffers of one query (defined in gst_dpbf.hip)
 
+// knobs chosen by gst_autotune for one graph (and group count / bound presence), applied to
+// solves whose options leave them at 0
+struct gst_tuned {
+    uint64_t uid = 0;
+    int g = 0;
+    bool ub = false;
+    int32_t chunk_arcs = 0, pull_permille = 0, push_lanes = 0, merge_tile_kb = 0;
+};
+struct gst_tuned_batch {
+    uint64_t uid = 0;
+    int32_t rounds_per_step = 0, push_lanes = 0;
+};
+
 struct gst_ctx {
+    std::vector<gst_tuned> tuned;
+    std::vector<gst_tuned_batch> tuned_batch;
     int device = 0;
     std::string last_error;
     void* stream = nullptr;     // hipStream_t
@@ -134,5 +149,21 @@ gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g);
 void gst_graph_device_free(gst_graph* g);
 
 gst_status gst_set_error(gst_ctx* ctx, gst_status s, const char* fmt, ...);
+// fill the knobs an opts leaves at 0 from the autotuner's choice for (graph, g, bound present)
+inline void gst_apply_tuned(const gst_ctx* ctx, const gst_graph* G, int g, bool has_ub, gst_opts& o) {
+    for (const gst_tuned& t : ctx->tuned) {
+        if (t.uid != G->uid || t.g != g || t.ub != has_ub) continue;
+        if (o.chunk_arcs == 0) o.chunk_arcs = t.chunk_arcs;
+        if (o.pull_permille == 0) o.pull_permille = t.pull_permille;
+        if (o.push_lanes == 0) o.push_lanes = t.push_lanes;
+        if (o.merge_tile_kb == 0) o.merge_tile_kb = t.merge_tile_kb;
+        return;
+    }
+}
+inline const gst_tuned_batch* gst_find_tuned_batch(const gst_ctx* ctx, uint64_t uid) {
+    for (const gst_tuned_batch& t : ctx->tuned_batch)
+        if (t.uid == uid) return &t;
+    return nullptr;
+}
 uint64_t gst_next_graph_uid();
 const gst_levels& gst_get_levels(gst_ctx* ctx, int g);

@@ -1157,6 +1157,7 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
     gst_opts so{};
     so.prune_lb = 1;
     so.want_tree = 0;   // the bound is a cost
+    so.auto_bound = 1;
     gst_result sr;
     std::memset(&sr, 0, sizeof(sr));
     double ub = 0.0;
@@ -1271,6 +1272,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             j.opts = gst_opts();
             j.opts.prune_lb = 1;
             j.opts.want_tree = 0;
+            j.opts.auto_bound = 1;
             j.temp_graph = true;
             j.res = &x.sub_res;
             j.tag = q;
@@ -1305,7 +1307,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         x.L.ub = x.have_ub ? x.ub : -1.0;
         if (lstats) lstats[q] = x.L;
     };
-    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats);
+    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid);
     for (auto& b : bufs) sub_free(b);
     for (auto& x : qs) gst_result_free(&x.sub_res);
     if (setup_err != GST_OK) return setup_err;

@@ -83,7 +83,12 @@ typedef struct {
                                   grow across the centre edge rebuild it; the tree-decomposition rule
                                   of PrunedDP, rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:795,
                                   src/DPQ.cu:255-258); -1 = grow every state below U */
-    int32_t reserved[4];
+    int32_t merge_tile_kb;     /* LDS budget of a merge tile's cost rows in KiB, 1..48 (0 = tuned default) */
+    int32_t auto_bound;        /* 1: once the group distances (level 1) are final, bound the rest of the solve
+                                  by the cheapest spider, min over vertices c of sum_i dist(c, G_i): the union
+                                  of those shortest paths holds a tree touching every group, so the bound is
+                                  feasible (with g >= 3; the half_bound rule applies to it too) */
+    int32_t reserved[2];
 } gst_opts;
 
 typedef struct {
@@ -173,7 +178,9 @@ typedef struct {
     int32_t slots;             /* queries in flight (0: tuned, at most 64, limited by device memory) */
     int32_t rounds_per_step;   /* grow rounds per step between host checks (0: tuned) */
     int32_t use_graph;         /* 1: replay each step as a HIP graph (default), 0: launch kernel by kernel */
-    int32_t reserved[5];
+    int32_t time_kernels;      /* 1: launch kernel by kernel with HIP events around every batched grow and
+                                  merge launch (t_grow_ms / t_merge_ms in the stats) */
+    int32_t reserved[4];
 } gst_batch_opts;
 
 typedef struct {
@@ -190,7 +197,11 @@ typedef struct {
     int32_t  slots;
     int32_t  rounds_per_step;
     int32_t  used_graph;
-    int32_t  reserved[5];
+    int32_t  timed_kernels;
+    double   t_grow_ms;        /* with time_kernels: device time of the batched grow launches (HIP events) */
+    double   t_merge_ms;       /* with time_kernels: device time of the batched merge launches */
+    uint64_t merge_launches;
+    int32_t  reserved[4];
 } gst_batch_stats;
 
 /* groups[nq]; opts: NULL (plain DPBF, trees returned) or nq entries, of which ub, prune_lb and
@@ -250,6 +261,28 @@ gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq,
                                    gst_model* m, double topk_frac, const gst_opts* opts,
                                    const gst_batch_opts* bopts, gst_result* results, gst_status* status,
                                    gst_learned_stats* lstats, gst_batch_stats* stats);
+
+/* --- autotuner ----------------------------------------------------------------------- */
+/* No counterpart in the reference (its launch shapes are fixed, src/DPQ.cu:597-610).  Times the
+ * sample (queries of one group count on graph g) under candidate launch knobs and keeps the fastest:
+ * for one-query solves the pull threshold (frontier size at which a grow round pulls over every
+ * vertex), the arcs per frontier work item, the lanes per work item in push rounds and the merge
+ * tile (vertices per merge workgroup); with GST_TUNE_BATCH also the batched engine's rounds per
+ * step and lanes per item.  Later solves on this graph that leave a knob at 0 use the choice.
+ * GST_TUNE_BOUNDED tunes bounded solves (bound = 1.05 x each sample query's optimum). */
+#define GST_TUNE_BOUNDED 1
+#define GST_TUNE_BATCH   2
+typedef struct {
+    int32_t chunk_arcs, pull_permille, push_lanes, merge_tile_kb;   /* chosen for one-query solves */
+    int32_t batch_rounds_per_step, batch_push_lanes;                /* chosen for the engine (GST_TUNE_BATCH) */
+    int32_t configs_tried;
+    int32_t pad;
+    double  default_ms, tuned_ms;          /* mean ms per sample query: built-in defaults, choice */
+    double  batch_default_qps, batch_tuned_qps;
+} gst_tune_report;
+
+gst_status gst_autotune(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* sample, int32_t flags,
+                        gst_tune_report* report);
 
 /* --- synthetic instances (host, deterministic) ------------------------------------------ */
 /* Host CSR owned by the library.  The generators use the library's own seeded RNG

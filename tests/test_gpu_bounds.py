@@ -100,3 +100,53 @@ def test_batch_bounded_half_rule(solver):
         _exact(G, q, r, w)
         assert r.used_ub == (w > 0)   # a bound of 0 means "no bound"
     dg.free()
+
+
+@pytest.mark.parametrize("g", [2, 3, 4, 6, 8])
+def test_auto_bound_single_and_batch(solver, g):
+    """The automatic (spider) bound keeps the oracle's optimum: alone, under a caller's bound, in
+    the batched engine, on int, zero-weight and float graphs."""
+    for kind in ("int", "zero", "float"):
+        if kind == "int":
+            G = gst.gen_random_connected(1000, 5000, 1, 100, seed=400 + g)
+        elif kind == "zero":
+            G = gst.gen_random_connected(600, 2400, 0, 3, seed=410 + g)
+        else:
+            G = gst.gen_rmat(10, 8, weights="lognormal", seed=420 + g)
+        dg = solver.upload(G)
+        qs = [gst.gen_groups(G.n, g, 2, 25, seed=430 + 10 * g + i) for i in range(3)]
+        want = [oracle.dpbf_cost(G, q) for q in qs]
+        tol = (lambda a, b: abs(a - b) <= 1e-6 * max(1.0, b)) if kind == "float" else (lambda a, b: a == b)
+        for q, w in zip(qs, want):
+            if not np.isfinite(w):   # R-MAT leaves some vertices unconnected: no tree exists
+                with pytest.raises(gst.GstError) as e:
+                    solver.solve(dg, q, auto_bound=True)
+                assert e.value.status == gst._abi.GST_INFEASIBLE
+                continue
+            r = solver.solve(dg, q, auto_bound=True)
+            assert tol(r.cost, w), (kind, r.cost, w)
+            assert tol(gst.validate_tree(G, q, r.eu, r.ev, r.root), w)
+            r = solver.solve(dg, q, auto_bound=True, ub=w * 1.3 + 1)
+            assert tol(r.cost, w)
+        res, _ = solver.solve_batch(dg, qs, slots=2, auto_bound=True, raise_on_error=False)
+        for q, r, w in zip(qs, res, want):
+            if not np.isfinite(w):
+                assert isinstance(r, gst.GstError) and r.status == gst._abi.GST_INFEASIBLE
+                continue
+            assert tol(r.cost, w), (kind, "batch", r.cost, w)
+            assert tol(gst.validate_tree(G, q, r.eu, r.ev, r.root), w)
+        dg.free()
+
+
+def test_auto_bound_prunes_ba20k(solver):
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    for q in range(3):
+        groups = gst.gen_groups(G.n, 6, 10, 200, seed=1000 + q)
+        want = GOLD["ba20k"]["costs"][str(q)]
+        plain = solver.solve(dg, groups)
+        r = solver.solve(dg, groups, auto_bound=True)
+        print(f"q{q}: auto-bound relax {r.stats['grow_relax']} vs plain {plain.stats['grow_relax']}", flush=True)
+        _exact(G, groups, r, want)
+        assert r.stats["grow_relax"] < plain.stats["grow_relax"]
+    dg.free()

@@ -157,3 +157,19 @@ def test_pruned_grow_limit_below_half_is_caught():
         b = oracle.bruteforce(G, groups)
         lost += oracle.dpbf_pruned(G, groups, b, 1.0 / 3.0)[0] != b
     assert lost >= 1
+
+
+@pytest.mark.parametrize("s", range(8))
+def test_spider_bound_is_feasible_and_keeps_optimum(s):
+    """The automatic bound min_c sum_i dist(c, G_i) is the cost of a connected subgraph touching
+    every group, so it is >= OPT (= OPT for g <= 3), and the bounded search under it keeps OPT."""
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=60 + s) if s % 2 == 0 else \
+        gst.gen_random_connected(600, 2400, 0, 5, seed=60 + s)
+    g = 3 + s % 5
+    groups = gst.gen_groups(G.n, g, 3, 20, seed=70 + s)
+    opt = oracle.dpbf_cost(G, groups)
+    spider = float(np.min(oracle.group_distances(G, groups).sum(axis=0)))
+    assert spider >= opt
+    if g == 3:
+        assert spider == opt
+    assert oracle.dpbf_pruned(G, groups, spider)[0] == opt
