@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--slots", type=int, default=64, help="queries in flight in the batched engine")
     ap.add_argument("--model", default=os.path.join(ROOT, "gpu4gst_code_amd", "models", "mpnn_ba50k.npy"))
     ap.add_argument("--autotune", type=int, default=1, help="1: pick launch knobs with gst_autotune before timing")
+    ap.add_argument("--rounds-per-step", type=int, default=0, help="engine rounds per step (0: tuned / default)")
     ap.add_argument("--cpu-sample", type=int, default=16, help="queries in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -89,13 +90,14 @@ def main():
     # warm-up: engine slots, level tables, captured steps, the model's buffers, the plain path
     if args.warmup > 0:
         solver.solve_batch_learned(dg, [queries[i % 64] for i in range(args.warmup)], model, args.frac,
-                                   slots=args.slots, time_kernels=True)
+                                   slots=args.slots, rounds_per_step=args.rounds_per_step, time_kernels=True)
         for i in range(min(args.warmup, 4)):
             solver.solve(dg, queries[i % 64])
 
     # ---- timed: batched learned-bound solve (each call ends with a stream synchronisation) ----
     t0 = time.perf_counter()
-    res, bst = solver.solve_batch_learned(dg, timed_q, model, args.frac, slots=args.slots, time_kernels=True)
+    res, bst = solver.solve_batch_learned(dg, timed_q, model, args.frac, slots=args.slots,
+                                          rounds_per_step=args.rounds_per_step, time_kernels=True)
     wall = time.perf_counter() - t0
     qps = args.steps / wall
 

@@ -591,7 +591,8 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
 
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
-                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid) {
+                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
+                         const std::function<void()>& idle) {
     gst_batch_opts bo{};
     if (bo_in) bo = *bo_in;
     if (!ctx->be) ctx->be = new gst_batch_engine();
@@ -701,6 +702,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         HIPCHK(hipGetLastError());
         steps++;
         HIPCHK(hipMemcpyAsync(E->h_slots, E->d_slots, sizeof(SlotDev) * (size_t)S, hipMemcpyDeviceToHost, s));
+        if (idle) idle();
         HIPCHK(hipStreamSynchronize(s));
         if (timed) {
             float ms = 0.f;

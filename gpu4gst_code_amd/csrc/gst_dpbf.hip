@@ -124,7 +124,9 @@ void gst_ctx_device_fini(gst_ctx* ctx) {
     hipSetDevice(ctx->device);
     gst_query_bufs_free(ctx);
     if (ctx->stream) hipStreamDestroy((hipStream_t)ctx->stream);
+    if (ctx->stream2) hipStreamDestroy((hipStream_t)ctx->stream2);
     ctx->stream = nullptr;
+    ctx->stream2 = nullptr;
 }
 
 gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {

@@ -129,6 +129,7 @@ struct gst_ctx {
     int device = 0;
     std::string last_error;
     void* stream = nullptr;     // hipStream_t
+    void* stream2 = nullptr;    // hipStream_t (default priority): learned-bound selection beside the engine
     gst_query_bufs* qb = nullptr;
     struct gst_batch_engine* be = nullptr;   // batched engine (gst_batch.hip), made on first use
     gst_levels lv_cache[GST_MAX_GROUPS + 1];

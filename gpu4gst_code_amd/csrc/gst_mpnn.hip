@@ -766,12 +766,17 @@ struct gst_model {
     uint64_t cap_scratch = 0;
     uint32_t* h_sel = nullptr;         // pinned readback of a selection (sizes, group ids)
     uint32_t cap_h_sel = 0;
+    void* sel_stream = nullptr;        // stream of the forward pass and selection (nullptr: the context's)
     uint32_t* hub_list = nullptr;   // hub row ids
     uint32_t* hub_items = nullptr;  // pairs (row, h | chunk << 20)
     uint32_t n_hub = 0, n_hub_items = 0;
 };
 
 namespace {
+
+hipStream_t mstream(const gst_model* M) {
+    return (hipStream_t)(M->sel_stream ? M->sel_stream : M->ctx->stream);
+}
 
 gst_status model_bufs(gst_model* M, const gst_graph* G) {
     gst_ctx* ctx = M->ctx;
@@ -800,7 +805,7 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
         if (!dalloc(M->hub_list, hl.size()) || !dalloc(M->hub_items, hi.size()) ||
             !dalloc(M->hub_acc, std::max<size_t>(hl.size(), 1) * HID))
             return gst_set_error(ctx, GST_OOM, "model: hub allocation failed");
-        hipStream_t st = (hipStream_t)ctx->stream;
+        hipStream_t st = mstream(M);
         if (!hl.empty()) {
             HIPCHK(hipMemcpyAsync(M->hub_list, hl.data(), hl.size() * 4, hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(M->hub_items, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, st));
@@ -815,7 +820,7 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
 
 gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* groups) {
     gst_ctx* ctx = M->ctx;
-    hipStream_t st = (hipStream_t)ctx->stream;
+    hipStream_t st = mstream(M);
     const int g = groups->n_groups;
     const uint32_t cnt = (uint32_t)(g + 1 + groups->grp_ptr[g]);
     if (M->cap_grp < cnt) {
@@ -856,7 +861,7 @@ gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups) {
     if (sst != GST_OK) return sst;
     sst = upload_groups(M, G, groups);
     if (sst != GST_OK) return sst;
-    hipStream_t st = (hipStream_t)ctx->stream;
+    hipStream_t st = mstream(M);
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
     const int dg = std::max(1, ctx->cu_count * 8);
@@ -1050,7 +1055,7 @@ bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms) {
 gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SubBuf& buf,
                             gst_graph& S, std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool want_host_rows) {
     gst_ctx* ctx = M->ctx;
-    hipStream_t st = (hipStream_t)ctx->stream;
+    hipStream_t st = mstream(M);
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
     gst_status sst = forward(M, G, groups);
@@ -1133,6 +1138,8 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
         HIPCHK(hipStreamSynchronize(st));
         S.h_row.assign(r32.begin(), r32.end());
     }
+    // the model's scratch is reused by the next selection, possibly on another stream
+    HIPCHK(hipStreamSynchronize(st));
     return GST_OK;
 }
 
@@ -1217,17 +1224,25 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         gst_learned_stats L{};
         double ub = 0.0;
         bool have_ub = false, retried = false;
-        double t_sub_fill = 0.0, t_full_fill = 0.0;
     };
     std::vector<Q> qs((size_t)std::max(nq, 0));
     std::vector<SubBuf> bufs;
     std::vector<int> free_bufs;
     std::vector<int32_t> pending_full;   // queries whose full solve may start
-    int32_t next = 0;
+    std::vector<int32_t> ready;          // queries whose subgraph is built (its solve may start)
+    size_t ready_head = 0;
+    int32_t next = 0;                    // next query to select
     int gmax = 1;
     for (int32_t q = 0; q < nq; ++q)
         if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
     gst_status setup_err = GST_OK;
+    // the selection of upcoming queries runs on a second stream while the engine's step runs
+    if (!ctx->stream2) {
+        hipStream_t s2;
+        HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        ctx->stream2 = (void*)s2;
+    }
+    const int lookahead = 8;
     auto full_job = [&](int32_t q, gst_batch_job& j) {
         Q& x = qs[(size_t)q];
         j = gst_batch_job();
@@ -1236,9 +1251,37 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         if (opts) j.opts = opts[q];
         else { j.opts = gst_opts(); j.opts.prune_lb = 1; j.opts.want_tree = 1; }
         j.opts.ub = x.have_ub && !x.retried ? x.ub : 0.0;
+        j.opts.auto_bound = 1;   // the spider bound tightens the subgraph bound where it is lower
         j.res = &results[q];
         j.tag = q;
         j.kind = 1;
+    };
+    // scores, selection and subgraph of query q on `stream` (nullptr: the context's stream); the
+    // query then waits in `ready`, or, without a subgraph, goes straight to its full solve
+    auto select = [&](int32_t q, void* stream) -> gst_status {
+        Q& x = qs[(size_t)q];
+        if (free_bufs.empty()) { bufs.emplace_back(); free_bufs.push_back((int)bufs.size() - 1); }
+        x.buf = free_bufs.back();
+        free_bufs.pop_back();
+        const double t0 = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        M->sel_stream = stream;
+        gst_status ss = learned_subgraph(M, G, &groups[q], frac, bufs[(size_t)x.buf], x.S, x.sub_vtx, x.L, false);
+        M->sel_stream = nullptr;
+        x.L.t_score_select_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+        if (ss != GST_OK && ss != GST_OOM) { status[q] = ss; free_bufs.push_back(x.buf); x.buf = -1; return ss; }
+        if (ss != GST_OK || x.S.n == 0) {
+            free_bufs.push_back(x.buf);
+            x.buf = -1;
+            pending_full.push_back(q);   // no subgraph: the plain solve runs (a fallback)
+            return GST_OK;
+        }
+        x.sg = gst_groups{groups[q].n_groups, groups[q].grp_ptr, x.sub_vtx.data()};
+        ready.push_back(q);
+        return GST_OK;
+    };
+    auto next_valid = [&]() {
+        while (next < nq && status[next] != GST_OK) ++next;
+        return next < nq;
     };
     auto next_job = [&](gst_batch_job& j) -> bool {
         if (!pending_full.empty()) {
@@ -1247,39 +1290,41 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             full_job(q, j);
             return true;
         }
-        while (next < nq && setup_err == GST_OK) {
-            const int32_t q = next++;
-            if (status[q] != GST_OK) continue;
-            Q& x = qs[(size_t)q];
-            if (free_bufs.empty()) { bufs.emplace_back(); free_bufs.push_back((int)bufs.size() - 1); }
-            x.buf = free_bufs.back();
-            free_bufs.pop_back();
-            const double t0 = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-            gst_status ss = learned_subgraph(M, G, &groups[q], frac, bufs[(size_t)x.buf], x.S, x.sub_vtx, x.L, false);
-            x.L.t_score_select_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
-            if (ss != GST_OK && ss != GST_OOM) { setup_err = ss; status[q] = ss; free_bufs.push_back(x.buf); x.buf = -1; return false; }
-            if (ss != GST_OK || x.S.n == 0) {
-                // no subgraph: the plain solve runs (a fallback)
-                free_bufs.push_back(x.buf);
-                x.buf = -1;
+        if (ready_head >= ready.size() && setup_err == GST_OK && next_valid()) {
+            // nothing prepared: select one query now, on the engine's stream
+            gst_status ss = select(next++, nullptr);
+            if (ss != GST_OK) { setup_err = ss; return false; }
+            if (!pending_full.empty()) {
+                const int32_t q = pending_full.back();
+                pending_full.pop_back();
                 full_job(q, j);
                 return true;
             }
-            x.sg = gst_groups{groups[q].n_groups, groups[q].grp_ptr, x.sub_vtx.data()};
-            j = gst_batch_job();
-            j.G = &x.S;
-            j.groups = &x.sg;
-            j.opts = gst_opts();
-            j.opts.prune_lb = 1;
-            j.opts.want_tree = 0;
-            j.opts.auto_bound = 1;
-            j.temp_graph = true;
-            j.res = &x.sub_res;
-            j.tag = q;
-            j.kind = 0;
-            return true;
         }
-        return false;
+        if (ready_head >= ready.size()) return false;
+        const int32_t q = ready[ready_head++];
+        Q& x = qs[(size_t)q];
+        j = gst_batch_job();
+        j.G = &x.S;
+        j.groups = &x.sg;
+        j.opts = gst_opts();
+        j.opts.prune_lb = 1;
+        j.opts.want_tree = 0;
+        j.opts.auto_bound = 1;
+        j.temp_graph = true;
+        j.res = &x.sub_res;
+        j.tag = q;
+        j.kind = 0;
+        return true;
+    };
+    auto idle = [&]() {
+        hipStream_t s1 = (hipStream_t)ctx->stream;
+        // prepare upcoming subgraphs while the step runs; stop when the step is done
+        while (setup_err == GST_OK && ready.size() - ready_head < (size_t)lookahead && next_valid() &&
+               hipStreamQuery(s1) == hipErrorNotReady) {
+            gst_status ss = select(next++, ctx->stream2);
+            if (ss != GST_OK) { setup_err = ss; break; }
+        }
     };
     std::vector<gst_status> job_fail;   // statuses of failed jobs (the engine returns the first of them)
     auto job_done = [&](gst_batch_job& j) {
@@ -1307,7 +1352,8 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         x.L.ub = x.have_ub ? x.ub : -1.0;
         if (lstats) lstats[q] = x.L;
     };
-    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid);
+    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid, idle);
+    M->sel_stream = nullptr;
     for (auto& b : bufs) sub_free(b);
     for (auto& x : qs) gst_result_free(&x.sub_res);
     if (setup_err != GST_OK) return setup_err;
