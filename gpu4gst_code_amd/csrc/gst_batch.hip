@@ -181,18 +181,24 @@ __global__ void __launch_bounds__(WAVE) k_plan_grow(BatchArgs b) {
                 const uint32_t r = c->round;
                 const uint32_t n_cur = c->cnt[r % 3], n_prv = c->cnt[(r + 2) % 3];
                 const bool pull = n_cur > 0 && n_cur >= a.pull_thresh && a.all_items != nullptr;
-                // the same sizing as a one-query round: a pull round takes every workgroup, a push
-                // round one wave per item (and one thread per item of the list it clears)
-                need = pull ? b.grid
-                            : max(1u, min(b.grid, max((n_cur + GROW_WAVES - 1) / GROW_WAVES,
-                                                      (n_prv + GROW_BLOCK - 1) / GROW_BLOCK)));
+                // workgroups in proportion to the round's work, so that when the grid is shared out
+                // every slot's round takes about as long: a pull round sweeps every light row
+                // (64/min(m,64) rows per wave) and every hub item (one per wave); a push round
+                // serves 64/push_lanes items per wave; the list of the previous round is cleared
+                // with one thread per item
+                const uint32_t mch = a.m < (uint32_t)WAVE ? a.m : (uint32_t)WAVE;
+                const uint32_t rows_per_wave = (uint32_t)WAVE / mch;
+                const uint32_t items_per_wave = (uint32_t)WAVE / a.push_lanes;
+                const uint32_t waves = pull ? (a.n + rows_per_wave - 1) / rows_per_wave + a.n_all
+                                            : (n_cur + items_per_wave - 1) / items_per_wave;
+                need = max(1u, max((waves + GROW_WAVES - 1) / GROW_WAVES, (n_prv + GROW_BLOCK - 1) / GROW_BLOCK));
                 lvl = sd.k;
             }
         }
     }
-    const uint32_t total = wave_sum32(need);
-    uint32_t alloc = need;
-    if (total > b.grid && need) alloc = max(1u, (uint32_t)((uint64_t)need * b.grid / total));
+    const uint32_t total = wave_sum32(min(need, 1u << 24));
+    uint32_t alloc = min(need, b.grid);
+    if (total > b.grid && need) alloc = max(1u, (uint32_t)((uint64_t)min(need, 1u << 24) * b.grid / total));
     uint32_t incl = alloc;
 #pragma unroll
     for (int d = 1; d < WAVE; d <<= 1) {

@@ -414,7 +414,8 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     // only as many workgroups as the round needs take part (a sparse round costs a few blocks)
     uint32_t part = nblk;
     if (!pull) {
-        const uint32_t need_b = (n_cur + GROW_WAVES - 1) / GROW_WAVES;
+        const uint32_t per_blk = GROW_WAVES * ((uint32_t)WAVE / a.push_lanes);   // items per workgroup step
+        const uint32_t need_b = (n_cur + per_blk - 1) / per_blk;
         const uint32_t need_a = (n_prv + GROW_BLOCK - 1) / GROW_BLOCK;
         part = max(1u, min(nblk, max(need_b, need_a)));
     }
