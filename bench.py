@@ -35,6 +35,9 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="ba200k", choices=["ba200k", "c4", "rmat22", "rmat21"],
+                    help="ba200k: the headline (default); c4: 256 mixed queries on the BA graph; rmat22: float "
+                         "R-MAT scale 22, g=7; rmat21: learned bound on int R-MAT scale 21, g=8")
     ap.add_argument("--steps", type=int, default=256, help="queries in the timed run (the 64 queries cycled)")
     ap.add_argument("--warmup", type=int, default=16, help="queries solved before the timed run")
     ap.add_argument("--n", type=int, default=200_000)
@@ -63,6 +66,8 @@ def main():
         sys.exit(2)
     import gpu4gst_code_amd as gst
 
+    if args.workload != "ba200k":
+        return other_workload(args, gst)
     G = gst.gen_barabasi_albert(args.n, 5, 1, 1000, seed=1)
     queries = [gst.gen_groups(G.n, args.groups, 10, 200, seed=1000 + q) for q in range(64)]
     timed_q = [queries[i % 64] for i in range(args.steps)]
@@ -107,6 +112,13 @@ def main():
     plain_wall = time.perf_counter() - t1
     plain_qps = len(queries) / plain_wall
 
+    # the same queries with the automatic spider bound alone (no network), batched the same way
+    t3 = time.perf_counter()
+    sres, _ = solver.solve_batch(dg, timed_q, slots=args.slots, rounds_per_step=args.rounds_per_step, auto_bound=True)
+    spider_qps = args.steps / (time.perf_counter() - t3)
+    for i, r in enumerate(sres):
+        assert r.cost == plain[i % 64].cost, (i, r.cost, plain[i % 64].cost)
+
     # exactness: every learned-bound cost equals plain DPBF; every tree is valid at that cost
     for i, r in enumerate(res):
         want = plain[i % 64].cost
@@ -140,6 +152,17 @@ def main():
         cpu = {"value": round(len(sample) / cw, 4), "unit": "queries/s", "cores": nth, "kind": "port",
                "sample": f"{len(sample)} of the 64 queries, label-setting DPBF (oracle/dpbf_oracle.c), "
                          f"{nth} threads, {cw:.1f} s wall"}
+        # PrunedDP++-style bounded label-setting search (one-label lower bound, grow <= U/2) with
+        # the spider bound computed on the host, one thread, on 4 of the queries
+        t4 = time.perf_counter()
+        for q in queries[:4]:
+            d = oracle.group_distances(G, q)
+            c, _ = oracle.dpbf_pruned(G, q, float(np.min(d.sum(axis=0))))
+            assert c == plain[queries.index(q)].cost
+        pw = time.perf_counter() - t4
+        cpu["pruned"] = {"value": round(4 / pw, 4), "unit": "queries/s", "cores": 1, "kind": "port",
+                         "sample": f"4 of the 64 queries, bounded label-setting DPBF (oracle orc_dpbf_pruned, "
+                                   f"spider bound, one-label LB, half-bound grow), {pw:.1f} s wall"}
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "dpbf_learned_batch_ba200k_g6_traffic.json")
@@ -167,6 +190,8 @@ def main():
                    "mode": "learned-UB batched DPBF"},
         "plain_dpbf": {"value": round(plain_qps, 3), "unit": "queries/s", "mode": "plain DPBF, one query at a time"},
         "speedup_vs_plain": round(qps / plain_qps, 3),
+        "spider_bound_batched": {"value": round(spider_qps, 3), "unit": "queries/s",
+                                 "mode": "automatic spider bound only (no network), same batched engine"},
         "learned": {"frac": args.frac, "fallback_rate": round(fallback, 4),
                     "ub_over_opt_mean": round(float(np.mean(ratios)), 5) if ratios else None,
                     "relax_vs_plain": round(sum(r.stats["grow_relax"] for r in res[:64]) / max(p_relax, 1), 5)},
@@ -190,6 +215,112 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    model.free()
+    solver.close()
+
+
+def _rates(results, key_ms):
+    return sum(r.stats[key_ms] for r in results)
+
+
+def other_workload(args, gst):
+    """The other measured configurations (one JSON line each; not the headline)."""
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    solver = gst.Solver(0)
+    model = gst.Model(solver)
+    model.load(args.model)
+    out = {"n_gpus": 1, "data": "synthetic (seeded generators)", "higher_is_better": True, "vs_baseline": None}
+    if args.workload == "c4":
+        # 256 queries on the BA V=200k graph, g uniform in 3..8, group sizes uniform in 5..500
+        G = gst.gen_barabasi_albert(200_000, 5, 1, 1000, seed=1)
+        dg = solver.upload(G)
+        rng = np.random.default_rng(5)
+        qs = [gst.gen_groups(G.n, int(rng.integers(3, 9)), 5, 500, seed=5000 + q) for q in range(256)]
+        log("graph and queries ready")
+        solver.autotune(dg, [q for q in qs if q.g == 6][:8], bounded=True, batch=True)
+        solver.solve_batch(dg, qs[:16], slots=args.slots)
+        t = time.perf_counter(); pb, pst = solver.solve_batch(dg, qs, slots=args.slots); t_pb = time.perf_counter() - t
+        log(f"plain batched {len(qs) / t_pb:.1f} q/s")
+        t = time.perf_counter(); sb, sst = solver.solve_batch(dg, qs, slots=args.slots, auto_bound=True)
+        t_sb = time.perf_counter() - t
+        solver.solve_batch_learned(dg, qs[:16], model, args.frac, slots=args.slots)
+        t = time.perf_counter(); lb, lst = solver.solve_batch_learned(dg, qs, model, args.frac, slots=args.slots)
+        t_lb = time.perf_counter() - t
+        t = time.perf_counter(); single = [solver.solve(dg, q) for q in qs[:64]]; t_s = time.perf_counter() - t
+        for i in range(len(qs)):
+            assert pb[i].cost == sb[i].cost == lb[i].cost, i
+            if i < 64:
+                assert single[i].cost == pb[i].cost, i
+                assert gst.validate_tree(G, qs[i], lb[i].eu, lb[i].ev, lb[i].root) == lb[i].cost
+        out.update({"metric": "exact-cost GST queries/s, 256 mixed queries (C4)", "value": round(len(qs) / t_lb, 3),
+                    "unit": "queries/s", "dtype": "u32",
+                    "config": {"workload": "BA V=200k m=5 int32 w[1,1000], 256 queries, g in 3..8, sizes 5..500",
+                               "mode": f"learned-UB batched (top {args.frac:.0%}), {lst['slots']} slots"},
+                    "learned_batched": {"value": round(len(qs) / t_lb, 3), "p50_ms": round(lst["latency_p50_ms"], 2),
+                                        "p99_ms": round(lst["latency_p99_ms"], 2),
+                                        "fallback_rate": round(sum(r.fell_back for r in lb) / len(lb), 4)},
+                    "plain_batched": {"value": round(len(qs) / t_pb, 3), "p50_ms": round(pst["latency_p50_ms"], 2),
+                                      "p99_ms": round(pst["latency_p99_ms"], 2)},
+                    "spider_batched": {"value": round(len(qs) / t_sb, 3), "p50_ms": round(sst["latency_p50_ms"], 2),
+                                       "p99_ms": round(sst["latency_p99_ms"], 2)},
+                    "plain_one_at_a_time_first64": {"value": round(64 / t_s, 3)}})
+    else:
+        if args.workload == "rmat22":
+            # float R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05), edge factor 16, lognormal(0,1) weights, g=7
+            G = gst.gen_rmat(22, 16, weights="lognormal", seed=22)
+            g, smin, smax, nq, frac = 7, 20, 500, 3, args.frac
+            dtype, metric = "f32", "exact-cost GST queries/s, float R-MAT scale 22 g=7 (C3)"
+        else:
+            # int R-MAT scale 21, weights [1,1000], g=8 groups of 10-300, learned bound on the top 5%
+            G = gst.gen_rmat(21, 16, weights="int", wlo=1, whi=1000, seed=21)
+            g, smin, smax, nq, frac = 8, 10, 300, 8, 0.05
+            dtype, metric = "u32", "exact-cost GST queries/s, learned bound vs plain, int R-MAT scale 21 g=8 (C5)"
+        log(f"graph V={G.n} arcs={G.m}")
+        dg = solver.upload(G)
+        log("uploaded")
+        qs, k = [], 0
+        while len(qs) < nq:   # R-MAT leaves isolated vertices: keep queries that have a tree
+            q = gst.gen_groups(G.n, g, smin, smax, seed=7000 + k)
+            k += 1
+            try:
+                solver.solve(dg, q, want_tree=False, auto_bound=True)
+                qs.append(q)
+            except gst.GstError:
+                pass
+        log(f"{nq} feasible queries ({k} drawn)")
+        t = time.perf_counter(); plain = [solver.solve(dg, q) for q in qs]; t_p = time.perf_counter() - t
+        log(f"plain {nq / t_p:.3f} q/s")
+        t = time.perf_counter(); spider = [solver.solve(dg, q, auto_bound=True) for q in qs]; t_s = time.perf_counter() - t
+        t = time.perf_counter(); lone = [solver.solve_learned(dg, q, model, frac) for q in qs]; t_l1 = time.perf_counter() - t
+        t = time.perf_counter(); lb, lst = solver.solve_batch_learned(dg, qs, model, frac); t_lb = time.perf_counter() - t
+        log(f"learned single {nq / t_l1:.3f} q/s, batched {nq / t_lb:.3f} q/s")
+        rel = 1e-6 if dtype == "f32" else 0.0
+        for i, q in enumerate(qs):
+            c = plain[i].cost
+            for r in (spider[i], lone[i], lb[i]):
+                assert abs(r.cost - c) <= rel * c, (i, r.cost, c)
+            tc = gst.validate_tree(G, q, plain[i].eu, plain[i].ev, plain[i].root)
+            assert abs(tc - c) <= 1e-6 * max(c, 1.0), (i, tc, c)
+        gms, mms = _rates(plain, "t_grow_ms"), _rates(plain, "t_merge_ms")
+        gb, mb = sum(r.stats["grow_bytes"] for r in plain), sum(r.stats["merge_bytes"] for r in plain)
+        gl = sum(r.stats["grow_launches"] for r in plain)
+        ratios = [r.stats["ub"] / p.cost for r, p in zip(lone, plain) if r.stats["ub"] > 0]
+        out.update({"metric": metric, "value": round(nq / t_lb, 4), "unit": "queries/s", "dtype": dtype,
+                    "config": {"workload": f"R-MAT V={G.n} arcs={G.m} g={g} sizes {smin}-{smax}, {nq} queries",
+                               "mode": f"learned-UB batched (top {frac:.0%})"},
+                    "plain_dpbf": {"value": round(nq / t_p, 4), "ms_per_query": round(t_p / nq * 1e3, 2)},
+                    "spider_bound": {"value": round(nq / t_s, 4)},
+                    "learned_one_at_a_time": {"value": round(nq / t_l1, 4)},
+                    "learned": {"fallback_rate": round(sum(r.fell_back for r in lb) / nq, 4),
+                                "ub_over_opt_mean": round(float(np.mean(ratios)), 5) if ratios else None},
+                    "speedup_vs_plain": round(t_p / t_lb, 3),
+                    "roofline": {"bound": "hbm", "kernel": "k_grow_round (plain)", "achieved": round(gbs(gb, gms), 2),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(gb, gms) / HBM_PEAK_GBS, 5),
+                                 "launches": gl, "ms_per_launch": round(gms / max(gl, 1), 5)},
+                    "merge_roofline": {"bound": "hbm", "kernel": "k_merge (plain)", "achieved": round(gbs(mb, mms), 2),
+                                       "frac": round(gbs(mb, mms) / HBM_PEAK_GBS, 5)},
+                    "state_relax_per_s": round(sum(r.stats["grow_relax"] for r in plain) / (gms * 1e-3), 1)})
+    print(json.dumps(out), flush=True)
     model.free()
     solver.close()
 

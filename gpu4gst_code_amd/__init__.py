@@ -434,7 +434,7 @@ def _solve_learned(self, graph: DeviceGraph, groups: Groups, model: Model, topk_
 def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model: Model, topk_frac: float = 0.05, *,
                          prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                          use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
-                         time_kernels: bool = False, raise_on_error: bool = True):
+                         pull_permille: int = 0, time_kernels: bool = False, raise_on_error: bool = True):
     """Exact solves of many queries with the learned bound, pipelined through the batched engine
     (gst_solve_batch_learned).  Returns (results, batch stats); each Result's stats also carry the
     query's learned-bound fields (sub_vertices, ub, t_sub_solve_ms, ...)."""
@@ -449,6 +449,7 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
         opts[i].want_tree = 1 if want_tree else 0
         opts[i].push_lanes = push_lanes
         opts[i].half_bound = 0 if half_bound else -1
+        opts[i].pull_permille = pull_permille
     bo = _abi.gst_batch_opts()
     bo.slots = slots
     bo.rounds_per_step = rounds_per_step

@@ -41,7 +41,9 @@
 
 namespace {
 
-constexpr uint32_t MAX_SLOTS = 64;
+constexpr uint32_t MAX_SLOTS = 256;
+constexpr uint32_t PLAN_THREADS = MAX_SLOTS;          // the plan kernels run one thread per slot
+constexpr uint32_t PLAN_WAVES = PLAN_THREADS / WAVE;
 enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3 };
 
 struct SlotDev {
@@ -74,11 +76,29 @@ struct BatchArgs {
     uint32_t grid;            // workgroups a grow round may use in total
 };
 
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+// exclusive prefix sum of x over the plan workgroup (one thread per slot); tot receives the sum
+__device__ __forceinline__ uint32_t plan_scan(uint32_t x, uint32_t& tot) {
+    __shared__ uint32_t wsum[PLAN_WAVES];
+    uint32_t incl = x;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, WAVE);
+        if (lane_id() >= (uint32_t)d) incl += y;
+    }
+    const uint32_t w = threadIdx.x / WAVE;
+    if (lane_id() == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    tot = 0;
+    for (uint32_t i = 0; i < PLAN_WAVES; ++i) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();   // wsum is reused by the next call
+    return off + incl - x;
 }
 
-__global__ void __launch_bounds__(WAVE) k_plan_step(BatchArgs b) {
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
     const uint32_t s = threadIdx.x;
     bool dm = false, dc = false, dt = false;
     uint32_t lvl = 0;
@@ -100,14 +120,17 @@ __global__ void __launch_bounds__(WAVE) k_plan_step(BatchArgs b) {
         }
     }
     BatchPlan* p = b.plan;
-    const uint64_t mm = __ballot(dm), mc = __ballot(dc), mt = __ballot(dt);
-    if (dm) { const uint32_t i = lanes_below(mm); p->merge_slot[i] = s; p->merge_level[i] = lvl; }
-    if (dc) { const uint32_t i = lanes_below(mc); p->compact_slot[i] = s; p->compact_level[i] = lvl; }
-    if (dt) p->trace_slot[lanes_below(mt)] = s;
+    uint32_t nm, nc, nt;
+    const uint32_t im = plan_scan(dm ? 1u : 0u, nm);
+    const uint32_t ic = plan_scan(dc ? 1u : 0u, nc);
+    const uint32_t it = plan_scan(dt ? 1u : 0u, nt);
+    if (dm) { p->merge_slot[im] = s; p->merge_level[im] = lvl; }
+    if (dc) { p->compact_slot[ic] = s; p->compact_level[ic] = lvl; }
+    if (dt) p->trace_slot[it] = s;
     if (s == 0) {
-        p->n_merge = (uint32_t)__popcll(mm);
-        p->n_compact = (uint32_t)__popcll(mc);
-        p->n_trace = (uint32_t)__popcll(mt);
+        p->n_merge = nm;
+        p->n_compact = nc;
+        p->n_trace = nt;
     }
 }
 
@@ -126,7 +149,7 @@ __global__ void __launch_bounds__(MERGE_THREADS) k_batch_merge(BatchArgs b) {
 // after the merges: a slot whose level-2 merge found a spider (automatic bound) tightens the bound
 // of its later levels in the argument blocks (read by every later launch)
 template <bool FLT>
-__global__ void __launch_bounds__(WAVE) k_batch_bound(BatchArgs b) {
+__global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
     const BatchPlan* p = b.plan;
     const uint32_t i = threadIdx.x;
     if (i >= p->n_merge || p->merge_level[i] != 2) return;
@@ -169,7 +192,7 @@ __global__ void __launch_bounds__(WAVE) k_batch_trace(BatchArgs b) {
     trace_body(b.targs[p->trace_slot[blockIdx.x]]);
 }
 
-__global__ void __launch_bounds__(WAVE) k_plan_grow(BatchArgs b) {
+__global__ void __launch_bounds__(PLAN_THREADS) k_plan_grow(BatchArgs b) {
     const uint32_t s = threadIdx.x;
     uint32_t need = 0, lvl = 0;
     if (s < b.S) {
@@ -196,26 +219,21 @@ __global__ void __launch_bounds__(WAVE) k_plan_grow(BatchArgs b) {
             }
         }
     }
-    const uint32_t total = wave_sum32(min(need, 1u << 24));
+    uint32_t total, n_act, all;
+    plan_scan(min(need, 1u << 24), total);
     uint32_t alloc = min(need, b.grid);
     if (total > b.grid && need) alloc = max(1u, (uint32_t)((uint64_t)min(need, 1u << 24) * b.grid / total));
-    uint32_t incl = alloc;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const uint32_t y = __shfl_up(incl, d, WAVE);
-        if (lane_id() >= (uint32_t)d) incl += y;
-    }
+    const uint32_t start = plan_scan(alloc, all);
+    const uint32_t rank = plan_scan(need != 0 ? 1u : 0u, n_act);
     BatchPlan* p = b.plan;
-    const uint64_t act = __ballot(need != 0);
     if (need) {
-        const uint32_t i = lanes_below(act);
-        p->grow_slot[i] = s;
-        p->grow_level[i] = lvl;
-        p->grow_start[i] = incl - alloc;
+        p->grow_slot[rank] = s;
+        p->grow_level[rank] = lvl;
+        p->grow_start[rank] = start;
     }
-    if (s == WAVE - 1) {
-        p->n_grow = (uint32_t)__popcll(act);
-        p->grow_start[__popcll(act)] = incl;
+    if (s == 0) {
+        p->n_grow = n_act;
+        p->grow_start[n_act] = all;
         p->seq = p->seq + 1;
     }
 }
@@ -451,7 +469,7 @@ static gst_status ensure_hub_items(gst_ctx* ctx, gst_batch_engine* E, const gst_
 // Enqueue the set-up of one query in slot `si`: group bits, level 1, its frontier, and the
 // argument blocks of every level.
 static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch_job& job, uint32_t chunk,
-                            uint32_t tuned_lanes) {
+                            uint32_t tuned_lanes, int32_t tuned_pull) {
     hipStream_t s = (hipStream_t)ctx->stream;
     auto& sl = E->slots[(size_t)si];
     const gst_graph* G = job.G;
@@ -522,7 +540,14 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.all_items = persistent ? E->all_items : nullptr;
         a.n_all = persistent ? E->n_all : 0u;
         a.n = n;
-        a.pull_thresh = persistent ? (uint32_t)std::max<uint64_t>(1, E->n_items_total * 450ull / 1000ull) : 0xFFFFFFFFu;
+        {
+            // the job's pull threshold (0: the tuned or built-in 450 per mille, < 0: never pull)
+            const int32_t pm = job.opts.pull_permille != 0 ? job.opts.pull_permille
+                                                           : (tuned_pull != 0 ? tuned_pull : 450);
+            a.pull_thresh = (persistent && pm > 0)
+                                ? (uint32_t)std::max<uint64_t>(1, E->n_items_total * (uint64_t)pm / 1000ull)
+                                : 0xFFFFFFFFu;
+        }
         a.rlog = nullptr;
         a.seq = 0;
         a.push_lanes = job.opts.push_lanes > 0 ? (uint32_t)job.opts.push_lanes
@@ -577,17 +602,17 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
     const int mgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 8);
     // one LDS size for every (g, level) a slot may merge: tile rows, bitmaps and split table
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + MERGE_PAIRS_LDS;
-    k_plan_step<<<1, WAVE, 0, s>>>(ba);
+    k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
     if (ev) hipEventRecord(ev[0], s);
     if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
     else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
     if (ev) hipEventRecord(ev[1], s);
-    if (flt) k_batch_bound<true><<<1, WAVE, 0, s>>>(ba);
-    else k_batch_bound<false><<<1, WAVE, 0, s>>>(ba);
+    if (flt) k_batch_bound<true><<<1, PLAN_THREADS, 0, s>>>(ba);
+    else k_batch_bound<false><<<1, PLAN_THREADS, 0, s>>>(ba);
     k_batch_compact<<<mgrid, 256, 0, s>>>(ba);
     k_batch_trace<<<MAX_SLOTS, WAVE, 0, s>>>(ba);
     for (int r = 0; r < R; ++r) {
-        k_plan_grow<<<1, WAVE, 0, s>>>(ba);
+        k_plan_grow<<<1, PLAN_THREADS, 0, s>>>(ba);
         if (ev) hipEventRecord(ev[2 + 2 * r], s);
         if (flt) k_batch_grow<true><<<ba.grid + MAX_SLOTS, GROW_BLOCK, 0, s>>>(ba);
         else k_batch_grow<false><<<ba.grid + MAX_SLOTS, GROW_BLOCK, 0, s>>>(ba);
@@ -627,6 +652,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     const gst_tuned_batch* tb = tune_uid ? gst_find_tuned_batch(ctx, tune_uid) : nullptr;
     const int R = bo.rounds_per_step > 0 ? bo.rounds_per_step : (tb && tb->rounds_per_step > 0 ? tb->rounds_per_step : 4);
     const uint32_t tuned_lanes = tb ? (uint32_t)tb->push_lanes : 0u;
+    const int32_t tuned_pull = tb ? tb->pull_permille : 0;
     const bool timed = bo.time_kernels != 0;
     const bool use_graph = bo.use_graph != 0 && !timed;
     std::vector<hipEvent_t> tev;
@@ -673,7 +699,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
             auto& sl = E->slots[(size_t)si];
             sl.job = job;
             sl.job.t_fill = now_ms();
-            gst_status fs = fill_slot(ctx, E, si, sl.job, chunk, tuned_lanes);
+            gst_status fs = fill_slot(ctx, E, si, sl.job, chunk, tuned_lanes, tuned_pull);
             if (fs != GST_OK) return fs;
             busy++;
             return GST_OK;

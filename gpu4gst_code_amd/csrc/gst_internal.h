@@ -120,7 +120,7 @@ struct gst_tuned {
 };
 struct gst_tuned_batch {
     uint64_t uid = 0;
-    int32_t rounds_per_step = 0, push_lanes = 0;
+    int32_t rounds_per_step = 0, push_lanes = 0, pull_permille = 0;
 };
 
 struct gst_ctx {
