@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_grow(BatchArgs b) {
 // one grow round of every growing slot; the grid is b.grid + MAX_SLOTS workgroups (the plan's
 // per-slot minimum of one workgroup can exceed b.grid by at most one per slot)
 template <bool FLT>
-__global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_batch_grow(BatchArgs b) {
+__global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu(GST_GROW_WPE))) k_batch_grow(BatchArgs b) {
     const BatchPlan* p = b.plan;
     const uint32_t ng = p->n_grow;
     const uint32_t bid = blockIdx.x;

@@ -299,7 +299,13 @@ __device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, uint32_t v, uin
     return lb;
 }
 
-constexpr uint32_t UNR = 4;   // arc steps issued together (independent gathers in flight per wave)
+#ifndef GST_GROW_UNR
+#define GST_GROW_UNR 4
+#endif
+#ifndef GST_GROW_WPE
+#define GST_GROW_WPE 5
+#endif
+constexpr uint32_t UNR = GST_GROW_UNR;   // arc steps issued together (independent gathers in flight per wave)
 // Rows longer than this are "hubs": pull rounds split them into chunk items combined with atomics;
 // shorter rows are pulled whole by one lane group with plain stores.
 constexpr uint32_t HUB_DEG = 128;
@@ -817,7 +823,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
 
 // 5 workgroups of 4 waves per CU must fit at once (the grid is sized for it): at most 96 VGPRs
 template <bool FLT>
-__global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_grow_round(GrowArgs a) {
+__global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu(GST_GROW_WPE))) k_grow_round(GrowArgs a) {
     grow_round_body<FLT>(a, blockIdx.x, gridDim.x, a.seq);
 }
 
