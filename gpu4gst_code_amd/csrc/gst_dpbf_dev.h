@@ -1201,11 +1201,19 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     uint32_t* rows = (uint32_t*)lds;                                            // VB * L costs
     unsigned long long* bm = lds + merge_rows_words(VB, L);                     // VB * W
     uint32_t* gms = (uint32_t*)(bm + (size_t)VB * W);                           // VB
+    // split table: in LDS transposed to [split][mask], so that the lanes of one wave (consecutive
+    // masks) read consecutive entries; else the global [mask][split] table
     const unsigned long long* pt = (const unsigned long long*)a.pairs;
+    uint32_t pr_stride = 1, ps_stride = S;   // entry (r, s2) at pt[r * ps_stride + s2 * pr_stride]
     if (a.pairs_in_lds) {
         unsigned long long* pl = (unsigned long long*)(gms + ((VB + 1) & ~1u));
-        for (uint32_t i = threadIdx.x; i < a.P; i += blockDim.x) pl[i] = a.pairs[i];
+        for (uint32_t i = threadIdx.x; i < a.P; i += blockDim.x) {
+            const uint32_t r = i / S, s2 = i - r * S;
+            pl[s2 * mk + r] = a.pairs[i];
+        }
         pt = pl;
+        pr_stride = mk;
+        ps_stride = 1;
     }
     const bool prune = a.ub != GST_INF_COST;
     const bool prune_lb = prune && a.use_lb && !a.is_full;
@@ -1267,9 +1275,9 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             const uint32_t gmv = gms[b];
             const uint32_t* rv = rows + (size_t)b * L;
             unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
-            const unsigned long long* pr = pt + (size_t)r * S;
+            const unsigned long long* pr = pt + (size_t)r * ps_stride;
             for (uint32_t s2 = 0; s2 < S; ++s2) {
-                const unsigned long long e = pr[s2];
+                const unsigned long long e = pr[(size_t)s2 * pr_stride];
                 const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
                 const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
                 const uint32_t c1 = rv[p1];
@@ -1292,13 +1300,25 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                 if (reaches_ub<FLT>((uint32_t)(best >> 32), lb, a.ub)) best = GST_INF_WORD;
             }
             ((uint64_t*)Tk_base)[(uint64_t)v0 * mk + i] = best;
+            bool flag = false;
             if (best != GST_INF_WORD) {
                 if (a.is_full) {
                     const unsigned long long rw = ((best >> 32) << 32) | v;
                     best_root = rw < best_root ? rw : best_root;
-                } else if ((uint32_t)(best >> 32) <= a.gmax) {
-                    atomicOr(&bm[b * W + (r >> 6)], 1ull << (r & 63));
+                } else {
+                    flag = (uint32_t)(best >> 32) <= a.gmax;
                 }
+            }
+            // frontier bits: lanes i, i+1, ... of one wave hold consecutive masks, so the bits of
+            // one (vertex, bitmap word) are a contiguous run of the wave's ballot; the first lane of
+            // each run ORs the run into the LDS bitmap (one LDS atomic per run, not per mask)
+            const uint64_t bal = __ballot(flag);
+            const uint32_t lane = lane_id();
+            const bool leader = lane == 0 || (r & 63u) == 0;
+            if (leader && bal) {
+                const uint32_t cnt = min(min(64u - (r & 63u), mk - r), 64u - lane);
+                const uint64_t run = (bal >> lane) & (cnt == 64u ? ~0ull : ((1ull << cnt) - 1ull));
+                if (run) atomicOr(&bm[b * W + (r >> 6)], (unsigned long long)(run << (r & 63u)));
             }
         }
         st_pairs += (uint64_t)items * S;
