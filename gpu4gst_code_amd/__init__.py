@@ -488,7 +488,8 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
     return out, stats
 
 
-def _autotune(self, graph: DeviceGraph, sample: List[Groups], *, bounded: bool = False, batch: bool = False) -> dict:
+def _autotune(self, graph: DeviceGraph, sample: List[Groups], *, bounded: bool = False, batch: bool = False,
+              eager: bool = False) -> dict:
     """Pick the launch knobs for this graph from a sample of queries (gst_autotune); later solves
     that leave a knob at 0 use the choice.  Returns the tuning report as a dict."""
     nq = len(sample)
@@ -496,7 +497,8 @@ def _autotune(self, graph: DeviceGraph, sample: List[Groups], *, bounded: bool =
     keep = [q.as_c() for q in sample]
     for i, c in enumerate(keep):
         grps[i] = c
-    flags = (_abi.GST_TUNE_BOUNDED if bounded else 0) | (_abi.GST_TUNE_BATCH if batch else 0)
+    flags = (_abi.GST_TUNE_BOUNDED if bounded else 0) | (_abi.GST_TUNE_BATCH if batch else 0) | \
+        (_abi.GST_TUNE_EAGER if eager else 0)
     rep = _abi.gst_tune_report()
     _check(self.lib, self.lib.gst_autotune(self.ctx, graph.handle, nq, grps, flags, C.byref(rep)), self.ctx)
     return {name: getattr(rep, name) for name, _ in _abi.gst_tune_report._fields_ if name != "pad"}

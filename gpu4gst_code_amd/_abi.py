@@ -85,6 +85,7 @@ class gst_tune_report(C.Structure):
 
 GST_TUNE_BOUNDED = 1
 GST_TUNE_BATCH = 2
+GST_TUNE_EAGER = 4
 
 
 class gst_host_csr(C.Structure):

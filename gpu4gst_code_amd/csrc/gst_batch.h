@@ -11,7 +11,7 @@ struct gst_batch_job {
     const gst_graph* G = nullptr;
     const gst_groups* groups = nullptr;
     gst_opts opts{};             // ub, prune_lb and want_tree are honoured
-    bool temp_graph = false;     // graph made for this job only: push rounds only (no hub item list)
+    bool temp_graph = false;     // graph made for this job only: its hub row items are built in the slot
     gst_result* res = nullptr;   // filled when the job finishes
     gst_status status = GST_OK;
     double t_fill = 0.0;         // host ms when the job entered a slot

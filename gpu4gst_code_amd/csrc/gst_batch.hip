@@ -212,7 +212,8 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_grow(BatchArgs b) {
                 const uint32_t mch = a.m < (uint32_t)WAVE ? a.m : (uint32_t)WAVE;
                 const uint32_t rows_per_wave = (uint32_t)WAVE / mch;
                 const uint32_t items_per_wave = (uint32_t)WAVE / a.push_lanes;
-                const uint32_t waves = pull ? (a.n + rows_per_wave - 1) / rows_per_wave + a.n_all
+                const uint32_t n_all = a.n_all_dev ? *a.n_all_dev : a.n_all;
+                const uint32_t waves = pull ? (a.n + rows_per_wave - 1) / rows_per_wave + n_all
                                             : (n_cur + items_per_wave - 1) / items_per_wave;
                 need = max(1u, max((waves + GROW_WAVES - 1) / GROW_WAVES, (n_prv + GROW_BLOCK - 1) / GROW_BLOCK));
                 lvl = sd.k;
@@ -283,6 +284,8 @@ struct gst_batch_engine {
         unsigned long long* stats = nullptr;
         uint32_t* edges = nullptr;          // 3 * ecap + 1
         uint32_t* spider = nullptr;         // cheapest spider of the level-2 merge (automatic bound)
+        uint64_t* hub_items = nullptr;      // hub row items of a job's own graph (pull rounds)
+        uint32_t* hub_cnt = nullptr;
         int32_t* grp = nullptr;
         size_t grp_cap = 0;
         // pinned host staging (args written at fill) and readback
@@ -336,6 +339,7 @@ static void free_slot(gst_batch_engine::Slot& sl) {
     hipFree(sl.T); hipFree(sl.gm);
     for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
     hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp); hipFree(sl.spider);
+    hipFree(sl.hub_items); hipFree(sl.hub_cnt);
     if (sl.h_grp) hipHostFree(sl.h_grp);
     if (sl.h_gargs) hipHostFree(sl.h_gargs);
     if (sl.h_margs) hipHostFree(sl.h_margs);
@@ -400,7 +404,8 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
         for (int j = 0; ok && j < 3; ++j)
             ok = dalloc(sl.chg[j], (size_t)n * W) && dalloc(sl.inq[j], n) && dalloc(sl.lists[j], items_cap);
         ok = ok && dalloc(sl.ctl, NCTL) && dalloc(sl.root, 1) && dalloc(sl.stats, ST_NSTAT) &&
-             dalloc(sl.edges, 3 * (size_t)ecap + 1) && dalloc(sl.spider, 1);
+             dalloc(sl.edges, 3 * (size_t)ecap + 1) && dalloc(sl.spider, 1) &&
+             dalloc(sl.hub_items, items_cap) && dalloc(sl.hub_cnt, 1);
         ok = ok && hipHostMalloc((void**)&sl.h_gargs, sizeof(GrowArgs) * NCTL) == hipSuccess &&
              hipHostMalloc((void**)&sl.h_margs, sizeof(MergeArgs) * NCTL) == hipSuccess &&
              hipHostMalloc((void**)&sl.h_targs, sizeof(TraceArgs)) == hipSuccess &&
@@ -517,6 +522,11 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         k_compact<<<fg, 256, 0, s>>>(sl.inq[0], G->d_row, n, chunk, sl.lists[0], &sl.ctl[1].cnt[0], sl.stats, 0u);
     else
         k_root_scan_l1<<<fg, 256, 0, s>>>(sl.T, n, sl.root);
+    if (!persistent) {
+        // a job's own graph gets its hub row items on the device, for pull rounds on dense frontiers
+        k_fill_u32<<<1, 64, 0, s>>>(sl.hub_cnt, 0u, 1);
+        k_compact<<<fg, 256, 0, s>>>(nullptr, G->d_row, n, chunk, sl.hub_items, sl.hub_cnt, nullptr, HUB_DEG);
+    }
 
     // argument blocks
     for (int k = 1; k < g; ++k) {
@@ -537,16 +547,17 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.gmax = bb.gmax;
         a.lbcap = bb.lbcap;
         a.use_lb = (job.opts.prune_lb && k >= 2) ? 1u : 0u;
-        a.all_items = persistent ? E->all_items : nullptr;
+        a.all_items = persistent ? E->all_items : sl.hub_items;
         a.n_all = persistent ? E->n_all : 0u;
+        a.n_all_dev = persistent ? nullptr : sl.hub_cnt;
         a.n = n;
         {
             // the job's pull threshold (0: the tuned or built-in 450 per mille, < 0: never pull)
             const int32_t pm = job.opts.pull_permille != 0 ? job.opts.pull_permille
                                                            : (tuned_pull != 0 ? tuned_pull : 450);
-            a.pull_thresh = (persistent && pm > 0)
-                                ? (uint32_t)std::max<uint64_t>(1, E->n_items_total * (uint64_t)pm / 1000ull)
-                                : 0xFFFFFFFFu;
+            // a job's own graph: its work items are at most n + m / chunk
+            const uint64_t items_total = persistent ? E->n_items_total : (uint64_t)n + (uint64_t)G->m / chunk;
+            a.pull_thresh = pm > 0 ? (uint32_t)std::max<uint64_t>(1, items_total * (uint64_t)pm / 1000ull) : 0xFFFFFFFFu;
         }
         a.rlog = nullptr;
         a.seq = 0;

@@ -450,7 +450,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         // the one-label lower bound reads the group distances of level 1, which are final only
         // after level 1 has converged: never prune with it inside level 1
         ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;
-        ga.all_items = B->all_items; ga.n_all = B->n_all; ga.n = n;
+        ga.all_items = B->all_items; ga.n_all = B->n_all; ga.n_all_dev = nullptr; ga.n = n;
         {
             const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
             ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu

@@ -57,6 +57,7 @@ struct GrowArgs {
     uint32_t use_lb;
     const uint64_t* all_items;  // (vertex, chunk) items of the hub rows, for pull rounds
     uint32_t n_all;
+    const uint32_t* n_all_dev;  // when set, the hub item count is read here (built on the device)
     uint32_t n;            // vertices
     uint32_t pull_thresh;  // a round with at least this many queued items pulls over all items
     // asynchronous (work-queue) mode
@@ -553,7 +554,8 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     }
 
     // ---- pull round, hub rows: (row, chunk) items, lanes = (arc slot, mask), combined by atomics ----
-    for (uint32_t i = gwave; pull && i < a.n_all; i += nwaves) {
+    const uint32_t n_all = pull ? (a.n_all_dev ? *a.n_all_dev : a.n_all) : 0u;
+    for (uint32_t i = gwave; pull && i < n_all; i += nwaves) {
         const uint64_t it = a.all_items[i];
         const uint32_t u = (uint32_t)it;
         const uint32_t c = (uint32_t)(it >> 32);

@@ -36,6 +36,7 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     if (rep) std::memset(rep, 0, sizeof(*rep));
     const bool bounded = (flags & GST_TUNE_BOUNDED) != 0;
     const bool batch = (flags & GST_TUNE_BATCH) != 0;
+    const bool eager = (flags & GST_TUNE_EAGER) != 0;
     int g = sample[0].n_groups;
     for (int32_t q = 1; q < nq; ++q)
         if (sample[q].n_groups != g) return gst_set_error(ctx, GST_INVALID_ARG, "autotune: the sample must share one group count");
@@ -133,7 +134,7 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             for (auto& o : bo) o.push_lanes = lanes;
             gst_batch_opts bopt{};
             bopt.rounds_per_step = R;
-            bopt.use_graph = 1;
+            bopt.use_graph = eager ? 0 : 1;
             double best_q = 0.0;
             for (int rep2 = 0; rep2 < 2; ++rep2) {   // the first run builds the captured step
                 int32_t next = 0;

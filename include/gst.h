@@ -272,6 +272,7 @@ gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq,
  * GST_TUNE_BOUNDED tunes bounded solves (bound = 1.05 x each sample query's optimum). */
 #define GST_TUNE_BOUNDED 1
 #define GST_TUNE_BATCH   2
+#define GST_TUNE_EAGER   4   /* time the engine launching kernel by kernel (as with time_kernels), not by graph replay */
 typedef struct {
     int32_t chunk_arcs, pull_permille, push_lanes, merge_tile_kb;   /* chosen for one-query solves */
     int32_t batch_rounds_per_step, batch_push_lanes;                /* chosen for the engine (GST_TUNE_BATCH) */
