@@ -297,7 +297,8 @@ class Solver:
     def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                     use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
-                    auto_bound: bool = False, time_kernels: bool = False, raise_on_error: bool = True):
+                    auto_bound: bool = False, pull_permille: int = 0, time_kernels: bool = False,
+                    raise_on_error: bool = True):
         """Solve many queries on one graph with the batched engine (gst_solve_batch).
 
         Returns (results, stats): results[i] is a Result, or a GstError for a query that failed
@@ -316,6 +317,7 @@ class Solver:
             opts[i].push_lanes = push_lanes
             opts[i].half_bound = 0 if half_bound else -1
             opts[i].auto_bound = 1 if auto_bound else 0
+            opts[i].pull_permille = pull_permille
         bo = _abi.gst_batch_opts()
         bo.slots = slots
         bo.rounds_per_step = rounds_per_step
