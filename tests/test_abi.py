@@ -113,3 +113,47 @@ def test_gstp_roundtrip(tmp_path):
     assert np.array_equal(G.row_ptr, G2.row_ptr) and np.array_equal(G.col, G2.col) and np.array_equal(G.w, G2.w)
     for i in range(3):
         assert sorted(groups.members(i).tolist()) == groups2.members(i).tolist()
+
+
+def test_dataset_triple_round_trip(tmp_path):
+    """The reference harness's .in / .g / .g7 files (rucgraph/test_gpu.h:115-121): write, read back,
+    and the queries' optima from the files equal those of the original groups."""
+    import oracle
+    from gpu4gst_code_amd import gstp
+    G = gst.gen_random_connected(300, 1200, 1, 50, seed=9)
+    rng = np.random.default_rng(3)
+    table = {i: sorted(set(rng.choice(G.n, size=int(rng.integers(2, 9)), replace=False).tolist())) for i in range(12)}
+    queries = [sorted(rng.choice(12, size=4, replace=False).tolist()) for _ in range(5)]
+    prefix = str(tmp_path / "toy")
+    gstp.write_dataset(prefix, G, table, queries, communities=[list(range(150)), list(range(150, 300))])
+    G2, comm = gstp.read_dataset_graph(prefix + ".in")
+    assert G2.n == G.n and G2.m == G.m
+    assert np.array_equal(G2.row_ptr, G.row_ptr) and np.array_equal(G2.col, G.col) and np.array_equal(G2.w, G.w)
+    assert [len(c) for c in comm] == [150, 150]
+    t2 = gstp.read_dataset_groups(prefix + ".g")
+    assert t2 == table
+    q2 = gstp.read_dataset_queries(prefix + ".g7", G2.n)
+    assert q2 == queries
+    for q in q2:
+        a = gstp.dataset_query(t2, q)
+        b = gst.groups_from_lists([table[i] for i in q])
+        assert oracle.dpbf_cost(G2, a) == oracle.dpbf_cost(G, b)
+
+
+def test_dataset_reader_reference_lines(tmp_path):
+    """Lines as the reference writes them: header "V E C", community field, g<id>: rows, query
+    lines ending in a field that is not a group."""
+    from gpu4gst_code_amd import gstp
+    (tmp_path / "d.in").write_text("4 3 2\ninput_graph Vertex 0 0 0\ninput_graph Vertex 1 0 0\n"
+                                   "input_graph Vertex 2 0 1\ninput_graph Vertex 3 0 1\n"
+                                   "input_graph Edge 0 1 5\ninput_graph Edge 1 2 7\ninput_graph Edge 2 3 2\n")
+    (tmp_path / "d.g").write_text("g1:0 3\ng2:2\ng3:1 2\n")
+    (tmp_path / "d.g7").write_text("4 5 6 \n6 4 \n")
+    G, comm = gstp.read_dataset_graph(str(tmp_path / "d.in"))
+    assert G.n == 4 and G.m == 6 and comm == [[0, 1], [2, 3]]
+    t = gstp.read_dataset_groups(str(tmp_path / "d.g"))
+    assert t == {0: [0, 3], 1: [2], 2: [1, 2]}
+    q = gstp.read_dataset_queries(str(tmp_path / "d.g7"), G.n)
+    assert q == [[0, 1, 2], [2, 0]]
+    grp = gstp.dataset_query(t, q[0])
+    assert grp.g == 3 and list(grp.members(0)) == [0, 3]
