@@ -50,6 +50,8 @@ def load():
     lib.orc_dpbf_pruned.restype = C.c_int
     lib.orc_dpbf_pruned.argtypes = [C.c_int32, P, P, P, C.c_int32, P, C.c_double, C.c_double, C.c_int32, P, P, P]
     lib.orc_dpbf_batch.restype = C.c_int
+    lib.orc_state_bounds.restype = C.c_int
+    lib.orc_state_bounds.argtypes = [C.c_int32, C.c_int32, P, P, C.c_int32, P]
     lib.orc_dpbf_batch.argtypes = [C.c_int32, C.c_int32, P, P, P, C.c_int32, P, P, P]
     _lib = lib
     return lib
@@ -100,21 +102,36 @@ def dpbf_cost(graph, groups) -> float:
     return dpbf(graph, groups)[0]
 
 
-def dpbf_pruned(graph, groups, ub: float, grow_frac: float = 0.5, use_lb: bool = True):
+def dpbf_pruned(graph, groups, ub: float, grow_frac: float = 0.5, use_lb=True):
     """Label-setting DPBF under a feasible cost ub (PrunedDP++ bounds): only states of cost
-    <= grow_frac * ub are grown (0: all).  Returns (cost, stats)."""
+    <= grow_frac * ub are grown (0: all); use_lb: False, True / 1 (one-label), 2 (one-label and
+    tour bounds), 3 (tour bounds with a planted mistake, for the tests).  Returns (cost, stats)."""
     lib = load()
     row, col, w = _csr(graph)
     n = row.shape[0] - 1
     gm = group_masks(n, groups)
+    use_lb = int(use_lb)
     dist = np.ascontiguousarray(group_distances(graph, groups)) if use_lb else np.zeros(1)
     cost = C.c_double(0)
     stats = np.zeros(3, np.int64)
     rc = lib.orc_dpbf_pruned(n, _p(row), _p(col), _p(w), groups.g, _p(gm), float(ub), float(grow_frac),
-                             1 if use_lb else 0, _p(dist), C.addressof(cost), _p(stats))
+                             use_lb, _p(dist), C.addressof(cost), _p(stats))
     if rc not in (0, 1):
         raise RuntimeError(f"orc_dpbf_pruned failed with code {rc}")
     return cost.value, {"pops": int(stats[0]), "relax": int(stats[1]), "merges": int(stats[2])}
+
+
+def state_bounds(graph, groups, use_lb: int = 2) -> np.ndarray:
+    """lb[v, p]: the lower bound that dpbf_pruned(use_lb=...) applies to state (v, p)."""
+    lib = load()
+    n = graph.row_ptr.shape[0] - 1
+    gm = group_masks(n, groups)
+    dist = np.ascontiguousarray(group_distances(graph, groups))
+    out = np.empty((n, 1 << groups.g), np.float64)
+    rc = lib.orc_state_bounds(n, groups.g, _p(gm), _p(dist), int(use_lb), _p(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_state_bounds failed with code {rc}")
+    return out
 
 
 def dijkstra(graph, init: np.ndarray) -> np.ndarray:

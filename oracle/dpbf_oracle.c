@@ -24,12 +24,22 @@
  *                    (rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:497-964):
  *                      a state is kept only while cost + LB(v,p) <= U, LB the one-label bound
  *                      max over groups i not in p of dist(v, G_i)                      (:471-488)
+ *                      and optionally the tour bounds lb1/lb2                          (:435-494)
+ *                      over a walk table W built from group-to-group distances    (:268-385, 565-584)
  *                      a popped state is grown only while cost <= U/2 (optional)       (:795)
- *                    it must return the same optimum as orc_dpbf whenever U >= OPT.
+ *                    it must return the same optimum as orc_dpbf whenever U >= OPT.  The walk
+ *                    table is a Held-Karp DP over the metric closure of the group distances
+ *                    instead of the reference's heap search over (i, j, set) with subset
+ *                    propagation; both give lower bounds of the same walks, so only the amount
+ *                    of pruning can differ, never the optimum.
+ *   orc_state_bounds the bound of every state, for the per-state tests.
  *
  * Pinning (tests/test_oracle.py): brute force on tiny graphs, the g=1 (cost 0) and g=2
  * (group-to-group shortest path) special cases, the g=3 spider and g=4 two-centre closed forms
- * built from orc_dijkstra, and trees re-validated edge by edge.
+ * built from orc_dijkstra, and trees re-validated edge by edge.  The bounds: every state's bound
+ * is at most the brute-force cost of the cheapest tree holding v and touching the missing groups,
+ * the bounded search keeps the brute-force optimum, and a planted max-for-min mistake in lb1
+ * loses it on some instance.
  */
 #include <math.h>
 #include <pthread.h>
@@ -107,12 +117,123 @@ static int64_t heap_pop(heap_t* H) {
     return top;
 }
 
+/*
+ * Tour bounds of PrunedDP++ (rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:435-494, 268-385,
+ * 565-584).  D[a][b] = distance between groups a and b = min over members u of G_b of dist(u, G_a);
+ * Dc = its closure over intermediate groups (Floyd-Warshall), so that walks through extra groups
+ * never beat the table; W[S][i][j] = shortest walk from group i to group j through every group of S
+ * (Held-Karp over Dc; W[S][i][i] closes the walk).  For a state (v, X) with uncovered set U:
+ *   lb1 = min over i, j in U of (d(v,i) + W[U][i][j] + d(v,j)) / 2     (a tour from v around the
+ *         rest of the tree passes every group of U and is at most twice its weight)
+ *   lb2 = max over i in U of (d(v,i) + min_j W[U][i][j] + min_j d(v,j)) / 2
+ * and the bound is max(lb1, lb2, one-label).  tabs: W of 2^g * g * g doubles.
+ */
+static void orc_tour_tables(int32_t g, const double* D, double* W) {
+    double Dc[20 * 20];
+    for (int32_t a = 0; a < g * g; ++a) Dc[a] = D[a];
+    for (int32_t k = 0; k < g; ++k)
+        for (int32_t a = 0; a < g; ++a)
+            for (int32_t b = 0; b < g; ++b)
+                if (Dc[a * g + k] + Dc[k * g + b] < Dc[a * g + b]) Dc[a * g + b] = Dc[a * g + k] + Dc[k * g + b];
+    const int64_t S = (int64_t)1 << g;
+    for (int64_t x = 0; x < S * g * g; ++x) W[x] = INFINITY;
+    for (int32_t i = 0; i < g; ++i) W[((int64_t)1 << i) * g * g + i * g + i] = 0.0;
+    for (int64_t m = 1; m < S; ++m) {
+        /* open walks i -> j (i != j) through all of m: the last step enters j from k */
+        for (int32_t i = 0; i < g; ++i) {
+            if (!((m >> i) & 1)) continue;
+            for (int32_t j = 0; j < g; ++j) {
+                if (j == i || !((m >> j) & 1)) continue;
+                const int64_t prev = m ^ ((int64_t)1 << j);
+                double best = INFINITY;
+                for (int32_t k = 0; k < g; ++k) {
+                    if (!((prev >> k) & 1)) continue;
+                    const double c = W[prev * g * g + i * g + k] + Dc[k * g + j];
+                    if (c < best) best = c;
+                }
+                W[m * g * g + i * g + j] = best;
+            }
+        }
+        /* closed walks i -> i through all of m (at least two groups) */
+        for (int32_t i = 0; i < g; ++i) {
+            if (!((m >> i) & 1) || m == ((int64_t)1 << i)) continue;
+            double best = INFINITY;
+            for (int32_t k = 0; k < g; ++k) {
+                if (k == i || !((m >> k) & 1)) continue;
+                const double c = W[m * g * g + i * g + k] + Dc[k * g + i];
+                if (c < best) best = c;
+            }
+            W[m * g * g + i * g + i] = best;
+        }
+    }
+}
+
+static double orc_tour_lb(const double* dist, const double* W, int32_t n, int32_t g, int32_t v, uint32_t p,
+                          int32_t planted) {
+    const uint32_t full = ((uint32_t)1 << g) - 1, U = full ^ p;
+    if (!U) return 0.0;
+    double lb1 = INFINITY, lb2 = 0.0, dmin = INFINITY;
+    for (int32_t j = 0; j < g; ++j)
+        if ((U >> j) & 1u && dist[(int64_t)j * n + v] < dmin) dmin = dist[(int64_t)j * n + v];
+    for (int32_t i = 0; i < g; ++i) {
+        if (!((U >> i) & 1u)) continue;
+        const double di = dist[(int64_t)i * n + v];
+        double w2 = INFINITY;
+        for (int32_t j = 0; j < g; ++j) {
+            if (!((U >> j) & 1u)) continue;
+            const double w = W[(int64_t)U * g * g + i * g + j];
+            if (w < w2) w2 = w;
+            const double c = (di + w + dist[(int64_t)j * n + v]) / 2.0;
+            /* planted mistake for the tests: max instead of min over the tour's end groups */
+            if (planted ? (lb1 == INFINITY || c > lb1) : (c < lb1)) lb1 = c;
+        }
+        const double c2 = (di + w2 + dmin) / 2.0;
+        if (c2 > lb2) lb2 = c2;
+    }
+    return lb1 > lb2 ? lb1 : lb2;
+}
+
 /* max over groups i not in p of dist[i][v] (dist: g x n) */
 static double orc_label_lb(const double* dist, int32_t n, int32_t g, int32_t v, uint32_t p) {
     double lb = 0.0;
     for (int32_t i = 0; i < g; ++i)
         if (!((p >> i) & 1u) && dist[(int64_t)i * n + v] > lb) lb = dist[(int64_t)i * n + v];
     return lb;
+}
+
+/* use_lb: 1 one-label, 2 also the tour bounds, 3 the tour bounds with a planted mistake (tests) */
+static double orc_lb(const double* dist, const double* W, int32_t n, int32_t g, int32_t v, uint32_t p, int32_t use_lb) {
+    double lb = orc_label_lb(dist, n, g, v, p);
+    if (use_lb >= 2) {
+        const double t = orc_tour_lb(dist, W, n, g, v, p, use_lb == 3);
+        if (t > lb) lb = t;
+    }
+    return lb;
+}
+
+/* The bound of every state (v, p) as orc_dpbf_pruned applies it, into out[v * 2^g + p]
+ * (tests compare each with the cost of the cheapest tree that holds v and touches the missing groups). */
+int orc_state_bounds(int32_t n, int32_t g, const uint32_t* gm, const double* dist, int32_t use_lb, double* out) {
+    if (n < 1 || g < 1 || g > 12 || !dist || use_lb < 1 || use_lb > 3) return ORC_INVALID;
+    const int64_t S = (int64_t)1 << g;
+    double* W = NULL;
+    if (use_lb >= 2) {
+        W = (double*)malloc(sizeof(double) * (size_t)(S * g * g));
+        if (!W) return ORC_OOM;
+        double D[20 * 20];
+        for (int32_t a = 0; a < g; ++a)
+            for (int32_t b = 0; b < g; ++b) {
+                double m = INFINITY;
+                for (int32_t u = 0; u < n; ++u)
+                    if (((gm[u] >> b) & 1u) && dist[(int64_t)a * n + u] < m) m = dist[(int64_t)a * n + u];
+                D[a * g + b] = m;
+            }
+        orc_tour_tables(g, D, W);
+    }
+    for (int32_t v = 0; v < n; ++v)
+        for (int64_t p = 0; p < S; ++p) out[(int64_t)v * S + p] = orc_lb(dist, W, n, g, v, (uint32_t)p, use_lb);
+    free(W);
+    return ORC_OK;
 }
 
 /* ---------------------------------------------------------------- DPBF */
@@ -241,22 +362,34 @@ int orc_dpbf(int32_t n, const int64_t* row, const int32_t* col, const double* w,
 int orc_dpbf_pruned(int32_t n, const int64_t* row, const int32_t* col, const double* w, int32_t g,
                     const uint32_t* gm, double U, double grow_frac, int32_t use_lb, const double* dist,
                     double* cost_out, int64_t* stats) {
-    if (n < 1 || g < 1 || g > 20 || (use_lb && !dist)) return ORC_INVALID;
+    if (n < 1 || g < 1 || g > 12 || (use_lb && !dist)) return ORC_INVALID;
     const int64_t S = (int64_t)1 << g;
     const uint32_t full = (uint32_t)(S - 1);
     const int64_t NS = (int64_t)n * S;
     double* cost = (double*)malloc(sizeof(double) * (size_t)NS);
     int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)NS);
     int64_t* hp = (int64_t*)malloc(sizeof(int64_t) * (size_t)NS);
-    if (!cost || !pos || !hp) {
-        free(cost); free(pos); free(hp);
+    double* W = use_lb >= 2 ? (double*)malloc(sizeof(double) * (size_t)(S * g * g)) : NULL;
+    if (!cost || !pos || !hp || (use_lb >= 2 && !W)) {
+        free(cost); free(pos); free(hp); free(W);
         return ORC_OOM;
+    }
+    if (use_lb >= 2) {
+        double D[20 * 20];
+        for (int32_t a = 0; a < g; ++a)
+            for (int32_t b = 0; b < g; ++b) {
+                double m = INFINITY;
+                for (int32_t u = 0; u < n; ++u)
+                    if (((gm[u] >> b) & 1u) && dist[(int64_t)a * n + u] < m) m = dist[(int64_t)a * n + u];
+                D[a * g + b] = m;
+            }
+        orc_tour_tables(g, D, W);
     }
     for (int64_t s = 0; s < NS; ++s) { cost[s] = INFINITY; pos[s] = -1; }
     heap_t H = {hp, pos, cost, 0};
     int64_t pops = 0, relax = 0, merges = 0;
     /* one-label bound of (v, p) */
-#define ORC_LB(v, p) (use_lb ? orc_label_lb(dist, n, g, (v), (p)) : 0.0)
+#define ORC_LB(v, p) (use_lb ? orc_lb(dist, W, n, g, (v), (p), use_lb) : 0.0)
     for (int32_t v = 0; v < n; ++v) {
         const uint32_t gv = gm[v] & full;
         if (!gv) continue;
@@ -305,7 +438,7 @@ int orc_dpbf_pruned(int32_t n, const int64_t* row, const int32_t* col, const dou
 #undef ORC_LB
     if (stats) { stats[0] = pops; stats[1] = relax; stats[2] = merges; }
     *cost_out = best;
-    free(cost); free(pos); free(hp);
+    free(cost); free(pos); free(hp); free(W);
     return best < INFINITY ? ORC_OK : ORC_INFEASIBLE;
 }
 

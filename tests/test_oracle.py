@@ -173,3 +173,70 @@ def test_spider_bound_is_feasible_and_keeps_optimum(s):
     if g == 3:
         assert spider == opt
     assert oracle.dpbf_pruned(G, groups, spider)[0] == opt
+
+
+@pytest.mark.parametrize("s", range(0, 60, 3))
+def test_tour_bound_keeps_optimum_vs_bruteforce(s):
+    """One-label plus the tour bounds of PrunedDP++ keep the brute-force optimum for U >= OPT."""
+    n = 10 + s % 5
+    g = 2 + s % 5
+    G = _zero_weight_graph(n, 8, 4, 9100 + s) if s % 3 == 0 else gst.gen_random_connected(n, n + 6, 1, 20, seed=5100 + s)
+    groups = gst.gen_groups(n, g, 1, 3, seed=6100 + s)
+    b = oracle.bruteforce(G, groups)
+    for U in (b, b + 1, 2 * b + 3):
+        assert oracle.dpbf_pruned(G, groups, U, 0.5, 2)[0] == b
+        assert oracle.dpbf_pruned(G, groups, U, 0.0, 2)[0] == b
+
+
+@pytest.mark.parametrize("s", range(6))
+def test_tour_bound_per_state_vs_bruteforce(s):
+    """Each state's bound is at most the cost of the cheapest tree that holds v and touches every
+    group missing from the state (brute force with {v} as an extra group), and is never below the
+    one-label bound; with one group missing the two bounds agree."""
+    n = 9 + s % 3
+    g = 3 + s % 2
+    G = gst.gen_random_connected(n, n + 5, 1, 12, seed=8100 + s) if s % 2 else _zero_weight_graph(n, 6, 3, 8200 + s)
+    groups = gst.gen_groups(n, g, 1, 3, seed=8300 + s)
+    lists = [list(groups.members(i)) for i in range(g)]
+    tour = oracle.state_bounds(G, groups, 2)
+    one = oracle.state_bounds(G, groups, 1)
+    full = (1 << g) - 1
+    for p in range(1, full):
+        miss = [i for i in range(g) if not (p >> i) & 1]
+        for v in range(n):
+            need = oracle.bruteforce(G, gst.groups_from_lists([lists[i] for i in miss] + [[v]]))
+            assert one[v, p] <= tour[v, p] <= need + 1e-12, (v, p, tour[v, p], need)
+            if len(miss) == 1:
+                assert tour[v, p] == one[v, p]
+
+
+def test_tour_bound_is_stronger_somewhere():
+    strict = 0
+    for s in range(6):
+        G = gst.gen_random_connected(200, 600, 1, 50, seed=8400 + s)
+        groups = gst.gen_groups(G.n, 5, 2, 6, seed=8500 + s)
+        strict += int(np.sum(oracle.state_bounds(G, groups, 2) > oracle.state_bounds(G, groups, 1)))
+    assert strict > 0
+
+
+def test_tour_bound_planted_mistake_is_caught():
+    """Planted mistake: taking lb1 as the max (not the min) over the tour's end groups overstates
+    the bound and loses the optimum on some instance, so the tests above would catch it."""
+    lost = 0
+    for s in range(30):
+        G = gst.gen_random_connected(12, 18, 1, 20, seed=7200 + s)
+        groups = gst.gen_groups(12, 3 + s % 3, 1, 2, seed=7300 + s)
+        b = oracle.bruteforce(G, groups)
+        lost += oracle.dpbf_pruned(G, groups, b, 0.5, 3)[0] != b
+    assert lost >= 1
+
+
+@pytest.mark.parametrize("s", range(4))
+def test_tour_bound_prunes_more_c1(s):
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=3 + s)
+    groups = gst.gen_groups(G.n, 6, 5, 20, seed=s + 1)
+    c0 = oracle.dpbf_cost(G, groups)
+    c1, st1 = oracle.dpbf_pruned(G, groups, c0, 0.5, 1)
+    c2, st2 = oracle.dpbf_pruned(G, groups, c0, 0.5, 2)
+    assert c1 == c2 == c0
+    assert st2["pops"] <= st1["pops"]
