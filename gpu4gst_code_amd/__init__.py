@@ -261,7 +261,7 @@ class Solver:
     def solve(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
               want_tree: bool = True, rounds_per_sync: int = 0, chunk_arcs: int = 0,
               pull_permille: int = 0, grow_mode: int = 0, push_lanes: int = 0, half_bound: bool = True,
-              auto_bound: bool = False, merge_tile_kb: int = 0) -> Result:
+              auto_bound: bool = False, merge_tile_kb: int = 0, tour_bound: bool = True) -> Result:
         opts = _abi.gst_opts()
         opts.ub = float(ub) if ub is not None else 0.0
         opts.prune_lb = 1 if prune_lb else 0
@@ -273,6 +273,7 @@ class Solver:
         opts.push_lanes = push_lanes
         opts.half_bound = 0 if half_bound else -1
         opts.auto_bound = 1 if auto_bound else 0
+        opts.tour_bound = 0 if tour_bound else -1
         opts.merge_tile_kb = merge_tile_kb
         res = _abi.gst_result()
         grp = groups.as_c()
@@ -297,7 +298,7 @@ class Solver:
     def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                     use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
-                    auto_bound: bool = False, pull_permille: int = 0, time_kernels: bool = False,
+                    auto_bound: bool = False, pull_permille: int = 0, time_kernels: bool = False, tour_bound: bool = True,
                     raise_on_error: bool = True):
         """Solve many queries on one graph with the batched engine (gst_solve_batch).
 
@@ -317,6 +318,7 @@ class Solver:
             opts[i].push_lanes = push_lanes
             opts[i].half_bound = 0 if half_bound else -1
             opts[i].auto_bound = 1 if auto_bound else 0
+            opts[i].tour_bound = 0 if tour_bound else -1
             opts[i].pull_permille = pull_permille
         bo = _abi.gst_batch_opts()
         bo.slots = slots
@@ -436,7 +438,8 @@ def _solve_learned(self, graph: DeviceGraph, groups: Groups, model: Model, topk_
 def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model: Model, topk_frac: float = 0.05, *,
                          prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                          use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
-                         pull_permille: int = 0, time_kernels: bool = False, raise_on_error: bool = True):
+                         pull_permille: int = 0, time_kernels: bool = False, raise_on_error: bool = True,
+                         tour_bound: bool = True):
     """Exact solves of many queries with the learned bound, pipelined through the batched engine
     (gst_solve_batch_learned).  Returns (results, batch stats); each Result's stats also carry the
     query's learned-bound fields (sub_vertices, ub, t_sub_solve_ms, ...)."""
@@ -451,6 +454,7 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
         opts[i].want_tree = 1 if want_tree else 0
         opts[i].push_lanes = push_lanes
         opts[i].half_bound = 0 if half_bound else -1
+        opts[i].tour_bound = 0 if tour_bound else -1
         opts[i].pull_permille = pull_permille
     bo = _abi.gst_batch_opts()
     bo.slots = slots

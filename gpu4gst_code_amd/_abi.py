@@ -38,7 +38,8 @@ class gst_opts(C.Structure):
     _fields_ = [("ub", C.c_double), ("prune_lb", C.c_int32), ("want_tree", C.c_int32),
                 ("rounds_per_sync", C.c_int32), ("chunk_arcs", C.c_int32), ("pull_permille", C.c_int32),
                 ("grow_mode", C.c_int32), ("push_lanes", C.c_int32), ("half_bound", C.c_int32),
-                ("merge_tile_kb", C.c_int32), ("auto_bound", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("merge_tile_kb", C.c_int32), ("auto_bound", C.c_int32), ("tour_bound", C.c_int32),
+                ("reserved", C.c_int32 * 1)]
 
 
 class gst_stats(C.Structure):

@@ -300,6 +300,8 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
         if (status[q] == GST_OK && opts && opts[q].half_bound != 0 && opts[q].half_bound != -1)
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1");
+        if (status[q] == GST_OK && opts && opts[q].tour_bound != 0 && opts[q].tour_bound != -1)
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "tour_bound must be 0 or -1");
         if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
         else if (first == GST_OK) first = status[q];
     }
@@ -361,6 +363,8 @@ gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq,
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
         if (status[q] == GST_OK && opts && opts[q].half_bound != 0 && opts[q].half_bound != -1)
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1");
+        if (status[q] == GST_OK && opts && opts[q].tour_bound != 0 && opts[q].tour_bound != -1)
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "tour_bound must be 0 or -1");
     }
     return gst_learned_solve_batch(ctx, g, nq, groups, m, topk_frac, opts, bopts, results, status, lstats, stats);
 }

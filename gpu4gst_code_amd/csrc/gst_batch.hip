@@ -175,6 +175,19 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
     }
 }
 
+// before the merges: a slot entering level 2 builds its tour table from its final level 1 (one
+// workgroup per slot; the level-2 merge is the table's first reader)
+template <bool FLT>
+__global__ void __launch_bounds__(256) k_batch_tour(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    const uint32_t i = blockIdx.x;
+    if (i >= p->n_merge || p->merge_level[i] != 2) return;
+    const MergeArgs& m2 = b.margs[p->merge_slot[i] * NCTL + 2];
+    if (!m2.tour) return;
+    tour_table_body<FLT>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
+                         const_cast<uint32_t*>(m2.tour));
+}
+
 __global__ void __launch_bounds__(256) k_batch_compact(BatchArgs b) {
     const BatchPlan* p = b.plan;
     const uint32_t nc = p->n_compact;
@@ -284,6 +297,8 @@ struct gst_batch_engine {
         unsigned long long* stats = nullptr;
         uint32_t* edges = nullptr;          // 3 * ecap + 1
         uint32_t* spider = nullptr;         // cheapest spider of the level-2 merge (automatic bound)
+        uint32_t* tour = nullptr;           // tour table 2^g * (g + 1) words, then walk scratch 2^g * g * g
+        int tour_g = 0;
         uint64_t* hub_items = nullptr;      // hub row items of a job's own graph (pull rounds)
         uint32_t* hub_cnt = nullptr;
         int32_t* grp = nullptr;
@@ -338,7 +353,7 @@ static size_t back_bytes() { return 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT +
 static void free_slot(gst_batch_engine::Slot& sl) {
     hipFree(sl.T); hipFree(sl.gm);
     for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
-    hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp); hipFree(sl.spider);
+    hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour);
     hipFree(sl.hub_items); hipFree(sl.hub_cnt);
     if (sl.h_grp) hipHostFree(sl.h_grp);
     if (sl.h_gargs) hipHostFree(sl.h_gargs);
@@ -505,6 +520,19 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     std::memcpy(sl.h_grp + g + 1, gr->grp_vtx, sizeof(int32_t) * (size_t)gr->grp_ptr[g]);
     HIPCHK(hipMemcpyAsync(sl.grp, sl.h_grp, gneed * 4, hipMemcpyHostToDevice, s));
 
+    // tour table (built on the device when the slot enters level 2)
+    const bool use_tour = job.opts.prune_lb && job.opts.tour_bound == 0 && g >= 3 &&
+                          (job.opts.auto_bound || (std::isfinite(job.opts.ub) && job.opts.ub > 0));
+    if (use_tour && sl.tour_g < g) {
+        hipFree(sl.tour);
+        sl.tour = nullptr;
+        sl.tour_g = 0;
+        if (!dalloc(sl.tour, ((size_t)1 << g) * (size_t)(g + 1 + g * g)))
+            return gst_set_error(ctx, GST_OOM, "batch: tour table allocation failed");
+        sl.tour_g = g;
+    }
+    const uint32_t* tour = use_tour ? sl.tour : nullptr;
+
     uint64_t lvl_base[GST_MAX_GROUPS + 2] = {0};
     for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
     const gst_bound_bits bb = gst_make_bound(job.opts.ub, flt, job.opts.half_bound);
@@ -547,6 +575,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.gmax = bb.gmax;
         a.lbcap = bb.lbcap;
         a.use_lb = (job.opts.prune_lb && k >= 2) ? 1u : 0u;
+        a.tour = k >= 2 ? tour : nullptr;
         a.all_items = persistent ? E->all_items : sl.hub_items;
         a.n_all = persistent ? E->n_all : 0u;
         a.n_all_dev = persistent ? nullptr : sl.hub_cnt;
@@ -575,6 +604,9 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ubb; ma.use_lb = job.opts.prune_lb ? 1u : 0u;
         ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
         ma.spider = (job.opts.auto_bound && k == 2 && g >= 3) ? sl.spider : nullptr;
+        ma.tour = tour;
+        ma.grp = sl.grp;
+        ma.tour_w = tour ? sl.tour + ((size_t)1 << g) * (size_t)(g + 1) : nullptr;
         ma.is_full = (k == g) ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
         ma.VB = merge_tile_vertices(Lk, Wk);
@@ -614,6 +646,8 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
     // one LDS size for every (g, level) a slot may merge: tile rows, bitmaps and split table
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + MERGE_PAIRS_LDS;
     k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
+    if (flt) k_batch_tour<true><<<MAX_SLOTS, 256, 0, s>>>(ba);
+    else k_batch_tour<false><<<MAX_SLOTS, 256, 0, s>>>(ba);
     if (ev) hipEventRecord(ev[0], s);
     if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
     else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);

@@ -52,6 +52,8 @@ struct gst_query_bufs {
     unsigned long long* root = nullptr;
     unsigned long long* stats = nullptr;
     uint32_t* spider = nullptr;  // cheapest spider (automatic bound)
+    uint32_t* tour = nullptr;    // tour table, 2^g * (g + 1) words, then the walk scratch 2^g * g * g
+    int tour_g = 0;              // group count tour is sized for
     uint32_t* edges = nullptr;   // 3 * cap_edges + 1 (count)
     int32_t* grp = nullptr;      // grp_ptr (g+1) then members
     // level tables for the current g
@@ -91,7 +93,7 @@ static void free_bufs(gst_query_bufs* b) {
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
     hipFree(b->all_items); hipFree(b->all_cnt);
-    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider);
+    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider); hipFree(b->tour);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -259,6 +261,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
     if (opts.half_bound != 0 && opts.half_bound != -1)
         return gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1 (got %d)", opts.half_bound);
+    if (opts.tour_bound != 0 && opts.tour_bound != -1)
+        return gst_set_error(ctx, GST_INVALID_ARG, "tour_bound must be 0 or -1 (got %d)", opts.tour_bound);
     if (opts.merge_tile_kb < 0 || opts.merge_tile_kb > 48)
         return gst_set_error(ctx, GST_INVALID_ARG, "merge_tile_kb must be 0..48 (got %d)", opts.merge_tile_kb);
     HIPCHK(hipSetDevice(ctx->device));
@@ -437,6 +441,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             return gst_set_error(ctx, GST_OOM, "round log allocation failed");
         k_fill_u32<<<64, 256, 0, s>>>(B->rlog, 0xFFFFFFFFu, (uint64_t)NCTL * RLOG_ROUNDS * 4);
     }
+    const uint32_t* tour_ptr = nullptr;   // set once level 1 is final
     auto run_grow = [&](int k) -> gst_status {
         GrowArgs ga;
         ga.Tk = B->T + lvl_base[k];
@@ -450,6 +455,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         // the one-label lower bound reads the group distances of level 1, which are final only
         // after level 1 has converged: never prune with it inside level 1
         ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;
+        ga.tour = k >= 2 ? tour_ptr : nullptr;
         ga.all_items = B->all_items; ga.n_all = B->n_all; ga.n_all_dev = nullptr; ga.n = n;
         {
             const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
@@ -563,6 +569,24 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             bb.ub = ub_bits;
         }
     }
+    // ---- tour table: group-to-group walks from the final level 1 ----
+    const uint32_t* tour = nullptr;
+    if (st == GST_OK && opts.prune_lb && opts.tour_bound == 0 && g >= 3 && ub_bits != GST_INF_COST) {
+        const size_t words = ((size_t)1 << g) * (size_t)(g + 1) + ((size_t)1 << g) * (size_t)(g * g);
+        if (B->tour_g < g) {
+            hipFree(B->tour);
+            B->tour = nullptr;
+            B->tour_g = 0;
+            if (hipMalloc(&B->tour, words * 4) != hipSuccess) return gst_set_error(ctx, GST_OOM, "solve: allocation failed");
+            B->tour_g = g;
+        }
+        uint32_t* Pw = B->tour + ((size_t)1 << g) * (size_t)(g + 1);
+        if (flt) k_tour_table<true><<<1, 256, 0, s>>>(B->T + lvl_base[1], B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+        else k_tour_table<false><<<1, 256, 0, s>>>(B->T + lvl_base[1], B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+        HIPCHK(hipGetLastError());
+        tour = B->tour;
+    }
+    tour_ptr = tour;
     // ---- levels 2..g ----
     for (int k = 2; k <= g && st == GST_OK; ++k) {
         const bool full = (k == g);
@@ -575,6 +599,9 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ub_bits; ma.use_lb = opts.prune_lb ? 1u : 0u;
         ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
         ma.spider = nullptr;
+        ma.tour = tour;
+        ma.grp = nullptr;
+        ma.tour_w = nullptr;
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;

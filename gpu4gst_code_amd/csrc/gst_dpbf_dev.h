@@ -70,6 +70,7 @@ struct GrowArgs {
     uint32_t gmax;              // a state is grown only while its cost is <= gmax (half the bound)
     uint32_t lbcap;             // level-1 distances above gmax are only known to exceed it: the lower
                                 // bound reads min(distance, lbcap) (GST_INF_COST: no cap)
+    const uint32_t* tour;       // tour table of the query (tour_table_body), nullptr: one-label only
 };
 
 // Lanes per push item unless the caller chose (measured on BA V=200k, g=6): one query alone is
@@ -119,6 +120,10 @@ struct MergeArgs {
     uint32_t gmax;         // merged states above this cost are not put on the frontier
     uint32_t lbcap;        // cap on level-1 distances in the lower bound (see GrowArgs)
     uint32_t* spider;      // k == 2 with an automatic bound: min over the tile's vertices of sum_i T(v,{i})
+    const uint32_t* tour;  // tour table of the query (the merge itself applies the one-label bound only)
+    // batched engine, level 2: the slot's groups and the walk scratch the tour table is built from
+    const int32_t* grp;    // grp_ptr[0..g] then the members
+    uint32_t* tour_w;      // 2^g * g * g words
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -288,16 +293,145 @@ __device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, 
 // 1 grew only states of cost <= gmax, a distance above gmax (or inf) says only that the true distance
 // exceeds gmax: each is capped at lbcap = gmax (bit patterns of non-negative floats order like the
 // floats, so the cap is a u32 min for both weight types).
-__device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, uint32_t v, uint32_t g, uint32_t X, uint32_t lbcap) {
-    uint32_t lb = 0;
-    const uint64_t* r1 = T1 + (uint64_t)v * g;
+//
+// With a tour table (tour != nullptr) the bound also takes the tour bounds of PrunedDP++
+// (rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:435-494).  Any tree that completes (v, X) holds v
+// and touches every missing group (the set U); walking around it from v passes every group of U and
+// costs at most twice its weight.  With W(i, U) the cheapest walk between groups that starts at
+// group i and passes every group of U (group-to-group steps priced by the closest pair of members)
+// and Wmin(U) = min_i W(i, U):
+//   lb2 = (max_i (d_i + W(i, U)) + min_i d_i) / 2          (the reference's lb2, :471-476)
+//   lb1 = min_i d_i + Wmin(U) / 2                          (a relaxation of the reference's lb1,
+//                                                           :463-470, which takes the walk's ends)
+// The table holds W(i, U) at tour[U * (g + 1) + i] and Wmin(U) at tour[U * (g + 1) + g]; an
+// entry of GST_INF_COST switches the tour terms off for that state.  Int costs round the halves up
+// (the completing tree's cost is an integer); float bounds are scaled by 0.9998, far beyond the
+// rounding of the few float additions behind them.
+template <bool FLT, class DistF>
+__device__ __forceinline__ uint32_t state_lb(DistF dist, const uint32_t* tour, uint32_t g, uint32_t X, uint32_t lbcap) {
+    uint32_t one = 0, dmin = GST_INF_COST;
+    const uint32_t U = ((1u << g) - 1u) & ~X;
+    const uint32_t* tu = (tour && (U & (U - 1u))) ? tour + (size_t)U * (g + 1u) : nullptr;
+    bool tour_ok = tu != nullptr;
+    uint64_t tmax_i = 0;
+    float tmax_f = 0.0f;
     for (uint32_t gi = 0; gi < g; ++gi) {
         if ((X >> gi) & 1u) continue;
-        const uint32_t di = min((uint32_t)(r1[gi] >> 32), lbcap);
+        const uint32_t di = min(dist(gi), lbcap);
         if (di == GST_INF_COST) return GST_INF_COST;
-        lb = max(lb, di);
+        one = max(one, di);
+        if (tour_ok) {
+            dmin = min(dmin, di);
+            const uint32_t w2 = tu[gi];
+            if (w2 == GST_INF_COST) {
+                tour_ok = false;
+            } else if (FLT) {
+                tmax_f = fmaxf(tmax_f, __fadd_rn(__uint_as_float(di), __uint_as_float(w2)));
+            } else {
+                tmax_i = max(tmax_i, (uint64_t)di + w2);
+            }
+        }
     }
-    return lb;
+    if (!tour_ok) return one;
+    const uint32_t wmin = tu[g];
+    if (wmin == GST_INF_COST) return one;
+    uint32_t lb2, lb1;
+    if (FLT) {
+        const float dm = __uint_as_float(dmin);
+        lb2 = __float_as_uint(__fmul_rn(__fmul_rn(__fadd_rn(tmax_f, dm), 0.5f), 0.9998f));
+        lb1 = __float_as_uint(__fmul_rn(__fadd_rn(dm, __fmul_rn(__uint_as_float(wmin), 0.5f)), 0.9998f));
+    } else {
+        const uint64_t a = (tmax_i + dmin + 1u) / 2u, b = (uint64_t)dmin + ((uint64_t)wmin + 1u) / 2u;
+        lb2 = (uint32_t)min<uint64_t>(a, GST_INF_COST - 1u);
+        lb1 = (uint32_t)min<uint64_t>(b, GST_INF_COST - 1u);
+    }
+    return max(one, max(lb2, lb1));
+}
+
+template <bool FLT>
+__device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, const uint32_t* tour, uint32_t v, uint32_t g,
+                                             uint32_t X, uint32_t lbcap) {
+    const uint64_t* r1 = T1 + (uint64_t)v * g;
+    return state_lb<FLT>([&](uint32_t gi) { return (uint32_t)(r1[gi] >> 32); }, tour, g, X, lbcap);
+}
+
+// Builds the tour table of one query with one workgroup, once level 1 is final:
+//   D(a, b) = min over members u of group b of min(T(u, {a}), lbcap)  (closest pair of members, as
+//             level 1 knows it: a distance above the cap is only known to exceed it)
+//   metric closure of D over intermediate groups, then a Held-Karp DP over group sets:
+//   P(m, i, j) = cheapest walk from group i to group j passing every group of m (i, j in m),
+//   one popcount layer of m at a time (open walks, then closed ones i == j of the same layer)
+// into tour[m * (g + 1) + i] = min_j P(m, i, j) and tour[m * (g + 1) + g] = min_i of those.
+// The walk table P lives in the scratch buffer Pw (2^g * g * g words); sums that overflow the int
+// range saturate to GST_INF_COST, which only ever weakens the bound.
+template <bool FLT>
+__device__ void tour_table_body(const uint64_t* T1, const int32_t* grp_ptr, const int32_t* grp_vtx, uint32_t g,
+                                uint32_t lbcap, uint32_t* Pw, uint32_t* tour) {
+    __shared__ uint32_t D[GST_MAX_GROUPS * GST_MAX_GROUPS];
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t G2 = g * g, S = 1u << g;
+    uint32_t o = 0;
+    for (uint32_t i = tid; i < G2; i += nt) D[i] = GST_INF_COST;
+    __syncthreads();
+    const uint32_t tot = (uint32_t)grp_ptr[g];
+    for (uint32_t e = tid; e < tot; e += nt) {
+        uint32_t b = 0;
+        while (b + 1 < g && (uint32_t)grp_ptr[b + 1] <= e) ++b;
+        const uint32_t u = (uint32_t)grp_vtx[e];
+        const uint64_t* r1 = T1 + (uint64_t)u * g;
+        for (uint32_t a = 0; a < g; ++a) atomicMin(&D[a * g + b], min((uint32_t)(r1[a] >> 32), lbcap));
+    }
+    __syncthreads();
+    // row and column kk do not change in round kk (D(kk, kk) = 0), so one round runs in parallel
+    for (uint32_t kk = 0; kk < g; ++kk) {
+        for (uint32_t i = tid; i < G2; i += nt) {
+            const uint32_t a = i / g, b = i - a * g;
+            const uint32_t c = add_cost<FLT>(D[a * g + kk], D[kk * g + b], o);
+            if (c < D[i]) D[i] = c;
+        }
+        __syncthreads();
+    }
+    for (uint32_t x = tid; x < S * G2; x += nt) Pw[x] = GST_INF_COST;
+    __syncthreads();
+    for (uint32_t i = tid; i < g; i += nt) Pw[(1u << i) * G2 + i * g + i] = 0u;
+    __syncthreads();
+    for (uint32_t c = 2; c <= g; ++c) {
+        for (uint32_t x = tid; x < S * G2; x += nt) {
+            const uint32_t m = x / G2, rem = x - m * G2, i = rem / g, j = rem - i * g;
+            if ((uint32_t)__popc(m) != c || i == j || !((m >> i) & 1u) || !((m >> j) & 1u)) continue;
+            const uint32_t prev = m ^ (1u << j);
+            uint32_t best = GST_INF_COST;
+            for (uint32_t kk = 0; kk < g; ++kk)
+                if ((prev >> kk) & 1u) best = min(best, add_cost<FLT>(Pw[prev * G2 + i * g + kk], D[kk * g + j], o));
+            Pw[x] = best;
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < S * g; x += nt) {
+            const uint32_t m = x / g, i = x - m * g;
+            if ((uint32_t)__popc(m) != c || !((m >> i) & 1u)) continue;
+            uint32_t best = GST_INF_COST;
+            for (uint32_t kk = 0; kk < g; ++kk)
+                if (kk != i && ((m >> kk) & 1u)) best = min(best, add_cost<FLT>(Pw[m * G2 + i * g + kk], D[kk * g + i], o));
+            Pw[m * G2 + i * g + i] = best;
+        }
+        __syncthreads();
+    }
+    for (uint32_t x = tid; x < S * (g + 1); x += nt) {
+        const uint32_t m = x / (g + 1), i = x - m * (g + 1);
+        uint32_t best = GST_INF_COST;
+        for (uint32_t a = 0; a < g; ++a) {
+            if (!((m >> a) & 1u) || (i < g && a != i)) continue;
+            for (uint32_t j = 0; j < g; ++j)
+                if ((m >> j) & 1u) best = min(best, Pw[m * G2 + a * g + j]);
+        }
+        tour[x] = best;
+    }
+}
+
+template <bool FLT>
+__global__ void __launch_bounds__(256) k_tour_table(const uint64_t* T1, const int32_t* grp, uint32_t g, uint32_t lbcap,
+                                                    uint32_t* Pw, uint32_t* tour) {
+    tour_table_body<FLT>(T1, grp, grp + g + 1, g, lbcap, Pw, tour);
 }
 
 #ifndef GST_GROW_UNR
@@ -491,7 +625,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
                 const unsigned long long own = lane_ok ? *addr : 0ull;
                 const uint32_t ocost = (uint32_t)(own >> 32);
-                const uint32_t lb = (prune_lb && lane_ok) ? label_lb(a.T1, u, a.g, a.masks[rk], a.lbcap) : 0u;
+                const uint32_t lb = (prune_lb && lane_ok) ? label_lb<FLT>(a.T1, a.tour, u, a.g, a.masks[rk], a.lbcap) : 0u;
                 unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;
                 for (uint32_t s = 0; s < maxdeg; s += UNR) {
                     uint32_t nb[UNR], wt[UNR], cn[UNR];
@@ -573,7 +707,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
             const unsigned long long own = *addr;
             const uint32_t ocost = (uint32_t)(own >> 32);
-            const uint32_t lb = prune_lb ? label_lb(a.T1, u, a.g, a.masks[rk], a.lbcap) : 0u;
+            const uint32_t lb = prune_lb ? label_lb<FLT>(a.T1, a.tour, u, a.g, a.masks[rk], a.lbcap) : 0u;
             unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;  // best via positive / zero-effect edges
             const uint32_t len = e1 - e0;
             const uint32_t steps = (len + A - 1) / A;
@@ -683,7 +817,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const uint32_t bit = select_bit(bm, t0);
                     const uint32_t rk = wd * 64 + bit;
                     const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
-                    const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk], a.lbcap) : 0u;
+                    const uint32_t lb = prune_lb ? label_lb<FLT>(a.T1, a.tour, v, a.g, a.masks[rk], a.lbcap) : 0u;
                     if (cv <= a.gmax && !reaches_ub<FLT>(cv, lb, a.ub)) keep |= 1ull << bit;
                 }
                 bm = group_or64(keep, PL);
@@ -887,7 +1021,7 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
             const uint32_t bit = select_bit(bm, lane);
             const uint32_t rk = wd * 64 + bit;
             const uint32_t cv = (uint32_t)(ld_agent((const unsigned long long*)(a.Tk + (uint64_t)v * m + rk)) >> 32);
-            const uint32_t lb = prune_lb ? label_lb(a.T1, v, a.g, a.masks[rk], a.lbcap) : 0u;
+            const uint32_t lb = prune_lb ? label_lb<FLT>(a.T1, a.tour, v, a.g, a.masks[rk], a.lbcap) : 0u;
             if (cv <= a.gmax && !reaches_ub<FLT>(cv, lb, a.ub)) keep = 1ull << bit;
         }
         bm = wave_or64(keep);
@@ -1291,14 +1425,10 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                 }
             }
             if (best != GST_INF_WORD && prune) {
-                uint32_t lb = 0;
-                if (prune_lb) {
-                    for (uint32_t gi = 0; gi < g; ++gi) {
-                        if ((X >> gi) & 1u) continue;
-                        const uint32_t di = min(rv[gi], a.lbcap);  // level-1 position gi = mask 1<<gi
-                        lb = (di == GST_INF_COST || lb == GST_INF_COST) ? GST_INF_COST : max(lb, di);
-                    }
-                }
+                // one-label bound only (level-1 position gi of the staged row is the mask 1<<gi):
+                // a merged state is grown only after the grow rounds' own check, which adds the
+                // tour bounds, so repeating those here would only cost the merge time
+                const uint32_t lb = prune_lb ? state_lb<FLT>([&](uint32_t gi) { return rv[gi]; }, nullptr, g, X, a.lbcap) : 0u;
                 if (reaches_ub<FLT>((uint32_t)(best >> 32), lb, a.ub)) best = GST_INF_WORD;
             }
             ((uint64_t*)Tk_base)[(uint64_t)v0 * mk + i] = best;

@@ -121,27 +121,33 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         ctx->tuned_batch.erase(std::remove_if(ctx->tuned_batch.begin(), ctx->tuned_batch.end(),
                                               [&](const gst_tuned_batch& b) { return b.uid == G->uid; }),
                                ctx->tuned_batch.end());
+        // the engine is timed as a stream: 64 slots (fewer when they do not fit) fed the sample
+        // over and over, 128 jobs at least, so that slots refill while others run as in a long batch
+        const int32_t njobs = std::max<int32_t>(nq, 128);
         std::vector<gst_opts> bo((size_t)nq);
-        std::vector<gst_result> res((size_t)nq);
-        std::vector<gst_status> st((size_t)nq);
+        std::vector<gst_result> res((size_t)njobs);
+        std::vector<gst_status> st((size_t)njobs);
         for (int32_t q = 0; q < nq; ++q) {
             bo[(size_t)q] = gst_opts();
             bo[(size_t)q].prune_lb = 1;
             bo[(size_t)q].want_tree = 1;
             bo[(size_t)q].ub = ubs[(size_t)q];
         }
+        int32_t tune_slots = 64;
         auto beval = [&](int32_t R, int32_t lanes, double* qps) -> gst_status {
             for (auto& o : bo) o.push_lanes = lanes;
             gst_batch_opts bopt{};
+            bopt.slots = tune_slots;
             bopt.rounds_per_step = R;
             bopt.use_graph = eager ? 0 : 1;
             double best_q = 0.0;
-            for (int rep2 = 0; rep2 < 2; ++rep2) {   // the first run builds the captured step
+            for (int rep2 = 0; rep2 < (eager ? 1 : 2); ++rep2) {   // the first run builds the captured step
                 int32_t next = 0;
                 auto next_job = [&](gst_batch_job& j) -> bool {
-                    if (next >= nq) return false;
+                    if (next >= njobs) return false;
+                    const int32_t q = next % nq;
                     j = gst_batch_job();
-                    j.G = G; j.groups = &sample[next]; j.opts = bo[(size_t)next]; j.res = &res[(size_t)next]; j.tag = next;
+                    j.G = G; j.groups = &sample[q]; j.opts = bo[(size_t)q]; j.res = &res[(size_t)next]; j.tag = next;
                     ++next;
                     return true;
                 };
@@ -150,8 +156,13 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
                 gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, g, G->m, &bopt, nullptr, 0);
                 const double ms = now_ms() - t0;
                 for (auto& r : res) gst_result_free(&r);
+                if (rs == GST_OOM && bopt.slots != 0) {   // 64 slots do not fit: the engine's own count
+                    tune_slots = bopt.slots = 0;
+                    --rep2;
+                    continue;
+                }
                 if (rs != GST_OK) return rs;
-                best_q = std::max(best_q, nq * 1000.0 / ms);
+                best_q = std::max(best_q, njobs * 1000.0 / ms);
             }
             *qps = best_q;
             tried++;

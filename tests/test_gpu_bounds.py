@@ -150,3 +150,80 @@ def test_auto_bound_prunes_ba20k(solver):
         _exact(G, groups, r, want)
         assert r.stats["grow_relax"] < plain.stats["grow_relax"]
     dg.free()
+
+
+def test_tour_bound_exact_and_prunes(solver):
+    """The tour bounds (on by default) keep the optimum at the tightest bound U = OPT and relax
+    fewer states than the one-label bound alone where groups lie far apart (C1-type graphs, g=6);
+    on the BA V=20k goldens, whose groups sit close together, they stay exact."""
+    G = gst.gen_random_connected(1000, 5000, 1, 100, seed=3)
+    dg = solver.upload(G)
+    less, tot_one, tot_tour = 0, 0, 0
+    for s in range(6):
+        groups = gst.gen_groups(G.n, 6, 5, 20, seed=s + 1)
+        want = oracle.dpbf_cost(G, groups)
+        for U in (want, want * 1.05):
+            one = solver.solve(dg, groups, ub=U, tour_bound=False)
+            tour = solver.solve(dg, groups, ub=U)
+            print(f"s{s} U {U}: relax tour {tour.stats['grow_relax']} vs one-label {one.stats['grow_relax']}", flush=True)
+            _exact(G, groups, one, want)
+            _exact(G, groups, tour, want)
+            less += tour.stats["grow_relax"] < one.stats["grow_relax"]
+            tot_one += one.stats["grow_relax"]
+            tot_tour += tour.stats["grow_relax"]
+    assert less >= 3 and tot_tour < tot_one
+    dg.free()
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    for q in range(3):
+        groups = gst.gen_groups(G.n, 6, 10, 200, seed=1000 + q)
+        want = GOLD["ba20k"]["costs"][str(q)]
+        _exact(G, groups, solver.solve(dg, groups, ub=want), want)
+        _exact(G, groups, solver.solve(dg, groups, auto_bound=True), want)
+    dg.free()
+
+
+@pytest.mark.parametrize("g", [3, 5, 9, 12])
+def test_tour_bound_group_counts_and_weights(solver, g):
+    """U = OPT (the tightest bound, where an overstated lower bound would lose the optimum) on int,
+    zero-weight and float graphs, for small and the largest group counts (tour tables up to
+    2^12 sets x 12 x 12 walks), single solves and the batched engine."""
+    for kind in ("int", "zero", "float"):
+        if kind == "int":
+            G = gst.gen_random_connected(300, 1200, 1, 50, seed=500 + g)
+        elif kind == "zero":
+            G = gst.gen_random_connected(300, 1200, 0, 3, seed=510 + g)
+        else:
+            G = gst.gen_rmat(9, 8, weights="lognormal", seed=520 + g)
+        dg = solver.upload(G)
+        qs = [gst.gen_groups(G.n, g, 1, 6, seed=530 + 10 * g + i) for i in range(2)]
+        want = [oracle.dpbf_cost(G, q) for q in qs]
+        ok = [np.isfinite(w) for w in want]
+        qs = [q for q, o in zip(qs, ok) if o]
+        want = [w for w, o in zip(want, ok) if o]
+        tol = (lambda a, b: abs(a - b) <= 1e-6 * max(1.0, b)) if kind == "float" else (lambda a, b: a == b)
+        for q, w in zip(qs, want):
+            r = solver.solve(dg, q, ub=w)
+            print(f"{kind} g {g}: {r.cost} want {w}", flush=True)
+            assert tol(r.cost, w), (kind, r.cost, w)
+            assert tol(gst.validate_tree(G, q, r.eu, r.ev, r.root), w)
+        if qs:
+            res, _ = solver.solve_batch(dg, qs, ubs=want, slots=2)
+            for q, r, w in zip(qs, res, want):
+                assert tol(r.cost, w), (kind, "batch", r.cost, w)
+                assert tol(gst.validate_tree(G, q, r.eu, r.ev, r.root), w)
+        dg.free()
+
+
+def test_tour_bound_rejects_bad_value(solver):
+    G = gst.gen_random_connected(100, 300, 1, 10, seed=1)
+    dg = solver.upload(G)
+    groups = gst.gen_groups(G.n, 3, 1, 4, seed=2)
+    opts = gst._abi.gst_opts()
+    opts.tour_bound = 5
+    import ctypes as C
+    res = gst._abi.gst_result()
+    grp = groups.as_c()
+    rc = solver.lib.gst_solve(solver.ctx, dg.handle, C.byref(grp), C.byref(opts), C.byref(res))
+    assert rc == gst._abi.GST_INVALID_ARG
+    dg.free()
