@@ -177,15 +177,16 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
 
 // before the merges: a slot entering level 2 builds its tour table from its final level 1 (one
 // workgroup per slot; the level-2 merge is the table's first reader)
-template <bool FLT>
+template <bool FLT, bool PW_LDS>
 __global__ void __launch_bounds__(256) k_batch_tour(BatchArgs b) {
     const BatchPlan* p = b.plan;
     const uint32_t i = blockIdx.x;
     if (i >= p->n_merge || p->merge_level[i] != 2) return;
     const MergeArgs& m2 = b.margs[p->merge_slot[i] * NCTL + 2];
     if (!m2.tour) return;
-    tour_table_body<FLT>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
-                         const_cast<uint32_t*>(m2.tour));
+    // the launch's LDS holds the walk table of the engine's largest group count
+    tour_table_body<FLT, PW_LDS>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
+                                 const_cast<uint32_t*>(m2.tour));
 }
 
 __global__ void __launch_bounds__(256) k_batch_compact(BatchArgs b) {
@@ -646,8 +647,14 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
     // one LDS size for every (g, level) a slot may merge: tile rows, bitmaps and split table
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + MERGE_PAIRS_LDS;
     k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
-    if (flt) k_batch_tour<true><<<MAX_SLOTS, 256, 0, s>>>(ba);
-    else k_batch_tour<false><<<MAX_SLOTS, 256, 0, s>>>(ba);
+    const size_t tl = tour_lds_bytes((uint32_t)E->g_cap);
+    if (tl) {
+        if (flt) k_batch_tour<true, true><<<MAX_SLOTS, 256, tl, s>>>(ba);
+        else k_batch_tour<false, true><<<MAX_SLOTS, 256, tl, s>>>(ba);
+    } else {
+        if (flt) k_batch_tour<true, false><<<MAX_SLOTS, 256, 0, s>>>(ba);
+        else k_batch_tour<false, false><<<MAX_SLOTS, 256, 0, s>>>(ba);
+    }
     if (ev) hipEventRecord(ev[0], s);
     if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
     else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);

@@ -581,8 +581,15 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             B->tour_g = g;
         }
         uint32_t* Pw = B->tour + ((size_t)1 << g) * (size_t)(g + 1);
-        if (flt) k_tour_table<true><<<1, 256, 0, s>>>(B->T + lvl_base[1], B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
-        else k_tour_table<false><<<1, 256, 0, s>>>(B->T + lvl_base[1], B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+        const size_t tl = tour_lds_bytes((uint32_t)g);
+        const uint64_t* T1 = B->T + lvl_base[1];
+        if (tl) {
+            if (flt) k_tour_table<true, true><<<1, 256, tl, s>>>(T1, B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+            else k_tour_table<false, true><<<1, 256, tl, s>>>(T1, B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+        } else {
+            if (flt) k_tour_table<true, false><<<1, 256, 0, s>>>(T1, B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+            else k_tour_table<false, false><<<1, 256, 0, s>>>(T1, B->grp, (uint32_t)g, bb.lbcap, Pw, B->tour);
+        }
         HIPCHK(hipGetLastError());
         tour = B->tour;
     }

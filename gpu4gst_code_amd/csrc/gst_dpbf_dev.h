@@ -362,12 +362,20 @@ __device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, const uint32_t*
 //   P(m, i, j) = cheapest walk from group i to group j passing every group of m (i, j in m),
 //   one popcount layer of m at a time (open walks, then closed ones i == j of the same layer)
 // into tour[m * (g + 1) + i] = min_j P(m, i, j) and tour[m * (g + 1) + g] = min_i of those.
-// The walk table P lives in the scratch buffer Pw (2^g * g * g words); sums that overflow the int
-// range saturate to GST_INF_COST, which only ever weakens the bound.
-template <bool FLT>
+// The walk table P (2^g * g * g words) lives in LDS when it fits (TOUR_LDS_MAX_G groups: 64 KiB,
+// given to the launch as dynamic LDS), else in the global scratch buffer Pw: each popcount layer
+// reads the previous one, so its latency repeats once per layer.  Sums that overflow the int range
+// saturate to GST_INF_COST, which only ever weakens the bound.
+constexpr uint32_t TOUR_LDS_MAX_G = 8;
+__host__ __device__ constexpr size_t tour_lds_bytes(uint32_t g) {
+    return g <= TOUR_LDS_MAX_G ? ((size_t)1 << g) * g * g * 4 : 0;
+}
+template <bool FLT, bool PW_LDS>
 __device__ void tour_table_body(const uint64_t* T1, const int32_t* grp_ptr, const int32_t* grp_vtx, uint32_t g,
-                                uint32_t lbcap, uint32_t* Pw, uint32_t* tour) {
+                                uint32_t lbcap, uint32_t* Pw_global, uint32_t* tour) {
     __shared__ uint32_t D[GST_MAX_GROUPS * GST_MAX_GROUPS];
+    extern __shared__ uint32_t tour_pw_lds[];
+    uint32_t* Pw = PW_LDS ? tour_pw_lds : Pw_global;
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint32_t G2 = g * g, S = 1u << g;
     uint32_t o = 0;
@@ -428,10 +436,10 @@ __device__ void tour_table_body(const uint64_t* T1, const int32_t* grp_ptr, cons
     }
 }
 
-template <bool FLT>
+template <bool FLT, bool PW_LDS>
 __global__ void __launch_bounds__(256) k_tour_table(const uint64_t* T1, const int32_t* grp, uint32_t g, uint32_t lbcap,
                                                     uint32_t* Pw, uint32_t* tour) {
-    tour_table_body<FLT>(T1, grp, grp + g + 1, g, lbcap, Pw, tour);
+    tour_table_body<FLT, PW_LDS>(T1, grp, grp + g + 1, g, lbcap, Pw, tour);
 }
 
 #ifndef GST_GROW_UNR

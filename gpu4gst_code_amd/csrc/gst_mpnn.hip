@@ -266,34 +266,44 @@ __global__ void __launch_bounds__(256) k_dense_fwd(const bf16* self, const bf16*
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * blockDim.x / 64;
     const uint32_t ntiles = (n + 15) / 16;
+    // the wave's B fragments (all of W, K/32 x 4 steps of 8 bf16 per lane) and biases stay in
+    // registers across its row tiles
+    constexpr int KS = K / 32;
+    bf16x8 bw[KS][4];
+    float bias[4];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) bw[ks][nb] = *(const bf16x8*)(Wt + (uint64_t)(nb * 16 + r) * K + ks * 32 + 8 * q);
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) bias[nb] = b[nb * 16 + r];
     for (uint32_t t = gwave; t < ntiles; t += nwaves) {
         const uint32_t v0 = t * 16;
         floatx4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
         const uint32_t va = v0 + r;
+        bf16x8 a[KS];
 #pragma unroll
-        for (int k0 = 0; k0 < K; k0 += 32) {
-            const int kk = k0 + 8 * (int)q;   // this lane's 8 consecutive k
-            bf16x8 a;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int kk = ks * 32 + 8 * (int)q;   // this lane's 8 consecutive k
             if (va < n) {
                 const bf16* src = kk < D ? self + (uint64_t)va * D + kk : nbr + (uint64_t)va * D + (kk - D);
-                a = *(const bf16x8*)src;
+                a[ks] = *(const bf16x8*)src;
             } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) a[j] = f2bf(0.f);
-            }
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                const bf16x8 bv = *(const bf16x8*)(Wt + (uint64_t)(nb * 16 + r) * K + kk);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[nb], 0, 0, 0);
+                for (int j = 0; j < 8; ++j) a[ks][j] = f2bf(0.f);
             }
         }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], bw[ks][nb], acc[nb], 0, 0, 0);
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
             const uint32_t col_n = nb * 16 + r;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint32_t v = v0 + 4 * q + i;
-                if (v < n) out[(uint64_t)v * HID + col_n] = f2bf(fmaxf(acc[nb][i] + b[col_n], 0.f));
+                if (v < n) out[(uint64_t)v * HID + col_n] = f2bf(fmaxf(acc[nb][i] + bias[nb], 0.f));
             }
         }
     }
@@ -642,34 +652,42 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* bsum) {
         out[i] += bsum[i / 1024];
 }
 
-// degree of v inside the selection
-__global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_t* sel, uint32_t n, uint32_t* deg) {
-    const uint32_t lane = __lane_id();
-    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
-    for (uint32_t v = gwave; v < n; v += nwaves) {
-        uint32_t c = 0;
-        if (sel[v]) {
-            for (uint32_t e = row[v] + lane; e < row[v + 1]; e += 64) c += sel[col[e]];
-        }
-        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-        if (lane == 0) deg[v] = sel[v] ? c : 0u;
+// list of the selected vertices by new id (old_of[newid[v]] = v), and every degree cleared
+__global__ void k_sub_list(const uint32_t* sel, const uint32_t* newid, uint32_t n, uint32_t* old_of, uint32_t* deg) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        deg[v] = 0u;
+        if (sel[v]) old_of[newid[v]] = v;
     }
 }
 
-// fill the subgraph CSR: new row r' = newid[v]; arcs keep their order (new ids are monotone in the
-// old ones, so rows stay in (weight, neighbour) order)
-__global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32_t* w, const uint32_t* sel,
-                           const uint32_t* newid, const uint32_t* arc_off, uint32_t n, uint32_t* srow, int32_t* scol,
-                           uint32_t* sw, uint32_t* old_of) {
+// degree of each selected vertex inside the selection (one wave per selected vertex; *ns of them)
+__global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_t* sel, const uint32_t* old_of,
+                          const uint32_t* ns, uint32_t* deg) {
     const uint32_t lane = __lane_id();
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * blockDim.x / 64;
-    for (uint32_t v = gwave; v < n; v += nwaves) {
-        if (!sel[v]) continue;
-        const uint32_t nv = newid[v];
+    const uint32_t cnt = *ns;
+    for (uint32_t i = gwave; i < cnt; i += nwaves) {
+        const uint32_t v = old_of[i];
+        uint32_t c = 0;
+        for (uint32_t e = row[v] + lane; e < row[v + 1]; e += 64) c += sel[col[e]];
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) deg[v] = c;
+    }
+}
+
+// fill the subgraph CSR: new row i holds old vertex old_of[i]; arcs keep their order (new ids are
+// monotone in the old ones, so rows stay in (weight, neighbour) order)
+__global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32_t* w, const uint32_t* sel,
+                           const uint32_t* newid, const uint32_t* arc_off, const uint32_t* old_of, uint32_t ns,
+                           uint32_t* srow, int32_t* scol, uint32_t* sw) {
+    const uint32_t lane = __lane_id();
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    for (uint32_t nv = gwave; nv < ns; nv += nwaves) {
+        const uint32_t v = old_of[nv];
         uint32_t pos = arc_off[v];
-        if (lane == 0) { srow[nv] = pos; old_of[nv] = v; }
+        if (lane == 0) srow[nv] = pos;
         for (uint32_t e0 = row[v]; e0 < row[v + 1]; e0 += 64) {
             const uint32_t e = e0 + lane;
             const bool ok = e < row[v + 1] && sel[col[e]];
@@ -712,10 +730,56 @@ __global__ void k_sub_groups(const int32_t* gv, uint32_t cnt, const uint32_t* ne
         out[i] = (int32_t)newid[gv[i]];
 }
 
-__global__ void k_set_bits(uint32_t* gm, const int32_t* ptr, const int32_t* vtx, int g) {
-    for (int i = 0; i < g; ++i)
-        for (int j = ptr[i] + blockIdx.x * blockDim.x + threadIdx.x; j < ptr[i + 1]; j += gridDim.x * blockDim.x)
-            atomicOr(&gm[vtx[j]], 1u << i);
+// Layer-1 aggregation of a query splits into a graph-only part and a group part: a vertex's input
+// features are two graph-only values (degree, lightest arc), the membership bits of groups 0..7 and
+// zeros, so the neighbour mean of the graph-only values is computed once per graph (ax_static,
+// with the gather kernel on features whose bits are all clear) and the mean of bit i is the count
+// of the row's neighbours in group i over the degree.  The counts come from a scatter along the
+// members' rows: one wave per member entry sets the member's group bit (for every group, as the
+// solver's masks need) and, the first time a (member, group < 8) pair is seen, adds 1 to the
+// fp32 count of each neighbour (integers, so exact in any order).
+__global__ void __launch_bounds__(256) k_set_bits_scatter(uint32_t* gm, const int32_t* ptr, const int32_t* vtx, int g,
+                                                          const uint32_t* row, const int32_t* col, float* gcnt) {
+    const uint32_t lane = __lane_id();
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    const uint32_t tot = (uint32_t)ptr[g];
+    for (uint32_t j = gwave; j < tot; j += nwaves) {
+        int i = 0;
+        while (i + 1 < g && (uint32_t)ptr[i + 1] <= j) ++i;
+        const uint32_t u = (uint32_t)vtx[j];
+        uint32_t old = 0;
+        if (lane == 0) old = atomicOr(&gm[u], 1u << i);
+        old = __shfl(old, 0, 64);
+        if (i >= 8 || ((old >> i) & 1u)) continue;
+        for (uint32_t e = row[u] + lane; e < row[u + 1]; e += 64) atomicAdd(&gcnt[(uint64_t)col[e] * 8 + i], 1.0f);
+    }
+}
+
+// ax[v] = the graph-only means of ax_static with the group means count / degree, rounded as the
+// gather rounds them (sum times 1/degree on light rows, sum / degree on hub rows)
+__global__ void k_ax_finish(const uint32_t* row, const bf16* ax_static, const float* gcnt, uint32_t n, bf16* ax) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t deg = row[v + 1] - row[v];
+        const float inv = deg ? 1.f / (float)deg : 0.f;
+        bf16x8 a = *(const bf16x8*)(ax_static + (uint64_t)v * FIN);
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = f2bf(0.f);
+        const float4 c0 = *(const float4*)(gcnt + (uint64_t)v * 8);
+        const float4 c1 = *(const float4*)(gcnt + (uint64_t)v * 8 + 4);
+        const float c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        float m[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m[i] = deg > (uint32_t)AGG_HUB ? c[i] / (float)deg : c[i] * inv;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a[2 + i] = f2bf(m[i]);
+        b[0] = f2bf(m[6]);
+        b[1] = f2bf(m[7]);
+        bf16x8* o = (bf16x8*)(ax + (uint64_t)v * FIN);
+        o[0] = a;
+        o[1] = b;
+    }
 }
 
 __global__ void k_fill32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -756,6 +820,9 @@ struct gst_model {
     bf16 *x = nullptr, *ax = nullptr, *h1 = nullptr, *ah1 = nullptr, *h2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
     float *s = nullptr, *dS = nullptr, *dX2 = nullptr, *dH1 = nullptr, *dA1 = nullptr, *dX1 = nullptr, *loss = nullptr;
     float* hub_acc = nullptr;
+    bf16* ax_static = nullptr;   // graph-only part of the layer-1 aggregation (per graph)
+    float* gcnt = nullptr;       // per vertex, neighbours in each of groups 0..7 (per query)
+    uint64_t static_uid = 0;     // graph ax_static belongs to
     uint32_t* gm = nullptr;
     uint8_t* y = nullptr;
     int32_t* grp = nullptr;
@@ -778,6 +845,9 @@ hipStream_t mstream(const gst_model* M) {
     return (hipStream_t)(M->sel_stream ? M->sel_stream : M->ctx->stream);
 }
 
+template <int D, bool BWD>
+void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipStream_t st);
+
 gst_status model_bufs(gst_model* M, const gst_graph* G) {
     gst_ctx* ctx = M->ctx;
     const uint32_t n = (uint32_t)G->n;
@@ -786,7 +856,9 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
                   dalloc(M->ah1, (size_t)n * HID) && dalloc(M->h2, (size_t)n * HID) && dalloc(M->dZ2, (size_t)n * HID) &&
                   dalloc(M->dZ1, (size_t)n * HID) && dalloc(M->s, n) && dalloc(M->dS, n) &&
                   dalloc(M->dX2, (size_t)n * K2) && dalloc(M->dH1, (size_t)n * HID) && dalloc(M->dA1, (size_t)n * HID) &&
-                  dalloc(M->gm, n) && dalloc(M->y, n);
+                  dalloc(M->gm, n) && dalloc(M->y, n) && dalloc(M->ax_static, (size_t)n * FIN) &&
+                  dalloc(M->gcnt, (size_t)n * 8);
+        M->static_uid = 0;
         if (!ok) { M->cap_n = 0; return gst_set_error(ctx, GST_OOM, "model: activation allocation failed (n=%u)", n); }
         M->cap_n = n;
     }
@@ -815,6 +887,16 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
         M->n_hub_items = (uint32_t)(hi.size() / 2);
         M->hub_uid = G->uid;
     }
+    if (M->static_uid != G->uid) {
+        // graph-only layer-1 means: the gather over features with every group bit clear
+        hipStream_t st = mstream(M);
+        const int fg = std::max(1, ctx->cu_count * 8);
+        k_fill32<<<gridn(n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)n);
+        k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, M->gm, n, M->x);
+        aggregate<FIN, false>(M, G, M->x, M->ax_static, st);
+        HIPCHK(hipGetLastError());
+        M->static_uid = G->uid;
+    }
     return GST_OK;
 }
 
@@ -834,7 +916,9 @@ gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* gro
     HIPCHK(hipStreamSynchronize(st));   // h is a local buffer
     const int fg = std::max(1, ctx->cu_count * 8);
     k_fill32<<<gridn(G->n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)G->n);
-    k_set_bits<<<64, 256, 0, st>>>(M->gm, M->grp, M->grp + g + 1, g);   // features use bits 0..7
+    k_fill32<<<gridn((uint64_t)G->n * 8, 256, fg), 256, 0, st>>>((uint32_t*)M->gcnt, 0u, (uint64_t)G->n * 8);
+    k_set_bits_scatter<<<gridn((uint64_t)groups->grp_ptr[g] * 64, 256, fg), 256, 0, st>>>(
+        M->gm, M->grp, M->grp + g + 1, g, G->d_row, G->d_col, M->gcnt);   // features use bits 0..7
     return GST_OK;
 }
 
@@ -866,7 +950,7 @@ gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups) {
     const int fg = std::max(1, ctx->cu_count * 8);
     const int dg = std::max(1, ctx->cu_count * 8);
     k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, M->gm, n, M->x);
-    aggregate<FIN, false>(M, G, M->x, M->ax, st);
+    k_ax_finish<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, M->ax_static, M->gcnt, n, M->ax);
     k_dense_fwd<FIN><<<dg, 256, 0, st>>>(M->x, M->ax, M->W1t, M->p + OFF_B1, n, M->h1);
     aggregate<HID, false>(M, G, M->h1, M->ah1, st);
     k_dense_fwd<HID><<<dg, 256, 0, st>>>(M->h1, M->ah1, M->W2t, M->p + OFF_B2, n, M->h2);
@@ -918,6 +1002,7 @@ void gst_model_free(gst_model* M) {
     if (!M) return;
     hipFree(M->p); hipFree(M->g); hipFree(M->m); hipFree(M->v);
     hipFree(M->W1t); hipFree(M->W2t); hipFree(M->W1kn); hipFree(M->W2kn); hipFree(M->loss);
+    hipFree(M->ax_static); hipFree(M->gcnt);
     hipFree(M->x); hipFree(M->ax); hipFree(M->h1); hipFree(M->ah1); hipFree(M->h2); hipFree(M->dZ2); hipFree(M->dZ1);
     hipFree(M->s); hipFree(M->dS); hipFree(M->dX2); hipFree(M->dH1); hipFree(M->dA1); hipFree(M->hub_acc);
     hipFree(M->gm); hipFree(M->y); hipFree(M->grp); hipFree(M->hub_list); hipFree(M->hub_items);
@@ -1065,9 +1150,9 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     const int g = groups->n_groups;
     const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
     // scratch: histogram (2048) | select state (8) | sel (n) | newid (n) | deg (n) | arc offsets (n) |
-    // block sums 2 (nbk + 2) | sub group ids (gcnt)
+    // old ids by new id (n) | block sums 2 (nbk + 2) | sub group ids (gcnt)
     const uint32_t nbk = (n + 1023) / 1024;
-    const uint64_t need = 2048ull + 8 + 4ull * n + 2ull * (nbk + 2) + gcnt + 64;
+    const uint64_t need = 2048ull + 8 + 5ull * n + 2ull * (nbk + 2) + gcnt + 64;
     if (M->cap_scratch < need) {
         if (!dalloc(M->sel_scratch, need)) { M->cap_scratch = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
         M->cap_scratch = need;
@@ -1078,7 +1163,8 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     uint32_t* d_newid = d_sel + n;
     uint32_t* d_deg = d_newid + n;
     uint32_t* d_arcoff = d_deg + n;
-    uint32_t* d_bsum = d_arcoff + n;     // 2 * (nbk + 2)
+    uint32_t* d_old = d_arcoff + n;
+    uint32_t* d_bsum = d_old + n;        // 2 * (nbk + 2)
     int32_t* d_subgrp = (int32_t*)(d_bsum + 2 * (nbk + 2));
     // pinned readback: ns, ms, then the subgraph ids of the group members
     if (M->cap_h_sel < gcnt + 2) {
@@ -1103,7 +1189,8 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     k_scan_blocks<<<nbk, 256, 0, st>>>(d_sel, n, d_newid, d_bsum);
     k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nbk);
     k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_newid, n, d_bsum);
-    k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, n, d_deg);
+    k_sub_list<<<gridn(n, 256, fg), 256, 0, st>>>(d_sel, d_newid, n, d_old, d_deg);
+    k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, d_old, d_bsum + nbk, d_deg);
     k_scan_blocks<<<nbk, 256, 0, st>>>(d_deg, n, d_arcoff, d_bsum + nbk + 2);
     k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nbk + 2, nbk);
     k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_arcoff, n, d_bsum + nbk + 2);
@@ -1126,9 +1213,8 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     S.d_row = buf.row; S.d_col = buf.col; S.d_w = buf.w; S.d_rev = buf.rev;
     S.max_deg = 0;
     if (ns == 0) return GST_OK;
-    // old ids of the subgraph vertices are not needed: the bound is a cost only
-    k_sub_fill<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, n,
-                                                             S.d_row, S.d_col, S.d_w, d_deg /* scratch for old ids */);
+    k_sub_fill<<<gridn((uint64_t)ns * 64, 256, ctx->cu_count * 8), 256, 0, st>>>(
+        G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, d_old, ns, S.d_row, S.d_col, S.d_w);
     k_set_u32<<<1, 1, 0, st>>>(S.d_row + ns, d_bsum + nbk + 2 + nbk);
     k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S.d_row, S.d_col, S.d_w, ns, S.d_rev);
     HIPCHK(hipGetLastError());
