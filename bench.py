@@ -154,17 +154,17 @@ def main():
         cpu = {"value": round(len(sample) / cw, 4), "unit": "queries/s", "cores": nth, "kind": "port",
                "sample": f"{len(sample)} of the 64 queries, label-setting DPBF (oracle/dpbf_oracle.c), "
                          f"{nth} threads, {cw:.1f} s wall"}
-        # PrunedDP++-style bounded label-setting search (one-label lower bound, grow <= U/2) with
-        # the spider bound computed on the host, one thread, on 4 of the queries
+        # PrunedDP++-style bounded label-setting search (one-label and tour lower bounds, grow <= U/2)
+        # with the spider bound computed on the host, one thread, on 4 of the queries
         t4 = time.perf_counter()
         for qi in range(4):
             d = oracle.group_distances(G, queries[qi])
-            c, _ = oracle.dpbf_pruned(G, queries[qi], float(np.min(d.sum(axis=0))))
+            c, _ = oracle.dpbf_pruned(G, queries[qi], float(np.min(d.sum(axis=0))), 0.5, 2)
             assert c == plain[qi].cost, (qi, c, plain[qi].cost)
         pw = time.perf_counter() - t4
         cpu["pruned"] = {"value": round(4 / pw, 4), "unit": "queries/s", "cores": 1, "kind": "port",
                          "sample": f"4 of the 64 queries, bounded label-setting DPBF (oracle orc_dpbf_pruned, "
-                                   f"spider bound, one-label LB, half-bound grow), {pw:.1f} s wall"}
+                                   f"spider bound, one-label and tour LBs, half-bound grow), {pw:.1f} s wall"}
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "dpbf_learned_batch_ba200k_g6_traffic.json")

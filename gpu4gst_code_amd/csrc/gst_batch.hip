@@ -702,7 +702,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     gst_status st = ensure_slots(ctx, E, S, n_cap, g_cap, m_cap, chunk);
     if (st != GST_OK) return st;   // an engine with more slots is reused; its extra slots stay idle
     const gst_tuned_batch* tb = tune_uid ? gst_find_tuned_batch(ctx, tune_uid) : nullptr;
-    const int R = bo.rounds_per_step > 0 ? bo.rounds_per_step : (tb && tb->rounds_per_step > 0 ? tb->rounds_per_step : 4);
+    const int R = bo.rounds_per_step > 0 ? bo.rounds_per_step : (tb && tb->rounds_per_step > 0 ? tb->rounds_per_step : 8);
     const uint32_t tuned_lanes = tb ? (uint32_t)tb->push_lanes : 0u;
     const int32_t tuned_pull = tb ? tb->pull_permille : 0;
     const bool timed = bo.time_kernels != 0;

@@ -5,7 +5,8 @@
 // push rounds (push_lanes: a wave serves 64/push_lanes items, each item's masks spread over its lanes)
 // and the merge tile (merge_tile_kb: the LDS budget that sets the vertices per merge workgroup).
 // Coordinate descent from the built-in defaults, two sweeps, a change kept only when it is at
-// least 2% faster.  Batched engine: rounds per step and push lanes, on the whole sample.
+// least 2% faster.  Batched engine: push lanes, then rounds per step, timed as a 64-slot stream
+// over the sample.
 // The choice is stored in the context and used by every later solve on the graph that leaves the
 // knob at 0.  No counterpart in the reference (its launch shapes are fixed, src/DPQ.cu:597-610).
 #include <algorithm>
@@ -168,19 +169,35 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             tried++;
             return GST_OK;
         };
-        int32_t bR = 4, bL = 16;
+        // lanes first at the default rounds per step (the larger effect), then rounds per step at
+        // the chosen lanes; a change must win by 3% (lanes) or 5% (rounds), so that timing noise
+        // between near-equal settings leaves the defaults in place
+        int32_t bR = 8, bL = 16;
         double bq = 0.0;
         s = beval(bR, bL, &bq);
         if (s != GST_OK) return s;
         const double bdef = bq;
-        for (int32_t R : {2, 4, 8}) {
-            for (int32_t lanes : {8, 16, 32}) {
-                if (R == 4 && lanes == 16) continue;
+        {
+            int32_t L0 = bL;
+            double q0 = bq;
+            for (int32_t lanes : {8, 32}) {
                 double q = 0.0;
-                s = beval(R, lanes, &q);
+                s = beval(bR, lanes, &q);
                 if (s != GST_OK) return s;
-                if (q > bq * 1.02) { bq = q; bR = R; bL = lanes; }
+                if (q > q0 * 1.03) { q0 = q; L0 = lanes; }
             }
+            bL = L0; bq = q0;
+        }
+        {
+            int32_t R0 = bR;
+            double q0 = bq;
+            for (int32_t R : {4, 16}) {
+                double q = 0.0;
+                s = beval(R, bL, &q);
+                if (s != GST_OK) return s;
+                if (q > q0 * 1.05) { q0 = q; R0 = R; }
+            }
+            bR = R0; bq = q0;
         }
         gst_tuned_batch tb;
         tb.uid = G->uid; tb.rounds_per_step = bR; tb.push_lanes = bL;
