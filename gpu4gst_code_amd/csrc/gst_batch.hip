@@ -644,8 +644,18 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
                          const hipEvent_t* ev = nullptr) {
     hipStream_t s = (hipStream_t)ctx->stream;
     const int mgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 8);
-    // one LDS size for every (g, level) a slot may merge: tile rows, bitmaps and split table
-    const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + MERGE_PAIRS_LDS;
+    // one LDS size for every (g <= g_cap, level) a slot may merge: tile rows, bitmaps, and the
+    // largest split table that is staged in LDS (sized for the engine's group count, so that small
+    // group counts leave room for more workgroups per CU)
+    size_t pairs_lds = 0;
+    for (int gg = 2; gg <= E->g_cap; ++gg) {
+        const gst_levels& L = gst_get_levels(ctx, gg);
+        for (int k = 2; k <= gg; ++k) {
+            const size_t pb = (size_t)L.pair_cnt[k] * 8;
+            if (pb <= MERGE_PAIRS_LDS) pairs_lds = std::max(pairs_lds, pb);
+        }
+    }
+    const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + pairs_lds;
     k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
     const size_t tl = tour_lds_bytes((uint32_t)E->g_cap);
     if (tl) {

@@ -23,7 +23,7 @@ def test_autotune_then_exact(solver, bounded):
     assert rep["chunk_arcs"] in (32, 16, 64)
     assert rep["push_lanes"] in (64, 16, 32)
     assert rep["merge_tile_kb"] in (16, 8, 32, 48)
-    assert rep["batch_rounds_per_step"] in (2, 4, 8)
+    assert rep["batch_rounds_per_step"] in (4, 8, 16)
     assert rep["batch_push_lanes"] in (8, 16, 32)
     assert rep["tuned_ms"] <= rep["default_ms"]
     assert rep["batch_tuned_qps"] >= rep["batch_default_qps"]
