@@ -64,13 +64,19 @@ struct BatchPlan {
     uint32_t compact_slot[MAX_SLOTS];
     uint32_t compact_level[MAX_SLOTS];
     uint32_t trace_slot[MAX_SLOTS];
+    SlotDev sd_out[MAX_SLOTS];             // the slots' phases after the step's plan (read by the host)
+};
+
+// a slot's argument blocks, contiguous so that filling a slot is one copy
+struct SlotBlock {
+    GrowArgs g[NCTL];      // by level
+    MergeArgs m[NCTL];     // by level
+    TraceArgs t;
+    SlotDev sd;
 };
 
 struct BatchArgs {
-    SlotDev* slots;
-    const GrowArgs* gargs;    // [slot * NCTL + level]
-    const MergeArgs* margs;   // [slot * NCTL + level]
-    const TraceArgs* targs;   // [slot]
+    SlotBlock* sb;            // [slot]
     BatchPlan* plan;
     uint32_t S;               // slots (<= MAX_SLOTS)
     uint32_t grid;            // workgroups a grow round may use in total
@@ -98,17 +104,21 @@ __device__ __forceinline__ uint32_t plan_scan(uint32_t x, uint32_t& tot) {
     return off + incl - x;
 }
 
+__global__ void k_reset_phases(SlotBlock* sb) {
+    if (threadIdx.x < MAX_SLOTS) sb[threadIdx.x].sd = SlotDev{PH_EMPTY, 0u, 0u, 0u};
+}
+
 __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
     const uint32_t s = threadIdx.x;
     bool dm = false, dc = false, dt = false;
     uint32_t lvl = 0;
     if (s < b.S) {
-        SlotDev* sd = &b.slots[s];
+        SlotDev* sd = &b.sb[s].sd;
         const uint32_t ph = sd->phase;
         if (ph == PH_TRACE) {
             dt = true;
             sd->phase = PH_DONE;
-        } else if (ph == PH_GROW && b.gargs[s * NCTL + sd->k].ctl->level_done) {
+        } else if (ph == PH_GROW && b.sb[s].g[sd->k].ctl->level_done) {
             lvl = sd->k + 1;
             dm = true;
             if (lvl == sd->g) {
@@ -127,6 +137,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
     if (dm) { p->merge_slot[im] = s; p->merge_level[im] = lvl; }
     if (dc) { p->compact_slot[ic] = s; p->compact_level[ic] = lvl; }
     if (dt) p->trace_slot[it] = s;
+    if (s < b.S) p->sd_out[s] = b.sb[s].sd;
     if (s == 0) {
         p->n_merge = nm;
         p->n_compact = nc;
@@ -143,7 +154,7 @@ __global__ void __launch_bounds__(MERGE_THREADS) k_batch_merge(BatchArgs b) {
     const uint32_t per = max(1u, gridDim.x / nm);   // gridDim.x >= MAX_SLOTS, so per >= 1 covers all
     const uint32_t i = blockIdx.x / per;
     if (i >= nm) return;
-    merge_body<FLT>(b.margs[p->merge_slot[i] * NCTL + p->merge_level[i]], lds, blockIdx.x - i * per, per);
+    merge_body<FLT>(b.sb[p->merge_slot[i]].m[p->merge_level[i]], lds, blockIdx.x - i * per, per);
 }
 
 // after the merges: a slot whose level-2 merge found a spider (automatic bound) tightens the bound
@@ -154,7 +165,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
     const uint32_t i = threadIdx.x;
     if (i >= p->n_merge || p->merge_level[i] != 2) return;
     const uint32_t s = p->merge_slot[i];
-    const MergeArgs& m2 = b.margs[s * NCTL + 2];
+    const MergeArgs& m2 = b.sb[s].m[2];
     if (!m2.spider) return;
     const uint32_t sp = *m2.spider;
     if (sp == GST_INF_COST) return;
@@ -163,12 +174,12 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
     const uint32_t g = m2.g;
     for (uint32_t k = 2; k <= g; ++k) {
         if (k < g) {
-            GrowArgs* ga = const_cast<GrowArgs*>(&b.gargs[s * NCTL + k]);
+            GrowArgs* ga = &b.sb[s].g[k];
             if (ub < ga->ub) ga->ub = ub;
             if (gmax < ga->gmax) ga->gmax = gmax;
         }
         if (k >= 3) {
-            MergeArgs* ma = const_cast<MergeArgs*>(&b.margs[s * NCTL + k]);
+            MergeArgs* ma = &b.sb[s].m[k];
             if (ub < ma->ub) ma->ub = ub;
             if (gmax < ma->gmax) ma->gmax = gmax;
         }
@@ -182,7 +193,7 @@ __global__ void __launch_bounds__(256) k_batch_tour(BatchArgs b) {
     const BatchPlan* p = b.plan;
     const uint32_t i = blockIdx.x;
     if (i >= p->n_merge || p->merge_level[i] != 2) return;
-    const MergeArgs& m2 = b.margs[p->merge_slot[i] * NCTL + 2];
+    const MergeArgs& m2 = b.sb[p->merge_slot[i]].m[2];
     if (!m2.tour) return;
     // the launch's LDS holds the walk table of the engine's largest group count
     tour_table_body<FLT, PW_LDS>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
@@ -196,23 +207,23 @@ __global__ void __launch_bounds__(256) k_batch_compact(BatchArgs b) {
     const uint32_t per = max(1u, gridDim.x / nc);
     const uint32_t i = blockIdx.x / per;
     if (i >= nc) return;
-    const GrowArgs& a = b.gargs[p->compact_slot[i] * NCTL + p->compact_level[i]];
+    const GrowArgs& a = b.sb[p->compact_slot[i]].g[p->compact_level[i]];
     compact_body(a.inq[0], a.row, a.n, a.chunk, a.list[0], &a.ctl->cnt[0], a.stats, 0u, blockIdx.x - i * per, per);
 }
 
 __global__ void __launch_bounds__(WAVE) k_batch_trace(BatchArgs b) {
     const BatchPlan* p = b.plan;
     if (blockIdx.x >= p->n_trace) return;
-    trace_body(b.targs[p->trace_slot[blockIdx.x]]);
+    trace_body(b.sb[p->trace_slot[blockIdx.x]].t);
 }
 
 __global__ void __launch_bounds__(PLAN_THREADS) k_plan_grow(BatchArgs b) {
     const uint32_t s = threadIdx.x;
     uint32_t need = 0, lvl = 0;
     if (s < b.S) {
-        const SlotDev sd = b.slots[s];
+        const SlotDev sd = b.sb[s].sd;
         if (sd.phase == PH_GROW) {
-            const GrowArgs& a = b.gargs[s * NCTL + sd.k];
+            const GrowArgs& a = b.sb[s].g[sd.k];
             const RoundCtl* c = a.ctl;
             if (!c->level_done) {
                 const uint32_t r = c->round;
@@ -267,7 +278,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu
         if (p->grow_start[mid] <= bid) lo = mid; else hi = mid;
     }
     const uint32_t st = p->grow_start[lo];
-    grow_round_body<FLT>(b.gargs[p->grow_slot[lo] * NCTL + p->grow_level[lo]], bid - st, p->grow_start[lo + 1] - st,
+    grow_round_body<FLT>(b.sb[p->grow_slot[lo]].g[p->grow_level[lo]], bid - st, p->grow_start[lo + 1] - st,
                          p->seq);
 }
 
@@ -293,10 +304,14 @@ struct gst_batch_engine {
         uint64_t* chg[3] = {nullptr, nullptr, nullptr};
         uint32_t* inq[3] = {nullptr, nullptr, nullptr};
         uint64_t* lists[3] = {nullptr, nullptr, nullptr};
+        // readback block (one copy per finished query): root | ctl[NCTL] | stats | edge count |
+        // edges as (u, v, w) triples, ecap of them
+        unsigned char* back = nullptr;
         RoundCtl* ctl = nullptr;
         unsigned long long* root = nullptr;
         unsigned long long* stats = nullptr;
-        uint32_t* edges = nullptr;          // 3 * ecap + 1
+        uint32_t* out_n = nullptr;
+        uint32_t* edges = nullptr;
         uint32_t* spider = nullptr;         // cheapest spider of the level-2 merge (automatic bound)
         uint32_t* tour = nullptr;           // tour table 2^g * (g + 1) words, then walk scratch 2^g * g * g
         int tour_g = 0;
@@ -306,12 +321,9 @@ struct gst_batch_engine {
         size_t grp_cap = 0;
         // pinned host staging (args written at fill) and readback
         int32_t* h_grp = nullptr;
-        GrowArgs* h_gargs = nullptr;        // NCTL
-        MergeArgs* h_margs = nullptr;       // NCTL
-        TraceArgs* h_targs = nullptr;
-        SlotDev* h_sd = nullptr;
-        unsigned char* h_back = nullptr;    // root | ctl[NCTL] | stats | out_n
-        uint32_t* h_edges = nullptr;        // 3 * ecap
+        SlotBlock* h_sb = nullptr;          // argument blocks
+        unsigned char* h_back = nullptr;    // header and the first EDGE_PREFIX edges of back
+        uint32_t* h_edges = nullptr;        // 3 * ecap (trees longer than the prefix)
         // host bookkeeping
         bool busy = false;
         gst_batch_job job;
@@ -325,12 +337,9 @@ struct gst_batch_engine {
     uint32_t W_cap = 1;
     uint32_t ecap = 0;
     std::vector<Slot> slots;
-    SlotDev* d_slots = nullptr;
-    GrowArgs* d_gargs = nullptr;
-    MergeArgs* d_margs = nullptr;
-    TraceArgs* d_targs = nullptr;
+    SlotBlock* d_sb = nullptr;              // [MAX_SLOTS]
     BatchPlan* d_plan = nullptr;
-    SlotDev* h_slots = nullptr;             // pinned phase snapshot
+    SlotDev* h_slots = nullptr;             // pinned phase snapshot (the plan's sd_out)
     // level tables per group count
     uint64_t* pairs[GST_MAX_GROUPS + 1] = {nullptr};
     uint32_t* masks[GST_MAX_GROUPS + 1] = {nullptr};
@@ -349,18 +358,21 @@ struct gst_batch_engine {
     int g_S = -1, g_R = -1, g_flt = -1;
 };
 
-static size_t back_bytes() { return 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT + 8; }
+// layout of a slot's readback block
+constexpr size_t BK_CTL = 8;
+constexpr size_t BK_STATS = (BK_CTL + sizeof(RoundCtl) * NCTL + 7) / 8 * 8;
+constexpr size_t BK_OUTN = BK_STATS + 8 * ST_NSTAT;
+constexpr size_t BK_EDGES = BK_OUTN + 8;
+constexpr uint32_t EDGE_PREFIX = 1024;   // edges read back with the header (12 KiB)
+static size_t back_bytes(uint32_t ecap) { return BK_EDGES + 12 * (size_t)ecap; }
 
 static void free_slot(gst_batch_engine::Slot& sl) {
     hipFree(sl.T); hipFree(sl.gm);
     for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
-    hipFree(sl.ctl); hipFree(sl.root); hipFree(sl.stats); hipFree(sl.edges); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour);
+    hipFree(sl.back); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour);
     hipFree(sl.hub_items); hipFree(sl.hub_cnt);
     if (sl.h_grp) hipHostFree(sl.h_grp);
-    if (sl.h_gargs) hipHostFree(sl.h_gargs);
-    if (sl.h_margs) hipHostFree(sl.h_margs);
-    if (sl.h_targs) hipHostFree(sl.h_targs);
-    if (sl.h_sd) hipHostFree(sl.h_sd);
+    if (sl.h_sb) hipHostFree(sl.h_sb);
     if (sl.h_back) hipHostFree(sl.h_back);
     if (sl.h_edges) hipHostFree(sl.h_edges);
     sl = gst_batch_engine::Slot();
@@ -369,8 +381,8 @@ static void free_slot(gst_batch_engine::Slot& sl) {
 static void free_engine_slots(gst_batch_engine* E) {
     for (auto& sl : E->slots) free_slot(sl);
     E->slots.clear();
-    hipFree(E->d_slots); hipFree(E->d_gargs); hipFree(E->d_margs); hipFree(E->d_targs); hipFree(E->d_plan);
-    E->d_slots = nullptr; E->d_gargs = nullptr; E->d_margs = nullptr; E->d_targs = nullptr; E->d_plan = nullptr;
+    hipFree(E->d_sb); hipFree(E->d_plan);
+    E->d_sb = nullptr; E->d_plan = nullptr;
     if (E->h_slots) hipHostFree(E->h_slots);
     E->h_slots = nullptr;
     if (E->gexec) hipGraphExecDestroy(E->gexec);
@@ -410,8 +422,7 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
     free_engine_slots(E);
     E->n_cap = n; E->g_cap = g; E->items_cap = items_cap; E->W_cap = W; E->ecap = ecap;
     E->slots.resize((size_t)S);
-    bool ok = dalloc(E->d_slots, MAX_SLOTS) && dalloc(E->d_gargs, (size_t)MAX_SLOTS * NCTL) &&
-              dalloc(E->d_margs, (size_t)MAX_SLOTS * NCTL) && dalloc(E->d_targs, MAX_SLOTS) && dalloc(E->d_plan, 1) &&
+    bool ok = dalloc(E->d_sb, MAX_SLOTS) && dalloc(E->d_plan, 1) &&
               hipHostMalloc((void**)&E->h_slots, sizeof(SlotDev) * MAX_SLOTS) == hipSuccess;
     const size_t words = (size_t)n * (((size_t)1 << g) - 1);
     for (int i = 0; ok && i < S; ++i) {
@@ -419,14 +430,17 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
         ok = ok && dalloc(sl.T, words) && dalloc(sl.gm, n);
         for (int j = 0; ok && j < 3; ++j)
             ok = dalloc(sl.chg[j], (size_t)n * W) && dalloc(sl.inq[j], n) && dalloc(sl.lists[j], items_cap);
-        ok = ok && dalloc(sl.ctl, NCTL) && dalloc(sl.root, 1) && dalloc(sl.stats, ST_NSTAT) &&
-             dalloc(sl.edges, 3 * (size_t)ecap + 1) && dalloc(sl.spider, 1) &&
+        ok = ok && dalloc(sl.back, back_bytes(ecap)) && dalloc(sl.spider, 1) &&
              dalloc(sl.hub_items, items_cap) && dalloc(sl.hub_cnt, 1);
-        ok = ok && hipHostMalloc((void**)&sl.h_gargs, sizeof(GrowArgs) * NCTL) == hipSuccess &&
-             hipHostMalloc((void**)&sl.h_margs, sizeof(MergeArgs) * NCTL) == hipSuccess &&
-             hipHostMalloc((void**)&sl.h_targs, sizeof(TraceArgs)) == hipSuccess &&
-             hipHostMalloc((void**)&sl.h_sd, sizeof(SlotDev)) == hipSuccess &&
-             hipHostMalloc((void**)&sl.h_back, back_bytes()) == hipSuccess &&
+        if (ok) {
+            sl.root = (unsigned long long*)sl.back;
+            sl.ctl = (RoundCtl*)(sl.back + BK_CTL);
+            sl.stats = (unsigned long long*)(sl.back + BK_STATS);
+            sl.out_n = (uint32_t*)(sl.back + BK_OUTN);
+            sl.edges = (uint32_t*)(sl.back + BK_EDGES);
+        }
+        ok = ok && hipHostMalloc((void**)&sl.h_sb, sizeof(SlotBlock)) == hipSuccess &&
+             hipHostMalloc((void**)&sl.h_back, back_bytes(std::min(ecap, EDGE_PREFIX))) == hipSuccess &&
              hipHostMalloc((void**)&sl.h_edges, sizeof(uint32_t) * 3 * (size_t)ecap) == hipSuccess;
         if (ok) {
             const int fg = std::max(1, ctx->cu_count * 4);
@@ -440,7 +454,7 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
         free_engine_slots(E);
         return gst_set_error(ctx, GST_OOM, "batch: device allocation of %d slots failed (V=%u, g=%d)", S, n, g);
     }
-    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)E->d_slots, 0u, sizeof(SlotDev) * MAX_SLOTS / 4);
+    k_fill_u32<<<64, 256, 0, s>>>((uint32_t*)E->d_sb, 0u, sizeof(SlotBlock) * MAX_SLOTS / 4);
     k_fill_u32<<<4, 256, 0, s>>>((uint32_t*)E->d_plan, 0u, sizeof(BatchPlan) / 4);
     E->S = S;
     HIPCHK(hipGetLastError());
@@ -559,7 +573,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
 
     // argument blocks
     for (int k = 1; k < g; ++k) {
-        GrowArgs& a = sl.h_gargs[k];
+        GrowArgs& a = sl.h_sb->g[k];
         std::memset(&a, 0, sizeof(a));
         a.Tk = sl.T + lvl_base[k];
         a.T1 = sl.T + lvl_base[1];
@@ -595,7 +609,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
                                                : (tuned_lanes ? tuned_lanes : default_push_lanes(true));
     }
     for (int k = 2; k <= g; ++k) {
-        MergeArgs& ma = sl.h_margs[k];
+        MergeArgs& ma = sl.h_sb->m[k];
         std::memset(&ma, 0, sizeof(ma));
         ma.T = sl.T; ma.pairs = E->pairs[g] + L.pair_off[k]; ma.masks = E->masks[g]; ma.gm = sl.gm; ma.row = G->d_row;
         ma.list0 = sl.lists[0]; ma.chg0 = sl.chg[0]; ma.inq0 = sl.inq[0]; ma.ctl = sl.ctl + k; ma.root = sl.root;
@@ -614,23 +628,20 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         ma.pairs_in_lds = ((size_t)L.pair_cnt[k] * 8 <= MERGE_PAIRS_LDS) ? 1u : 0u;
     }
     {
-        TraceArgs& ta = *sl.h_targs;
+        TraceArgs& ta = sl.h_sb->t;
         std::memset(&ta, 0, sizeof(ta));
         ta.T = sl.T; ta.pos_of = E->pos_of[g]; ta.col = G->d_col; ta.w = G->d_w; ta.root = sl.root;
-        ta.out_u = sl.edges; ta.out_v = sl.edges + E->ecap; ta.out_w = sl.edges + 2 * (size_t)E->ecap;
-        ta.out_n = sl.edges + 3 * (size_t)E->ecap;
+        ta.out_u = sl.edges; ta.out_v = sl.edges + 1; ta.out_w = sl.edges + 2; ta.ostride = 3;
+        ta.out_n = sl.out_n;
         ta.ctl = sl.ctl;
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ta.lvl_base[j] = lvl_base[j]; ta.m[j] = (uint32_t)L.m[j]; ta.off[j] = (uint32_t)L.off[j]; }
         ta.full = (uint32_t)L.full; ta.cap = E->ecap; ta.max_iter = n + 4u * (uint32_t)g + 64u;
     }
-    sl.h_sd->phase = g > 1 ? PH_GROW : PH_TRACE;
-    sl.h_sd->k = 1;
-    sl.h_sd->g = (uint32_t)g;
-    sl.h_sd->pad = 0;
-    HIPCHK(hipMemcpyAsync(E->d_gargs + (size_t)si * NCTL, sl.h_gargs, sizeof(GrowArgs) * NCTL, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(E->d_margs + (size_t)si * NCTL, sl.h_margs, sizeof(MergeArgs) * NCTL, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(E->d_targs + si, sl.h_targs, sizeof(TraceArgs), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(E->d_slots + si, sl.h_sd, sizeof(SlotDev), hipMemcpyHostToDevice, s));
+    sl.h_sb->sd.phase = g > 1 ? PH_GROW : PH_TRACE;
+    sl.h_sb->sd.k = 1;
+    sl.h_sb->sd.g = (uint32_t)g;
+    sl.h_sb->sd.pad = 0;
+    HIPCHK(hipMemcpyAsync(E->d_sb + si, sl.h_sb, sizeof(SlotBlock), hipMemcpyHostToDevice, s));
     HIPCHK(hipGetLastError());
     sl.busy = true;
     sl.g = g;
@@ -730,10 +741,10 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     } ev_free{tev};
 
     BatchArgs ba;
-    ba.slots = E->d_slots; ba.gargs = E->d_gargs; ba.margs = E->d_margs; ba.targs = E->d_targs; ba.plan = E->d_plan;
+    ba.sb = E->d_sb; ba.plan = E->d_plan;
     ba.S = (uint32_t)S;
     ba.grid = (uint32_t)std::max(1, std::min(ctx->cu_count * 5, 64 * 32 - (int)MAX_SLOTS));
-    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)E->d_slots, 0u, sizeof(SlotDev) * MAX_SLOTS / 4);
+    k_reset_phases<<<1, MAX_SLOTS, 0, s>>>(E->d_sb);
 
     std::vector<double> lat;
     uint64_t steps = 0, n_done = 0, relax = 0, gbytes = 0, mbytes = 0;
@@ -795,7 +806,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         else enqueue_step(ctx, E, ba, R, flt, timed ? tev.data() : nullptr);
         HIPCHK(hipGetLastError());
         steps++;
-        HIPCHK(hipMemcpyAsync(E->h_slots, E->d_slots, sizeof(SlotDev) * (size_t)S, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(E->h_slots, E->d_plan->sd_out, sizeof(SlotDev) * (size_t)S, hipMemcpyDeviceToHost, s));
         if (idle) idle();
         HIPCHK(hipStreamSynchronize(s));
         if (timed) {
@@ -814,40 +825,38 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         for (int i = 0; i < S; ++i)
             if (E->slots[(size_t)i].busy && E->h_slots[i].phase == PH_DONE) fin.push_back(i);
         if (fin.empty()) continue;
+        // one copy per finished query: the readback block's header, plus the first EDGE_PREFIX
+        // edges when the tree is wanted; longer trees take a second copy
+        const uint32_t pre = std::min(E->ecap, EDGE_PREFIX);
         for (int i : fin) {
             auto& sl = E->slots[(size_t)i];
-            unsigned char* hb = sl.h_back;
-            HIPCHK(hipMemcpyAsync(hb, sl.root, 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(hb + 8, sl.ctl, sizeof(RoundCtl) * NCTL, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(hb + 8 + sizeof(RoundCtl) * NCTL, sl.stats, 8 * ST_NSTAT, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(hb + 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT, sl.edges + 3 * (size_t)E->ecap, 4,
-                                  hipMemcpyDeviceToHost, s));
+            const size_t nb = sl.job.opts.want_tree ? back_bytes(pre) : BK_EDGES;
+            HIPCHK(hipMemcpyAsync(sl.h_back, sl.back, nb, hipMemcpyDeviceToHost, s));
         }
         HIPCHK(hipStreamSynchronize(s));
+        bool more = false;
         for (int i : fin) {
             auto& sl = E->slots[(size_t)i];
-            const unsigned char* hb = sl.h_back;
             uint32_t nout = 0;
-            std::memcpy(&nout, hb + 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT, 4);
-            if (sl.job.opts.want_tree && nout > 0 && nout <= E->ecap) {
-                HIPCHK(hipMemcpyAsync(sl.h_edges, sl.edges, nout * 4ull, hipMemcpyDeviceToHost, s));
-                HIPCHK(hipMemcpyAsync(sl.h_edges + E->ecap, sl.edges + E->ecap, nout * 4ull, hipMemcpyDeviceToHost, s));
-                HIPCHK(hipMemcpyAsync(sl.h_edges + 2 * (size_t)E->ecap, sl.edges + 2 * (size_t)E->ecap, nout * 4ull,
-                                      hipMemcpyDeviceToHost, s));
+            std::memcpy(&nout, sl.h_back + BK_OUTN, 4);
+            if (sl.job.opts.want_tree && nout > pre && nout <= E->ecap) {
+                HIPCHK(hipMemcpyAsync(sl.h_edges, sl.edges, 12ull * nout, hipMemcpyDeviceToHost, s));
+                more = true;
             }
         }
-        HIPCHK(hipStreamSynchronize(s));
+        if (more) HIPCHK(hipStreamSynchronize(s));
+        std::vector<uint32_t> eu, ev, ew;
         for (int i : fin) {
             auto& sl = E->slots[(size_t)i];
             gst_batch_job& job = sl.job;
             const unsigned char* hb = sl.h_back;
             unsigned long long rw;
             std::memcpy(&rw, hb, 8);
-            const RoundCtl* c = (const RoundCtl*)(hb + 8);
+            const RoundCtl* c = (const RoundCtl*)(hb + BK_CTL);
             unsigned long long hs[ST_NSTAT];
-            std::memcpy(hs, hb + 8 + sizeof(RoundCtl) * NCTL, sizeof(hs));
+            std::memcpy(hs, hb + BK_STATS, sizeof(hs));
             uint32_t nout = 0;
-            std::memcpy(&nout, hb + 8 + sizeof(RoundCtl) * NCTL + 8 * ST_NSTAT, 4);
+            std::memcpy(&nout, hb + BK_OUTN, 4);
             uint32_t err = 0, rounds = 0;
             for (int k = 0; k <= sl.g; ++k) err |= c[k].err;
             for (int k = 1; k < sl.g; ++k) rounds += c[k].round + 1;
@@ -882,8 +891,13 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
                 if (sl.flt) { float f; std::memcpy(&f, &cb, 4); res->cost = (double)f; }
                 else res->cost = (double)cb;
                 res->root = (int32_t)(rw & 0xFFFFFFFFu);
-                if (job.opts.want_tree)
-                    gst_finish_tree(sl.flt, nout, sl.h_edges, sl.h_edges + E->ecap, sl.h_edges + 2 * (size_t)E->ecap, res);
+                if (job.opts.want_tree) {
+                    // edges come as (u, v, w) triples: from the prefix, or from the second copy
+                    const uint32_t* tri = nout <= pre ? (const uint32_t*)(hb + BK_EDGES) : sl.h_edges;
+                    eu.resize(nout); ev.resize(nout); ew.resize(nout);
+                    for (uint32_t e = 0; e < nout; ++e) { eu[e] = tri[3 * e]; ev[e] = tri[3 * e + 1]; ew[e] = tri[3 * e + 2]; }
+                    gst_finish_tree(sl.flt, nout, eu.data(), ev.data(), ew.data(), res);
+                }
                 job.status = GST_OK;
             }
             if (job.status != GST_OK && first_err == GST_OK) first_err = job.status;
@@ -905,8 +919,8 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
             st = try_fill(i);
             if (st != GST_OK) return st;
             if (!sl.busy) {
-                sl.h_sd->phase = PH_EMPTY;
-                HIPCHK(hipMemcpyAsync(E->d_slots + i, sl.h_sd, sizeof(SlotDev), hipMemcpyHostToDevice, s));
+                sl.h_sb->sd.phase = PH_EMPTY;
+                HIPCHK(hipMemcpyAsync(&E->d_sb[i].sd, &sl.h_sb->sd, sizeof(SlotDev), hipMemcpyHostToDevice, s));
             }
         }
     }

@@ -698,6 +698,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         TraceArgs ta;
         ta.T = B->T; ta.pos_of = B->pos_of; ta.col = G->d_col; ta.w = G->d_w; ta.root = B->root;
         ta.out_u = B->edges; ta.out_v = B->edges + ecap; ta.out_w = B->edges + 2 * ecap; ta.out_n = B->edges + 3 * ecap;
+        ta.ostride = 1;
         ta.ctl = B->ctl;   // entry 0 carries query-level flags
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ta.lvl_base[j] = lvl_base[j]; ta.m[j] = (uint32_t)L.m[j]; ta.off[j] = (uint32_t)L.off[j]; }
         ta.full = (uint32_t)L.full; ta.cap = ecap; ta.max_iter = n + 4u * (uint32_t)g + 64u;

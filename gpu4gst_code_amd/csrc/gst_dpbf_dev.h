@@ -1549,6 +1549,7 @@ struct TraceArgs {
     uint32_t full;
     uint32_t cap;             // capacity of the edge output
     uint32_t max_iter;
+    uint32_t ostride;         // edge i is out_u[i * ostride], out_v[i * ostride], out_w[i * ostride]
 };
 
 __device__ __forceinline__ void trace_body(const TraceArgs& a) {
@@ -1620,7 +1621,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a) {
             }
             if (emit) {
                 const uint32_t o = atomicAdd(&nout_s, 1u);
-                if (o < a.cap) { a.out_u[o] = eu; a.out_v[o] = ev; a.out_w[o] = ew; }
+                if (o < a.cap) {
+                    const uint64_t oo = (uint64_t)o * a.ostride;
+                    a.out_u[oo] = eu; a.out_v[oo] = ev; a.out_w[oo] = ew;
+                }
                 else fail_s = 1;
             }
         }

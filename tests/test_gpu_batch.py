@@ -105,3 +105,21 @@ def test_zero_weights_batch(solver):
         assert r.cost == oracle.dpbf_cost(G, q)
         _valid(G, q, r)
     dg.free()
+
+
+def test_long_trees_batch(solver):
+    """Trees longer than the edges read back with a finished query's header (1024) take a second
+    copy: paths of 3,000 and 1,500 edges next to short trees in the same batch."""
+    n = 3000
+    src = np.arange(n - 1)
+    G = gst.HostGraph.from_edges(n, src, src + 1, np.ones(n - 1, np.int32))
+    dg = solver.upload(G)
+    qs = [gst.groups_from_lists([[0], [n - 1]]), gst.groups_from_lists([[10], [1510], [700]]),
+          gst.groups_from_lists([[5], [9]]), gst.groups_from_lists([[100], [101], [102]])]
+    want = [n - 1, 1500, 4, 2]
+    res, _ = solver.solve_batch(dg, qs, slots=2)
+    for q, r, w in zip(qs, res, want):
+        assert r.cost == w
+        assert len(r.eu) == w
+        assert gst.validate_tree(G, q, r.eu, r.ev, r.root) == w
+    dg.free()
