@@ -1301,8 +1301,17 @@ __global__ void k_async_begin(const RoundCtl* ctl, AsyncCtl* ac) {
 // then a coalesced store of the level-k rows.  No atomics except the LDS bitmap of finite masks.
 // ---------------------------------------------------------------------------------------
 constexpr int MERGE_THREADS = 256;
-// default LDS budget of a merge tile's cost rows: small enough for several workgroups per CU
-constexpr size_t MERGE_TILE_BYTES = 16 * 1024;
+// default LDS budget of a merge tile's cost rows (measured on BA V=200k g=6: 32 KiB beats 8, 16,
+// 48 and 64; four workgroups per CU with tiles of 2x the vertices of 16 KiB)
+#ifndef GST_MERGE_TILE_KB
+#define GST_MERGE_TILE_KB 32
+#endif
+constexpr size_t MERGE_TILE_BYTES = GST_MERGE_TILE_KB * 1024;
+// loads in flight per thread while a merge tile's rows are staged
+#ifndef GST_MERGE_UNR
+#define GST_MERGE_UNR 4
+#endif
+constexpr int MERGE_UNR = GST_MERGE_UNR;
 // split tables up to this size are staged in LDS
 constexpr size_t MERGE_PAIRS_LDS = 16 * 1024;
 
@@ -1372,22 +1381,22 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
         const uint32_t nv = min(VB, a.n - v0);
         __syncthreads();
         // stage the cost halves of the lower-level runs (each level's run of the tile is one
-        // contiguous block of nv * m_j words), four loads in flight per thread
+        // contiguous block of nv * m_j words), MERGE_UNR loads in flight per thread
         for (uint32_t j = 1; j < k; ++j) {
             const uint32_t mj = a.m[j];
             const float inv_mj = 1.0f / (float)mj;
             const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v0 * mj;
             const uint32_t cnt = nv * mj;
             const uint32_t oj = a.off[j];
-            for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += 4 * blockDim.x) {
-                uint64_t x[4];
+            for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += MERGE_UNR * blockDim.x) {
+                uint64_t x[MERGE_UNR];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < MERGE_UNR; ++u) {
                     const uint32_t i = i0 + u * blockDim.x;
                     x[u] = i < cnt ? src[i] : 0ull;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < MERGE_UNR; ++u) {
                     const uint32_t i = i0 + u * blockDim.x;
                     if (i < cnt) {
                         const uint32_t b = div_small(i, inv_mj), t = i - b * mj;
