@@ -42,6 +42,10 @@
 namespace {
 
 constexpr uint32_t MAX_SLOTS = 256;
+// workgroups a batched grow round may use, per CU
+#ifndef GST_BATCH_GRID_MULT
+#define GST_BATCH_GRID_MULT 5
+#endif
 constexpr uint32_t PLAN_THREADS = MAX_SLOTS;          // the plan kernels run one thread per slot
 constexpr uint32_t PLAN_WAVES = PLAN_THREADS / WAVE;
 enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3 };
@@ -743,7 +747,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     BatchArgs ba;
     ba.sb = E->d_sb; ba.plan = E->d_plan;
     ba.S = (uint32_t)S;
-    ba.grid = (uint32_t)std::max(1, std::min(ctx->cu_count * 5, 64 * 32 - (int)MAX_SLOTS));
+    ba.grid = (uint32_t)std::max(1, std::min(ctx->cu_count * GST_BATCH_GRID_MULT, 64 * 32 - (int)MAX_SLOTS));
     k_reset_phases<<<1, MAX_SLOTS, 0, s>>>(E->d_sb);
 
     std::vector<double> lat;
