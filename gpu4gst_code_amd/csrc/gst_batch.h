@@ -25,10 +25,11 @@ struct gst_batch_job {
 // and result set) and may make more jobs available to next_job, whose slot is refilled right after.  n_cap / g_cap / m_cap bound every job's graph size and
 // group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
 // tune_uid names the graph whose autotuned engine knobs apply (0: none).  idle (may be empty) runs
-// on the host after each step is enqueued and before the host waits for it, so host-side or
-// other-stream work overlaps the step.
+// on the host after each step is enqueued, while the host waits for the step's first part (plan,
+// merges, tracebacks), whose completion event (a hipEvent_t) it receives: host-side or other-stream
+// work overlaps the step, and hipEventQuery on the event tells when to stop.
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bopts, gst_batch_stats* stats, uint64_t tune_uid = 0,
-                         const std::function<void()>& idle = std::function<void()>());
+                         const std::function<void(void* a_done)>& idle = std::function<void(void*)>());
 void gst_batch_free(gst_ctx* ctx);

@@ -357,8 +357,12 @@ struct gst_batch_engine {
     uint64_t n_items_total = 0;
     uint32_t* all_cnt = nullptr;
     // captured step
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;     // part A of a step
     hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec_b = nullptr;   // part B
+    hipGraph_t graph_b = nullptr;
+    hipStream_t cstream = nullptr;      // readbacks of finished queries
+    hipEvent_t a_done = nullptr;        // end of the last step's part A
     int g_S = -1, g_R = -1, g_flt = -1;
 };
 
@@ -390,6 +394,9 @@ static void free_engine_slots(gst_batch_engine* E) {
     if (E->h_slots) hipHostFree(E->h_slots);
     E->h_slots = nullptr;
     if (E->gexec) hipGraphExecDestroy(E->gexec);
+    if (E->gexec_b) hipGraphExecDestroy(E->gexec_b);
+    if (E->graph_b) hipGraphDestroy(E->graph_b);
+    E->gexec_b = nullptr; E->graph_b = nullptr;
     if (E->graph) hipGraphDestroy(E->graph);
     E->gexec = nullptr; E->graph = nullptr; E->g_S = -1;
     E->S = 0; E->n_cap = 0; E->g_cap = 0;
@@ -402,6 +409,8 @@ void gst_batch_free(gst_ctx* ctx) {
     free_engine_slots(E);
     for (int g = 0; g <= GST_MAX_GROUPS; ++g) { hipFree(E->pairs[g]); hipFree(E->masks[g]); hipFree(E->pos_of[g]); }
     hipFree(E->all_items); hipFree(E->all_cnt);
+    if (E->cstream) hipStreamDestroy(E->cstream);
+    if (E->a_done) hipEventDestroy(E->a_done);
     delete E;
     ctx->be = nullptr;
 }
@@ -653,10 +662,12 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     return GST_OK;
 }
 
-// enqueue one step (plan/merge/compact/trace + R grow rounds); ev (2 + 2R events, or nullptr)
-// brackets the merge launch and every grow launch
-static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba, int R, bool flt,
-                         const hipEvent_t* ev = nullptr) {
+// A step is two parts.  Part A: plan, tour tables, merges, bounds, compaction, traceback, then the
+// slots' phase snapshot to the host; part B: R grow rounds.  The host reads part A's snapshot and
+// the finished queries' results while part B runs.  ev (2 + 2R events, or nullptr) brackets the
+// merge launch and every grow launch.
+static void enqueue_step_a(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba, bool flt,
+                           const hipEvent_t* ev = nullptr) {
     hipStream_t s = (hipStream_t)ctx->stream;
     const int mgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 8);
     // one LDS size for every (g <= g_cap, level) a slot may merge: tile rows, bitmaps, and the
@@ -688,6 +699,11 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
     else k_batch_bound<false><<<1, PLAN_THREADS, 0, s>>>(ba);
     k_batch_compact<<<mgrid, 256, 0, s>>>(ba);
     k_batch_trace<<<MAX_SLOTS, WAVE, 0, s>>>(ba);
+    hipMemcpyAsync(E->h_slots, E->d_plan->sd_out, sizeof(SlotDev) * (size_t)ba.S, hipMemcpyDeviceToHost, s);
+}
+
+static void enqueue_step_b(gst_ctx* ctx, const BatchArgs& ba, int R, bool flt, const hipEvent_t* ev = nullptr) {
+    hipStream_t s = (hipStream_t)ctx->stream;
     for (int r = 0; r < R; ++r) {
         k_plan_grow<<<1, PLAN_THREADS, 0, s>>>(ba);
         if (ev) hipEventRecord(ev[2 + 2 * r], s);
@@ -700,7 +716,7 @@ static void enqueue_step(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& ba,
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
-                         const std::function<void()>& idle) {
+                         const std::function<void(void*)>& idle) {
     gst_batch_opts bo{};
     if (bo_in) bo = *bo_in;
     if (!ctx->be) ctx->be = new gst_batch_engine();
@@ -735,10 +751,15 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     std::vector<hipEvent_t> tev;
     double t_grow = 0.0, t_merge = 0.0;
     uint64_t n_merge_launch = 0;
+    const size_t nev = 2 + 2 * (size_t)R;   // per step; two sets by step parity (a step's part B
+                                            // may still run when the next step is enqueued)
     if (timed) {
-        tev.resize(2 + 2 * (size_t)R);
+        tev.resize(2 * nev);
         for (auto& e : tev) HIPCHK(hipEventCreate(&e));
     }
+    if (!E->cstream) HIPCHK(hipStreamCreateWithFlags(&E->cstream, hipStreamNonBlocking));
+    if (!E->a_done) HIPCHK(hipEventCreateWithFlags(&E->a_done, hipEventDisableTiming));
+    hipStream_t cs = E->cstream;
     struct EvFree {
         std::vector<hipEvent_t>& v;
         ~EvFree() { for (auto& e : v) hipEventDestroy(e); }
@@ -794,10 +815,16 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         } else {
             if (E->gexec) { hipGraphExecDestroy(E->gexec); E->gexec = nullptr; }
             if (E->graph) { hipGraphDestroy(E->graph); E->graph = nullptr; }
+            if (E->gexec_b) { hipGraphExecDestroy(E->gexec_b); E->gexec_b = nullptr; }
+            if (E->graph_b) { hipGraphDestroy(E->graph_b); E->graph_b = nullptr; }
             HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            enqueue_step(ctx, E, ba, R, flt);
+            enqueue_step_a(ctx, E, ba, flt);
             HIPCHK(hipStreamEndCapture(s, &E->graph));
             HIPCHK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueue_step_b(ctx, ba, R, flt);
+            HIPCHK(hipStreamEndCapture(s, &E->graph_b));
+            HIPCHK(hipGraphInstantiate(&E->gexec_b, E->graph_b, nullptr, nullptr, 0));
             E->g_S = S; E->g_R = R; E->g_flt = (int)flt;
             graph_ready = true;
         }
@@ -805,23 +832,36 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     // a step runs rounds_per_step grow rounds of every query in flight; every level converges, so
     // this bound (minutes of work) only turns an engine defect into an error instead of a spin
     const uint64_t max_steps = 2ull * 1000 * 1000;
+    // kernel times of one step (its events are complete once the next step's part A is)
+    auto add_times = [&](uint64_t step) -> gst_status {
+        const hipEvent_t* te = tev.data() + (size_t)(step & 1) * nev;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, te[0], te[1]));
+        t_merge += ms;
+        n_merge_launch++;
+        for (int r = 0; r < R; ++r) {
+            HIPCHK(hipEventElapsedTime(&ms, te[2 + 2 * (size_t)r], te[3 + 2 * (size_t)r]));
+            t_grow += ms;
+        }
+        return GST_OK;
+    };
+    // Per step: enqueue part A, its completion event, part B; wait for part A; collect the queries
+    // it finished (their results are final after its traceback) and refill their slots while part
+    // B runs: the refills queue behind part B on the engine stream, so the next step sees them.
     while (busy > 0) {
+        const hipEvent_t* te = timed ? tev.data() + (size_t)(steps & 1) * nev : nullptr;
         if (graph_ready) HIPCHK(hipGraphLaunch(E->gexec, s));
-        else enqueue_step(ctx, E, ba, R, flt, timed ? tev.data() : nullptr);
+        else enqueue_step_a(ctx, E, ba, flt, te);
+        HIPCHK(hipEventRecord(E->a_done, s));
+        if (graph_ready) HIPCHK(hipGraphLaunch(E->gexec_b, s));
+        else enqueue_step_b(ctx, ba, R, flt, te);
         HIPCHK(hipGetLastError());
         steps++;
-        HIPCHK(hipMemcpyAsync(E->h_slots, E->d_plan->sd_out, sizeof(SlotDev) * (size_t)S, hipMemcpyDeviceToHost, s));
-        if (idle) idle();
-        HIPCHK(hipStreamSynchronize(s));
-        if (timed) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, tev[0], tev[1]));
-            t_merge += ms;
-            n_merge_launch++;
-            for (int r = 0; r < R; ++r) {
-                HIPCHK(hipEventElapsedTime(&ms, tev[2 + 2 * (size_t)r], tev[3 + 2 * (size_t)r]));
-                t_grow += ms;
-            }
+        if (idle) idle((void*)E->a_done);
+        HIPCHK(hipEventSynchronize(E->a_done));
+        if (timed && steps >= 2) {
+            st = add_times(steps - 2);
+            if (st != GST_OK) return st;
         }
         if (steps > max_steps) return gst_set_error(ctx, GST_INTERNAL, "batch: queries did not finish");
         // finished queries: read root, control blocks, statistics and edge count, then the edges
@@ -832,23 +872,24 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         // one copy per finished query: the readback block's header, plus the first EDGE_PREFIX
         // edges when the tree is wanted; longer trees take a second copy
         const uint32_t pre = std::min(E->ecap, EDGE_PREFIX);
+        HIPCHK(hipStreamWaitEvent(cs, E->a_done, 0));
         for (int i : fin) {
             auto& sl = E->slots[(size_t)i];
             const size_t nb = sl.job.opts.want_tree ? back_bytes(pre) : BK_EDGES;
-            HIPCHK(hipMemcpyAsync(sl.h_back, sl.back, nb, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(sl.h_back, sl.back, nb, hipMemcpyDeviceToHost, cs));
         }
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipStreamSynchronize(cs));
         bool more = false;
         for (int i : fin) {
             auto& sl = E->slots[(size_t)i];
             uint32_t nout = 0;
             std::memcpy(&nout, sl.h_back + BK_OUTN, 4);
             if (sl.job.opts.want_tree && nout > pre && nout <= E->ecap) {
-                HIPCHK(hipMemcpyAsync(sl.h_edges, sl.edges, 12ull * nout, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipMemcpyAsync(sl.h_edges, sl.edges, 12ull * nout, hipMemcpyDeviceToHost, cs));
                 more = true;
             }
         }
-        if (more) HIPCHK(hipStreamSynchronize(s));
+        if (more) HIPCHK(hipStreamSynchronize(cs));
         std::vector<uint32_t> eu, ev, ew;
         for (int i : fin) {
             auto& sl = E->slots[(size_t)i];
@@ -929,6 +970,10 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         }
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (timed && steps >= 1) {
+        st = add_times(steps - 1);
+        if (st != GST_OK) return st;
+    }
     if (bst) {
         std::memset(bst, 0, sizeof(*bst));
         bst->t_total_ms = now_ms() - t0;

@@ -1377,8 +1377,9 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             return true;
         }
         if (ready_head >= ready.size() && setup_err == GST_OK && next_valid()) {
-            // nothing prepared: select one query now, on the engine's stream
-            gst_status ss = select(next++, nullptr);
+            // nothing prepared: select one query now (on the selection stream: the engine's stream
+            // may still be running a step)
+            gst_status ss = select(next++, ctx->stream2);
             if (ss != GST_OK) { setup_err = ss; return false; }
             if (!pending_full.empty()) {
                 const int32_t q = pending_full.back();
@@ -1403,11 +1404,12 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         j.kind = 0;
         return true;
     };
-    auto idle = [&]() {
-        hipStream_t s1 = (hipStream_t)ctx->stream;
-        // prepare upcoming subgraphs while the step runs; stop when the step is done
+    auto idle = [&](void* a_done) {
+        // prepare upcoming subgraphs while the step runs; stop when its first part is done (the
+        // engine then collects finished queries)
+        const hipEvent_t done = (hipEvent_t)a_done;
         while (setup_err == GST_OK && ready.size() - ready_head < (size_t)lookahead && next_valid() &&
-               hipStreamQuery(s1) == hipErrorNotReady) {
+               hipEventQuery(done) == hipErrorNotReady) {
             gst_status ss = select(next++, ctx->stream2);
             if (ss != GST_OK) { setup_err = ss; break; }
         }
