@@ -17,6 +17,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <thread>
 #include <vector>
 
 #include "gst_batch.h"
@@ -803,6 +805,29 @@ bool dalloc(T*& p, size_t elems) {
 }  // namespace
 
 // =======================================================================================
+// One selection in flight (scores, top-k, induced subgraph of one query): its own scratch, pinned
+// staging and events, so that several selections can be queued on one stream without the host
+// waiting between them (they share the model's activations in stream order).
+struct SelSlot {
+    uint32_t* scr = nullptr;     // device: hist 2048 | state 8 | sel n | newid n | deg n | arcoff n | old n |
+    uint64_t cap = 0;            //         block sums 2 (nbk + 2) | sub group ids gcnt
+    uint32_t* h_rb = nullptr;    // pinned readback: ns, ms, sub group ids
+    uint32_t h_cap = 0;
+    int32_t* h_grp = nullptr;    // pinned group lists of the query (upload source)
+    uint32_t hg_cap = 0;
+    hipEvent_t e_sizes = nullptr, e_built = nullptr;
+    uint32_t n = 0, nbk = 0, gcnt = 0;
+};
+
+void sel_free(SelSlot& sl) {
+    hipFree(sl.scr);
+    if (sl.h_rb) hipHostFree(sl.h_rb);
+    if (sl.h_grp) hipHostFree(sl.h_grp);
+    if (sl.e_sizes) hipEventDestroy(sl.e_sizes);
+    if (sl.e_built) hipEventDestroy(sl.e_built);
+    sl = SelSlot();
+}
+
 struct gst_model {
     gst_ctx* ctx = nullptr;
     float* p = nullptr;   // NPARAM fp32 master
@@ -829,10 +854,8 @@ struct gst_model {
     uint32_t cap_grp = 0;
     // hub rows of the last graph
     uint64_t hub_uid = 0;
-    uint32_t* sel_scratch = nullptr;   // top-k / subgraph scratch (u32)
-    uint64_t cap_scratch = 0;
-    uint32_t* h_sel = nullptr;         // pinned readback of a selection (sizes, group ids)
-    uint32_t cap_h_sel = 0;
+    SelSlot sync_sel;                  // the selection of the one-query path
+    std::vector<SelSlot> batch_sel;    // selections in flight of the batched path
     void* sel_stream = nullptr;        // stream of the forward pass and selection (nullptr: the context's)
     uint32_t* hub_list = nullptr;   // hub row ids
     uint32_t* hub_items = nullptr;  // pairs (row, h | chunk << 20)
@@ -900,20 +923,25 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
     return GST_OK;
 }
 
-gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* groups) {
+// pinned (may be nullptr): a staging buffer of at least g + 1 + members words that stays untouched
+// until the stream has run the copy; without it the copy is waited for
+gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* groups, int32_t* pinned = nullptr) {
     gst_ctx* ctx = M->ctx;
     hipStream_t st = mstream(M);
     const int g = groups->n_groups;
     const uint32_t cnt = (uint32_t)(g + 1 + groups->grp_ptr[g]);
     if (M->cap_grp < cnt) {
+        HIPCHK(hipStreamSynchronize(st));   // earlier selections may still read the old buffer
         if (!dalloc(M->grp, cnt)) return gst_set_error(ctx, GST_OOM, "model: group allocation failed");
         M->cap_grp = cnt;
     }
-    std::vector<int32_t> h(cnt);
+    std::vector<int32_t> hv;
+    int32_t* h = pinned;
+    if (!h) { hv.resize(cnt); h = hv.data(); }
     for (int i = 0; i <= g; ++i) h[(size_t)i] = groups->grp_ptr[i];
     for (int32_t j = 0; j < groups->grp_ptr[g]; ++j) h[(size_t)(g + 1 + j)] = groups->grp_vtx[j];
-    HIPCHK(hipMemcpyAsync(M->grp, h.data(), cnt * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));   // h is a local buffer
+    HIPCHK(hipMemcpyAsync(M->grp, h, cnt * 4, hipMemcpyHostToDevice, st));
+    if (!pinned) HIPCHK(hipStreamSynchronize(st));   // h is a local buffer
     const int fg = std::max(1, ctx->cu_count * 8);
     k_fill32<<<gridn(G->n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)G->n);
     k_fill32<<<gridn((uint64_t)G->n * 8, 256, fg), 256, 0, st>>>((uint32_t*)M->gcnt, 0u, (uint64_t)G->n * 8);
@@ -939,11 +967,11 @@ void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipS
                                                                                         M->n_hub, M->hub_acc, out);
 }
 
-gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups) {
+gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups, int32_t* pinned_groups = nullptr) {
     gst_ctx* ctx = M->ctx;
     gst_status sst = model_bufs(M, G);
     if (sst != GST_OK) return sst;
-    sst = upload_groups(M, G, groups);
+    sst = upload_groups(M, G, groups, pinned_groups);
     if (sst != GST_OK) return sst;
     hipStream_t st = mstream(M);
     const uint32_t n = (uint32_t)G->n;
@@ -1006,8 +1034,8 @@ void gst_model_free(gst_model* M) {
     hipFree(M->x); hipFree(M->ax); hipFree(M->h1); hipFree(M->ah1); hipFree(M->h2); hipFree(M->dZ2); hipFree(M->dZ1);
     hipFree(M->s); hipFree(M->dS); hipFree(M->dX2); hipFree(M->dH1); hipFree(M->dA1); hipFree(M->hub_acc);
     hipFree(M->gm); hipFree(M->y); hipFree(M->grp); hipFree(M->hub_list); hipFree(M->hub_items);
-    hipFree(M->sel_scratch);
-    if (M->h_sel) hipHostFree(M->h_sel);
+    sel_free(M->sync_sel);
+    for (auto& sl : M->batch_sel) sel_free(sl);
     delete M;
 }
 
@@ -1134,30 +1162,48 @@ bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms) {
 }
 
 // Scores, top-k selection and the subgraph induced by the top-k vertices plus every group member,
-// for one query, all on the context's stream with one host synchronisation (for the subgraph's
-// sizes and its group lists).  S is filled as a view of buf (never freed as a graph); with
-// want_host_rows the host row offsets are copied too (the one-query solver reads them).
-gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SubBuf& buf,
-                            gst_graph& S, std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool want_host_rows) {
+// for one query, in two parts on the model's stream.  select_begin queues the network, the top-k
+// and the subgraph's sizes and group lists (readback into sl.h_rb, event sl.e_sizes); nothing waits.
+// select_finish (once e_sizes is complete) reserves buf for those sizes, queues the subgraph's CSR
+// (event sl.e_built) and fills S as a view of buf (never freed as a graph) and sub_vtx with the
+// members' subgraph ids.  A slot's scratch is reused only after its e_built.
+gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SelSlot& sl) {
     gst_ctx* ctx = M->ctx;
     hipStream_t st = mstream(M);
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
-    gst_status sst = forward(M, G, groups);
+    const int g = groups->n_groups;
+    const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
+    const uint32_t nbk = (n + 1023) / 1024;
+    const uint64_t need = 2048ull + 8 + 5ull * n + 2ull * (nbk + 2) + gcnt + 64;
+    if (sl.cap < need) {
+        if (!dalloc(sl.scr, need)) { sl.cap = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
+        sl.cap = need;
+    }
+    if (sl.h_cap < gcnt + 2) {
+        if (sl.h_rb) hipHostFree(sl.h_rb);
+        sl.h_rb = nullptr;
+        sl.h_cap = 0;
+        if (hipHostMalloc((void**)&sl.h_rb, (gcnt + 2 + 1024) * 4ull) != hipSuccess)
+            return gst_set_error(ctx, GST_OOM, "learned solve: pinned allocation failed");
+        sl.h_cap = gcnt + 2 + 1024;
+    }
+    if (sl.hg_cap < (uint32_t)g + 1 + gcnt) {
+        if (sl.h_grp) hipHostFree(sl.h_grp);
+        sl.h_grp = nullptr;
+        sl.hg_cap = 0;
+        if (hipHostMalloc((void**)&sl.h_grp, ((size_t)g + 1 + gcnt + 1024) * 4ull) != hipSuccess)
+            return gst_set_error(ctx, GST_OOM, "learned solve: pinned allocation failed");
+        sl.hg_cap = (uint32_t)g + 1 + gcnt + 1024;
+    }
+    if (!sl.e_sizes) HIPCHK(hipEventCreateWithFlags(&sl.e_sizes, hipEventDisableTiming));
+    if (!sl.e_built) HIPCHK(hipEventCreateWithFlags(&sl.e_built, hipEventDisableTiming));
+    sl.n = n; sl.nbk = nbk; sl.gcnt = gcnt;
+    gst_status sst = forward(M, G, groups, sl.h_grp);
     if (sst != GST_OK) return sst;
     uint32_t k_target = (uint32_t)std::min<double>(n, std::ceil(frac * n));
     if (k_target < 1) k_target = 1;
-    const int g = groups->n_groups;
-    const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
-    // scratch: histogram (2048) | select state (8) | sel (n) | newid (n) | deg (n) | arc offsets (n) |
-    // old ids by new id (n) | block sums 2 (nbk + 2) | sub group ids (gcnt)
-    const uint32_t nbk = (n + 1023) / 1024;
-    const uint64_t need = 2048ull + 8 + 5ull * n + 2ull * (nbk + 2) + gcnt + 64;
-    if (M->cap_scratch < need) {
-        if (!dalloc(M->sel_scratch, need)) { M->cap_scratch = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
-        M->cap_scratch = need;
-    }
-    uint32_t* d_hist = M->sel_scratch;
+    uint32_t* d_hist = sl.scr;
     uint32_t* d_state = d_hist + 2048;
     uint32_t* d_sel = d_state + 8;
     uint32_t* d_newid = d_sel + n;
@@ -1166,15 +1212,6 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     uint32_t* d_old = d_arcoff + n;
     uint32_t* d_bsum = d_old + n;        // 2 * (nbk + 2)
     int32_t* d_subgrp = (int32_t*)(d_bsum + 2 * (nbk + 2));
-    // pinned readback: ns, ms, then the subgraph ids of the group members
-    if (M->cap_h_sel < gcnt + 2) {
-        if (M->h_sel) hipHostFree(M->h_sel);
-        M->h_sel = nullptr;
-        M->cap_h_sel = 0;
-        if (hipHostMalloc((void**)&M->h_sel, (gcnt + 2 + 1024) * 4ull) != hipSuccess)
-            return gst_set_error(ctx, GST_OOM, "learned solve: pinned allocation failed");
-        M->cap_h_sel = gcnt + 2 + 1024;
-    }
     // ---- top-k threshold on the device ----
     k_fill32<<<1, 64, 0, st>>>(d_state, 0u, 8);
     k_fill32<<<1, 64, 0, st>>>(d_state, k_target, 1);
@@ -1195,12 +1232,21 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nbk + 2, nbk);
     k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_arcoff, n, d_bsum + nbk + 2);
     if (gcnt) k_sub_groups<<<gridn(gcnt, 256, 64), 256, 0, st>>>(M->grp + g + 1, gcnt, d_newid, d_subgrp);
-    HIPCHK(hipMemcpyAsync(&M->h_sel[0], d_bsum + nbk, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&M->h_sel[1], d_bsum + nbk + 2 + nbk, 4, hipMemcpyDeviceToHost, st));
-    if (gcnt) HIPCHK(hipMemcpyAsync(&M->h_sel[2], d_subgrp, gcnt * 4ull, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const uint32_t ns = M->h_sel[0], ms = M->h_sel[1];
-    sub_vtx.assign((const int32_t*)&M->h_sel[2], (const int32_t*)&M->h_sel[2] + gcnt);
+    HIPCHK(hipMemcpyAsync(&sl.h_rb[0], d_bsum + nbk, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&sl.h_rb[1], d_bsum + nbk + 2 + nbk, 4, hipMemcpyDeviceToHost, st));
+    if (gcnt) HIPCHK(hipMemcpyAsync(&sl.h_rb[2], d_subgrp, gcnt * 4ull, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(sl.e_sizes, st));
+    HIPCHK(hipGetLastError());
+    return GST_OK;
+}
+
+gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& buf, gst_graph& S,
+                         std::vector<int32_t>& sub_vtx, gst_learned_stats& L) {
+    gst_ctx* ctx = M->ctx;
+    hipStream_t st = mstream(M);
+    const uint32_t n = sl.n, nbk = sl.nbk, gcnt = sl.gcnt;
+    const uint32_t ns = sl.h_rb[0], ms = sl.h_rb[1];
+    sub_vtx.assign((const int32_t*)&sl.h_rb[2], (const int32_t*)&sl.h_rb[2] + gcnt);
     L.sub_vertices = ns;
     L.sub_arcs = ms;
     if (!sub_reserve(buf, ns, ms)) return gst_set_error(ctx, GST_OOM, "learned solve: subgraph allocation failed");
@@ -1212,20 +1258,39 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
     S.w_type = G->w_type;
     S.d_row = buf.row; S.d_col = buf.col; S.d_w = buf.w; S.d_rev = buf.rev;
     S.max_deg = 0;
-    if (ns == 0) return GST_OK;
-    k_sub_fill<<<gridn((uint64_t)ns * 64, 256, ctx->cu_count * 8), 256, 0, st>>>(
-        G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, d_old, ns, S.d_row, S.d_col, S.d_w);
-    k_set_u32<<<1, 1, 0, st>>>(S.d_row + ns, d_bsum + nbk + 2 + nbk);
-    k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S.d_row, S.d_col, S.d_w, ns, S.d_rev);
+    if (ns > 0) {
+        uint32_t* d_sel = sl.scr + 2048 + 8;
+        uint32_t* d_newid = d_sel + n;
+        uint32_t* d_arcoff = d_newid + 2 * (size_t)n;
+        uint32_t* d_old = d_arcoff + n;
+        uint32_t* d_bsum = d_old + n;
+        k_sub_fill<<<gridn((uint64_t)ns * 64, 256, ctx->cu_count * 8), 256, 0, st>>>(
+            G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, d_old, ns, S.d_row, S.d_col, S.d_w);
+        k_set_u32<<<1, 1, 0, st>>>(S.d_row + ns, d_bsum + nbk + 2 + nbk);
+        k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S.d_row, S.d_col, S.d_w, ns, S.d_rev);
+    }
+    HIPCHK(hipEventRecord(sl.e_built, st));
     HIPCHK(hipGetLastError());
-    if (want_host_rows) {
-        std::vector<uint32_t> r32((size_t)ns + 1);
-        HIPCHK(hipMemcpyAsync(r32.data(), S.d_row, (ns + 1) * 4ull, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+    return GST_OK;
+}
+
+// both parts, waited for (the one-query path); with want_host_rows the host row offsets are
+// copied too (the one-query solver reads them)
+gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SubBuf& buf,
+                            gst_graph& S, std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool want_host_rows) {
+    gst_status sst = select_begin(M, G, groups, frac, M->sync_sel);
+    if (sst != GST_OK) return sst;
+    gst_ctx* ctx = M->ctx;
+    HIPCHK(hipEventSynchronize(M->sync_sel.e_sizes));
+    sst = select_finish(M, G, M->sync_sel, buf, S, sub_vtx, L);
+    if (sst != GST_OK) return sst;
+    HIPCHK(hipEventSynchronize(M->sync_sel.e_built));
+    if (want_host_rows && S.n > 0) {
+        std::vector<uint32_t> r32((size_t)S.n + 1);
+        HIPCHK(hipMemcpyAsync(r32.data(), S.d_row, ((size_t)S.n + 1) * 4ull, hipMemcpyDeviceToHost, mstream(M)));
+        HIPCHK(hipStreamSynchronize(mstream(M)));
         S.h_row.assign(r32.begin(), r32.end());
     }
-    // the model's scratch is reused by the next selection, possibly on another stream
-    HIPCHK(hipStreamSynchronize(st));
     return GST_OK;
 }
 
@@ -1309,6 +1374,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         gst_result sub_res{};
         gst_learned_stats L{};
         double ub = 0.0;
+        double t_sel = 0.0;             // host ms when its selection was queued
         bool have_ub = false, retried = false;
     };
     std::vector<Q> qs((size_t)std::max(nq, 0));
@@ -1329,6 +1395,10 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         ctx->stream2 = (void*)s2;
     }
     const int lookahead = 8;
+    M->sel_stream = ctx->stream2;   // every selection of the run queues on the selection stream
+    auto now = []() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
     auto full_job = [&](int32_t q, gst_batch_job& j) {
         Q& x = qs[(size_t)q];
         j = gst_batch_job();
@@ -1342,51 +1412,91 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         j.tag = q;
         j.kind = 1;
     };
-    // scores, selection and subgraph of query q on `stream` (nullptr: the context's stream); the
-    // query then waits in `ready`, or, without a subgraph, goes straight to its full solve
-    auto select = [&](int32_t q, void* stream) -> gst_status {
-        Q& x = qs[(size_t)q];
-        if (free_bufs.empty()) { bufs.emplace_back(); free_bufs.push_back((int)bufs.size() - 1); }
-        x.buf = free_bufs.back();
-        free_bufs.pop_back();
-        const double t0 = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-        M->sel_stream = stream;
-        gst_status ss = learned_subgraph(M, G, &groups[q], frac, bufs[(size_t)x.buf], x.S, x.sub_vtx, x.L, false);
-        M->sel_stream = nullptr;
-        x.L.t_score_select_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
-        if (ss != GST_OK && ss != GST_OOM) { status[q] = ss; free_bufs.push_back(x.buf); x.buf = -1; return ss; }
-        if (ss != GST_OK || x.S.n == 0) {
-            free_bufs.push_back(x.buf);
-            x.buf = -1;
-            pending_full.push_back(q);   // no subgraph: the plain solve runs (a fallback)
-            return GST_OK;
-        }
-        x.sg = gst_groups{groups[q].n_groups, groups[q].grp_ptr, x.sub_vtx.data()};
-        ready.push_back(q);
-        return GST_OK;
-    };
+    // Selections in flight, oldest first: queued (sizes pending) -> subgraph queued -> ready.
+    // Nothing here waits on the GPU except wait_oldest, called only when no job is ready.
+    struct Sel { SelSlot* sl = nullptr; int32_t q = -1; int state = 0; };   // state 1: sizes pending, 2: built pending
+    if (M->batch_sel.size() < (size_t)lookahead) M->batch_sel.resize((size_t)lookahead);
+    std::vector<Sel> sels((size_t)lookahead);
+    for (int i = 0; i < lookahead; ++i) sels[(size_t)i].sl = &M->batch_sel[(size_t)i];
+    std::deque<int> inflight;   // indices into sels, in queue order
     auto next_valid = [&]() {
         while (next < nq && status[next] != GST_OK) ++next;
         return next < nq;
     };
+    auto begin_sel = [&]() -> bool {
+        if (setup_err != GST_OK || !next_valid() || ready.size() - ready_head + inflight.size() >= (size_t)lookahead)
+            return false;
+        int k = -1;
+        for (int i = 0; i < lookahead; ++i)
+            if (sels[(size_t)i].state == 0) { k = i; break; }
+        if (k < 0) return false;
+        const int32_t q = next++;
+        qs[(size_t)q].t_sel = now();
+        gst_status ss = select_begin(M, G, &groups[q], frac, *sels[(size_t)k].sl);
+        if (ss != GST_OK) {
+            if (ss == GST_OOM) pending_full.push_back(q);   // no subgraph: the plain solve runs (a fallback)
+            else { status[q] = ss; setup_err = ss; }
+            return true;
+        }
+        sels[(size_t)k].q = q;
+        sels[(size_t)k].state = 1;
+        inflight.push_back(k);
+        return true;
+    };
+    // advance the oldest selection if its GPU work is done (wait: block until it is)
+    auto step_oldest = [&](bool wait) -> bool {
+        if (inflight.empty()) return false;
+        Sel& e = sels[(size_t)inflight.front()];
+        Q& x = qs[(size_t)e.q];
+        if (e.state == 1) {
+            if (wait) { if (hipEventSynchronize(e.sl->e_sizes) != hipSuccess) { setup_err = GST_HIP_ERROR; return false; } }
+            else if (hipEventQuery(e.sl->e_sizes) != hipSuccess) return false;
+            if (free_bufs.empty()) { bufs.emplace_back(); free_bufs.push_back((int)bufs.size() - 1); }
+            x.buf = free_bufs.back();
+            free_bufs.pop_back();
+            gst_status ss = select_finish(M, G, *e.sl, bufs[(size_t)x.buf], x.S, x.sub_vtx, x.L);
+            if (ss != GST_OK || x.S.n == 0) {
+                free_bufs.push_back(x.buf);
+                x.buf = -1;
+                if (ss != GST_OK && ss != GST_OOM) { status[e.q] = ss; setup_err = ss; }
+                else pending_full.push_back(e.q);   // no subgraph: the plain solve runs (a fallback)
+                // the slot's scratch is free once its (possibly empty) build has run
+                if (hipEventSynchronize(e.sl->e_built) != hipSuccess) setup_err = GST_HIP_ERROR;
+                e.state = 0;
+                inflight.pop_front();
+                return true;
+            }
+            e.state = 2;
+            return true;
+        }
+        if (wait) { if (hipEventSynchronize(e.sl->e_built) != hipSuccess) { setup_err = GST_HIP_ERROR; return false; } }
+        else if (hipEventQuery(e.sl->e_built) != hipSuccess) return false;
+        x.L.t_score_select_ms = now() - x.t_sel;
+        x.sg = gst_groups{groups[e.q].n_groups, groups[e.q].grp_ptr, x.sub_vtx.data()};
+        ready.push_back(e.q);
+        e.state = 0;
+        inflight.pop_front();
+        return true;
+    };
+    auto pump = [&]() {
+        bool any = true;
+        while (any) {
+            any = step_oldest(false);
+            any = begin_sel() || any;
+        }
+    };
     auto next_job = [&](gst_batch_job& j) -> bool {
+        pump();
+        // nothing ready: wait for the oldest selection (queue one first when none is in flight)
+        while (pending_full.empty() && ready_head >= ready.size() && setup_err == GST_OK) {
+            if (inflight.empty() && !begin_sel()) break;
+            if (!step_oldest(true) && setup_err == GST_OK && inflight.empty()) break;
+        }
         if (!pending_full.empty()) {
             const int32_t q = pending_full.back();
             pending_full.pop_back();
             full_job(q, j);
             return true;
-        }
-        if (ready_head >= ready.size() && setup_err == GST_OK && next_valid()) {
-            // nothing prepared: select one query now (on the selection stream: the engine's stream
-            // may still be running a step)
-            gst_status ss = select(next++, ctx->stream2);
-            if (ss != GST_OK) { setup_err = ss; return false; }
-            if (!pending_full.empty()) {
-                const int32_t q = pending_full.back();
-                pending_full.pop_back();
-                full_job(q, j);
-                return true;
-            }
         }
         if (ready_head >= ready.size()) return false;
         const int32_t q = ready[ready_head++];
@@ -1405,14 +1515,15 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         return true;
     };
     auto idle = [&](void* a_done) {
-        // prepare upcoming subgraphs while the step runs; stop when its first part is done (the
-        // engine then collects finished queries)
+        // queue and advance upcoming selections while the step runs; stop when its first part is
+        // done (the engine then collects finished queries)
         const hipEvent_t done = (hipEvent_t)a_done;
-        while (setup_err == GST_OK && ready.size() - ready_head < (size_t)lookahead && next_valid() &&
-               hipEventQuery(done) == hipErrorNotReady) {
-            gst_status ss = select(next++, ctx->stream2);
-            if (ss != GST_OK) { setup_err = ss; break; }
-        }
+        do {
+            pump();
+            if (setup_err != GST_OK) break;
+            if (inflight.empty() && (ready.size() - ready_head >= (size_t)lookahead || !next_valid())) break;
+            std::this_thread::yield();
+        } while (hipEventQuery(done) == hipErrorNotReady);
     };
     std::vector<gst_status> job_fail;   // statuses of failed jobs (the engine returns the first of them)
     auto job_done = [&](gst_batch_job& j) {
@@ -1441,6 +1552,8 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         if (lstats) lstats[q] = x.L;
     };
     gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid, idle);
+    // selections still in flight (an engine error ends the run early) finish before their buffers go
+    HIPCHK(hipStreamSynchronize((hipStream_t)ctx->stream2));
     M->sel_stream = nullptr;
     for (auto& b : bufs) sub_free(b);
     for (auto& x : qs) gst_result_free(&x.sub_res);
