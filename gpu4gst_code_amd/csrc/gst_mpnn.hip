@@ -22,6 +22,11 @@
 #include <vector>
 
 #include "gst_batch.h"
+
+// selections kept in flight ahead of the batched engine
+#ifndef GST_SELECT_AHEAD
+#define GST_SELECT_AHEAD 8
+#endif
 #include "gst_internal.h"
 #include "gst_mpnn.h"
 
@@ -1394,7 +1399,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         ctx->stream2 = (void*)s2;
     }
-    const int lookahead = 8;
+    const int lookahead = GST_SELECT_AHEAD;
     M->sel_stream = ctx->stream2;   // every selection of the run queues on the selection stream
     auto now = []() {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
