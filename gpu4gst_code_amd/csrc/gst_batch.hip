@@ -108,6 +108,23 @@ __device__ __forceinline__ uint32_t plan_scan(uint32_t x, uint32_t& tot) {
     return off + incl - x;
 }
 
+// a slot's per-job state before a fill: group bits cleared, root = inf, statistics, spider bound,
+// control blocks and hub item count reset (one launch instead of six)
+__global__ void k_slot_reset(uint32_t* gm, uint32_t n, unsigned long long* root, unsigned long long* stats,
+                             uint32_t* spider, uint32_t* ctl, uint32_t* hub_cnt) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t v = tid; v < n; v += gridDim.x * blockDim.x) gm[v] = 0u;
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(RoundCtl) * NCTL / 4); i += blockDim.x) ctl[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)ST_NSTAT; i += blockDim.x) stats[i] = 0ull;
+        if (threadIdx.x == 0) {
+            *root = GST_INF_WORD;
+            *spider = GST_INF_COST;
+            *hub_cnt = 0u;
+        }
+    }
+}
+
 __global__ void k_reset_phases(SlotBlock* sb) {
     if (threadIdx.x < MAX_SLOTS) sb[threadIdx.x].sd = SlotDev{PH_EMPTY, 0u, 0u, 0u};
 }
@@ -566,12 +583,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     const gst_bound_bits bb = gst_make_bound(job.opts.ub, flt, job.opts.half_bound);
     const uint32_t ubb = bb.ub;
     const int fg = std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8));
-    k_fill_u32<<<fg, 256, 0, s>>>(sl.gm, 0u, n);
+    k_slot_reset<<<fg, 256, 0, s>>>(sl.gm, n, sl.root, sl.stats, sl.spider, (uint32_t*)sl.ctl, sl.hub_cnt);
     k_set_group_bits<<<64, 256, 0, s>>>(sl.gm, sl.grp, sl.grp + g + 1, g);
-    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)sl.stats, 0ull, ST_NSTAT);
-    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)sl.root, GST_INF_WORD, 1);
-    k_fill_u32<<<1, 64, 0, s>>>(sl.spider, GST_INF_COST, 1);
-    k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)sl.ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
     k_init_level1<<<fg, 256, 0, s>>>(sl.T + lvl_base[1], sl.gm, G->d_row, n, (uint32_t)g, sl.chg[0], sl.inq[0],
                                     g > 1 ? 1 : 0);
     if (g > 1)
@@ -580,7 +593,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         k_root_scan_l1<<<fg, 256, 0, s>>>(sl.T, n, sl.root);
     if (!persistent) {
         // a job's own graph gets its hub row items on the device, for pull rounds on dense frontiers
-        k_fill_u32<<<1, 64, 0, s>>>(sl.hub_cnt, 0u, 1);
+        // (the count was cleared by k_slot_reset)
         k_compact<<<fg, 256, 0, s>>>(nullptr, G->d_row, n, chunk, sl.hub_items, sl.hub_cnt, nullptr, HUB_DEG);
     }
 
