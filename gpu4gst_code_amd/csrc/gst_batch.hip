@@ -219,6 +219,13 @@ __global__ void __launch_bounds__(256) k_batch_tour(BatchArgs b) {
     // the launch's LDS holds the walk table of the engine's largest group count
     tour_table_body<FLT, PW_LDS>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
                                  const_cast<uint32_t*>(m2.tour));
+    // a table switched off (walks too short to prune) leaves the slot's grow rounds: they then read
+    // no table at all
+    __syncthreads();
+    if (threadIdx.x == 0 && m2.tour[0] == 0u) {
+        SlotBlock& sb = b.sb[p->merge_slot[i]];
+        for (uint32_t k = 2; k < m2.g; ++k) sb.g[k].tour = nullptr;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_batch_compact(BatchArgs b) {
@@ -566,7 +573,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     HIPCHK(hipMemcpyAsync(sl.grp, sl.h_grp, gneed * 4, hipMemcpyHostToDevice, s));
 
     // tour table (built on the device when the slot enters level 2)
-    const bool use_tour = job.opts.prune_lb && job.opts.tour_bound == 0 && g >= 3 &&
+    const bool use_tour = job.opts.prune_lb && job.opts.tour_bound == 0 && g >= 4 &&
                           (job.opts.auto_bound || (std::isfinite(job.opts.ub) && job.opts.ub > 0));
     if (use_tour && sl.tour_g < g) {
         hipFree(sl.tour);
@@ -633,6 +640,14 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.seq = 0;
         a.push_lanes = job.opts.push_lanes > 0 ? (uint32_t)job.opts.push_lanes
                                                : (tuned_lanes ? tuned_lanes : default_push_lanes(true));
+        if (job.opts.push_lanes == 0) {
+            // a level of m masks gets at least the power of two >= m lanes per item (8..64): an
+            // item's changed masks then spread over its lanes instead of looping (measured: +5% on
+            // the g=6 batch, +27% on mixed g = 3..8, whose middle levels hold up to 70 masks)
+            uint32_t want = 8;
+            while (want < (uint32_t)L.m[k] && want < 64) want <<= 1;
+            if (want > a.push_lanes) a.push_lanes = want;
+        }
     }
     for (int k = 2; k <= g; ++k) {
         MergeArgs& ma = sl.h_sb->m[k];

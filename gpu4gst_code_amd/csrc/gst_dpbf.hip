@@ -571,7 +571,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     }
     // ---- tour table: group-to-group walks from the final level 1 ----
     const uint32_t* tour = nullptr;
-    if (st == GST_OK && opts.prune_lb && opts.tour_bound == 0 && g >= 3 && ub_bits != GST_INF_COST) {
+    // (with three groups a level-2 state misses one group, where the tour bound is the one-label one)
+    if (st == GST_OK && opts.prune_lb && opts.tour_bound == 0 && g >= 4 && ub_bits != GST_INF_COST) {
         const size_t words = ((size_t)1 << g) * (size_t)(g + 1) + ((size_t)1 << g) * (size_t)(g * g);
         if (B->tour_g < g) {
             hipFree(B->tour);

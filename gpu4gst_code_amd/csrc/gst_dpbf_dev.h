@@ -304,14 +304,15 @@ __device__ __forceinline__ uint32_t num_chunks(const uint32_t* row, uint32_t v, 
 //   lb1 = min_i d_i + Wmin(U) / 2                          (a relaxation of the reference's lb1,
 //                                                           :463-470, which takes the walk's ends)
 // The table holds W(i, U) at tour[U * (g + 1) + i] and Wmin(U) at tour[U * (g + 1) + g]; an
-// entry of GST_INF_COST switches the tour terms off for that state.  Int costs round the halves up
+// entry of GST_INF_COST switches the tour terms off for that state, tour[0] == 0 for every state.  Int costs round the halves up
 // (the completing tree's cost is an integer); float bounds are scaled by 0.9998, far beyond the
 // rounding of the few float additions behind them.
 template <bool FLT, class DistF>
 __device__ __forceinline__ uint32_t state_lb(DistF dist, const uint32_t* tour, uint32_t g, uint32_t X, uint32_t lbcap) {
     uint32_t one = 0, dmin = GST_INF_COST;
     const uint32_t U = ((1u << g) - 1u) & ~X;
-    const uint32_t* tu = (tour && (U & (U - 1u))) ? tour + (size_t)U * (g + 1u) : nullptr;
+    // tour[0] (the row of the empty set, never a missing set) says whether the table is in use
+    const uint32_t* tu = (tour && (U & (U - 1u)) && tour[0]) ? tour + (size_t)U * (g + 1u) : nullptr;
     bool tour_ok = tu != nullptr;
     uint64_t tmax_i = 0;
     float tmax_f = 0.0f;
@@ -362,6 +363,10 @@ __device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, const uint32_t*
 //   P(m, i, j) = cheapest walk from group i to group j passing every group of m (i, j in m),
 //   one popcount layer of m at a time (open walks, then closed ones i == j of the same layer)
 // into tour[m * (g + 1) + i] = min_j P(m, i, j) and tour[m * (g + 1) + g] = min_i of those.
+// The tour terms exceed the one-label bound by at most half a walk, and a grown state has cost
+// <= gmax, so walks much shorter than the grow limit cannot prune: when the cheapest walk through
+// every group is below 2/5 of lbcap (about U/5; groups that overlap or lie close together), tour[0]
+// = 0 switches the table off and the grow rounds skip its loads.
 // The walk table P (2^g * g * g words) lives in LDS when it fits (TOUR_LDS_MAX_G groups: 64 KiB,
 // given to the launch as dynamic LDS), else in the global scratch buffer Pw: each popcount layer
 // reads the previous one, so its latency repeats once per layer.  Sums that overflow the int range
@@ -433,6 +438,16 @@ __device__ void tour_table_body(const uint64_t* T1, const int32_t* grp_ptr, cons
                 if ((m >> j) & 1u) best = min(best, Pw[m * G2 + a * g + j]);
         }
         tour[x] = best;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t wall = tour[(size_t)(S - 1) * (g + 1) + g];
+        bool use = wall != GST_INF_COST && wall != 0u;
+        if (use && lbcap != GST_INF_COST) {
+            if (FLT) use = __uint_as_float(wall) >= 0.4f * __uint_as_float(lbcap);
+            else use = (uint64_t)wall * 5u >= (uint64_t)lbcap * 2u;
+        }
+        tour[0] = use ? 1u : 0u;
     }
 }
 

@@ -88,10 +88,11 @@ typedef struct {
                                   by the cheapest spider, min over vertices c of sum_i dist(c, G_i): the union
                                   of those shortest paths holds a tree touching every group, so the bound is
                                   feasible (with g >= 3; the half_bound rule applies to it too) */
-    int32_t tour_bound;        /* with prune_lb, a bound and g >= 3: 0 = also use the tour bounds of PrunedDP++
+    int32_t tour_bound;        /* with prune_lb, a bound and g >= 4: 0 = also use the tour bounds of PrunedDP++
                                   once level 1 is final (walks around the completing tree through the
                                   missing groups; rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:435-494,
-                                  268-385, 565-584), -1 = one-label bound only */
+                                  268-385, 565-584), unless the walk through all groups is too short to
+                                  prune (below ~U/5: overlapping or close groups); -1 = one-label only */
     int32_t reserved[1];
 } gst_opts;
 
