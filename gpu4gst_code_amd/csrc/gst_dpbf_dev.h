@@ -372,6 +372,10 @@ __device__ __forceinline__ uint32_t label_lb(const uint64_t* T1, const uint32_t*
 // reads the previous one, so its latency repeats once per layer.  Sums that overflow the int range
 // saturate to GST_INF_COST, which only ever weakens the bound.
 constexpr uint32_t TOUR_LDS_MAX_G = 8;
+// the table is used when the walk through all groups is at least this many tenths of lbcap
+#ifndef GST_TOUR_MIN_TENTHS
+#define GST_TOUR_MIN_TENTHS 4
+#endif
 __host__ __device__ constexpr size_t tour_lds_bytes(uint32_t g) {
     return g <= TOUR_LDS_MAX_G ? ((size_t)1 << g) * g * g * 4 : 0;
 }
@@ -444,8 +448,8 @@ __device__ void tour_table_body(const uint64_t* T1, const int32_t* grp_ptr, cons
         const uint32_t wall = tour[(size_t)(S - 1) * (g + 1) + g];
         bool use = wall != GST_INF_COST && wall != 0u;
         if (use && lbcap != GST_INF_COST) {
-            if (FLT) use = __uint_as_float(wall) >= 0.4f * __uint_as_float(lbcap);
-            else use = (uint64_t)wall * 5u >= (uint64_t)lbcap * 2u;
+            if (FLT) use = __uint_as_float(wall) * 10.0f >= (float)GST_TOUR_MIN_TENTHS * __uint_as_float(lbcap);
+            else use = (uint64_t)wall * 10u >= (uint64_t)lbcap * GST_TOUR_MIN_TENTHS;
         }
         tour[0] = use ? 1u : 0u;
     }
