@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=16, help="queries solved before the timed run")
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--groups", type=int, default=6)
-    ap.add_argument("--frac", type=float, default=0.03, help="top-scored fraction of vertices kept in the subgraph")
-    ap.add_argument("--slots", type=int, default=64, help="queries in flight in the batched engine")
+    ap.add_argument("--frac", type=float, default=0.05, help="top-scored fraction of vertices kept in the subgraph")
+    ap.add_argument("--slots", type=int, default=96, help="queries in flight in the batched engine")
     ap.add_argument("--model", default=os.path.join(ROOT, "gpu4gst_code_amd", "models", "mpnn_ba50k.npy"))
     ap.add_argument("--autotune", type=int, default=1, help="1: pick launch knobs with gst_autotune before timing")
     ap.add_argument("--rounds-per-step", type=int, default=0, help="engine rounds per step (0: tuned / default)")
@@ -270,7 +270,7 @@ def other_workload(args, gst):
         if args.workload == "rmat22":
             # float R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05), edge factor 16, lognormal(0,1) weights, g=7
             G = gst.gen_rmat(22, 16, weights="lognormal", seed=22)
-            g, smin, smax, nq, frac = 7, 20, 500, 3, args.frac
+            g, smin, smax, nq, frac = 7, 20, 500, 3, 0.03
             dtype, metric = "f32", "exact-cost GST queries/s, float R-MAT scale 22 g=7 (C3)"
         else:
             # int R-MAT scale 21, weights [1,1000], g=8 groups of 10-300, learned bound on the top 5%
