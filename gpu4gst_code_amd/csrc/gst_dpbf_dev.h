@@ -1319,11 +1319,15 @@ __global__ void k_async_begin(const RoundCtl* ctl, AsyncCtl* ac) {
 // LDS, and each thread owns (vertex, mask) outputs: min over the 2^(k-1)-1 splits in registers,
 // then a coalesced store of the level-k rows.  No atomics except the LDS bitmap of finite masks.
 // ---------------------------------------------------------------------------------------
-constexpr int MERGE_THREADS = 256;
-// default LDS budget of a merge tile's cost rows (measured on BA V=200k g=6: 32 KiB beats 8, 16,
-// 48 and 64; four workgroups per CU with tiles of 2x the vertices of 16 KiB)
+// merge workgroup: 8 waves (measured on BA V=200k g=6 against 4 and 16, with the tile below)
+#ifndef GST_MERGE_THREADS
+#define GST_MERGE_THREADS 512
+#endif
+constexpr int MERGE_THREADS = GST_MERGE_THREADS;
+// default LDS budget of a merge tile's cost rows (measured on BA V=200k g=6 with 512-thread
+// workgroups: 48 KiB beats 32 and 64; three workgroups per CU)
 #ifndef GST_MERGE_TILE_KB
-#define GST_MERGE_TILE_KB 32
+#define GST_MERGE_TILE_KB 48
 #endif
 constexpr size_t MERGE_TILE_BYTES = GST_MERGE_TILE_KB * 1024;
 // loads in flight per thread while a merge tile's rows are staged

@@ -85,8 +85,8 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     const std::vector<int32_t> c_pull = {450, -1, 250, 650, 850};
     const std::vector<int32_t> c_chunk = {32, 16, 64};
     const std::vector<int32_t> c_lanes = {64, 16, 32};
-    const std::vector<int32_t> c_tile = {32, 16, 48, 8};
-    Knobs best{450, 32, 64, 32};
+    const std::vector<int32_t> c_tile = {48, 32, 16, 8};
+    Knobs best{450, 32, 64, 48};
     double best_ms = 0.0;
     gst_status s = eval(best, &best_ms);
     if (s != GST_OK) return s;
