@@ -11,7 +11,10 @@
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int GROW_BLOCK = 256;              // 4 waves
+#ifndef GST_GROW_BLOCK
+#define GST_GROW_BLOCK 256
+#endif
+constexpr int GROW_BLOCK = GST_GROW_BLOCK;   // 4 waves
 constexpr int GROW_WAVES = GROW_BLOCK / WAVE;
 constexpr int TRACE_STACK = 512;
 constexpr int NCTL = GST_MAX_GROUPS + 2;     // one round-control block per level (entry 0: query flags)
