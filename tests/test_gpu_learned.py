@@ -173,6 +173,45 @@ def test_batch_learned_float_and_mixed_groups(solver):
     dg.free()
 
 
+def test_batch_learned_infeasible_and_long_runs(solver):
+    """Infeasible queries (groups in different components) fail alone, with GST_INFEASIBLE, while the
+    others stay exact; more queries than selections in flight and slots (40 queries, 3 slots) run
+    the selection pipeline through many refills."""
+    # two components: a C1-type graph on 0..499 and another on 500..999
+    A = gst.gen_random_connected(500, 2000, 1, 50, seed=31)
+    B = gst.gen_random_connected(500, 2000, 1, 50, seed=32)
+    src = np.concatenate([np.repeat(np.arange(500), np.diff(A.row_ptr)), 500 + np.repeat(np.arange(500), np.diff(B.row_ptr))])
+    dst = np.concatenate([A.col, 500 + B.col])
+    w = np.concatenate([A.w, B.w]).astype(np.int32)
+    keep = src < dst
+    G = gst.HostGraph.from_edges(1000, src[keep], dst[keep], w[keep])
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=5)
+    rng = np.random.default_rng(23)
+    qs, bad = [], []
+    for i in range(40):
+        g = int(rng.integers(2, 7))
+        if i % 7 == 3:   # one group in each component
+            lists = [list(rng.choice(500, 3, replace=False))] + \
+                    [list(500 + rng.choice(500, 3, replace=False)) for _ in range(g - 1)]
+            qs.append(gst.groups_from_lists(lists))
+            bad.append(True)
+        else:
+            base = 0 if i % 2 else 500
+            qs.append(gst.groups_from_lists([list(base + rng.choice(500, 4, replace=False)) for _ in range(g)]))
+            bad.append(False)
+    res, _ = solver.solve_batch_learned(dg, qs, model, 0.1, slots=3, raise_on_error=False)
+    for q, r, b in zip(qs, res, bad):
+        if b:
+            assert isinstance(r, gst.GstError) and r.status == gst._abi.GST_INFEASIBLE
+            continue
+        want = oracle.dpbf_cost(G, q)
+        assert r.cost == want
+        assert gst.validate_tree(G, q, r.eu, r.ev, r.root) == want
+    model.free()
+    dg.free()
+
+
 def test_trained_network_gain(solver):
     """What the trained network gains over the same method without training: on BA V=50k queries
     (g=6, groups of 10-200) the trained scores give a feasible subgraph tree far more often and a
