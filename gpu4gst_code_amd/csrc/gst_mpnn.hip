@@ -263,10 +263,11 @@ __global__ void k_aggregate_hub_finish(const uint32_t* row, const uint32_t* hub_
 // ---------------------------------------------------------------------------------------
 
 // out[v][n] = relu(sum_k X[v][k] Wt[n][k] + b[n]), X = [self | nbr] (D each), K = 2D, N = 64.
-// One wave per 16-row tile.
-template <int D>
+// One wave per 16-row tile.  HEAD: also the score s[v] = b3 + sum_n w3[n] out[v][n] (the bf16
+// activations, summed over each row's 16 lanes by shuffles), and out may be nullptr (inference).
+template <int D, bool HEAD>
 __global__ void __launch_bounds__(256) k_dense_fwd(const bf16* self, const bf16* nbr, const bf16* Wt, const float* b,
-                                                   uint32_t n, bf16* out) {
+                                                   uint32_t n, bf16* out, const float* w3, const float* b3, float* s) {
     constexpr int K = 2 * D;
     const uint32_t lane = __lane_id();
     const uint32_t r = lane % 16, q = lane / 16;
@@ -284,6 +285,13 @@ __global__ void __launch_bounds__(256) k_dense_fwd(const bf16* self, const bf16*
         for (int nb = 0; nb < 4; ++nb) bw[ks][nb] = *(const bf16x8*)(Wt + (uint64_t)(nb * 16 + r) * K + ks * 32 + 8 * q);
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) bias[nb] = b[nb * 16 + r];
+    float w3v[4] = {0.f, 0.f, 0.f, 0.f};
+    float b3v = 0.f;
+    if (HEAD) {
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) w3v[nb] = w3[nb * 16 + r];
+        b3v = b3[0];
+    }
     for (uint32_t t = gwave; t < ntiles; t += nwaves) {
         const uint32_t v0 = t * 16;
         floatx4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -304,13 +312,30 @@ __global__ void __launch_bounds__(256) k_dense_fwd(const bf16* self, const bf16*
         for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], bw[ks][nb], acc[nb], 0, 0, 0);
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const uint32_t col_n = nb * 16 + r;
+        if (HEAD) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint32_t v = v0 + 4 * q + i;
-                if (v < n) out[(uint64_t)v * HID + col_n] = f2bf(fmaxf(acc[nb][i] + bias[nb], 0.f));
+                float part = 0.f;
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    const bf16 h = f2bf(fmaxf(acc[nb][i] + bias[nb], 0.f));
+                    if (out && v < n) out[(uint64_t)v * HID + nb * 16 + r] = h;
+                    part += w3v[nb] * bf2f(h);
+                }
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) part += __shfl_xor(part, o, 64);   // the row's 16 lanes
+                if (r == 0 && v < n) s[v] = b3v + part;
+            }
+        } else {
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const uint32_t col_n = nb * 16 + r;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t v = v0 + 4 * q + i;
+                    if (v < n) out[(uint64_t)v * HID + col_n] = f2bf(fmaxf(acc[nb][i] + bias[nb], 0.f));
+                }
             }
         }
     }
@@ -407,16 +432,6 @@ __global__ void __launch_bounds__(256) k_dense_bwd_w(const bf16* dZ, const bf16*
 }
 
 // s_v = w3 . h2_v + b3
-__global__ void k_head(const bf16* h2, const float* w3, const float* b3, uint32_t n, float* s) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        float acc = b3[0];
-        const bf16* hv = h2 + (uint64_t)v * HID;
-#pragma unroll 8
-        for (int f = 0; f < HID; ++f) acc += w3[f] * bf2f(hv[f]);
-        s[v] = acc;
-    }
-}
-
 // weighted logistic loss: L = sum_v wt_v * (softplus(s) - y s); dS = wt_v * (sigmoid(s) - y)
 // wt_v = pos_weight for y=1, 1 otherwise; everything divided by n.
 __global__ void k_loss(const float* s, const uint8_t* y, uint32_t n, float pos_weight, float* dS, float* loss) {
@@ -764,8 +779,9 @@ __global__ void __launch_bounds__(256) k_set_bits_scatter(uint32_t* gm, const in
 }
 
 // ax[v] = the graph-only means of ax_static with the group means count / degree, rounded as the
-// gather rounds them (sum times 1/degree on light rows, sum / degree on hub rows)
-__global__ void k_ax_finish(const uint32_t* row, const bf16* ax_static, const float* gcnt, uint32_t n, bf16* ax) {
+// gather rounds them (sum times 1/degree on light rows, sum / degree on hub rows); the counts are
+// cleared behind the read, ready for the next query's scatter
+__global__ void k_ax_finish(const uint32_t* row, const bf16* ax_static, float* gcnt, uint32_t n, bf16* ax) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const uint32_t deg = row[v + 1] - row[v];
         const float inv = deg ? 1.f / (float)deg : 0.f;
@@ -776,6 +792,8 @@ __global__ void k_ax_finish(const uint32_t* row, const bf16* ax_static, const fl
         const float4 c0 = *(const float4*)(gcnt + (uint64_t)v * 8);
         const float4 c1 = *(const float4*)(gcnt + (uint64_t)v * 8 + 4);
         const float c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        *(float4*)(gcnt + (uint64_t)v * 8) = make_float4(0.f, 0.f, 0.f, 0.f);
+        *(float4*)(gcnt + (uint64_t)v * 8 + 4) = make_float4(0.f, 0.f, 0.f, 0.f);
         float m[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) m[i] = deg > (uint32_t)AGG_HUB ? c[i] / (float)deg : c[i] * inv;
@@ -889,6 +907,9 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
         M->static_uid = 0;
         if (!ok) { M->cap_n = 0; return gst_set_error(ctx, GST_OOM, "model: activation allocation failed (n=%u)", n); }
         M->cap_n = n;
+        // the group counts start clear; every forward pass clears them again after reading
+        const int fgz = std::max(1, ctx->cu_count * 8);
+        k_fill32<<<gridn((uint64_t)n * 8, 256, fgz), 256, 0, mstream(M)>>>((uint32_t*)M->gcnt, 0u, (uint64_t)n * 8);
     }
     if (M->hub_uid != G->uid) {
         // hub rows (host side, from the graph's host CSR)
@@ -949,7 +970,6 @@ gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* gro
     if (!pinned) HIPCHK(hipStreamSynchronize(st));   // h is a local buffer
     const int fg = std::max(1, ctx->cu_count * 8);
     k_fill32<<<gridn(G->n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)G->n);
-    k_fill32<<<gridn((uint64_t)G->n * 8, 256, fg), 256, 0, st>>>((uint32_t*)M->gcnt, 0u, (uint64_t)G->n * 8);
     k_set_bits_scatter<<<gridn((uint64_t)groups->grp_ptr[g] * 64, 256, fg), 256, 0, st>>>(
         M->gm, M->grp, M->grp + g + 1, g, G->d_row, G->d_col, M->gcnt);   // features use bits 0..7
     return GST_OK;
@@ -972,7 +992,9 @@ void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipS
                                                                                         M->n_hub, M->hub_acc, out);
 }
 
-gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups, int32_t* pinned_groups = nullptr) {
+// need_h2: keep the layer-2 activations (the backward pass reads them); the scores alone need not
+gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups, int32_t* pinned_groups = nullptr,
+                   bool need_h2 = false) {
     gst_ctx* ctx = M->ctx;
     gst_status sst = model_bufs(M, G);
     if (sst != GST_OK) return sst;
@@ -984,10 +1006,10 @@ gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups, i
     const int dg = std::max(1, ctx->cu_count * 8);
     k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, M->gm, n, M->x);
     k_ax_finish<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, M->ax_static, M->gcnt, n, M->ax);
-    k_dense_fwd<FIN><<<dg, 256, 0, st>>>(M->x, M->ax, M->W1t, M->p + OFF_B1, n, M->h1);
+    k_dense_fwd<FIN, false><<<dg, 256, 0, st>>>(M->x, M->ax, M->W1t, M->p + OFF_B1, n, M->h1, nullptr, nullptr, nullptr);
     aggregate<HID, false>(M, G, M->h1, M->ah1, st);
-    k_dense_fwd<HID><<<dg, 256, 0, st>>>(M->h1, M->ah1, M->W2t, M->p + OFF_B2, n, M->h2);
-    k_head<<<gridn(n, 256, fg), 256, 0, st>>>(M->h2, M->p + OFF_W3, M->p + OFF_B3, n, M->s);
+    k_dense_fwd<HID, true><<<dg, 256, 0, st>>>(M->h1, M->ah1, M->W2t, M->p + OFF_B2, n, need_h2 ? M->h2 : nullptr,
+                                               M->p + OFF_W3, M->p + OFF_B3, M->s);
     HIPCHK(hipGetLastError());
     return GST_OK;
 }
@@ -1083,7 +1105,7 @@ gst_status gst_model_loss_grad(gst_model* M, const gst_graph* G, const gst_group
                                float pos_weight, float* loss_out, float* grad_out) {
     if (!M || !G || !groups || !labels) return GST_INVALID_ARG;
     gst_ctx* ctx = M->ctx;
-    gst_status sst = forward(M, G, groups);
+    gst_status sst = forward(M, G, groups, nullptr, true);
     if (sst != GST_OK) return sst;
     hipStream_t st = (hipStream_t)ctx->stream;
     const uint32_t n = (uint32_t)G->n;
