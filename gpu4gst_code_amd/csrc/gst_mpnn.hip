@@ -575,8 +575,9 @@ __global__ void __launch_bounds__(256) k_key_hist(const float* s, uint32_t n, in
 
 // the digit whose bin, counted from the top, first reaches the wanted number (1 <= wanted <= keys
 // left): thread t sums nbins/256 bins from the top, a prefix over the 256 partial sums finds the
-// thread whose range holds the crossing, and that thread walks its bins and appends the digit
-__global__ void __launch_bounds__(256) k_pick_bin(const uint32_t* hist, uint32_t* sel_state, int pass) {
+// thread whose range holds the crossing, and that thread walks its bins and appends the digit.
+// Each thread then clears its own bins, so the next pass (or selection) starts from a zero histogram.
+__global__ void __launch_bounds__(256) k_pick_bin(uint32_t* hist, uint32_t* sel_state, int pass) {
     __shared__ uint32_t part[256];
     uint32_t lo, nb;
     radix_digit(pass, lo, nb);
@@ -602,6 +603,11 @@ __global__ void __launch_bounds__(256) k_pick_bin(const uint32_t* hist, uint32_t
             acc += h;
         }
     }
+    for (uint32_t j = 0; j < per; ++j) hist[top - j] = 0u;
+}
+
+__global__ void k_sel_init(uint32_t* sel_state, uint32_t k_target) {
+    if (threadIdx.x < 8) sel_state[threadIdx.x] = threadIdx.x == 0 ? k_target : 0u;
 }
 
 // select v when key > threshold or key == threshold within the tie budget, plus every group member
@@ -1206,6 +1212,7 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
     if (sl.cap < need) {
         if (!dalloc(sl.scr, need)) { sl.cap = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
         sl.cap = need;
+        k_fill32<<<8, 256, 0, st>>>(sl.scr, 0u, 2048);   // the histogram starts clear; each pass clears it
     }
     if (sl.h_cap < gcnt + 2) {
         if (sl.h_rb) hipHostFree(sl.h_rb);
@@ -1240,11 +1247,9 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
     uint32_t* d_bsum = d_old + n;        // 2 * (nbk + 2)
     int32_t* d_subgrp = (int32_t*)(d_bsum + 2 * (nbk + 2));
     // ---- top-k threshold on the device ----
-    k_fill32<<<1, 64, 0, st>>>(d_state, 0u, 8);
-    k_fill32<<<1, 64, 0, st>>>(d_state, k_target, 1);
+    k_sel_init<<<1, 64, 0, st>>>(d_state, k_target);
     const int hg = std::max(1, std::min(ctx->cu_count, (int)((n + 4095) / 4096)));
     for (int pass = 0; pass < 3; ++pass) {
-        k_fill32<<<8, 256, 0, st>>>(d_hist, 0u, 2048);
         k_key_hist<<<hg, 256, 0, st>>>(M->s, n, pass, d_state, d_hist);
         k_pick_bin<<<1, 256, 0, st>>>(d_hist, d_state, pass);
     }
