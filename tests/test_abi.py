@@ -157,3 +157,36 @@ def test_dataset_reader_reference_lines(tmp_path):
     assert q == [[0, 1, 2], [2, 0]]
     grp = gstp.dataset_query(t, q[0])
     assert grp.g == 3 and list(grp.members(0)) == [0, 3]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors in gpu4gst_code_amd/_abi.py have the sizes and field offsets the C compiler
+    gives the structs of include/gst.h (a field added to one side only fails here)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    from gpu4gst_code_amd import _abi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"gst_csr": _abi.gst_csr, "gst_groups": _abi.gst_groups, "gst_opts": _abi.gst_opts,
+               "gst_stats": _abi.gst_stats, "gst_result": _abi.gst_result, "gst_learned_stats": _abi.gst_learned_stats,
+               "gst_batch_opts": _abi.gst_batch_opts, "gst_batch_stats": _abi.gst_batch_stats,
+               "gst_tune_report": _abi.gst_tune_report, "gst_host_csr": _abi.gst_host_csr}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gst.h"', 'int main(void) {']
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append('return 0; }')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for name, cls in structs.items():
+        assert int(got[f"{name} size"]) == C.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, (name, f)
