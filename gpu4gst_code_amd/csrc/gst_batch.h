@@ -27,9 +27,11 @@ struct gst_batch_job {
 // tune_uid names the graph whose autotuned engine knobs apply (0: none).  idle (may be empty) runs
 // on the host after each step is enqueued, while the host waits for the step's first part (plan,
 // merges, tracebacks), whose completion event (a hipEvent_t) it receives: host-side or other-stream
-// work overlaps the step, and hipEventQuery on the event tells when to stop.
+// work overlaps the step, and hipEventQuery on the event tells when to stop.  max_jobs (0: unknown)
+// bounds the jobs in flight at once: without an explicit slot count the engine sizes no more slots.
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bopts, gst_batch_stats* stats, uint64_t tune_uid = 0,
-                         const std::function<void(void* a_done)>& idle = std::function<void(void*)>());
+                         const std::function<void(void* a_done)>& idle = std::function<void(void*)>(),
+                         int64_t max_jobs = 0);
 void gst_batch_free(gst_ctx* ctx);

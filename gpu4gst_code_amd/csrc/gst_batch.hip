@@ -744,7 +744,7 @@ static void enqueue_step_b(gst_ctx* ctx, const BatchArgs& ba, int R, bool flt, c
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
-                         const std::function<void(void*)>& idle) {
+                         const std::function<void(void*)>& idle, int64_t max_jobs) {
     gst_batch_opts bo{};
     if (bo_in) bo = *bo_in;
     if (!ctx->be) ctx->be = new gst_batch_engine();
@@ -766,6 +766,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         if (E->S > 0) fre += (size_t)E->S * per;   // the engine's own slots would be reused
         const int fit = (int)std::min<size_t>(MAX_SLOTS, fre / 2 / std::max<size_t>(per, 1));
         if (S == 0) S = std::min(32, fit);
+        if (bo.slots <= 0 && max_jobs > 0 && S > max_jobs) S = (int)max_jobs;   // no slot would ever be used
         if (S < 1) return gst_set_error(ctx, GST_OOM, "batch: one slot needs %.2f GB of device memory", per / 1e9);
     }
     gst_status st = ensure_slots(ctx, E, S, n_cap, g_cap, m_cap, chunk);

@@ -1583,7 +1583,10 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         x.L.ub = x.have_ub ? x.ub : -1.0;
         if (lstats) lstats[q] = x.L;
     };
-    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid, idle);
+    int64_t valid = 0;   // a query has one job (its subgraph's or its full solve) in flight at a time
+    for (int32_t q = 0; q < nq; ++q) valid += status[q] == GST_OK ? 1 : 0;
+    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid, idle,
+                                  std::max<int64_t>(valid, 1));
     // selections still in flight (an engine error ends the run early) finish before their buffers go
     HIPCHK(hipStreamSynchronize((hipStream_t)ctx->stream2));
     M->sel_stream = nullptr;

@@ -180,7 +180,8 @@ gst_status gst_validate_tree(const gst_csr* csr, const gst_groups* groups, int32
  * graph.  Each query gets what gst_solve returns for it: the same exact cost (the tree may be
  * another tree of that cost). */
 typedef struct {
-    int32_t slots;             /* queries in flight (0: tuned, at most 256, limited by device memory) */
+    int32_t slots;             /* queries in flight, at most 256 (0: as many as fit in half of the free device
+                                  memory, at most 32 and at most the number of valid queries) */
     int32_t rounds_per_step;   /* grow rounds per step between host checks (0: tuned, else 8) */
     int32_t use_graph;         /* 1: replay each step as a HIP graph (default), 0: launch kernel by kernel */
     int32_t time_kernels;      /* 1: launch kernel by kernel with HIP events around every batched grow and

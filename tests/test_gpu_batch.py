@@ -123,3 +123,20 @@ def test_long_trees_batch(solver):
         assert len(r.eu) == w
         assert gst.validate_tree(G, q, r.eu, r.ev, r.root) == w
     dg.free()
+
+
+def test_default_slots_follow_query_count(solver):
+    """Without a slot count the engine sizes no more slots than there are queries to hold
+    (queries that fail validation hold none); an explicit count is kept.  The count reported is
+    the number of slots the run used (an engine with more slots from an earlier run keeps them idle)."""
+    G = gst.gen_random_connected(500, 2000, 1, 50, seed=4)
+    dg = solver.upload(G)
+    qs = [gst.gen_groups(G.n, 4, 2, 10, seed=90 + i) for i in range(3)]
+    bad = gst.groups_from_lists([[0], []])
+    res, st = solver.solve_batch(dg, qs + [bad], raise_on_error=False)
+    assert st["slots"] == 3
+    for q, r in zip(qs, res):
+        assert r.cost == oracle.dpbf_cost(G, q)
+    _, st = solver.solve_batch(dg, qs, slots=5)
+    assert st["slots"] == 5
+    dg.free()
