@@ -39,7 +39,8 @@ def parse():
                     help="ba200k: the headline (default); c4: 256 mixed queries on the BA graph; rmat22: float "
                          "R-MAT scale 22, g=7; rmat21: learned bound on int R-MAT scale 21, g=8")
     ap.add_argument("--steps", type=int, default=256, help="queries in the timed run (the 64 queries cycled)")
-    ap.add_argument("--warmup", type=int, default=16, help="queries solved before the timed run")
+    ap.add_argument("--warmup", type=int, default=128,
+                    help="queries solved before the timed run (enough to fill the engine's slots once)")
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--groups", type=int, default=6)
     ap.add_argument("--frac", type=float, default=0.05, help="top-scored fraction of vertices kept in the subgraph")
@@ -245,7 +246,7 @@ def other_workload(args, gst):
         log(f"plain batched {len(qs) / t_pb:.1f} q/s")
         t = time.perf_counter(); sb, sst = solver.solve_batch(dg, qs, slots=args.slots, auto_bound=True)
         t_sb = time.perf_counter() - t
-        solver.solve_batch_learned(dg, qs[:16], model, args.frac, slots=args.slots)
+        solver.solve_batch_learned(dg, qs[:128], model, args.frac, slots=args.slots)
         t = time.perf_counter(); lb, lst = solver.solve_batch_learned(dg, qs, model, args.frac, slots=args.slots)
         t_lb = time.perf_counter() - t
         t = time.perf_counter(); single = [solver.solve(dg, q) for q in qs[:64]]; t_s = time.perf_counter() - t
@@ -294,9 +295,9 @@ def other_workload(args, gst):
         log(f"plain {nq / t_p:.3f} q/s")
         t = time.perf_counter(); spider = [solver.solve(dg, q, auto_bound=True) for q in qs]; t_s = time.perf_counter() - t
         t = time.perf_counter(); lone = [solver.solve_learned(dg, q, model, frac) for q in qs]; t_l1 = time.perf_counter() - t
-        # one slot per query (a query has one job in flight at a time); the engine's slots and
-        # captured steps are set up by a warm-up call outside the timed region
-        solver.solve_batch_learned(dg, qs[:1], model, frac, slots=nq)
+        # one slot per query (a query has one job in flight at a time); the engine's slots, captured
+        # steps and subgraph buffers are set up by a warm-up call outside the timed region
+        solver.solve_batch_learned(dg, qs, model, frac, slots=nq)
         t = time.perf_counter(); lb, lst = solver.solve_batch_learned(dg, qs, model, frac, slots=nq)
         t_lb = time.perf_counter() - t
         log(f"learned single {nq / t_l1:.3f} q/s, batched {nq / t_lb:.3f} q/s")

@@ -22,6 +22,7 @@ gst_status gst_set_error(gst_ctx* ctx, gst_status s, const char* fmt, ...) {
         va_start(ap, fmt);
         std::vsnprintf(buf, sizeof(buf), fmt, ap);
         va_end(ap);
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
         ctx->last_error = buf;
     }
     return s;
@@ -306,7 +307,7 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
         else if (first == GST_OK) first = status[q];
     }
     int32_t next = 0;
-    auto next_job = [&](gst_batch_job& j) -> bool {
+    auto next_job = [&](gst_batch_job& j, bool) -> bool {   // every query is ready at once
         while (next < nq && status[next] != GST_OK) ++next;
         if (next >= nq) return false;
         j = gst_batch_job();

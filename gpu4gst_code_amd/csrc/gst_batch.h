@@ -20,16 +20,18 @@ struct gst_batch_job {
     int kind = 0;
 };
 
-// Runs jobs until every slot is idle and next_job has no job.  next_job is asked for a job whenever
-// a slot is free (also after it once returned false); job_done receives each finished job (status
-// and result set) and may make more jobs available to next_job, whose slot is refilled right after.  n_cap / g_cap / m_cap bound every job's graph size and
-// group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
+// Runs jobs until every slot is idle and next_job has no job.  next_job(job, wait) is asked for a job
+// whenever a slot is free, after every step (also after it once returned false): with wait false it
+// returns a job only if one is ready now, so the engine steps on with the queries it holds; with wait
+// true (asked only when no slot is busy) it blocks until a job is ready, and false then means no job
+// will come.  job_done receives each finished job (status and result set) and may make more jobs
+// available to next_job.  n_cap / g_cap / m_cap bound every job's graph size and group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
 // tune_uid names the graph whose autotuned engine knobs apply (0: none).  idle (may be empty) runs
 // on the host after each step is enqueued, while the host waits for the step's first part (plan,
 // merges, tracebacks), whose completion event (a hipEvent_t) it receives: host-side or other-stream
 // work overlaps the step, and hipEventQuery on the event tells when to stop.  max_jobs (0: unknown)
 // bounds the jobs in flight at once: without an explicit slot count the engine sizes no more slots.
-gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
+gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool wait)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bopts, gst_batch_stats* stats, uint64_t tune_uid = 0,
                          const std::function<void(void* a_done)>& idle = std::function<void(void*)>(),

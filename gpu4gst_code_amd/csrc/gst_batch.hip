@@ -741,7 +741,7 @@ static void enqueue_step_b(gst_ctx* ctx, const BatchArgs& ba, int R, bool flt, c
     }
 }
 
-gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>& next_job,
+gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
                          const std::function<void(void*)>& idle, int64_t max_jobs) {
@@ -805,12 +805,13 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
     gst_status first_err = GST_OK;
     bool flt_set = false, flt = false;
     int busy = 0;
-    // next_job may have nothing now and more later (a finished job can make new work), so it is
-    // asked again whenever a slot frees; the run ends when no slot is busy
-    auto try_fill = [&](int si) -> gst_status {
+    // next_job may have nothing now and more later (a finished job or the caller's own pipeline can
+    // make new work), so free slots are offered again after every step; the run ends when no slot is
+    // busy and a waiting call has no job
+    auto try_fill = [&](int si, bool wait) -> gst_status {
         for (;;) {
             gst_batch_job job;
-            if (!next_job(job)) break;
+            if (!next_job(job, wait)) break;
             const bool jf = job.G->w_type == GST_W_FLOAT32;
             if (!flt_set) { flt = jf; flt_set = true; }
             if (jf != flt) {
@@ -833,10 +834,25 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         }
         return GST_OK;
     };
-    for (int i = 0; i < S; ++i) {
-        st = try_fill(i);
-        if (st != GST_OK) return st;
-    }
+    // free slots take the jobs that are ready (stop at the first that gets none); with no slot busy,
+    // wait for one job
+    auto fill_free = [&]() -> gst_status {
+        for (int i = 0; i < S; ++i) {
+            if (E->slots[(size_t)i].busy) continue;
+            const gst_status fs = try_fill(i, false);
+            if (fs != GST_OK) return fs;
+            if (!E->slots[(size_t)i].busy) break;
+        }
+        for (int i = 0; i < S && busy == 0; ++i) {
+            if (E->slots[(size_t)i].busy) continue;
+            const gst_status fs = try_fill(i, true);
+            if (fs != GST_OK) return fs;
+            if (!E->slots[(size_t)i].busy) break;
+        }
+        return GST_OK;
+    };
+    st = fill_free();
+    if (st != GST_OK) return st;
     bool graph_ready = false;
     if (use_graph && busy > 0) {
         if (E->gexec && E->g_S == S && E->g_R == R && E->g_flt == (int)flt) {
@@ -897,7 +913,12 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
         std::vector<int> fin;
         for (int i = 0; i < S; ++i)
             if (E->slots[(size_t)i].busy && E->h_slots[i].phase == PH_DONE) fin.push_back(i);
-        if (fin.empty()) continue;
+        if (fin.empty()) {
+            // jobs that became ready since the last step take the free slots (they join the next step)
+            st = fill_free();
+            if (st != GST_OK) return st;
+            continue;
+        }
         // one copy per finished query: the readback block's header, plus the first EDGE_PREFIX
         // edges when the tree is wanted; longer trees take a second copy
         const uint32_t pre = std::min(E->ecap, EDGE_PREFIX);
@@ -989,14 +1010,17 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&)>
             sl.busy = false;
             busy--;
             job_done(job);
-            // refill the slot, or park it (its phase block is written by exactly one of the two)
-            st = try_fill(i);
+            // refill the slot with a ready job, or park it (a parked slot's phase block is written
+            // again only by a later fill, which copies the whole block behind this copy)
+            st = try_fill(i, false);
             if (st != GST_OK) return st;
             if (!sl.busy) {
                 sl.h_sb->sd.phase = PH_EMPTY;
                 HIPCHK(hipMemcpyAsync(&E->d_sb[i].sd, &sl.h_sb->sd, sizeof(SlotDev), hipMemcpyHostToDevice, s));
             }
         }
+        st = fill_free();
+        if (st != GST_OK) return st;
     }
     HIPCHK(hipStreamSynchronize(s));
     if (timed && steps >= 1) {

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -128,6 +129,7 @@ struct gst_ctx {
     std::vector<gst_tuned_batch> tuned_batch;
     int device = 0;
     std::string last_error;
+    std::mutex err_mu;          // last_error is set from the learned batch's selection thread too
     void* stream = nullptr;     // hipStream_t
     void* stream2 = nullptr;    // hipStream_t (default priority): learned-bound selection beside the engine
     gst_query_bufs* qb = nullptr;

@@ -17,13 +17,15 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
 #include <deque>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "gst_batch.h"
 
-// selections kept in flight ahead of the batched engine
+// selected queries kept ready ahead of the batched engine
 #ifndef GST_SELECT_AHEAD
 #define GST_SELECT_AHEAD 8
 #endif
@@ -857,6 +859,39 @@ void sel_free(SelSlot& sl) {
     sl = SelSlot();
 }
 
+// device CSR of one induced subgraph (reused across queries and calls while large enough)
+struct SubBuf {
+    uint32_t* row = nullptr;
+    int32_t* col = nullptr;
+    uint32_t* w = nullptr;
+    uint32_t* rev = nullptr;
+    uint32_t cap_n = 0;
+    uint64_t cap_m = 0;
+};
+
+static void sub_free(SubBuf& b) {
+    hipFree(b.row); hipFree(b.col); hipFree(b.w); hipFree(b.rev);
+    b = SubBuf();
+}
+
+// room for ns vertices and ms arcs; a new allocation takes a quarter more, and at least
+// floor_n / floor_m (a pool passes the largest sizes it has seen, so that its buffers rarely
+// grow again: a free waits for the whole device)
+static bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms, uint32_t floor_n = 0, uint64_t floor_m = 0) {
+    if (b.cap_n >= ns + 1 && b.cap_m >= std::max(ms, 1u)) return true;
+    sub_free(b);
+    const uint32_t cn = std::max<uint32_t>(ns + 1 + ns / 4 + 64, floor_n);
+    const uint64_t cm = std::max<uint64_t>((uint64_t)std::max(ms, 1u) + ms / 4 + 256, floor_m);
+    if (hipMalloc(&b.row, cn * 4ull) != hipSuccess || hipMalloc(&b.col, cm * 4) != hipSuccess ||
+        hipMalloc(&b.w, cm * 4) != hipSuccess || hipMalloc(&b.rev, cm * 4) != hipSuccess) {
+        sub_free(b);
+        return false;
+    }
+    b.cap_n = cn;
+    b.cap_m = cm;
+    return true;
+}
+
 struct gst_model {
     gst_ctx* ctx = nullptr;
     float* p = nullptr;   // NPARAM fp32 master
@@ -885,6 +920,10 @@ struct gst_model {
     uint64_t hub_uid = 0;
     SelSlot sync_sel;                  // the selection of the one-query path
     std::vector<SelSlot> batch_sel;    // selections in flight of the batched path
+    SubBuf sync_buf;                   // subgraph CSR of the one-query path, kept across calls
+    std::vector<SubBuf> sub_pool;      // subgraph CSRs of the batched path, kept across calls
+    uint32_t sub_max_n = 0;            // largest subgraph rows / arcs a pool buffer was sized for
+    uint64_t sub_max_m = 0;
     void* sel_stream = nullptr;        // stream of the forward pass and selection (nullptr: the context's)
     uint32_t* hub_list = nullptr;   // hub row ids
     uint32_t* hub_items = nullptr;  // pairs (row, h | chunk << 20)
@@ -1069,6 +1108,8 @@ void gst_model_free(gst_model* M) {
     hipFree(M->gm); hipFree(M->y); hipFree(M->grp); hipFree(M->hub_list); hipFree(M->hub_items);
     sel_free(M->sync_sel);
     for (auto& sl : M->batch_sel) sel_free(sl);
+    for (auto& b : M->sub_pool) sub_free(b);
+    sub_free(M->sync_buf);
     delete M;
 }
 
@@ -1164,36 +1205,6 @@ gst_status gst_model_adam_step(gst_model* M, float lr) {
 // ---------------------------------------------------------------------------------------
 namespace {
 
-// device CSR of one induced subgraph (reused across queries while large enough)
-struct SubBuf {
-    uint32_t* row = nullptr;
-    int32_t* col = nullptr;
-    uint32_t* w = nullptr;
-    uint32_t* rev = nullptr;
-    uint32_t cap_n = 0;
-    uint64_t cap_m = 0;
-};
-
-void sub_free(SubBuf& b) {
-    hipFree(b.row); hipFree(b.col); hipFree(b.w); hipFree(b.rev);
-    b = SubBuf();
-}
-
-bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms) {
-    if (b.cap_n >= ns + 1 && b.cap_m >= std::max(ms, 1u)) return true;
-    sub_free(b);
-    const uint32_t cn = ns + 1 + ns / 4 + 64;
-    const uint64_t cm = (uint64_t)std::max(ms, 1u) + ms / 4 + 256;
-    if (hipMalloc(&b.row, cn * 4ull) != hipSuccess || hipMalloc(&b.col, cm * 4) != hipSuccess ||
-        hipMalloc(&b.w, cm * 4) != hipSuccess || hipMalloc(&b.rev, cm * 4) != hipSuccess) {
-        sub_free(b);
-        return false;
-    }
-    b.cap_n = cn;
-    b.cap_m = cm;
-    return true;
-}
-
 // Scores, top-k selection and the subgraph induced by the top-k vertices plus every group member,
 // for one query, in two parts on the model's stream.  select_begin queues the network, the top-k
 // and the subgraph's sizes and group lists (readback into sl.h_rb, event sl.e_sizes); nothing waits.
@@ -1273,7 +1284,7 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
 }
 
 gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& buf, gst_graph& S,
-                         std::vector<int32_t>& sub_vtx, gst_learned_stats& L) {
+                         std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool pooled = false) {
     gst_ctx* ctx = M->ctx;
     hipStream_t st = mstream(M);
     const uint32_t n = sl.n, nbk = sl.nbk, gcnt = sl.gcnt;
@@ -1281,7 +1292,12 @@ gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& 
     sub_vtx.assign((const int32_t*)&sl.h_rb[2], (const int32_t*)&sl.h_rb[2] + gcnt);
     L.sub_vertices = ns;
     L.sub_arcs = ms;
-    if (!sub_reserve(buf, ns, ms)) return gst_set_error(ctx, GST_OOM, "learned solve: subgraph allocation failed");
+    if (!sub_reserve(buf, ns, ms, pooled ? M->sub_max_n : 0u, pooled ? M->sub_max_m : 0ull))
+        return gst_set_error(ctx, GST_OOM, "learned solve: subgraph allocation failed");
+    if (pooled) {
+        M->sub_max_n = std::max(M->sub_max_n, buf.cap_n);
+        M->sub_max_m = std::max(M->sub_max_m, buf.cap_m);
+    }
     S.ctx = ctx;
     S.uid = gst_next_graph_uid();
     S.budget_key = G->uid | (1ull << 63);   // learned subgraphs of one graph share round budgets
@@ -1332,14 +1348,18 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
                              const gst_opts* opts_in, gst_result* res, gst_learned_stats* ls) {
     hipStream_t st = (hipStream_t)ctx->stream;
     gst_learned_stats L{};
-    hipEvent_t e0, e1, e2, e3;
-    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2)); HIPCHK(hipEventCreate(&e3));
+    struct Events {   // destroyed on every return
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events() { for (auto x : e) if (x) hipEventDestroy(x); }
+    } ev;
+    for (auto& x : ev.e) HIPCHK(hipEventCreate(&x));
+    const hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2], e3 = ev.e[3];
     HIPCHK(hipEventRecord(e0, st));
-    SubBuf buf;
+    SubBuf& buf = M->sync_buf;   // reused while large enough (an allocation costs more than a selection)
     gst_graph S;
     std::vector<int32_t> sub_vtx;
     gst_status sst = learned_subgraph(M, G, groups, frac, buf, S, sub_vtx, L, true);
-    if (sst != GST_OK) { sub_free(buf); return sst; }
+    if (sst != GST_OK) return sst;
     const uint32_t ns = (uint32_t)S.n;
     HIPCHK(hipEventRecord(e1, st));
     const int g = groups->n_groups;
@@ -1359,7 +1379,6 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
     }
     HIPCHK(hipEventRecord(e2, st));
     HIPCHK(hipStreamSynchronize(st));
-    sub_free(buf);
     // ---- full solve, bounded by the subgraph tree when there is one ----
     gst_opts fo{};
     if (opts_in) fo = *opts_in;
@@ -1385,7 +1404,6 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
     L.t_sub_solve_ms = b;
     L.t_full_solve_ms = c;
     L.ub = have_ub ? ub : -1.0;
-    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
     if (ls) *ls = L;
     return fs;
 }
@@ -1410,12 +1428,12 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         bool have_ub = false, retried = false;
     };
     std::vector<Q> qs((size_t)std::max(nq, 0));
-    std::vector<SubBuf> bufs;
+    // subgraph CSRs: the model's pool, kept across calls (an allocation or free costs far more than a
+    // selection; a pool buffer is in use from its selection's build to its subgraph job's end)
+    std::vector<SubBuf>& bufs = M->sub_pool;
     std::vector<int> free_bufs;
-    std::vector<int32_t> pending_full;   // queries whose full solve may start
-    std::vector<int32_t> ready;          // queries whose subgraph is built (its solve may start)
-    size_t ready_head = 0;
-    int32_t next = 0;                    // next query to select
+    for (int i = (int)bufs.size() - 1; i >= 0; --i) free_bufs.push_back(i);
+    std::vector<int32_t> pending_full;   // queries whose full solve may start (engine thread only)
     int gmax = 1;
     for (int32_t q = 0; q < nq; ++q)
         if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
@@ -1444,94 +1462,122 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         j.tag = q;
         j.kind = 1;
     };
-    // Selections in flight, oldest first: queued (sizes pending) -> subgraph queued -> ready.
-    // Nothing here waits on the GPU except wait_oldest, called only when no job is ready.
-    struct Sel { SelSlot* sl = nullptr; int32_t q = -1; int state = 0; };   // state 1: sizes pending, 2: built pending
-    if (M->batch_sel.size() < (size_t)lookahead) M->batch_sel.resize((size_t)lookahead);
-    std::vector<Sel> sels((size_t)lookahead);
-    for (int i = 0; i < lookahead; ++i) sels[(size_t)i].sl = &M->batch_sel[(size_t)i];
-    std::deque<int> inflight;   // indices into sels, in queue order
-    auto next_valid = [&]() {
-        while (next < nq && status[next] != GST_OK) ++next;
-        return next < nq;
-    };
-    auto begin_sel = [&]() -> bool {
-        if (setup_err != GST_OK || !next_valid() || ready.size() - ready_head + inflight.size() >= (size_t)lookahead)
-            return false;
-        int k = -1;
-        for (int i = 0; i < lookahead; ++i)
-            if (sels[(size_t)i].state == 0) { k = i; break; }
-        if (k < 0) return false;
-        const int32_t q = next++;
-        qs[(size_t)q].t_sel = now();
-        gst_status ss = select_begin(M, G, &groups[q], frac, *sels[(size_t)k].sl);
-        if (ss != GST_OK) {
-            if (ss == GST_OOM) pending_full.push_back(q);   // no subgraph: the plain solve runs (a fallback)
-            else { status[q] = ss; setup_err = ss; }
-            return true;
-        }
-        sels[(size_t)k].q = q;
-        sels[(size_t)k].state = 1;
-        inflight.push_back(k);
-        return true;
-    };
-    // advance the oldest selection if its GPU work is done (wait: block until it is)
-    auto step_oldest = [&](bool wait) -> bool {
-        if (inflight.empty()) return false;
-        Sel& e = sels[(size_t)inflight.front()];
-        Q& x = qs[(size_t)e.q];
-        if (e.state == 1) {
-            if (wait) { if (hipEventSynchronize(e.sl->e_sizes) != hipSuccess) { setup_err = GST_HIP_ERROR; return false; } }
-            else if (hipEventQuery(e.sl->e_sizes) != hipSuccess) return false;
-            if (free_bufs.empty()) { bufs.emplace_back(); free_bufs.push_back((int)bufs.size() - 1); }
-            x.buf = free_bufs.back();
-            free_bufs.pop_back();
-            gst_status ss = select_finish(M, G, *e.sl, bufs[(size_t)x.buf], x.S, x.sub_vtx, x.L);
-            if (ss != GST_OK || x.S.n == 0) {
-                free_bufs.push_back(x.buf);
-                x.buf = -1;
-                if (ss != GST_OK && ss != GST_OOM) { status[e.q] = ss; setup_err = ss; }
-                else pending_full.push_back(e.q);   // no subgraph: the plain solve runs (a fallback)
-                // the slot's scratch is free once its (possibly empty) build has run
-                if (hipEventSynchronize(e.sl->e_built) != hipSuccess) setup_err = GST_HIP_ERROR;
-                e.state = 0;
-                inflight.pop_front();
-                return true;
+    // Selection runs on a host thread of its own: a query's network pass, top-k and subgraph build
+    // are ~30 launches on the selection stream, and the engine's thread only fills slots and steps.
+    // Under mu: the queues of selected queries, the free subgraph buffers, the count of selections
+    // begun and not yet taken by the engine (at most `lookahead`), and the selection thread's end.
+    const size_t depth = 3;   // selections queued on the stream at once (the oldest finishes first)
+    if (M->batch_sel.size() < depth) M->batch_sel.resize(depth);
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int32_t> ready_q;   // subgraph built: its subgraph job may start
+    std::deque<int32_t> nosub_q;   // no subgraph (empty, or no memory for it): the plain solve runs
+    size_t untaken = 0;
+    bool sel_end = false, stop = false;
+    auto selector = [&]() {
+        gst_status err = GST_OK;
+        struct Sel { SelSlot* sl; int32_t q; };
+        std::deque<Sel> inflight;
+        std::vector<SelSlot*> idle_slots;
+        for (size_t i = 0; i < depth; ++i) idle_slots.push_back(&M->batch_sel[i]);
+        auto publish = [&](std::deque<int32_t>& to, int32_t q) {
+            { std::lock_guard<std::mutex> lk(mu); to.push_back(q); }
+            cv.notify_all();
+        };
+        // the oldest selection to its end: sizes, subgraph build, then ready
+        auto finish_oldest = [&]() -> gst_status {
+            const Sel e = inflight.front();
+            inflight.pop_front();
+            Q& x = qs[(size_t)e.q];
+            HIPCHK(hipEventSynchronize(e.sl->e_sizes));
+            int bi = -1;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!free_bufs.empty()) { bi = free_bufs.back(); free_bufs.pop_back(); }
             }
-            e.state = 2;
-            return true;
+            if (bi < 0) { bufs.emplace_back(); bi = (int)bufs.size() - 1; }   // only this thread grows the pool
+            x.buf = bi;
+            const gst_status ss = select_finish(M, G, *e.sl, bufs[(size_t)bi], x.S, x.sub_vtx, x.L, true);
+            // the slot's scratch is free once its (possibly empty) build has run
+            HIPCHK(hipEventSynchronize(e.sl->e_built));
+            idle_slots.push_back(e.sl);
+            if (ss != GST_OK || x.S.n == 0) {
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    free_bufs.push_back(bi);
+                }
+                x.buf = -1;
+                if (ss != GST_OK && ss != GST_OOM) { status[e.q] = ss; return ss; }
+                publish(nosub_q, e.q);
+                return GST_OK;
+            }
+            x.L.t_score_select_ms = now() - x.t_sel;
+            x.sg = gst_groups{groups[e.q].n_groups, groups[e.q].grp_ptr, x.sub_vtx.data()};
+            publish(ready_q, e.q);
+            return GST_OK;
+        };
+        auto run = [&]() -> gst_status {
+            HIPCHK(hipSetDevice(ctx->device));
+            int32_t next = 0;   // queries below next were begun or failed validation
+            for (;;) {
+                while (next < nq && status[next] != GST_OK) ++next;
+                bool begin = false;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    if (stop) return GST_OK;
+                    begin = next < nq && !idle_slots.empty() && untaken < (size_t)lookahead;
+                    if (begin) ++untaken;
+                    else if (inflight.empty()) {
+                        if (next >= nq) return GST_OK;
+                        // the engine holds `lookahead` selected queries: wait until it takes one
+                        cv.wait(lk, [&] { return stop || untaken < (size_t)lookahead; });
+                        continue;
+                    }
+                }
+                if (begin) {
+                    const int32_t q = next++;
+                    qs[(size_t)q].t_sel = now();
+                    SelSlot* sl = idle_slots.back();
+                    idle_slots.pop_back();
+                    const gst_status ss = select_begin(M, G, &groups[q], frac, *sl);
+                    if (ss == GST_OK) { inflight.push_back({sl, q}); continue; }
+                    idle_slots.push_back(sl);
+                    if (ss != GST_OOM) { status[q] = ss; return ss; }
+                    publish(nosub_q, q);   // no subgraph: the plain solve runs (a fallback)
+                    continue;
+                }
+                const gst_status fs = finish_oldest();
+                if (fs != GST_OK) return fs;
+            }
+        };
+        err = run();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (err != GST_OK && setup_err == GST_OK) setup_err = err;
+            sel_end = true;
         }
-        if (wait) { if (hipEventSynchronize(e.sl->e_built) != hipSuccess) { setup_err = GST_HIP_ERROR; return false; } }
-        else if (hipEventQuery(e.sl->e_built) != hipSuccess) return false;
-        x.L.t_score_select_ms = now() - x.t_sel;
-        x.sg = gst_groups{groups[e.q].n_groups, groups[e.q].grp_ptr, x.sub_vtx.data()};
-        ready.push_back(e.q);
-        e.state = 0;
-        inflight.pop_front();
-        return true;
+        cv.notify_all();
     };
-    auto pump = [&]() {
-        bool any = true;
-        while (any) {
-            any = step_oldest(false);
-            any = begin_sel() || any;
-        }
-    };
-    auto next_job = [&](gst_batch_job& j) -> bool {
-        pump();
-        // nothing ready: wait for the oldest selection (queue one first when none is in flight)
-        while (pending_full.empty() && ready_head >= ready.size() && setup_err == GST_OK) {
-            if (inflight.empty() && !begin_sel()) break;
-            if (!step_oldest(true) && setup_err == GST_OK && inflight.empty()) break;
-        }
-        if (!pending_full.empty()) {
+    auto next_job = [&](gst_batch_job& j, bool wait) -> bool {
+        if (!pending_full.empty()) {   // the engine's thread alone adds and takes these
             const int32_t q = pending_full.back();
             pending_full.pop_back();
             full_job(q, j);
             return true;
         }
-        if (ready_head >= ready.size()) return false;
-        const int32_t q = ready[ready_head++];
+        int32_t q = -1;
+        bool sub = false;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            // nothing ready: the engine steps on with the queries it holds, or (no slot busy) waits
+            if (wait) cv.wait(lk, [&] { return !ready_q.empty() || !nosub_q.empty() || sel_end; });
+            if (!nosub_q.empty()) { q = nosub_q.front(); nosub_q.pop_front(); }
+            else if (!ready_q.empty()) { q = ready_q.front(); ready_q.pop_front(); sub = true; }
+            else return false;
+            --untaken;
+        }
+        cv.notify_all();
+        if (!sub) { full_job(q, j); return true; }
         Q& x = qs[(size_t)q];
         j = gst_batch_job();
         j.G = &x.S;
@@ -1546,17 +1592,6 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         j.kind = 0;
         return true;
     };
-    auto idle = [&](void* a_done) {
-        // queue and advance upcoming selections while the step runs; stop when its first part is
-        // done (the engine then collects finished queries)
-        const hipEvent_t done = (hipEvent_t)a_done;
-        do {
-            pump();
-            if (setup_err != GST_OK) break;
-            if (inflight.empty() && (ready.size() - ready_head >= (size_t)lookahead || !next_valid())) break;
-            std::this_thread::yield();
-        } while (hipEventQuery(done) == hipErrorNotReady);
-    };
     std::vector<gst_status> job_fail;   // statuses of failed jobs (the engine returns the first of them)
     auto job_done = [&](gst_batch_job& j) {
         const int32_t q = (int32_t)j.tag;
@@ -1566,7 +1601,11 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             x.L.t_sub_solve_ms = j.t_done - j.t_fill;
             if (j.status == GST_OK) { x.have_ub = true; x.ub = x.sub_res.cost; }
             gst_result_free(&x.sub_res);
-            if (x.buf >= 0) { free_bufs.push_back(x.buf); x.buf = -1; }
+            if (x.buf >= 0) {
+                std::lock_guard<std::mutex> lk(mu);
+                free_bufs.push_back(x.buf);
+            }
+            x.buf = -1;
             pending_full.push_back(q);
             return;
         }
@@ -1585,12 +1624,18 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     };
     int64_t valid = 0;   // a query has one job (its subgraph's or its full solve) in flight at a time
     for (int32_t q = 0; q < nq; ++q) valid += status[q] == GST_OK ? 1 : 0;
-    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid, idle,
-                                  std::max<int64_t>(valid, 1));
-    // selections still in flight (an engine error ends the run early) finish before their buffers go
+    std::thread sel_th(selector);
+    gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid,
+                                  std::function<void(void*)>(), std::max<int64_t>(valid, 1));
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;   // an engine error ends the run early
+    }
+    cv.notify_all();
+    sel_th.join();
+    // selections still queued finish before their buffers are reused
     HIPCHK(hipStreamSynchronize((hipStream_t)ctx->stream2));
     M->sel_stream = nullptr;
-    for (auto& b : bufs) sub_free(b);
     for (auto& x : qs) gst_result_free(&x.sub_res);
     if (setup_err != GST_OK) return setup_err;
     // a failed subgraph solve (no tree inside the subgraph) is no failure of its query: the

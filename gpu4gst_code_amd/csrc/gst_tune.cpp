@@ -144,7 +144,7 @@ extern "C" gst_status gst_autotune(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             double best_q = 0.0;
             for (int rep2 = 0; rep2 < (eager ? 1 : 2); ++rep2) {   // the first run builds the captured step
                 int32_t next = 0;
-                auto next_job = [&](gst_batch_job& j) -> bool {
+                auto next_job = [&](gst_batch_job& j, bool) -> bool {
                     if (next >= njobs) return false;
                     const int32_t q = next % nq;
                     j = gst_batch_job();

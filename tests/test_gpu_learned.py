@@ -212,6 +212,34 @@ def test_batch_learned_infeasible_and_long_runs(solver):
     dg.free()
 
 
+def test_buffers_reused_across_calls_and_graphs(solver):
+    """The model keeps its subgraph buffers between calls: calls on one graph with growing subgraphs
+    (top 0.5%, then 20%: every pooled buffer grows), then a smaller graph, then the one-query path
+    twice, all return the golden or oracle optimum."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=3)
+    keys = sorted(GOLD["ba20k"]["costs"], key=int)
+    qs = [gst.gen_groups(G.n, 6, 10, 200, seed=1000 + int(k)) for k in keys]
+    want = [GOLD["ba20k"]["costs"][k] for k in keys]
+    for frac, slots in ((0.005, 4), (0.2, 6), (0.05, 2)):
+        res, _ = solver.solve_batch_learned(dg, qs, model, frac, slots=slots)
+        print(f"frac {frac} slots {slots}: {[r.cost for r in res]}", flush=True)
+        assert [r.cost for r in res] == want
+    for frac in (0.2, 0.01):
+        r = solver.solve_learned(dg, qs[0], model, frac)
+        assert r.cost == want[0]
+    H = gst.gen_random_connected(1000, 5000, 1, 100, seed=3)
+    dh = solver.upload(H)
+    hq = [gst.gen_groups(H.n, 5, 2, 20, seed=4100 + i) for i in range(6)]
+    res, _ = solver.solve_batch_learned(dh, hq, model, 0.1, slots=3)
+    for q, r in zip(hq, res):
+        assert r.cost == oracle.dpbf_cost(H, q)
+    model.free()
+    dh.free()
+    dg.free()
+
+
 def test_trained_network_gain(solver):
     """What the trained network gains over the same method without training: on BA V=50k queries
     (g=6, groups of 10-200) the trained scores give a feasible subgraph tree far more often and a
