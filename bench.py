@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--groups", type=int, default=6)
     ap.add_argument("--frac", type=float, default=0.05, help="top-scored fraction of vertices kept in the subgraph")
-    ap.add_argument("--slots", type=int, default=96, help="queries in flight in the batched engine")
+    ap.add_argument("--slots", type=int, default=128, help="queries in flight in the batched engine")
     ap.add_argument("--model", default=os.path.join(ROOT, "gpu4gst_code_amd", "models", "mpnn_ba50k.npy"))
     ap.add_argument("--autotune", type=int, default=1, help="1: pick launch knobs with gst_autotune before timing")
     ap.add_argument("--rounds-per-step", type=int, default=0, help="engine rounds per step (0: tuned / default)")

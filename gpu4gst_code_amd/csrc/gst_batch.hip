@@ -166,16 +166,38 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
     }
 }
 
-template <bool FLT>
+// Merges of every slot whose level finished, and the tour tables of the slots entering level 2.
+// The first MAX_SLOTS workgroups build the tables (one per slot, from its final level 1): the grow
+// rounds are their first readers (the merge applies the one-label bound only), so the table builds,
+// short chains of dependent phases, run beside the merges instead of before them.  The other
+// workgroups (at least MAX_SLOTS) split among the merges.
+template <bool FLT, bool PW_LDS>
 __global__ void __launch_bounds__(MERGE_THREADS) k_batch_merge(BatchArgs b) {
     extern __shared__ unsigned long long lds[];
     const BatchPlan* p = b.plan;
     const uint32_t nm = p->n_merge;
     if (nm == 0) return;
-    const uint32_t per = max(1u, gridDim.x / nm);   // gridDim.x >= MAX_SLOTS, so per >= 1 covers all
-    const uint32_t i = blockIdx.x / per;
+    if (blockIdx.x < MAX_SLOTS) {
+        const uint32_t i = blockIdx.x;
+        if (i >= nm || p->merge_level[i] != 2) return;
+        const MergeArgs& m2 = b.sb[p->merge_slot[i]].m[2];
+        if (!m2.tour) return;
+        tour_table_body<FLT, PW_LDS>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
+                                     const_cast<uint32_t*>(m2.tour));
+        // a table switched off (walks too short to prune) leaves the slot's grow rounds: they then
+        // read no table at all
+        __syncthreads();
+        if (threadIdx.x == 0 && m2.tour[0] == 0u) {
+            SlotBlock& sb = b.sb[p->merge_slot[i]];
+            for (uint32_t k = 2; k < m2.g; ++k) sb.g[k].tour = nullptr;
+        }
+        return;
+    }
+    const uint32_t bid = blockIdx.x - MAX_SLOTS, nblk = gridDim.x - MAX_SLOTS;
+    const uint32_t per = max(1u, nblk / nm);   // nblk >= MAX_SLOTS, so per >= 1 covers all
+    const uint32_t i = bid / per;
     if (i >= nm) return;
-    merge_body<FLT>(b.sb[p->merge_slot[i]].m[p->merge_level[i]], lds, blockIdx.x - i * per, per);
+    merge_body<FLT>(b.sb[p->merge_slot[i]].m[p->merge_level[i]], lds, bid - i * per, per);
 }
 
 // after the merges: a slot whose level-2 merge found a spider (automatic bound) tightens the bound
@@ -204,27 +226,6 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
             if (ub < ma->ub) ma->ub = ub;
             if (gmax < ma->gmax) ma->gmax = gmax;
         }
-    }
-}
-
-// before the merges: a slot entering level 2 builds its tour table from its final level 1 (one
-// workgroup per slot; the level-2 merge is the table's first reader)
-template <bool FLT, bool PW_LDS>
-__global__ void __launch_bounds__(256) k_batch_tour(BatchArgs b) {
-    const BatchPlan* p = b.plan;
-    const uint32_t i = blockIdx.x;
-    if (i >= p->n_merge || p->merge_level[i] != 2) return;
-    const MergeArgs& m2 = b.sb[p->merge_slot[i]].m[2];
-    if (!m2.tour) return;
-    // the launch's LDS holds the walk table of the engine's largest group count
-    tour_table_body<FLT, PW_LDS>(m2.T + m2.lvl_base[1], m2.grp, m2.grp + m2.g + 1, m2.g, m2.lbcap, m2.tour_w,
-                                 const_cast<uint32_t*>(m2.tour));
-    // a table switched off (walks too short to prune) leaves the slot's grow rounds: they then read
-    // no table at all
-    __syncthreads();
-    if (threadIdx.x == 0 && m2.tour[0] == 0u) {
-        SlotBlock& sb = b.sb[p->merge_slot[i]];
-        for (uint32_t k = 2; k < m2.g; ++k) sb.g[k].tour = nullptr;
     }
 }
 
@@ -711,17 +712,18 @@ static void enqueue_step_a(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& b
     }
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + pairs_lds;
     k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
+    // the tour tables' walk table sits in the merge launch's LDS when it fits there (the launch's LDS,
+    // and so the merges' workgroups per CU, stays as the merges need), else in the slot's scratch
     const size_t tl = tour_lds_bytes((uint32_t)E->g_cap);
-    if (tl) {
-        if (flt) k_batch_tour<true, true><<<MAX_SLOTS, 256, tl, s>>>(ba);
-        else k_batch_tour<false, true><<<MAX_SLOTS, 256, tl, s>>>(ba);
-    } else {
-        if (flt) k_batch_tour<true, false><<<MAX_SLOTS, 256, 0, s>>>(ba);
-        else k_batch_tour<false, false><<<MAX_SLOTS, 256, 0, s>>>(ba);
-    }
+    const bool pw_lds = tl != 0 && tl <= mlds;
     if (ev) hipEventRecord(ev[0], s);
-    if (flt) k_batch_merge<true><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
-    else k_batch_merge<false><<<mgrid, MERGE_THREADS, mlds, s>>>(ba);
+    if (pw_lds) {
+        if (flt) k_batch_merge<true, true><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);
+        else k_batch_merge<false, true><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);
+    } else {
+        if (flt) k_batch_merge<true, false><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);
+        else k_batch_merge<false, false><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);
+    }
     if (ev) hipEventRecord(ev[1], s);
     if (flt) k_batch_bound<true><<<1, PLAN_THREADS, 0, s>>>(ba);
     else k_batch_bound<false><<<1, PLAN_THREADS, 0, s>>>(ba);
