@@ -579,13 +579,15 @@ __global__ void __launch_bounds__(256) k_key_hist(const float* s, uint32_t n, in
 // left): thread t sums nbins/256 bins from the top, a prefix over the 256 partial sums finds the
 // thread whose range holds the crossing, and that thread walks its bins and appends the digit.
 // Each thread then clears its own bins, so the next pass (or selection) starts from a zero histogram.
-__global__ void __launch_bounds__(256) k_pick_bin(uint32_t* hist, uint32_t* sel_state, int pass) {
+// The first pass starts the state itself: k_target keys wanted, no digits yet.
+__global__ void __launch_bounds__(256) k_pick_bin(uint32_t* hist, uint32_t* sel_state, int pass, uint32_t k_target) {
     __shared__ uint32_t part[256];
     uint32_t lo, nb;
     radix_digit(pass, lo, nb);
     const uint32_t nbins = 1u << nb, per = nbins / 256u;
     const uint32_t t = threadIdx.x;
-    const uint32_t need = sel_state[0];
+    const uint32_t need = pass == 0 ? k_target : sel_state[0];
+    const uint32_t prefix = pass == 0 ? 0u : sel_state[1];   // read before the barrier below
     const uint32_t top = nbins - 1u - per * t;
     uint32_t sum = 0;
     for (uint32_t j = 0; j < per; ++j) sum += hist[top - j];
@@ -598,7 +600,7 @@ __global__ void __launch_bounds__(256) k_pick_bin(uint32_t* hist, uint32_t* sel_
         for (uint32_t j = 0; j < per; ++j) {
             const uint32_t h = hist[top - j];
             if (acc + h >= need) {
-                sel_state[1] = (sel_state[1] << nb) | (top - j);
+                sel_state[1] = (prefix << nb) | (top - j);
                 sel_state[0] = need - acc;
                 break;
             }
@@ -606,23 +608,6 @@ __global__ void __launch_bounds__(256) k_pick_bin(uint32_t* hist, uint32_t* sel_
         }
     }
     for (uint32_t j = 0; j < per; ++j) hist[top - j] = 0u;
-}
-
-__global__ void k_sel_init(uint32_t* sel_state, uint32_t k_target) {
-    if (threadIdx.x < 8) sel_state[threadIdx.x] = threadIdx.x == 0 ? k_target : 0u;
-}
-
-// select v when key > threshold or key == threshold within the tie budget, plus every group member
-__global__ void k_select(const float* s, const uint32_t* gm, uint32_t n, uint32_t* sel_state, uint32_t* sel) {
-    const uint32_t thr = sel_state[1];
-    int* tie_budget = (int*)&sel_state[0];
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        const uint32_t k = score_key(s[v]);
-        uint32_t x = (k > thr) ? 1u : 0u;
-        if (k == thr && atomicSub(tie_budget, 1) > 0) x = 1u;   // signed: stays <= 0 once spent
-        if (gm[v]) x = 1u;
-        sel[v] = x;
-    }
 }
 
 // per-block exclusive scan of a u32 array (block = 1024 elements), block sums out
@@ -657,8 +642,55 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* in, uint32_
     if (threadIdx.x == 0) bsum[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
 }
 
-// exclusive scan of the block sums by one wave (64 at a time with a running offset), total in bsum[nb]
-__global__ void __launch_bounds__(64) k_scan_sums(uint32_t* bsum, uint32_t nb) {
+// the selection and the first pass of its scan in one: v is selected when key > threshold or key ==
+// threshold within the tie budget, plus every group member; sel[v] out, then as k_scan_blocks
+// (exclusive per-block offsets of the selected vertices, block sums out)
+__global__ void __launch_bounds__(256) k_select_scan(const float* s, const uint32_t* gm, uint32_t n,
+                                                     uint32_t* sel_state, uint32_t* sel, uint32_t* out,
+                                                     uint32_t* bsum) {
+    __shared__ uint32_t sw[4];
+    const uint32_t thr = sel_state[1];
+    int* tie_budget = (int*)&sel_state[0];
+    const uint32_t base = blockIdx.x * 1024;
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+    uint32_t vals[4], tot = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t v = base + threadIdx.x * 4 + j;
+        uint32_t x = 0;
+        if (v < n) {
+            const uint32_t k = score_key(s[v]);
+            x = (k > thr) ? 1u : 0u;
+            if (k == thr && atomicSub(tie_budget, 1) > 0) x = 1u;   // signed: stays <= 0 once spent
+            if (gm[v]) x = 1u;
+            sel[v] = x;
+        }
+        vals[j] = x;
+        tot += x;
+    }
+    uint32_t incl = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    if (lane == 63) sw[wid] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t w2 = 0; w2 < wid; ++w2) off += sw[w2];
+    uint32_t run = off + incl - tot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = base + threadIdx.x * 4 + j;
+        if (i < n) out[i] = run;
+        run += vals[j];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];
+}
+
+// exclusive scan of the block sums by one wave (64 at a time with a running offset), total in
+// bsum[nb] and, when given, in *total_out
+__global__ void __launch_bounds__(64) k_scan_sums(uint32_t* bsum, uint32_t nb, uint32_t* total_out) {
     if (blockIdx.x != 0) return;
     const uint32_t lane = threadIdx.x;
     uint32_t run = 0;
@@ -674,25 +706,32 @@ __global__ void __launch_bounds__(64) k_scan_sums(uint32_t* bsum, uint32_t nb) {
         if (i < nb) bsum[i] = run + incl - x;
         run += __shfl(incl, 63, 64);
     }
-    if (lane == 0) bsum[nb] = run;
+    if (lane == 0) {
+        bsum[nb] = run;
+        if (total_out) *total_out = run;
+    }
 }
 
-__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* bsum) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] += bsum[i / 1024];
-}
 
-// list of the selected vertices by new id (old_of[newid[v]] = v), and every degree cleared
-__global__ void k_sub_list(const uint32_t* sel, const uint32_t* newid, uint32_t n, uint32_t* old_of, uint32_t* deg) {
+// the last pass of the new-id scan (block offsets added), the list of the selected vertices by new
+// id (old_of[newid[v]] = v), and every degree cleared
+__global__ void k_scan_add_list(uint32_t* newid, uint32_t n, const uint32_t* bsum, const uint32_t* sel,
+                                uint32_t* old_of, uint32_t* deg) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t id = newid[v] + bsum[v / 1024];
+        newid[v] = id;
         deg[v] = 0u;
-        if (sel[v]) old_of[newid[v]] = v;
+        if (sel[v]) old_of[id] = v;
     }
 }
 
 // degree of each selected vertex inside the selection (one wave per selected vertex; *ns of them)
+// The group members' new ids (gv: cnt member entries) are written first, one thread per entry.
 __global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_t* sel, const uint32_t* old_of,
-                          const uint32_t* ns, uint32_t* deg) {
+                          const uint32_t* ns, uint32_t* deg, const int32_t* gv, uint32_t gcnt, const uint32_t* newid,
+                          int32_t* sub_gv) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < gcnt; i += gridDim.x * blockDim.x)
+        sub_gv[i] = (int32_t)newid[gv[i]];
     const uint32_t lane = __lane_id();
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * blockDim.x / 64;
@@ -708,15 +747,19 @@ __global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_
 
 // fill the subgraph CSR: new row i holds old vertex old_of[i]; arcs keep their order (new ids are
 // monotone in the old ones, so rows stay in (weight, neighbour) order)
+// arc_off: per-block exclusive offsets of the arc scan, arc_bsum: its scanned block sums (added here,
+// as the scan's last pass), total: the arc count (the subgraph's closing row offset)
 __global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32_t* w, const uint32_t* sel,
-                           const uint32_t* newid, const uint32_t* arc_off, const uint32_t* old_of, uint32_t ns,
-                           uint32_t* srow, int32_t* scol, uint32_t* sw) {
+                           const uint32_t* newid, const uint32_t* arc_off, const uint32_t* arc_bsum,
+                           const uint32_t* total, const uint32_t* old_of, uint32_t ns, uint32_t* srow, int32_t* scol,
+                           uint32_t* sw) {
     const uint32_t lane = __lane_id();
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+    if (blockIdx.x == 0 && threadIdx.x == 0) srow[ns] = *total;
     for (uint32_t nv = gwave; nv < ns; nv += nwaves) {
         const uint32_t v = old_of[nv];
-        uint32_t pos = arc_off[v];
+        uint32_t pos = arc_off[v] + arc_bsum[v / 1024];
         if (lane == 0) srow[nv] = pos;
         for (uint32_t e0 = row[v]; e0 < row[v + 1]; e0 += 64) {
             const uint32_t e = e0 + lane;
@@ -751,14 +794,6 @@ __global__ void k_sub_rev(const uint32_t* srow, const int32_t* scol, const uint3
     }
 }
 
-__global__ void k_set_u32(uint32_t* p, const uint32_t* val) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = *val;
-}
-
-__global__ void k_sub_groups(const int32_t* gv, uint32_t cnt, const uint32_t* newid, int32_t* out) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
-        out[i] = (int32_t)newid[gv[i]];
-}
 
 // Layer-1 aggregation of a query splits into a graph-only part and a group part: a vertex's input
 // features are two graph-only values (degree, lightest arc), the membership bits of groups 0..7 and
@@ -841,7 +876,7 @@ bool dalloc(T*& p, size_t elems) {
 // waiting between them (they share the model's activations in stream order).
 struct SelSlot {
     uint32_t* scr = nullptr;     // device: hist 2048 | state 8 | sel n | newid n | deg n | arcoff n | old n |
-    uint64_t cap = 0;            //         block sums 2 (nbk + 2) | sub group ids gcnt
+    uint64_t cap = 0;            //         block sums 2 (nbk + 2) | ns, ms, sub group ids gcnt
     uint32_t* h_rb = nullptr;    // pinned readback: ns, ms, sub group ids
     uint32_t h_cap = 0;
     int32_t* h_grp = nullptr;    // pinned group lists of the query (upload source)
@@ -1219,7 +1254,7 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
     const int g = groups->n_groups;
     const uint32_t gcnt = (uint32_t)groups->grp_ptr[g];
     const uint32_t nbk = (n + 1023) / 1024;
-    const uint64_t need = 2048ull + 8 + 5ull * n + 2ull * (nbk + 2) + gcnt + 64;
+    const uint64_t need = 2048ull + 8 + 5ull * n + 2ull * (nbk + 2) + 2 + gcnt + 64;
     if (sl.cap < need) {
         if (!dalloc(sl.scr, need)) { sl.cap = 0; return gst_set_error(ctx, GST_OOM, "learned solve: scratch allocation failed"); }
         sl.cap = need;
@@ -1256,28 +1291,23 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
     uint32_t* d_arcoff = d_deg + n;
     uint32_t* d_old = d_arcoff + n;
     uint32_t* d_bsum = d_old + n;        // 2 * (nbk + 2)
-    int32_t* d_subgrp = (int32_t*)(d_bsum + 2 * (nbk + 2));
-    // ---- top-k threshold on the device ----
-    k_sel_init<<<1, 64, 0, st>>>(d_state, k_target);
+    uint32_t* d_hdr = d_bsum + 2 * (nbk + 2);   // vertex count, arc count, then the members' new ids
+    int32_t* d_subgrp = (int32_t*)(d_hdr + 2);
+    // ---- top-k threshold on the device (the first pass starts the state) ----
     const int hg = std::max(1, std::min(ctx->cu_count, (int)((n + 4095) / 4096)));
     for (int pass = 0; pass < 3; ++pass) {
         k_key_hist<<<hg, 256, 0, st>>>(M->s, n, pass, d_state, d_hist);
-        k_pick_bin<<<1, 256, 0, st>>>(d_hist, d_state, pass);
+        k_pick_bin<<<1, 256, 0, st>>>(d_hist, d_state, pass, k_target);
     }
-    k_select<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, M->gm, n, d_state, d_sel);
-    // ---- induced subgraph: new ids, arc offsets, group lists in new ids ----
-    k_scan_blocks<<<nbk, 256, 0, st>>>(d_sel, n, d_newid, d_bsum);
-    k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nbk);
-    k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_newid, n, d_bsum);
-    k_sub_list<<<gridn(n, 256, fg), 256, 0, st>>>(d_sel, d_newid, n, d_old, d_deg);
-    k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, d_old, d_bsum + nbk, d_deg);
+    // ---- induced subgraph: selection and new ids, arc offsets, group lists in new ids ----
+    k_select_scan<<<nbk, 256, 0, st>>>(M->s, M->gm, n, d_state, d_sel, d_newid, d_bsum);
+    k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nbk, d_hdr);
+    k_scan_add_list<<<gridn(n, 256, fg), 256, 0, st>>>(d_newid, n, d_bsum, d_sel, d_old, d_deg);
+    k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, d_old, d_bsum + nbk, d_deg,
+                                                              M->grp + g + 1, gcnt, d_newid, d_subgrp);
     k_scan_blocks<<<nbk, 256, 0, st>>>(d_deg, n, d_arcoff, d_bsum + nbk + 2);
-    k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nbk + 2, nbk);
-    k_scan_add<<<gridn(n, 256, fg), 256, 0, st>>>(d_arcoff, n, d_bsum + nbk + 2);
-    if (gcnt) k_sub_groups<<<gridn(gcnt, 256, 64), 256, 0, st>>>(M->grp + g + 1, gcnt, d_newid, d_subgrp);
-    HIPCHK(hipMemcpyAsync(&sl.h_rb[0], d_bsum + nbk, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&sl.h_rb[1], d_bsum + nbk + 2 + nbk, 4, hipMemcpyDeviceToHost, st));
-    if (gcnt) HIPCHK(hipMemcpyAsync(&sl.h_rb[2], d_subgrp, gcnt * 4ull, hipMemcpyDeviceToHost, st));
+    k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nbk + 2, nbk, d_hdr + 1);   // offsets completed by k_sub_fill
+    HIPCHK(hipMemcpyAsync(sl.h_rb, d_hdr, (2ull + gcnt) * 4ull, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(sl.e_sizes, st));
     HIPCHK(hipGetLastError());
     return GST_OK;
@@ -1313,8 +1343,8 @@ gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& 
         uint32_t* d_old = d_arcoff + n;
         uint32_t* d_bsum = d_old + n;
         k_sub_fill<<<gridn((uint64_t)ns * 64, 256, ctx->cu_count * 8), 256, 0, st>>>(
-            G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, d_old, ns, S.d_row, S.d_col, S.d_w);
-        k_set_u32<<<1, 1, 0, st>>>(S.d_row + ns, d_bsum + nbk + 2 + nbk);
+            G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, d_bsum + nbk + 2, d_bsum + nbk + 2 + nbk, d_old, ns,
+            S.d_row, S.d_col, S.d_w);
         k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S.d_row, S.d_col, S.d_w, ns, S.d_rev);
     }
     HIPCHK(hipEventRecord(sl.e_built, st));
