@@ -167,12 +167,18 @@ def main():
                          "sample": f"4 of the 64 queries, bounded label-setting DPBF (oracle orc_dpbf_pruned, "
                                    f"spider bound, one-label and tour LBs, half-bound grow), {pw:.1f} s wall"}
 
-    traffic = None
+    glaunch = max(int(bst["grow_launches"]), 1)
+    # HBM traffic per grow launch from the committed rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE,
+    # calibrated): their bytes per dispatch over their algorithmic bytes per dispatch, applied to this
+    # run's algorithmic bytes per launch (under the profiler the host is slower, so its steps hold
+    # fewer queries and its launches are smaller than here)
+    traffic, t_ratio = None, None
     tpath = os.path.join(ROOT, "profiles", "dpbf_learned_batch_ba200k_g6_traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("k_batch_grow", {}).get("traffic_bytes_per_dispatch")
-
-    glaunch = max(int(bst["grow_launches"]), 1)
+        tg = json.load(open(tpath)).get("k_batch_grow", {})
+        if tg.get("traffic_bytes_per_dispatch") and tg.get("algorithmic_bytes_per_dispatch"):
+            t_ratio = tg["traffic_bytes_per_dispatch"] / tg["algorithmic_bytes_per_dispatch"]
+            traffic = int(t_ratio * bst["grow_bytes"] / glaunch)
     mlaunch = max(int(bst["merge_launches"]), 1)
     line = {
         "metric": "exact-cost GST queries/s (learned-UB batched DPBF vs plain DPBF, BA V=200k g=6)",
@@ -203,6 +209,7 @@ def main():
         "state_relax_per_s": round(p_relax / (p_grow_ms * 1e-3), 1) if p_grow_ms > 0 else None,
         "roofline": {"bound": "hbm", "achieved": round(b_grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_grow_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_over_algorithmic": round(t_ratio, 2) if t_ratio else None,
                      "kernel": "k_batch_grow", "launches": glaunch,
                      "bytes_per_launch": round(bst["grow_bytes"] / glaunch, 1),
                      "ms_per_launch": round(bst["t_grow_ms"] / glaunch, 5)},
