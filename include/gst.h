@@ -262,7 +262,9 @@ gst_status gst_solve_learned(gst_ctx* ctx, const gst_graph* g, const gst_groups*
  * exact optimum, as from gst_solve.  opts (NULL or nq entries): prune_lb, want_tree, push_lanes and
  * half_bound are used (ub is set from the subgraph).  lstats (NULL or nq entries) receives each
  * query's subgraph size, bound and times (t_sub_solve_ms / t_full_solve_ms: job latency in the
- * engine).  No counterpart in the reference. */
+ * engine).  The call runs the selections on a host thread of its own (device set to the context's,
+ * joined before the call returns) beside the calling thread, which runs the engine; the model keeps
+ * its subgraph buffers for later calls.  No counterpart in the reference. */
 gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* groups,
                                    gst_model* m, double topk_frac, const gst_opts* opts,
                                    const gst_batch_opts* bopts, gst_result* results, gst_status* status,
