@@ -1338,6 +1338,13 @@ constexpr size_t MERGE_TILE_BYTES = GST_MERGE_TILE_KB * 1024;
 #define GST_MERGE_UNR 4
 #endif
 constexpr int MERGE_UNR = GST_MERGE_UNR;
+// at most this many vertices per merge tile: low levels (few lower-level words per vertex) would
+// otherwise take tiles far below the LDS budget (measured on BA V=200k g=6: 512 takes the one-query
+// merge from 272 to 246 us per query and the batched merge 2% down against 256; 1024 is no better,
+// 128 is 15% slower)
+#ifndef GST_MERGE_VB_MAX
+#define GST_MERGE_VB_MAX 512
+#endif
 // split tables up to this size are staged in LDS
 constexpr size_t MERGE_PAIRS_LDS = 16 * 1024;
 
@@ -1351,7 +1358,7 @@ __host__ __device__ __forceinline__ uint32_t merge_tile_vertices(uint32_t L, uin
     const size_t per = (size_t)L * 4 + (size_t)W * 8 + 4;
     size_t vb = (tile_bytes ? tile_bytes : MERGE_TILE_BYTES) / per;
     if (vb < 1) vb = 1;
-    if (vb > 256) vb = 256;
+    if (vb > GST_MERGE_VB_MAX) vb = GST_MERGE_VB_MAX;
     return (uint32_t)vb;
 }
 // dynamic LDS bytes of a merge launch: rows, bitmaps, group masks of the tile, split table
