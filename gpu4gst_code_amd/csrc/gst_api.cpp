@@ -324,7 +324,7 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
     int64_t valid = 0;
     for (int32_t q = 0; q < nq; ++q) valid += status[q] == GST_OK ? 1 : 0;
     const gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)g->n, gmax, g->m, bopts, stats, g->uid,
-                                        std::function<void(void*)>(), std::max<int64_t>(valid, 1));
+                                        std::max<int64_t>(valid, 1));
     if (first == GST_OK) first = rs;
     return first;
 }

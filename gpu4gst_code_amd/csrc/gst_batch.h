@@ -26,14 +26,10 @@ struct gst_batch_job {
 // true (asked only when no slot is busy) it blocks until a job is ready, and false then means no job
 // will come.  job_done receives each finished job (status and result set) and may make more jobs
 // available to next_job.  n_cap / g_cap / m_cap bound every job's graph size and group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
-// tune_uid names the graph whose autotuned engine knobs apply (0: none).  idle (may be empty) runs
-// on the host after each step is enqueued, while the host waits for the step's first part (plan,
-// merges, tracebacks), whose completion event (a hipEvent_t) it receives: host-side or other-stream
-// work overlaps the step, and hipEventQuery on the event tells when to stop.  max_jobs (0: unknown)
+// tune_uid names the graph whose autotuned engine knobs apply (0: none).  max_jobs (0: unknown)
 // bounds the jobs in flight at once: without an explicit slot count the engine sizes no more slots.
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool wait)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bopts, gst_batch_stats* stats, uint64_t tune_uid = 0,
-                         const std::function<void(void* a_done)>& idle = std::function<void(void*)>(),
                          int64_t max_jobs = 0);
 void gst_batch_free(gst_ctx* ctx);

@@ -746,7 +746,7 @@ static void enqueue_step_b(gst_ctx* ctx, const BatchArgs& ba, int R, bool flt, c
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
-                         const std::function<void(void*)>& idle, int64_t max_jobs) {
+                         int64_t max_jobs) {
     gst_batch_opts bo{};
     if (bo_in) bo = *bo_in;
     if (!ctx->be) ctx->be = new gst_batch_engine();
@@ -904,7 +904,6 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, 
         else enqueue_step_b(ctx, ba, R, flt, te);
         HIPCHK(hipGetLastError());
         steps++;
-        if (idle) idle((void*)E->a_done);
         HIPCHK(hipEventSynchronize(E->a_done));
         if (timed && steps >= 2) {
             st = add_times(steps - 2);

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <condition_variable>
 #include <deque>
+#include <exception>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -1654,9 +1655,15 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     };
     int64_t valid = 0;   // a query has one job (its subgraph's or its full solve) in flight at a time
     for (int32_t q = 0; q < nq; ++q) valid += status[q] == GST_OK ? 1 : 0;
-    std::thread sel_th(selector);
+    std::thread sel_th;
+    try {
+        sel_th = std::thread(selector);
+    } catch (const std::exception&) {   // no exception crosses the C ABI
+        M->sel_stream = nullptr;
+        return gst_set_error(ctx, GST_INTERNAL, "learned batch: cannot start the selection thread");
+    }
     gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid,
-                                  std::function<void(void*)>(), std::max<int64_t>(valid, 1));
+                                  std::max<int64_t>(valid, 1));
     {
         std::lock_guard<std::mutex> lk(mu);
         stop = true;   // an engine error ends the run early
