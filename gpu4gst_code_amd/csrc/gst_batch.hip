@@ -43,12 +43,16 @@ namespace {
 
 constexpr uint32_t MAX_SLOTS = 256;
 // workgroups a batched grow round may use, per CU
+// 1: a new query's set-up (g >= 2) runs batched in the next step's first part; 0: at once, per query
+#ifndef GST_BATCH_DEFER_FILL
+#define GST_BATCH_DEFER_FILL 1
+#endif
 #ifndef GST_BATCH_GRID_MULT
 #define GST_BATCH_GRID_MULT 5
 #endif
 constexpr uint32_t PLAN_THREADS = MAX_SLOTS;          // the plan kernels run one thread per slot
 constexpr uint32_t PLAN_WAVES = PLAN_THREADS / WAVE;
-enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3 };
+enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3, PH_FILL = 4 };
 
 struct SlotDev {
     uint32_t phase;
@@ -68,7 +72,29 @@ struct BatchPlan {
     uint32_t compact_slot[MAX_SLOTS];
     uint32_t compact_level[MAX_SLOTS];
     uint32_t trace_slot[MAX_SLOTS];
+    uint32_t n_fill;
+    uint32_t fill_slot[MAX_SLOTS];         // slots whose new query is set up by this step's part A
     SlotDev sd_out[MAX_SLOTS];             // the slots' phases after the step's plan (read by the host)
+};
+
+// set-up of a slot's new query (g >= 2), run by the next step's part A: group bits, level 1 and
+// its frontier, the per-job counters cleared
+struct FillArgs {
+    uint32_t* gm;
+    unsigned long long* root;
+    unsigned long long* stats;
+    uint32_t* spider;
+    uint32_t* ctl;            // the slot's round-control blocks, as words
+    uint32_t* hub_cnt;
+    const int32_t* grp;       // grp_ptr[0..g] then the members
+    uint64_t* T1;
+    const uint32_t* row;
+    uint64_t* chg0;
+    uint32_t* inq0;
+    uint64_t* list0;
+    uint32_t* cnt0;           // level 1's first frontier count
+    uint64_t* hub_items;      // a job's own graph: its hub row items are listed too (nullptr: none)
+    uint32_t n, g, chunk, pad;
 };
 
 // a slot's argument blocks, contiguous so that filling a slot is one copy
@@ -76,6 +102,7 @@ struct SlotBlock {
     GrowArgs g[NCTL];      // by level
     MergeArgs m[NCTL];     // by level
     TraceArgs t;
+    FillArgs f;
     SlotDev sd;
 };
 
@@ -125,18 +152,90 @@ __global__ void k_slot_reset(uint32_t* gm, uint32_t n, unsigned long long* root,
     }
 }
 
+// The slots set up by a step share the launch's workgroups: slot i of the fill list takes
+// workgroups [i * per, (i + 1) * per) (per >= 1 since the grid has at least MAX_SLOTS workgroups).
+__device__ __forceinline__ const FillArgs* fill_share(const BatchArgs& b, uint32_t& bid, uint32_t& nblk) {
+    const BatchPlan* p = b.plan;
+    const uint32_t nf = p->n_fill;
+    if (nf == 0) return nullptr;
+    const uint32_t per = max(1u, gridDim.x / nf);
+    const uint32_t i = blockIdx.x / per;
+    if (i >= nf) return nullptr;
+    bid = blockIdx.x - i * per;
+    nblk = per;
+    return &b.sb[p->fill_slot[i]].f;
+}
+
+// fill, first pass: group bits cleared, root = inf, statistics, spider bound, control blocks and hub
+// item count reset
+__global__ void __launch_bounds__(256) k_batch_fill_reset(BatchArgs b) {
+    uint32_t bid, nblk;
+    const FillArgs* f = fill_share(b, bid, nblk);
+    if (!f) return;
+    for (uint32_t v = bid * blockDim.x + threadIdx.x; v < f->n; v += nblk * blockDim.x) f->gm[v] = 0u;
+    if (bid == 0) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(RoundCtl) * NCTL / 4); i += blockDim.x) f->ctl[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)ST_NSTAT; i += blockDim.x) f->stats[i] = 0ull;
+        if (threadIdx.x == 0) {
+            *f->root = GST_INF_WORD;
+            *f->spider = GST_INF_COST;
+            *f->hub_cnt = 0u;
+        }
+    }
+}
+
+// fill, second pass: the members' group bits
+__global__ void __launch_bounds__(256) k_batch_fill_bits(BatchArgs b) {
+    uint32_t bid, nblk;
+    const FillArgs* f = fill_share(b, bid, nblk);
+    if (!f) return;
+    for (uint32_t i = 0; i < f->g; ++i) {
+        const int32_t s0 = f->grp[i], e = f->grp[i + 1];
+        for (int32_t j = s0 + (int32_t)(bid * blockDim.x + threadIdx.x); j < e; j += (int32_t)(nblk * blockDim.x))
+            atomicOr(&f->gm[f->grp[f->g + 1 + j]], 1u << i);
+    }
+}
+
+// fill, third pass: level 1 (T(v,{i}) = 0 for members of group i, inf elsewhere), members with arcs
+// flagged as the first frontier
+__global__ void __launch_bounds__(256) k_batch_fill_level1(BatchArgs b) {
+    uint32_t bid, nblk;
+    const FillArgs* f = fill_share(b, bid, nblk);
+    if (!f) return;
+    const uint32_t g = f->g;
+    for (uint32_t v = bid * blockDim.x + threadIdx.x; v < f->n; v += nblk * blockDim.x) {
+        const uint32_t gmv = f->gm[v];
+        for (uint32_t i = 0; i < g; ++i)
+            f->T1[(uint64_t)v * g + i] = ((gmv >> i) & 1u) ? (uint64_t)GST_PRED_LEAF : GST_INF_WORD;
+        if (gmv && f->row[v + 1] > f->row[v]) { f->chg0[v] = gmv; f->inq0[v] = 1u; }
+    }
+}
+
+// fill, last pass: level 1's work-item list, and a job's own graph's hub row items
+__global__ void __launch_bounds__(256) k_batch_fill_compact(BatchArgs b) {
+    uint32_t bid, nblk;
+    const FillArgs* f = fill_share(b, bid, nblk);
+    if (!f) return;
+    compact_body(f->inq0, f->row, f->n, f->chunk, f->list0, f->cnt0, f->stats, 0u, bid, nblk);
+    if (f->hub_items)
+        compact_body(nullptr, f->row, f->n, f->chunk, f->hub_items, f->hub_cnt, nullptr, HUB_DEG, bid, nblk);
+}
+
 __global__ void k_reset_phases(SlotBlock* sb) {
     if (threadIdx.x < MAX_SLOTS) sb[threadIdx.x].sd = SlotDev{PH_EMPTY, 0u, 0u, 0u};
 }
 
 __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
     const uint32_t s = threadIdx.x;
-    bool dm = false, dc = false, dt = false;
+    bool dm = false, dc = false, dt = false, df = false;
     uint32_t lvl = 0;
     if (s < b.S) {
         SlotDev* sd = &b.sb[s].sd;
         const uint32_t ph = sd->phase;
-        if (ph == PH_TRACE) {
+        if (ph == PH_FILL) {
+            df = true;
+            sd->phase = PH_GROW;          // set up by this part A, grown from part B on
+        } else if (ph == PH_TRACE) {
             dt = true;
             sd->phase = PH_DONE;
         } else if (ph == PH_GROW && b.sb[s].g[sd->k].ctl->level_done) {
@@ -151,18 +250,21 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
         }
     }
     BatchPlan* p = b.plan;
-    uint32_t nm, nc, nt;
+    uint32_t nm, nc, nt, nf;
     const uint32_t im = plan_scan(dm ? 1u : 0u, nm);
     const uint32_t ic = plan_scan(dc ? 1u : 0u, nc);
     const uint32_t it = plan_scan(dt ? 1u : 0u, nt);
+    const uint32_t jf = plan_scan(df ? 1u : 0u, nf);
     if (dm) { p->merge_slot[im] = s; p->merge_level[im] = lvl; }
     if (dc) { p->compact_slot[ic] = s; p->compact_level[ic] = lvl; }
     if (dt) p->trace_slot[it] = s;
+    if (df) p->fill_slot[jf] = s;
     if (s < b.S) p->sd_out[s] = b.sb[s].sd;
     if (s == 0) {
         p->n_merge = nm;
         p->n_compact = nc;
         p->n_trace = nt;
+        p->n_fill = nf;
     }
 }
 
@@ -590,19 +692,33 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
     const gst_bound_bits bb = gst_make_bound(job.opts.ub, flt, job.opts.half_bound);
     const uint32_t ubb = bb.ub;
-    const int fg = std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8));
-    k_slot_reset<<<fg, 256, 0, s>>>(sl.gm, n, sl.root, sl.stats, sl.spider, (uint32_t*)sl.ctl, sl.hub_cnt);
-    k_set_group_bits<<<64, 256, 0, s>>>(sl.gm, sl.grp, sl.grp + g + 1, g);
-    k_init_level1<<<fg, 256, 0, s>>>(sl.T + lvl_base[1], sl.gm, G->d_row, n, (uint32_t)g, sl.chg[0], sl.inq[0],
-                                    g > 1 ? 1 : 0);
-    if (g > 1)
-        k_compact<<<fg, 256, 0, s>>>(sl.inq[0], G->d_row, n, chunk, sl.lists[0], &sl.ctl[1].cnt[0], sl.stats, 0u);
-    else
-        k_root_scan_l1<<<fg, 256, 0, s>>>(sl.T, n, sl.root);
-    if (!persistent) {
+    // g >= 2: the set-up runs in the next step's part A, batched with the other slots filled this
+    // step (four launches per step instead of five per query); g = 1 is set up here
+    const bool defer = GST_BATCH_DEFER_FILL && g > 1;
+    FillArgs& fa = sl.h_sb->f;
+    std::memset(&fa, 0, sizeof(fa));
+    if (defer) {
+        fa.gm = sl.gm; fa.root = sl.root; fa.stats = sl.stats; fa.spider = sl.spider; fa.ctl = (uint32_t*)sl.ctl;
+        fa.hub_cnt = sl.hub_cnt; fa.grp = sl.grp; fa.T1 = sl.T + lvl_base[1]; fa.row = G->d_row;
+        fa.chg0 = sl.chg[0]; fa.inq0 = sl.inq[0]; fa.list0 = sl.lists[0]; fa.cnt0 = &sl.ctl[1].cnt[0];
         // a job's own graph gets its hub row items on the device, for pull rounds on dense frontiers
-        // (the count was cleared by k_slot_reset)
-        k_compact<<<fg, 256, 0, s>>>(nullptr, G->d_row, n, chunk, sl.hub_items, sl.hub_cnt, nullptr, HUB_DEG);
+        fa.hub_items = persistent ? nullptr : sl.hub_items;
+        fa.n = n; fa.g = (uint32_t)g; fa.chunk = chunk;
+    } else {
+        const int fg = std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8));
+        k_slot_reset<<<fg, 256, 0, s>>>(sl.gm, n, sl.root, sl.stats, sl.spider, (uint32_t*)sl.ctl, sl.hub_cnt);
+        k_set_group_bits<<<64, 256, 0, s>>>(sl.gm, sl.grp, sl.grp + g + 1, g);
+        k_init_level1<<<fg, 256, 0, s>>>(sl.T + lvl_base[1], sl.gm, G->d_row, n, (uint32_t)g, sl.chg[0], sl.inq[0],
+                                        g > 1 ? 1 : 0);
+        if (g > 1)
+            k_compact<<<fg, 256, 0, s>>>(sl.inq[0], G->d_row, n, chunk, sl.lists[0], &sl.ctl[1].cnt[0], sl.stats, 0u);
+        else
+            k_root_scan_l1<<<fg, 256, 0, s>>>(sl.T, n, sl.root);
+        if (!persistent) {
+            // a job's own graph gets its hub row items on the device, for pull rounds on dense
+            // frontiers (the count was cleared by k_slot_reset)
+            k_compact<<<fg, 256, 0, s>>>(nullptr, G->d_row, n, chunk, sl.hub_items, sl.hub_cnt, nullptr, HUB_DEG);
+        }
     }
 
     // argument blocks
@@ -679,7 +795,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         for (int j = 0; j < GST_MAX_GROUPS + 2; ++j) { ta.lvl_base[j] = lvl_base[j]; ta.m[j] = (uint32_t)L.m[j]; ta.off[j] = (uint32_t)L.off[j]; }
         ta.full = (uint32_t)L.full; ta.cap = E->ecap; ta.max_iter = n + 4u * (uint32_t)g + 64u;
     }
-    sl.h_sb->sd.phase = g > 1 ? PH_GROW : PH_TRACE;
+    sl.h_sb->sd.phase = g > 1 ? (defer ? PH_FILL : PH_GROW) : PH_TRACE;
     sl.h_sb->sd.k = 1;
     sl.h_sb->sd.g = (uint32_t)g;
     sl.h_sb->sd.pad = 0;
@@ -712,6 +828,12 @@ static void enqueue_step_a(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& b
     }
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + pairs_lds;
     k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
+    // the queries that entered slots since the last step: set-up in four passes over their slots
+    const int fgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 4);
+    k_batch_fill_reset<<<fgrid, 256, 0, s>>>(ba);
+    k_batch_fill_bits<<<fgrid, 256, 0, s>>>(ba);
+    k_batch_fill_level1<<<fgrid, 256, 0, s>>>(ba);
+    k_batch_fill_compact<<<fgrid, 256, 0, s>>>(ba);
     // the tour tables' walk table sits in the merge launch's LDS when it fits there (the launch's LDS,
     // and so the merges' workgroups per CU, stays as the merges need), else in the slot's scratch
     const size_t tl = tour_lds_bytes((uint32_t)E->g_cap);
