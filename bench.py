@@ -1,20 +1,26 @@
 """Exact group Steiner tree queries/s on one MI355X: learned upper bound vs plain DPBF.
 
-Workload (one step = one exact GST query, solved through the C-ABI with the graph resident in HBM):
+Workload (one step = one pass over a batch of 64 exact GST queries, solved through the C-ABI with
+the graph resident in HBM):
   Barabasi-Albert V=200,000, m=5 (~1.0M undirected edges), int32 weights uniform in [1,1000],
-  g=6 groups of sizes uniform in [10,200]; query q uses group seed 1000+q (64 distinct queries,
-  cycled to --steps).
+  g=6 groups of sizes uniform in [10,200]; query q of a step uses group seed 1000+q.
 
-Headline `value`: queries/s of the batched learned-bound solve (gst_solve_batch_learned): the
-trained network (gpu4gst_code_amd/models/mpnn_ba50k.npy) scores vertices, the DPBF optimum of
-the top-3% induced subgraph bounds the full DPBF, subgraph and bounded full solves of up to 64
-queries share the batched engine's launches.  Every cost is checked against the plain DPBF of the
-same query, run one query at a time right after (reported as `plain_dpbf`).
+Headline `value`: queries/s of the batched learned-bound solve (gst_solve_batch_learned) over
+`--steps` passes of the 64-query batch streamed through the engine (each step's queries enter slots
+as earlier ones finish), with the engine's default launch path, HIP-graph replay of every step
+(`--engine eager` launches kernel by kernel instead: the path profilers run on).  The trained network
+(gpu4gst_code_amd/models/mpnn_ba50k.npy) scores vertices, the DPBF optimum of the top-5% induced
+subgraph bounds the full DPBF; subgraph and bounded full solves share the engine's launches.  Every
+cost is checked against the plain DPBF of the same query, run one query at a time (`plain_dpbf`);
+the same stream with the automatic spider bound alone (no network) is timed beside it.
 
-Printed: one JSON line with the metric, the speedup over plain DPBF, the roofline of the dominant
-kernel of the timed run (batched grow: algorithmic bytes / HIP-event kernel time vs 8 TB/s) and of
-the batched merge, the plain path's grow/merge rooflines, state relaxations/s, and a CPU baseline
-(the oracle's label-setting DPBF on the host cores over a bounded sample of the same queries).
+The kernel rooflines come from a separate eager pass after the timed runs (labelled in the line):
+the 64 queries' full solves with the bounds of the learned run, 64 slots, every query in a slot
+from the start (so the launch sequence does not depend on host timing), launched kernel by kernel
+with HIP events around every batched grow and merge launch on the engine's stream.  The autotuner
+picks the launch knobs on 16 held-out queries (group seeds 9000..9015).  CPU baselines on the host
+cores: the oracle's label-setting DPBF and the bounded (PrunedDP++ bounds) search, 16 threads each
+over 16 of the queries, with median single-query latency.
 """
 from __future__ import annotations
 
@@ -23,6 +29,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -30,6 +37,9 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+QPS = 64            # queries per step
+TUNE_SEEDS = range(9000, 9016)
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_batch_grow_ba200k_g6_traffic.json")
 
 
 def parse():
@@ -38,9 +48,10 @@ def parse():
     ap.add_argument("--workload", default="ba200k", choices=["ba200k", "c4", "rmat22", "rmat21"],
                     help="ba200k: the headline (default); c4: 256 mixed queries on the BA graph; rmat22: float "
                          "R-MAT scale 22, g=7; rmat21: learned bound on int R-MAT scale 21, g=8")
-    ap.add_argument("--steps", type=int, default=256, help="queries in the timed run (the 64 queries cycled)")
-    ap.add_argument("--warmup", type=int, default=128,
-                    help="queries solved before the timed run (enough to fill the engine's slots once)")
+    ap.add_argument("--steps", type=int, default=8, help="timed steps (passes over the 64-query batch)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps before the timed ones")
+    ap.add_argument("--engine", default="graph", choices=["graph", "eager"],
+                    help="graph: the engine replays each captured step (default); eager: kernel by kernel")
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--groups", type=int, default=6)
     ap.add_argument("--frac", type=float, default=0.05, help="top-scored fraction of vertices kept in the subgraph")
@@ -50,13 +61,81 @@ def parse():
     ap.add_argument("--rounds-per-step", type=int, default=0, help="engine rounds per step (0: tuned / default)")
     ap.add_argument("--cpu-sample", type=int, default=16, help="queries in the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--calibrate-copy", type=int, default=0,
-                    help="MiB copied by one plain device copy before the run (counter calibration)")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="MiB of the device copy whose bandwidth is reported")
     return ap.parse_args()
 
 
 def gbs(nbytes, ms):
     return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+
+
+def copy_bandwidth(mib):
+    """GB/s (read + written) of a plain device copy kernel (gpu4gst_code_amd/csrc/gst_calib.hip)."""
+    import ctypes
+    calib = ctypes.CDLL(os.path.join(ROOT, "gpu4gst_code_amd", "libgst_calib.so"))
+    calib.gst_calib_copy_gbs.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = ctypes.c_double(0.0)
+    rc = calib.gst_calib_copy_gbs(0, mib << 20, 10, ctypes.byref(out))
+    if rc != 0:
+        raise SystemExit(f"bench.py: copy calibration failed ({rc})")
+    return out.value
+
+
+def cpu_baselines(G, sample, plain_costs, threads):
+    """The oracle's label-setting DPBF and its bounded search (spider bound, PrunedDP++'s one-label
+    and tour lower bounds, grow <= U/2, feasible tree at every pop with the exact early exit), each
+    over the sample on `threads` host threads (ctypes releases the GIL), costs checked."""
+    import oracle
+    nth = max(1, min(threads, len(sample)))
+
+    def timed(fn):
+        with ThreadPoolExecutor(nth) as ex:
+            t0 = time.perf_counter()
+            out = list(ex.map(fn, sample))
+            return out, time.perf_counter() - t0
+
+    def plain_one(q):
+        t = time.perf_counter()
+        c = oracle.dpbf_cost(G, q)
+        return c, time.perf_counter() - t
+
+    def bounded_one(q):
+        t = time.perf_counter()
+        d = oracle.group_distances(G, q)
+        c, _ = oracle.dpbf_pruned(G, q, float(np.min(d.sum(axis=0))), 0.5, 2, pop_ub=True, dist=d)
+        return c, time.perf_counter() - t
+
+    res = {}
+    for name, fn in (("dpbf", plain_one), ("bounded", bounded_one)):
+        out, wall = timed(fn)
+        for i, (c, _) in enumerate(out):
+            assert c == plain_costs[i], (name, i, c, plain_costs[i])
+        res[name] = (len(sample) / wall, float(np.median([t for _, t in out])), wall)
+    k = len(sample)
+    qps, med, wall = res["dpbf"]
+    bq, bmed, bwall = res["bounded"]
+    return {"value": round(qps, 4), "unit": "queries/s", "cores": nth, "kind": "port",
+            "median_latency_s": round(med, 3),
+            "sample": f"{k} of the 64 queries, label-setting DPBF (oracle/dpbf_oracle.c orc_dpbf), {nth} threads, "
+                      f"{wall:.1f} s wall",
+            "bounded": {"value": round(bq, 4), "unit": "queries/s", "cores": nth, "kind": "port",
+                        "median_latency_s": round(bmed, 3),
+                        "sample": f"{k} of the 64 queries, bounded label-setting DPBF (oracle orc_dpbf_pruned: spider "
+                                  f"bound, PrunedDP++ one-label and tour lower bounds, grow <= U/2, feasible tree "
+                                  f"at every pop, exact early exit), {nth} threads, {bwall:.1f} s wall"}}
+
+
+def grow_traffic(bytes_per_launch):
+    """HBM bytes per k_batch_grow dispatch of the kernel pass, measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE runs of this script, calibrated) and committed under profiles/; given
+    only when the profiled pass's algorithmic bytes per dispatch are within 10% of this run's."""
+    if not os.path.exists(TRAFFIC_JSON):
+        return None, None
+    t = json.load(open(TRAFFIC_JSON))
+    a = t.get("algorithmic_bytes_per_dispatch")
+    if not a or abs(bytes_per_launch / a - 1.0) > 0.10:
+        return None, a
+    return int(t["traffic_bytes_per_dispatch"]), a
 
 
 def main():
@@ -69,72 +148,80 @@ def main():
 
     if args.workload != "ba200k":
         return other_workload(args, gst)
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     G = gst.gen_barabasi_albert(args.n, 5, 1, 1000, seed=1)
-    queries = [gst.gen_groups(G.n, args.groups, 10, 200, seed=1000 + q) for q in range(64)]
-    timed_q = [queries[i % 64] for i in range(args.steps)]
+    queries = [gst.gen_groups(G.n, args.groups, 10, 200, seed=1000 + q) for q in range(QPS)]
+    held_out = [gst.gen_groups(G.n, args.groups, 10, 200, seed=s) for s in TUNE_SEEDS]
     solver = gst.Solver(0)
     dg = solver.upload(G)
     if not os.path.exists(args.model):
         raise SystemExit(f"bench.py: trained model {args.model} is missing")
     model = gst.Model(solver)
     model.load(args.model)
-
-    if args.calibrate_copy > 0:
-        import ctypes
-        calib = ctypes.CDLL(os.path.join(ROOT, "gpu4gst_code_amd", "libgst_calib.so"))
-        calib.gst_calib_copy.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
-        assert calib.gst_calib_copy(0, args.calibrate_copy << 20, 1) == 0
+    copy_gbs = copy_bandwidth(args.copy_mib) if args.copy_mib > 0 else None
+    use_graph = args.engine == "graph"
+    log(f"graph ready; copy {copy_gbs:.0f} GB/s" if copy_gbs else "graph ready")
 
     # launch knobs for this graph (bounded one-query solves and the batched engine), picked by the
-    # project's autotuner on 16 of the queries, the engine timed as in the timed run (kernel by kernel
-    # with events); plain solves keep the built-in defaults
+    # project's autotuner on held-out queries, the engine timed on the launch path the run uses
     tune = None
     if args.autotune:
         t_tune = time.perf_counter()
-        tune = solver.autotune(dg, queries[:16], bounded=True, batch=True, eager=True)
+        tune = solver.autotune(dg, held_out, bounded=True, batch=True, eager=not use_graph)
         tune["seconds"] = round(time.perf_counter() - t_tune, 2)
+        log(f"autotuned in {tune['seconds']} s")
 
+    kw = dict(slots=args.slots, rounds_per_step=args.rounds_per_step, use_graph=use_graph)
     # warm-up: engine slots, level tables, captured steps, the model's buffers, the plain path
     if args.warmup > 0:
-        solver.solve_batch_learned(dg, [queries[i % 64] for i in range(args.warmup)], model, args.frac,
-                                   slots=args.slots, rounds_per_step=args.rounds_per_step, time_kernels=True)
-        for i in range(min(args.warmup, 4)):
-            solver.solve(dg, queries[i % 64])
+        solver.solve_batch_learned(dg, queries * args.warmup, model, args.frac, **kw)
+        solver.solve_batch(dg, queries, auto_bound=True, **kw)
+        for i in range(4):
+            solver.solve(dg, queries[i])
 
-    # ---- timed: batched learned-bound solve (each call ends with a stream synchronisation) ----
+    # ---- timed: the learned-bound stream (every call ends with a stream synchronisation) ----
+    stream = queries * args.steps
     t0 = time.perf_counter()
-    res, bst = solver.solve_batch_learned(dg, timed_q, model, args.frac, slots=args.slots,
-                                          rounds_per_step=args.rounds_per_step, time_kernels=True)
+    res, bst = solver.solve_batch_learned(dg, stream, model, args.frac, **kw)
     wall = time.perf_counter() - t0
-    qps = args.steps / wall
+    qps = len(stream) / wall
+    log(f"learned {qps:.1f} q/s")
 
-    # ---- plain DPBF, one query at a time, on the 64 distinct queries ----
+    # the same stream with the automatic spider bound alone (no network), batched the same way
+    t3 = time.perf_counter()
+    sres, sst = solver.solve_batch(dg, stream, auto_bound=True, **kw)
+    spider_qps = len(stream) / (time.perf_counter() - t3)
+    log(f"spider {spider_qps:.1f} q/s")
+
+    # ---- plain DPBF, one query at a time, on the 64 queries ----
     t1 = time.perf_counter()
     plain = [solver.solve(dg, q) for q in queries]
-    plain_wall = time.perf_counter() - t1
-    plain_qps = len(queries) / plain_wall
+    plain_qps = len(queries) / (time.perf_counter() - t1)
+    log(f"plain {plain_qps:.1f} q/s")
 
-    # the same queries with the automatic spider bound alone (no network), batched the same way
-    t3 = time.perf_counter()
-    sres, _ = solver.solve_batch(dg, timed_q, slots=args.slots, rounds_per_step=args.rounds_per_step, auto_bound=True,
-                                 time_kernels=True)
-    spider_qps = args.steps / (time.perf_counter() - t3)
-    for i, r in enumerate(sres):
-        assert r.cost == plain[i % 64].cost, (i, r.cost, plain[i % 64].cost)
-
-    # exactness: every learned-bound cost equals plain DPBF; every tree is valid at that cost
-    for i, r in enumerate(res):
-        want = plain[i % 64].cost
+    # exactness: every cost equals plain DPBF; every tree of the first pass is valid at that cost
+    for i, (r, sr) in enumerate(zip(res, sres)):
+        want = plain[i % QPS].cost
         assert r.cost == want, (i, r.cost, want)
-        if i < 64:
-            tc = gst.validate_tree(G, timed_q[i], r.eu, r.ev, r.root)
-            assert tc == want, (i, tc, want)
+        assert sr.cost == want, (i, sr.cost, want)
+        if i < QPS:
+            assert gst.validate_tree(G, queries[i], r.eu, r.ev, r.root) == want, i
     fallback = sum(r.fell_back for r in res) / len(res)
-    ratios = [r.stats["ub"] / plain[i % 64].cost for i, r in enumerate(res) if r.stats["ub"] > 0 and plain[i % 64].cost > 0]
+    ratios = [r.stats["ub"] / plain[i % QPS].cost for i, r in enumerate(res) if r.stats["ub"] > 0]
 
-    # rooflines: batched engine (timed run) and plain path
-    b_grow_gbs = gbs(bst["grow_bytes"], bst["t_grow_ms"])
-    b_merge_gbs = gbs(bst["merge_bytes"], bst["t_merge_ms"])
+    # ---- eager kernel pass: the 64 bounded full solves, every query in a slot from the start ----
+    ubs = [res[i].stats["ub"] if res[i].stats["ub"] > 0 else None for i in range(QPS)]
+    kres, kst = solver.solve_batch(dg, queries, ubs=ubs, auto_bound=True, slots=QPS,
+                                   rounds_per_step=args.rounds_per_step, time_kernels=True)
+    for i, r in enumerate(kres):
+        assert r.cost == plain[i].cost, (i, r.cost, plain[i].cost)
+    glaunch = max(int(kst["grow_launches"]), 1)
+    mlaunch = max(int(kst["merge_launches"]), 1)
+    b_grow_gbs = gbs(kst["grow_bytes"], kst["t_grow_ms"])
+    b_merge_gbs = gbs(kst["merge_bytes"], kst["t_merge_ms"])
+    bpl = kst["grow_bytes"] / glaunch
+    traffic, prof_bytes = grow_traffic(bpl)
+
     p_grow_ms = sum(r.stats["t_grow_ms"] for r in plain)
     p_merge_ms = sum(r.stats["t_merge_ms"] for r in plain)
     p_grow_bytes = sum(r.stats["grow_bytes"] for r in plain)
@@ -144,42 +231,10 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
-        import oracle
-        sample = queries[: args.cpu_sample]
-        nth = min(args.cpu_threads, len(sample), os.cpu_count() or 1)
-        t2 = time.perf_counter()
-        costs = oracle.dpbf_batch(G, sample, nthreads=nth)
-        cw = time.perf_counter() - t2
-        for i, c in enumerate(costs):
-            assert c == plain[i].cost, (i, c, plain[i].cost)
-        cpu = {"value": round(len(sample) / cw, 4), "unit": "queries/s", "cores": nth, "kind": "port",
-               "sample": f"{len(sample)} of the 64 queries, label-setting DPBF (oracle/dpbf_oracle.c), "
-                         f"{nth} threads, {cw:.1f} s wall"}
-        # PrunedDP++-style bounded label-setting search (one-label and tour lower bounds, grow <= U/2)
-        # with the spider bound computed on the host, one thread, on 4 of the queries
-        t4 = time.perf_counter()
-        for qi in range(4):
-            d = oracle.group_distances(G, queries[qi])
-            c, _ = oracle.dpbf_pruned(G, queries[qi], float(np.min(d.sum(axis=0))), 0.5, 2)
-            assert c == plain[qi].cost, (qi, c, plain[qi].cost)
-        pw = time.perf_counter() - t4
-        cpu["pruned"] = {"value": round(4 / pw, 4), "unit": "queries/s", "cores": 1, "kind": "port",
-                         "sample": f"4 of the 64 queries, bounded label-setting DPBF (oracle orc_dpbf_pruned, "
-                                   f"spider bound, one-label and tour LBs, half-bound grow), {pw:.1f} s wall"}
+        k = min(args.cpu_sample, QPS)
+        cpu = cpu_baselines(G, queries[:k], [p.cost for p in plain[:k]], args.cpu_threads)
+        log(f"cpu {cpu['value']} q/s")
 
-    glaunch = max(int(bst["grow_launches"]), 1)
-    # HBM traffic per grow launch from the committed rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE,
-    # calibrated): their bytes per dispatch over their algorithmic bytes per dispatch, applied to this
-    # run's algorithmic bytes per launch (under the profiler the host is slower, so its steps hold
-    # fewer queries and its launches are smaller than here)
-    traffic, t_ratio = None, None
-    tpath = os.path.join(ROOT, "profiles", "dpbf_learned_batch_ba200k_g6_traffic.json")
-    if os.path.exists(tpath):
-        tg = json.load(open(tpath)).get("k_batch_grow", {})
-        if tg.get("traffic_bytes_per_dispatch") and tg.get("algorithmic_bytes_per_dispatch"):
-            t_ratio = tg["traffic_bytes_per_dispatch"] / tg["algorithmic_bytes_per_dispatch"]
-            traffic = int(t_ratio * bst["grow_bytes"] / glaunch)
-    mlaunch = max(int(bst["merge_launches"]), 1)
     line = {
         "metric": "exact-cost GST queries/s (learned-UB batched DPBF vs plain DPBF, BA V=200k g=6)",
         "value": round(qps, 3),
@@ -194,34 +249,49 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded BA graph and groups)",
         "config": {"workload": f"BA V={G.n} m=5 E={G.m // 2} int32 w[1,1000], g={args.groups} sizes 10-200, "
-                               f"{args.steps} queries (64 distinct), learned bound top {args.frac:.0%} subgraph, "
-                               f"batched engine {args.slots} slots",
+                               f"{QPS} queries per step (group seeds 1000-1063), learned bound top {args.frac:.0%} "
+                               f"subgraph, batched engine {args.slots} slots",
+                   "queries_per_step": QPS, "timed_queries": len(stream),
+                   "engine": "HIP-graph replay of every step" if use_graph else "kernel by kernel",
                    "mode": "learned-UB batched DPBF"},
         "plain_dpbf": {"value": round(plain_qps, 3), "unit": "queries/s", "mode": "plain DPBF, one query at a time"},
         "speedup_vs_plain": round(qps / plain_qps, 3),
         "spider_bound_batched": {"value": round(spider_qps, 3), "unit": "queries/s",
-                                 "mode": "automatic spider bound only (no network), same batched engine"},
+                                 "latency_ms": {"p50": round(sst["latency_p50_ms"], 3),
+                                                "p99": round(sst["latency_p99_ms"], 3)},
+                                 "mode": "automatic spider bound only (no network), same stream and engine"},
+        "learned_over_spider": round(qps / spider_qps, 3),
         "learned": {"frac": args.frac, "fallback_rate": round(fallback, 4),
                     "ub_over_opt_mean": round(float(np.mean(ratios)), 5) if ratios else None,
-                    "relax_vs_plain": round(sum(r.stats["grow_relax"] for r in res[:64]) / max(p_relax, 1), 5)},
+                    "relax_vs_plain": round(sum(r.stats["grow_relax"] for r in res[:QPS]) / max(p_relax, 1), 5),
+                    "relax_vs_spider": round(sum(r.stats["grow_relax"] for r in res[:QPS]) /
+                                             max(sum(r.stats["grow_relax"] for r in sres[:QPS]), 1), 5)},
         "latency_ms": {"p50": round(bst["latency_p50_ms"], 3), "p99": round(bst["latency_p99_ms"], 3)},
         "autotune": tune,
         "state_relax_per_s": round(p_relax / (p_grow_ms * 1e-3), 1) if p_grow_ms > 0 else None,
+        "copy_peak": {"achieved": round(copy_gbs, 1) if copy_gbs else None, "unit": "GB/s",
+                      "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4) if copy_gbs else None,
+                      "kernel": f"gst_calib_copy_kernel, {args.copy_mib} MiB read + written, 10 copies"},
         "roofline": {"bound": "hbm", "achieved": round(b_grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_grow_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "traffic_over_algorithmic": round(t_ratio, 2) if t_ratio else None,
-                     "kernel": "k_batch_grow", "launches": glaunch,
-                     "bytes_per_launch": round(bst["grow_bytes"] / glaunch, 1),
-                     "ms_per_launch": round(bst["t_grow_ms"] / glaunch, 5)},
+                     "traffic_over_algorithmic": round(traffic / bpl, 2) if traffic else None,
+                     "frac_of_copy_peak": round(b_grow_gbs / copy_gbs, 5) if copy_gbs else None,
+                     "kernel": "k_batch_grow", "launches": glaunch, "bytes_per_launch": round(bpl, 1),
+                     "profiled_bytes_per_launch": prof_bytes,
+                     "ms_per_launch": round(kst["t_grow_ms"] / glaunch, 5),
+                     "pass": "eager kernel pass: the 64 full solves bounded as in the learned run, 64 slots, all "
+                             "in slots from the start, HIP events around every launch on the engine stream"},
         "merge_roofline": {"bound": "hbm", "achieved": round(b_merge_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(b_merge_gbs / HBM_PEAK_GBS, 5), "kernel": "k_batch_merge",
-                           "launches": mlaunch, "ms_per_launch": round(bst["t_merge_ms"] / mlaunch, 5)},
+                           "launches": mlaunch, "bytes_per_launch": round(kst["merge_bytes"] / mlaunch, 1),
+                           "ms_per_launch": round(kst["t_merge_ms"] / mlaunch, 5), "pass": "eager kernel pass"},
         "plain_roofline": {"grow": {"kernel": "k_grow_round", "achieved": round(gbs(p_grow_bytes, p_grow_ms), 2),
                                     "frac": round(gbs(p_grow_bytes, p_grow_ms) / HBM_PEAK_GBS, 5),
                                     "launches": p_launches,
                                     "ms_per_launch": round(p_grow_ms / max(p_launches, 1), 5)},
                            "merge": {"kernel": "k_merge", "achieved": round(gbs(p_merge_bytes, p_merge_ms), 2),
-                                     "frac": round(gbs(p_merge_bytes, p_merge_ms) / HBM_PEAK_GBS, 5)}},
+                                     "frac": round(gbs(p_merge_bytes, p_merge_ms) / HBM_PEAK_GBS, 5),
+                                     "launches": len(plain) * (args.groups - 1)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -247,7 +317,8 @@ def other_workload(args, gst):
         rng = np.random.default_rng(5)
         qs = [gst.gen_groups(G.n, int(rng.integers(3, 9)), 5, 500, seed=5000 + q) for q in range(256)]
         log("graph and queries ready")
-        solver.autotune(dg, [q for q in qs if q.g == 6][:8], bounded=True, batch=True)
+        # tuned on held-out queries (group seeds 9100..9107), not on the timed ones
+        solver.autotune(dg, [gst.gen_groups(G.n, 6, 5, 500, seed=9100 + i) for i in range(8)], bounded=True, batch=True)
         solver.solve_batch(dg, qs[:16], slots=args.slots)
         t = time.perf_counter(); pb, pst = solver.solve_batch(dg, qs, slots=args.slots); t_pb = time.perf_counter() - t
         log(f"plain batched {len(qs) / t_pb:.1f} q/s")

@@ -299,7 +299,7 @@ class Solver:
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                     use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
                     auto_bound: bool = False, pull_permille: int = 0, time_kernels: bool = False, tour_bound: bool = True,
-                    raise_on_error: bool = True):
+                    raise_on_error: bool = True, max_steps: int = 0):
         """Solve many queries on one graph with the batched engine (gst_solve_batch).
 
         Returns (results, stats): results[i] is a Result, or a GstError for a query that failed
@@ -325,6 +325,7 @@ class Solver:
         bo.rounds_per_step = rounds_per_step
         bo.use_graph = 1 if use_graph else 0
         bo.time_kernels = 1 if time_kernels else 0
+        bo.max_steps = max_steps
         res = (_abi.gst_result * max(nq, 1))()
         status = (C.c_int * max(nq, 1))()
         bs = _abi.gst_batch_stats()
@@ -439,7 +440,7 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
                          prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                          use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,
                          pull_permille: int = 0, time_kernels: bool = False, raise_on_error: bool = True,
-                         tour_bound: bool = True):
+                         tour_bound: bool = True, max_steps: int = 0):
     """Exact solves of many queries with the learned bound, pipelined through the batched engine
     (gst_solve_batch_learned).  Returns (results, batch stats); each Result's stats also carry the
     query's learned-bound fields (sub_vertices, ub, t_sub_solve_ms, ...)."""
@@ -461,6 +462,7 @@ def _solve_batch_learned(self, graph: DeviceGraph, queries: List[Groups], model:
     bo.rounds_per_step = rounds_per_step
     bo.use_graph = 1 if use_graph else 0
     bo.time_kernels = 1 if time_kernels else 0
+    bo.max_steps = max_steps
     res = (_abi.gst_result * max(nq, 1))()
     status = (C.c_int * max(nq, 1))()
     ls = (_abi.gst_learned_stats * max(nq, 1))()

@@ -65,7 +65,7 @@ class gst_learned_stats(C.Structure):
 
 class gst_batch_opts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("rounds_per_step", C.c_int32), ("use_graph", C.c_int32),
-                ("time_kernels", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("time_kernels", C.c_int32), ("max_steps", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class gst_batch_stats(C.Structure):

@@ -320,11 +320,25 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
         ++next;
         return true;
     };
-    auto job_done = [&](gst_batch_job& j) { status[j.tag] = j.status; };
+    std::vector<uint8_t> done((size_t)std::max(nq, 0), 0);
+    auto job_done = [&](gst_batch_job& j) {
+        status[j.tag] = j.status;
+        done[(size_t)j.tag] = 1;
+    };
     int64_t valid = 0;
     for (int32_t q = 0; q < nq; ++q) valid += status[q] == GST_OK ? 1 : 0;
+    bool engine_error = false;
     const gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)g->n, gmax, g->m, bopts, stats, g->uid,
-                                        std::max<int64_t>(valid, 1));
+                                        std::max<int64_t>(valid, 1), &engine_error);
+    if (engine_error) {
+        // the engine stopped early: every valid query without a result carries its error
+        for (int32_t q = 0; q < nq; ++q)
+            if (status[q] == GST_OK && !done[(size_t)q]) {
+                gst_result_free(&results[q]);
+                results[q].root = -1;
+                status[q] = rs;
+            }
+    }
     if (first == GST_OK) first = rs;
     return first;
 }

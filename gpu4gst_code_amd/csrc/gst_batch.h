@@ -28,8 +28,13 @@ struct gst_batch_job {
 // available to next_job.  n_cap / g_cap / m_cap bound every job's graph size and group count.  Returns the first non-OK job status (the other jobs still run) or an engine error.
 // tune_uid names the graph whose autotuned engine knobs apply (0: none).  max_jobs (0: unknown)
 // bounds the jobs in flight at once: without an explicit slot count the engine sizes no more slots.
+// *engine_error (may be null) tells the two kinds of non-OK return apart: false, a job's own status
+// (every job finished and went through job_done); true, an engine error (a HIP call failed, a fill
+// failed, the step bound was hit): the run stopped early, nothing of it still runs on the device,
+// and the jobs that were in slots (or being filled) never reach job_done, so the caller must mark
+// every job it handed out and did not see finish with the returned status.
 gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool wait)>& next_job,
                          const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
                          int64_t m_cap, const gst_batch_opts* bopts, gst_batch_stats* stats, uint64_t tune_uid = 0,
-                         int64_t max_jobs = 0);
+                         int64_t max_jobs = 0, bool* engine_error = nullptr);
 void gst_batch_free(gst_ctx* ctx);

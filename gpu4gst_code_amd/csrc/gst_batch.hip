@@ -865,10 +865,13 @@ static void enqueue_step_b(gst_ctx* ctx, const BatchArgs& ba, int R, bool flt, c
     }
 }
 
-gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool)>& next_job,
-                         const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
-                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
-                         int64_t max_jobs) {
+// The engine loop.  Returns the first failing job status once every job has finished (completed =
+// true), or an engine error at once (completed = false: jobs may still be in slots).
+static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool)>& next_job,
+                                 const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
+                                 int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
+                                 int64_t max_jobs, bool& completed) {
+    completed = false;
     gst_batch_opts bo{};
     if (bo_in) bo = *bo_in;
     if (!ctx->be) ctx->be = new gst_batch_engine();
@@ -1000,7 +1003,7 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, 
     }
     // a step runs rounds_per_step grow rounds of every query in flight; every level converges, so
     // this bound (minutes of work) only turns an engine defect into an error instead of a spin
-    const uint64_t max_steps = 2ull * 1000 * 1000;
+    const uint64_t max_steps = bo.max_steps > 0 ? (uint64_t)bo.max_steps : 2ull * 1000 * 1000;
     // kernel times of one step (its events are complete once the next step's part A is)
     auto add_times = [&](uint64_t step) -> gst_status {
         const hipEvent_t* te = tev.data() + (size_t)(step & 1) * nev;
@@ -1031,7 +1034,8 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, 
             st = add_times(steps - 2);
             if (st != GST_OK) return st;
         }
-        if (steps > max_steps) return gst_set_error(ctx, GST_INTERNAL, "batch: queries did not finish");
+        if (steps >= max_steps) return gst_set_error(ctx, GST_INTERNAL, "batch: queries did not finish in %llu steps",
+                                                     (unsigned long long)steps);
         // finished queries: read root, control blocks, statistics and edge count, then the edges
         std::vector<int> fin;
         for (int i = 0; i < S; ++i)
@@ -1174,5 +1178,38 @@ gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, 
         bst->merge_launches = n_merge_launch;
         bst->queries = (int64_t)n_done;
     }
+    completed = true;
     return first_err;
+}
+
+gst_status gst_batch_run(gst_ctx* ctx, const std::function<bool(gst_batch_job&, bool)>& next_job,
+                         const std::function<void(gst_batch_job&)>& job_done, uint32_t n_cap, int g_cap,
+                         int64_t m_cap, const gst_batch_opts* bo_in, gst_batch_stats* bst, uint64_t tune_uid,
+                         int64_t max_jobs, bool* engine_error) {
+    bool completed = false;
+    const gst_status st =
+        batch_run_impl(ctx, next_job, job_done, n_cap, g_cap, m_cap, bo_in, bst, tune_uid, max_jobs, completed);
+    if (engine_error) *engine_error = !completed;
+    if (completed) return st;
+    // An engine error left the loop early: wait until nothing of this call runs on the engine's
+    // streams (the caller frees what the jobs point at), then free every slot.  Jobs still in slots
+    // and the job a failed fill took are not reported to job_done: the caller marks every query whose
+    // job never finished with the returned status.  An aborted level can leave change flags and queue
+    // flags behind, so every slot's frontier state is cleared for the next call.
+    gst_batch_engine* E = ctx->be;
+    (void)hipStreamSynchronize((hipStream_t)ctx->stream);
+    if (E && E->cstream) (void)hipStreamSynchronize(E->cstream);
+    if (E) {
+        hipStream_t s = (hipStream_t)ctx->stream;
+        const int fg = std::max(1, ctx->cu_count * 4);
+        for (auto& sl : E->slots) {
+            sl.busy = false;
+            for (int j = 0; j < 3; ++j) {
+                if (sl.chg[j]) k_fill_u64<<<fg, 256, 0, s>>>(sl.chg[j], 0ull, (uint64_t)E->n_cap * E->W_cap);
+                if (sl.inq[j]) k_fill_u32<<<fg, 256, 0, s>>>(sl.inq[j], 0u, E->n_cap);
+            }
+        }
+        (void)hipStreamSynchronize(s);
+    }
+    return st;
 }

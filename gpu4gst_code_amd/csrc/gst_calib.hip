@@ -23,3 +23,32 @@ extern "C" int gst_calib_copy(int device, uint64_t bytes, int iters) {
     (void)hipFree(b);
     return e == hipSuccess ? 0 : 3;
 }
+
+// Copy bandwidth of the device: one untimed copy, then `iters` copies of `bytes` timed with HIP
+// events on the copy's stream; *gbs = bytes read plus bytes written per second / 1e9.  Returns 0 on
+// success.
+extern "C" int gst_calib_copy_gbs(int device, uint64_t bytes, int iters, double* gbs) {
+    if (!gbs || iters < 1 || hipSetDevice(device) != hipSuccess) return 1;
+    const uint64_t n = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return 2;
+    if (hipMalloc(&b, n * 16) != hipSuccess) { (void)hipFree(a); return 2; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = 3;
+    if (rc == 0) {
+        (void)hipMemset(a, 1, n * 16);
+        gst_calib_copy_kernel<<<4096, 256>>>(a, b, n);
+        (void)hipEventRecord(e0, nullptr);
+        for (int i = 0; i < iters; ++i) gst_calib_copy_kernel<<<4096, 256>>>(a, b, n);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.f) rc = 3;
+        else *gbs = 2.0 * (double)(n * 16) * iters / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}

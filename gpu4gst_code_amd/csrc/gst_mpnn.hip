@@ -1623,11 +1623,10 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         j.kind = 0;
         return true;
     };
-    std::vector<gst_status> job_fail;   // statuses of failed jobs (the engine returns the first of them)
+    std::vector<uint8_t> done((size_t)std::max(nq, 0), 0);   // the query's full solve finished
     auto job_done = [&](gst_batch_job& j) {
         const int32_t q = (int32_t)j.tag;
         Q& x = qs[(size_t)q];
-        if (j.status != GST_OK) job_fail.push_back(j.status);
         if (j.kind == 0) {
             x.L.t_sub_solve_ms = j.t_done - j.t_fill;
             if (j.status == GST_OK) { x.have_ub = true; x.ub = x.sub_res.cost; }
@@ -1649,6 +1648,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             return;
         }
         status[q] = j.status;
+        done[(size_t)q] = 1;
         if (j.status == GST_OK) results[q].fell_back = (!x.have_ub || x.retried) ? 1 : 0;
         x.L.ub = x.have_ub ? x.ub : -1.0;
         if (lstats) lstats[q] = x.L;
@@ -1662,8 +1662,9 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
         M->sel_stream = nullptr;
         return gst_set_error(ctx, GST_INTERNAL, "learned batch: cannot start the selection thread");
     }
+    bool engine_error = false;
     gst_status rs = gst_batch_run(ctx, next_job, job_done, (uint32_t)G->n, gmax, G->m, bopts, bstats, G->uid,
-                                  std::max<int64_t>(valid, 1));
+                                  std::max<int64_t>(valid, 1), &engine_error);
     {
         std::lock_guard<std::mutex> lk(mu);
         stop = true;   // an engine error ends the run early
@@ -1674,10 +1675,21 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     HIPCHK(hipStreamSynchronize((hipStream_t)ctx->stream2));
     M->sel_stream = nullptr;
     for (auto& x : qs) gst_result_free(&x.sub_res);
+    // a query whose full solve never finished (the engine or the selection stopped early) carries
+    // the error that stopped it, never a zeroed result marked OK
+    const gst_status stop_err = engine_error ? rs : setup_err;
+    if (stop_err != GST_OK)
+        for (int32_t q = 0; q < nq; ++q)
+            if (status[q] == GST_OK && !done[(size_t)q]) {
+                gst_result_free(&results[q]);
+                std::memset(&results[q], 0, sizeof(results[q]));
+                results[q].root = -1;
+                status[q] = stop_err;
+            }
     if (setup_err != GST_OK) return setup_err;
     // a failed subgraph solve (no tree inside the subgraph) is no failure of its query: the
     // engine's own errors are returned, job outcomes are read from status[]
-    if (rs != GST_OK && std::find(job_fail.begin(), job_fail.end(), rs) == job_fail.end()) return rs;
+    if (engine_error) return rs;
     gst_status first = GST_OK;
     for (int32_t q = 0; q < nq && first == GST_OK; ++q)
         if (status[q] != GST_OK) first = status[q];

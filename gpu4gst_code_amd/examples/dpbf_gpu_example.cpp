@@ -1,7 +1,8 @@
 // Caller written against the reference-shaped C++ API (include/gst.hpp): builds a
 // graph_v_of_v_idealID and a group graph the way the reference's harness does
 // (rucgraph/test_gpu.h:115-196: input graph, group vertices V..V+g-1 whose rows list members),
-// calls gst::DPBF_GPU and checks the tree.  Usage: dpbf_gpu_example <V> <E> <g> <seed>
+// calls gst::DPBF_GPU and gst::DPBF_GPU_T with the reference's scalar types and checks the trees
+// with the reference's own check names.  Usage: dpbf_gpu_example <V> <E> <g> <seed>
 #include <cstdio>
 #include <cstdlib>
 
@@ -43,13 +44,28 @@ int main(int argc, char** argv) {
 
         gst::context ctx(0);
         gst::device_graph dg(ctx, input_graph);
-        double real_cost = 0, rt = 0;
-        long long pointer1 = 0;
-        gst::tree t = gst::DPBF_GPU(dg, cumpulsory_group_vertices, group_graph, &real_cost, &rt, &pointer1);
-        const double tc = gst::check_tree(dg, lists, t);
-        std::printf("cost %.1f tree_cost %.1f vertices %zu edges %zu rt %.6f items %lld\n", real_cost, tc,
-                    t.num_vertices(), t.num_edges(), rt, pointer1);
-        return tc == real_cost ? 0 : 1;
+        // the reference's call (rucgraph/test_gpu.h:203) with its own types: int* pointer1,
+        // int* real_cost, double* rt
+        int pointer1 = 0, cost = 0;
+        double rt = 0;
+        int* real_cost = &cost;
+        gst::graph_hash_of_mixed_weighted solu =
+            gst::DPBF_GPU(dg, cumpulsory_group_vertices, group_graph, &pointer1, real_cost, &rt);
+        const int sum = gst::graph_hash_of_mixed_weighted_sum_of_ec(solu);   // rucgraph/test_gpu.h:207
+        if (!gst::this_is_a_feasible_solution_gpu(solu, group_graph, cumpulsory_group_vertices)) return 4;
+        const double tc = gst::check_tree(dg, lists, solu);
+        // DPBF_GPU_T (rucgraph/DPQ.cuh:58): bounded by a feasible cost found on the way, same optimum
+        int pointer1_t = 0, cost_t = 0;
+        double rt_t = 0;
+        gst::graph_hash_of_mixed_weighted solu_t =
+            gst::DPBF_GPU_T(dg, cumpulsory_group_vertices, group_graph, &pointer1_t, &cost_t, &rt_t);
+        if (!gst::this_is_a_feasible_solution_gpu(solu_t, group_graph, cumpulsory_group_vertices)) return 4;
+        std::printf("cost %d sum_of_ec %d tree_cost %.1f vertices %zu edges %zu rt %.6f items %d; DPBF_GPU_T cost %d "
+                    "sum_of_ec %d items %d\n",
+                    *real_cost, sum, tc, solu.num_vertices(), solu.num_edges(), rt, pointer1, cost_t,
+                    gst::graph_hash_of_mixed_weighted_sum_of_ec(solu_t), pointer1_t);
+        return (tc == *real_cost && sum == *real_cost && cost_t == *real_cost &&
+                gst::graph_hash_of_mixed_weighted_sum_of_ec(solu_t) == cost_t) ? 0 : 1;
     } catch (const gst::error& e) {
         std::fprintf(stderr, "gst error %d: %s\n", (int)e.status, e.what());
         return 3;

@@ -186,7 +186,10 @@ typedef struct {
     int32_t use_graph;         /* 1: replay each step as a HIP graph (default), 0: launch kernel by kernel */
     int32_t time_kernels;      /* 1: launch kernel by kernel with HIP events around every batched grow and
                                   merge launch (t_grow_ms / t_merge_ms in the stats) */
-    int32_t reserved[4];
+    int32_t max_steps;         /* > 0: stop with GST_INTERNAL after this many steps (0: a bound of minutes of
+                                  work that only a defect reaches); every query not finished by then
+                                  gets that status */
+    int32_t reserved[3];
 } gst_batch_opts;
 
 typedef struct {

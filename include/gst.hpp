@@ -1,16 +1,27 @@
 // gst.hpp -- header-only C++ host API over the C-ABI in gst.h, shaped like the reference's
-// host API so that a caller of the reference can switch:
+// host API so that a caller of the reference can switch with the same types and argument order:
 //
 //   reference                                                       here
 //   graph_v_of_v_idealID (rucgraph/graph_v_of_v_idealID/           gst::graph_v_of_v_idealID (same type)
 //     graph_v_of_v_idealID.h:50)
 //   toCSR(graph) (rucgraph/graph_v_of_v_idealID/csr_graph.hpp:29)  gst::device_graph(ctx, graph)
-//   DPBF_GPU(..., cumpulsory_group_vertices, group_graph,           gst::DPBF_GPU(ctx, dgraph, cumpulsory_group_vertices,
-//            input_graph, pointer1, real_cost, ..., rt)                           group_graph, &real_cost, &rt)
+//   DPBF_GPU(host_tree, host_tree_one_d, csr, cumpulsory_group_     gst::DPBF_GPU(dg, cumpulsory_group_vertices,
+//            vertices, group_graph, input_graph, int* pointer1,                   group_graph, int* pointer1,
+//            int* real_cost, community, c_size, s, double* rt)                    int* real_cost, double* rt)
 //     (rucgraph/DPQ.cuh:57)
-//   graph_hash_of_mixed_weighted (returned tree,                    gst::tree (add_vertex / add_edge /
-//     rucgraph/graph_hash_of_mixed_weighted/                           edge_weight / hash_of_vectors)
-//     graph_hash_of_mixed_weighted.h:20-126)
+//   DPBF_GPU_T(..., int* pointer1, int* real_cost, ..., double* rt) gst::DPBF_GPU_T(dg, cumpulsory_group_vertices,
+//     (rucgraph/DPQ.cuh:58)                                                        group_graph, int* pointer1,
+//                                                                                  int* real_cost, double* rt)
+//   graph_hash_of_mixed_weighted (returned tree,                    gst::graph_hash_of_mixed_weighted (= gst::tree:
+//     rucgraph/graph_hash_of_mixed_weighted/                           hash_of_vectors[v].adj_vertices, hash_of_hashs,
+//     graph_hash_of_mixed_weighted.h:10-126)                           add_vertex / add_edge / edge_weight / degree)
+//   graph_hash_of_mixed_weighted_sum_of_ec                          gst::graph_hash_of_mixed_weighted_sum_of_ec
+//     (rucgraph/graph_hash_of_mixed_weighted_sum_of_nw_ec.h:58)
+//   this_is_a_feasible_solution_gpu (rucgraph/test_gpu.h:45-77)     gst::this_is_a_feasible_solution_gpu
+//
+// The arguments the reference needs only for its own engine (the V x 2^g host table, the CSR, the
+// input graph again, the disjoint-set list, the community arrays) have no counterpart: the table
+// lives in HBM and the graph was uploaded once by device_graph.
 //
 // Group convention as in the reference: every id in cumpulsory_group_vertices is a group vertex
 // whose group_graph row lists its members (src/DPQ.cu:27-31).  Group bit i is the i-th id in the
@@ -42,35 +53,94 @@ public:
     gst_status status;
 };
 
-// The part of graph_hash_of_mixed_weighted that callers of DPBF_GPU use.
+// One vertex of a returned tree, as graph_hash_of_mixed_weighted_vectors
+// (rucgraph/graph_hash_of_mixed_weighted/graph_hash_of_mixed_weighted.h:10-20).
+struct graph_hash_of_mixed_weighted_vectors {
+    double vertex_weight = 0.0;
+    std::vector<std::pair<int, double>> adj_vertices;   // (neighbour, edge weight)
+};
+
+// The returned tree: the part of graph_hash_of_mixed_weighted that callers of DPBF_GPU use.  Every
+// vertex sits in hash_of_vectors; hash_of_hashs stays empty (the reference keeps only high-degree
+// vertices there, and its readers fall back to hash_of_vectors).
 struct tree {
-    std::unordered_map<int, std::vector<std::pair<int, double>>> hash_of_vectors;
-    void add_vertex(int v) { hash_of_vectors[v]; }
+    std::unordered_map<int, graph_hash_of_mixed_weighted_vectors> hash_of_vectors;
+    std::unordered_map<int, std::unordered_map<int, double>> hash_of_hashs;
+    void add_vertex(int v, double w = 0.0) { hash_of_vectors[v].vertex_weight = w; }
     void add_edge(int u, int v, double w) {
-        hash_of_vectors[u].push_back({v, w});
-        hash_of_vectors[v].push_back({u, w});
+        hash_of_vectors[u].adj_vertices.push_back({v, w});
+        hash_of_vectors[v].adj_vertices.push_back({u, w});
     }
     double edge_weight(int u, int v) const {
         auto it = hash_of_vectors.find(u);
         if (it == hash_of_vectors.end()) return INFINITY;
-        for (auto& p : it->second)
+        for (auto& p : it->second.adj_vertices)
             if (p.first == v) return p.second;
         return INFINITY;
+    }
+    bool contain_edge(int u, int v) const { return edge_weight(u, v) != INFINITY; }
+    int degree(int v) const {
+        auto it = hash_of_vectors.find(v);
+        return it == hash_of_vectors.end() ? 0 : (int)it->second.adj_vertices.size();
+    }
+    std::vector<int> adj_v(int v) const {
+        std::vector<int> out;
+        auto it = hash_of_vectors.find(v);
+        if (it != hash_of_vectors.end())
+            for (auto& p : it->second.adj_vertices) out.push_back(p.first);
+        return out;
     }
     size_t num_vertices() const { return hash_of_vectors.size(); }
     size_t num_edges() const {
         size_t s = 0;
-        for (auto& kv : hash_of_vectors) s += kv.second.size();
+        for (auto& kv : hash_of_vectors) s += kv.second.adj_vertices.size();
         return s / 2;
     }
     double cost() const {
         double c = 0;
         for (auto& kv : hash_of_vectors)
-            for (auto& p : kv.second)
+            for (auto& p : kv.second.adj_vertices)
                 if (kv.first < p.first) c += p.second;
         return c;
     }
 };
+typedef tree graph_hash_of_mixed_weighted;
+
+// Sum of the edge weights in int, each weight truncated as the reference adds them
+// (graph_hash_of_mixed_weighted_sum_of_ec, rucgraph/graph_hash_of_mixed_weighted_sum_of_nw_ec.h:58-87).
+inline int graph_hash_of_mixed_weighted_sum_of_ec(const tree& t) {
+    int sum = 0;
+    for (auto& kv : t.hash_of_vectors)
+        for (auto& p : kv.second.adj_vertices)
+            if (kv.first < p.first) sum += (int)p.second;
+    return sum;
+}
+
+// this_is_a_feasible_solution_gpu (rucgraph/test_gpu.h:45-77): the tree is one connected component
+// and some vertex of it is a member of every group (a group vertex gv's group_graph row lists its
+// members).
+inline bool this_is_a_feasible_solution_gpu(const tree& solu, const graph_v_of_v_idealID& group_graph,
+                                            const std::unordered_set<int>& group_vertices) {
+    if (solu.hash_of_vectors.empty()) return false;
+    std::unordered_set<int> seen;
+    std::vector<int> stack{solu.hash_of_vectors.begin()->first};
+    seen.insert(stack.back());
+    while (!stack.empty()) {
+        const int v = stack.back();
+        stack.pop_back();
+        for (int u : solu.adj_v(v))
+            if (seen.insert(u).second) stack.push_back(u);
+    }
+    if (seen.size() != solu.hash_of_vectors.size()) return false;
+    for (int gv : group_vertices) {
+        bool covered = false;
+        if (gv >= 0 && (size_t)gv < group_graph.size())
+            for (auto& p : group_graph[(size_t)gv])
+                if (solu.hash_of_vectors.count(p.first)) { covered = true; break; }
+        if (!covered) return false;
+    }
+    return true;
+}
 
 class context {
 public:
@@ -167,12 +237,9 @@ inline tree solve(device_graph& dg, const std::vector<std::vector<int>>& groups,
     return t;
 }
 
-// DPBF_GPU (rucgraph/DPQ.cuh:57): tree of minimum cost touching every group.
-// real_cost <- optimal cost (reference: *real_cost), rt <- seconds of device work (reference: *rt),
-// pointer1 += work items processed (reference: dequeued states).
-inline tree DPBF_GPU(device_graph& dg, const std::unordered_set<int>& cumpulsory_group_vertices,
-                     const graph_v_of_v_idealID& group_graph, double* real_cost, double* rt = nullptr,
-                     long long* pointer1 = nullptr) {
+// group vertices -> member lists, in the set's iteration order (group bit i = i-th id)
+inline std::vector<std::vector<int>> member_lists(const std::unordered_set<int>& cumpulsory_group_vertices,
+                                                  const graph_v_of_v_idealID& group_graph) {
     std::vector<std::vector<int>> groups;
     for (int gv : cumpulsory_group_vertices) {
         std::vector<int> mem;
@@ -180,11 +247,59 @@ inline tree DPBF_GPU(device_graph& dg, const std::unordered_set<int>& cumpulsory
             for (auto& p : group_graph[(size_t)gv]) mem.push_back(p.first);
         groups.push_back(std::move(mem));
     }
+    return groups;
+}
+
+// DPBF_GPU with this build's own output types: real_cost <- optimal cost as a double (exact for
+// int weights, the fp32 DP value for float weights), rt <- seconds of device work,
+// pointer1 += work items processed (reference: dequeued states).
+inline tree DPBF_GPU(device_graph& dg, const std::unordered_set<int>& cumpulsory_group_vertices,
+                     const graph_v_of_v_idealID& group_graph, double* real_cost, double* rt = nullptr,
+                     long long* pointer1 = nullptr) {
     gst_stats st;
-    tree t = solve(dg, groups, real_cost, nullptr, &st);
+    tree t = solve(dg, member_lists(cumpulsory_group_vertices, group_graph), real_cost, nullptr, &st);
     if (rt) *rt = (st.t_grow_ms + st.t_merge_ms + st.t_trace_ms) / 1e3;
     if (pointer1) *pointer1 += (long long)st.grow_items;
     return t;
+}
+
+namespace detail {
+inline tree dpbf_int(device_graph& dg, const std::unordered_set<int>& cumpulsory_group_vertices,
+                     const graph_v_of_v_idealID& group_graph, int* pointer1, int* real_cost, double* rt,
+                     const gst_opts* opts) {
+    gst_stats st;
+    double c = 0.0;
+    tree t = solve(dg, member_lists(cumpulsory_group_vertices, group_graph), &c, opts, &st);
+    if (real_cost) *real_cost = (int)c;   // truncated, as the reference's int costs of truncated weights
+    if (rt) *rt = (st.t_grow_ms + st.t_merge_ms + st.t_trace_ms) / 1e3;
+    if (pointer1) *pointer1 += (int)st.grow_items;
+    return t;
+}
+}  // namespace detail
+
+// DPBF_GPU with the reference's own scalar types and argument order (rucgraph/DPQ.cuh:57):
+// pointer1 += work items processed, real_cost <- optimal cost as int, rt <- seconds of device work.
+// For float weights the int cost is the optimum truncated toward zero; the double overload above
+// returns it whole.
+inline graph_hash_of_mixed_weighted DPBF_GPU(device_graph& dg, const std::unordered_set<int>& cumpulsory_group_vertices,
+                                             const graph_v_of_v_idealID& group_graph, int* pointer1, int* real_cost,
+                                             double* rt) {
+    return detail::dpbf_int(dg, cumpulsory_group_vertices, group_graph, pointer1, real_cost, rt, nullptr);
+}
+
+// DPBF_GPU_T (rucgraph/DPQ.cuh:58, src/DPQ.cu:769-1040): the reference first runs a few unbounded
+// rounds to find a feasible cost, then prunes with it and the one-label bound (and stops after 20
+// rounds, so its answer may be above the optimum).  Here: the automatic spider bound (the cheapest
+// union of shortest paths from one vertex to every group, a feasible cost once the group distances
+// are final) with the one-label and tour lower bounds; the answer is the exact optimum.
+inline graph_hash_of_mixed_weighted DPBF_GPU_T(device_graph& dg, const std::unordered_set<int>& cumpulsory_group_vertices,
+                                               const graph_v_of_v_idealID& group_graph, int* pointer1, int* real_cost,
+                                               double* rt) {
+    gst_opts o{};
+    o.prune_lb = 1;
+    o.want_tree = 1;
+    o.auto_bound = 1;
+    return detail::dpbf_int(dg, cumpulsory_group_vertices, group_graph, pointer1, real_cost, rt, &o);
 }
 
 // this_is_a_feasible_solution_gpu (rucgraph/test_gpu.h:45-77), strengthened: the edge set must be
@@ -194,7 +309,7 @@ inline double check_tree(device_graph& dg, const std::vector<std::vector<int>>& 
     int32_t root = -1;
     for (auto& kv : t.hash_of_vectors) {
         root = kv.first;
-        for (auto& p : kv.second)
+        for (auto& p : kv.second.adj_vertices)
             if (kv.first < p.first) { eu.push_back(kv.first); ev.push_back(p.first); }
     }
     std::vector<int32_t> ptr(groups.size() + 1, 0), vtx;

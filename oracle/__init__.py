@@ -102,16 +102,24 @@ def dpbf_cost(graph, groups) -> float:
     return dpbf(graph, groups)[0]
 
 
-def dpbf_pruned(graph, groups, ub: float, grow_frac: float = 0.5, use_lb=True):
+def dpbf_pruned(graph, groups, ub: float, grow_frac: float = 0.5, use_lb=True, pop_ub: bool = False, dist=None):
     """Label-setting DPBF under a feasible cost ub (PrunedDP++ bounds): only states of cost
     <= grow_frac * ub are grown (0: all); use_lb: False, True / 1 (one-label), 2 (one-label and
-    tour bounds), 3 (tour bounds with a planted mistake, for the tests).  Returns (cost, stats)."""
+    tour bounds), 3 (tour bounds with a planted mistake, for the tests); pop_ub: PrunedDP++'s
+    feasible tree at every pop (popped tree + shortest paths to the missing groups) lowers ub, and
+    the search ends once the popped cost reaches it.  dist: group_distances(graph, groups) when the
+    caller has them.  Returns (cost, stats)."""
     lib = load()
     row, col, w = _csr(graph)
     n = row.shape[0] - 1
     gm = group_masks(n, groups)
     use_lb = int(use_lb)
-    dist = np.ascontiguousarray(group_distances(graph, groups)) if use_lb else np.zeros(1)
+    if use_lb or pop_ub:
+        dist = np.ascontiguousarray(group_distances(graph, groups) if dist is None else dist, dtype=np.float64)
+    else:
+        dist = np.zeros(1)
+    if pop_ub:
+        use_lb |= 16
     cost = C.c_double(0)
     stats = np.zeros(3, np.int64)
     rc = lib.orc_dpbf_pruned(n, _p(row), _p(col), _p(w), groups.g, _p(gm), float(ub), float(grow_frac),

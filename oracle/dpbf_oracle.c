@@ -357,12 +357,22 @@ int orc_dpbf(int32_t n, const int64_t* row, const int32_t* col, const double* w,
  * As orc_dpbf (cost only), under a bound U >= OPT: states with cost + lb > U are never stored,
  * and with grow_frac > 0 only states with cost <= grow_frac * U are grown (PrunedDP++: 1/2).  dist (g x n, may be NULL when
  * use_lb == 0) holds the exact group distances for the one-label lower bound.
+ * use_lb & ORC_POP_UB (with dist): PrunedDP++'s feasible tree at every pop
+ * (rucgraph/graph_v_of_v_idealID_PrunedDPPlusPlus.h:668-711): the popped tree T(v,p) plus a shortest
+ * path from v to every missing group touches every group, so U = min(U, cost + sum_{i not in p}
+ * dist(v, G_i)) (the cost of that union, an upper bound of the tree inside it); and the exact form of
+ * its early exit (:716-730): once the popped cost reaches U, every later tree costs at least U, so
+ * OPT = U.  (The reference's own exit tests an int ratio, best / popped < 2, so it can stop up to 2x
+ * above the optimum.)
  * stats[0] = states popped, stats[1] = grow relaxations, stats[2] = merge evaluations.
  */
+#define ORC_POP_UB 16
 int orc_dpbf_pruned(int32_t n, const int64_t* row, const int32_t* col, const double* w, int32_t g,
                     const uint32_t* gm, double U, double grow_frac, int32_t use_lb, const double* dist,
                     double* cost_out, int64_t* stats) {
-    if (n < 1 || g < 1 || g > 12 || (use_lb && !dist)) return ORC_INVALID;
+    const int pop_ub = (use_lb & ORC_POP_UB) != 0;
+    use_lb &= ~ORC_POP_UB;
+    if (n < 1 || g < 1 || g > 12 || (use_lb && !dist) || (pop_ub && !dist)) return ORC_INVALID;
     const int64_t S = (int64_t)1 << g;
     const uint32_t full = (uint32_t)(S - 1);
     const int64_t NS = (int64_t)n * S;
@@ -408,6 +418,13 @@ int orc_dpbf_pruned(int32_t n, const int64_t* row, const int32_t* col, const dou
         const uint32_t p = (uint32_t)(s % S);
         const double cvp = cost[s];
         if (p == full) { best = cvp; break; }
+        if (pop_ub) {
+            if (cvp >= U) { best = U; break; }   /* nothing left costs less than the feasible U */
+            double f = cvp;
+            for (int32_t i = 0; i < g; ++i)
+                if (!((p >> i) & 1u)) f += dist[(int64_t)i * n + v];
+            if (f < U) U = f;
+        }
         if (!(grow_frac > 0.0) || cvp <= grow_frac * U) {
             for (int64_t e = row[v]; e < row[v + 1]; ++e) {
                 const int32_t u = col[e];

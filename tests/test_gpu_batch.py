@@ -140,3 +140,34 @@ def test_default_slots_follow_query_count(solver):
     _, st = solver.solve_batch(dg, qs, slots=5)
     assert st["slots"] == 5
     dg.free()
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_engine_stop_marks_every_unfinished_query(solver, learned):
+    """A run stopped early by the engine (here its step limit, set from the host: every kernel still
+    ends normally) reports an error for every query it did not finish, never a zeroed result marked
+    OK; the next call on the same engine solves the same queries exactly."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    qs = [gst.gen_groups(G.n, 6, 10, 200, seed=1000 + i) for i in range(12)]
+    if learned:
+        model = gst.Model(solver, seed=3)
+        res, _ = solver.solve_batch_learned(dg, qs, model, 0.05, slots=4, max_steps=3, raise_on_error=False)
+    else:
+        res, _ = solver.solve_batch(dg, qs, slots=4, max_steps=3, raise_on_error=False)
+    bad = [r for r in res if isinstance(r, gst.GstError)]
+    print(f"learned={learned}: {len(bad)} of {len(qs)} queries stopped", flush=True)
+    assert bad, "three steps cannot finish twelve g=6 queries on four slots"
+    for q, r in zip(qs, res):
+        if isinstance(r, gst.GstError):
+            assert r.status == 8   # GST_INTERNAL
+        else:
+            assert r.root >= 0 and gst.validate_tree(G, q, r.eu, r.ev, r.root) == r.cost
+    again, _ = (solver.solve_batch_learned(dg, qs, model, 0.05, slots=4) if learned
+                else solver.solve_batch(dg, qs, slots=4))
+    for q, r in zip(qs, again):
+        assert r.cost == solver.solve(dg, q, want_tree=False).cost
+        _valid(G, q, r)
+    if learned:
+        model.free()
+    dg.free()

@@ -162,17 +162,20 @@ def test_upper_bound_keeps_exact_cost(solver, slack):
 
 
 def test_cpp_wrapper_example():
-    """The reference-shaped C++ API (include/gst.hpp, DPBF_GPU) through the C-ABI."""
+    """The reference-shaped C++ API (include/gst.hpp: DPBF_GPU and DPBF_GPU_T with the reference's
+    int* pointer1 / int* real_cost / double* rt, graph_hash_of_mixed_weighted_sum_of_ec,
+    this_is_a_feasible_solution_gpu) through the C-ABI; the example exits non-zero unless both
+    calls return feasible trees whose int edge sums equal the returned cost."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "gpu4gst_code_amd", "_build", "dpbf_gpu_example")
-    for seed in (1, 2, 3):
-        out = subprocess.run([exe, "1000", "5000", "3", str(seed)], capture_output=True, text=True, timeout=120)
+    for n, m, g, seed in ((1000, 5000, 3, 1), (1000, 5000, 3, 2), (1000, 5000, 3, 3), (1000, 5000, 6, 4)):
+        out = subprocess.run([exe, str(n), str(m), str(g), str(seed)], capture_output=True, text=True, timeout=120)
         print(out.stdout, out.stderr, flush=True)
         assert out.returncode == 0
         cost = float(out.stdout.split()[1])
-        G = gst.gen_random_connected(1000, 5000, 1, 100, seed=seed)
-        groups = gst.gen_groups(G.n, 3, 5, 20, seed=seed + 1)
+        G = gst.gen_random_connected(n, m, 1, 100, seed=seed)
+        groups = gst.gen_groups(G.n, g, 5, 20, seed=seed + 1)
         assert cost == oracle.dpbf_cost(G, groups)
 
 

@@ -188,6 +188,40 @@ def test_tour_bound_keeps_optimum_vs_bruteforce(s):
         assert oracle.dpbf_pruned(G, groups, U, 0.0, 2)[0] == b
 
 
+@pytest.mark.parametrize("s", range(0, 60, 4))
+def test_pop_upper_bound_keeps_optimum_vs_bruteforce(s):
+    """PrunedDP++'s feasible tree at every pop (popped tree + shortest paths to the missing
+    groups) and its exact early exit keep the brute-force optimum for every U >= OPT, from an
+    unbounded start too (U = inf: the pops alone supply the bound), zero weights included."""
+    n = 10 + s % 5
+    g = 2 + s % 5
+    G = _zero_weight_graph(n, 8, 4, 9300 + s) if s % 3 == 0 else gst.gen_random_connected(n, n + 6, 1, 20, seed=5300 + s)
+    groups = gst.gen_groups(n, g, 1, 3, seed=6300 + s)
+    b = oracle.bruteforce(G, groups)
+    for U in (b, b + 1, 2 * b + 3, np.inf):
+        for lb in (1, 2):
+            for frac in (0.5, 0.0):
+                c, _ = oracle.dpbf_pruned(G, groups, U, frac if np.isfinite(U) else 0.0, lb, pop_ub=True)
+                assert c == b, (U, lb, frac, c, b)
+
+
+def test_pop_upper_bound_prunes_on_c1():
+    """On C1-sized instances the per-pop bound and early exit pop fewer states than the bounded
+    search without them, at the same optimum."""
+    fewer = 0
+    for s in range(4):
+        G = gst.gen_random_connected(1000, 5000, 1, 100, seed=s)
+        groups = gst.gen_groups(G.n, 5, 5, 20, seed=s + 1)
+        c0 = oracle.dpbf_cost(G, groups)
+        d = oracle.group_distances(G, groups)
+        spider = float(np.min(d.sum(axis=0)))
+        c1, st1 = oracle.dpbf_pruned(G, groups, spider, 0.5, 2, dist=d)
+        c2, st2 = oracle.dpbf_pruned(G, groups, spider, 0.5, 2, pop_ub=True, dist=d)
+        assert c1 == c0 and c2 == c0
+        fewer += st2["pops"] < st1["pops"]
+    assert fewer >= 1
+
+
 @pytest.mark.parametrize("s", range(6))
 def test_tour_bound_per_state_vs_bruteforce(s):
     """Each state's bound is at most the cost of the cheapest tree that holds v and touches every
