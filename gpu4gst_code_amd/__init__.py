@@ -295,6 +295,33 @@ class Solver:
         finally:
             self.lib.gst_result_free(C.byref(res))
 
+    def solve_noalloc(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
+                      want_tree: bool = True, edge_cap: int = 16384, auto_bound: bool = False, half_bound: bool = True,
+                      tour_bound: bool = True, push_lanes: int = 0, pull_permille: int = 0) -> Result:
+        """One exact query through gst_solve_noalloc: no allocation inside the call (the device
+        state lives in the library's fixed scratch array); the tree goes into arrays made here."""
+        opts = _abi.gst_opts()
+        opts.ub = float(ub) if ub is not None else 0.0
+        opts.prune_lb = 1 if prune_lb else 0
+        opts.want_tree = 1 if want_tree else 0
+        opts.push_lanes = push_lanes
+        opts.pull_permille = pull_permille
+        opts.half_bound = 0 if half_bound else -1
+        opts.auto_bound = 1 if auto_bound else 0
+        opts.tour_bound = 0 if tour_bound else -1
+        eu = np.zeros(max(edge_cap, 1), np.int32)
+        ev = np.zeros(max(edge_cap, 1), np.int32)
+        ew = np.zeros(max(edge_cap, 1), np.float64)
+        res = _abi.gst_result()
+        grp = groups.as_c()
+        st = self.lib.gst_solve_noalloc(self.ctx, graph.handle, C.byref(grp), C.byref(opts), int(edge_cap), _ptr(eu),
+                                        _ptr(ev), _ptr(ew), C.byref(res))
+        _check(self.lib, st, self.ctx)
+        ne = res.n_edges
+        stats = {name: getattr(res.stats, name) for name, _ in _abi.gst_stats._fields_}
+        return Result(res.cost, res.tree_cost, res.root, eu[:ne].copy(), ev[:ne].copy(), ew[:ne].copy(),
+                      bool(res.used_ub), bool(res.fell_back), stats)
+
     def solve_batch(self, graph: DeviceGraph, queries: List[Groups], *, ubs: Optional[Sequence[float]] = None,
                     prune_lb: bool = True, want_tree: bool = True, slots: int = 0, rounds_per_step: int = 0,
                     use_graph: bool = True, push_lanes: int = 0, half_bound: bool = True,

@@ -5,6 +5,8 @@ import ctypes as C
 import os
 
 GST_MAX_GROUPS = 12
+GST_NOALLOC_SCRATCH_BYTES = 128 << 20
+GST_NOALLOC_MAX_EDGES = 16384
 
 GST_OK = 0
 GST_INFEASIBLE = 1
@@ -107,6 +109,8 @@ SIGNATURES = {
     "gst_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.POINTER(gst_opts),
                             C.POINTER(gst_result)]),
     "gst_result_free": (None, [C.POINTER(gst_result)]),
+    "gst_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.POINTER(gst_opts), C.c_int32,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(gst_result)]),
     "gst_solve_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(gst_groups), C.POINTER(gst_opts),
                                   C.POINTER(gst_batch_opts), C.POINTER(gst_result), C.POINTER(C.c_int),
                                   C.POINTER(gst_batch_stats)]),

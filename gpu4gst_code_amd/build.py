@@ -21,7 +21,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("GST_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["gst_api.cpp", "gst_gen.cpp", "gst_tune.cpp"]
-HIP_SOURCES = ["gst_dpbf.hip", "gst_batch.hip", "gst_mpnn.hip"]
+HIP_SOURCES = ["gst_dpbf.hip", "gst_batch.hip", "gst_mpnn.hip", "gst_noalloc.hip"]
 
 # Packed-FP32 instructions are switched off for all device code (they have returned wrong results
 # on this platform); hipcc's host pass warns that it does not know the feature, which is harmless.

@@ -281,6 +281,18 @@ gst_status gst_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
     return gst_dpbf_solve(ctx, g, groups, opts, result);
 }
 
+gst_status gst_solve_noalloc(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, const gst_opts* opts,
+                             int32_t edge_cap, int32_t* eu, int32_t* ev, double* ew, gst_result* result) {
+    if (!ctx || !g || !groups || !result) return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "null argument") : GST_INVALID_ARG;
+    std::memset(result, 0, sizeof(*result));
+    result->root = -1;
+    if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
+    if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
+    const gst_status cs = check_query(ctx, g, groups);
+    if (cs != GST_OK) return cs;
+    return gst_dpbf_solve_noalloc(ctx, g, groups, opts, edge_cap, eu, ev, ew, result);
+}
+
 gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const gst_groups* groups,
                            const gst_opts* opts, const gst_batch_opts* bopts, gst_result* results,
                            gst_status* status, gst_batch_stats* stats) {

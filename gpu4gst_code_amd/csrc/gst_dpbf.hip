@@ -117,12 +117,13 @@ gst_status gst_ctx_device_init(gst_ctx* ctx) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
     ctx->cu_count = prop.multiProcessorCount;
-    return GST_OK;
+    return gst_noalloc_init(ctx);
 }
 
 void gst_ctx_device_fini(gst_ctx* ctx) {
     if (!ctx) return;
     gst_batch_free(ctx);
+    gst_noalloc_fini(ctx);
     hipSetDevice(ctx->device);
     gst_query_bufs_free(ctx);
     if (ctx->stream) hipStreamDestroy((hipStream_t)ctx->stream);

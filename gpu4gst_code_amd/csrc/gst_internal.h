@@ -134,6 +134,7 @@ struct gst_ctx {
     void* stream2 = nullptr;    // hipStream_t (default priority): learned-bound selection beside the engine
     gst_query_bufs* qb = nullptr;
     struct gst_batch_engine* be = nullptr;   // batched engine (gst_batch.hip), made on first use
+    struct gst_noalloc_host* na = nullptr;   // no-allocation solve (gst_noalloc.hip): made at set-up
     gst_levels lv_cache[GST_MAX_GROUPS + 1];
     bool lv_ready[GST_MAX_GROUPS + 1] = {false};
     int cu_count = 0;
@@ -147,6 +148,11 @@ void gst_query_bufs_free(gst_ctx* ctx);
 void gst_finish_tree(bool flt, uint32_t nout, const uint32_t* hu, const uint32_t* hv, const uint32_t* hw,
                      gst_result* res);
 gst_status gst_ctx_device_init(gst_ctx* ctx);
+// implemented in gst_noalloc.hip: set-up at context creation, the allocation-free one-query solve
+gst_status gst_noalloc_init(gst_ctx* ctx);
+void gst_noalloc_fini(gst_ctx* ctx);
+gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, const gst_opts* opts,
+                                  int32_t edge_cap, int32_t* eu, int32_t* ev, double* ew, gst_result* res);
 void gst_ctx_device_fini(gst_ctx* ctx);
 gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g);
 void gst_graph_device_free(gst_graph* g);

@@ -170,6 +170,25 @@ gst_status gst_validate_tree(const gst_csr* csr, const gst_groups* groups, int32
                              const int32_t* eu, const int32_t* ev, int32_t root,
                              double* cost_out);
 
+/* --- one query without allocation ------------------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation): the same exact solve as gst_solve (same
+ * kernels, the round path of grow_mode 0/1; 2 and 3 are rejected), with every per-query device buffer
+ * carved out of one fixed device array of GST_NOALLOC_SCRATCH_BYTES per GPU (held from the first
+ * context on that GPU until the process ends) and host work in buffers made by gst_ctx_create.  The
+ * call makes no device or host allocation: it rejects a query whose buffers exceed the array
+ * (GST_INVALID_ARG, with the size it needs in gst_last_error) or whose tree exceeds
+ * GST_NOALLOC_MAX_EDGES edges.  The graph is uploaded beforehand (gst_graph_upload allocates).
+ * With want_tree, the tree's edges go into the caller's eu/ev/ew (edge_cap entries each) and
+ * result->n_edges counts them; result->eu/ev/ew stay NULL (gst_result_free is not needed).  No
+ * HIP events are made: the t_*_ms stats stay 0.  The array is shared by every call on one GPU and
+ * keeps the last call's contents (each call clears what must start at zero): calls that overlap on
+ * one GPU, from two contexts of one device used by two threads at once, race on it. */
+#define GST_NOALLOC_SCRATCH_BYTES (128u << 20)
+#define GST_NOALLOC_MAX_EDGES 16384
+gst_status gst_solve_noalloc(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, const gst_opts* opts,
+                             int32_t edge_cap, int32_t* eu, int32_t* ev, double* ew, gst_result* result);
+
 /* --- batched queries --------------------------------------------------------------------- */
 /* Many exact queries on one graph in flight together.  Replaces calling DPBF_GPU once per query
  * (rucgraph/test_gpu.h:188-283 loops over queries; src/DPQ.cu:501-768 syncs with the host every
