@@ -445,6 +445,7 @@ struct gst_batch_engine {
         uint32_t* edges = nullptr;
         uint32_t* spider = nullptr;         // cheapest spider of the level-2 merge (automatic bound)
         uint32_t* tour = nullptr;           // tour table 2^g * (g + 1) words, then walk scratch 2^g * g * g
+        uint8_t* act = nullptr;             // activity bytes act[k * n + v] of the job's levels
         int tour_g = 0;
         uint64_t* hub_items = nullptr;      // hub row items of a job's own graph (pull rounds)
         uint32_t* hub_cnt = nullptr;
@@ -504,7 +505,7 @@ static size_t back_bytes(uint32_t ecap) { return BK_EDGES + 12 * (size_t)ecap; }
 static void free_slot(gst_batch_engine::Slot& sl) {
     hipFree(sl.T); hipFree(sl.gm);
     for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
-    hipFree(sl.back); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour);
+    hipFree(sl.back); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour); hipFree(sl.act);
     hipFree(sl.hub_items); hipFree(sl.hub_cnt);
     if (sl.h_grp) hipHostFree(sl.h_grp);
     if (sl.h_sb) hipHostFree(sl.h_sb);
@@ -545,7 +546,7 @@ void gst_batch_free(gst_ctx* ctx) {
 // bytes of device memory one slot needs
 static size_t slot_bytes(uint32_t n, int g, uint64_t items_cap, uint32_t W, uint32_t ecap) {
     const size_t words = (size_t)n * (((size_t)1 << g) - 1);
-    return words * 8 + (size_t)n * 4 + 3 * ((size_t)n * W * 8 + (size_t)n * 4 + items_cap * 8) +
+    return words * 8 + (size_t)n * 4 + (size_t)n * (size_t)(g + 1) + 3 * ((size_t)n * W * 8 + (size_t)n * 4 + items_cap * 8) +
            sizeof(RoundCtl) * NCTL + 8 + 8 * ST_NSTAT + (3 * (size_t)ecap + 1) * 4 + 64 * 1024;
 }
 
@@ -567,7 +568,7 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
     const size_t words = (size_t)n * (((size_t)1 << g) - 1);
     for (int i = 0; ok && i < S; ++i) {
         auto& sl = E->slots[(size_t)i];
-        ok = ok && dalloc(sl.T, words) && dalloc(sl.gm, n);
+        ok = ok && dalloc(sl.T, words) && dalloc(sl.gm, n) && dalloc(sl.act, (size_t)n * (size_t)(g + 1));
         for (int j = 0; ok && j < 3; ++j)
             ok = dalloc(sl.chg[j], (size_t)n * W) && dalloc(sl.inq[j], n) && dalloc(sl.lists[j], items_cap);
         ok = ok && dalloc(sl.back, back_bytes(ecap)) && dalloc(sl.spider, 1) &&
@@ -692,6 +693,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     for (int k = 1; k <= g; ++k) lvl_base[k] = (uint64_t)n * (uint64_t)L.off[k];
     const gst_bound_bits bb = gst_make_bound(job.opts.ub, flt, job.opts.half_bound);
     const uint32_t ubb = bb.ub;
+    // activity bytes for bounded jobs (a caller's bound or the automatic one)
+    const bool use_act = GST_MERGE_SKIP && (ubb != GST_INF_COST || job.opts.auto_bound);
     // g >= 2: the set-up runs in the next step's part A, batched with the other slots filled this
     // step (four launches per step instead of five per query); g = 1 is set up here
     const bool defer = GST_BATCH_DEFER_FILL && g > 1;
@@ -741,6 +744,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.lbcap = bb.lbcap;
         a.use_lb = (job.opts.prune_lb && k >= 2) ? 1u : 0u;
         a.tour = k >= 2 ? tour : nullptr;
+        a.act = (k >= 2 && use_act) ? sl.act + (size_t)k * n : nullptr;
         a.all_items = persistent ? E->all_items : sl.hub_items;
         a.n_all = persistent ? E->n_all : 0u;
         a.n_all_dev = persistent ? nullptr : sl.hub_cnt;
@@ -780,6 +784,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         ma.tour = tour;
         ma.grp = sl.grp;
         ma.tour_w = tour ? sl.tour + ((size_t)1 << g) * (size_t)(g + 1) : nullptr;
+        ma.act = use_act ? sl.act : nullptr;
         ma.is_full = (k == g) ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
         ma.VB = merge_tile_vertices(Lk, Wk);
@@ -1094,12 +1099,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
             S2.merge_vertices = hs[ST_MVERT]; S2.merge_pairs = hs[ST_MPAIRS];
             S2.grow_rounds = rounds;
             S2.grow_bytes = S2.grow_items * 24ull + S2.grow_arcs * 12ull + S2.grow_relax * 8ull + S2.grow_updates * 16ull;
-            {
-                const gst_levels& L = gst_get_levels(ctx, sl.g);
-                uint64_t mb = 0;
-                for (int k = 2; k <= sl.g; ++k) mb += (uint64_t)(L.off[k] + L.m[k]) * 8ull;
-                S2.merge_bytes = (uint64_t)job.G->n * mb;
-            }
+            S2.merge_bytes = hs[ST_MBYTES];
             relax += S2.grow_relax; gbytes += S2.grow_bytes; mbytes += S2.merge_bytes;
             if (err & 1u) {
                 job.status = gst_set_error(ctx, GST_COST_OVERFLOW, "integer path cost exceeds 32 bits");

@@ -84,6 +84,8 @@ struct gst_query_bufs {
     RoundCtl* h_ctl = nullptr;   // pinned
     std::vector<hipEvent_t> evs;
     uint32_t* rlog = nullptr;    // round log of every level (GST_TRACE_ROUNDS), NCTL * RLOG_ROUNDS * 4 words
+    uint8_t* act = nullptr;      // activity bytes, act[k * n + v] (levels 2..g-1)
+    uint64_t cap_act = 0;
 };
 
 static void free_bufs(gst_query_bufs* b) {
@@ -93,7 +95,7 @@ static void free_bufs(gst_query_bufs* b) {
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
     hipFree(b->all_items); hipFree(b->all_cnt);
-    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider); hipFree(b->tour);
+    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider); hipFree(b->tour); hipFree(b->act);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -341,6 +343,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         // capacity bookkeeping: cap_edges counts words; recover the per-array capacity
     }
     ok = ok && grow_alloc(B->grp, B->cap_grp, grp_need, 4);
+    ok = ok && grow_alloc(B->act, B->cap_act, (uint64_t)n * (uint64_t)(g + 1), 1);
     if (!ok) {
         gst_query_bufs_free(ctx);
         return gst_set_error(ctx, GST_OOM, "solve: device allocation failed (V=%u, g=%d, table %.2f GB)",
@@ -457,6 +460,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         // after level 1 has converged: never prune with it inside level 1
         ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;
         ga.tour = k >= 2 ? tour_ptr : nullptr;
+        // activity bytes once a bound is known (after level 1 with the automatic bound)
+        ga.act = (k >= 2 && GST_MERGE_SKIP && ub_bits != GST_INF_COST) ? B->act + (size_t)k * n : nullptr;
         ga.all_items = B->all_items; ga.n_all = B->n_all; ga.n_all_dev = nullptr; ga.n = n;
         {
             const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
@@ -611,6 +616,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ma.tour = tour;
         ma.grp = nullptr;
         ma.tour_w = nullptr;
+        ma.act = (GST_MERGE_SKIP && ub_bits != GST_INF_COST) ? B->act : nullptr;
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;
@@ -764,9 +770,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     // algorithmic byte model: see DESIGN (grow: item 8 + row 8 + bitmap 8W + own costs 8/relax-lane-group
     // approximated by 12 B per scanned arc + 8 B read + 16 B RMW per update; merge: 8 B per word in/out)
     S.grow_bytes = S.grow_items * (8ull + 8ull + 8ull) + S.grow_arcs * 12ull + S.grow_relax * 8ull + S.grow_updates * 16ull;
-    uint64_t mb = 0;
-    for (int k = 2; k <= g; ++k) mb += (uint64_t)(L.off[k] + L.m[k]) * 8ull;
-    S.merge_bytes = (uint64_t)n * mb;
+    S.merge_bytes = hs[ST_MBYTES];
     S.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return GST_OK;
 }

@@ -36,7 +36,7 @@ struct RoundCtl {
 };
 
 enum StatIdx {
-    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_MVERT, ST_MPAIRS, ST_NSTAT
+    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_MVERT, ST_MPAIRS, ST_MBYTES, ST_NSTAT
 };
 
 struct GrowArgs {
@@ -74,7 +74,22 @@ struct GrowArgs {
     uint32_t lbcap;             // level-1 distances above gmax are only known to exceed it: the lower
                                 // bound reads min(distance, lbcap) (GST_INF_COST: no cap)
     const uint32_t* tour;       // tour table of the query (tour_table_body), nullptr: one-label only
+    uint8_t* act;               // level k's activity bytes (act[v] = 1 once some (v, X) of level k is
+                                // finite; read by the merges above), nullptr at level 1
 };
+
+// Activity bytes (and with them the merges' skipping of vertices with no finite lower state) are
+// kept for bounded solves only: an unbounded solve makes almost every state finite, so there the
+// checks would cost without skipping anything.  0 switches them off everywhere.
+#ifndef GST_MERGE_SKIP
+#define GST_MERGE_SKIP 1
+#endif
+
+// A state of the grow's level was lowered from inf (old word `old`): its vertex now holds a finite
+// state at this level.  Every writer stores the same byte, so a plain store serves.
+__device__ __forceinline__ void mark_active(uint8_t* act, uint32_t u, unsigned long long old) {
+    if (act && (uint32_t)(old >> 32) == 0xFFFFFFFFu) act[u] = 1;
+}
 
 // Lanes per push item unless the caller chose (measured on BA V=200k, g=6): one query alone is
 // latency-bound and fastest with a whole wave per item; a batch of queries is throughput-bound and
@@ -127,6 +142,7 @@ struct MergeArgs {
     // batched engine, level 2: the slot's groups and the walk scratch the tour table is built from
     const int32_t* grp;    // grp_ptr[0..g] then the members
     uint32_t* tour_w;      // 2^g * g * g words
+    uint8_t* act;          // activity bytes of every level, act[j * n + v] (nullptr: no skipping)
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -696,6 +712,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 }
                 const bool improved = lane_ok && (uint32_t)(nwv >> 32) < ocost;
                 if (lane_ok && nwv != own) *addr = nwv;
+                if (improved) mark_active(a.act, u, own);
                 const uint64_t lmask = mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1);
                 const uint64_t imp = __ballot(improved);
                 // only states that may still grow go on the frontier
@@ -792,13 +809,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     unsigned long long curw = own;
                     while (cz < (uint32_t)(curw >> 32)) {
                         const unsigned long long old = atomicCAS(addr, curw, bz);
-                        if (old == curw) { improved = true; break; }
+                        if (old == curw) { improved = true; mark_active(a.act, u, old); break; }
                         curw = old;
                     }
                     ncost = cz;
                 } else if (bs < own) {
                     const unsigned long long old = atomicMin(addr, bs);
                     improved = (uint32_t)(bs >> 32) < (uint32_t)(old >> 32);
+                    if (improved) mark_active(a.act, u, old);
                     ncost = (uint32_t)(bs >> 32);
                 }
                 if (improved) st_upd++;
@@ -906,12 +924,13 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                         // equal-cost predecessors can never close a cycle
                                         while (cand < (uint32_t)(curw >> 32)) {
                                             const unsigned long long old = atomicCAS(addr, curw, nw);
-                                            if (old == curw) { improved = true; break; }
+                                            if (old == curw) { improved = true; mark_active(a.act, u, old); break; }
                                             curw = old;
                                         }
                                     } else {
                                         const unsigned long long old = atomicMin(addr, nw);
                                         improved = cand < (uint32_t)(old >> 32);
+                                        if (improved) mark_active(a.act, u, old);
                                     }
                                 }
                                 if (improved) st_upd++;
@@ -1107,12 +1126,13 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
                 if (cand == cv) {
                     while (cand < (uint32_t)(curw >> 32)) {
                         const unsigned long long old = atomicCAS(addr, curw, nw);
-                        if (old == curw) { improved = true; break; }
+                        if (old == curw) { improved = true; mark_active(a.act, u, old); break; }
                         curw = old;
                     }
                 } else {
                     const unsigned long long old = atomicMin(addr, nw);
                     improved = cand < (uint32_t)(old >> 32);
+                    if (improved) mark_active(a.act, u, old);
                 }
             }
             if (!improved) continue;
@@ -1355,16 +1375,17 @@ __host__ __device__ __forceinline__ size_t merge_rows_words(uint32_t VB, uint32_
 // vertices per merge tile for L lower-level costs and W bitmap words per vertex within a budget of
 // tile_bytes (0: the default)
 __host__ __device__ __forceinline__ uint32_t merge_tile_vertices(uint32_t L, uint32_t W, size_t tile_bytes = 0) {
-    const size_t per = (size_t)L * 4 + (size_t)W * 8 + 4;
+    const size_t per = (size_t)L * 4 + (size_t)W * 8 + 12;
     size_t vb = (tile_bytes ? tile_bytes : MERGE_TILE_BYTES) / per;
     if (vb < 1) vb = 1;
     if (vb > GST_MERGE_VB_MAX) vb = GST_MERGE_VB_MAX;
     return (uint32_t)vb;
 }
-// dynamic LDS bytes of a merge launch: rows, bitmaps, group masks of the tile, split table
+// dynamic LDS bytes of a merge launch: rows, bitmaps, group masks, live list and finite flags of the
+// tile, split table
 __host__ __device__ __forceinline__ size_t merge_lds_bytes(uint32_t VB, uint32_t L, uint32_t W, size_t pair_bytes,
                                                          bool pin) {
-    return merge_rows_words(VB, L) * 8 + (size_t)VB * W * 8 + ((size_t)VB + 1) / 2 * 8 + (pin ? pair_bytes : 0);
+    return merge_rows_words(VB, L) * 8 + (size_t)VB * W * 8 + 3 * (((size_t)VB + 1) / 2 * 8) + (pin ? pair_bytes : 0);
 }
 
 // i / d for i < 2^20 and 1 <= d < 2^12 without an integer division: (i + 0.5) / d in float is
@@ -1375,6 +1396,7 @@ __device__ __forceinline__ uint32_t div_small(uint32_t i, float inv_d) {
 
 template <bool FLT>
 __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long long* lds, uint32_t bid, uint32_t nblk) {
+    __shared__ uint32_t s_nl;
     const uint32_t k = a.k;
     const uint32_t L = a.off[k];             // lower-level words per vertex
     const uint32_t mk = a.m[k];
@@ -1382,17 +1404,21 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     const uint32_t VB = a.VB;
     const uint32_t W = a.W;
     const uint32_t g = a.g;
-    // LDS: the tile's lower-level costs (the merge reads no predecessors), the bitmaps of finite
-    // outputs, the group masks of the tile's vertices, then the split table
+    // LDS: the live vertices' lower-level costs (the merge reads no predecessors), the bitmaps of
+    // finite outputs, the group masks of the tile's vertices (bit 31: live), the list of live tile
+    // vertices, their finite-output flags, then the split table
+    const uint32_t VB2 = (VB + 1) & ~1u;
     uint32_t* rows = (uint32_t*)lds;                                            // VB * L costs
     unsigned long long* bm = lds + merge_rows_words(VB, L);                     // VB * W
     uint32_t* gms = (uint32_t*)(bm + (size_t)VB * W);                           // VB
+    uint32_t* lst = gms + VB2;                                                  // VB
+    uint32_t* fin = lst + VB2;                                                  // VB
     // split table: in LDS transposed to [split][mask], so that the lanes of one wave (consecutive
     // masks) read consecutive entries; else the global [mask][split] table
     const unsigned long long* pt = (const unsigned long long*)a.pairs;
     uint32_t pr_stride = 1, ps_stride = S;   // entry (r, s2) at pt[r * ps_stride + s2 * pr_stride]
     if (a.pairs_in_lds) {
-        unsigned long long* pl = (unsigned long long*)(gms + ((VB + 1) & ~1u));
+        unsigned long long* pl = (unsigned long long*)(fin + VB2);
         for (uint32_t i = threadIdx.x; i < a.P; i += blockDim.x) {
             const uint32_t r = i / S, s2 = i - r * S;
             pl[s2 * mk + r] = a.pairs[i];
@@ -1406,27 +1432,69 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     const float inv_mk = 1.0f / (float)mk;
     uint32_t ovf = 0;
     unsigned long long best_root = GST_INF_WORD;
-    uint64_t st_pairs = 0, st_vert = 0;
+    uint64_t st_pairs = 0, st_vert = 0, st_bytes = 0;
     const uint64_t* Tk_base = a.T + a.lvl_base[k];
     const uint32_t* mk_masks = a.masks + a.off[k];
+    // Levels k >= 3 with activity bytes: a vertex whose levels 2..k-1 hold no finite state and which
+    // is not a member of k groups has no finite pair and no leaf at level k, so its level-k words are
+    // all inf and its lower rows are never read ("live" vertices are the others).  Level 2 merges
+    // every vertex (its pairs are level-1 distances).
+    const bool skip = a.act != nullptr && k >= 3;
+    uint8_t* act_out = (a.act && !a.is_full) ? a.act + (size_t)k * a.n : nullptr;
 
     for (uint32_t v0 = bid * VB; v0 < a.n; v0 += nblk * VB) {
         const uint32_t nv = min(VB, a.n - v0);
         __syncthreads();
-        // stage the cost halves of the lower-level runs (each level's run of the tile is one
-        // contiguous block of nv * m_j words), MERGE_UNR loads in flight per thread
+        if (threadIdx.x == 0) s_nl = skip ? 0u : nv;
+        for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
+            gms[b] = a.gm[v0 + b];
+            fin[b] = 0u;
+            if (!skip) lst[b] = b;
+        }
+        __syncthreads();
+        if (skip) {
+            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
+                bool live = (uint32_t)__popc(gms[b]) >= k;
+                for (uint32_t j = 2; j < k && !live; ++j) live = a.act[(size_t)j * a.n + v0 + b] != 0;
+                if (live) {
+                    lst[atomicAdd(&s_nl, 1u)] = b;
+                    gms[b] |= 0x80000000u;
+                }
+            }
+            __syncthreads();
+        }
+        const uint32_t nl = s_nl;
+        if (skip && nl < nv) {
+            // the non-live vertices' level-k words: inf, written without reading their rows
+            for (uint32_t i = threadIdx.x; i < nv * mk; i += blockDim.x) {
+                const uint32_t b = div_small(i, inv_mk);
+                if (!(gms[b] & 0x80000000u)) ((uint64_t*)Tk_base)[(uint64_t)v0 * mk + i] = GST_INF_WORD;
+            }
+        }
+        // stage the cost halves of the live vertices' lower-level rows (each a contiguous run of m_j
+        // words; without skipping, the tile's run of each level is one contiguous block of nv * m_j
+        // words), MERGE_UNR loads in flight per thread
         for (uint32_t j = 1; j < k; ++j) {
             const uint32_t mj = a.m[j];
             const float inv_mj = 1.0f / (float)mj;
             const uint64_t* src = a.T + a.lvl_base[j] + (uint64_t)v0 * mj;
-            const uint32_t cnt = nv * mj;
+            const uint32_t cnt = nl * mj;
             const uint32_t oj = a.off[j];
             for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += MERGE_UNR * blockDim.x) {
                 uint64_t x[MERGE_UNR];
 #pragma unroll
                 for (int u = 0; u < MERGE_UNR; ++u) {
                     const uint32_t i = i0 + u * blockDim.x;
-                    x[u] = i < cnt ? src[i] : 0ull;
+                    uint64_t val = 0ull;
+                    if (i < cnt) {
+                        if (skip) {
+                            const uint32_t li = div_small(i, inv_mj);
+                            val = src[(uint64_t)lst[li] * mj + (i - li * mj)];
+                        } else {
+                            val = src[i];
+                        }
+                    }
+                    x[u] = val;
                 }
 #pragma unroll
                 for (int u = 0; u < MERGE_UNR; ++u) {
@@ -1439,13 +1507,12 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             }
         }
         for (uint32_t i = threadIdx.x; i < nv * W; i += blockDim.x) bm[i] = 0;
-        for (uint32_t i = threadIdx.x; i < nv; i += blockDim.x) gms[i] = a.gm[v0 + i];
         __syncthreads();
         if (a.spider) {
-            // cheapest spider of the tile: sum of the group distances of one vertex (positions
-            // 0..g-1 of the staged rows are the level-1 masks 1<<i)
+            // cheapest spider of the tile (level 2: every vertex is staged): sum of the group
+            // distances of one vertex (positions 0..g-1 of the staged rows are the masks 1<<i)
             uint32_t sm = GST_INF_COST;
-            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
+            for (uint32_t b = threadIdx.x; b < nl; b += blockDim.x) {
                 uint32_t acc = 0, o2 = 0;
                 for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, rows[b * L + gi], o2);
                 sm = min(sm, o2 ? GST_INF_COST : acc);
@@ -1453,13 +1520,14 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             sm = wave_min32(sm);
             if (lane_id() == 0 && sm != GST_INF_COST) atomicMin(a.spider, sm);
         }
-        const uint32_t items = nv * mk;
+        const uint32_t items = nl * mk;
         for (uint32_t i = threadIdx.x; i < items; i += blockDim.x) {
-            const uint32_t b = div_small(i, inv_mk), r = i - b * mk;
+            const uint32_t li = div_small(i, inv_mk), r = i - li * mk;
+            const uint32_t b = lst[li];
             const uint32_t v = v0 + b;
             const uint32_t X = mk_masks[r];
-            const uint32_t gmv = gms[b];
-            const uint32_t* rv = rows + (size_t)b * L;
+            const uint32_t gmv = gms[b] & 0x7FFFFFFFu;
+            const uint32_t* rv = rows + (size_t)li * L;
             unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
             const unsigned long long* pr = pt + (size_t)r * ps_stride;
             for (uint32_t s2 = 0; s2 < S; ++s2) {
@@ -1481,9 +1549,10 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                 const uint32_t lb = prune_lb ? state_lb<FLT>([&](uint32_t gi) { return rv[gi]; }, nullptr, g, X, a.lbcap) : 0u;
                 if (reaches_ub<FLT>((uint32_t)(best >> 32), lb, a.ub)) best = GST_INF_WORD;
             }
-            ((uint64_t*)Tk_base)[(uint64_t)v0 * mk + i] = best;
+            ((uint64_t*)Tk_base)[(uint64_t)v * mk + r] = best;
             bool flag = false;
             if (best != GST_INF_WORD) {
+                if (act_out) fin[b] = 1u;
                 if (a.is_full) {
                     const unsigned long long rw = ((best >> 32) << 32) | v;
                     best_root = rw < best_root ? rw : best_root;
@@ -1504,8 +1573,13 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             }
         }
         st_pairs += (uint64_t)items * S;
-        st_vert += nv;
+        st_vert += nl;
+        // bytes: live rows read, every level-k word written, group masks, activity bytes
+        st_bytes += (uint64_t)nl * L * 8 + (uint64_t)nv * mk * 8 + (uint64_t)nv * 4 +
+                    (skip ? (uint64_t)nv * (k - 2) : 0) + (act_out ? nv : 0);
         __syncthreads();
+        if (act_out)
+            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) act_out[v0 + b] = (uint8_t)fin[b];
         if (!a.is_full) {
             for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
                 const uint32_t v = v0 + b;
@@ -1528,6 +1602,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     if (threadIdx.x == 0) {
         atomicAdd(&a.stats[ST_MPAIRS], (unsigned long long)st_pairs);
         atomicAdd(&a.stats[ST_MVERT], (unsigned long long)st_vert);
+        atomicAdd(&a.stats[ST_MBYTES], (unsigned long long)st_bytes);
     }
     if (lane_id() == 0 && so) atomicOr(&a.ctl->err, 1u);
 }

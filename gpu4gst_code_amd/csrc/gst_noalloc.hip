@@ -215,6 +215,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         uint64_t* T; uint32_t* gm; uint64_t* chg[3]; uint32_t* inq[3]; uint64_t* lists[3]; RoundCtl* ctl;
         unsigned long long* root; unsigned long long* stats; uint32_t* spider; uint32_t* tour; uint32_t* edges;
         int32_t* grp; uint64_t* pairs; uint32_t* masks; uint32_t* pos_of; uint64_t* all_items; uint32_t* all_cnt;
+        uint8_t* act;
     } B;
     auto layout = [&](Carve& c) {
         B.T = c.take<uint64_t>(words);
@@ -236,6 +237,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         B.pos_of = c.take<uint32_t>(L.pos_of.size());
         B.all_items = c.take<uint64_t>(items_cap);
         B.all_cnt = c.take<uint32_t>(1);
+        B.act = c.take<uint8_t>((size_t)n * (size_t)(g + 1));
     };
     {
         Carve dry{nullptr};
@@ -297,6 +299,8 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         ga.chunk = chunk; ga.ub = ub_bits; ga.gmax = bb.gmax; ga.lbcap = bb.lbcap;
         ga.use_lb = (opts.prune_lb && k >= 2) ? 1u : 0u;   // level 1's distances are final only after it
         ga.tour = k >= 2 ? tour : nullptr;
+        // activity bytes once a bound is known (after level 1 with the automatic bound)
+        ga.act = (k >= 2 && GST_MERGE_SKIP && ub_bits != GST_INF_COST) ? B.act + (size_t)k * n : nullptr;
         ga.all_items = B.all_items; ga.n_all = n_all; ga.n_all_dev = nullptr; ga.n = n;
         const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
         ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu : (uint32_t)std::max<uint64_t>(1, n_items_total * (uint64_t)pm / 1000);
@@ -376,6 +380,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         ma.W = (uint32_t)((L.m[k] + 63) / 64); ma.chunk = chunk; ma.ub = ub_bits; ma.use_lb = opts.prune_lb ? 1u : 0u;
         ma.gmax = bb.gmax; ma.lbcap = bb.lbcap;
         ma.tour = tour;
+        ma.act = (GST_MERGE_SKIP && ub_bits != GST_INF_COST) ? B.act : nullptr;
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;
@@ -457,8 +462,6 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     S.merge_vertices = H->stats[ST_MVERT]; S.merge_pairs = H->stats[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.grow_bytes = S.grow_items * 24ull + S.grow_arcs * 12ull + S.grow_relax * 8ull + S.grow_updates * 16ull;
-    uint64_t mb = 0;
-    for (int k = 2; k <= g; ++k) mb += (uint64_t)(L.off[k] + L.m[k]) * 8ull;
-    S.merge_bytes = (uint64_t)n * mb;
+    S.merge_bytes = H->stats[ST_MBYTES];
     return GST_OK;
 }

@@ -227,3 +227,26 @@ def test_tour_bound_rejects_bad_value(solver):
     rc = solver.lib.gst_solve(solver.ctx, dg.handle, C.byref(grp), C.byref(opts), C.byref(res))
     assert rc == gst._abi.GST_INVALID_ARG
     dg.free()
+
+
+def test_merge_reads_only_active_vertices(solver):
+    """Under a bound, levels >= 3 merge only vertices with a finite state at some level 2..k-1 (or
+    in k groups): the others' words are inf without their rows being read.  Same exact cost, far
+    fewer vertex rows merged than the unbounded solve, and each batched query agrees."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    qs = [gst.gen_groups(G.n, 6, 10, 200, seed=1000 + i) for i in range(4)]
+    for q in qs:
+        p = solver.solve(dg, q)
+        b = solver.solve(dg, q, ub=p.cost)
+        a = solver.solve(dg, q, auto_bound=True)
+        print(f"cost {p.cost}: merged rows plain {p.stats['merge_vertices']} bounded {b.stats['merge_vertices']} "
+              f"spider {a.stats['merge_vertices']}", flush=True)
+        assert b.cost == p.cost == a.cost
+        assert b.stats["merge_vertices"] < p.stats["merge_vertices"] / 2
+        assert b.stats["merge_bytes"] < p.stats["merge_bytes"]
+        assert gst.validate_tree(G, q, b.eu, b.ev, b.root) == p.cost
+    res, _ = solver.solve_batch(dg, qs, auto_bound=True, slots=2)
+    for q, r in zip(qs, res):
+        assert r.cost == solver.solve(dg, q, want_tree=False).cost
+    dg.free()
