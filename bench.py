@@ -221,7 +221,7 @@ def main():
     for i, r in enumerate(kres):
         assert r.cost == plain[i].cost, (i, r.cost, plain[i].cost)
     kcnt = {k: int(sum(r.stats[k] for r in kres)) for k in
-            ("grow_items", "grow_arcs", "grow_relax", "grow_updates", "grow_enqueued")}
+            ("grow_items", "grow_arcs", "grow_relax", "grow_updates", "grow_enqueued", "grow_new_states")}
     glaunch = max(int(kst["grow_launches"]), 1)
     mlaunch = max(int(kst["merge_launches"]), 1)
     b_grow_gbs = gbs(kst["grow_bytes"], kst["t_grow_ms"])

@@ -731,7 +731,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.Tk = sl.T + lvl_base[k];
         a.T1 = sl.T + lvl_base[1];
         a.masks = E->masks[g] + L.off[k];
-        a.row = G->d_row; a.col = G->d_col; a.w = G->d_w; a.rev = G->d_rev;
+        a.row = G->d_row; a.arc = (const uint2*)G->d_arc;
         for (int i = 0; i < 3; ++i) { a.list[i] = sl.lists[i]; a.chg[i] = sl.chg[i]; a.inq[i] = sl.inq[i]; }
         a.ctl = sl.ctl + k;
         a.stats = sl.stats;
@@ -793,7 +793,8 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
     {
         TraceArgs& ta = sl.h_sb->t;
         std::memset(&ta, 0, sizeof(ta));
-        ta.T = sl.T; ta.pos_of = E->pos_of[g]; ta.col = G->d_col; ta.w = G->d_w; ta.root = sl.root;
+        ta.T = sl.T; ta.pos_of = E->pos_of[g]; ta.row = G->d_row; ta.arc = (const uint2*)G->d_arc; ta.root = sl.root;
+        ta.flt = flt ? 1u : 0u;
         ta.out_u = sl.edges; ta.out_v = sl.edges + 1; ta.out_w = sl.edges + 2; ta.ostride = 3;
         ta.out_n = sl.out_n;
         ta.ctl = sl.ctl;
@@ -1095,7 +1096,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
             res->used_ub = has_ub ? 1 : 0;
             gst_stats& S2 = res->stats;
             S2.grow_items = hs[ST_ITEMS]; S2.grow_arcs = hs[ST_ARCS]; S2.grow_relax = hs[ST_RELAX];
-            S2.grow_updates = hs[ST_UPD]; S2.grow_enqueued = hs[ST_ENQ];
+            S2.grow_updates = hs[ST_UPD]; S2.grow_enqueued = hs[ST_ENQ]; S2.grow_new_states = hs[ST_NEW];
             S2.merge_vertices = hs[ST_MVERT]; S2.merge_pairs = hs[ST_MPAIRS];
             S2.grow_rounds = rounds;
             S2.grow_bytes = S2.grow_items * 24ull + S2.grow_arcs * 12ull + S2.grow_relax * 8ull + S2.grow_updates * 16ull;

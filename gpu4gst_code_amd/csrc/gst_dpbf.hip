@@ -139,17 +139,11 @@ gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {
     const size_t n = (size_t)g->n, m = (size_t)g->m;
     std::vector<uint32_t> row32(n + 1);
     for (size_t i = 0; i <= n; ++i) row32[i] = (uint32_t)g->h_row[i];
-    std::vector<uint32_t> rev(std::max<size_t>(m, 1));
-    for (size_t v = 0; v < n; ++v) {
-        for (int64_t e = g->h_row[v]; e < g->h_row[v + 1]; ++e) {
-            const int32_t u = g->h_col[(size_t)e];
-            rev[(size_t)e] = (uint32_t)gst_find_arc(g->h_row.data(), g->h_col.data(), g->h_w.data(), u, (int32_t)v,
-                                                    g->h_w[(size_t)e]);
-        }
-    }
+    std::vector<uint64_t> arc(std::max<size_t>(m, 1));
+    for (size_t e = 0; e < m; ++e) arc[e] = ((uint64_t)g->h_w[e] << 32) | (uint32_t)g->h_col[e];
     const size_t mm = std::max<size_t>(m, 1);
     if (hipMalloc(&g->d_row, (n + 1) * 4) != hipSuccess || hipMalloc(&g->d_col, mm * 4) != hipSuccess ||
-        hipMalloc(&g->d_w, mm * 4) != hipSuccess || hipMalloc(&g->d_rev, mm * 4) != hipSuccess) {
+        hipMalloc(&g->d_w, mm * 4) != hipSuccess || hipMalloc(&g->d_arc, mm * 8) != hipSuccess) {
         gst_graph_device_free(g);
         return gst_set_error(ctx, GST_OOM, "graph upload: device allocation failed (%zu arcs)", m);
     }
@@ -158,7 +152,7 @@ gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {
     if (m) {
         HIPCHK(hipMemcpyAsync(g->d_col, g->h_col.data(), m * 4, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(g->d_w, g->h_w.data(), m * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(g->d_rev, rev.data(), m * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(g->d_arc, arc.data(), m * 8, hipMemcpyHostToDevice, s));
     }
     HIPCHK(hipStreamSynchronize(s));
     return GST_OK;
@@ -167,8 +161,8 @@ gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {
 void gst_graph_device_free(gst_graph* g) {
     if (!g) return;
     if (g->ctx) hipSetDevice(g->ctx->device);
-    hipFree(g->d_row); hipFree(g->d_col); hipFree(g->d_w); hipFree(g->d_rev);
-    g->d_row = nullptr; g->d_col = nullptr; g->d_w = nullptr; g->d_rev = nullptr;
+    hipFree(g->d_row); hipFree(g->d_col); hipFree(g->d_w); hipFree(g->d_arc);
+    g->d_row = nullptr; g->d_col = nullptr; g->d_w = nullptr; g->d_arc = nullptr;
 }
 
 namespace {
@@ -451,7 +445,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ga.Tk = B->T + lvl_base[k];
         ga.T1 = B->T + lvl_base[1];
         ga.masks = B->masks + L.off[k];
-        ga.row = G->d_row; ga.col = G->d_col; ga.w = G->d_w; ga.rev = G->d_rev;
+        ga.row = G->d_row; ga.arc = (const uint2*)G->d_arc;
         for (int i = 0; i < 3; ++i) { ga.list[i] = B->lists[i]; ga.chg[i] = B->chg[i]; ga.inq[i] = B->inq[i]; }
         ga.ctl = B->ctl + k; ga.stats = B->stats;
         ga.m = (uint32_t)L.m[k]; ga.W = (uint32_t)((L.m[k] + 63) / 64); ga.g = (uint32_t)g;
@@ -704,7 +698,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
 
     if (opts.want_tree) {
         TraceArgs ta;
-        ta.T = B->T; ta.pos_of = B->pos_of; ta.col = G->d_col; ta.w = G->d_w; ta.root = B->root;
+        ta.T = B->T; ta.pos_of = B->pos_of; ta.row = G->d_row; ta.arc = (const uint2*)G->d_arc; ta.root = B->root;
+        ta.flt = flt ? 1u : 0u;
         ta.out_u = B->edges; ta.out_v = B->edges + ecap; ta.out_w = B->edges + 2 * ecap; ta.out_n = B->edges + 3 * ecap;
         ta.ostride = 1;
         ta.ctl = B->ctl;   // entry 0 carries query-level flags
@@ -755,7 +750,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     HIPCHK(hipStreamSynchronize(s));
     gst_stats& S = res->stats;
     S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
-    S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ];
+    S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW];
     S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.async_launches = async_launches;

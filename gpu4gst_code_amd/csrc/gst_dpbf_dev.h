@@ -36,7 +36,7 @@ struct RoundCtl {
 };
 
 enum StatIdx {
-    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_MVERT, ST_MPAIRS, ST_MBYTES, ST_NSTAT
+    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_NEW, ST_MVERT, ST_MPAIRS, ST_MBYTES, ST_NSTAT
 };
 
 struct GrowArgs {
@@ -44,9 +44,7 @@ struct GrowArgs {
     const uint64_t* T1;    // level-1 block (group distances, for the lower bound)
     const uint32_t* masks; // masks of level k by rank
     const uint32_t* row;
-    const int32_t* col;
-    const uint32_t* w;
-    const uint32_t* rev;
+    const uint2* arc;      // arc records: .x = neighbour, .y = weight bits
     uint64_t* list[3];
     uint64_t* chg[3];
     uint32_t* inq[3];
@@ -87,8 +85,10 @@ struct GrowArgs {
 
 // A state of the grow's level was lowered from inf (old word `old`): its vertex now holds a finite
 // state at this level.  Every writer stores the same byte, so a plain store serves.
-__device__ __forceinline__ void mark_active(uint8_t* act, uint32_t u, unsigned long long old) {
-    if (act && (uint32_t)(old >> 32) == 0xFFFFFFFFu) act[u] = 1;
+__device__ __forceinline__ uint32_t mark_active(uint8_t* act, uint32_t u, unsigned long long old) {
+    const bool fresh = (uint32_t)(old >> 32) == 0xFFFFFFFFu;
+    if (act && fresh) act[u] = 1;
+    return fresh ? 1u : 0u;   // counted: states made finite (the rest of the updates lower a finite cost)
 }
 
 // Lanes per push item unless the caller chose (measured on BA V=200k, g=6): one query alone is
@@ -633,7 +633,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     __syncthreads();
 
     // phase B: relax the items of round r
-    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0;
+    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, st_new = 0;
     uint32_t ovf = 0;
     const uint32_t gwave = bid * GROW_WAVES + (threadIdx.x / WAVE);
     const uint32_t nwaves = part * GROW_WAVES;
@@ -661,7 +661,6 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             const bool light = row_ok && re > rs && re - rs <= HUB_DEG;
             const uint32_t deg = light ? re - rs : 0u;
             st_items += wave_sum32(light && t0 == 0 ? 1u : 0u);
-            st_arcs += wave_sum32(light && t0 == 0 ? deg : 0u);
             uint32_t maxdeg = deg;
 #pragma unroll
             for (int d = 1; d < WAVE; d <<= 1) maxdeg = max(maxdeg, (uint32_t)__shfl_xor(maxdeg, d, WAVE));
@@ -679,14 +678,15 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j) {
                         const bool ok = lane_ok && s + j < deg;
-                        wt[j] = ok ? a.w[rs + s + j] : 0u;
+                        const uint2 r = ok ? a.arc[rs + s + j] : make_uint2(0xFFFFFFFFu, 0u);
+                        wt[j] = r.y;
                         live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
+                        nb[j] = live[j] ? r.x : 0xFFFFFFFFu;
+                        st_arcs += (ok && t0 == 0) ? 1u : 0u;   // arcs read (once per row, lane t0 = 0)
                     }
                     // rows run in increasing weight: when the step's first arc is dead for every
                     // lane, the rest of every row is dead too
                     if (__ballot(live[0]) == 0) break;
-#pragma unroll
-                    for (uint32_t j = 0; j < UNR; ++j) nb[j] = live[j] ? (uint32_t)a.col[rs + s + j] : 0xFFFFFFFFu;
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j)
                         cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
@@ -697,7 +697,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         if (cn[j] < ocost && cn[j] <= a.gmax) {
                             const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
                             if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
-                                const unsigned long long nw = ((unsigned long long)cand << 32) | (rs + s + j);
+                                const unsigned long long nw = ((unsigned long long)cand << 32) | nb[j];
                                 if (cand > cn[j]) bs = nw < bs ? nw : bs;
                                 else if (cand < ocost) bz = nw < bz ? nw : bz;
                             }
@@ -712,7 +712,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 }
                 const bool improved = lane_ok && (uint32_t)(nwv >> 32) < ocost;
                 if (lane_ok && nwv != own) *addr = nwv;
-                if (improved) mark_active(a.act, u, own);
+                if (improved) st_new += mark_active(a.act, u, own);
                 const uint64_t lmask = mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1);
                 const uint64_t imp = __ballot(improved);
                 // only states that may still grow go on the frontier
@@ -744,7 +744,6 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
         const uint32_t e0 = rs + c * a.chunk;
         const uint32_t e1 = min(re, e0 + a.chunk);
         st_items++;
-        st_arcs += e1 - e0;
         for (uint32_t mb = 0; mb < m; mb += WAVE) {
             const uint32_t na = min((uint32_t)WAVE, m - mb);
             const uint32_t A = WAVE / na;
@@ -765,13 +764,13 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 for (uint32_t j = 0; j < UNR; ++j) {
                     const uint32_t e = e0 + (s + j) * A + s_arc;
                     const bool ok = lane_ok && e < e1;
-                    wt[j] = ok ? a.w[e] : 0u;
+                    const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
+                    wt[j] = r.y;
                     live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
+                    nb[j] = live[j] ? r.x : 0xFFFFFFFFu;
+                    st_arcs += (ok && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
                 }
                 if (__ballot(live[0]) == 0) break;   // each lane's arcs run in increasing weight
-#pragma unroll
-                for (uint32_t j = 0; j < UNR; ++j)
-                    nb[j] = live[j] ? (uint32_t)a.col[e0 + (s + j) * A + s_arc] : 0xFFFFFFFFu;
 #pragma unroll
                 for (uint32_t j = 0; j < UNR; ++j)
                     cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
@@ -783,8 +782,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     if (cn[j] < ocost && cn[j] <= a.gmax) {
                         const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
                         if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
-                            const uint32_t e = e0 + (s + j) * A + s_arc;
-                            const unsigned long long nw = ((unsigned long long)cand << 32) | e;
+                            const unsigned long long nw = ((unsigned long long)cand << 32) | nb[j];
                             if (cand > cn[j]) bs = nw < bs ? nw : bs;
                             else if (cand < ocost) bz = nw < bz ? nw : bz;
                         }
@@ -809,14 +807,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     unsigned long long curw = own;
                     while (cz < (uint32_t)(curw >> 32)) {
                         const unsigned long long old = atomicCAS(addr, curw, bz);
-                        if (old == curw) { improved = true; mark_active(a.act, u, old); break; }
+                        if (old == curw) { improved = true; st_new += mark_active(a.act, u, old); break; }
                         curw = old;
                     }
                     ncost = cz;
                 } else if (bs < own) {
                     const unsigned long long old = atomicMin(addr, bs);
                     improved = (uint32_t)(bs >> 32) < (uint32_t)(old >> 32);
-                    if (improved) mark_active(a.act, u, old);
+                    if (improved) st_new += mark_active(a.act, u, old);
                     ncost = (uint32_t)(bs >> 32);
                 }
                 if (improved) st_upd++;
@@ -854,7 +852,6 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             e1 = min(re, e0 + a.chunk);
         }
         st_items += wave_sum32(has && pl == 0 ? 1u : 0u);
-        st_arcs += wave_sum32(has && pl == 0 ? e1 - e0 : 0u);
         for (uint32_t wd = 0; wd < W; ++wd) {
             uint64_t bm = has ? ccur[(uint64_t)v * W + wd] : 0ull;
             // drop masks whose cost (plus lower bound) reaches the bound
@@ -884,24 +881,21 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 const uint32_t len = e1 - e0;
                 const uint32_t steps = wave_max32(lane_ok ? (len + A - 1) / A : 0u);
                 for (uint32_t s = 0; s < steps; s += UNR) {
-                    uint32_t uu[UNR], wt[UNR], ra[UNR];
+                    uint32_t uu[UNR], wt[UNR];
                     unsigned long long cw0[UNR];
                     bool live[UNR];
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j) {
                         const uint32_t e = e0 + (s + j) * A + s_arc;
                         const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-                        wt[j] = ok ? a.w[e] : 0u;
+                        const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
+                        wt[j] = r.y;
                         // under a bound, cv + w >= ub holds for every heavier arc of the row as well
                         live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
+                        uu[j] = live[j] ? r.x : 0xFFFFFFFFu;
+                        st_arcs += (ok && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
                     }
                     if (__ballot(live[0]) == 0) break;
-#pragma unroll
-                    for (uint32_t j = 0; j < UNR; ++j) {
-                        const uint32_t e = e0 + (s + j) * A + s_arc;
-                        uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
-                        ra[j] = live[j] ? a.rev[e] : 0u;
-                    }
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j)
                         cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
@@ -916,7 +910,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                             if (cand != GST_INF_COST && cand <= (uint32_t)(curw >> 32) &&
                                 (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
                                 unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-                                const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
+                                const unsigned long long nw = ((unsigned long long)cand << 32) | v;
                                 bool improved = false;
                                 if (nw < curw) {
                                     if (cand == cv) {
@@ -924,13 +918,13 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                         // equal-cost predecessors can never close a cycle
                                         while (cand < (uint32_t)(curw >> 32)) {
                                             const unsigned long long old = atomicCAS(addr, curw, nw);
-                                            if (old == curw) { improved = true; mark_active(a.act, u, old); break; }
+                                            if (old == curw) { improved = true; st_new += mark_active(a.act, u, old); break; }
                                             curw = old;
                                         }
                                     } else {
                                         const unsigned long long old = atomicMin(addr, nw);
                                         improved = cand < (uint32_t)(old >> 32);
-                                        if (improved) mark_active(a.act, u, old);
+                                        if (improved) st_new += mark_active(a.act, u, old);
                                     }
                                 }
                                 if (improved) st_upd++;
@@ -960,12 +954,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     // statistics: LDS reduction, one global atomic per workgroup and counter
     {
         const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
-        const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
+        const uint32_t sa = wave_sum32(st_arcs);
         const uint32_t sr = wave_sum32(st_relax);
         const uint32_t su = wave_sum32(st_upd);
         const uint32_t se = wave_sum32(st_enq);
+        const uint32_t sn = wave_sum32(st_new);
         const uint32_t so = wave_sum32(ovf);
         if (lane == 0) {
+            if (sn) atomicAdd(&s_stats[ST_NEW], (unsigned long long)sn);
             if (si) atomicAdd(&s_stats[ST_ITEMS], (unsigned long long)si);
             if (sa) atomicAdd(&s_stats[ST_ARCS], (unsigned long long)sa);
             if (sr) atomicAdd(&s_stats[ST_RELAX], (unsigned long long)sr);
@@ -1056,7 +1052,8 @@ __device__ __forceinline__ void task_write(const GrowArgs& a, uint32_t n0, uint3
 // Convergent: all lanes of the wave call it with the same arguments.
 template <bool FLT>
 __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t e0, uint32_t e1, uint32_t wd,
-                           uint64_t bm, uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq, uint32_t& ovf) {
+                           uint64_t bm, uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq, uint32_t& ovf,
+                           uint32_t& st_new, uint32_t& st_arcs) {
     const uint32_t lane = lane_id();
     const uint32_t m = a.m, W = a.W;
     uint64_t* chg = a.chg[a.ctl->round % 3];
@@ -1087,23 +1084,20 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
     const uint32_t wsel = rk >> 6;
     const uint32_t steps = (e1 - e0 + A - 1) / A;
     for (uint32_t s = 0; s < steps; s += UNR) {
-        uint32_t uu[UNR], wt[UNR], ra[UNR];
+        uint32_t uu[UNR], wt[UNR];
         unsigned long long cw0[UNR];
         bool live[UNR];
 #pragma unroll
         for (uint32_t j = 0; j < UNR; ++j) {
             const uint32_t e = e0 + (s + j) * A + s_arc;
             const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-            wt[j] = ok ? a.w[e] : 0u;
+            const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
+            wt[j] = r.y;
             live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
+            uu[j] = live[j] ? r.x : 0xFFFFFFFFu;
+            st_arcs += (ok && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
         }
         if (__ballot(live[0]) == 0) break;   // arcs run in increasing weight
-#pragma unroll
-        for (uint32_t j = 0; j < UNR; ++j) {
-            const uint32_t e = e0 + (s + j) * A + s_arc;
-            uu[j] = live[j] ? (uint32_t)a.col[e] : 0xFFFFFFFFu;
-            ra[j] = live[j] ? a.rev[e] : 0u;
-        }
 #pragma unroll
         for (uint32_t j = 0; j < UNR; ++j)
             cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
@@ -1120,19 +1114,19 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
             if (cand == GST_INF_COST || cand > (uint32_t)(curw >> 32)) continue;
             if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand, 0u, a.ub)) continue;
             unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
-            const unsigned long long nw = ((unsigned long long)cand << 32) | ra[j];
+            const unsigned long long nw = ((unsigned long long)cand << 32) | v;
             bool improved = false;
             if (nw < curw) {
                 if (cand == cv) {
                     while (cand < (uint32_t)(curw >> 32)) {
                         const unsigned long long old = atomicCAS(addr, curw, nw);
-                        if (old == curw) { improved = true; mark_active(a.act, u, old); break; }
+                        if (old == curw) { improved = true; st_new += mark_active(a.act, u, old); break; }
                         curw = old;
                     }
                 } else {
                     const unsigned long long old = atomicMin(addr, nw);
                     improved = cand < (uint32_t)(old >> 32);
-                    if (improved) mark_active(a.act, u, old);
+                    if (improved) st_new += mark_active(a.act, u, old);
                 }
             }
             if (!improved) continue;
@@ -1192,7 +1186,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
     uint64_t* chg = a.chg[cur];
     uint32_t* inq = a.inq[cur];
     const uint64_t* lcur = a.list[cur];
-    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, ovf = 0;
+    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, ovf = 0, st_new = 0;
     uint32_t done_local = 0;
     bool stop = false;
     while (!stop) {
@@ -1267,9 +1261,8 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
                 const uint32_t nc = (re - rs + a.chunk - 1) / a.chunk;
                 if (nz && nc == 1) {
                     st_items++;
-                    st_arcs += re - rs;
                     for (uint32_t w = 0; w < W && w < 16; ++w)
-                        if (bms[w]) async_relax<FLT>(a, n0, v, rs, re, w, bms[w], st_relax, st_upd, st_enq, ovf);
+                        if (bms[w]) async_relax<FLT>(a, n0, v, rs, re, w, bms[w], st_relax, st_upd, st_enq, ovf, st_new, st_arcs);
                 } else if (nz && nc > 1) {
                     // split the row: one CHUNK task per (chunk, non-empty word)
                     const uint32_t ntask = nc * nz;
@@ -1294,8 +1287,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
                 const uint32_t e0 = rs + c * a.chunk;
                 const uint32_t e1 = min(re, e0 + a.chunk);
                 st_items++;
-                st_arcs += e1 - e0;
-                async_relax<FLT>(a, n0, v, e0, e1, wd, bm, st_relax, st_upd, st_enq, ovf);
+                async_relax<FLT>(a, n0, v, e0, e1, wd, bm, st_relax, st_upd, st_enq, ovf, st_new, st_arcs);
             }
             done_local++;
         }
@@ -1303,12 +1295,14 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
     }
     if (lane == 0 && done_local) atomicSub(&ac->pending, done_local);
     const uint32_t si = wave_sum32(lane == 0 ? st_items : 0u);
-    const uint32_t sa = wave_sum32(lane == 0 ? st_arcs : 0u);
+    const uint32_t sa = wave_sum32(st_arcs);
     const uint32_t sr = wave_sum32(st_relax);
     const uint32_t su = wave_sum32(st_upd);
     const uint32_t se = wave_sum32(st_enq);
+    const uint32_t sn = wave_sum32(st_new);
     const uint32_t so = wave_sum32(ovf);
     if (lane == 0) {
+        if (sn) atomicAdd(&a.stats[ST_NEW], (unsigned long long)sn);
         if (si) atomicAdd(&a.stats[ST_ITEMS], (unsigned long long)si);
         if (sa) atomicAdd(&a.stats[ST_ARCS], (unsigned long long)sa);
         if (sr) atomicAdd(&a.stats[ST_RELAX], (unsigned long long)sr);
@@ -1652,8 +1646,9 @@ __global__ void k_root_scan_l1(const uint64_t* T1, uint32_t n, unsigned long lon
 struct TraceArgs {
     const uint64_t* T;
     const uint32_t* pos_of;   // mask -> row position
-    const int32_t* col;
-    const uint32_t* w;
+    const uint32_t* row;
+    const uint2* arc;         // arc records: .x = neighbour, .y = weight bits
+    uint32_t flt;             // weights are float32 bits
     const unsigned long long* root;
     uint32_t* out_u;
     uint32_t* out_v;
@@ -1668,6 +1663,42 @@ struct TraceArgs {
     uint32_t max_iter;
     uint32_t ostride;         // edge i is out_u[i * ostride], out_v[i * ostride], out_w[i * ostride]
 };
+
+// Weight bits of the arc (v, x) for a grown state (v, X) of cost bits cv whose predecessor is x with
+// cost bits cx.  The grow set cv = cx + w(v, x) (ints exactly; floats rounded once), and rows are
+// ordered by (weight, neighbour): a binary search for the first arc of weight >= the estimate (less a
+// rounding margin for floats) and a short forward scan find it; parallel arcs were collapsed, so the
+// neighbour is met once.  A full scan of the row is the fallback.  found = false: no such arc.
+__device__ __forceinline__ uint32_t trace_arc_weight(const TraceArgs& a, uint32_t v, uint32_t x, uint32_t cv,
+                                                     uint32_t cx, bool& found) {
+    const uint32_t rs = a.row[v], re = a.row[v + 1];
+    uint32_t wlo, whi;
+    if (a.flt) {
+        const float fv = __uint_as_float(cv), fx = __uint_as_float(cx);
+        const float d = fv - fx, eps = fv * 1e-5f + 1e-30f;
+        wlo = __float_as_uint(fmaxf(d - eps, 0.0f));
+        whi = __float_as_uint(d + eps);
+    } else {
+        wlo = cv >= cx ? cv - cx : 0u;
+        whi = wlo;
+    }
+    uint32_t lo = rs, hi = re;   // first arc of weight >= wlo
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.arc[mid].y < wlo) lo = mid + 1; else hi = mid;
+    }
+    for (uint32_t e = lo; e < re; ++e) {
+        const uint2 r = a.arc[e];
+        if (r.y > whi) break;
+        if (r.x == x) { found = true; return r.y; }
+    }
+    for (uint32_t e = rs; e < re; ++e) {
+        const uint2 r = a.arc[e];
+        if (r.x == x) { found = true; return r.y; }
+    }
+    found = false;
+    return 0u;
+}
 
 __device__ __forceinline__ void trace_body(const TraceArgs& a) {
     __shared__ uint64_t stk[TRACE_STACK];
@@ -1719,11 +1750,15 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a) {
                     npush = 2;
                 }
             } else {
-                const uint32_t u = (uint32_t)a.col[pred];
-                eu = v; ev = u; ew = a.w[pred];
-                emit = true;
+                const uint32_t u = pred;   // the neighbour this state grew from
+                const uint64_t wu = a.T[a.lvl_base[k] + (uint64_t)u * a.m[k] + rk];
+                bool found = false;
+                ew = trace_arc_weight(a, v, u, (uint32_t)(wv >> 32), (uint32_t)(wu >> 32), found);
+                if (!found || wu == GST_INF_WORD) fail_s = 1;
+                eu = v; ev = u;
+                emit = found;
                 push0 = ((uint64_t)X << 32) | u;
-                npush = 1;
+                npush = found ? 1 : 0;
             }
         }
         __syncthreads();

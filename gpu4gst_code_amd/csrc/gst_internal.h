@@ -15,8 +15,8 @@
 // Packed DP state word: high 32 bits = cost, low 32 bits = predecessor.
 //   cost  : uint32 for integer weights, IEEE-754 bits of a non-negative float for float32
 //           weights (non-negative float bit patterns order like the floats); 0xFFFFFFFF = inf.
-//   pred  : < 0x80000000  -> grow: index of the arc (u -> src) in the row of this state's
-//                            vertex u; the tree edge is (u, col[pred]) with weight w[pred]
+//   pred  : < 0x80000000  -> grow: the neighbour x whose state (x, X) grew into this one over the
+//                            arc (u, x); the tree edge is (u, x)
 //           0x80000000     -> leaf: v itself covers the mask
 //           0x80000000|X1  -> merge of (v,X1) and (v,X^X1), X1 holds the lowest bit of X
 // One 64-bit atomic-min keeps cost and predecessor consistent (the reference updates cost by
@@ -53,7 +53,8 @@ struct gst_graph {
     uint32_t* d_row = nullptr;   // n+1 arc offsets (arcs < 2^31)
     int32_t*  d_col = nullptr;   // m
     uint32_t* d_w = nullptr;     // m (int32 or float bits)
-    uint32_t* d_rev = nullptr;   // m: index of the reverse arc
+    uint64_t* d_arc = nullptr;   // m arc records, (weight bits << 32) | neighbour: the grow and the
+                                 // traceback read both in one 8-byte load
     // host copy for validation / traceback weight lookups
     std::vector<int64_t> h_row;
     std::vector<int32_t> h_col;

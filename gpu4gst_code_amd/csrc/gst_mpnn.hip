@@ -747,13 +747,13 @@ __global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_
 }
 
 // fill the subgraph CSR: new row i holds old vertex old_of[i]; arcs keep their order (new ids are
-// monotone in the old ones, so rows stay in (weight, neighbour) order)
+// monotone in the old ones, so rows stay in (weight, neighbour) order); each arc is one record
+// (weight bits << 32 | new neighbour id), as the solver reads them
 // arc_off: per-block exclusive offsets of the arc scan, arc_bsum: its scanned block sums (added here,
 // as the scan's last pass), total: the arc count (the subgraph's closing row offset)
-__global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32_t* w, const uint32_t* sel,
-                           const uint32_t* newid, const uint32_t* arc_off, const uint32_t* arc_bsum,
-                           const uint32_t* total, const uint32_t* old_of, uint32_t ns, uint32_t* srow, int32_t* scol,
-                           uint32_t* sw) {
+__global__ void k_sub_fill(const uint32_t* row, const uint64_t* arc, const uint32_t* sel, const uint32_t* newid,
+                           const uint32_t* arc_off, const uint32_t* arc_bsum, const uint32_t* total,
+                           const uint32_t* old_of, uint32_t ns, uint32_t* srow, uint64_t* sarc) {
     const uint32_t lane = __lane_id();
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * blockDim.x / 64;
@@ -762,35 +762,18 @@ __global__ void k_sub_fill(const uint32_t* row, const int32_t* col, const uint32
         const uint32_t v = old_of[nv];
         uint32_t pos = arc_off[v] + arc_bsum[v / 1024];
         if (lane == 0) srow[nv] = pos;
-        for (uint32_t e0 = row[v]; e0 < row[v + 1]; e0 += 64) {
+        const uint32_t re = row[v + 1];
+        for (uint32_t e0 = row[v]; e0 < re; e0 += 64) {
             const uint32_t e = e0 + lane;
-            const bool ok = e < row[v + 1] && sel[col[e]];
+            const uint64_t r = e < re ? arc[e] : 0ull;
+            const uint32_t x = (uint32_t)r;
+            const bool ok = e < re && sel[x];
             const uint64_t bal = __ballot(ok);
             if (ok) {
                 const uint32_t p = pos + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-                scol[p] = (int32_t)newid[col[e]];
-                sw[p] = w[e];
+                sarc[p] = (r & 0xFFFFFFFF00000000ull) | newid[x];
             }
             pos += (uint32_t)__popcll(bal);
-        }
-    }
-}
-
-__global__ void k_sub_rev(const uint32_t* srow, const int32_t* scol, const uint32_t* sw, uint32_t ns, uint32_t* srev) {
-    const uint32_t lane = __lane_id();
-    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const uint32_t nwaves = gridDim.x * blockDim.x / 64;
-    for (uint32_t v = gwave; v < ns; v += nwaves) {
-        for (uint32_t e = srow[v] + lane; e < srow[v + 1]; e += 64) {
-            // rows are ordered by (weight, neighbour): the reverse arc has the same weight
-            const uint32_t u = (uint32_t)scol[e];
-            const uint64_t key = ((uint64_t)sw[e] << 32) | v;
-            uint32_t lo = srow[u], hi = srow[u + 1];
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) / 2;
-                if ((((uint64_t)sw[mid] << 32) | (uint32_t)scol[mid]) < key) lo = mid + 1; else hi = mid;
-            }
-            srev[e] = lo;
         }
     }
 }
@@ -898,15 +881,13 @@ void sel_free(SelSlot& sl) {
 // device CSR of one induced subgraph (reused across queries and calls while large enough)
 struct SubBuf {
     uint32_t* row = nullptr;
-    int32_t* col = nullptr;
-    uint32_t* w = nullptr;
-    uint32_t* rev = nullptr;
+    uint64_t* arc = nullptr;   // arc records (weight bits << 32 | neighbour)
     uint32_t cap_n = 0;
     uint64_t cap_m = 0;
 };
 
 static void sub_free(SubBuf& b) {
-    hipFree(b.row); hipFree(b.col); hipFree(b.w); hipFree(b.rev);
+    hipFree(b.row); hipFree(b.arc);
     b = SubBuf();
 }
 
@@ -918,8 +899,7 @@ static bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms, uint32_t floor_n = 
     sub_free(b);
     const uint32_t cn = std::max<uint32_t>(ns + 1 + ns / 4 + 64, floor_n);
     const uint64_t cm = std::max<uint64_t>((uint64_t)std::max(ms, 1u) + ms / 4 + 256, floor_m);
-    if (hipMalloc(&b.row, cn * 4ull) != hipSuccess || hipMalloc(&b.col, cm * 4) != hipSuccess ||
-        hipMalloc(&b.w, cm * 4) != hipSuccess || hipMalloc(&b.rev, cm * 4) != hipSuccess) {
+    if (hipMalloc(&b.row, cn * 4ull) != hipSuccess || hipMalloc(&b.arc, cm * 8) != hipSuccess) {
         sub_free(b);
         return false;
     }
@@ -1335,7 +1315,8 @@ gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& 
     S.n = (int32_t)ns;
     S.m = ms;
     S.w_type = G->w_type;
-    S.d_row = buf.row; S.d_col = buf.col; S.d_w = buf.w; S.d_rev = buf.rev;
+    // the solver reads rows and arc records only (the network never runs on a subgraph)
+    S.d_row = buf.row; S.d_arc = buf.arc; S.d_col = nullptr; S.d_w = nullptr;
     S.max_deg = 0;
     if (ns > 0) {
         uint32_t* d_sel = sl.scr + 2048 + 8;
@@ -1344,9 +1325,8 @@ gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& 
         uint32_t* d_old = d_arcoff + n;
         uint32_t* d_bsum = d_old + n;
         k_sub_fill<<<gridn((uint64_t)ns * 64, 256, ctx->cu_count * 8), 256, 0, st>>>(
-            G->d_row, G->d_col, G->d_w, d_sel, d_newid, d_arcoff, d_bsum + nbk + 2, d_bsum + nbk + 2 + nbk, d_old, ns,
-            S.d_row, S.d_col, S.d_w);
-        k_sub_rev<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(S.d_row, S.d_col, S.d_w, ns, S.d_rev);
+            G->d_row, G->d_arc, d_sel, d_newid, d_arcoff, d_bsum + nbk + 2, d_bsum + nbk + 2 + nbk, d_old, ns,
+            S.d_row, S.d_arc);
     }
     HIPCHK(hipEventRecord(sl.e_built, st));
     HIPCHK(hipGetLastError());

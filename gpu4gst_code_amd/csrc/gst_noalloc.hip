@@ -292,7 +292,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         ga.Tk = B.T + lvl_base[k];
         ga.T1 = B.T + lvl_base[1];
         ga.masks = B.masks + L.off[k];
-        ga.row = G->d_row; ga.col = G->d_col; ga.w = G->d_w; ga.rev = G->d_rev;
+        ga.row = G->d_row; ga.arc = (const uint2*)G->d_arc;
         for (int i = 0; i < 3; ++i) { ga.list[i] = B.lists[i]; ga.chg[i] = B.chg[i]; ga.inq[i] = B.inq[i]; }
         ga.ctl = B.ctl + k; ga.stats = B.stats;
         ga.m = (uint32_t)L.m[k]; ga.W = (uint32_t)((L.m[k] + 63) / 64); ga.g = (uint32_t)g;
@@ -428,7 +428,8 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     if (opts.want_tree) {
         TraceArgs ta;
         std::memset(&ta, 0, sizeof(ta));
-        ta.T = B.T; ta.pos_of = B.pos_of; ta.col = G->d_col; ta.w = G->d_w; ta.root = B.root;
+        ta.T = B.T; ta.pos_of = B.pos_of; ta.row = G->d_row; ta.arc = (const uint2*)G->d_arc; ta.root = B.root;
+        ta.flt = flt ? 1u : 0u;
         ta.out_u = B.edges; ta.out_v = B.edges + 1; ta.out_w = B.edges + 2; ta.ostride = 3;
         ta.out_n = B.edges + 3 * (size_t)ecap;
         ta.ctl = B.ctl;
@@ -458,7 +459,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     HIPCHK(hipStreamSynchronize(s));
     gst_stats& S = res->stats;
     S.grow_items = H->stats[ST_ITEMS]; S.grow_arcs = H->stats[ST_ARCS]; S.grow_relax = H->stats[ST_RELAX];
-    S.grow_updates = H->stats[ST_UPD]; S.grow_enqueued = H->stats[ST_ENQ];
+    S.grow_updates = H->stats[ST_UPD]; S.grow_enqueued = H->stats[ST_ENQ]; S.grow_new_states = H->stats[ST_NEW];
     S.merge_vertices = H->stats[ST_MVERT]; S.merge_pairs = H->stats[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.grow_bytes = S.grow_items * 24ull + S.grow_arcs * 12ull + S.grow_relax * 8ull + S.grow_updates * 16ull;

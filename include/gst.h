@@ -104,8 +104,9 @@ typedef struct {
     uint64_t grow_enqueued;    /* work items enqueued */
     uint64_t merge_vertices;   /* vertex rows merged */
     uint64_t merge_pairs;      /* (submask, complement) pairs evaluated */
-    uint64_t grow_bytes;       /* algorithmic bytes of the grow kernel (model in DESIGN) */
-    uint64_t merge_bytes;      /* algorithmic bytes of the merge kernel */
+    uint64_t grow_bytes;       /* algorithmic bytes of the grow kernel: 24 per item, 12 per arc, 8 per
+                                  relaxation, 16 per update */
+    uint64_t merge_bytes;      /* algorithmic bytes of the merge kernel: rows read, words written, flags */
     uint32_t grow_rounds;
     uint32_t grow_launches;
     uint32_t merge_launches;
@@ -114,6 +115,8 @@ typedef struct {
     double   t_grow_ms;        /* device time of grow kernels (HIP events) */
     double   t_merge_ms;       /* device time of merge kernels (HIP events) */
     double   t_trace_ms;       /* device time of the traceback */
+    uint64_t grow_new_states;  /* grow updates that made an infinite state finite (the others lowered a finite
+                                  cost again: grow_updates / grow_new_states is the rounds' redundancy) */
 } gst_stats;
 
 typedef struct {
