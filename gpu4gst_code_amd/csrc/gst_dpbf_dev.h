@@ -480,6 +480,15 @@ __global__ void __launch_bounds__(256) k_tour_table(const uint64_t* T1, const in
     tour_table_body<FLT, PW_LDS>(T1, grp, grp + g + 1, g, lbcap, Pw, tour);
 }
 
+// push rounds: 1 = level 1 relaxes its changed masks without re-checking them against the bound
+#ifndef GST_LEVEL1_NOFILTER
+#define GST_LEVEL1_NOFILTER 1
+#endif
+// push rounds: 1 = read a neighbour's change word before the atomic OR that sets its bit (skips the
+// atomic when the bit is set already, at the cost of one more dependent load when it is not)
+#ifndef GST_CHG_PREREAD
+#define GST_CHG_PREREAD 0
+#endif
 #ifndef GST_GROW_UNR
 #define GST_GROW_UNR 4
 #endif
@@ -825,7 +834,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
                     if (first) {
                         enq_v = u;
-                        enq_n = num_chunks(a.row, u, a.chunk);
+                        enq_n = (re - rs + a.chunk - 1) / a.chunk;
                     }
                 }
             }
@@ -854,8 +863,10 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
         st_items += wave_sum32(has && pl == 0 ? 1u : 0u);
         for (uint32_t wd = 0; wd < W; ++wd) {
             uint64_t bm = has ? ccur[(uint64_t)v * W + wd] : 0ull;
-            // drop masks whose cost (plus lower bound) reaches the bound
-            if (a.ub != GST_INF_COST) {
+            // drop masks whose cost (plus lower bound) reaches the bound.  Level 1 skips the check:
+            // its bits are set only by its own rounds (and the members' leaves), for costs <= gmax,
+            // which only fall, and its bound does not change while it grows (no lower bound there)
+            if (a.ub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1)) {
                 const uint32_t na0 = (uint32_t)__popcll(bm);
                 uint64_t keep = 0;
                 for (uint32_t t0 = pl; t0 < na0; t0 += PL) {
@@ -887,13 +898,15 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j) {
                         const uint32_t e = e0 + (s + j) * A + s_arc;
-                        const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-                        const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
+                        // the arc load waits only for the item, not for the own cost (both in flight)
+                        const bool in_row = lane_ok && e < e1;
+                        const bool ok = in_row && cv != GST_INF_COST;
+                        const uint2 r = in_row ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
                         wt[j] = r.y;
                         // under a bound, cv + w >= ub holds for every heavier arc of the row as well
                         live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
                         uu[j] = live[j] ? r.x : 0xFFFFFFFFu;
-                        st_arcs += (ok && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
+                        st_arcs += (in_row && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
                     }
                     if (__ballot(live[0]) == 0) break;
 #pragma unroll
@@ -930,14 +943,16 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                 if (improved) st_upd++;
                                 if (improved && cand <= a.gmax) {
                                     unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
-                                    if (!(*cwp & bitv)) {
+                                    // u's row bounds are read beside the atomic OR, not after it
+                                    const uint32_t ru0 = a.row[u], ru1 = a.row[u + 1];
+                                    if (!GST_CHG_PREREAD || !(*cwp & bitv)) {
                                         const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
                                         // first change of u in this round: queue it (one word per
                                         // vertex: the word itself was empty; several words: flag)
                                         const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
                                         if (first) {
                                             enq_v = u;
-                                            enq_n = num_chunks(a.row, u, a.chunk);
+                                            enq_n = (ru1 - ru0 + a.chunk - 1) / a.chunk;
                                         }
                                     }
                                 }
@@ -1090,12 +1105,13 @@ __device__ void async_relax(const GrowArgs& a, uint32_t n0, uint32_t v, uint32_t
 #pragma unroll
         for (uint32_t j = 0; j < UNR; ++j) {
             const uint32_t e = e0 + (s + j) * A + s_arc;
-            const bool ok = lane_ok && e < e1 && cv != GST_INF_COST;
-            const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
+            const bool in_row = lane_ok && e < e1;
+            const bool ok = in_row && cv != GST_INF_COST;
+            const uint2 r = in_row ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
             wt[j] = r.y;
             live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
             uu[j] = live[j] ? r.x : 0xFFFFFFFFu;
-            st_arcs += (ok && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
+            st_arcs += (in_row && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
         }
         if (__ballot(live[0]) == 0) break;   // arcs run in increasing weight
 #pragma unroll
