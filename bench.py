@@ -382,6 +382,8 @@ def other_workload(args, gst):
         log(f"plain {nq / t_p:.3f} q/s")
         t = time.perf_counter(); spider = [solver.solve(dg, q, auto_bound=True) for q in qs]; t_s = time.perf_counter() - t
         t = time.perf_counter(); lone = [solver.solve_learned(dg, q, model, frac) for q in qs]; t_l1 = time.perf_counter() - t
+        solver.solve_batch(dg, qs, auto_bound=True, slots=nq)
+        t = time.perf_counter(); sb, _ = solver.solve_batch(dg, qs, auto_bound=True, slots=nq); t_sb = time.perf_counter() - t
         # one slot per query (a query has one job in flight at a time); the engine's slots, captured
         # steps and subgraph buffers are set up by a warm-up call outside the timed region
         solver.solve_batch_learned(dg, qs, model, frac, slots=nq)
@@ -391,7 +393,7 @@ def other_workload(args, gst):
         rel = 1e-6 if dtype == "f32" else 0.0
         for i, q in enumerate(qs):
             c = plain[i].cost
-            for r in (spider[i], lone[i], lb[i]):
+            for r in (spider[i], lone[i], lb[i], sb[i]):
                 assert abs(r.cost - c) <= rel * c, (i, r.cost, c)
             tc = gst.validate_tree(G, q, plain[i].eu, plain[i].ev, plain[i].root)
             assert abs(tc - c) <= 1e-6 * max(c, 1.0), (i, tc, c)
@@ -403,10 +405,17 @@ def other_workload(args, gst):
                     "config": {"workload": f"R-MAT V={G.n} arcs={G.m} g={g} sizes {smin}-{smax}, {nq} queries",
                                "mode": f"learned-UB batched (top {frac:.0%}), {nq} slots"},
                     "plain_dpbf": {"value": round(nq / t_p, 4), "ms_per_query": round(t_p / nq * 1e3, 2)},
-                    "spider_bound": {"value": round(nq / t_s, 4)},
+                    "spider_bound": {"value": round(nq / t_s, 4), "mode": "one query at a time"},
+                    "spider_bound_batched": {"value": round(nq / t_sb, 4), "mode": f"{nq} slots"},
+                    "learned_over_spider_batched": round(t_sb / t_lb, 3),
                     "learned_one_at_a_time": {"value": round(nq / t_l1, 4)},
                     "learned": {"fallback_rate": round(sum(r.fell_back for r in lb) / nq, 4),
-                                "ub_over_opt_mean": round(float(np.mean(ratios)), 5) if ratios else None},
+                                "ub_over_opt_mean": round(float(np.mean(ratios)), 5) if ratios else None,
+                                "one_at_a_time_ms": {k: round(float(np.mean([r.stats[k] for r in lone])), 3) for k in
+                                                     ("t_score_select_ms", "t_sub_solve_ms", "t_full_solve_ms")},
+                                "sub_vertices_mean": int(np.mean([r.stats["sub_vertices"] for r in lone])),
+                                "sub_arcs_mean": int(np.mean([r.stats["sub_arcs"] for r in lone]))},
+                    "spider_ms_per_query": round(t_s / nq * 1e3, 2),
                     "speedup_vs_plain": round(t_p / t_lb, 3),
                     "roofline": {"bound": "hbm", "kernel": "k_grow_round (plain)", "achieved": round(gbs(gb, gms), 2),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(gb, gms) / HBM_PEAK_GBS, 5),

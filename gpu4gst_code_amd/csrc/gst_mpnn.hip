@@ -27,6 +27,13 @@
 #include "gst_batch.h"
 
 // selected queries kept ready ahead of the batched engine
+// A subgraph holding more than this many per mille of the graph's arcs is not solved: its DPBF
+// would cost as much as the bounded full solve it is meant to speed up (on R-MAT graphs the top-scored
+// vertices are the hubs, and their induced subgraph holds most of the arcs).  Two such subgraphs on
+// one graph and the model stops selecting on it: its queries run with the spider bound alone.
+#ifndef GST_SUB_ARC_PERMILLE
+#define GST_SUB_ARC_PERMILLE 250
+#endif
 #ifndef GST_SELECT_AHEAD
 #define GST_SELECT_AHEAD 8
 #endif
@@ -944,12 +951,32 @@ struct gst_model {
     uint32_t* hub_list = nullptr;   // hub row ids
     uint32_t* hub_items = nullptr;  // pairs (row, h | chunk << 20)
     uint32_t n_hub = 0, n_hub_items = 0;
+    // graphs whose selections gave oversized subgraphs: (graph uid, how many); set and read by one
+    // host thread at a time (the caller's, or a batched call's selection thread)
+    std::vector<std::pair<uint64_t, int>> oversized;
 };
 
 namespace {
 
 hipStream_t mstream(const gst_model* M) {
     return (hipStream_t)(M->sel_stream ? M->sel_stream : M->ctx->stream);
+}
+
+// a selection of ms arcs on G: is the subgraph too large to solve?  (counted per graph)
+bool sub_oversized(gst_model* M, const gst_graph* G, uint64_t ms) {
+    if (ms * 1000ull <= (uint64_t)G->m * GST_SUB_ARC_PERMILLE) return false;
+    for (auto& p : M->oversized)
+        if (p.first == G->uid) { p.second++; return true; }
+    if (M->oversized.size() >= 64) M->oversized.erase(M->oversized.begin());
+    M->oversized.push_back({G->uid, 1});
+    return true;
+}
+
+// has the model stopped selecting on G (two oversized subgraphs)?
+bool sub_declined(const gst_model* M, const gst_graph* G) {
+    for (auto& p : M->oversized)
+        if (p.first == G->uid) return p.second >= 2;
+    return false;
 }
 
 template <int D, bool BWD>
@@ -1337,10 +1364,17 @@ gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& 
 // copied too (the one-query solver reads them)
 gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SubBuf& buf,
                             gst_graph& S, std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool want_host_rows) {
+    S.n = 0;   // no subgraph unless one is built below
+    if (sub_declined(M, G)) return GST_OK;
     gst_status sst = select_begin(M, G, groups, frac, M->sync_sel);
     if (sst != GST_OK) return sst;
     gst_ctx* ctx = M->ctx;
     HIPCHK(hipEventSynchronize(M->sync_sel.e_sizes));
+    if (sub_oversized(M, G, M->sync_sel.h_rb[1])) {
+        L.sub_vertices = M->sync_sel.h_rb[0];
+        L.sub_arcs = M->sync_sel.h_rb[1];
+        return GST_OK;
+    }
     sst = select_finish(M, G, M->sync_sel, buf, S, sub_vtx, L);
     if (sst != GST_OK) return sst;
     HIPCHK(hipEventSynchronize(M->sync_sel.e_built));
@@ -1395,6 +1429,7 @@ gst_status gst_learned_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups*
     if (opts_in) fo = *opts_in;
     else { fo.prune_lb = 1; fo.want_tree = 1; }
     if (have_ub) fo.ub = ub;
+    fo.auto_bound = 1;   // the spider bound tightens the subgraph bound where it is lower (as in the batch)
     gst_status fs = gst_dpbf_solve(ctx, G, groups, &fo, res);
     if (fs == GST_INFEASIBLE && have_ub) {
         // the bound cannot be below the optimum; reaching here means float rounding in the bound:
@@ -1501,6 +1536,14 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             inflight.pop_front();
             Q& x = qs[(size_t)e.q];
             HIPCHK(hipEventSynchronize(e.sl->e_sizes));
+            if (sub_oversized(M, G, e.sl->h_rb[1])) {
+                // too large to pay for itself: no subgraph, the spider-bounded full solve runs
+                x.L.sub_vertices = e.sl->h_rb[0];
+                x.L.sub_arcs = e.sl->h_rb[1];
+                idle_slots.push_back(e.sl);
+                publish(nosub_q, e.q);
+                return GST_OK;
+            }
             int bi = -1;
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -1547,6 +1590,10 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
                 }
                 if (begin) {
                     const int32_t q = next++;
+                    if (sub_declined(M, G)) {   // the model no longer selects on this graph
+                        publish(nosub_q, q);
+                        continue;
+                    }
                     qs[(size_t)q].t_sel = now();
                     SelSlot* sl = idle_slots.back();
                     idle_slots.pop_back();

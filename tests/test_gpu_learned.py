@@ -275,11 +275,12 @@ def test_trained_network_gain(solver):
 @pytest.mark.parametrize("frac", [0.01, 0.05, 0.3, 1.0])
 def test_topk_selection_size(solver, frac):
     """The device radix select keeps exactly ceil(frac n) top-scored vertices: the subgraph holds
-    them plus the group members outside them (checked where no score ties the threshold)."""
+    them plus the group members outside them (checked where no score ties the threshold).  A model
+    per query: large fractions give oversized subgraphs, which a model stops selecting after two."""
     G = gst.gen_barabasi_albert(30000, 4, 1, 500, seed=19)
     dg = solver.upload(G)
-    model = gst.Model(solver, params=_params(4))
     for q in range(3):
+        model = gst.Model(solver, params=_params(4))
         groups = gst.gen_groups(G.n, 5, 5, 60, seed=600 + q)
         sc = model.scores(dg, groups)
         k = min(G.n, int(np.ceil(frac * G.n)))
@@ -287,6 +288,7 @@ def test_topk_selection_size(solver, frac):
         thr = sc[order[k - 1]]
         members = np.unique(np.concatenate([groups.members(i) for i in range(groups.g)]))
         r = solver.solve_learned(dg, groups, model, frac)
+        model.free()
         if k < G.n and np.sum(sc == thr) > 1:
             continue   # ties at the threshold: any of them may be kept
         top = np.zeros(G.n, bool)
@@ -294,5 +296,38 @@ def test_topk_selection_size(solver, frac):
         want = k + int(np.sum(~top[members]))
         print(f"frac {frac} q{q}: sub_vertices {r.stats['sub_vertices']} want {want}", flush=True)
         assert r.stats["sub_vertices"] == want
+    dg.free()
+
+
+def test_oversized_subgraphs_are_declined(solver):
+    """A selection whose induced subgraph holds over a quarter of the graph's arcs is not solved (its
+    DPBF would cost as much as the full solve); after two such selections on one graph the model stops
+    selecting there.  Every answer stays exact, and such queries report the learned bound unused."""
+    G = gst.gen_rmat(12, 8, weights="int", wlo=1, whi=100, seed=12)
+    dg = solver.upload(G)
+    model = gst.Model(solver, seed=5)
+    qs = []
+    k = 0
+    while len(qs) < 5:
+        q = gst.gen_groups(G.n, 5, 5, 30, seed=300 + k)
+        k += 1
+        try:
+            solver.solve(dg, q, want_tree=False)
+            qs.append(q)
+        except gst.GstError:
+            pass
+    for q in qs[:3]:   # every vertex selected: the subgraph is the graph
+        r = solver.solve_learned(dg, q, model, 1.0)
+        p = solver.solve(dg, q)
+        print(f"one query: cost {r.cost} plain {p.cost} sub arcs {r.stats['sub_arcs']} of {G.m} fell_back {r.fell_back}",
+              flush=True)
+        assert r.cost == p.cost and r.fell_back
+        assert gst.validate_tree(G, q, r.eu, r.ev, r.root) == p.cost
+    res, _ = solver.solve_batch_learned(dg, qs, model, 1.0, slots=3)
+    for q, r in zip(qs, res):
+        assert r.cost == solver.solve(dg, q, want_tree=False).cost and r.fell_back
+    # a small fraction on the same graph is declined too now (the graph is marked), still exact
+    r = solver.solve_learned(dg, qs[0], model, 0.01)
+    assert r.fell_back and r.stats["sub_vertices"] == 0 and r.cost == solver.solve(dg, qs[0]).cost
     model.free()
     dg.free()
