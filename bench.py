@@ -125,17 +125,19 @@ def cpu_baselines(G, sample, plain_costs, threads):
                                   f"at every pop, exact early exit), {nth} threads, {bwall:.1f} s wall"}}
 
 
-def grow_traffic(bytes_per_launch):
-    """HBM bytes per k_batch_grow dispatch of the kernel pass, measured by rocprofv3 --pmc
-    (FETCH_SIZE and WRITE_SIZE runs of this script, calibrated) and committed under profiles/; given
-    only when the profiled pass's algorithmic bytes per dispatch are within 10% of this run's."""
+def grow_traffic(pass_bytes, launches):
+    """HBM bytes of the kernel pass's batched grow, measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE
+    runs of this script with --engine eager, calibrated against a copy of known size) and committed
+    under profiles/: the pass's total traffic over this run's launch count, given only when the
+    profiled pass did the same algorithmic work as this one (within 10%; its launch count may differ by
+    a step, as rounds race).  Returns (traffic per launch, traffic / algorithmic, profiled pass bytes)."""
     if not os.path.exists(TRAFFIC_JSON):
-        return None, None
-    t = json.load(open(TRAFFIC_JSON))
-    a = t.get("algorithmic_bytes_per_dispatch")
-    if not a or abs(bytes_per_launch / a - 1.0) > 0.10:
-        return None, a
-    return int(t["traffic_bytes_per_dispatch"]), a
+        return None, None, None
+    t = json.load(open(TRAFFIC_JSON)).get("k_batch_grow", {})
+    a = t.get("pass_algorithmic_bytes")
+    if not a or abs(pass_bytes / a - 1.0) > 0.10:
+        return None, None, a
+    return int(t["pass_traffic_bytes"] / launches), t["pass_traffic_bytes"] / a, a
 
 
 def main():
@@ -227,7 +229,7 @@ def main():
     b_grow_gbs = gbs(kst["grow_bytes"], kst["t_grow_ms"])
     b_merge_gbs = gbs(kst["merge_bytes"], kst["t_merge_ms"])
     bpl = kst["grow_bytes"] / glaunch
-    traffic, prof_bytes = grow_traffic(bpl)
+    traffic, t_ratio, prof_bytes = grow_traffic(kst["grow_bytes"], glaunch)
 
     p_grow_ms = sum(r.stats["t_grow_ms"] for r in plain)
     p_merge_ms = sum(r.stats["t_merge_ms"] for r in plain)
@@ -281,10 +283,10 @@ def main():
                       "kernel": f"gst_calib_copy_kernel, {args.copy_mib} MiB read + written, 10 copies"},
         "roofline": {"bound": "hbm", "achieved": round(b_grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(b_grow_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "traffic_over_algorithmic": round(traffic / bpl, 2) if traffic else None,
+                     "traffic_over_algorithmic": round(t_ratio, 2) if t_ratio else None,
                      "frac_of_copy_peak": round(b_grow_gbs / copy_gbs, 5) if copy_gbs else None,
                      "kernel": "k_batch_grow", "launches": glaunch, "bytes_per_launch": round(bpl, 1),
-                     "profiled_bytes_per_launch": prof_bytes,
+                     "pass_bytes": int(kst["grow_bytes"]), "profiled_pass_bytes": prof_bytes,
                      "ms_per_launch": round(kst["t_grow_ms"] / glaunch, 5),
                      "pass": "eager kernel pass: the 64 full solves bounded as in the learned run, 64 slots, all "
                              "in slots from the start, built-in engine knobs, HIP events around every launch on "
