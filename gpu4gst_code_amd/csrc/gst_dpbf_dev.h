@@ -1593,14 +1593,14 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
         if (!a.is_full) {
             for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
                 const uint32_t v = v0 + b;
-                if (a.row[v + 1] == a.row[v]) continue;
                 unsigned long long any = 0;
+                for (uint32_t w = 0; w < W; ++w) any |= bm[b * W + w];
+                if (!any || a.row[v + 1] == a.row[v]) continue;   // nothing to grow, or no arcs
                 for (uint32_t w = 0; w < W; ++w) {
                     const unsigned long long x = bm[b * W + w];
-                    any |= x;
                     if (x) a.chg0[(uint64_t)v * W + w] = x;
                 }
-                if (any) a.inq0[v] = 1;
+                a.inq0[v] = 1;
             }
         }
     }
@@ -1769,7 +1769,16 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a) {
                 const uint32_t u = pred;   // the neighbour this state grew from
                 const uint64_t wu = a.T[a.lvl_base[k] + (uint64_t)u * a.m[k] + rk];
                 bool found = false;
-                ew = trace_arc_weight(a, v, u, (uint32_t)(wv >> 32), (uint32_t)(wu >> 32), found);
+                if (a.flt) {
+                    ew = trace_arc_weight(a, v, u, (uint32_t)(wv >> 32), (uint32_t)(wu >> 32), found);
+                } else {
+                    // ints: the grow set cost(v) = cost(u) + w(v, u) exactly, and a later drop of
+                    // cost(u) (<= gmax, so grown again) would have lowered cost(v) or moved its
+                    // predecessor: the weight is the difference
+                    const uint32_t cv = (uint32_t)(wv >> 32), cu = (uint32_t)(wu >> 32);
+                    found = cu <= cv;
+                    ew = cv - cu;
+                }
                 if (!found || wu == GST_INF_WORD) fail_s = 1;
                 eu = v; ev = u;
                 emit = found;
