@@ -76,7 +76,8 @@ class gst_batch_stats(C.Structure):
                 ("grow_relax", C.c_uint64), ("grow_bytes", C.c_uint64), ("merge_bytes", C.c_uint64),
                 ("queries", C.c_int64), ("slots", C.c_int32), ("rounds_per_step", C.c_int32),
                 ("used_graph", C.c_int32), ("timed_kernels", C.c_int32), ("t_grow_ms", C.c_double),
-                ("t_merge_ms", C.c_double), ("merge_launches", C.c_uint64), ("reserved", C.c_int32 * 4)]
+                ("t_merge_ms", C.c_double), ("merge_launches", C.c_uint64), ("busy_slot_steps", C.c_uint64),
+                ("reserved", C.c_int32 * 2)]
 
 
 class gst_tune_report(C.Structure):

@@ -934,7 +934,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
     k_reset_phases<<<1, MAX_SLOTS, 0, s>>>(E->d_sb);
 
     std::vector<double> lat;
-    uint64_t steps = 0, n_done = 0, relax = 0, gbytes = 0, mbytes = 0;
+    uint64_t steps = 0, n_done = 0, relax = 0, gbytes = 0, mbytes = 0, busy_steps = 0;
     gst_status first_err = GST_OK;
     bool flt_set = false, flt = false;
     int busy = 0;
@@ -1035,6 +1035,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
         else enqueue_step_b(ctx, ba, R, flt, te);
         HIPCHK(hipGetLastError());
         steps++;
+        busy_steps += (uint64_t)busy;
         HIPCHK(hipEventSynchronize(E->a_done));
         if (timed && steps >= 2) {
             st = add_times(steps - 2);
@@ -1099,7 +1100,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
             S2.grow_updates = hs[ST_UPD]; S2.grow_enqueued = hs[ST_ENQ]; S2.grow_new_states = hs[ST_NEW];
             S2.merge_vertices = hs[ST_MVERT]; S2.merge_pairs = hs[ST_MPAIRS];
             S2.grow_rounds = rounds;
-            S2.grow_bytes = S2.grow_items * 24ull + S2.grow_arcs * 12ull + S2.grow_relax * 8ull + S2.grow_updates * 16ull;
+            S2.grow_bytes = gst_grow_bytes(S2);
             S2.merge_bytes = hs[ST_MBYTES];
             relax += S2.grow_relax; gbytes += S2.grow_bytes; mbytes += S2.merge_bytes;
             if (err & 1u) {
@@ -1177,6 +1178,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
         bst->t_grow_ms = t_grow;
         bst->t_merge_ms = t_merge;
         bst->merge_launches = n_merge_launch;
+        bst->busy_slot_steps = busy_steps;
         bst->queries = (int64_t)n_done;
     }
     completed = true;

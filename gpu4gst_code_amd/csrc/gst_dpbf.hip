@@ -130,8 +130,10 @@ void gst_ctx_device_fini(gst_ctx* ctx) {
     gst_query_bufs_free(ctx);
     if (ctx->stream) hipStreamDestroy((hipStream_t)ctx->stream);
     if (ctx->stream2) hipStreamDestroy((hipStream_t)ctx->stream2);
+    if (ctx->stream3) hipStreamDestroy((hipStream_t)ctx->stream3);
     ctx->stream = nullptr;
     ctx->stream2 = nullptr;
+    ctx->stream3 = nullptr;
 }
 
 gst_status gst_graph_device_upload(gst_ctx* ctx, gst_graph* g) {
@@ -762,9 +764,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         else S.t_trace_ms += ms;
     }
     cleanup_evs();
-    // algorithmic byte model: see DESIGN (grow: item 8 + row 8 + bitmap 8W + own costs 8/relax-lane-group
-    // approximated by 12 B per scanned arc + 8 B read + 16 B RMW per update; merge: 8 B per word in/out)
-    S.grow_bytes = S.grow_items * (8ull + 8ull + 8ull) + S.grow_arcs * 12ull + S.grow_relax * 8ull + S.grow_updates * 16ull;
+    S.grow_bytes = gst_grow_bytes(S);
     S.merge_bytes = hs[ST_MBYTES];
     S.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return GST_OK;

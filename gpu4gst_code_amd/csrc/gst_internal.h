@@ -62,6 +62,16 @@ struct gst_graph {
     int32_t max_deg = 0;
 };
 
+// Algorithmic bytes of the grow kernels, from their counters: a work item reads its id, its row
+// bounds and its change word (24 B) and was written once when enqueued (8 B); a scanned arc is one
+// 8-byte record (neighbour, weight); a relaxation reads the neighbour's 8-byte state; an update
+// writes that state and ORs the neighbour's change word (16 B).  The own costs an item reads
+// (8 B per changed mask) are not counted, so the figure is a lower bound of the kernel's needs.
+inline uint64_t gst_grow_bytes(const gst_stats& s) {
+    return s.grow_items * 24ull + s.grow_enqueued * 8ull + s.grow_arcs * 8ull + s.grow_relax * 8ull +
+           s.grow_updates * 16ull;
+}
+
 // Rows of an uploaded graph are ordered by (weight bits, neighbour); weights are non-negative
 // int32 or non-negative float32 (never -0.0f), so the bit patterns order like the weights.
 inline uint64_t arc_key(uint32_t w, int32_t col) { return ((uint64_t)w << 32) | (uint32_t)col; }
@@ -133,6 +143,7 @@ struct gst_ctx {
     std::mutex err_mu;          // last_error is set from the learned batch's selection thread too
     void* stream = nullptr;     // hipStream_t
     void* stream2 = nullptr;    // hipStream_t (default priority): learned-bound selection beside the engine
+    void* stream3 = nullptr;    // hipStream_t (default priority): a second selection stream
     gst_query_bufs* qb = nullptr;
     struct gst_batch_engine* be = nullptr;   // batched engine (gst_batch.hip), made on first use
     struct gst_noalloc_host* na = nullptr;   // no-allocation solve (gst_noalloc.hip): made at set-up

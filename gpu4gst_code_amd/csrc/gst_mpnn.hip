@@ -34,6 +34,12 @@
 #ifndef GST_SUB_ARC_PERMILLE
 #define GST_SUB_ARC_PERMILLE 250
 #endif
+#ifndef GST_SELECT_LANES
+#define GST_SELECT_LANES 2   // selection streams of the batched path, each with its own activations
+#endif
+#ifndef GST_SELECT_DEPTH
+#define GST_SELECT_DEPTH 4   // selections queued at once over all selection streams
+#endif
 #ifndef GST_SELECT_AHEAD
 #define GST_SELECT_AHEAD 8
 #endif
@@ -874,6 +880,7 @@ struct SelSlot {
     uint32_t hg_cap = 0;
     hipEvent_t e_sizes = nullptr, e_built = nullptr;
     uint32_t n = 0, nbk = 0, gcnt = 0;
+    hipStream_t st = nullptr;    // stream of the selection in flight (its activation set's)
 };
 
 void sel_free(SelSlot& sl) {
@@ -915,6 +922,20 @@ static bool sub_reserve(SubBuf& b, uint32_t ns, uint32_t ms, uint32_t floor_n = 
     return true;
 }
 
+// Activations of one forward pass.  The batched learned path runs two sets, each on a stream of its
+// own, so that the selections of two queries overlap; training and the one-query path use set 0.
+struct FwdSet {
+    bf16 *x = nullptr, *ax = nullptr, *h1 = nullptr, *ah1 = nullptr;
+    float* s = nullptr;          // scores
+    uint32_t* gm = nullptr;      // group bits per vertex
+    float* gcnt = nullptr;       // per vertex, neighbours in each of groups 0..7 (cleared after reading)
+    float* hub_acc = nullptr;    // hub-row partial sums of an aggregation
+    int32_t* grp = nullptr;      // the query's group lists
+    uint32_t cap_n = 0, cap_grp = 0, cap_hub = 0;
+    hipStream_t st = nullptr;    // nullptr: the model's stream
+};
+constexpr int FWD_SETS = 2;
+
 struct gst_model {
     gst_ctx* ctx = nullptr;
     float* p = nullptr;   // NPARAM fp32 master
@@ -927,18 +948,16 @@ struct gst_model {
     bf16* W2kn = nullptr; // [K2][HID]
     int steps = 0;
     int grad_count = 0;
-    // activations for one graph (grown on demand)
-    uint32_t cap_n = 0, cap_hub = 0;
-    bf16 *x = nullptr, *ax = nullptr, *h1 = nullptr, *ah1 = nullptr, *h2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
-    float *s = nullptr, *dS = nullptr, *dX2 = nullptr, *dH1 = nullptr, *dA1 = nullptr, *dX1 = nullptr, *loss = nullptr;
-    float* hub_acc = nullptr;
+    // forward activations (set 0: training and the one-query path; both: batched selections)
+    FwdSet f[FWD_SETS];
+    // training buffers and the per-graph part of layer 1 (grown on demand)
+    uint32_t cap_n = 0;
+    bf16 *h2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
+    float *dS = nullptr, *dX2 = nullptr, *dH1 = nullptr, *dA1 = nullptr, *loss = nullptr;
     bf16* ax_static = nullptr;   // graph-only part of the layer-1 aggregation (per graph)
-    float* gcnt = nullptr;       // per vertex, neighbours in each of groups 0..7 (per query)
     uint64_t static_uid = 0;     // graph ax_static belongs to
-    uint32_t* gm = nullptr;
+    hipEvent_t e_static = nullptr;   // ax_static written (other streams wait for it)
     uint8_t* y = nullptr;
-    int32_t* grp = nullptr;
-    uint32_t cap_grp = 0;
     // hub rows of the last graph
     uint64_t hub_uid = 0;
     SelSlot sync_sel;                  // the selection of the one-query path
@@ -980,24 +999,42 @@ bool sub_declined(const gst_model* M, const gst_graph* G) {
 }
 
 template <int D, bool BWD>
-void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipStream_t st);
+void aggregate(gst_model* M, const gst_graph* G, float* hub_acc, const void* in, void* out, hipStream_t st);
+
+hipStream_t fstream(const gst_model* M, const FwdSet& F) { return F.st ? F.st : mstream(M); }
+
+// one activation set's buffers for an n-vertex graph with n_hub hub rows (its group counts start clear;
+// every forward pass clears them again after reading)
+gst_status fwd_bufs(gst_model* M, FwdSet& F, uint32_t n) {
+    gst_ctx* ctx = M->ctx;
+    if (F.cap_n < n) {
+        const bool ok = dalloc(F.x, (size_t)n * FIN) && dalloc(F.ax, (size_t)n * FIN) && dalloc(F.h1, (size_t)n * HID) &&
+                        dalloc(F.ah1, (size_t)n * HID) && dalloc(F.s, n) && dalloc(F.gm, n) && dalloc(F.gcnt, (size_t)n * 8);
+        if (!ok) { F.cap_n = 0; return gst_set_error(ctx, GST_OOM, "model: activation allocation failed (n=%u)", n); }
+        F.cap_n = n;
+        const int fgz = std::max(1, ctx->cu_count * 8);
+        k_fill32<<<gridn((uint64_t)n * 8, 256, fgz), 256, 0, fstream(M, F)>>>((uint32_t*)F.gcnt, 0u, (uint64_t)n * 8);
+    }
+    if (F.cap_hub < std::max(M->n_hub, 1u)) {
+        if (!dalloc(F.hub_acc, (size_t)std::max(M->n_hub, 1u) * HID)) {
+            F.cap_hub = 0;
+            return gst_set_error(ctx, GST_OOM, "model: hub allocation failed");
+        }
+        F.cap_hub = std::max(M->n_hub, 1u);
+    }
+    return GST_OK;
+}
 
 gst_status model_bufs(gst_model* M, const gst_graph* G) {
     gst_ctx* ctx = M->ctx;
     const uint32_t n = (uint32_t)G->n;
     if (M->cap_n < n) {
-        bool ok = dalloc(M->x, (size_t)n * FIN) && dalloc(M->ax, (size_t)n * FIN) && dalloc(M->h1, (size_t)n * HID) &&
-                  dalloc(M->ah1, (size_t)n * HID) && dalloc(M->h2, (size_t)n * HID) && dalloc(M->dZ2, (size_t)n * HID) &&
-                  dalloc(M->dZ1, (size_t)n * HID) && dalloc(M->s, n) && dalloc(M->dS, n) &&
-                  dalloc(M->dX2, (size_t)n * K2) && dalloc(M->dH1, (size_t)n * HID) && dalloc(M->dA1, (size_t)n * HID) &&
-                  dalloc(M->gm, n) && dalloc(M->y, n) && dalloc(M->ax_static, (size_t)n * FIN) &&
-                  dalloc(M->gcnt, (size_t)n * 8);
+        bool ok = dalloc(M->h2, (size_t)n * HID) && dalloc(M->dZ2, (size_t)n * HID) && dalloc(M->dZ1, (size_t)n * HID) &&
+                  dalloc(M->dS, n) && dalloc(M->dX2, (size_t)n * K2) && dalloc(M->dH1, (size_t)n * HID) &&
+                  dalloc(M->dA1, (size_t)n * HID) && dalloc(M->y, n) && dalloc(M->ax_static, (size_t)n * FIN);
         M->static_uid = 0;
         if (!ok) { M->cap_n = 0; return gst_set_error(ctx, GST_OOM, "model: activation allocation failed (n=%u)", n); }
         M->cap_n = n;
-        // the group counts start clear; every forward pass clears them again after reading
-        const int fgz = std::max(1, ctx->cu_count * 8);
-        k_fill32<<<gridn((uint64_t)n * 8, 256, fgz), 256, 0, mstream(M)>>>((uint32_t*)M->gcnt, 0u, (uint64_t)n * 8);
     }
     if (M->hub_uid != G->uid) {
         // hub rows (host side, from the graph's host CSR)
@@ -1011,8 +1048,7 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
             }
         }
         if (hl.size() >= (1u << 20)) return gst_set_error(ctx, GST_INVALID_ARG, "model: too many hub rows");
-        if (!dalloc(M->hub_list, hl.size()) || !dalloc(M->hub_items, hi.size()) ||
-            !dalloc(M->hub_acc, std::max<size_t>(hl.size(), 1) * HID))
+        if (!dalloc(M->hub_list, hl.size()) || !dalloc(M->hub_items, hi.size()))
             return gst_set_error(ctx, GST_OOM, "model: hub allocation failed");
         hipStream_t st = mstream(M);
         if (!hl.empty()) {
@@ -1024,13 +1060,19 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
         M->n_hub_items = (uint32_t)(hi.size() / 2);
         M->hub_uid = G->uid;
     }
+    gst_status sst = fwd_bufs(M, M->f[0], n);
+    if (sst != GST_OK) return sst;
     if (M->static_uid != G->uid) {
-        // graph-only layer-1 means: the gather over features with every group bit clear
-        hipStream_t st = mstream(M);
+        // graph-only layer-1 means: the gather over features with every group bit clear (set 0's
+        // buffers as scratch, on its stream; the other set's stream waits for e_static)
+        FwdSet& F = M->f[0];
+        hipStream_t st = fstream(M, F);
         const int fg = std::max(1, ctx->cu_count * 8);
-        k_fill32<<<gridn(n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)n);
-        k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, M->gm, n, M->x);
-        aggregate<FIN, false>(M, G, M->x, M->ax_static, st);
+        k_fill32<<<gridn(n, 256, fg), 256, 0, st>>>(F.gm, 0u, (uint64_t)n);
+        k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, F.gm, n, F.x);
+        aggregate<FIN, false>(M, G, F.hub_acc, F.x, M->ax_static, st);
+        if (!M->e_static) HIPCHK(hipEventCreateWithFlags(&M->e_static, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(M->e_static, st));
         HIPCHK(hipGetLastError());
         M->static_uid = G->uid;
     }
@@ -1039,65 +1081,69 @@ gst_status model_bufs(gst_model* M, const gst_graph* G) {
 
 // pinned (may be nullptr): a staging buffer of at least g + 1 + members words that stays untouched
 // until the stream has run the copy; without it the copy is waited for
-gst_status upload_groups(gst_model* M, const gst_graph* G, const gst_groups* groups, int32_t* pinned = nullptr) {
+gst_status upload_groups(gst_model* M, FwdSet& F, const gst_graph* G, const gst_groups* groups, int32_t* pinned = nullptr) {
     gst_ctx* ctx = M->ctx;
-    hipStream_t st = mstream(M);
+    hipStream_t st = fstream(M, F);
     const int g = groups->n_groups;
     const uint32_t cnt = (uint32_t)(g + 1 + groups->grp_ptr[g]);
-    if (M->cap_grp < cnt) {
+    if (F.cap_grp < cnt) {
         HIPCHK(hipStreamSynchronize(st));   // earlier selections may still read the old buffer
-        if (!dalloc(M->grp, cnt)) return gst_set_error(ctx, GST_OOM, "model: group allocation failed");
-        M->cap_grp = cnt;
+        if (!dalloc(F.grp, cnt)) { F.cap_grp = 0; return gst_set_error(ctx, GST_OOM, "model: group allocation failed"); }
+        F.cap_grp = cnt;
     }
     std::vector<int32_t> hv;
     int32_t* h = pinned;
     if (!h) { hv.resize(cnt); h = hv.data(); }
     for (int i = 0; i <= g; ++i) h[(size_t)i] = groups->grp_ptr[i];
     for (int32_t j = 0; j < groups->grp_ptr[g]; ++j) h[(size_t)(g + 1 + j)] = groups->grp_vtx[j];
-    HIPCHK(hipMemcpyAsync(M->grp, h, cnt * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(F.grp, h, cnt * 4, hipMemcpyHostToDevice, st));
     if (!pinned) HIPCHK(hipStreamSynchronize(st));   // h is a local buffer
     const int fg = std::max(1, ctx->cu_count * 8);
-    k_fill32<<<gridn(G->n, 256, fg), 256, 0, st>>>(M->gm, 0u, (uint64_t)G->n);
+    k_fill32<<<gridn(G->n, 256, fg), 256, 0, st>>>(F.gm, 0u, (uint64_t)G->n);
     k_set_bits_scatter<<<gridn((uint64_t)groups->grp_ptr[g] * 64, 256, fg), 256, 0, st>>>(
-        M->gm, M->grp, M->grp + g + 1, g, G->d_row, G->d_col, M->gcnt);   // features use bits 0..7
+        F.gm, F.grp, F.grp + g + 1, g, G->d_row, G->d_col, F.gcnt);   // features use bits 0..7
     return GST_OK;
 }
 
 template <int D, bool BWD>
-void aggregate(gst_model* M, const gst_graph* G, const void* in, void* out, hipStream_t st) {
+void aggregate(gst_model* M, const gst_graph* G, float* hub_acc, const void* in, void* out, hipStream_t st) {
     const int fg = std::max(1, M->ctx->cu_count * 8);
-    if (M->n_hub) k_fill32<<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>((uint32_t*)M->hub_acc, 0u,
+    if (M->n_hub) k_fill32<<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>((uint32_t*)hub_acc, 0u,
                                                                                (uint64_t)M->n_hub * D);
     if (BWD)
         k_aggregate_bwd<D, BWD><<<std::max(1, M->ctx->cu_count * 16), 256, 0, st>>>(G->d_row, G->d_col, (uint32_t)G->n, in,
-                                                                                  out, M->hub_acc, M->hub_items, M->n_hub_items);
+                                                                                  out, hub_acc, M->hub_items, M->n_hub_items);
     else
         k_aggregate_fwd<D><<<std::max(1, M->ctx->cu_count * 16), 256, 0, st>>>(G->d_row, G->d_col, (uint32_t)G->n,
-                                                                             (const bf16*)in, (bf16*)out, M->hub_acc,
+                                                                             (const bf16*)in, (bf16*)out, hub_acc,
                                                                              M->hub_items, M->n_hub_items);
     if (M->n_hub)
         k_aggregate_hub_finish<D, BWD><<<gridn((uint64_t)M->n_hub * D, 256, fg), 256, 0, st>>>(G->d_row, M->hub_list,
-                                                                                        M->n_hub, M->hub_acc, out);
+                                                                                        M->n_hub, hub_acc, out);
 }
 
-// need_h2: keep the layer-2 activations (the backward pass reads them); the scores alone need not
-gst_status forward(gst_model* M, const gst_graph* G, const gst_groups* groups, int32_t* pinned_groups = nullptr,
-                   bool need_h2 = false) {
+// the forward pass of one query into activation set F, on F's stream.  need_h2: keep the layer-2
+// activations (the backward pass reads them; set 0 only); the scores alone need not
+gst_status forward(gst_model* M, FwdSet& F, const gst_graph* G, const gst_groups* groups,
+                   int32_t* pinned_groups = nullptr, bool need_h2 = false) {
     gst_ctx* ctx = M->ctx;
     gst_status sst = model_bufs(M, G);
     if (sst != GST_OK) return sst;
-    sst = upload_groups(M, G, groups, pinned_groups);
+    sst = fwd_bufs(M, F, (uint32_t)G->n);
     if (sst != GST_OK) return sst;
-    hipStream_t st = mstream(M);
+    hipStream_t st = fstream(M, F);
+    if (st != fstream(M, M->f[0])) HIPCHK(hipStreamWaitEvent(st, M->e_static, 0));
+    sst = upload_groups(M, F, G, groups, pinned_groups);
+    if (sst != GST_OK) return sst;
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
     const int dg = std::max(1, ctx->cu_count * 8);
-    k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, M->gm, n, M->x);
-    k_ax_finish<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, M->ax_static, M->gcnt, n, M->ax);
-    k_dense_fwd<FIN, false><<<dg, 256, 0, st>>>(M->x, M->ax, M->W1t, M->p + OFF_B1, n, M->h1, nullptr, nullptr, nullptr);
-    aggregate<HID, false>(M, G, M->h1, M->ah1, st);
-    k_dense_fwd<HID, true><<<dg, 256, 0, st>>>(M->h1, M->ah1, M->W2t, M->p + OFF_B2, n, need_h2 ? M->h2 : nullptr,
-                                               M->p + OFF_W3, M->p + OFF_B3, M->s);
+    k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, F.gm, n, F.x);
+    k_ax_finish<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, M->ax_static, F.gcnt, n, F.ax);
+    k_dense_fwd<FIN, false><<<dg, 256, 0, st>>>(F.x, F.ax, M->W1t, M->p + OFF_B1, n, F.h1, nullptr, nullptr, nullptr);
+    aggregate<HID, false>(M, G, F.hub_acc, F.h1, F.ah1, st);
+    k_dense_fwd<HID, true><<<dg, 256, 0, st>>>(F.h1, F.ah1, M->W2t, M->p + OFF_B2, n, need_h2 ? M->h2 : nullptr,
+                                               M->p + OFF_W3, M->p + OFF_B3, F.s);
     HIPCHK(hipGetLastError());
     return GST_OK;
 }
@@ -1145,10 +1191,15 @@ void gst_model_free(gst_model* M) {
     if (!M) return;
     hipFree(M->p); hipFree(M->g); hipFree(M->m); hipFree(M->v);
     hipFree(M->W1t); hipFree(M->W2t); hipFree(M->W1kn); hipFree(M->W2kn); hipFree(M->loss);
-    hipFree(M->ax_static); hipFree(M->gcnt);
-    hipFree(M->x); hipFree(M->ax); hipFree(M->h1); hipFree(M->ah1); hipFree(M->h2); hipFree(M->dZ2); hipFree(M->dZ1);
-    hipFree(M->s); hipFree(M->dS); hipFree(M->dX2); hipFree(M->dH1); hipFree(M->dA1); hipFree(M->hub_acc);
-    hipFree(M->gm); hipFree(M->y); hipFree(M->grp); hipFree(M->hub_list); hipFree(M->hub_items);
+    hipFree(M->ax_static);
+    for (FwdSet& F : M->f) {
+        hipFree(F.x); hipFree(F.ax); hipFree(F.h1); hipFree(F.ah1); hipFree(F.s); hipFree(F.gm); hipFree(F.gcnt);
+        hipFree(F.hub_acc); hipFree(F.grp);
+    }
+    if (M->e_static) hipEventDestroy(M->e_static);
+    hipFree(M->h2); hipFree(M->dZ2); hipFree(M->dZ1);
+    hipFree(M->dS); hipFree(M->dX2); hipFree(M->dH1); hipFree(M->dA1);
+    hipFree(M->y); hipFree(M->hub_list); hipFree(M->hub_items);
     sel_free(M->sync_sel);
     for (auto& sl : M->batch_sel) sel_free(sl);
     for (auto& b : M->sub_pool) sub_free(b);
@@ -1183,10 +1234,10 @@ gst_status gst_model_get_params(gst_model* M, float* out) {
 gst_status gst_model_scores(gst_model* M, const gst_graph* G, const gst_groups* groups, float* scores) {
     if (!M || !G || !groups || !scores) return GST_INVALID_ARG;
     gst_ctx* ctx = M->ctx;
-    gst_status s = forward(M, G, groups);
+    gst_status s = forward(M, M->f[0], G, groups);
     if (s != GST_OK) return s;
     hipStream_t st = (hipStream_t)ctx->stream;
-    HIPCHK(hipMemcpyAsync(scores, M->s, (size_t)G->n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(scores, M->f[0].s, (size_t)G->n * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return GST_OK;
 }
@@ -1195,25 +1246,26 @@ gst_status gst_model_loss_grad(gst_model* M, const gst_graph* G, const gst_group
                                float pos_weight, float* loss_out, float* grad_out) {
     if (!M || !G || !groups || !labels) return GST_INVALID_ARG;
     gst_ctx* ctx = M->ctx;
-    gst_status sst = forward(M, G, groups, nullptr, true);
+    gst_status sst = forward(M, M->f[0], G, groups, nullptr, true);
     if (sst != GST_OK) return sst;
     hipStream_t st = (hipStream_t)ctx->stream;
+    const FwdSet& F = M->f[0];
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
     const int dg = std::max(1, ctx->cu_count * 8);
     HIPCHK(hipMemcpyAsync(M->y, labels, n, hipMemcpyHostToDevice, st));
     k_fill32<<<1, 64, 0, st>>>((uint32_t*)M->loss, 0u, 1);
     if (grad_out) k_fill32<<<gridn(NPARAM, 256, fg), 256, 0, st>>>((uint32_t*)M->g, 0u, NPARAM);
-    k_loss<<<gridn(n, 256, fg), 256, 0, st>>>(M->s, M->y, n, pos_weight, M->dS, M->loss);
+    k_loss<<<gridn(n, 256, fg), 256, 0, st>>>(F.s, M->y, n, pos_weight, M->dS, M->loss);
     k_head_bwd<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dS, M->h2, M->p + OFF_W3, n, M->dZ2,
                                                                    M->g + OFF_W3, M->g + OFF_B3);
     k_bias_grad<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dZ2, n, M->g + OFF_B2);
-    k_dense_bwd_w<HID><<<std::max(1, ctx->cu_count * 2), 256, 0, st>>>(M->dZ2, M->h1, M->ah1, n, M->g + OFF_W2);
+    k_dense_bwd_w<HID><<<std::max(1, ctx->cu_count * 2), 256, 0, st>>>(M->dZ2, F.h1, F.ah1, n, M->g + OFF_W2);
     k_dense_bwd_x<K2><<<dg, 256, 0, st>>>(M->dZ2, M->W2kn, n, M->dX2);
     k_split_self<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dX2, n, M->dH1, M->dA1);
-    aggregate<HID, true>(M, G, M->dA1, M->dH1, st);
-    k_relu_mask_bias<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dH1, M->h1, n, M->dZ1, M->g + OFF_B1);
-    k_dense_bwd_w<FIN><<<std::max(1, ctx->cu_count * 2), 256, 0, st>>>(M->dZ1, M->x, M->ax, n, M->g + OFF_W1);
+    aggregate<HID, true>(M, G, F.hub_acc, M->dA1, M->dH1, st);
+    k_relu_mask_bias<<<gridn((uint64_t)n * HID, 256, fg), 256, 0, st>>>(M->dH1, F.h1, n, M->dZ1, M->g + OFF_B1);
+    k_dense_bwd_w<FIN><<<std::max(1, ctx->cu_count * 2), 256, 0, st>>>(M->dZ1, F.x, F.ax, n, M->g + OFF_W1);
     HIPCHK(hipGetLastError());
     float l = 0;
     HIPCHK(hipMemcpyAsync(&l, M->loss, 4, hipMemcpyDeviceToHost, st));
@@ -1254,9 +1306,11 @@ namespace {
 // select_finish (once e_sizes is complete) reserves buf for those sizes, queues the subgraph's CSR
 // (event sl.e_built) and fills S as a view of buf (never freed as a graph) and sub_vtx with the
 // members' subgraph ids.  A slot's scratch is reused only after its e_built.
-gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* groups, double frac, SelSlot& sl) {
+gst_status select_begin(gst_model* M, FwdSet& F, const gst_graph* G, const gst_groups* groups, double frac,
+                        SelSlot& sl) {
     gst_ctx* ctx = M->ctx;
-    hipStream_t st = mstream(M);
+    hipStream_t st = fstream(M, F);
+    sl.st = st;
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
     const int g = groups->n_groups;
@@ -1287,7 +1341,7 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
     if (!sl.e_sizes) HIPCHK(hipEventCreateWithFlags(&sl.e_sizes, hipEventDisableTiming));
     if (!sl.e_built) HIPCHK(hipEventCreateWithFlags(&sl.e_built, hipEventDisableTiming));
     sl.n = n; sl.nbk = nbk; sl.gcnt = gcnt;
-    gst_status sst = forward(M, G, groups, sl.h_grp);
+    gst_status sst = forward(M, F, G, groups, sl.h_grp);
     if (sst != GST_OK) return sst;
     uint32_t k_target = (uint32_t)std::min<double>(n, std::ceil(frac * n));
     if (k_target < 1) k_target = 1;
@@ -1304,15 +1358,15 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
     // ---- top-k threshold on the device (the first pass starts the state) ----
     const int hg = std::max(1, std::min(ctx->cu_count, (int)((n + 4095) / 4096)));
     for (int pass = 0; pass < 3; ++pass) {
-        k_key_hist<<<hg, 256, 0, st>>>(M->s, n, pass, d_state, d_hist);
+        k_key_hist<<<hg, 256, 0, st>>>(F.s, n, pass, d_state, d_hist);
         k_pick_bin<<<1, 256, 0, st>>>(d_hist, d_state, pass, k_target);
     }
     // ---- induced subgraph: selection and new ids, arc offsets, group lists in new ids ----
-    k_select_scan<<<nbk, 256, 0, st>>>(M->s, M->gm, n, d_state, d_sel, d_newid, d_bsum);
+    k_select_scan<<<nbk, 256, 0, st>>>(F.s, F.gm, n, d_state, d_sel, d_newid, d_bsum);
     k_scan_sums<<<1, 64, 0, st>>>(d_bsum, nbk, d_hdr);
     k_scan_add_list<<<gridn(n, 256, fg), 256, 0, st>>>(d_newid, n, d_bsum, d_sel, d_old, d_deg);
     k_sub_deg<<<std::max(1, ctx->cu_count * 8), 256, 0, st>>>(G->d_row, G->d_col, d_sel, d_old, d_bsum + nbk, d_deg,
-                                                              M->grp + g + 1, gcnt, d_newid, d_subgrp);
+                                                              F.grp + g + 1, gcnt, d_newid, d_subgrp);
     k_scan_blocks<<<nbk, 256, 0, st>>>(d_deg, n, d_arcoff, d_bsum + nbk + 2);
     k_scan_sums<<<1, 64, 0, st>>>(d_bsum + nbk + 2, nbk, d_hdr + 1);   // offsets completed by k_sub_fill
     HIPCHK(hipMemcpyAsync(sl.h_rb, d_hdr, (2ull + gcnt) * 4ull, hipMemcpyDeviceToHost, st));
@@ -1324,7 +1378,7 @@ gst_status select_begin(gst_model* M, const gst_graph* G, const gst_groups* grou
 gst_status select_finish(gst_model* M, const gst_graph* G, SelSlot& sl, SubBuf& buf, gst_graph& S,
                          std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool pooled = false) {
     gst_ctx* ctx = M->ctx;
-    hipStream_t st = mstream(M);
+    hipStream_t st = sl.st;   // the stream its select_begin ran on
     const uint32_t n = sl.n, nbk = sl.nbk, gcnt = sl.gcnt;
     const uint32_t ns = sl.h_rb[0], ms = sl.h_rb[1];
     sub_vtx.assign((const int32_t*)&sl.h_rb[2], (const int32_t*)&sl.h_rb[2] + gcnt);
@@ -1366,7 +1420,7 @@ gst_status learned_subgraph(gst_model* M, const gst_graph* G, const gst_groups* 
                             gst_graph& S, std::vector<int32_t>& sub_vtx, gst_learned_stats& L, bool want_host_rows) {
     S.n = 0;   // no subgraph unless one is built below
     if (sub_declined(M, G)) return GST_OK;
-    gst_status sst = select_begin(M, G, groups, frac, M->sync_sel);
+    gst_status sst = select_begin(M, M->f[0], G, groups, frac, M->sync_sel);
     if (sst != GST_OK) return sst;
     gst_ctx* ctx = M->ctx;
     HIPCHK(hipEventSynchronize(M->sync_sel.e_sizes));
@@ -1484,14 +1538,28 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     for (int32_t q = 0; q < nq; ++q)
         if (status[q] == GST_OK) gmax = std::max(gmax, groups[q].n_groups);
     gst_status setup_err = GST_OK;
-    // the selection of upcoming queries runs on a second stream while the engine's step runs
+    // the selection of upcoming queries runs on streams of its own while the engine's step runs: set 0
+    // of the activations on stream2, set 1 on stream3, so that two selections overlap (each is some
+    // 20 short launches whose gaps the other fills)
+    static_assert(GST_SELECT_LANES >= 1 && GST_SELECT_LANES <= FWD_SETS, "selection lanes");
+    const int lanes = GST_SELECT_LANES;
     if (!ctx->stream2) {
         hipStream_t s2;
         HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         ctx->stream2 = (void*)s2;
     }
+    if (lanes > 1 && !ctx->stream3) {
+        hipStream_t s3;
+        HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+        ctx->stream3 = (void*)s3;
+    }
     const int lookahead = GST_SELECT_AHEAD;
-    M->sel_stream = ctx->stream2;   // every selection of the run queues on the selection stream
+    M->sel_stream = ctx->stream2;   // set 0's selections (and the model's set-up) queue on stream2
+    if (lanes > 1) M->f[1].st = (hipStream_t)ctx->stream3;
+    auto end_streams = [&]() {
+        M->sel_stream = nullptr;
+        M->f[1].st = nullptr;
+    };
     auto now = []() {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
     };
@@ -1512,7 +1580,8 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     // are ~30 launches on the selection stream, and the engine's thread only fills slots and steps.
     // Under mu: the queues of selected queries, the free subgraph buffers, the count of selections
     // begun and not yet taken by the engine (at most `lookahead`), and the selection thread's end.
-    const size_t depth = 3;   // selections queued on the stream at once (the oldest finishes first)
+    // selections queued at once (the oldest finishes first); slot i runs on activation set i % lanes
+    const size_t depth = std::max<size_t>(GST_SELECT_DEPTH, (size_t)lanes);
     if (M->batch_sel.size() < depth) M->batch_sel.resize(depth);
     std::mutex mu;
     std::condition_variable cv;
@@ -1570,6 +1639,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
             publish(ready_q, e.q);
             return GST_OK;
         };
+        int last_lane = -1;
         auto run = [&]() -> gst_status {
             HIPCHK(hipSetDevice(ctx->device));
             int32_t next = 0;   // queries below next were begun or failed validation
@@ -1595,9 +1665,14 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
                         continue;
                     }
                     qs[(size_t)q].t_sel = now();
-                    SelSlot* sl = idle_slots.back();
-                    idle_slots.pop_back();
-                    const gst_status ss = select_begin(M, G, &groups[q], frac, *sl);
+                    // an idle slot on another set than the last selection's, so that the sets alternate
+                    size_t pick = idle_slots.size() - 1;
+                    for (size_t i = 0; i < idle_slots.size(); ++i)
+                        if ((int)((idle_slots[i] - M->batch_sel.data()) % lanes) != last_lane) { pick = i; break; }
+                    SelSlot* sl = idle_slots[pick];
+                    idle_slots.erase(idle_slots.begin() + (ptrdiff_t)pick);
+                    last_lane = (int)((sl - M->batch_sel.data()) % lanes);
+                    const gst_status ss = select_begin(M, M->f[last_lane], G, &groups[q], frac, *sl);
                     if (ss == GST_OK) { inflight.push_back({sl, q}); continue; }
                     idle_slots.push_back(sl);
                     if (ss != GST_OOM) { status[q] = ss; return ss; }
@@ -1686,7 +1761,7 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     try {
         sel_th = std::thread(selector);
     } catch (const std::exception&) {   // no exception crosses the C ABI
-        M->sel_stream = nullptr;
+        end_streams();
         return gst_set_error(ctx, GST_INTERNAL, "learned batch: cannot start the selection thread");
     }
     bool engine_error = false;
@@ -1700,7 +1775,8 @@ gst_status gst_learned_solve_batch(gst_ctx* ctx, const gst_graph* G, int32_t nq,
     sel_th.join();
     // selections still queued finish before their buffers are reused
     HIPCHK(hipStreamSynchronize((hipStream_t)ctx->stream2));
-    M->sel_stream = nullptr;
+    if (lanes > 1) HIPCHK(hipStreamSynchronize((hipStream_t)ctx->stream3));
+    end_streams();
     for (auto& x : qs) gst_result_free(&x.sub_res);
     // a query whose full solve never finished (the engine or the selection stopped early) carries
     // the error that stopped it, never a zeroed result marked OK

@@ -462,7 +462,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     S.grow_updates = H->stats[ST_UPD]; S.grow_enqueued = H->stats[ST_ENQ]; S.grow_new_states = H->stats[ST_NEW];
     S.merge_vertices = H->stats[ST_MVERT]; S.merge_pairs = H->stats[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
-    S.grow_bytes = S.grow_items * 24ull + S.grow_arcs * 12ull + S.grow_relax * 8ull + S.grow_updates * 16ull;
+    S.grow_bytes = gst_grow_bytes(S);
     S.merge_bytes = H->stats[ST_MBYTES];
     return GST_OK;
 }

@@ -104,8 +104,8 @@ typedef struct {
     uint64_t grow_enqueued;    /* work items enqueued */
     uint64_t merge_vertices;   /* vertex rows merged */
     uint64_t merge_pairs;      /* (submask, complement) pairs evaluated */
-    uint64_t grow_bytes;       /* algorithmic bytes of the grow kernel: 24 per item, 12 per arc, 8 per
-                                  relaxation, 16 per update */
+    uint64_t grow_bytes;       /* algorithmic bytes of the grow kernel: 24 per item, 8 per enqueue,
+                                  8 per arc, 8 per relaxation, 16 per update */
     uint64_t merge_bytes;      /* algorithmic bytes of the merge kernel: rows read, words written, flags */
     uint32_t grow_rounds;
     uint32_t grow_launches;
@@ -232,7 +232,8 @@ typedef struct {
     double   t_grow_ms;        /* with time_kernels: device time of the batched grow launches (HIP events) */
     double   t_merge_ms;       /* with time_kernels: device time of the batched merge launches */
     uint64_t merge_launches;
-    int32_t  reserved[4];
+    uint64_t busy_slot_steps;  /* slots holding a query, summed over steps (/ steps: mean slots busy) */
+    int32_t  reserved[2];
 } gst_batch_stats;
 
 /* groups[nq]; opts: NULL (plain DPBF, trees returned) or nq entries, of which ub, prune_lb and

@@ -267,6 +267,13 @@ def test_trained_network_gain(solver):
     assert out["trained"][0] < out["untrained"][0] or out["trained"][1] < out["untrained"][1]
     assert out["trained"][2] < out["untrained"][2]
     assert out["trained"][2] < 0.5
+    # the method without the network: the automatic spider bound alone, same engine, same queries
+    spider, _ = solver.solve_batch(dg, qs, auto_bound=True, slots=6)
+    for r, p in zip(spider, plain):
+        assert r.cost == p.cost
+    s_relax = sum(r.stats["grow_relax"] for r in spider) / sum(p.stats["grow_relax"] for p in plain)
+    print(f"spider bound only: relaxations vs plain {s_relax:.4f}", flush=True)
+    assert out["trained"][2] < s_relax
     trained.free()
     untrained.free()
     dg.free()
