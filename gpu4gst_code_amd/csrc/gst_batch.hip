@@ -143,7 +143,7 @@ __global__ void k_slot_reset(uint32_t* gm, uint32_t n, unsigned long long* root,
     for (uint32_t v = tid; v < n; v += gridDim.x * blockDim.x) gm[v] = 0u;
     if (blockIdx.x == 0) {
         for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(RoundCtl) * NCTL / 4); i += blockDim.x) ctl[i] = 0u;
-        for (uint32_t i = threadIdx.x; i < (uint32_t)ST_NSTAT; i += blockDim.x) stats[i] = 0ull;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)STAT_WORDS; i += blockDim.x) stats[i] = 0ull;
         if (threadIdx.x == 0) {
             *root = GST_INF_WORD;
             *spider = GST_INF_COST;
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(256) k_batch_fill_reset(BatchArgs b) {
     for (uint32_t v = bid * blockDim.x + threadIdx.x; v < f->n; v += nblk * blockDim.x) f->gm[v] = 0u;
     if (bid == 0) {
         for (uint32_t i = threadIdx.x; i < (uint32_t)(sizeof(RoundCtl) * NCTL / 4); i += blockDim.x) f->ctl[i] = 0u;
-        for (uint32_t i = threadIdx.x; i < (uint32_t)ST_NSTAT; i += blockDim.x) f->stats[i] = 0ull;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)STAT_WORDS; i += blockDim.x) f->stats[i] = 0ull;
         if (threadIdx.x == 0) {
             *f->root = GST_INF_WORD;
             *f->spider = GST_INF_COST;
@@ -497,7 +497,7 @@ struct gst_batch_engine {
 // layout of a slot's readback block
 constexpr size_t BK_CTL = 8;
 constexpr size_t BK_STATS = (BK_CTL + sizeof(RoundCtl) * NCTL + 7) / 8 * 8;
-constexpr size_t BK_OUTN = BK_STATS + 8 * ST_NSTAT;
+constexpr size_t BK_OUTN = BK_STATS + 8 * STAT_WORDS;
 constexpr size_t BK_EDGES = BK_OUTN + 8;
 constexpr uint32_t EDGE_PREFIX = 1024;   // edges read back with the header (12 KiB)
 static size_t back_bytes(uint32_t ecap) { return BK_EDGES + 12 * (size_t)ecap; }
@@ -547,7 +547,7 @@ void gst_batch_free(gst_ctx* ctx) {
 static size_t slot_bytes(uint32_t n, int g, uint64_t items_cap, uint32_t W, uint32_t ecap) {
     const size_t words = (size_t)n * (((size_t)1 << g) - 1);
     return words * 8 + (size_t)n * 4 + (size_t)n * (size_t)(g + 1) + 3 * ((size_t)n * W * 8 + (size_t)n * 4 + items_cap * 8) +
-           sizeof(RoundCtl) * NCTL + 8 + 8 * ST_NSTAT + (3 * (size_t)ecap + 1) * 4 + 64 * 1024;
+           sizeof(RoundCtl) * NCTL + 8 + 8 * STAT_WORDS + (3 * (size_t)ecap + 1) * 4 + 64 * 1024;
 }
 
 static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_t n, int g, int64_t m, uint32_t chunk) {
@@ -1082,8 +1082,9 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
             unsigned long long rw;
             std::memcpy(&rw, hb, 8);
             const RoundCtl* c = (const RoundCtl*)(hb + BK_CTL);
-            unsigned long long hs[ST_NSTAT];
-            std::memcpy(hs, hb + BK_STATS, sizeof(hs));
+            unsigned long long hst[STAT_WORDS], hs[ST_NSTAT];
+            std::memcpy(hst, hb + BK_STATS, sizeof(hst));
+            stat_sum(hst, hs);
             uint32_t nout = 0;
             std::memcpy(&nout, hb + BK_OUTN, 4);
             uint32_t err = 0, rounds = 0;

@@ -331,7 +331,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         uint64_t d = 0;
         ok = ok && grow_alloc(B->ctl, d, NCTL, sizeof(RoundCtl));
         d = 0; ok = ok && grow_alloc(B->root, d, 1, 8);
-        d = 0; ok = ok && grow_alloc(B->stats, d, ST_NSTAT, 8);
+        d = 0; ok = ok && grow_alloc(B->stats, d, STAT_WORDS, 8);
         if (hipHostMalloc((void**)&B->h_ctl, sizeof(RoundCtl) * NCTL) != hipSuccess) ok = false;
     }
     ok = ok && grow_alloc(B->edges, B->cap_edges, 3 * edges_cap + 1, 4);
@@ -401,7 +401,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     HIPCHK(hipMemcpyAsync(B->grp, hgrp.data(), grp_need * 4, hipMemcpyHostToDevice, s));
     k_fill_u32<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->gm, 0u, n);
     k_set_group_bits<<<64, 256, 0, s>>>(B->gm, B->grp, B->grp + g + 1, g);
-    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->stats, 0ull, ST_NSTAT);
+    k_fill_u64<<<1, 256, 0, s>>>((uint64_t*)B->stats, 0ull, STAT_WORDS);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->root, GST_INF_WORD, 1);
     k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)B->ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)B->actl, 0u, sizeof(AsyncCtl) / 4);
@@ -745,11 +745,20 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             }
             std::fprintf(stderr, "\n");
         }
+        std::fprintf(stderr, "[merges] us per launch, levels 2..%d:", g);
+        for (auto& e : evs) {
+            if (e.kind != 1) continue;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+            std::fprintf(stderr, " %.1f", ms * 1000.0);
+        }
+        std::fprintf(stderr, "\n");
     }
     // ---- stats ----
-    unsigned long long hs[ST_NSTAT];
-    HIPCHK(hipMemcpyAsync(hs, B->stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+    unsigned long long hst[STAT_WORDS], hs[ST_NSTAT];
+    HIPCHK(hipMemcpyAsync(hst, B->stats, sizeof(hst), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    stat_sum(hst, hs);
     gst_stats& S = res->stats;
     S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
     S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW];

@@ -38,6 +38,21 @@ struct RoundCtl {
 enum StatIdx {
     ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_NEW, ST_MVERT, ST_MPAIRS, ST_MBYTES, ST_NSTAT
 };
+// The counters are striped: workgroup b adds into stripe b % STAT_STRIPES (ST_NSTAT words each), so
+// that the one atomic per workgroup and counter at the end of a launch spreads over several
+// addresses instead of queueing on one (measured: up to 15 us per merge launch of 1000+
+// workgroups); readers sum the stripes (stat_sum).
+constexpr int STAT_STRIPES = 16;
+constexpr int STAT_WORDS = ST_NSTAT * STAT_STRIPES;
+__device__ __forceinline__ unsigned long long* stat_stripe(unsigned long long* stats, uint32_t bid) {
+    return stats + (size_t)(bid % STAT_STRIPES) * ST_NSTAT;
+}
+inline void stat_sum(const unsigned long long* striped, unsigned long long* out) {
+    for (int i = 0; i < ST_NSTAT; ++i) out[i] = 0;
+    for (int s = 0; s < STAT_STRIPES; ++s)
+        for (int i = 0; i < ST_NSTAT; ++i) out[i] += striped[(size_t)s * ST_NSTAT + i];
+}
+
 
 struct GrowArgs {
     uint64_t* Tk;          // level-k block
@@ -568,7 +583,7 @@ __device__ __forceinline__ void compact_body(const uint32_t* flags, const uint32
         if (threadIdx.x == 0) {
             const uint32_t tot = s_wtot[0] + s_wtot[1] + s_wtot[2] + s_wtot[3];
             s_base = tot ? atomicAdd(cnt_out, tot) : 0u;
-            if (tot && stats) atomicAdd(&stats[ST_ENQ], (unsigned long long)tot);
+            if (tot && stats) atomicAdd(&stat_stripe(stats, bid)[ST_ENQ], (unsigned long long)tot);
         }
         __syncthreads();
         uint32_t off = s_base;
@@ -985,7 +1000,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             if (so) s_ovf = 1;
         }
         __syncthreads();
-        if (threadIdx.x < ST_MVERT && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+        if (threadIdx.x < ST_MVERT && s_stats[threadIdx.x]) atomicAdd(&stat_stripe(a.stats, bid)[threadIdx.x], s_stats[threadIdx.x]);
         if (threadIdx.x == 0 && s_ovf) atomicOr(&ctl->err, 1u);
     }
 
@@ -1318,12 +1333,13 @@ __global__ void __launch_bounds__(GROW_BLOCK) k_grow_async(GrowArgs a) {
     const uint32_t sn = wave_sum32(st_new);
     const uint32_t so = wave_sum32(ovf);
     if (lane == 0) {
-        if (sn) atomicAdd(&a.stats[ST_NEW], (unsigned long long)sn);
-        if (si) atomicAdd(&a.stats[ST_ITEMS], (unsigned long long)si);
-        if (sa) atomicAdd(&a.stats[ST_ARCS], (unsigned long long)sa);
-        if (sr) atomicAdd(&a.stats[ST_RELAX], (unsigned long long)sr);
-        if (su) atomicAdd(&a.stats[ST_UPD], (unsigned long long)su);
-        if (se) atomicAdd(&a.stats[ST_ENQ], (unsigned long long)se);
+        unsigned long long* sp = stat_stripe(a.stats, blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+        if (sn) atomicAdd(&sp[ST_NEW], (unsigned long long)sn);
+        if (si) atomicAdd(&sp[ST_ITEMS], (unsigned long long)si);
+        if (sa) atomicAdd(&sp[ST_ARCS], (unsigned long long)sa);
+        if (sr) atomicAdd(&sp[ST_RELAX], (unsigned long long)sr);
+        if (su) atomicAdd(&sp[ST_UPD], (unsigned long long)su);
+        if (se) atomicAdd(&sp[ST_ENQ], (unsigned long long)se);
         if (so) atomicOr(&ac->err, 1u);
     }
 }
@@ -1610,9 +1626,10 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     }
     const uint32_t so = wave_sum32(ovf);
     if (threadIdx.x == 0) {
-        atomicAdd(&a.stats[ST_MPAIRS], (unsigned long long)st_pairs);
-        atomicAdd(&a.stats[ST_MVERT], (unsigned long long)st_vert);
-        atomicAdd(&a.stats[ST_MBYTES], (unsigned long long)st_bytes);
+        unsigned long long* sp = stat_stripe(a.stats, bid);
+        atomicAdd(&sp[ST_MPAIRS], (unsigned long long)st_pairs);
+        atomicAdd(&sp[ST_MVERT], (unsigned long long)st_vert);
+        atomicAdd(&sp[ST_MBYTES], (unsigned long long)st_bytes);
     }
     if (lane_id() == 0 && so) atomicOr(&a.ctl->err, 1u);
 }

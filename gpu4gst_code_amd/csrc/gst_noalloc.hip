@@ -71,7 +71,7 @@ struct gst_noalloc_host {
     gst_status init_status = GST_OK;
     char init_msg[256] = {0};
     RoundCtl ctl[NCTL];                      // control blocks read back
-    unsigned long long stats[ST_NSTAT];
+    unsigned long long stats[STAT_WORDS];    // striped counters, summed by stat_sum
     uint32_t tri[3 * GST_NOALLOC_MAX_EDGES + 1];   // traced (u, v, w) triples, then their count
     NaEdge edges[GST_NOALLOC_MAX_EDGES];
     uint32_t verts[2 * GST_NOALLOC_MAX_EDGES];
@@ -227,7 +227,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         }
         B.ctl = c.take<RoundCtl>(NCTL);
         B.root = c.take<unsigned long long>(1);
-        B.stats = c.take<unsigned long long>(ST_NSTAT);
+        B.stats = c.take<unsigned long long>(STAT_WORDS);
         B.spider = c.take<uint32_t>(1);
         B.tour = c.take<uint32_t>(tour_words);
         B.edges = c.take<uint32_t>(3 * (size_t)ecap + 1);
@@ -258,7 +258,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         k_fill_u32<<<fill_grid, 256, 0, s>>>(B.inq[i], 0u, n);
     }
     k_fill_u32<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B.gm, 0u, n);
-    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B.stats, 0ull, ST_NSTAT);
+    k_fill_u64<<<1, 256, 0, s>>>((uint64_t*)B.stats, 0ull, STAT_WORDS);
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B.root, GST_INF_WORD, 1);
     k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)B.ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
     k_fill_u32<<<1, 64, 0, s>>>(B.all_cnt, 0u, 1);
@@ -457,12 +457,14 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     // ---- stats (no HIP events here: creating them is the runtime's business, not this call's) ----
     HIPCHK(hipMemcpyAsync(H->stats, B.stats, sizeof(H->stats), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    unsigned long long hs[ST_NSTAT];
+    stat_sum(H->stats, hs);
     gst_stats& S = res->stats;
-    S.grow_items = H->stats[ST_ITEMS]; S.grow_arcs = H->stats[ST_ARCS]; S.grow_relax = H->stats[ST_RELAX];
-    S.grow_updates = H->stats[ST_UPD]; S.grow_enqueued = H->stats[ST_ENQ]; S.grow_new_states = H->stats[ST_NEW];
-    S.merge_vertices = H->stats[ST_MVERT]; S.merge_pairs = H->stats[ST_MPAIRS];
+    S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
+    S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW];
+    S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.grow_bytes = gst_grow_bytes(S);
-    S.merge_bytes = H->stats[ST_MBYTES];
+    S.merge_bytes = hs[ST_MBYTES];
     return GST_OK;
 }
