@@ -752,8 +752,20 @@ __global__ void k_sub_deg(const uint32_t* row, const int32_t* col, const uint32_
     const uint32_t cnt = *ns;
     for (uint32_t i = gwave; i < cnt; i += nwaves) {
         const uint32_t v = old_of[i];
+        const uint32_t re = row[v + 1];
         uint32_t c = 0;
-        for (uint32_t e = row[v] + lane; e < row[v + 1]; e += 64) c += sel[col[e]];
+        // 4 x 64 arcs per step: their column loads, then their selection gathers, all independent
+        // (a hub row of thousands of arcs is one wave's work)
+        for (uint32_t e0 = row[v]; e0 < re; e0 += 256) {
+            uint32_t cc[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = e0 + j * 64 + lane;
+                cc[j] = e < re ? (uint32_t)col[e] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c += cc[j] != 0xFFFFFFFFu ? sel[cc[j]] : 0u;
+        }
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if (lane == 0) deg[v] = c;
     }
@@ -776,17 +788,33 @@ __global__ void k_sub_fill(const uint32_t* row, const uint64_t* arc, const uint3
         uint32_t pos = arc_off[v] + arc_bsum[v / 1024];
         if (lane == 0) srow[nv] = pos;
         const uint32_t re = row[v + 1];
-        for (uint32_t e0 = row[v]; e0 < re; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            const uint64_t r = e < re ? arc[e] : 0ull;
-            const uint32_t x = (uint32_t)r;
-            const bool ok = e < re && sel[x];
-            const uint64_t bal = __ballot(ok);
-            if (ok) {
-                const uint32_t p = pos + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-                sarc[p] = (r & 0xFFFFFFFF00000000ull) | newid[x];
+        // 4 x 64 arcs per step: their records, then their selection flags and new ids, all
+        // independent loads; the positions follow from the ballots in arc order
+        for (uint32_t e0 = row[v]; e0 < re; e0 += 256) {
+            uint64_t r[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = e0 + j * 64 + lane;
+                r[j] = e < re ? arc[e] : ~0ull;
             }
-            pos += (uint32_t)__popcll(bal);
+            uint32_t sl[4], nid[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t x = (uint32_t)r[j];
+                const bool in = r[j] != ~0ull;
+                sl[j] = in ? sel[x] : 0u;
+                nid[j] = in ? newid[x] : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = sl[j] != 0u;
+                const uint64_t bal = __ballot(ok);
+                if (ok) {
+                    const uint32_t p = pos + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    sarc[p] = (r[j] & 0xFFFFFFFF00000000ull) | nid[j];
+                }
+                pos += (uint32_t)__popcll(bal);
+            }
         }
     }
 }
