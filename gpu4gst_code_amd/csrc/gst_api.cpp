@@ -310,7 +310,7 @@ gst_status gst_solve_batch(gst_ctx* ctx, const gst_graph* g, int32_t nq, const g
         results[q].root = -1;
         status[q] = check_query(ctx, g, &groups[q]);
         if (status[q] == GST_OK && opts && !gst_push_lanes_ok(opts[q].push_lanes))
-            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 1, 8, 16, 32 or 64");
         if (status[q] == GST_OK && opts && opts[q].half_bound != 0 && opts[q].half_bound != -1)
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1");
         if (status[q] == GST_OK && opts && opts[q].tour_bound != 0 && opts[q].tour_bound != -1)
@@ -390,7 +390,7 @@ gst_status gst_solve_batch_learned(gst_ctx* ctx, const gst_graph* g, int32_t nq,
         if (lstats) { std::memset(&lstats[q], 0, sizeof(lstats[q])); lstats[q].ub = -1.0; }
         status[q] = check_query(ctx, g, &groups[q]);
         if (status[q] == GST_OK && opts && !gst_push_lanes_ok(opts[q].push_lanes))
-            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64");
+            status[q] = gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 1, 8, 16, 32 or 64");
         if (status[q] == GST_OK && opts && opts[q].half_bound != 0 && opts[q].half_bound != -1)
             status[q] = gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1");
         if (status[q] == GST_OK && opts && opts[q].tour_bound != 0 && opts[q].tour_bound != -1)

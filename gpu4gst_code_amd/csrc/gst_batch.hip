@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_grow(BatchArgs b) {
                 // with one thread per item
                 const uint32_t mch = a.m < (uint32_t)WAVE ? a.m : (uint32_t)WAVE;
                 const uint32_t rows_per_wave = (uint32_t)WAVE / mch;
-                const uint32_t items_per_wave = (uint32_t)WAVE / a.push_lanes;
+                const uint32_t items_per_wave = push_items_per_wave(round_push_lanes(a.push_lanes, a.push_lanes_sparse, n_cur));
                 const uint32_t n_all = a.n_all_dev ? *a.n_all_dev : a.n_all;
                 const uint32_t waves = pull ? (a.n + rows_per_wave - 1) / rows_per_wave + n_all
                                             : (n_cur + items_per_wave - 1) / items_per_wave;
@@ -761,13 +761,19 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.seq = 0;
         a.push_lanes = job.opts.push_lanes > 0 ? (uint32_t)job.opts.push_lanes
                                                : (tuned_lanes ? tuned_lanes : default_push_lanes(true));
-        if (job.opts.push_lanes == 0) {
+        a.push_lanes_sparse = 0;
+        if (job.opts.push_lanes == 0 && a.push_lanes != PUSH_ITEM_PAR) {
             // a level of m masks gets at least the power of two >= m lanes per item (8..64): an
             // item's changed masks then spread over its lanes instead of looping (measured: +5% on
             // the g=6 batch, +27% on mixed g = 3..8, whose middle levels hold up to 70 masks)
             uint32_t want = 8;
             while (want < (uint32_t)L.m[k] && want < 64) want <<= 1;
             if (want > a.push_lanes) a.push_lanes = want;
+            // dense rounds of the chosen levels one item per lane, the others in these lane groups
+            if (itempar_level(GST_BATCH_ITEMPAR, k)) {
+                a.push_lanes_sparse = a.push_lanes;
+                a.push_lanes = PUSH_ITEM_PAR;
+            }
         }
     }
     for (int k = 2; k <= g; ++k) {

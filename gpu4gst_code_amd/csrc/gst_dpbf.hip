@@ -257,7 +257,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     opts.ub = 0; opts.prune_lb = 1; opts.want_tree = 1;
     if (opts_in) opts = *opts_in;
     if (!gst_push_lanes_ok(opts.push_lanes))
-        return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
+        return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 1, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
     if (opts.half_bound != 0 && opts.half_bound != -1)
         return gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1 (got %d)", opts.half_bound);
     if (opts.tour_bound != 0 && opts.tour_bound != -1)
@@ -468,7 +468,10 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ga.rlog = trace_rounds ? B->rlog + (size_t)k * RLOG_ROUNDS * 4 : nullptr;
         uint32_t launch_seq = 0;   // the level's control block starts with seq 0
         ga.seq = 0;
-        ga.push_lanes = opts.push_lanes > 0 ? (uint32_t)opts.push_lanes : default_push_lanes(false);
+        ga.push_lanes = opts.push_lanes > 0 ? (uint32_t)opts.push_lanes
+                        : itempar_level(GST_SOLO_ITEMPAR, k) ? PUSH_ITEM_PAR : default_push_lanes(false);
+        // the library's item-parallel choice: sparse rounds in its default lane groups
+        ga.push_lanes_sparse = (opts.push_lanes == 0 && ga.push_lanes == PUSH_ITEM_PAR) ? default_push_lanes(false) : 0u;
         auto launch_async = [&]() {
             Ev* e = new_ev(0);
             hipEventRecord(e->a, s);

@@ -82,7 +82,10 @@ struct GrowArgs {
     struct AsyncCtl* actl;
     uint32_t* rlog;             // optional round log (RLOG_ROUNDS x 4 words), nullptr: off
     uint32_t seq;               // launch number within the level (1, 2, ...), set per launch
-    uint32_t push_lanes;        // lanes per work item in push rounds (8, 16, 32 or 64)
+    uint32_t push_lanes;        // lanes per work item in push rounds (8, 16, 32 or 64; PUSH_ITEM_PAR: one
+                                // item per lane)
+    uint32_t push_lanes_sparse; // with PUSH_ITEM_PAR: lanes per item in rounds below GST_IP_MIN_ITEMS items
+                                // (0: the item-parallel shape in every round)
     uint32_t gmax;              // a state is grown only while its cost is <= gmax (half the bound)
     uint32_t lbcap;             // level-1 distances above gmax are only known to exceed it: the lower
                                 // bound reads min(distance, lbcap) (GST_INF_COST: no cap)
@@ -109,7 +112,54 @@ __device__ __forceinline__ uint32_t mark_active(uint8_t* act, uint32_t u, unsign
 // Lanes per push item unless the caller chose (measured on BA V=200k, g=6): one query alone is
 // latency-bound and fastest with a whole wave per item; a batch of queries is throughput-bound and
 // fastest with four items per wave, whose chains of dependent loads then overlap.
-__host__ __device__ __forceinline__ uint32_t default_push_lanes(bool batched) { return batched ? 16u : 64u; }
+#ifndef GST_BATCH_PUSH_LANES
+#define GST_BATCH_PUSH_LANES 16
+#endif
+#ifndef GST_SOLO_PUSH_LANES
+#define GST_SOLO_PUSH_LANES 64
+#endif
+__host__ __device__ __forceinline__ uint32_t default_push_lanes(bool batched) {
+    return batched ? (uint32_t)GST_BATCH_PUSH_LANES : (uint32_t)GST_SOLO_PUSH_LANES;
+}
+
+// push_lanes == PUSH_ITEM_PAR selects the item-parallel push round (push_items_body): a wave takes up
+// to 64 items, one per lane, and spreads their (item, mask) pairs and then their (pair, arc)
+// relaxations over its lanes.  Round sizing counts ITEMPAR_PLAN items per wave for it.
+constexpr uint32_t PUSH_ITEM_PAR = 1;
+// relaxations in flight per lane in the item-parallel push round (IP_UNR x 64 per wave)
+#ifndef GST_IP_UNR
+#define GST_IP_UNR 2
+#endif
+constexpr uint32_t IP_UNR = GST_IP_UNR;
+#ifndef GST_ITEMPAR_PLAN
+#define GST_ITEMPAR_PLAN 4
+#endif
+// Levels whose dense rounds take the item-parallel shape when the caller leaves push_lanes to the
+// library (batched engine / one-query solves): 0 = none, 1 = level 1 (the group distances, whose
+// rounds are the densest), 2 = every level.  Measured on BA V=200k g=6 with 128 slots: level 1
+// item-parallel takes the batched grow of 64 spider- and learned-bounded full solves from 245 to
+// 272 GB/s and the spider-bounded stream up 6%, but the learned-bound stream down 6% (1429 vs 1522
+// queries/s), and every level item-parallel loses on both; so the default is none.
+#ifndef GST_BATCH_ITEMPAR
+#define GST_BATCH_ITEMPAR 0
+#endif
+#ifndef GST_SOLO_ITEMPAR
+#define GST_SOLO_ITEMPAR 0
+#endif
+__host__ __device__ __forceinline__ bool itempar_level(int mode, int k) { return mode == 2 || (mode == 1 && k == 1); }
+// The item-parallel shape serves dense rounds: when the library chose it (sparse_lanes != 0), a
+// round of fewer than GST_IP_MIN_ITEMS items, which is latency-bound, runs in lane groups of
+// sparse_lanes lanes per item instead.
+#ifndef GST_IP_MIN_ITEMS
+#define GST_IP_MIN_ITEMS 8192
+#endif
+__host__ __device__ __forceinline__ uint32_t round_push_lanes(uint32_t push_lanes, uint32_t sparse_lanes, uint32_t n_cur) {
+    return (push_lanes == PUSH_ITEM_PAR && sparse_lanes != 0 && n_cur < (uint32_t)GST_IP_MIN_ITEMS) ? sparse_lanes
+                                                                                                  : push_lanes;
+}
+__host__ __device__ __forceinline__ uint32_t push_items_per_wave(uint32_t push_lanes) {
+    return push_lanes == PUSH_ITEM_PAR ? (uint32_t)GST_ITEMPAR_PLAN : 64u / push_lanes;
+}
 
 // round log (GST_TRACE_ROUNDS=1): per round {queued items | pull<<31, workgroups taking part,
 // wall clock at start (block 0), wall clock when the last workgroup finished}
@@ -303,6 +353,45 @@ __device__ __forceinline__ void bq_push(BlockQueue& q, uint64_t* list, uint32_t*
         else list[gb + off + c] = item;
     }
     enq_stat += n;
+}
+
+// Append, per lane, n[j] items (v[j], chunk 0..n[j]-1) for j < N with one reservation for the
+// wave.  Convergent: every lane of the wave calls it.
+template <int N>
+__device__ __forceinline__ void bq_push_multi(BlockQueue& q, uint64_t* list, uint32_t* cnt, const uint32_t (&v)[N],
+                                              const uint32_t (&n)[N], uint32_t& enq_stat) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) tot += n[j];
+    if (__ballot(tot != 0) == 0) return;
+    const uint32_t lane = lane_id();
+    uint32_t incl = tot;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, WAVE);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+    uint32_t b = 0, gb = 0xFFFFFFFFu;
+    if (lane == 0) {
+        b = atomicAdd(&q.n, total);
+        if (b + total > QCAP) {
+            atomicMin(&q.valid, b);
+            gb = atomicAdd(cnt, total);
+        }
+    }
+    b = __shfl(b, 0, WAVE);
+    gb = __shfl(gb, 0, WAVE);
+    uint32_t off = incl - tot;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        for (uint32_t c = 0; c < n[j]; ++c, ++off) {
+            const uint64_t item = (uint64_t)v[j] | ((uint64_t)c << 32);
+            if (gb == 0xFFFFFFFFu) q.items[b + off] = item;
+            else list[gb + off] = item;
+        }
+    }
+    enq_stat += tot;
 }
 
 // Publish the workgroup's queued items (all threads of the block call it).
@@ -504,6 +593,11 @@ __global__ void __launch_bounds__(256) k_tour_table(const uint64_t* T1, const in
 #ifndef GST_CHG_PREREAD
 #define GST_CHG_PREREAD 0
 #endif
+// push rounds: 1 = the atomics of a lane's UNR arcs are issued together (the change bit beside the
+// atomic-min), 0 = each arc's atomic-min, then its change bit, then its queue push, one arc at a time
+#ifndef GST_PUSH_BATCH_ATOMICS
+#define GST_PUSH_BATCH_ATOMICS 1
+#endif
 #ifndef GST_GROW_UNR
 #define GST_GROW_UNR 4
 #endif
@@ -601,11 +695,259 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* flags, const ui
 }
 
 // ---------------------------------------------------------------------------------------
+// grow, item-parallel push round.  A wave takes up to 64 queued items at once, one per lane
+// (item loads, row bounds, change words: one round trip for all of them), spreads their changed
+// masks over its lanes as (item, mask) pairs, up to 64 at a time (own costs and lower bounds: one
+// round trip), keeps the pairs under the bound, and spreads their arcs over its lanes as
+// (pair, arc) relaxations, UNR x 64 at a time (arc records, neighbour states, then the atomics:
+// three round trips).  Each dependent step of the relaxation chain thus serves up to 64
+// independent accesses of the wave, where a lane group per item serves one.
+// ---------------------------------------------------------------------------------------
+// one wave's staging: the pairs of the current batch, then the kept ones
+struct PushWaveLds {
+    uint32_t src[WAVE];    // batch pair q: source lane | bit << 8
+    uint32_t v[WAVE];      // kept pair k: vertex
+    uint32_t rk[WAVE];     // mask rank | (first kept pair of its item) << 31
+    uint32_t cv[WAVE];     // own cost
+    uint32_t e0[WAVE];     // first arc of the item's chunk
+    uint32_t incl[WAVE];   // inclusive prefix of the kept pairs' arc counts
+    uint32_t cut[WAVE];    // arcs of the pair that may still be live (rows run in increasing weight)
+};
+
+// orders the LDS accesses of one wave's lanes (the hardware runs a wave's LDS operations in order;
+// this keeps the compiler from moving them across each other)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool FLT>
+__device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q, PushWaveLds& S, const uint64_t* lcur,
+                                                const uint64_t* ccur, uint64_t* cnxt, uint32_t* inxt, uint64_t* lnxt,
+                                                uint32_t* cntn, uint32_t n_cur, uint32_t gwave, uint32_t nwaves,
+                                                bool prune_lb, uint32_t& st_items, uint32_t& st_arcs,
+                                                uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq,
+                                                uint32_t& st_new, uint32_t& ovf) {
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t W = a.W, m = a.m;
+    // Level 1 skips the bound check of its masks: its bits are set only by its own rounds (and the
+    // members' leaves), for costs <= gmax, which only fall, and its bound is fixed while it grows
+    const bool filter = a.ub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1);
+    // items per wave and step: the round's items spread over every wave taking part
+    const uint32_t ipw = min((uint32_t)WAVE, max(1u, (n_cur + nwaves - 1) / nwaves));
+    for (uint32_t base = gwave * ipw; base < n_cur; base += nwaves * ipw) {
+        const uint32_t i = base + lane;
+        const bool has = lane < ipw && i < n_cur;
+        uint32_t v = 0, e0 = 0, len = 0;
+        if (has) {
+            const uint64_t it = lcur[i];
+            v = (uint32_t)it;
+            const uint32_t c = (uint32_t)(it >> 32);
+            const uint32_t rs = a.row[v], re = a.row[v + 1];
+            e0 = rs + c * a.chunk;
+            len = min(re, e0 + a.chunk) - e0;
+        }
+        st_items += wave_sum32(has ? 1u : 0u);
+        for (uint32_t wd = 0; wd < W; ++wd) {
+            const uint64_t bm = has ? ccur[(uint64_t)v * W + wd] : 0ull;
+            const uint32_t nb = (uint32_t)__popcll(bm);
+            uint32_t pin = nb;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t y = __shfl_up(pin, d, WAVE);
+                if (lane >= (uint32_t)d) pin += y;
+            }
+            const uint32_t tot = __shfl(pin, WAVE - 1, WAVE);
+            const uint32_t pofs = pin - nb;
+            for (uint32_t p0 = 0; p0 < tot; p0 += WAVE) {
+                // batch pairs p0 .. p0+63: each lane writes those of its own changed masks
+                for (uint32_t t = p0 > pofs ? p0 - pofs : 0u; t < nb && pofs + t < p0 + WAVE; ++t)
+                    S.src[pofs + t - p0] = lane | (select_bit(bm, t) << 8);
+                wave_lds_sync();
+                const uint32_t np = min((uint32_t)WAVE, tot - p0);
+                const bool pk = lane < np;
+                const uint32_t src = pk ? S.src[lane] : 0u;
+                const uint32_t sl = src & 0xFFu;
+                const uint32_t pv = __shfl(v, sl, WAVE), pe0 = __shfl(e0, sl, WAVE), plen = __shfl(len, sl, WAVE);
+                const uint32_t prk = wd * 64u + (src >> 8);
+                uint32_t pcv = GST_INF_COST, lb = 0;
+                if (pk) {
+                    pcv = (uint32_t)(a.Tk[(uint64_t)pv * m + prk] >> 32);
+                    if (filter && prune_lb) lb = label_lb<FLT>(a.T1, a.tour, pv, a.g, a.masks[prk], a.lbcap);
+                }
+                bool keep = pk && plen != 0 && pcv != GST_INF_COST;
+                if (filter) keep = keep && pcv <= a.gmax && !reaches_ub<FLT>(pcv, lb, a.ub);
+                const uint64_t kb = __ballot(keep);
+                wave_lds_sync();   // every lane has read its src entry before the next batch's writes
+                if (kb == 0) continue;
+                const uint64_t below = kb & lt;
+                const uint32_t kpos = (uint32_t)__popcll(below);
+                const uint32_t nk = (uint32_t)__popcll(kb);
+                // the pairs of one item are adjacent in batch order: the first kept one counts the
+                // item's arcs (the others read the same records again)
+                const uint32_t qprev = below ? 63u - (uint32_t)__clzll(below) : 0u;
+                const uint32_t prev_sl = __shfl(sl, qprev, WAVE);
+                const bool first = below == 0 || prev_sl != sl;
+                uint32_t ain = keep ? plen : 0u;
+#pragma unroll
+                for (int d = 1; d < WAVE; d <<= 1) {
+                    const uint32_t y = __shfl_up(ain, d, WAVE);
+                    if (lane >= (uint32_t)d) ain += y;
+                }
+                const uint32_t tot_arcs = __shfl(ain, WAVE - 1, WAVE);
+                if (keep) {
+                    S.v[kpos] = pv;
+                    S.rk[kpos] = prk | (first ? 0x80000000u : 0u);
+                    S.cv[kpos] = pcv;
+                    S.e0[kpos] = pe0;
+                    S.incl[kpos] = ain;
+                    S.cut[kpos] = plen;
+                }
+                wave_lds_sync();
+                for (uint32_t r0 = 0; r0 < tot_arcs; r0 += IP_UNR * WAVE) {
+                    uint32_t uu[IP_UNR], wt[IP_UNR], xv[IP_UNR], xrk[IP_UNR], xcv[IP_UNR], xp[IP_UNR], xoff[IP_UNR];
+                    uint2 rec[IP_UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        const uint32_t r = r0 + j * WAVE + lane;
+                        xp[j] = 0xFFFFFFFFu;
+                        xoff[j] = 0;
+                        rec[j] = make_uint2(0xFFFFFFFFu, 0u);
+                        xv[j] = 0; xrk[j] = 0; xcv[j] = GST_INF_COST;
+                        if (r < tot_arcs) {
+                            // the pair of relaxation r: the first kept pair whose prefix exceeds r
+                            uint32_t lo = 0, hi = nk - 1;
+                            while (lo < hi) {
+                                const uint32_t mid = (lo + hi) >> 1;
+                                if (S.incl[mid] > r) hi = mid; else lo = mid + 1;
+                            }
+                            const uint32_t off = r - (lo ? S.incl[lo - 1] : 0u);
+                            if (off < S.cut[lo]) {
+                                xp[j] = lo;
+                                xoff[j] = off;
+                                const uint32_t rkf = S.rk[lo];
+                                xv[j] = S.v[lo];
+                                xrk[j] = rkf & 0x7FFFFFFFu;
+                                xcv[j] = S.cv[lo];
+                                st_arcs += rkf >> 31;
+                                rec[j] = a.arc[S.e0[lo] + off];
+                            }
+                        }
+                    }
+                    // arcs of a pair that the bound cuts: every heavier arc of the row is cut too
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        uu[j] = 0xFFFFFFFFu;
+                        wt[j] = rec[j].y;
+                        if (xp[j] == 0xFFFFFFFFu) continue;
+                        if (a.ub == GST_INF_COST || !reaches_ub<FLT>(xcv[j], wt[j], a.ub)) uu[j] = rec[j].x;
+                        else atomicMin(&S.cut[xp[j]], xoff[j]);
+                    }
+                    // the neighbour's state and its change word, loaded together: a bit already set
+                    // in this round needs no atomic OR (bits of the next round's words are only ever
+                    // set during this round, so a stale read can miss a bit but never show one)
+                    unsigned long long cw0[IP_UNR], cwp[IP_UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + xrk[j]) : 0ull;
+                        cwp[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)&cnxt[(uint64_t)uu[j] * W + (xrk[j] >> 6)] : 0ull;
+                    }
+                    uint32_t cand[IP_UNR];
+                    uint32_t mpos = 0, mzero = 0, mgrow = 0;
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        cand[j] = GST_INF_COST;
+                        if (uu[j] == 0xFFFFFFFFu) continue;
+                        st_relax++;
+                        cand[j] = add_cost<FLT>(xcv[j], wt[j], ovf);
+                        const unsigned long long curw = cw0[j];
+                        if (cand[j] == GST_INF_COST || cand[j] > (uint32_t)(curw >> 32)) continue;
+                        if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand[j], 0u, a.ub)) continue;
+                        if ((((unsigned long long)cand[j] << 32) | xv[j]) >= curw) continue;
+                        if (cand[j] == xcv[j]) mzero |= 1u << j; else mpos |= 1u << j;
+                        if (cand[j] <= a.gmax && !((cwp[j] >> (xrk[j] & 63u)) & 1ull)) mgrow |= 1u << j;
+                    }
+                    // the atomics of all IP_UNR relaxations back to back; the change bit beside the
+                    // atomic-min (a bit set for a min that lost to a concurrent lower cost only
+                    // makes the next round relax the state from the cost it holds)
+                    const uint32_t mbit = mpos & mgrow;
+                    unsigned long long ob[IP_UNR], old[IP_UNR];
+                    uint32_t ru0[IP_UNR], ru1[IP_UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        ob[j] = 0ull;
+                        ru0[j] = ru1[j] = 0u;
+                        if ((mbit >> j) & 1u) {
+                            ru0[j] = a.row[uu[j]];
+                            ru1[j] = a.row[uu[j] + 1];
+                            ob[j] = atomicOr((unsigned long long*)&cnxt[(uint64_t)uu[j] * W + (xrk[j] >> 6)],
+                                             1ull << (xrk[j] & 63u));
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        old[j] = 0ull;
+                        if ((mpos >> j) & 1u)
+                            old[j] = atomicMin((unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + xrk[j]),
+                                               ((unsigned long long)cand[j] << 32) | xv[j]);
+                    }
+                    uint32_t enq_n[IP_UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < IP_UNR; ++j) {
+                        enq_n[j] = 0u;
+                        if (((mpos >> j) & 1u) && cand[j] < (uint32_t)(old[j] >> 32)) {
+                            st_upd++;
+                            st_new += mark_active(a.act, uu[j], old[j]);
+                        }
+                        const uint64_t bitv = 1ull << (xrk[j] & 63u);
+                        if (((mbit >> j) & 1u) &&
+                            ((W == 1) ? (ob[j] == 0ull) : (!(ob[j] & bitv) && atomicExch(&inxt[uu[j]], 1u) == 0u)))
+                            enq_n[j] = (ru1[j] - ru0[j] + a.chunk - 1) / a.chunk;
+                    }
+                    if (mzero) {
+                        // zero-effect edges: strict cost decreases only, by compare-and-swap, so that
+                        // equal-cost predecessors can never close a cycle; the bit after a success
+#pragma unroll
+                        for (uint32_t j = 0; j < IP_UNR; ++j) {
+                            if (!((mzero >> j) & 1u)) continue;
+                            const uint32_t u = uu[j];
+                            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + xrk[j]);
+                            const unsigned long long nw = ((unsigned long long)cand[j] << 32) | xv[j];
+                            unsigned long long curw = cw0[j];
+                            bool improved = false;
+                            while (cand[j] < (uint32_t)(curw >> 32)) {
+                                const unsigned long long o = atomicCAS(addr, curw, nw);
+                                if (o == curw) { improved = true; st_new += mark_active(a.act, u, o); break; }
+                                curw = o;
+                            }
+                            if (!improved) continue;
+                            st_upd++;
+                            if (cand[j] > a.gmax) continue;
+                            const uint64_t bitv = 1ull << (xrk[j] & 63u);
+                            const uint32_t r0u = a.row[u], r1u = a.row[u + 1];
+                            const unsigned long long o2 =
+                                atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + (xrk[j] >> 6)], bitv);
+                            if ((W == 1) ? (o2 == 0ull) : (!(o2 & bitv) && atomicExch(&inxt[u], 1u) == 0u))
+                                enq_n[j] = (r1u - r0u + a.chunk - 1) / a.chunk;
+                        }
+                    }
+                    bq_push_multi<IP_UNR>(q, lnxt, cntn, uu, enq_n, st_enq);
+                }
+                wave_lds_sync();   // the kept pairs are read before the next batch rewrites them
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // grow: one round of frontier relaxation for all masks of one level
 // ---------------------------------------------------------------------------------------
 template <bool FLT>
 __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid, uint32_t nblk, uint32_t seq) {
     __shared__ BlockQueue q;
+    __shared__ PushWaveLds s_pw[GROW_WAVES];
     __shared__ unsigned long long s_stats[ST_NSTAT];
     __shared__ uint32_t s_ovf;
     RoundCtl* ctl = a.ctl;
@@ -625,7 +967,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     // only as many workgroups as the round needs take part (a sparse round costs a few blocks)
     uint32_t part = nblk;
     if (!pull) {
-        const uint32_t per_blk = GROW_WAVES * ((uint32_t)WAVE / a.push_lanes);   // items per workgroup step
+        const uint32_t per_blk = GROW_WAVES * push_items_per_wave(round_push_lanes(a.push_lanes, a.push_lanes_sparse, n_cur));   // items per workgroup step
         const uint32_t need_b = (n_cur + per_blk - 1) / per_blk;
         const uint32_t need_a = (n_prv + GROW_BLOCK - 1) / GROW_BLOCK;
         part = max(1u, min(nblk, max(need_b, need_a)));
@@ -861,9 +1203,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     // neighbours.  A wave serves PG items at once, one group of PL lanes each, so that the chains
     // of dependent loads of several items are in flight together; inside a group the lanes are
     // (mask, arc slot) pairs over blocks of up to PL masks. ----
-    const uint32_t PL = a.push_lanes, PG = WAVE / PL;   // PL: 8, 16, 32 or 64
+    const uint32_t rpl = round_push_lanes(a.push_lanes, a.push_lanes_sparse, n_cur);
+    const bool item_par = rpl == PUSH_ITEM_PAR;
+    if (!pull && item_par)
+        push_items_body<FLT>(a, q, s_pw[threadIdx.x / WAVE], lcur, ccur, cnxt, inxt, lnxt, cntn, n_cur, gwave, nwaves,
+                             prune_lb, st_items, st_arcs, st_relax, st_upd, st_enq, st_new, ovf);
+    const uint32_t PL = item_par ? (uint32_t)WAVE : rpl, PG = WAVE / PL;   // PL: 8, 16, 32 or 64
     const uint32_t pgrp = lane / PL, pl = lane % PL;
-    for (uint32_t base = gwave * PG; !pull && base < n_cur; base += nwaves * PG) {
+    for (uint32_t base = gwave * PG; !pull && !item_par && base < n_cur; base += nwaves * PG) {
         const uint32_t i = base + pgrp;
         const bool has = i < n_cur;
         uint32_t v = 0, e0 = 0, e1 = 0;
@@ -927,6 +1274,92 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j)
                         cw0[j] = uu[j] != 0xFFFFFFFFu ? *(const unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk) : 0ull;
+#if GST_PUSH_BATCH_ATOMICS
+                    // Every arc's update is decided first and the atomics of all UNR arcs are then
+                    // issued back to back, so that their round trips overlap instead of queueing
+                    // one behind the other.  A neighbour's change bit is set beside its atomic-min,
+                    // before the outcome is known: when the min loses to a concurrent lower cost,
+                    // the bit only makes the next round relax (u, X) from the cost it holds, which
+                    // is always a valid step; every state that was lowered still gets its bit.
+                    uint32_t cand[UNR];
+                    uint32_t mpos = 0, mzero = 0, mgrow = 0;
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        cand[j] = GST_INF_COST;
+                        if (uu[j] == 0xFFFFFFFFu) continue;
+                        st_relax++;
+                        cand[j] = add_cost<FLT>(cv, wt[j], ovf);
+                        const unsigned long long curw = cw0[j];
+                        if (cand[j] == GST_INF_COST || cand[j] > (uint32_t)(curw >> 32)) continue;
+                        if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand[j], 0u, a.ub)) continue;
+                        if ((((unsigned long long)cand[j] << 32) | v) >= curw) continue;
+                        // cand == cv: an effective zero-weight edge (w == 0, or a float sum that
+                        // rounds back to cv), taken only on a strict cost decrease below
+                        if (cand[j] == cv) mzero |= 1u << j; else mpos |= 1u << j;
+                        if (cand[j] <= a.gmax) mgrow |= 1u << j;
+                    }
+                    const uint32_t mbit = mpos & mgrow;
+                    unsigned long long ob[UNR], old[UNR];
+                    uint32_t ru0[UNR], ru1[UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        ob[j] = 0ull;
+                        ru0[j] = ru1[j] = 0u;
+                        if ((mbit >> j) & 1u) {
+                            ru0[j] = a.row[uu[j]];
+                            ru1[j] = a.row[uu[j] + 1];
+                            ob[j] = atomicOr((unsigned long long*)&cnxt[(uint64_t)uu[j] * W + wsel], (unsigned long long)bitv);
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        old[j] = 0ull;
+                        if ((mpos >> j) & 1u)
+                            old[j] = atomicMin((unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk),
+                                               ((unsigned long long)cand[j] << 32) | v);
+                    }
+                    uint32_t enq_n[UNR];
+#pragma unroll
+                    for (uint32_t j = 0; j < UNR; ++j) {
+                        enq_n[j] = 0u;
+                        if (((mpos >> j) & 1u) && cand[j] < (uint32_t)(old[j] >> 32)) {
+                            st_upd++;
+                            st_new += mark_active(a.act, uu[j], old[j]);
+                        }
+                        // first change of u in this round: queue it (one word per vertex: the word
+                        // itself was empty; several words: u's queue flag)
+                        if (((mbit >> j) & 1u) &&
+                            ((W == 1) ? (ob[j] == 0ull) : (!(ob[j] & bitv) && atomicExch(&inxt[uu[j]], 1u) == 0u)))
+                            enq_n[j] = (ru1[j] - ru0[j] + a.chunk - 1) / a.chunk;
+                    }
+                    if (mzero) {
+                        // zero-effect edges: strict cost decreases only, by compare-and-swap, so that
+                        // equal-cost predecessors can never close a cycle; the bit after a success
+#pragma unroll
+                        for (uint32_t j = 0; j < UNR; ++j) {
+                            if (!((mzero >> j) & 1u)) continue;
+                            const uint32_t u = uu[j];
+                            unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
+                            const unsigned long long nw = ((unsigned long long)cand[j] << 32) | v;
+                            unsigned long long curw = cw0[j];
+                            bool improved = false;
+                            while (cand[j] < (uint32_t)(curw >> 32)) {
+                                const unsigned long long o = atomicCAS(addr, curw, nw);
+                                if (o == curw) { improved = true; st_new += mark_active(a.act, u, o); break; }
+                                curw = o;
+                            }
+                            if (!improved) continue;
+                            st_upd++;
+                            if (cand[j] > a.gmax) continue;
+                            const uint32_t r0 = a.row[u], r1 = a.row[u + 1];
+                            const unsigned long long o2 = atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + wsel],
+                                                                   (unsigned long long)bitv);
+                            if ((W == 1) ? (o2 == 0ull) : (!(o2 & bitv) && atomicExch(&inxt[u], 1u) == 0u))
+                                enq_n[j] = (r1 - r0 + a.chunk - 1) / a.chunk;
+                        }
+                    }
+                    bq_push_multi<UNR>(q, lnxt, cntn, uu, enq_n, st_enq);
+#else
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j) {
                         uint32_t enq_v = 0, enq_n = 0;
@@ -975,6 +1408,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         }
                         bq_push(q, lnxt, cntn, enq_v, enq_n, st_enq);
                     }
+#endif
                 }
             }
         }

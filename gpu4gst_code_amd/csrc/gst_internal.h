@@ -118,7 +118,8 @@ inline gst_bound_bits gst_make_bound(double U, bool flt, int32_t half_bound) {
     return b;
 }
 
-inline bool gst_push_lanes_ok(int32_t p) { return p == 0 || p == 8 || p == 16 || p == 32 || p == 64; }
+// 1 selects the item-parallel push round (one item per lane); 8..64 lanes per item otherwise
+inline bool gst_push_lanes_ok(int32_t p) { return p == 0 || p == 1 || p == 8 || p == 16 || p == 32 || p == 64; }
 
 struct gst_query_bufs;  // device buffers of one query (defined in gst_dpbf.hip)
 

@@ -180,7 +180,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     opts.want_tree = 1;
     if (opts_in) opts = *opts_in;
     if (!gst_push_lanes_ok(opts.push_lanes))
-        return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
+        return gst_set_error(ctx, GST_INVALID_ARG, "push_lanes must be 0, 1, 8, 16, 32 or 64 (got %d)", opts.push_lanes);
     if (opts.half_bound != 0 && opts.half_bound != -1)
         return gst_set_error(ctx, GST_INVALID_ARG, "half_bound must be 0 or -1 (got %d)", opts.half_bound);
     if (opts.tour_bound != 0 && opts.tour_bound != -1)
@@ -305,6 +305,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
         ga.pull_thresh = pm < 0 ? 0xFFFFFFFFu : (uint32_t)std::max<uint64_t>(1, n_items_total * (uint64_t)pm / 1000);
         ga.push_lanes = opts.push_lanes > 0 ? (uint32_t)opts.push_lanes : default_push_lanes(false);
+        ga.push_lanes_sparse = 0;
         uint32_t seq = 0;
         for (int it = 0; it < 100000; ++it) {
             for (int rr = 0; rr < batch; ++rr) {

@@ -76,7 +76,8 @@ typedef struct {
     int32_t grow_mode;         /* 0 = tuned (pull/push rounds), 1 = rounds only, 2 = asynchronous device work
                                   queue for every level, 3 = rounds while the frontier is dense, then the
                                   work queue */
-    int32_t push_lanes;        /* lanes per frontier item in push rounds: 8, 16, 32 or 64 (0 = tuned) */
+    int32_t push_lanes;        /* lanes per frontier item in push rounds: 8, 16, 32 or 64; 1 = one item per lane, its
+                                  masks and arcs spread over the wave (0 = tuned) */
     int32_t half_bound;        /* with a bound U: 0 = grow only states of cost <= U/2 (exact: every optimal
                                   tree splits at a point of its weighted centre into parts of weight
                                   <= OPT/2, whose states are then all exact, and merges plus one last

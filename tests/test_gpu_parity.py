@@ -213,3 +213,38 @@ def test_work_queue_float_and_zero_weights(solver):
         r = solver.solve(dg, groups, grow_mode=2)
         _check_tree(G0, groups, r, oracle.dpbf_cost(G0, groups))
     dg.free()
+
+
+@pytest.mark.parametrize("lanes", [1, 8, 32])
+def test_push_round_shapes_agree(solver, lanes):
+    """Every push-round shape gives the oracle's cost and a valid tree: one item per lane with its
+    masks and arcs spread over the wave (lanes = 1) or a lane group per item (8 or 32 lanes);
+    unbounded and bounded (U = OPT with the half rule, and the automatic spider bound), one query
+    at a time and batched, on int weights with g = 3..8, zero-weight edges and float weights."""
+    G = gst.gen_barabasi_albert(5000, 4, 1, 100, seed=31)
+    dg = solver.upload(G)
+    qs = [gst.gen_groups(G.n, g, 3, 40, seed=60 + g) for g in (3, 5, 6, 8)]
+    want = [oracle.dpbf_cost(G, q) for q in qs]
+    for q, w in zip(qs, want):
+        for kw in ({}, {"ub": w}, {"auto_bound": True}):
+            r = solver.solve(dg, q, push_lanes=lanes, pull_permille=-1, **kw)
+            print(f"lanes {lanes} g {q.g} {kw}: gpu {r.cost} oracle {w}", flush=True)
+            _check_tree(G, q, r, w)
+    res, _ = solver.solve_batch(dg, qs + qs, ubs=[None] * len(qs) + want, push_lanes=lanes, slots=5)
+    for i, r in enumerate(res):
+        _check_tree(G, qs[i % len(qs)], r, want[i % len(qs)])
+    dg.free()
+    G0 = gst.gen_random_connected(400, 1600, 0, 3, seed=71)
+    q0 = gst.gen_groups(G0.n, 5, 2, 10, seed=81)
+    dg0 = solver.upload(G0)
+    w0 = oracle.dpbf_cost(G0, q0)
+    for kw in ({}, {"auto_bound": True}):
+        _check_tree(G0, q0, solver.solve(dg0, q0, push_lanes=lanes, pull_permille=-1, **kw), w0)
+    dg0.free()
+    Gf = gst.gen_rmat(12, 8, weights="lognormal", seed=21)
+    qf = gst.gen_groups(Gf.n, 5, 2, 30, seed=52)
+    dgf = solver.upload(Gf)
+    wf = oracle.dpbf_cost(Gf, qf)
+    for kw in ({}, {"auto_bound": True}):
+        _check_tree(Gf, qf, solver.solve(dgf, qf, push_lanes=lanes, pull_permille=-1, **kw), wf, rel=1e-6)
+    dgf.free()
