@@ -81,6 +81,20 @@ def copy_bandwidth(mib):
     return out.value
 
 
+def random_rates(table_mib, n):
+    """Billions of random 8-byte gathers and of random 64-bit atomic-mins per second over a device table
+    of table_mib MiB (gpu4gst_code_amd/csrc/gst_calib.hip): the ceiling of scattered word accesses."""
+    import ctypes
+    calib = ctypes.CDLL(os.path.join(ROOT, "gpu4gst_code_amd", "libgst_calib.so"))
+    calib.gst_calib_random_gops.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    g, a = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    rc = calib.gst_calib_random_gops(0, table_mib << 20, n, 3, ctypes.byref(g), ctypes.byref(a))
+    if rc != 0:
+        raise SystemExit(f"bench.py: random-access calibration failed ({rc})")
+    return g.value, a.value
+
+
 def cpu_baselines(G, sample, plain_costs, threads):
     """The oracle's label-setting DPBF and its bounded search (spider bound, PrunedDP++'s one-label
     and tour lower bounds, grow <= U/2, feasible tree at every pop with the exact early exit), each
@@ -161,6 +175,9 @@ def main():
     model = gst.Model(solver)
     model.load(args.model)
     copy_gbs = copy_bandwidth(args.copy_mib) if args.copy_mib > 0 else None
+    # random 8-byte gathers and 64-bit atomics over a 4 GiB table (larger than the 256 MiB last-level
+    # cache, as the engine's 64 state tables are): the rates that bound the grow's scattered accesses
+    rnd_gather, rnd_atomic = random_rates(4096, 64 << 20)
     use_graph = args.engine == "graph"
     log(f"graph ready; copy {copy_gbs:.0f} GB/s" if copy_gbs else "graph ready")
 
@@ -223,7 +240,8 @@ def main():
     for i, r in enumerate(kres):
         assert r.cost == plain[i].cost, (i, r.cost, plain[i].cost)
     kcnt = {k: int(sum(r.stats[k] for r in kres)) for k in
-            ("grow_items", "grow_arcs", "grow_relax", "grow_updates", "grow_enqueued", "grow_new_states")}
+            ("grow_items", "grow_arcs", "grow_relax", "grow_updates", "grow_enqueued", "grow_new_states",
+             "grow_atomics")}
     glaunch = max(int(kst["grow_launches"]), 1)
     mlaunch = max(int(kst["merge_launches"]), 1)
     b_grow_gbs = gbs(kst["grow_bytes"], kst["t_grow_ms"])
@@ -292,6 +310,7 @@ def main():
                              "in slots from the start, built-in engine knobs, HIP events around every launch on "
                              "the engine stream",
                      "pass_counts": kcnt},
+        "random_access": random_access_roof(kcnt, kst["t_grow_ms"], rnd_gather, rnd_atomic),
         "merge_roofline": {"bound": "hbm", "achieved": round(b_merge_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(b_merge_gbs / HBM_PEAK_GBS, 5), "kernel": "k_batch_merge",
                            "launches": mlaunch, "bytes_per_launch": round(kst["merge_bytes"] / mlaunch, 1),
@@ -308,6 +327,20 @@ def main():
     print(json.dumps(line), flush=True)
     model.free()
     solver.close()
+
+
+def random_access_roof(cnt, grow_ms, gather_gops, atomic_gops):
+    """The grow kernel against the random-access ceiling: its scattered accesses (a lower count: one
+    neighbour-state gather per relaxation, three gathers per work item for its row bounds, change word
+    and own cost; every device-scope atomic, counted by the kernel) at the calibrated rates take
+    op_ms; frac = op_ms over the kernel's measured time."""
+    gathers = cnt["grow_relax"] + 3 * cnt["grow_items"]
+    atomics = cnt["grow_atomics"]
+    op_ms = (gathers / (gather_gops * 1e9) + atomics / (atomic_gops * 1e9)) * 1e3
+    return {"kernel": "k_batch_grow", "gather_peak_gops": round(gather_gops, 2), "atomic_peak_gops": round(atomic_gops, 2),
+            "table": "4 GiB, uniform random words", "gathers": int(gathers), "atomics": int(atomics),
+            "op_ms": round(op_ms, 3), "kernel_ms": round(grow_ms, 3), "frac": round(op_ms / grow_ms, 4) if grow_ms > 0 else None,
+            "pass": "eager kernel pass"}
 
 
 def _rates(results, key_ms):

@@ -50,7 +50,8 @@ class gst_stats(C.Structure):
                 ("merge_pairs", C.c_uint64), ("grow_bytes", C.c_uint64), ("merge_bytes", C.c_uint64),
                 ("grow_rounds", C.c_uint32), ("grow_launches", C.c_uint32), ("merge_launches", C.c_uint32),
                 ("async_launches", C.c_uint32), ("t_total_ms", C.c_double), ("t_grow_ms", C.c_double),
-                ("t_merge_ms", C.c_double), ("t_trace_ms", C.c_double), ("grow_new_states", C.c_uint64)]
+                ("t_merge_ms", C.c_double), ("t_trace_ms", C.c_double), ("grow_new_states", C.c_uint64),
+                ("grow_atomics", C.c_uint64)]
 
 
 class gst_result(C.Structure):

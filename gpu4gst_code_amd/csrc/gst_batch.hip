@@ -1104,7 +1104,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
             res->used_ub = has_ub ? 1 : 0;
             gst_stats& S2 = res->stats;
             S2.grow_items = hs[ST_ITEMS]; S2.grow_arcs = hs[ST_ARCS]; S2.grow_relax = hs[ST_RELAX];
-            S2.grow_updates = hs[ST_UPD]; S2.grow_enqueued = hs[ST_ENQ]; S2.grow_new_states = hs[ST_NEW];
+            S2.grow_updates = hs[ST_UPD]; S2.grow_enqueued = hs[ST_ENQ]; S2.grow_new_states = hs[ST_NEW]; S2.grow_atomics = hs[ST_ATOM];
             S2.merge_vertices = hs[ST_MVERT]; S2.merge_pairs = hs[ST_MPAIRS];
             S2.grow_rounds = rounds;
             S2.grow_bytes = gst_grow_bytes(S2);

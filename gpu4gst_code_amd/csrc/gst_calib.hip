@@ -81,3 +81,76 @@ extern "C" int gst_calib_copy_gbs(int device, uint64_t bytes, int iters, double*
     (void)hipFree(b);
     return rc;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Random-access rates: the ceiling of kernels whose accesses are scattered 8-byte words (the DP's
+// grow rounds).  A table of `table_bytes` (rounded down to a power of two of u64 words) is accessed
+// at `n` pseudo-random word indices (splitmix64 of the access number masked to the table size, so
+// every index is in the table): 8-byte gathers, then 64-bit atomic-min (device scope).  Each access touches its own
+// 64-byte line, so these rates, not bytes per second, bound such a kernel.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t calib_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+// words: a power of two
+__global__ void __launch_bounds__(256) gst_calib_gather_kernel(const unsigned long long* __restrict__ t, uint64_t words,
+                                                               uint64_t n, unsigned long long* sink) {
+    unsigned long long acc = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        acc += t[calib_mix(i) & (words - 1)];
+    if (acc == 0x1234567ull) sink[0] = acc;   // keeps the loads; never true for the filled table
+}
+__global__ void __launch_bounds__(256) gst_calib_atomic_kernel(unsigned long long* t, uint64_t words, uint64_t n,
+                                                               unsigned long long* sink) {
+    unsigned long long acc = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        acc += atomicMin(&t[calib_mix(i) & (words - 1)], (unsigned long long)i);
+    if (acc == 0x1234567ull) sink[0] = acc;
+}
+__global__ void __launch_bounds__(256) gst_calib_fill_kernel(unsigned long long* t, uint64_t words) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        t[i] = ~0ull;
+}
+
+// *gather_gops, *atomic_gops: billions of random 8-byte gathers / 64-bit atomic-mins per second
+// over a table of table_bytes, best of `reps` timed launches of n accesses each.  Returns 0 on
+// success.
+extern "C" int gst_calib_random_gops(int device, uint64_t table_bytes, uint64_t n, int reps, double* gather_gops,
+                                     double* atomic_gops) {
+    if (!gather_gops || !atomic_gops || reps < 1 || n == 0 || table_bytes < 8 || hipSetDevice(device) != hipSuccess)
+        return 1;
+    uint64_t words = 1;
+    while (words * 2 <= table_bytes / 8) words *= 2;
+    unsigned long long *t = nullptr, *sink = nullptr;
+    if (hipMalloc(&t, words * 8) != hipSuccess) return 2;
+    if (hipMalloc(&sink, 64) != hipSuccess) { (void)hipFree(t); return 2; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = 3;
+    *gather_gops = *atomic_gops = 0.0;
+    for (int kind = 0; kind < 2 && rc == 0; ++kind) {
+        for (int r = 0; r <= reps && rc == 0; ++r) {   // the first launch is untimed
+            gst_calib_fill_kernel<<<2048, 256>>>(t, words);
+            (void)hipEventRecord(e0, nullptr);
+            if (kind == 0) gst_calib_gather_kernel<<<4096, 256>>>(t, words, n, sink);
+            else gst_calib_atomic_kernel<<<4096, 256>>>(t, words, n, sink);
+            (void)hipEventRecord(e1, nullptr);
+            float ms = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.f) {
+                rc = 3;
+                break;
+            }
+            if (r == 0) continue;
+            double& best = kind == 0 ? *gather_gops : *atomic_gops;
+            best = std::max(best, (double)n / (ms * 1e-3) / 1e9);
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(t);
+    (void)hipFree(sink);
+    return rc;
+}

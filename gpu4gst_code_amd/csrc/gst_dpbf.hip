@@ -764,7 +764,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     stat_sum(hst, hs);
     gst_stats& S = res->stats;
     S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
-    S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW];
+    S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW]; S.grow_atomics = hs[ST_ATOM];
     S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.async_launches = async_launches;

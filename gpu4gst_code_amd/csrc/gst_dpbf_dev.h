@@ -36,7 +36,7 @@ struct RoundCtl {
 };
 
 enum StatIdx {
-    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_NEW, ST_MVERT, ST_MPAIRS, ST_MBYTES, ST_NSTAT
+    ST_ITEMS = 0, ST_ARCS, ST_RELAX, ST_UPD, ST_ENQ, ST_NEW, ST_ATOM, ST_MVERT, ST_MPAIRS, ST_MBYTES, ST_NSTAT
 };
 // The counters are striped: workgroup b adds into stripe b % STAT_STRIPES (ST_NSTAT words each), so
 // that the one atomic per workgroup and counter at the end of a launch spreads over several
@@ -728,7 +728,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                                                 uint32_t* cntn, uint32_t n_cur, uint32_t gwave, uint32_t nwaves,
                                                 bool prune_lb, uint32_t& st_items, uint32_t& st_arcs,
                                                 uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq,
-                                                uint32_t& st_new, uint32_t& ovf) {
+                                                uint32_t& st_new, uint32_t& st_atom, uint32_t& ovf) {
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t W = a.W, m = a.m;
@@ -884,14 +884,18 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                             ru1[j] = a.row[uu[j] + 1];
                             ob[j] = atomicOr((unsigned long long*)&cnxt[(uint64_t)uu[j] * W + (xrk[j] >> 6)],
                                              1ull << (xrk[j] & 63u));
+                            st_atom++;
                         }
                     }
 #pragma unroll
                     for (uint32_t j = 0; j < IP_UNR; ++j) {
                         old[j] = 0ull;
                         if ((mpos >> j) & 1u)
+                        {
                             old[j] = atomicMin((unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + xrk[j]),
                                                ((unsigned long long)cand[j] << 32) | xv[j]);
+                            st_atom++;
+                        }
                     }
                     uint32_t enq_n[IP_UNR];
 #pragma unroll
@@ -902,6 +906,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                             st_new += mark_active(a.act, uu[j], old[j]);
                         }
                         const uint64_t bitv = 1ull << (xrk[j] & 63u);
+                        if (((mbit >> j) & 1u) && W > 1 && !(ob[j] & bitv)) st_atom++;
                         if (((mbit >> j) & 1u) &&
                             ((W == 1) ? (ob[j] == 0ull) : (!(ob[j] & bitv) && atomicExch(&inxt[uu[j]], 1u) == 0u)))
                             enq_n[j] = (ru1[j] - ru0[j] + a.chunk - 1) / a.chunk;
@@ -918,6 +923,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                             unsigned long long curw = cw0[j];
                             bool improved = false;
                             while (cand[j] < (uint32_t)(curw >> 32)) {
+                                st_atom++;
                                 const unsigned long long o = atomicCAS(addr, curw, nw);
                                 if (o == curw) { improved = true; st_new += mark_active(a.act, u, o); break; }
                                 curw = o;
@@ -929,6 +935,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                             const uint32_t r0u = a.row[u], r1u = a.row[u + 1];
                             const unsigned long long o2 =
                                 atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + (xrk[j] >> 6)], bitv);
+                            st_atom += (W > 1 && !(o2 & bitv)) ? 2u : 1u;
                             if ((W == 1) ? (o2 == 0ull) : (!(o2 & bitv) && atomicExch(&inxt[u], 1u) == 0u))
                                 enq_n[j] = (r1u - r0u + a.chunk - 1) / a.chunk;
                         }
@@ -999,7 +1006,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     __syncthreads();
 
     // phase B: relax the items of round r
-    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, st_new = 0;
+    uint32_t st_items = 0, st_arcs = 0, st_relax = 0, st_upd = 0, st_enq = 0, st_new = 0, st_atom = 0;
     uint32_t ovf = 0;
     const uint32_t gwave = bid * GROW_WAVES + (threadIdx.x / WAVE);
     const uint32_t nwaves = part * GROW_WAVES;
@@ -1091,6 +1098,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         // single writer of u's change word in a pull round; the flag marks the
                         // first masks block that changed
                         cnxt[(uint64_t)u * W + (mb >> 6)] = word;
+                        st_atom += W > 1 ? 1u : 0u;
                         const bool first = (W == 1) ? true : (atomicExch(&inxt[u], 1u) == 0u);
                         if (first) { enq_v = u; enq_n = (deg + a.chunk - 1) / a.chunk; }
                     }
@@ -1172,12 +1180,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const uint32_t cz = (uint32_t)(bz >> 32);
                     unsigned long long curw = own;
                     while (cz < (uint32_t)(curw >> 32)) {
+                        st_atom++;
                         const unsigned long long old = atomicCAS(addr, curw, bz);
                         if (old == curw) { improved = true; st_new += mark_active(a.act, u, old); break; }
                         curw = old;
                     }
                     ncost = cz;
                 } else if (bs < own) {
+                    st_atom++;
                     const unsigned long long old = atomicMin(addr, bs);
                     improved = (uint32_t)(bs >> 32) < (uint32_t)(old >> 32);
                     if (improved) st_new += mark_active(a.act, u, old);
@@ -1188,6 +1198,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const uint64_t bitv = 1ull << (rk & 63);
                     unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + (rk >> 6)];
                     const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
+                    st_atom += (W > 1 && !(ob & bitv)) ? 2u : 1u;
                     const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
                     if (first) {
                         enq_v = u;
@@ -1207,7 +1218,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     const bool item_par = rpl == PUSH_ITEM_PAR;
     if (!pull && item_par)
         push_items_body<FLT>(a, q, s_pw[threadIdx.x / WAVE], lcur, ccur, cnxt, inxt, lnxt, cntn, n_cur, gwave, nwaves,
-                             prune_lb, st_items, st_arcs, st_relax, st_upd, st_enq, st_new, ovf);
+                             prune_lb, st_items, st_arcs, st_relax, st_upd, st_enq, st_new, st_atom, ovf);
     const uint32_t PL = item_par ? (uint32_t)WAVE : rpl, PG = WAVE / PL;   // PL: 8, 16, 32 or 64
     const uint32_t pgrp = lane / PL, pl = lane % PL;
     for (uint32_t base = gwave * PG; !pull && !item_par && base < n_cur; base += nwaves * PG) {
@@ -1309,14 +1320,17 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                             ru0[j] = a.row[uu[j]];
                             ru1[j] = a.row[uu[j] + 1];
                             ob[j] = atomicOr((unsigned long long*)&cnxt[(uint64_t)uu[j] * W + wsel], (unsigned long long)bitv);
+                            st_atom++;
                         }
                     }
 #pragma unroll
                     for (uint32_t j = 0; j < UNR; ++j) {
                         old[j] = 0ull;
-                        if ((mpos >> j) & 1u)
+                        if ((mpos >> j) & 1u) {
                             old[j] = atomicMin((unsigned long long*)(a.Tk + (uint64_t)uu[j] * m + rk),
                                                ((unsigned long long)cand[j] << 32) | v);
+                            st_atom++;
+                        }
                     }
                     uint32_t enq_n[UNR];
 #pragma unroll
@@ -1328,6 +1342,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         }
                         // first change of u in this round: queue it (one word per vertex: the word
                         // itself was empty; several words: u's queue flag)
+                        if (((mbit >> j) & 1u) && W > 1 && !(ob[j] & bitv)) st_atom++;
                         if (((mbit >> j) & 1u) &&
                             ((W == 1) ? (ob[j] == 0ull) : (!(ob[j] & bitv) && atomicExch(&inxt[uu[j]], 1u) == 0u)))
                             enq_n[j] = (ru1[j] - ru0[j] + a.chunk - 1) / a.chunk;
@@ -1344,6 +1359,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                             unsigned long long curw = cw0[j];
                             bool improved = false;
                             while (cand[j] < (uint32_t)(curw >> 32)) {
+                                st_atom++;
                                 const unsigned long long o = atomicCAS(addr, curw, nw);
                                 if (o == curw) { improved = true; st_new += mark_active(a.act, u, o); break; }
                                 curw = o;
@@ -1354,6 +1370,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                             const uint32_t r0 = a.row[u], r1 = a.row[u + 1];
                             const unsigned long long o2 = atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + wsel],
                                                                    (unsigned long long)bitv);
+                            st_atom += (W > 1 && !(o2 & bitv)) ? 2u : 1u;
                             if ((W == 1) ? (o2 == 0ull) : (!(o2 & bitv) && atomicExch(&inxt[u], 1u) == 0u))
                                 enq_n[j] = (r1 - r0 + a.chunk - 1) / a.chunk;
                         }
@@ -1378,12 +1395,14 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                         // effective zero-weight edge: only strict cost decreases, so
                                         // equal-cost predecessors can never close a cycle
                                         while (cand < (uint32_t)(curw >> 32)) {
+                                            st_atom++;
                                             const unsigned long long old = atomicCAS(addr, curw, nw);
                                             if (old == curw) { improved = true; st_new += mark_active(a.act, u, old); break; }
                                             curw = old;
                                         }
                                     } else {
                                         const unsigned long long old = atomicMin(addr, nw);
+                                        st_atom++;
                                         improved = cand < (uint32_t)(old >> 32);
                                         if (improved) st_new += mark_active(a.act, u, old);
                                     }
@@ -1395,6 +1414,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                     const uint32_t ru0 = a.row[u], ru1 = a.row[u + 1];
                                     if (!GST_CHG_PREREAD || !(*cwp & bitv)) {
                                         const unsigned long long ob = atomicOr(cwp, (unsigned long long)bitv);
+                                        st_atom += (W > 1 && !(ob & bitv)) ? 2u : 1u;
                                         // first change of u in this round: queue it (one word per
                                         // vertex: the word itself was empty; several words: flag)
                                         const bool first = (W == 1) ? (ob == 0) : (!(ob & bitv) && atomicExch(&inxt[u], 1u) == 0u);
@@ -1423,9 +1443,11 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
         const uint32_t su = wave_sum32(st_upd);
         const uint32_t se = wave_sum32(st_enq);
         const uint32_t sn = wave_sum32(st_new);
+        const uint32_t sx = wave_sum32(st_atom);
         const uint32_t so = wave_sum32(ovf);
         if (lane == 0) {
             if (sn) atomicAdd(&s_stats[ST_NEW], (unsigned long long)sn);
+            if (sx) atomicAdd(&s_stats[ST_ATOM], (unsigned long long)sx);
             if (si) atomicAdd(&s_stats[ST_ITEMS], (unsigned long long)si);
             if (sa) atomicAdd(&s_stats[ST_ARCS], (unsigned long long)sa);
             if (sr) atomicAdd(&s_stats[ST_RELAX], (unsigned long long)sr);

@@ -118,6 +118,9 @@ typedef struct {
     double   t_trace_ms;       /* device time of the traceback */
     uint64_t grow_new_states;  /* grow updates that made an infinite state finite (the others lowered a finite
                                   cost again: grow_updates / grow_new_states is the rounds' redundancy) */
+    uint64_t grow_atomics;     /* device-scope atomic operations of the grow kernels (min, compare-and-swap,
+                                  change-bit OR, queue-flag exchange): each is a random 64-byte line
+                                  read-modify-write beyond the L2, the grow's costliest operation */
 } gst_stats;
 
 typedef struct {
