@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
 // short chains of dependent phases, run beside the merges instead of before them.  The other
 // workgroups (at least MAX_SLOTS) split among the merges.
 template <bool FLT, bool PW_LDS>
-__global__ void __launch_bounds__(MERGE_THREADS) k_batch_merge(BatchArgs b) {
+__global__ void __launch_bounds__(MERGE_THREADS) __attribute__((amdgpu_waves_per_eu(GST_MERGE_WPE))) k_batch_merge(BatchArgs b) {
     extern __shared__ unsigned long long lds[];
     const BatchPlan* p = b.plan;
     const uint32_t nm = p->n_merge;

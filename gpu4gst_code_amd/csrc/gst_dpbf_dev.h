@@ -1849,15 +1849,30 @@ constexpr int MERGE_UNR = GST_MERGE_UNR;
 #endif
 // split tables up to this size are staged in LDS
 constexpr size_t MERGE_PAIRS_LDS = 16 * 1024;
+// levels of at most this many splits per mask (k <= 4) take their splits from registers; the others
+// read the staged table (the full level, with one mask, as LDS broadcasts: every lane reads the same
+// entry).  Measured on BA V=200k g=6: 1, 4 or 8 give the same merge time, 16 spills registers (-20%)
+#ifndef GST_MERGE_REG_SPLITS
+#define GST_MERGE_REG_SPLITS 8
+#endif
+constexpr uint32_t MERGE_REG_SPLITS = GST_MERGE_REG_SPLITS;
+// merge waves per SIMD the register allocation must allow (three 48 KiB-tile workgroups of 8 waves
+// per CU: 6)
+#ifndef GST_MERGE_WPE
+#define GST_MERGE_WPE 6
+#endif
 
+// A tile's staged rows are VB rows of L 4-byte costs at an odd stride, so that lanes reading the
+// same position of different vertices' rows hit different LDS banks
+__host__ __device__ __forceinline__ uint32_t merge_row_stride(uint32_t L) { return L | 1u; }
 // 8-byte words taken by VB rows of L 4-byte costs
 __host__ __device__ __forceinline__ size_t merge_rows_words(uint32_t VB, uint32_t L) {
-    return ((size_t)VB * L + 1) / 2;
+    return ((size_t)VB * merge_row_stride(L) + 1) / 2;
 }
 // vertices per merge tile for L lower-level costs and W bitmap words per vertex within a budget of
 // tile_bytes (0: the default)
 __host__ __device__ __forceinline__ uint32_t merge_tile_vertices(uint32_t L, uint32_t W, size_t tile_bytes = 0) {
-    const size_t per = (size_t)L * 4 + (size_t)W * 8 + 12;
+    const size_t per = (size_t)merge_row_stride(L) * 4 + (size_t)W * 8 + 12;
     size_t vb = (tile_bytes ? tile_bytes : MERGE_TILE_BYTES) / per;
     if (vb < 1) vb = 1;
     if (vb > GST_MERGE_VB_MAX) vb = GST_MERGE_VB_MAX;
@@ -1881,6 +1896,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     __shared__ uint32_t s_nl;
     const uint32_t k = a.k;
     const uint32_t L = a.off[k];             // lower-level words per vertex
+    const uint32_t Lp = merge_row_stride(L); // their stride in LDS
     const uint32_t mk = a.m[k];
     const uint32_t S = (1u << (k - 1)) - 1;  // splits per mask
     const uint32_t VB = a.VB;
@@ -1924,6 +1940,29 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     const bool skip = a.act != nullptr && k >= 3;
     uint8_t* act_out = (a.act && !a.is_full) ? a.act + (size_t)k * a.n : nullptr;
 
+    // Splits in registers: when a level has at most MERGE_REG_SPLITS splits per mask and at most a
+    // workgroup's worth of masks, thread t serves the (vertex, mask) items t, t + istride, ... with
+    // istride a multiple of the mask count, so its mask is fixed; its splits are enumerated once into
+    // registers as (position of X1, position of X \ X1, X1), 10 bits each (g <= 10), and the split
+    // loop reads only the two staged costs per split.
+    const bool reg_splits = S <= MERGE_REG_SPLITS && mk <= blockDim.x && g <= 10;
+    const uint32_t istride = reg_splits ? (blockDim.x / mk) * mk : blockDim.x;
+    uint32_t sp[MERGE_REG_SPLITS];
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < MERGE_REG_SPLITS; ++s2) sp[s2] = 0u;
+    if (reg_splits) {
+        __syncthreads();   // the split table is staged
+        const unsigned long long* pr = pt + (size_t)(threadIdx.x % mk) * ps_stride;
+#pragma unroll
+        for (uint32_t s2 = 0; s2 < MERGE_REG_SPLITS; ++s2) {
+            if (s2 < S) {
+                const unsigned long long e = pr[(size_t)s2 * pr_stride];
+                sp[s2] = (uint32_t)((e >> 16) & 0x3FFu) | ((uint32_t)((e >> 32) & 0x3FFu) << 10) |
+                         ((uint32_t)(e >> 48) << 20);
+            }
+        }
+    }
+
     for (uint32_t v0 = bid * VB; v0 < a.n; v0 += nblk * VB) {
         const uint32_t nv = min(VB, a.n - v0);
         __syncthreads();
@@ -1963,17 +2002,20 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             const uint32_t cnt = nl * mj;
             const uint32_t oj = a.off[j];
             for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += MERGE_UNR * blockDim.x) {
-                uint64_t x[MERGE_UNR];
+                // only the cost half of each word is read (4-byte loads of the high halves): the
+                // merge reads no predecessors, and each load in flight holds one register
+                const uint32_t* src_hi = (const uint32_t*)src + 1;
+                uint32_t x[MERGE_UNR];
 #pragma unroll
                 for (int u = 0; u < MERGE_UNR; ++u) {
                     const uint32_t i = i0 + u * blockDim.x;
-                    uint64_t val = 0ull;
+                    uint32_t val = 0u;
                     if (i < cnt) {
                         if (skip) {
                             const uint32_t li = div_small(i, inv_mj);
-                            val = src[(uint64_t)lst[li] * mj + (i - li * mj)];
+                            val = src_hi[2 * ((uint64_t)lst[li] * mj + (i - li * mj))];
                         } else {
-                            val = src[i];
+                            val = src_hi[2 * (uint64_t)i];
                         }
                     }
                     x[u] = val;
@@ -1983,7 +2025,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
                     const uint32_t i = i0 + u * blockDim.x;
                     if (i < cnt) {
                         const uint32_t b = div_small(i, inv_mj), t = i - b * mj;
-                        rows[b * L + oj + t] = (uint32_t)(x[u] >> 32);
+                        rows[b * Lp + oj + t] = x[u];
                     }
                 }
             }
@@ -1996,23 +2038,36 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             uint32_t sm = GST_INF_COST;
             for (uint32_t b = threadIdx.x; b < nl; b += blockDim.x) {
                 uint32_t acc = 0, o2 = 0;
-                for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, rows[b * L + gi], o2);
+                for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, rows[b * Lp + gi], o2);
                 sm = min(sm, o2 ? GST_INF_COST : acc);
             }
             sm = wave_min32(sm);
             if (lane_id() == 0 && sm != GST_INF_COST) atomicMin(a.spider, sm);
         }
         const uint32_t items = nl * mk;
-        for (uint32_t i = threadIdx.x; i < items; i += blockDim.x) {
+        for (uint32_t i = threadIdx.x; threadIdx.x < istride && i < items; i += istride) {
             const uint32_t li = div_small(i, inv_mk), r = i - li * mk;
             const uint32_t b = lst[li];
             const uint32_t v = v0 + b;
             const uint32_t X = mk_masks[r];
             const uint32_t gmv = gms[b] & 0x7FFFFFFFu;
-            const uint32_t* rv = rows + (size_t)li * L;
+            const uint32_t* rv = rows + (size_t)li * Lp;
             unsigned long long best = ((X & gmv) == X) ? (unsigned long long)GST_PRED_LEAF : GST_INF_WORD;
             const unsigned long long* pr = pt + (size_t)r * ps_stride;
-            for (uint32_t s2 = 0; s2 < S; ++s2) {
+            if (reg_splits) {
+#pragma unroll
+                for (uint32_t s2 = 0; s2 < MERGE_REG_SPLITS; ++s2) {
+                    if (s2 < S) {
+                        const uint32_t e = sp[s2];
+                        const uint32_t sum = add_cost<FLT>(rv[e & 0x3FFu], rv[(e >> 10) & 0x3FFu], ovf);
+                        if (sum != GST_INF_COST) {
+                            const unsigned long long wv = ((unsigned long long)sum << 32) | (GST_PRED_MERGE | (e >> 20));
+                            best = wv < best ? wv : best;
+                        }
+                    }
+                }
+            }
+            for (uint32_t s2 = 0; !reg_splits && s2 < S; ++s2) {
                 const unsigned long long e = pr[(size_t)s2 * pr_stride];
                 const uint32_t p1 = (uint32_t)((e >> 16) & 0xFFFF);
                 const uint32_t p2 = (uint32_t)((e >> 32) & 0xFFFF);
@@ -2091,7 +2146,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
 }
 
 template <bool FLT>
-__global__ void __launch_bounds__(MERGE_THREADS) k_merge(MergeArgs a) {
+__global__ void __launch_bounds__(MERGE_THREADS) __attribute__((amdgpu_waves_per_eu(GST_MERGE_WPE))) k_merge(MergeArgs a) {
     extern __shared__ unsigned long long lds[];
     merge_body<FLT>(a, lds, blockIdx.x, gridDim.x);
 }
