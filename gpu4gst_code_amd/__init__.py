@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -294,6 +294,22 @@ class Solver:
                           stats)
         finally:
             self.lib.gst_result_free(C.byref(res))
+
+    def mst_postprocess(self, graph: DeviceGraph, vertices) -> Tuple[np.ndarray, np.ndarray, np.ndarray, float]:
+        """The minimum spanning forest of the subgraph induced by `vertices` (gst_mst_postprocess; the
+        reference's graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_vertices): (eu, ev, ew, cost)."""
+        vs = np.ascontiguousarray(np.asarray(vertices, dtype=np.int32))
+        n = max(int(vs.size), 1)
+        eu = np.zeros(n, np.int32)
+        ev = np.zeros(n, np.int32)
+        ew = np.zeros(n, np.float64)
+        ne = C.c_int32(0)
+        cost = C.c_double(0.0)
+        st = self.lib.gst_mst_postprocess(self.ctx, graph.handle, _ptr(vs), int(vs.size), _ptr(eu), _ptr(ev), _ptr(ew),
+                                          C.byref(ne), C.byref(cost))
+        _check(self.lib, st, self.ctx)
+        k = ne.value
+        return eu[:k].copy(), ev[:k].copy(), ew[:k].copy(), cost.value
 
     def solve_noalloc(self, graph: DeviceGraph, groups: Groups, *, ub: Optional[float] = None, prune_lb: bool = True,
                       want_tree: bool = True, edge_cap: int = 16384, auto_bound: bool = False, half_bound: bool = True,

@@ -51,7 +51,7 @@ class gst_stats(C.Structure):
                 ("grow_rounds", C.c_uint32), ("grow_launches", C.c_uint32), ("merge_launches", C.c_uint32),
                 ("async_launches", C.c_uint32), ("t_total_ms", C.c_double), ("t_grow_ms", C.c_double),
                 ("t_merge_ms", C.c_double), ("t_trace_ms", C.c_double), ("grow_new_states", C.c_uint64),
-                ("grow_atomics", C.c_uint64)]
+                ("grow_atomics", C.c_uint64), ("spider_cost", C.c_double), ("spider_tree_cost", C.c_double)]
 
 
 class gst_result(C.Structure):
@@ -111,6 +111,8 @@ SIGNATURES = {
     "gst_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.POINTER(gst_opts),
                             C.POINTER(gst_result)]),
     "gst_result_free": (None, [C.POINTER(gst_result)]),
+    "gst_mst_postprocess": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "gst_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gst_groups), C.POINTER(gst_opts), C.c_int32,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(gst_result)]),
     "gst_solve_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(gst_groups), C.POINTER(gst_opts),

@@ -281,6 +281,21 @@ gst_status gst_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
     return gst_dpbf_solve(ctx, g, groups, opts, result);
 }
 
+gst_status gst_mst_postprocess(gst_ctx* ctx, const gst_graph* g, const int32_t* vertices, int32_t nv, int32_t* eu,
+                               int32_t* ev, double* ew, int32_t* ne, double* cost) {
+    if (!ctx || !g || nv < 1 || !vertices || !eu || !ev || !ew || !ne || !cost)
+        return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "mst_postprocess: null argument or empty set") : GST_INVALID_ARG;
+    *ne = 0;
+    *cost = 0.0;
+    if (!ctx->stream) return gst_set_error(ctx, GST_NO_DEVICE, "context has no device");
+    if (g->ctx != ctx) return gst_set_error(ctx, GST_INVALID_ARG, "graph belongs to another context");
+    if (nv > 4 * 512) return gst_set_error(ctx, GST_INVALID_ARG, "mst_postprocess: too many vertices (%d)", nv);
+    for (int32_t i = 0; i < nv; ++i)
+        if (vertices[i] < 0 || vertices[i] >= g->n)
+            return gst_set_error(ctx, GST_INVALID_ARG, "mst_postprocess: vertex %d out of range", vertices[i]);
+    return gst_dpbf_mst_postprocess(ctx, g, vertices, nv, eu, ev, ew, ne, cost);
+}
+
 gst_status gst_solve_noalloc(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups, const gst_opts* opts,
                              int32_t edge_cap, int32_t* eu, int32_t* ev, double* ew, gst_result* result) {
     if (!ctx || !g || !groups || !result) return ctx ? gst_set_error(ctx, GST_INVALID_ARG, "null argument") : GST_INVALID_ARG;

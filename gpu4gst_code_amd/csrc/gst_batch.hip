@@ -83,7 +83,7 @@ struct FillArgs {
     uint32_t* gm;
     unsigned long long* root;
     unsigned long long* stats;
-    uint32_t* spider;
+    unsigned long long* spider;
     uint32_t* ctl;            // the slot's round-control blocks, as words
     uint32_t* hub_cnt;
     const int32_t* grp;       // grp_ptr[0..g] then the members
@@ -138,7 +138,7 @@ __device__ __forceinline__ uint32_t plan_scan(uint32_t x, uint32_t& tot) {
 // a slot's per-job state before a fill: group bits cleared, root = inf, statistics, spider bound,
 // control blocks and hub item count reset (one launch instead of six)
 __global__ void k_slot_reset(uint32_t* gm, uint32_t n, unsigned long long* root, unsigned long long* stats,
-                             uint32_t* spider, uint32_t* ctl, uint32_t* hub_cnt) {
+                             unsigned long long* spider, uint32_t* ctl, uint32_t* hub_cnt) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t v = tid; v < n; v += gridDim.x * blockDim.x) gm[v] = 0u;
     if (blockIdx.x == 0) {
@@ -146,7 +146,7 @@ __global__ void k_slot_reset(uint32_t* gm, uint32_t n, unsigned long long* root,
         for (uint32_t i = threadIdx.x; i < (uint32_t)STAT_WORDS; i += blockDim.x) stats[i] = 0ull;
         if (threadIdx.x == 0) {
             *root = GST_INF_WORD;
-            *spider = GST_INF_COST;
+            *spider = GST_INF_WORD;
             *hub_cnt = 0u;
         }
     }
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_batch_fill_reset(BatchArgs b) {
         for (uint32_t i = threadIdx.x; i < (uint32_t)STAT_WORDS; i += blockDim.x) f->stats[i] = 0ull;
         if (threadIdx.x == 0) {
             *f->root = GST_INF_WORD;
-            *f->spider = GST_INF_COST;
+            *f->spider = GST_INF_WORD;
             *f->hub_cnt = 0u;
         }
     }
@@ -302,6 +302,21 @@ __global__ void __launch_bounds__(MERGE_THREADS) __attribute__((amdgpu_waves_per
     merge_body<FLT>(b.sb[p->merge_slot[i]].m[p->merge_level[i]], lds, bid - i * per, per);
 }
 
+// after the merges: one workgroup per slot whose level-2 merge found a spider (automatic bound)
+// post-processes the spider's paths into a tree (MST of their vertices, unneeded leaves removed) and
+// lowers the spider's cost to the tree's when cheaper
+template <bool FLT>
+__global__ void __launch_bounds__(MST_THREADS) k_batch_spider_tree(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    const uint32_t i = blockIdx.x;
+    if (i >= p->n_merge || p->merge_level[i] != 2) return;
+    const SlotBlock& sb = b.sb[p->merge_slot[i]];
+    const MergeArgs& m2 = sb.m[2];
+    if (!m2.spider) return;
+    __shared__ MstLds L;
+    spider_tree_body<FLT>(L, m2.T + m2.lvl_base[1], m2.g, m2.gm, sb.g[1].row, sb.g[1].arc, m2.spider, nullptr);
+}
+
 // after the merges: a slot whose level-2 merge found a spider (automatic bound) tightens the bound
 // of its later levels in the argument blocks (read by every later launch)
 template <bool FLT>
@@ -312,7 +327,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_batch_bound(BatchArgs b) {
     const uint32_t s = p->merge_slot[i];
     const MergeArgs& m2 = b.sb[s].m[2];
     if (!m2.spider) return;
-    const uint32_t sp = *m2.spider;
+    const uint32_t sp = (uint32_t)(*m2.spider >> 32);
     if (sp == GST_INF_COST) return;
     uint32_t ub, gmax;
     spider_bound<FLT>(sp, ub, gmax);
@@ -443,7 +458,8 @@ struct gst_batch_engine {
         unsigned long long* stats = nullptr;
         uint32_t* out_n = nullptr;
         uint32_t* edges = nullptr;
-        uint32_t* spider = nullptr;         // cheapest spider of the level-2 merge (automatic bound)
+        unsigned long long* spider = nullptr;   // cheapest spider of the level-2 merge (automatic bound):
+                                                // (cost << 32 | centre), lowered by its post-processed tree
         uint32_t* tour = nullptr;           // tour table 2^g * (g + 1) words, then walk scratch 2^g * g * g
         uint8_t* act = nullptr;             // activity bytes act[k * n + v] of the job's levels
         int tour_g = 0;
@@ -859,6 +875,8 @@ static void enqueue_step_a(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& b
         else k_batch_merge<false, false><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);
     }
     if (ev) hipEventRecord(ev[1], s);
+    if (flt) k_batch_spider_tree<true><<<MAX_SLOTS, MST_THREADS, 0, s>>>(ba);
+    else k_batch_spider_tree<false><<<MAX_SLOTS, MST_THREADS, 0, s>>>(ba);
     if (flt) k_batch_bound<true><<<1, PLAN_THREADS, 0, s>>>(ba);
     else k_batch_bound<false><<<1, PLAN_THREADS, 0, s>>>(ba);
     k_batch_compact<<<mgrid, 256, 0, s>>>(ba);

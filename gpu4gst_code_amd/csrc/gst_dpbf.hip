@@ -51,7 +51,7 @@ struct gst_query_bufs {
     RoundCtl* ctl = nullptr;
     unsigned long long* root = nullptr;
     unsigned long long* stats = nullptr;
-    uint32_t* spider = nullptr;  // cheapest spider (automatic bound)
+    unsigned long long* spider = nullptr;  // [0] spider tree (cost << 32 | centre), [1] cheapest spider
     uint32_t* tour = nullptr;    // tour table, 2^g * (g + 1) words, then the walk scratch 2^g * g * g
     int tour_g = 0;              // group count tour is sized for
     uint32_t* edges = nullptr;   // 3 * cap_edges + 1 (count)
@@ -556,14 +556,29 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         else k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T, n, B->root);
     }
     // ---- automatic bound: the cheapest spider once the group distances are final ----
+    double spider_costs[2] = {0.0, 0.0};   // the spider's sum of distances, its post-processed tree
     if (st == GST_OK && opts.auto_bound && g >= 3) {
-        if (!B->spider && hipMalloc(&B->spider, 8) != hipSuccess) return gst_set_error(ctx, GST_OOM, "solve: allocation failed");
-        k_fill_u32<<<1, 64, 0, s>>>(B->spider, GST_INF_COST, 1);
-        if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], n, (uint32_t)g, B->spider);
-        else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B->T + lvl_base[1], n, (uint32_t)g, B->spider);
-        uint32_t sp = GST_INF_COST;
-        HIPCHK(hipMemcpyAsync(&sp, B->spider, 4, hipMemcpyDeviceToHost, s));
+        if (!B->spider && hipMalloc(&B->spider, 16) != hipSuccess) return gst_set_error(ctx, GST_OOM, "solve: allocation failed");
+        k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->spider, GST_INF_WORD, 1);
+        const uint64_t* T1 = B->T + lvl_base[1];
+        if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1, n, (uint32_t)g, B->spider);
+        else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1, n, (uint32_t)g, B->spider);
+        // keep the spider's own cost, then MST post-process its paths (lowers word 0 when cheaper)
+        HIPCHK(hipMemcpyAsync(B->spider + 1, B->spider, 8, hipMemcpyDeviceToDevice, s));
+        if (flt) k_spider_tree<true><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr);
+        else k_spider_tree<false><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr);
+        unsigned long long spw[2] = {GST_INF_WORD, GST_INF_WORD};
+        HIPCHK(hipMemcpyAsync(spw, B->spider, 16, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        const uint32_t sp = (uint32_t)(spw[0] >> 32);
+        if (sp != GST_INF_COST) {
+            const uint32_t sp1 = (uint32_t)(spw[1] >> 32);
+            float f0, f1;
+            std::memcpy(&f0, &sp, 4);
+            std::memcpy(&f1, &sp1, 4);
+            spider_costs[0] = flt ? (double)f1 : (double)sp1;
+            spider_costs[1] = flt ? (double)f0 : (double)sp;
+        }
         if (sp != GST_INF_COST) {
             double U;
             if (flt) { float f; std::memcpy(&f, &sp, 4); U = (double)f; } else U = (double)sp;
@@ -766,6 +781,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
     S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW]; S.grow_atomics = hs[ST_ATOM];
     S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
+    S.spider_cost = spider_costs[0]; S.spider_tree_cost = spider_costs[1];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.async_launches = async_launches;
     for (auto& e : evs) {
@@ -779,5 +795,45 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     S.grow_bytes = gst_grow_bytes(S);
     S.merge_bytes = hs[ST_MBYTES];
     S.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return GST_OK;
+}
+
+// MST post-processing of a caller's vertex set (k_mst_induced): one workgroup on the context's
+// stream; the caller has checked the arguments.
+gst_status gst_dpbf_mst_postprocess(gst_ctx* ctx, const gst_graph* G, const int32_t* vertices, int32_t nv, int32_t* eu,
+                                    int32_t* ev, double* ew, int32_t* ne, double* cost) {
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)ctx->stream;
+    const bool flt = G->w_type == GST_W_FLOAT32;
+    const size_t out_words = 1 + 3 * (size_t)MST_SMAX;
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, ((size_t)nv + out_words) * 4) != hipSuccess)
+        return gst_set_error(ctx, GST_OOM, "mst_postprocess: allocation failed");
+    std::vector<uint32_t> out(out_words);
+    hipError_t e = hipMemcpyAsync(d, vertices, (size_t)nv * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        if (flt) k_mst_induced<true><<<1, MST_THREADS, 0, s>>>(d, (uint32_t)nv, G->d_row, (const uint2*)G->d_arc, d + nv);
+        else k_mst_induced<false><<<1, MST_THREADS, 0, s>>>(d, (uint32_t)nv, G->d_row, (const uint2*)G->d_arc, d + nv);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d + nv, out_words * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d);
+    if (e != hipSuccess) return gst_set_error(ctx, GST_HIP_ERROR, "mst_postprocess: %s", hipGetErrorString(e));
+    if (out[0] == MST_EMPTY)
+        return gst_set_error(ctx, GST_INVALID_ARG, "mst_postprocess: more than %u distinct vertices or %u induced edges",
+                             MST_SMAX, MST_EMAX);
+    double c = 0.0;
+    for (uint32_t t = 0; t < out[0]; ++t) {
+        eu[t] = (int32_t)out[1 + 3 * t];
+        ev[t] = (int32_t)out[2 + 3 * t];
+        const uint32_t wb = out[3 + 3 * t];
+        float f;
+        std::memcpy(&f, &wb, 4);
+        ew[t] = flt ? (double)f : (double)wb;
+        c += ew[t];
+    }
+    *ne = (int32_t)out[0];
+    *cost = c;
     return GST_OK;
 }

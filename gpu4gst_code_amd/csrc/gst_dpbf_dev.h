@@ -202,7 +202,8 @@ struct MergeArgs {
     uint32_t pairs_in_lds;
     uint32_t gmax;         // merged states above this cost are not put on the frontier
     uint32_t lbcap;        // cap on level-1 distances in the lower bound (see GrowArgs)
-    uint32_t* spider;      // k == 2 with an automatic bound: min over the tile's vertices of sum_i T(v,{i})
+    unsigned long long* spider;  // k == 2 with an automatic bound: min over the tile's vertices v of
+                                 // (sum_i T(v,{i}) << 32 | v)
     const uint32_t* tour;  // tour table of the query (the merge itself applies the one-label bound only)
     // batched engine, level 2: the slot's groups and the walk scratch the tour table is built from
     const int32_t* grp;    // grp_ptr[0..g] then the members
@@ -2034,15 +2035,19 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
         __syncthreads();
         if (a.spider) {
             // cheapest spider of the tile (level 2: every vertex is staged): sum of the group
-            // distances of one vertex (positions 0..g-1 of the staged rows are the masks 1<<i)
-            uint32_t sm = GST_INF_COST;
+            // distances of one vertex (positions 0..g-1 of the staged rows are the masks 1<<i), with
+            // the vertex (the spider's centre)
+            unsigned long long sm = GST_INF_WORD;
             for (uint32_t b = threadIdx.x; b < nl; b += blockDim.x) {
                 uint32_t acc = 0, o2 = 0;
                 for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, rows[b * Lp + gi], o2);
-                sm = min(sm, o2 ? GST_INF_COST : acc);
+                if (!o2 && acc != GST_INF_COST) {
+                    const unsigned long long w = ((unsigned long long)acc << 32) | (v0 + lst[b]);
+                    sm = w < sm ? w : sm;
+                }
             }
-            sm = wave_min32(sm);
-            if (lane_id() == 0 && sm != GST_INF_COST) atomicMin(a.spider, sm);
+            sm = wave_min64(sm);
+            if (lane_id() == 0 && sm != GST_INF_WORD) atomicMin(a.spider, sm);
         }
         const uint32_t items = nl * mk;
         for (uint32_t i = threadIdx.x; threadIdx.x < istride && i < items; i += istride) {
@@ -2152,17 +2157,289 @@ __global__ void __launch_bounds__(MERGE_THREADS) __attribute__((amdgpu_waves_per
 }
 
 // cheapest spider: min over v of sum_i T(v,{i}) (the union of v's shortest paths to every group
-// holds a tree of at most that cost)
+// holds a tree of at most that cost), as the word (cost << 32 | v): the smallest cost, then the
+// smallest centre
 template <bool FLT>
-__global__ void k_spider_scan(const uint64_t* T1, uint32_t n, uint32_t g, uint32_t* out) {
-    uint32_t sm = GST_INF_COST;
+__global__ void k_spider_scan(const uint64_t* T1, uint32_t n, uint32_t g, unsigned long long* out) {
+    unsigned long long sm = GST_INF_WORD;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t acc = 0, o2 = 0;
         for (uint32_t gi = 0; gi < g; ++gi) acc = add_cost<FLT>(acc, (uint32_t)(T1[v * g + gi] >> 32), o2);
-        sm = min(sm, o2 ? GST_INF_COST : acc);
+        if (!o2 && acc != GST_INF_COST) {
+            const unsigned long long w = ((unsigned long long)acc << 32) | (uint32_t)v;
+            sm = w < sm ? w : sm;
+        }
     }
-    sm = wave_min32(sm);
-    if (lane_id() == 0 && sm != GST_INF_COST) atomicMin(out, sm);
+    sm = wave_min64(sm);
+    if (lane_id() == 0 && sm != GST_INF_WORD) atomicMin(out, sm);
+}
+
+// ---------------------------------------------------------------------------------------
+// MST post-processing (rucgraph/graph_hash_of_mixed_weighted_MST_postprocessing.h:73-103 with the
+// Prim of graph_hash_of_mixed_weighted/common_algorithms/graph_hash_of_mixed_weighted_minimum_spanning_tree.h:29-132):
+// the minimum spanning tree of the subgraph of the input graph induced by a vertex set.  One
+// workgroup: the set in an LDS hash, the induced edges gathered by waves scanning the members' rows,
+// sorted by weight (bitonic, in LDS), then Kruskal (one thread, union-find).  A disconnected set
+// gives a spanning forest (the reference's Prim stops after the component of its first vertex;
+// every set the solver passes is connected, where the two agree).
+//   spider tree (the automatic bound): the set is the union of the spider centre's shortest paths
+//   to every group, walked along the level-1 predecessors; after the MST, leaves that no group needs
+//   are removed.  The tree is feasible (connected, every group touched), so its cost is a bound no
+//   lower than the optimum, and no higher than the spider's sum of distances.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t MST_SMAX = 512;     // vertices of the set
+constexpr uint32_t MST_HCAP = 1024;    // hash slots (power of two)
+constexpr uint32_t MST_EMAX = 2048;    // induced edges (power of two)
+constexpr uint32_t MST_WALK = 256;     // hops of one shortest path
+constexpr uint32_t MST_EMPTY = 0xFFFFFFFFu;
+constexpr int MST_THREADS = 256;
+
+struct MstLds {
+    uint32_t hkey[MST_HCAP];           // vertex id, MST_EMPTY when free
+    uint32_t hval[MST_HCAP];           // its index in vs
+    uint32_t vs[MST_SMAX];             // the set, by index
+    uint32_t nb[MST_SMAX];             // tree adjacency (pruning): first incident tree edge
+    uint32_t deg[MST_SMAX];
+    unsigned long long e[MST_EMAX];    // (weight bits << 32) | (a << 16) | b, a < b indices; bit 63: in the tree
+    uint32_t par[MST_SMAX];
+    uint32_t enext[2 * MST_SMAX];      // tree adjacency lists: entry 2j / 2j+1 for the two ends of tree edge j
+    uint32_t tedge[MST_SMAX];          // the tree's edges, as indices into e
+    uint32_t nv, ne, nt, fail;
+};
+
+__device__ __forceinline__ uint32_t mst_hash(uint32_t v) { return (v * 2654435761u) & (MST_HCAP - 1); }
+
+// inserts v (any thread, concurrently); false when the table is full
+__device__ __forceinline__ bool mst_insert(MstLds& L, uint32_t v) {
+    uint32_t h = mst_hash(v);
+    for (uint32_t p = 0; p < MST_HCAP; ++p, h = (h + 1) & (MST_HCAP - 1)) {
+        const uint32_t old = atomicCAS(&L.hkey[h], MST_EMPTY, v);
+        if (old == MST_EMPTY || old == v) return true;
+    }
+    return false;
+}
+
+// index of v in the set, MST_EMPTY when absent (after mst_number)
+__device__ __forceinline__ uint32_t mst_find(const MstLds& L, uint32_t v) {
+    uint32_t h = mst_hash(v);
+    for (uint32_t p = 0; p < MST_HCAP; ++p, h = (h + 1) & (MST_HCAP - 1)) {
+        const uint32_t k = L.hkey[h];
+        if (k == v) return L.hval[h];
+        if (k == MST_EMPTY) return MST_EMPTY;
+    }
+    return MST_EMPTY;
+}
+
+__device__ __forceinline__ uint32_t mst_root(MstLds& L, uint32_t x) {
+    while (L.par[x] != x) {
+        L.par[x] = L.par[L.par[x]];
+        x = L.par[x];
+    }
+    return x;
+}
+
+__device__ __forceinline__ void mst_clear(MstLds& L) {
+    for (uint32_t i = threadIdx.x; i < MST_HCAP; i += blockDim.x) L.hkey[i] = MST_EMPTY;
+    if (threadIdx.x == 0) { L.nv = 0; L.ne = 0; L.nt = 0; L.fail = 0; }
+    __syncthreads();
+}
+
+// After the inserts: numbers the set (vs, hval), gathers the induced edges, sorts them and keeps a
+// minimum spanning forest (bit 63 of the kept edges, their indices in tedge[0..nt)).  All threads.
+template <bool FLT>
+__device__ void mst_build(MstLds& L, const uint32_t* row, const uint2* arc) {
+    __syncthreads();
+    for (uint32_t h = threadIdx.x; h < MST_HCAP; h += blockDim.x) {
+        const uint32_t k = L.hkey[h];
+        if (k == MST_EMPTY) continue;
+        const uint32_t i = atomicAdd(&L.nv, 1u);
+        if (i < MST_SMAX) { L.vs[i] = k; L.hval[h] = i; }
+        else L.fail = 1;
+    }
+    __syncthreads();
+    if (L.fail) return;
+    const uint32_t nv = L.nv;
+    // induced edges: wave w scans the rows of members w, w + waves, ...; each undirected edge is taken
+    // from the row of its smaller endpoint
+    const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    for (uint32_t i = wv; i < nv; i += nw) {
+        const uint32_t v = L.vs[i];
+        const uint32_t r0 = row[v], r1 = row[v + 1];
+        for (uint32_t e = r0 + lane; e < r1; e += WAVE) {
+            const uint2 rc = arc[e];
+            if (rc.x <= v) continue;
+            const uint32_t j = mst_find(L, rc.x);
+            if (j == MST_EMPTY) continue;
+            const uint32_t slot = atomicAdd(&L.ne, 1u);
+            if (slot < MST_EMAX) L.e[slot] = ((unsigned long long)rc.y << 32) | ((unsigned long long)i << 16) | j;
+            else L.fail = 1;
+        }
+    }
+    __syncthreads();
+    if (L.fail) return;
+    const uint32_t ne = L.ne;
+    uint32_t P = 1;
+    while (P < ne) P <<= 1;
+    for (uint32_t i = ne + threadIdx.x; i < P; i += blockDim.x) L.e[i] = ~0ull >> 1;   // after every real edge
+    __syncthreads();
+    // bitonic sort of e[0..P) by (weight, a, b)
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const unsigned long long x = L.e[i], y = L.e[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { L.e[i] = y; L.e[l] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // Kruskal over the sorted edges
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 0; i < nv; ++i) L.par[i] = i;
+        uint32_t nt = 0;
+        for (uint32_t i = 0; i < ne && nt + 1 < nv; ++i) {
+            const unsigned long long x = L.e[i];
+            const uint32_t ra = mst_root(L, (uint32_t)(x >> 16) & 0xFFFFu), rb = mst_root(L, (uint32_t)x & 0xFFFFu);
+            if (ra == rb) continue;
+            L.par[ra] = rb;
+            L.e[i] = x | (1ull << 63);
+            L.tedge[nt++] = i;
+        }
+        L.nt = nt;
+    }
+    __syncthreads();
+}
+
+// Removes, one thread, tree leaves whose groups (group bits gbit(v)) are all touched by another
+// kept vertex, until none is left; returns the kept edges' cost (GST_INF_COST on overflow).
+template <bool FLT, class GroupF>
+__device__ uint32_t mst_prune_cost(MstLds& L, uint32_t g, GroupF gbit) {
+    const uint32_t nv = L.nv, nt = L.nt;
+    uint32_t cover[GST_MAX_GROUPS];
+    for (uint32_t i = 0; i < g; ++i) cover[i] = 0;
+    for (uint32_t i = 0; i < nv; ++i) {
+        L.deg[i] = 0;
+        L.nb[i] = MST_EMPTY;
+        const uint32_t b = gbit(L.vs[i]);
+        for (uint32_t q = 0; q < g; ++q) cover[q] += (b >> q) & 1u;
+    }
+    for (uint32_t t = 0; t < nt; ++t) {
+        const unsigned long long x = L.e[L.tedge[t]];
+        const uint32_t a = (uint32_t)(x >> 16) & 0xFFFFu, b = (uint32_t)x & 0xFFFFu;
+        L.deg[a]++; L.deg[b]++;
+        L.enext[2 * t] = L.nb[a]; L.nb[a] = 2 * t;
+        L.enext[2 * t + 1] = L.nb[b]; L.nb[b] = 2 * t + 1;
+    }
+    // par[] is reused as the stack of leaves to look at
+    uint32_t sp = 0;
+    for (uint32_t i = 0; i < nv; ++i) if (L.deg[i] == 1) L.par[sp++] = i;
+    while (sp) {
+        const uint32_t v = L.par[--sp];
+        if (L.deg[v] != 1) continue;
+        const uint32_t b = gbit(L.vs[v]);
+        bool need = false;
+        for (uint32_t q = 0; q < g; ++q) need = need || (((b >> q) & 1u) && cover[q] <= 1u);
+        if (need) continue;
+        // its one live tree edge
+        for (uint32_t h = L.nb[v]; h != MST_EMPTY; h = L.enext[h]) {
+            const uint32_t t = h >> 1;
+            unsigned long long& x = L.e[L.tedge[t]];
+            if (!(x >> 63)) continue;
+            x &= ~(1ull << 63);
+            const uint32_t a = (uint32_t)(x >> 16) & 0xFFFFu, c = (uint32_t)x & 0xFFFFu;
+            const uint32_t o = a == v ? c : a;
+            L.deg[v] = 0;
+            for (uint32_t q = 0; q < g; ++q) cover[q] -= (b >> q) & 1u;
+            if (--L.deg[o] == 1) L.par[sp++] = o;
+            break;
+        }
+    }
+    uint32_t cost = 0, ovf = 0;
+    for (uint32_t t = 0; t < nt; ++t) {
+        const unsigned long long x = L.e[L.tedge[t]];
+        if (x >> 63) cost = add_cost<FLT>(cost, (uint32_t)((x >> 32) & 0x7FFFFFFFu), ovf);
+    }
+    return ovf ? GST_INF_COST : cost;
+}
+
+// The spider's tree: the centre c and its cost come from *spw (cost << 32 | c); the union of c's
+// shortest paths (level-1 predecessors) is MST post-processed and pruned, and *spw's cost is lowered
+// to the tree's when that is cheaper.  When tree != nullptr its edges go there as (u, v, weight
+// bits) triples after a count word (at most MST_SMAX - 1 edges); nothing is written when the set or
+// its edges overflow the tables.  One workgroup of MST_THREADS threads.
+template <bool FLT>
+__device__ void spider_tree_body(MstLds& L, const uint64_t* T1, uint32_t g, const uint32_t* gm, const uint32_t* row,
+                                 const uint2* arc, unsigned long long* spw, uint32_t* tree) {
+    const unsigned long long w0 = *spw;
+    if (w0 == GST_INF_WORD || g < 2) return;
+    const uint32_t c = (uint32_t)w0, sp0 = (uint32_t)(w0 >> 32);
+    mst_clear(L);
+    if (threadIdx.x < g) {
+        // thread i walks group i's path from c: level-1 predecessors lead to a member (leaf word)
+        const uint32_t i = threadIdx.x;
+        uint32_t v = c;
+        bool ok = mst_insert(L, v);
+        for (uint32_t hop = 0; ok; ++hop) {
+            const uint64_t w = T1[(uint64_t)v * g + i];
+            const uint32_t pred = (uint32_t)w;
+            if ((uint32_t)(w >> 32) == GST_INF_COST || hop == MST_WALK) { ok = false; break; }
+            if (pred & GST_PRED_LEAF) break;   // v is a member of group i
+            v = pred;
+            ok = mst_insert(L, v);
+        }
+        if (!ok) L.fail = 1;
+    }
+    mst_build<FLT>(L, row, arc);
+    if (L.fail) return;
+    if (threadIdx.x == 0) {
+        const uint32_t full = (1u << g) - 1u;
+        const uint32_t cost = mst_prune_cost<FLT>(L, g, [&](uint32_t v) { return gm[v] & full; });
+        if (cost != GST_INF_COST && cost < sp0) atomicMin(spw, ((unsigned long long)cost << 32) | c);
+        if (tree) {
+            uint32_t m = 0;
+            for (uint32_t t = 0; t < L.nt; ++t) {
+                const unsigned long long x = L.e[L.tedge[t]];
+                if (!(x >> 63)) continue;
+                tree[1 + 3 * m] = L.vs[(x >> 16) & 0xFFFFu];
+                tree[2 + 3 * m] = L.vs[x & 0xFFFFu];
+                tree[3 + 3 * m] = (uint32_t)((x >> 32) & 0x7FFFFFFFu);
+                ++m;
+            }
+            tree[0] = m;
+        }
+    }
+}
+
+template <bool FLT>
+__global__ void __launch_bounds__(MST_THREADS) k_spider_tree(const uint64_t* T1, uint32_t g, const uint32_t* gm,
+                                                           const uint32_t* row, const uint2* arc,
+                                                           unsigned long long* spw, uint32_t* tree) {
+    __shared__ MstLds L;
+    spider_tree_body<FLT>(L, T1, g, gm, row, arc, spw, tree);
+}
+
+// The reference's post-processing of a caller's vertex set: out[0] = edge count (MST_EMPTY when the
+// set or its induced edges overflow the tables), then (u, v, weight bits) per forest edge.
+template <bool FLT>
+__global__ void __launch_bounds__(MST_THREADS) k_mst_induced(const uint32_t* verts, uint32_t nverts, const uint32_t* row,
+                                                           const uint2* arc, uint32_t* out) {
+    __shared__ MstLds L;
+    mst_clear(L);
+    for (uint32_t i = threadIdx.x; i < nverts; i += blockDim.x)
+        if (!mst_insert(L, verts[i])) L.fail = 1;
+    mst_build<FLT>(L, row, arc);
+    if (threadIdx.x != 0) return;
+    if (L.fail) { out[0] = MST_EMPTY; return; }
+    for (uint32_t t = 0; t < L.nt; ++t) {
+        const unsigned long long x = L.e[L.tedge[t]];
+        out[1 + 3 * t] = L.vs[(x >> 16) & 0xFFFFu];
+        out[2 + 3 * t] = L.vs[x & 0xFFFFu];
+        out[3 + 3 * t] = (uint32_t)((x >> 32) & 0x7FFFFFFFu);
+    }
+    out[0] = L.nt;
 }
 
 // g == 1: the full mask is level 1; its minimum is any member (cost 0)

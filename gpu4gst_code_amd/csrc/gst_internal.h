@@ -157,6 +157,8 @@ struct gst_ctx {
 gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* g, const gst_groups* groups,
                           const gst_opts* opts, gst_result* res);
 void gst_query_bufs_free(gst_ctx* ctx);
+gst_status gst_dpbf_mst_postprocess(gst_ctx* ctx, const gst_graph* G, const int32_t* vertices, int32_t nv, int32_t* eu,
+                                    int32_t* ev, double* ew, int32_t* ne, double* cost);
 // tree of a traced edge list (duplicates removed, spanning forest of the union) into res
 void gst_finish_tree(bool flt, uint32_t nout, const uint32_t* hu, const uint32_t* hv, const uint32_t* hw,
                      gst_result* res);

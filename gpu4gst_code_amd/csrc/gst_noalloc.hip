@@ -213,7 +213,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     const size_t tour_words = ((size_t)1 << g) * (size_t)(g + 1 + g * g);
     struct Bufs {
         uint64_t* T; uint32_t* gm; uint64_t* chg[3]; uint32_t* inq[3]; uint64_t* lists[3]; RoundCtl* ctl;
-        unsigned long long* root; unsigned long long* stats; uint32_t* spider; uint32_t* tour; uint32_t* edges;
+        unsigned long long* root; unsigned long long* stats; unsigned long long* spider; uint32_t* tour; uint32_t* edges;
         int32_t* grp; uint64_t* pairs; uint32_t* masks; uint32_t* pos_of; uint64_t* all_items; uint32_t* all_cnt;
         uint8_t* act;
     } B;
@@ -228,7 +228,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         B.ctl = c.take<RoundCtl>(NCTL);
         B.root = c.take<unsigned long long>(1);
         B.stats = c.take<unsigned long long>(STAT_WORDS);
-        B.spider = c.take<uint32_t>(1);
+        B.spider = c.take<unsigned long long>(2);
         B.tour = c.take<uint32_t>(tour_words);
         B.edges = c.take<uint32_t>(3 * (size_t)ecap + 1);
         B.grp = c.take<int32_t>((size_t)g + 1 + members);
@@ -262,7 +262,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B.root, GST_INF_WORD, 1);
     k_fill_u32<<<1, 256, 0, s>>>((uint32_t*)B.ctl, 0u, sizeof(RoundCtl) * NCTL / 4);
     k_fill_u32<<<1, 64, 0, s>>>(B.all_cnt, 0u, 1);
-    k_fill_u32<<<1, 64, 0, s>>>(B.spider, GST_INF_COST, 1);
+    k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B.spider, GST_INF_WORD, 2);
     HIPCHK(hipMemcpyAsync(B.grp, groups->grp_ptr, ((size_t)g + 1) * 4, hipMemcpyHostToDevice, s));
     if (members) HIPCHK(hipMemcpyAsync(B.grp + g + 1, groups->grp_vtx, (size_t)members * 4, hipMemcpyHostToDevice, s));
     if (!L.pairs.empty()) HIPCHK(hipMemcpyAsync(B.pairs, L.pairs.data(), L.pairs.size() * 8, hipMemcpyHostToDevice, s));
@@ -338,12 +338,27 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
         k_root_scan_l1<<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B.T, n, B.root);
     }
     // ---- automatic bound: the cheapest spider once the group distances are final ----
+    double spider_costs[2] = {0.0, 0.0};   // the spider's sum of distances, its post-processed tree
     if (st == GST_OK && opts.auto_bound && g >= 3) {
-        if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B.T + lvl_base[1], n, (uint32_t)g, B.spider);
-        else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(B.T + lvl_base[1], n, (uint32_t)g, B.spider);
-        uint32_t sp = GST_INF_COST;
-        HIPCHK(hipMemcpyAsync(&sp, B.spider, 4, hipMemcpyDeviceToHost, s));
+        const uint64_t* T1s = B.T + lvl_base[1];
+        if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1s, n, (uint32_t)g, B.spider);
+        else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1s, n, (uint32_t)g, B.spider);
+        // keep the spider's own cost, then MST post-process its paths (lowers word 0 when cheaper)
+        HIPCHK(hipMemcpyAsync(B.spider + 1, B.spider, 8, hipMemcpyDeviceToDevice, s));
+        if (flt) k_spider_tree<true><<<1, MST_THREADS, 0, s>>>(T1s, (uint32_t)g, B.gm, G->d_row, (const uint2*)G->d_arc, B.spider, nullptr);
+        else k_spider_tree<false><<<1, MST_THREADS, 0, s>>>(T1s, (uint32_t)g, B.gm, G->d_row, (const uint2*)G->d_arc, B.spider, nullptr);
+        unsigned long long spw[2] = {GST_INF_WORD, GST_INF_WORD};
+        HIPCHK(hipMemcpyAsync(spw, B.spider, 16, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        const uint32_t sp = (uint32_t)(spw[0] >> 32);
+        if (sp != GST_INF_COST) {
+            const uint32_t sp1 = (uint32_t)(spw[1] >> 32);
+            float f0, f1;
+            std::memcpy(&f0, &sp, 4);
+            std::memcpy(&f1, &sp1, 4);
+            spider_costs[0] = flt ? (double)f1 : (double)sp1;
+            spider_costs[1] = flt ? (double)f0 : (double)sp;
+        }
         if (sp != GST_INF_COST) {
             double U;
             if (flt) { float f; std::memcpy(&f, &sp, 4); U = (double)f; } else U = (double)sp;
@@ -464,6 +479,7 @@ gst_status gst_dpbf_solve_noalloc(gst_ctx* ctx, const gst_graph* G, const gst_gr
     S.grow_items = hs[ST_ITEMS]; S.grow_arcs = hs[ST_ARCS]; S.grow_relax = hs[ST_RELAX];
     S.grow_updates = hs[ST_UPD]; S.grow_enqueued = hs[ST_ENQ]; S.grow_new_states = hs[ST_NEW]; S.grow_atomics = hs[ST_ATOM];
     S.merge_vertices = hs[ST_MVERT]; S.merge_pairs = hs[ST_MPAIRS];
+    S.spider_cost = spider_costs[0]; S.spider_tree_cost = spider_costs[1];
     S.grow_rounds = total_rounds; S.grow_launches = grow_launches; S.merge_launches = merge_launches;
     S.grow_bytes = gst_grow_bytes(S);
     S.merge_bytes = hs[ST_MBYTES];

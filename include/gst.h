@@ -121,6 +121,10 @@ typedef struct {
     uint64_t grow_atomics;     /* device-scope atomic operations of the grow kernels (min, compare-and-swap,
                                   change-bit OR, queue-flag exchange): each is a random 64-byte line
                                   read-modify-write beyond the L2, the grow's costliest operation */
+    double   spider_cost;      /* automatic bound (one-query solves): min over centres c of sum_i dist(c, G_i),
+                                  0 when not computed */
+    double   spider_tree_cost; /* the same centre's shortest paths MST post-processed and pruned
+                                  (<= spider_cost; the bound used), 0 when not computed */
 } gst_stats;
 
 typedef struct {
@@ -176,6 +180,18 @@ void       gst_result_free(gst_result* r);
 gst_status gst_validate_tree(const gst_csr* csr, const gst_groups* groups, int32_t n_edges,
                              const int32_t* eu, const int32_t* ev, int32_t root,
                              double* cost_out);
+
+/* MST post-processing of a vertex set (rucgraph/graph_hash_of_mixed_weighted_MST_postprocessing.h:73-103,
+ * graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_vertices, whose spanning tree is the Prim of
+ * graph_hash_of_mixed_weighted/common_algorithms/graph_hash_of_mixed_weighted_minimum_spanning_tree.h:29-132):
+ * the minimum spanning forest of the subgraph of `graph` induced by vertices[0..nv) (repeats ignored),
+ * computed on the device by one workgroup.  On return *ne edges (eu, ev, ew; arrays of nv entries)
+ * and *cost = the double sum of their weights.  At most 512 distinct vertices with at most 2048
+ * induced edges (GST_INVALID_ARG beyond).  Where the set is disconnected the reference's Prim stops
+ * after its first vertex's component; this returns a spanning forest of every component (on the
+ * connected sets the reference passes, the two agree). */
+gst_status gst_mst_postprocess(gst_ctx* ctx, const gst_graph* graph, const int32_t* vertices, int32_t nv,
+                               int32_t* eu, int32_t* ev, double* ew, int32_t* ne, double* cost);
 
 /* --- one query without allocation ------------------------------------------------------- */
 /* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph capture,

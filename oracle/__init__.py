@@ -203,3 +203,49 @@ def dpbf_batch(graph, group_sets, nthreads: int = 1) -> np.ndarray:
     costs = np.empty(nq, np.float64)
     lib.orc_dpbf_batch(nthreads, n, _p(row), _p(col), _p(w), nq, _p(gq), _p(np.ascontiguousarray(gms)), _p(costs))
     return costs
+
+
+def mst_induced(graph, vertices):
+    """Minimum spanning forest of the subgraph induced by `vertices`: restates the MST post-processing
+    rucgraph/graph_hash_of_mixed_weighted_MST_postprocessing.h:73-103 (the induced subgraph of the
+    vertex set, then the Prim of graph_hash_of_mixed_weighted/common_algorithms/
+    graph_hash_of_mixed_weighted_minimum_spanning_tree.h:29-132, here restarted in every component, so
+    that a disconnected set gives a spanning forest).  Returns (edges as (u, v, w) with u < v, cost)."""
+    import heapq
+    vs = sorted(set(int(x) for x in vertices))
+    inset = set(vs)
+    rp, col, w = graph.row_ptr, graph.col, graph.w
+    adj = {v: [] for v in vs}
+    for v in vs:
+        best = {}
+        for e in range(int(rp[v]), int(rp[v + 1])):
+            x = int(col[e])
+            if x in inset and x != v:
+                wx = float(w[e])
+                if x not in best or wx < best[x]:
+                    best[x] = wx
+        adj[v] = sorted(best.items())
+    done, edges, cost = set(), [], 0.0
+    for s in vs:
+        if s in done:
+            continue
+        done.add(s)
+        heap = [(wx, x, s) for x, wx in adj[s]]
+        heapq.heapify(heap)
+        while heap:
+            wx, x, p = heapq.heappop(heap)
+            if x in done:
+                continue
+            done.add(x)
+            edges.append((min(x, p), max(x, p), wx))
+            cost += wx
+            for y, wy in adj[x]:
+                if y not in done:
+                    heapq.heappush(heap, (wy, y, x))
+    return edges, cost
+
+
+def spider_cost(graph, groups) -> float:
+    """The automatic bound before post-processing: min over centres c of sum_i d(c, G_i)."""
+    d = group_distances(graph, groups)
+    return float(np.min(d.sum(axis=0)))

@@ -274,3 +274,86 @@ def test_tour_bound_prunes_more_c1(s):
     c2, st2 = oracle.dpbf_pruned(G, groups, c0, 0.5, 2)
     assert c1 == c2 == c0
     assert st2["pops"] <= st1["pops"]
+
+
+def _brute_msf_cost(n, edges):
+    """Minimum spanning forest cost by enumerating edge subsets (tiny graphs only)."""
+    import itertools
+    # components of the whole edge set fix the forest size
+    par = list(range(n))
+
+    def root(x):
+        while par[x] != x:
+            x = par[x]
+        return x
+    for a, b, _ in edges:
+        ra, rb = root(a), root(b)
+        if ra != rb:
+            par[ra] = rb
+    k = n - len({root(x) for x in range(n)})
+    best = None
+    for sub in itertools.combinations(edges, k):
+        p2 = list(range(n))
+
+        def r2(x):
+            while p2[x] != x:
+                x = p2[x]
+            return x
+        ok = True
+        for a, b, _ in sub:
+            ra, rb = r2(a), r2(b)
+            if ra == rb:
+                ok = False
+                break
+            p2[ra] = rb
+        if ok:
+            c = sum(wt for _, _, wt in sub)
+            best = c if best is None else min(best, c)
+    return best if best is not None else 0.0
+
+
+def test_mst_induced_pinned():
+    """oracle.mst_induced (the reference's MST post-processing) against brute force over every
+    edge subset on tiny graphs and against scipy's minimum spanning tree on larger induced sets,
+    disconnected sets included.  A planted mistake (the same Prim taking the heaviest edge: a maximum
+    spanning forest) fails the brute-force check on some instance, so the check can see it."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import minimum_spanning_tree
+
+    class Neg:   # the graph with negated weights: Prim on it is the planted maximum spanning forest
+        def __init__(self, G):
+            self.row_ptr, self.col, self.w = G.row_ptr, G.col, -np.asarray(G.w, dtype=np.float64)
+
+    rng = np.random.default_rng(5)
+    planted_caught = 0
+    for t in range(40):
+        n = int(rng.integers(3, 7))
+        G = gst.gen_random_connected(n, min(n * (n - 1) // 2, n + int(rng.integers(1, 6))), 1, 9, seed=300 + t)
+        vs = [v for v in range(n) if rng.random() < 0.85] or [0]
+        edges, cost = oracle.mst_induced(G, vs)
+        idx = {v: i for i, v in enumerate(vs)}
+        ind = []
+        for v in vs:
+            for e in range(int(G.row_ptr[v]), int(G.row_ptr[v + 1])):
+                x = int(G.col[e])
+                if x in idx and v < x:
+                    ind.append((idx[v], idx[x], float(G.w[e])))
+        want = _brute_msf_cost(len(vs), ind)
+        assert cost == want, (t, cost, want)
+        assert all(u < v and u in idx and v in idx for u, v, _ in edges)
+        _, neg = oracle.mst_induced(Neg(G), vs)
+        planted_caught += (-neg != want)
+    assert planted_caught > 0
+    for t in range(10):
+        G = gst.gen_random_connected(600, 3000, 1, 100, seed=400 + t)
+        vs = np.unique(rng.integers(0, 600, size=int(rng.integers(20, 400))))
+        edges, cost = oracle.mst_induced(G, vs)
+        sub = {int(v): i for i, v in enumerate(vs)}
+        rows, cols, ws = [], [], []
+        for v in vs:
+            for e in range(int(G.row_ptr[v]), int(G.row_ptr[v + 1])):
+                x = int(G.col[e])
+                if x in sub:
+                    rows.append(sub[int(v)]); cols.append(sub[x]); ws.append(float(G.w[e]))
+        M = sp.csr_matrix((ws, (rows, cols)), shape=(len(vs), len(vs)))
+        assert abs(minimum_spanning_tree(M).sum() - cost) < 1e-9, t
