@@ -42,14 +42,20 @@
 namespace {
 
 constexpr uint32_t MAX_SLOTS = 256;
-// workgroups a batched grow round may use, per CU
 // 1: a new query's set-up (g >= 2) runs batched in the next step's first part; 0: at once, per query
 #ifndef GST_BATCH_DEFER_FILL
 #define GST_BATCH_DEFER_FILL 1
 #endif
+// workgroups a batched grow round may use, per CU.  The grid exceeds what is resident at once: each
+// slot's round gets the workgroups its work needs (at most GROW_MAX_PART), and the dispatcher hands
+// the CUs that slots with short rounds free to the workgroups still waiting, so a slot with a long
+// round no longer runs on a scaled-down share while the others idle.  Measured on BA V=200k g=6,
+// 128 slots: x5 / 8 / 12 / 20 give the spider-bounded stream 821 / 904 / 991 / 1076 queries/s and the
+// learned-bound stream 1526 / 1536 / 1525 / 1472.
 #ifndef GST_BATCH_GRID_MULT
-#define GST_BATCH_GRID_MULT 5
+#define GST_BATCH_GRID_MULT 12
 #endif
+constexpr uint32_t GROW_MAX_PART = 64 * 32;   // 64 arrival groups of 32 workgroups (RoundCtl::sub)
 constexpr uint32_t PLAN_THREADS = MAX_SLOTS;          // the plan kernels run one thread per slot
 constexpr uint32_t PLAN_WAVES = PLAN_THREADS / WAVE;
 enum : uint32_t { PH_EMPTY = 0, PH_GROW = 1, PH_TRACE = 2, PH_DONE = 3, PH_FILL = 4 };
@@ -393,8 +399,9 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_grow(BatchArgs b) {
     }
     uint32_t total, n_act, all;
     plan_scan(min(need, 1u << 24), total);
-    uint32_t alloc = min(need, b.grid);
-    if (total > b.grid && need) alloc = max(1u, (uint32_t)((uint64_t)min(need, 1u << 24) * b.grid / total));
+    // a slot's round takes at most GROW_MAX_PART workgroups (its arrival counters count 64 groups of 32)
+    uint32_t alloc = min(need, min(b.grid, GROW_MAX_PART));
+    if (total > b.grid && need) alloc = min(GROW_MAX_PART, max(1u, (uint32_t)((uint64_t)min(need, 1u << 24) * b.grid / total)));
     const uint32_t start = plan_scan(alloc, all);
     const uint32_t rank = plan_scan(need != 0 ? 1u : 0u, n_act);
     BatchPlan* p = b.plan;
@@ -954,7 +961,7 @@ static gst_status batch_run_impl(gst_ctx* ctx, const std::function<bool(gst_batc
     BatchArgs ba;
     ba.sb = E->d_sb; ba.plan = E->d_plan;
     ba.S = (uint32_t)S;
-    ba.grid = (uint32_t)std::max(1, std::min(ctx->cu_count * GST_BATCH_GRID_MULT, 64 * 32 - (int)MAX_SLOTS));
+    ba.grid = (uint32_t)std::max(1, std::min(ctx->cu_count * GST_BATCH_GRID_MULT, 16384));
     k_reset_phases<<<1, MAX_SLOTS, 0, s>>>(E->d_sb);
 
     std::vector<double> lat;

@@ -42,7 +42,10 @@ enum StatIdx {
 // that the one atomic per workgroup and counter at the end of a launch spreads over several
 // addresses instead of queueing on one (measured: up to 15 us per merge launch of 1000+
 // workgroups); readers sum the stripes (stat_sum).
-constexpr int STAT_STRIPES = 16;
+#ifndef GST_STAT_STRIPES
+#define GST_STAT_STRIPES 16
+#endif
+constexpr int STAT_STRIPES = GST_STAT_STRIPES;
 constexpr int STAT_WORDS = ST_NSTAT * STAT_STRIPES;
 __device__ __forceinline__ unsigned long long* stat_stripe(unsigned long long* stats, uint32_t bid) {
     return stats + (size_t)(bid % STAT_STRIPES) * ST_NSTAT;
