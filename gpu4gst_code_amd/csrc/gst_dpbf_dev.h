@@ -602,8 +602,12 @@ __global__ void __launch_bounds__(256) k_tour_table(const uint64_t* T1, const in
 #ifndef GST_PUSH_BATCH_ATOMICS
 #define GST_PUSH_BATCH_ATOMICS 1
 #endif
+// arc steps per lane issued together.  With the atomics of a step batched, 4 steps spill 38-45
+// registers per lane into scratch memory, whose traffic doubled the grow's HBM bytes; 2 spill 7
+// (measured on BA V=200k g=6, 128 slots: 1 / 2 / 3 / 4 steps give the learned-bound stream 1508 /
+// 1616 / 1586 / 1530 queries/s)
 #ifndef GST_GROW_UNR
-#define GST_GROW_UNR 4
+#define GST_GROW_UNR 2
 #endif
 #ifndef GST_GROW_WPE
 #define GST_GROW_WPE 5
@@ -726,6 +730,11 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// GST_ITEMPAR_CODE 0 leaves this body out of the grow kernels (push_lanes = 1 then runs 64-lane
+// groups).  Inlined: out of line, the call and its scratch frame made the one-query grow 2x slower.
+#ifndef GST_ITEMPAR_CODE
+#define GST_ITEMPAR_CODE 1
+#endif
 template <bool FLT>
 __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q, PushWaveLds& S, const uint64_t* lcur,
                                                 const uint64_t* ccur, uint64_t* cnxt, uint32_t* inxt, uint64_t* lnxt,
@@ -1220,7 +1229,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     // (mask, arc slot) pairs over blocks of up to PL masks. ----
     const uint32_t rpl = round_push_lanes(a.push_lanes, a.push_lanes_sparse, n_cur);
     const bool item_par = rpl == PUSH_ITEM_PAR;
-    if (!pull && item_par)
+    if (GST_ITEMPAR_CODE && !pull && item_par)
         push_items_body<FLT>(a, q, s_pw[threadIdx.x / WAVE], lcur, ccur, cnxt, inxt, lnxt, cntn, n_cur, gwave, nwaves,
                              prune_lb, st_items, st_arcs, st_relax, st_upd, st_enq, st_new, st_atom, ovf);
     const uint32_t PL = item_par ? (uint32_t)WAVE : rpl, PG = WAVE / PL;   // PL: 8, 16, 32 or 64

@@ -1,8 +1,8 @@
 // Caller written against the reference-shaped C++ API (include/gst.hpp): builds a
 // graph_v_of_v_idealID and a group graph the way the reference's harness does
 // (rucgraph/test_gpu.h:115-196: input graph, group vertices V..V+g-1 whose rows list members),
-// calls gst::DPBF_GPU and gst::DPBF_GPU_T with the reference's scalar types and checks the trees
-// with the reference's own check names.  Usage: dpbf_gpu_example <V> <E> <g> <seed>
+// calls gst::DPBF_GPU and gst::DPBF_GPU_T with the reference's scalar types, checks the trees
+// with the reference's own check names and MST post-processes the optimal tree's vertex set.  Usage: dpbf_gpu_example <V> <E> <g> <seed>
 #include <cstdio>
 #include <cstdlib>
 
@@ -60,12 +60,20 @@ int main(int argc, char** argv) {
         gst::graph_hash_of_mixed_weighted solu_t =
             gst::DPBF_GPU_T(dg, cumpulsory_group_vertices, group_graph, &pointer1_t, &cost_t, &rt_t);
         if (!gst::this_is_a_feasible_solution_gpu(solu_t, group_graph, cumpulsory_group_vertices)) return 4;
+        // MST post-processing of the optimal tree's vertices (graph_hash_of_mixed_weighted_MST_postprocessing.h:73):
+        // an optimal tree is a minimum spanning tree of its own vertex set, so the cost stays
+        std::unordered_set<int> hash_of_v;
+        for (auto& kv : solu.hash_of_vectors) hash_of_v.insert(kv.first);
+        gst::graph_hash_of_mixed_weighted mst =
+            gst::graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_vertices(dg, hash_of_v);
+        const int mst_sum = gst::graph_hash_of_mixed_weighted_sum_of_ec(mst);
         std::printf("cost %d sum_of_ec %d tree_cost %.1f vertices %zu edges %zu rt %.6f items %d; DPBF_GPU_T cost %d "
-                    "sum_of_ec %d items %d\n",
+                    "sum_of_ec %d items %d; MST post-processing %d\n",
                     *real_cost, sum, tc, solu.num_vertices(), solu.num_edges(), rt, pointer1, cost_t,
-                    gst::graph_hash_of_mixed_weighted_sum_of_ec(solu_t), pointer1_t);
+                    gst::graph_hash_of_mixed_weighted_sum_of_ec(solu_t), pointer1_t, mst_sum);
         return (tc == *real_cost && sum == *real_cost && cost_t == *real_cost &&
-                gst::graph_hash_of_mixed_weighted_sum_of_ec(solu_t) == cost_t) ? 0 : 1;
+                gst::graph_hash_of_mixed_weighted_sum_of_ec(solu_t) == cost_t && mst_sum == *real_cost &&
+                mst.num_vertices() == solu.num_vertices()) ? 0 : 1;
     } catch (const gst::error& e) {
         std::fprintf(stderr, "gst error %d: %s\n", (int)e.status, e.what());
         return 3;

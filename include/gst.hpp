@@ -18,6 +18,8 @@
 //   graph_hash_of_mixed_weighted_sum_of_ec                          gst::graph_hash_of_mixed_weighted_sum_of_ec
 //     (rucgraph/graph_hash_of_mixed_weighted_sum_of_nw_ec.h:58)
 //   this_is_a_feasible_solution_gpu (rucgraph/test_gpu.h:45-77)     gst::this_is_a_feasible_solution_gpu
+//   graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_vertices gst::graph_hash_of_mixed_weighted_MST_
+//     (rucgraph/graph_hash_of_mixed_weighted_MST_postprocessing.h:73)  postprocessing_hash_of_vertices(dg, set)
 //
 // The arguments the reference needs only for its own engine (the V x 2^g host table, the CSR, the
 // input graph again, the disjoint-set list, the community arrays) have no counterpart: the table
@@ -300,6 +302,27 @@ inline graph_hash_of_mixed_weighted DPBF_GPU_T(device_graph& dg, const std::unor
     o.want_tree = 1;
     o.auto_bound = 1;
     return detail::dpbf_int(dg, cumpulsory_group_vertices, group_graph, pointer1, real_cost, rt, &o);
+}
+
+// graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_vertices
+// (rucgraph/graph_hash_of_mixed_weighted_MST_postprocessing.h:73-103): the minimum spanning tree of the
+// subgraph induced by hash_of_v, every vertex of the set included (a disconnected set gives a
+// spanning forest; the reference's Prim keeps only its first vertex's component).  On the device, by
+// gst_mst_postprocess: at most 512 distinct vertices.
+inline graph_hash_of_mixed_weighted graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_vertices(
+    device_graph& dg, const std::unordered_set<int>& hash_of_v) {
+    std::vector<int32_t> vs(hash_of_v.begin(), hash_of_v.end());
+    tree t;
+    if (vs.empty()) return t;
+    std::vector<int32_t> eu(vs.size()), ev(vs.size());
+    std::vector<double> ew(vs.size());
+    int32_t ne = 0;
+    double cost = 0.0;
+    dg.ctx().check(gst_mst_postprocess(dg.ctx().get(), dg.get(), vs.data(), (int32_t)vs.size(), eu.data(), ev.data(),
+                                       ew.data(), &ne, &cost));
+    for (int v : vs) t.add_vertex(v);
+    for (int32_t i = 0; i < ne; ++i) t.add_edge(eu[(size_t)i], ev[(size_t)i], ew[(size_t)i]);
+    return t;
 }
 
 // this_is_a_feasible_solution_gpu (rucgraph/test_gpu.h:45-77), strengthened: the edge set must be

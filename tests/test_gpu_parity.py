@@ -165,7 +165,9 @@ def test_cpp_wrapper_example():
     """The reference-shaped C++ API (include/gst.hpp: DPBF_GPU and DPBF_GPU_T with the reference's
     int* pointer1 / int* real_cost / double* rt, graph_hash_of_mixed_weighted_sum_of_ec,
     this_is_a_feasible_solution_gpu) through the C-ABI; the example exits non-zero unless both
-    calls return feasible trees whose int edge sums equal the returned cost."""
+    calls return feasible trees whose int edge sums equal the returned cost, and the MST post-
+    processing of the optimal tree's vertex set (graph_hash_of_mixed_weighted_MST_postprocessing_hash_of_
+    vertices) keeps that cost."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.dirname(__file__)), "gpu4gst_code_amd", "_build", "dpbf_gpu_example")
