@@ -319,6 +319,9 @@ __global__ void __launch_bounds__(MST_THREADS) k_batch_spider_tree(BatchArgs b) 
     const SlotBlock& sb = b.sb[p->merge_slot[i]];
     const MergeArgs& m2 = sb.m[2];
     if (!m2.spider) return;
+    // a job whose own bound (the learned one, typically within a few per cent of the optimum) is
+    // already well below the spider needs no tree (its walk costs ~50 us of the step)
+    if (spider_tree_useless<FLT>(m2.ub, (uint32_t)(*m2.spider >> 32))) return;
     __shared__ MstLds L;
     spider_tree_body<FLT>(L, m2.T + m2.lvl_base[1], m2.g, m2.gm, sb.g[1].row, sb.g[1].arc, m2.spider, nullptr);
 }

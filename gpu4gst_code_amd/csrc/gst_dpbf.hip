@@ -563,10 +563,11 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         const uint64_t* T1 = B->T + lvl_base[1];
         if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1, n, (uint32_t)g, B->spider);
         else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1, n, (uint32_t)g, B->spider);
-        // keep the spider's own cost, then MST post-process its paths (lowers word 0 when cheaper)
+        // keep the spider's own cost, then MST post-process its paths (lowers word 0 when cheaper;
+        // skipped on the device when the caller's bound is at or below 90% of the spider's cost)
         HIPCHK(hipMemcpyAsync(B->spider + 1, B->spider, 8, hipMemcpyDeviceToDevice, s));
-        if (flt) k_spider_tree<true><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr);
-        else k_spider_tree<false><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr);
+        if (flt) k_spider_tree<true><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr, ub_bits);
+        else k_spider_tree<false><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr, ub_bits);
         unsigned long long spw[2] = {GST_INF_WORD, GST_INF_WORD};
         HIPCHK(hipMemcpyAsync(spw, B->spider, 16, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

@@ -2425,10 +2425,19 @@ __device__ void spider_tree_body(MstLds& L, const uint64_t* T1, uint32_t g, cons
     }
 }
 
+// a job bound (cost bits ub, GST_INF_COST: none) at or below 90% of the spider's cost leaves the spider
+// no use: post-processing lowers it by a few per cent
+template <bool FLT>
+__device__ __forceinline__ bool spider_tree_useless(uint32_t ub, uint32_t sp) {
+    if (ub == GST_INF_COST || sp == GST_INF_COST) return false;
+    return FLT ? __uint_as_float(ub) <= 0.9f * __uint_as_float(sp) : (uint64_t)ub * 10u <= (uint64_t)sp * 9u;
+}
+
 template <bool FLT>
 __global__ void __launch_bounds__(MST_THREADS) k_spider_tree(const uint64_t* T1, uint32_t g, const uint32_t* gm,
                                                            const uint32_t* row, const uint2* arc,
-                                                           unsigned long long* spw, uint32_t* tree) {
+                                                           unsigned long long* spw, uint32_t* tree, uint32_t ub) {
+    if (spider_tree_useless<FLT>(ub, (uint32_t)(*spw >> 32))) return;
     __shared__ MstLds L;
     spider_tree_body<FLT>(L, T1, g, gm, row, arc, spw, tree);
 }
