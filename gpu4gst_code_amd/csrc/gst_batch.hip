@@ -231,6 +231,30 @@ __global__ void k_reset_phases(SlotBlock* sb) {
     if (threadIdx.x < MAX_SLOTS) sb[threadIdx.x].sd = SlotDev{PH_EMPTY, 0u, 0u, 0u};
 }
 
+// a slot whose level 1 (grown under its running spider) has ended: the spider's bound bounds the later
+// levels, and their lower bounds read level 1 capped at its half (every state at or below it is
+// exact: the last round's cap was at least that, and the caps before it were higher)
+template <bool FLT>
+__device__ __forceinline__ void level1_bound_to_later(SlotBlock& sb, uint32_t g) {
+    const unsigned long long sw = *(volatile unsigned long long*)sb.g[1].run_sp;
+    if ((uint32_t)(sw >> 32) == GST_INF_COST) return;
+    uint32_t ub = GST_INF_COST, gmax = GST_INF_COST;
+    running_spider_bound<FLT>(sw, ub, gmax);
+    for (uint32_t k = 2; k <= g; ++k) {
+        if (k < g) {
+            GrowArgs* ga = &sb.g[k];
+            ga->ub = min(ga->ub, ub);
+            ga->gmax = min(ga->gmax, gmax);
+            ga->lbcap = min(ga->lbcap, gmax);
+        }
+        MergeArgs* ma = &sb.m[k];
+        ma->ub = min(ma->ub, ub);
+        ma->gmax = min(ma->gmax, gmax);
+        ma->lbcap = min(ma->lbcap, gmax);
+    }
+}
+
+template <bool FLT>
 __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
     const uint32_t s = threadIdx.x;
     bool dm = false, dc = false, dt = false, df = false;
@@ -245,6 +269,7 @@ __global__ void __launch_bounds__(PLAN_THREADS) k_plan_step(BatchArgs b) {
             dt = true;
             sd->phase = PH_DONE;
         } else if (ph == PH_GROW && b.sb[s].g[sd->k].ctl->level_done) {
+            if (sd->k == 1 && b.sb[s].g[1].run_sp) level1_bound_to_later<FLT>(b.sb[s], sd->g);
             lvl = sd->k + 1;
             dm = true;
             if (lvl == sd->g) {
@@ -771,6 +796,11 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         a.use_lb = (job.opts.prune_lb && k >= 2) ? 1u : 0u;
         a.tour = k >= 2 ? tour : nullptr;
         a.act = (k >= 2 && use_act) ? sl.act + (size_t)k * n : nullptr;
+        // level 1 of an automatically bounded job is bounded by its running spider (the same word the
+        // level-2 merge lowers to the cheapest spider); the plan of the step that ends level 1 moves
+        // that bound to the later levels
+        a.run_sp = (k == 1 && job.opts.auto_bound && g >= 3 && job.opts.half_bound == 0 && GST_RUN_SPIDER) ? sl.spider
+                                                                                                          : nullptr;
         a.all_items = persistent ? E->all_items : sl.hub_items;
         a.n_all = persistent ? E->n_all : 0u;
         a.n_all_dev = persistent ? nullptr : sl.hub_cnt;
@@ -865,7 +895,8 @@ static void enqueue_step_a(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& b
         }
     }
     const size_t mlds = MERGE_TILE_BYTES + 8 * 256 + 8 + pairs_lds;
-    k_plan_step<<<1, PLAN_THREADS, 0, s>>>(ba);
+    if (flt) k_plan_step<true><<<1, PLAN_THREADS, 0, s>>>(ba);
+    else k_plan_step<false><<<1, PLAN_THREADS, 0, s>>>(ba);
     // the queries that entered slots since the last step: set-up in four passes over their slots
     const int fgrid = std::max<int>((int)MAX_SLOTS, ctx->cu_count * 4);
     k_batch_fill_reset<<<fgrid, 256, 0, s>>>(ba);

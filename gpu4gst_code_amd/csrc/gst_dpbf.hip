@@ -441,9 +441,13 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
             return gst_set_error(ctx, GST_OOM, "round log allocation failed");
         k_fill_u32<<<64, 256, 0, s>>>(B->rlog, 0xFFFFFFFFu, (uint64_t)NCTL * RLOG_ROUNDS * 4);
     }
+    // the automatic bound: the spider (g >= 3), and level 1 under its running spider (half bound on)
+    const bool spider_on = opts.auto_bound && g >= 3;
+    const bool run_spider = spider_on && opts.half_bound == 0 && GST_RUN_SPIDER;
     const uint32_t* tour_ptr = nullptr;   // set once level 1 is final
     auto run_grow = [&](int k) -> gst_status {
         GrowArgs ga;
+        std::memset(&ga, 0, sizeof(ga));
         ga.Tk = B->T + lvl_base[k];
         ga.T1 = B->T + lvl_base[1];
         ga.masks = B->masks + L.off[k];
@@ -458,6 +462,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ga.tour = k >= 2 ? tour_ptr : nullptr;
         // activity bytes once a bound is known (after level 1 with the automatic bound)
         ga.act = (k >= 2 && GST_MERGE_SKIP && ub_bits != GST_INF_COST) ? B->act + (size_t)k * n : nullptr;
+        // level 1 of an automatically bounded solve grows under its running spider (word 0)
+        ga.run_sp = (k == 1 && run_spider) ? B->spider : nullptr;
         ga.all_items = B->all_items; ga.n_all = B->n_all; ga.n_all_dev = nullptr; ga.n = n;
         {
             const int pm = opts.pull_permille == 0 ? 450 : opts.pull_permille;
@@ -542,6 +548,12 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     };
 
     gst_status st = GST_OK;
+    // the spider words: [0] the spider (tree) bound, [1] the cheapest spider's own cost, [2] the running
+    // spider as level 1 ended
+    if (spider_on) {
+        if (!B->spider && hipMalloc(&B->spider, 24) != hipSuccess) return gst_set_error(ctx, GST_OOM, "solve: allocation failed");
+        k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->spider, GST_INF_WORD, 3);
+    }
     // ---- level 1 ----
     {
         Ev* e = new_ev(1);
@@ -557,9 +569,10 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     }
     // ---- automatic bound: the cheapest spider once the group distances are final ----
     double spider_costs[2] = {0.0, 0.0};   // the spider's sum of distances, its post-processed tree
-    if (st == GST_OK && opts.auto_bound && g >= 3) {
-        if (!B->spider && hipMalloc(&B->spider, 16) != hipSuccess) return gst_set_error(ctx, GST_OOM, "solve: allocation failed");
-        k_fill_u64<<<1, 64, 0, s>>>((uint64_t*)B->spider, GST_INF_WORD, 1);
+    if (st == GST_OK && spider_on) {
+        // word 0 holds the running spider of level 1 (or inf): keep it, then lower word 0 to the
+        // cheapest spider of the final distances
+        HIPCHK(hipMemcpyAsync(B->spider + 2, B->spider, 8, hipMemcpyDeviceToDevice, s));
         const uint64_t* T1 = B->T + lvl_base[1];
         if (flt) k_spider_scan<true><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1, n, (uint32_t)g, B->spider);
         else k_spider_scan<false><<<grid_for(n, 256, fill_grid), 256, 0, s>>>(T1, n, (uint32_t)g, B->spider);
@@ -568,9 +581,13 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         HIPCHK(hipMemcpyAsync(B->spider + 1, B->spider, 8, hipMemcpyDeviceToDevice, s));
         if (flt) k_spider_tree<true><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr, ub_bits);
         else k_spider_tree<false><<<1, MST_THREADS, 0, s>>>(T1, (uint32_t)g, B->gm, G->d_row, (const uint2*)G->d_arc, B->spider, nullptr, ub_bits);
-        unsigned long long spw[2] = {GST_INF_WORD, GST_INF_WORD};
-        HIPCHK(hipMemcpyAsync(spw, B->spider, 16, hipMemcpyDeviceToHost, s));
+        unsigned long long spw[3] = {GST_INF_WORD, GST_INF_WORD, GST_INF_WORD};
+        HIPCHK(hipMemcpyAsync(spw, B->spider, 24, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        // level 1 grew under its running spider: its states up to half of the spider as it stood at
+        // the level's end are exact, so the later levels' lower bounds read level 1 capped there
+        if (run_spider && (uint32_t)(spw[2] >> 32) != GST_INF_COST)
+            bb.lbcap = std::min(bb.lbcap, gst_spider_gmax_bits((uint32_t)(spw[2] >> 32), flt));
         const uint32_t sp = (uint32_t)(spw[0] >> 32);
         if (sp != GST_INF_COST) {
             const uint32_t sp1 = (uint32_t)(spw[1] >> 32);

@@ -95,6 +95,9 @@ struct GrowArgs {
     const uint32_t* tour;       // tour table of the query (tour_table_body), nullptr: one-label only
     uint8_t* act;               // level k's activity bytes (act[v] = 1 once some (v, X) of level k is
                                 // finite; read by the merges above), nullptr at level 1
+    unsigned long long* run_sp; // level 1 of a solve with the automatic bound: the running spider word
+                                // (cost << 32 | centre), lowered by the push items and read as a bound by
+                                // every round (see run_spider_item); nullptr: off
 };
 
 // Activity bytes (and with them the merges' skipping of vertices with no finite lower state) are
@@ -102,6 +105,12 @@ struct GrowArgs {
 // checks would cost without skipping anything.  0 switches them off everywhere.
 #ifndef GST_MERGE_SKIP
 #define GST_MERGE_SKIP 1
+#endif
+
+// Level 1 of a solve with the automatic bound grows under its running spider (run_spider_item);
+// 0 grows it unbounded until the level-2 merge finds the spider.
+#ifndef GST_RUN_SPIDER
+#define GST_RUN_SPIDER 1
 #endif
 
 // A state of the grow's level was lowered from inf (old word `old`): its vertex now holds a finite
@@ -735,11 +744,42 @@ __device__ __forceinline__ void wave_lds_sync() {
 #ifndef GST_ITEMPAR_CODE
 #define GST_ITEMPAR_CODE 1
 #endif
+// Running spider of level 1 (automatic bound).  While level 1 grows, every tentative group distance
+// d_i(v) is the cost of a real path from v to group i (its predecessor chain), so for a vertex v that
+// holds a distance to every group, sum_i d_i(v) is the cost of a feasible structure (the union of
+// those paths, which contains a tree touching every group): an upper bound of the optimum.  The push
+// items of level 1 offer their vertex's sum (the lanes of the item's group add the row; chunk 0 of
+// an item only) to the slot's spider word by atomic-min, and every later round grows only states of
+// cost <= half of it (and records only states below it).  The bound only falls, so the level grows
+// a superset of what one run under the final bound grows, and every state of cost <= half the
+// final bound is exact: the lower bounds of the later levels read level 1 capped there.
+template <bool FLT>
+__device__ __forceinline__ void run_spider_item(const GrowArgs& a, bool on, uint32_t v, uint32_t pl, uint32_t PL) {
+    uint32_t acc = 0, o = 0;   // o: an int sum past 32 bits is only an infinite candidate here
+    for (uint32_t j = pl; on && j < a.g; j += PL) acc = add_cost<FLT>(acc, (uint32_t)(a.T1[(uint64_t)v * a.g + j] >> 32), o);
+    if (!on) acc = GST_INF_COST;
+    for (uint32_t d = 1; d < PL; d <<= 1) acc = add_cost<FLT>(acc, (uint32_t)__shfl_xor(acc, d, WAVE), o);
+    if (on && pl == 0 && acc != GST_INF_COST) {
+        const unsigned long long w = ((unsigned long long)acc << 32) | v;
+        if (w < *(volatile unsigned long long*)a.run_sp) atomicMin(a.run_sp, w);
+    }
+}
+// the bound bits of the running spider word sw, applied to (ub, gmax)
+template <bool FLT>
+__device__ __forceinline__ void running_spider_bound(unsigned long long sw, uint32_t& ub, uint32_t& gmax) {
+    const uint32_t sp = (uint32_t)(sw >> 32);
+    if (sp == GST_INF_COST) return;
+    uint32_t u, gm;
+    spider_bound<FLT>(sp, u, gm);
+    ub = min(ub, u);
+    gmax = min(gmax, gm);
+}
+
 template <bool FLT>
 __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q, PushWaveLds& S, const uint64_t* lcur,
                                                 const uint64_t* ccur, uint64_t* cnxt, uint32_t* inxt, uint64_t* lnxt,
                                                 uint32_t* cntn, uint32_t n_cur, uint32_t gwave, uint32_t nwaves,
-                                                bool prune_lb, uint32_t& st_items, uint32_t& st_arcs,
+                                                uint32_t rub, uint32_t rgmax, bool prune_lb, uint32_t& st_items, uint32_t& st_arcs,
                                                 uint32_t& st_relax, uint32_t& st_upd, uint32_t& st_enq,
                                                 uint32_t& st_new, uint32_t& st_atom, uint32_t& ovf) {
     const uint32_t lane = lane_id();
@@ -747,7 +787,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
     const uint32_t W = a.W, m = a.m;
     // Level 1 skips the bound check of its masks: its bits are set only by its own rounds (and the
     // members' leaves), for costs <= gmax, which only fall, and its bound is fixed while it grows
-    const bool filter = a.ub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1);
+    const bool filter = rub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1);
     // items per wave and step: the round's items spread over every wave taking part
     const uint32_t ipw = min((uint32_t)WAVE, max(1u, (n_cur + nwaves - 1) / nwaves));
     for (uint32_t base = gwave * ipw; base < n_cur; base += nwaves * ipw) {
@@ -791,7 +831,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                     if (filter && prune_lb) lb = label_lb<FLT>(a.T1, a.tour, pv, a.g, a.masks[prk], a.lbcap);
                 }
                 bool keep = pk && plen != 0 && pcv != GST_INF_COST;
-                if (filter) keep = keep && pcv <= a.gmax && !reaches_ub<FLT>(pcv, lb, a.ub);
+                if (filter) keep = keep && pcv <= rgmax && !reaches_ub<FLT>(pcv, lb, rub);
                 const uint64_t kb = __ballot(keep);
                 wave_lds_sync();   // every lane has read its src entry before the next batch's writes
                 if (kb == 0) continue;
@@ -855,7 +895,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                         uu[j] = 0xFFFFFFFFu;
                         wt[j] = rec[j].y;
                         if (xp[j] == 0xFFFFFFFFu) continue;
-                        if (a.ub == GST_INF_COST || !reaches_ub<FLT>(xcv[j], wt[j], a.ub)) uu[j] = rec[j].x;
+                        if (rub == GST_INF_COST || !reaches_ub<FLT>(xcv[j], wt[j], rub)) uu[j] = rec[j].x;
                         else atomicMin(&S.cut[xp[j]], xoff[j]);
                     }
                     // the neighbour's state and its change word, loaded together: a bit already set
@@ -877,10 +917,10 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                         cand[j] = add_cost<FLT>(xcv[j], wt[j], ovf);
                         const unsigned long long curw = cw0[j];
                         if (cand[j] == GST_INF_COST || cand[j] > (uint32_t)(curw >> 32)) continue;
-                        if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand[j], 0u, a.ub)) continue;
+                        if (rub != GST_INF_COST && reaches_ub<FLT>(cand[j], 0u, rub)) continue;
                         if ((((unsigned long long)cand[j] << 32) | xv[j]) >= curw) continue;
                         if (cand[j] == xcv[j]) mzero |= 1u << j; else mpos |= 1u << j;
-                        if (cand[j] <= a.gmax && !((cwp[j] >> (xrk[j] & 63u)) & 1ull)) mgrow |= 1u << j;
+                        if (cand[j] <= rgmax && !((cwp[j] >> (xrk[j] & 63u)) & 1ull)) mgrow |= 1u << j;
                     }
                     // the atomics of all IP_UNR relaxations back to back; the change bit beside the
                     // atomic-min (a bit set for a min that lost to a concurrent lower cost only
@@ -943,7 +983,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
                             }
                             if (!improved) continue;
                             st_upd++;
-                            if (cand[j] > a.gmax) continue;
+                            if (cand[j] > rgmax) continue;
                             const uint64_t bitv = 1ull << (xrk[j] & 63u);
                             const uint32_t r0u = a.row[u], r1u = a.row[u + 1];
                             const unsigned long long o2 =
@@ -972,6 +1012,10 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     __shared__ uint32_t s_ovf;
     RoundCtl* ctl = a.ctl;
     if (ctl->level_done) return;
+    // the round's bound: the level's own, lowered on level 1 of an automatically bounded solve by the
+    // running spider as it stood when the round started
+    uint32_t rub = a.ub, rgmax = a.gmax;
+    if (a.run_sp) running_spider_bound<FLT>(*a.run_sp, rub, rgmax);
     // (round, seq) is one 8-byte word: a workgroup that starts only after this launch's round has
     // ended (its participants all arrived and advanced the round) sees its own seq and stays out
     const unsigned long long rsq =
@@ -1029,7 +1073,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     uint32_t* inxt = a.inq[nxt];
     uint64_t* lnxt = a.list[nxt];
     uint32_t* cntn = &ctl->cnt[nxt];
-    const bool prune_lb = a.use_lb && a.ub != GST_INF_COST;
+    const bool prune_lb = a.use_lb && rub != GST_INF_COST;
 
     // ---- pull round, light rows: each lane group owns one row and each lane one mask of it ----
     // (tropical SpMM row-split: the lane scans the whole row, so no cross-lane reduction, and the
@@ -1066,7 +1110,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         const bool ok = lane_ok && s + j < deg;
                         const uint2 r = ok ? a.arc[rs + s + j] : make_uint2(0xFFFFFFFFu, 0u);
                         wt[j] = r.y;
-                        live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
+                        live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, rub);
                         nb[j] = live[j] ? r.x : 0xFFFFFFFFu;
                         st_arcs += (ok && t0 == 0) ? 1u : 0u;   // arcs read (once per row, lane t0 = 0)
                     }
@@ -1080,9 +1124,9 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     for (uint32_t j = 0; j < UNR; ++j) {
                         if (nb[j] == 0xFFFFFFFFu) continue;
                         st_relax++;
-                        if (cn[j] < ocost && cn[j] <= a.gmax) {
+                        if (cn[j] < ocost && cn[j] <= rgmax) {
                             const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
-                            if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
+                            if (cand != GST_INF_COST && !(rub != GST_INF_COST && reaches_ub<FLT>(cand, lb, rub))) {
                                 const unsigned long long nw = ((unsigned long long)cand << 32) | nb[j];
                                 if (cand > cn[j]) bs = nw < bs ? nw : bs;
                                 else if (cand < ocost) bz = nw < bz ? nw : bz;
@@ -1102,7 +1146,20 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 const uint64_t lmask = mchunk == 64 ? ~0ull : ((1ull << mchunk) - 1);
                 const uint64_t imp = __ballot(improved);
                 // only states that may still grow go on the frontier
-                const uint64_t fl = __ballot(improved && (uint32_t)(nwv >> 32) <= a.gmax);
+                const uint64_t fl = __ballot(improved && (uint32_t)(nwv >> 32) <= rgmax);
+                if (a.run_sp) {
+                    // level 1 (one mask per group: the whole row in the lane group): a row that
+                    // changed offers its vertex's sum of group distances to the running spider
+                    const uint32_t cst = lane_ok ? (uint32_t)(nwv >> 32) : GST_INF_COST;
+                    uint32_t acc = 0, o = 0;
+                    for (uint32_t j = 0; j < mchunk; ++j)
+                        acc = add_cost<FLT>(acc, (uint32_t)__shfl(cst, (int)(gslot * mchunk + j), WAVE), o);
+                    const bool row_imp = slot_ok && ((imp >> (gslot * mchunk)) & lmask) != 0;
+                    if (row_imp && t0 == 0 && acc != GST_INF_COST) {
+                        const unsigned long long w = ((unsigned long long)acc << 32) | u;
+                        if (w < *(volatile unsigned long long*)a.run_sp) atomicMin(a.run_sp, w);
+                    }
+                }
                 uint32_t enq_v = 0, enq_n = 0;
                 if (lane_ok && t0 == 0) {
                     st_upd += (uint32_t)__popcll((imp >> (gslot * mchunk)) & lmask);
@@ -1153,7 +1210,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const bool ok = lane_ok && e < e1;
                     const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
                     wt[j] = r.y;
-                    live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, a.ub);
+                    live[j] = ok && pull_arc_live<FLT>(wt[j], ocost, lb, rub);
                     nb[j] = live[j] ? r.x : 0xFFFFFFFFu;
                     st_arcs += (ok && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
                 }
@@ -1166,9 +1223,9 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     if (nb[j] == 0xFFFFFFFFu) continue;
                     st_relax++;
                     // only a cheaper neighbour can lower (or tie) our cost, and only one still grown
-                    if (cn[j] < ocost && cn[j] <= a.gmax) {
+                    if (cn[j] < ocost && cn[j] <= rgmax) {
                         const uint32_t cand = add_cost<FLT>(cn[j], wt[j], ovf);
-                        if (cand != GST_INF_COST && !(a.ub != GST_INF_COST && reaches_ub<FLT>(cand, lb, a.ub))) {
+                        if (cand != GST_INF_COST && !(rub != GST_INF_COST && reaches_ub<FLT>(cand, lb, rub))) {
                             const unsigned long long nw = ((unsigned long long)cand << 32) | nb[j];
                             if (cand > cn[j]) bs = nw < bs ? nw : bs;
                             else if (cand < ocost) bz = nw < bz ? nw : bz;
@@ -1207,7 +1264,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     ncost = (uint32_t)(bs >> 32);
                 }
                 if (improved) st_upd++;
-                if (improved && ncost <= a.gmax) {
+                if (improved && ncost <= rgmax) {
                     const uint64_t bitv = 1ull << (rk & 63);
                     unsigned long long* cw = (unsigned long long*)&cnxt[(uint64_t)u * W + (rk >> 6)];
                     const unsigned long long ob = atomicOr(cw, (unsigned long long)bitv);
@@ -1231,28 +1288,29 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
     const bool item_par = rpl == PUSH_ITEM_PAR;
     if (GST_ITEMPAR_CODE && !pull && item_par)
         push_items_body<FLT>(a, q, s_pw[threadIdx.x / WAVE], lcur, ccur, cnxt, inxt, lnxt, cntn, n_cur, gwave, nwaves,
-                             prune_lb, st_items, st_arcs, st_relax, st_upd, st_enq, st_new, st_atom, ovf);
+                             rub, rgmax, prune_lb, st_items, st_arcs, st_relax, st_upd, st_enq, st_new, st_atom, ovf);
     const uint32_t PL = item_par ? (uint32_t)WAVE : rpl, PG = WAVE / PL;   // PL: 8, 16, 32 or 64
     const uint32_t pgrp = lane / PL, pl = lane % PL;
     for (uint32_t base = gwave * PG; !pull && !item_par && base < n_cur; base += nwaves * PG) {
         const uint32_t i = base + pgrp;
         const bool has = i < n_cur;
-        uint32_t v = 0, e0 = 0, e1 = 0;
+        uint32_t v = 0, e0 = 0, e1 = 0, c = 0;
         if (has) {
             const uint64_t it = lcur[i];
             v = (uint32_t)it;
-            const uint32_t c = (uint32_t)(it >> 32);
+            c = (uint32_t)(it >> 32);
             const uint32_t rs = a.row[v], re = a.row[v + 1];
             e0 = rs + c * a.chunk;
             e1 = min(re, e0 + a.chunk);
         }
         st_items += wave_sum32(has && pl == 0 ? 1u : 0u);
+        if (a.run_sp) run_spider_item<FLT>(a, has && c == 0, v, pl, PL);
         for (uint32_t wd = 0; wd < W; ++wd) {
             uint64_t bm = has ? ccur[(uint64_t)v * W + wd] : 0ull;
             // drop masks whose cost (plus lower bound) reaches the bound.  Level 1 skips the check:
             // its bits are set only by its own rounds (and the members' leaves), for costs <= gmax,
             // which only fall, and its bound does not change while it grows (no lower bound there)
-            if (a.ub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1)) {
+            if (rub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1)) {
                 const uint32_t na0 = (uint32_t)__popcll(bm);
                 uint64_t keep = 0;
                 for (uint32_t t0 = pl; t0 < na0; t0 += PL) {
@@ -1260,7 +1318,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     const uint32_t rk = wd * 64 + bit;
                     const uint32_t cv = (uint32_t)(a.Tk[(uint64_t)v * m + rk] >> 32);
                     const uint32_t lb = prune_lb ? label_lb<FLT>(a.T1, a.tour, v, a.g, a.masks[rk], a.lbcap) : 0u;
-                    if (cv <= a.gmax && !reaches_ub<FLT>(cv, lb, a.ub)) keep |= 1ull << bit;
+                    if (cv <= rgmax && !reaches_ub<FLT>(cv, lb, rub)) keep |= 1ull << bit;
                 }
                 bm = group_or64(keep, PL);
             }
@@ -1290,7 +1348,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         const uint2 r = in_row ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
                         wt[j] = r.y;
                         // under a bound, cv + w >= ub holds for every heavier arc of the row as well
-                        live[j] = ok && (a.ub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], a.ub));
+                        live[j] = ok && (rub == GST_INF_COST || !reaches_ub<FLT>(cv, wt[j], rub));
                         uu[j] = live[j] ? r.x : 0xFFFFFFFFu;
                         st_arcs += (in_row && t == 0) ? 1u : 0u;   // arcs read (once per arc slot, mask lane 0)
                     }
@@ -1315,12 +1373,12 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                         cand[j] = add_cost<FLT>(cv, wt[j], ovf);
                         const unsigned long long curw = cw0[j];
                         if (cand[j] == GST_INF_COST || cand[j] > (uint32_t)(curw >> 32)) continue;
-                        if (a.ub != GST_INF_COST && reaches_ub<FLT>(cand[j], 0u, a.ub)) continue;
+                        if (rub != GST_INF_COST && reaches_ub<FLT>(cand[j], 0u, rub)) continue;
                         if ((((unsigned long long)cand[j] << 32) | v) >= curw) continue;
                         // cand == cv: an effective zero-weight edge (w == 0, or a float sum that
                         // rounds back to cv), taken only on a strict cost decrease below
                         if (cand[j] == cv) mzero |= 1u << j; else mpos |= 1u << j;
-                        if (cand[j] <= a.gmax) mgrow |= 1u << j;
+                        if (cand[j] <= rgmax) mgrow |= 1u << j;
                     }
                     const uint32_t mbit = mpos & mgrow;
                     unsigned long long ob[UNR], old[UNR];
@@ -1379,7 +1437,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                             }
                             if (!improved) continue;
                             st_upd++;
-                            if (cand[j] > a.gmax) continue;
+                            if (cand[j] > rgmax) continue;
                             const uint32_t r0 = a.row[u], r1 = a.row[u + 1];
                             const unsigned long long o2 = atomicOr((unsigned long long*)&cnxt[(uint64_t)u * W + wsel],
                                                                    (unsigned long long)bitv);
@@ -1399,7 +1457,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                             const uint32_t cand = add_cost<FLT>(cv, wt[j], ovf);
                             unsigned long long curw = cw0[j];
                             if (cand != GST_INF_COST && cand <= (uint32_t)(curw >> 32) &&
-                                (a.ub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, a.ub))) {
+                                (rub == GST_INF_COST || !reaches_ub<FLT>(cand, 0u, rub))) {
                                 unsigned long long* addr = (unsigned long long*)(a.Tk + (uint64_t)u * m + rk);
                                 const unsigned long long nw = ((unsigned long long)cand << 32) | v;
                                 bool improved = false;
@@ -1421,7 +1479,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                                     }
                                 }
                                 if (improved) st_upd++;
-                                if (improved && cand <= a.gmax) {
+                                if (improved && cand <= rgmax) {
                                     unsigned long long* cwp = (unsigned long long*)&cnxt[(uint64_t)u * W + wsel];
                                     // u's row bounds are read beside the atomic OR, not after it
                                     const uint32_t ru0 = a.row[u], ru1 = a.row[u + 1];

@@ -98,6 +98,24 @@ inline int64_t gst_find_arc(const int64_t* row, const int32_t* col, const uint32
 struct gst_bound_bits {
     uint32_t ub = 0xFFFFFFFFu, gmax = 0xFFFFFFFEu, lbcap = 0xFFFFFFFFu;
 };
+// The grow cap (gmax bits) the device derives from a spider cost sp (spider_bound in the device code,
+// the same float operations: single precision, round to nearest).
+inline uint32_t gst_spider_gmax_bits(uint32_t sp, bool flt) {
+    if (!flt) return sp / 2u;
+    float f;
+    std::memcpy(&f, &sp, 4);
+    const float u = f * 1.0001f;
+    uint32_t ub;
+    std::memcpy(&ub, &u, 4);
+    ub += 1u;
+    float uf;
+    std::memcpy(&uf, &ub, 4);
+    const float h = uf * 0.5f;
+    uint32_t gm;
+    std::memcpy(&gm, &h, 4);
+    return gm + 1u;
+}
+
 inline gst_bound_bits gst_make_bound(double U, bool flt, int32_t half_bound) {
     gst_bound_bits b;
     if (!(std::isfinite(U) && U > 0)) return b;

@@ -40,6 +40,11 @@
 #ifndef GST_SELECT_DEPTH
 #define GST_SELECT_DEPTH 4   // selections queued at once over all selection streams
 #endif
+// forward dense layers: workgroups per CU (each wave holds all of W in registers, loaded once, so
+// fewer waves with more row tiles each read W less often)
+#ifndef GST_DENSE_GRID
+#define GST_DENSE_GRID 2
+#endif
 #ifndef GST_SELECT_AHEAD
 #define GST_SELECT_AHEAD 8
 #endif
@@ -1165,7 +1170,7 @@ gst_status forward(gst_model* M, FwdSet& F, const gst_graph* G, const gst_groups
     if (sst != GST_OK) return sst;
     const uint32_t n = (uint32_t)G->n;
     const int fg = std::max(1, ctx->cu_count * 8);
-    const int dg = std::max(1, ctx->cu_count * 8);
+    const int dg = std::max(1, ctx->cu_count * GST_DENSE_GRID);
     k_features<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, G->d_w, G->w_type == GST_W_FLOAT32 ? 1 : 0, F.gm, n, F.x);
     k_ax_finish<<<gridn(n, 256, fg), 256, 0, st>>>(G->d_row, M->ax_static, F.gcnt, n, F.ax);
     k_dense_fwd<FIN, false><<<dg, 256, 0, st>>>(F.x, F.ax, M->W1t, M->p + OFF_B1, n, F.h1, nullptr, nullptr, nullptr);

@@ -152,6 +152,31 @@ def test_auto_bound_prunes_ba20k(solver):
     dg.free()
 
 
+def test_running_spider_bounds_level1(solver):
+    """Level 1 of an automatically bounded solve grows under its running spider (the cheapest sum of
+    tentative group distances seen so far, each a feasible structure's cost), not unbounded until the
+    level-2 merge: its relaxations stay within 11x of the same solve given the final spider-tree bound
+    up front (without the running bound level 1 grows to every distance: 12-24x on these queries, 16x
+    summed), one query at a time and in the batched engine, and the optimum is kept."""
+    G = gst.gen_barabasi_albert(20000, 5, 1, 1000, seed=11)
+    dg = solver.upload(G)
+    qs = [gst.gen_groups(G.n, 6, 10, 200, seed=1000 + q) for q in range(3)]
+    want = [GOLD["ba20k"]["costs"][str(q)] for q in range(3)]
+    auto = [solver.solve(dg, q, auto_bound=True) for q in qs]
+    tree_ub = [a.stats["spider_tree_cost"] for a in auto]
+    upfront = [solver.solve(dg, q, auto_bound=True, ub=u) for q, u in zip(qs, tree_ub)]
+    bauto, _ = solver.solve_batch(dg, qs, auto_bound=True, slots=3)
+    bup, _ = solver.solve_batch(dg, qs, auto_bound=True, slots=3, ubs=tree_ub)
+    for q, w, a, u, b, c in zip(qs, want, auto, upfront, bauto, bup):
+        for r in (a, u, b, c):
+            _exact(G, q, r, w)
+    for name, xs, ys in (("one query", auto, upfront), ("batched", bauto, bup)):
+        ratio = sum(r.stats["grow_relax"] for r in xs) / sum(r.stats["grow_relax"] for r in ys)
+        print(f"{name}: auto-bound relaxations {ratio:.2f}x those under the spider tree given up front", flush=True)
+        assert ratio < 11.0, (name, ratio)
+    dg.free()
+
+
 def test_tour_bound_exact_and_prunes(solver):
     """The tour bounds (on by default) keep the optimum at the tightest bound U = OPT and relax
     fewer states than the one-label bound alone where groups lie far apart (C1-type graphs, g=6);
