@@ -330,7 +330,22 @@ __global__ void __launch_bounds__(MERGE_THREADS) __attribute__((amdgpu_waves_per
     const uint32_t per = max(1u, nblk / nm);   // nblk >= MAX_SLOTS, so per >= 1 covers all
     const uint32_t i = bid / per;
     if (i >= nm) return;
-    merge_body<FLT>(b.sb[p->merge_slot[i]].m[p->merge_level[i]], lds, bid - i * per, per);
+    const MergeArgs& ma = b.sb[p->merge_slot[i]].m[p->merge_level[i]];
+    if (merge_uses_list(ma)) merge_body<FLT, true>(ma, lds, bid - i * per, per);
+    else merge_body<FLT, false>(ma, lds, bid - i * per, per);
+}
+
+// before the merges: the pre-pass of every merge that walks its live list (dead vertices written,
+// live ones listed), the grid split evenly among the listed merges
+__global__ void __launch_bounds__(256) k_batch_merge_live(BatchArgs b) {
+    const BatchPlan* p = b.plan;
+    const uint32_t nm = p->n_merge;
+    if (nm == 0) return;
+    const uint32_t per = max(1u, gridDim.x / nm);
+    const uint32_t i = blockIdx.x / per;
+    if (i >= nm) return;
+    const MergeArgs& ma = b.sb[p->merge_slot[i]].m[p->merge_level[i]];
+    if (merge_uses_list(ma)) merge_live_body(ma, blockIdx.x - i * per, per);
 }
 
 // after the merges: one workgroup per slot whose level-2 merge found a spider (automatic bound)
@@ -497,6 +512,7 @@ struct gst_batch_engine {
                                                 // (cost << 32 | centre), lowered by its post-processed tree
         uint32_t* tour = nullptr;           // tour table 2^g * (g + 1) words, then walk scratch 2^g * g * g
         uint8_t* act = nullptr;             // activity bytes act[k * n + v] of the job's levels
+        uint32_t* live = nullptr;           // the live list of the slot's current merge (n entries)
         int tour_g = 0;
         uint64_t* hub_items = nullptr;      // hub row items of a job's own graph (pull rounds)
         uint32_t* hub_cnt = nullptr;
@@ -556,7 +572,7 @@ static size_t back_bytes(uint32_t ecap) { return BK_EDGES + 12 * (size_t)ecap; }
 static void free_slot(gst_batch_engine::Slot& sl) {
     hipFree(sl.T); hipFree(sl.gm);
     for (int i = 0; i < 3; ++i) { hipFree(sl.chg[i]); hipFree(sl.inq[i]); hipFree(sl.lists[i]); }
-    hipFree(sl.back); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour); hipFree(sl.act);
+    hipFree(sl.back); hipFree(sl.grp); hipFree(sl.spider); hipFree(sl.tour); hipFree(sl.act); hipFree(sl.live);
     hipFree(sl.hub_items); hipFree(sl.hub_cnt);
     if (sl.h_grp) hipHostFree(sl.h_grp);
     if (sl.h_sb) hipHostFree(sl.h_sb);
@@ -597,7 +613,7 @@ void gst_batch_free(gst_ctx* ctx) {
 // bytes of device memory one slot needs
 static size_t slot_bytes(uint32_t n, int g, uint64_t items_cap, uint32_t W, uint32_t ecap) {
     const size_t words = (size_t)n * (((size_t)1 << g) - 1);
-    return words * 8 + (size_t)n * 4 + (size_t)n * (size_t)(g + 1) + 3 * ((size_t)n * W * 8 + (size_t)n * 4 + items_cap * 8) +
+    return words * 8 + (size_t)n * 8 + (size_t)n * (size_t)(g + 1) + 3 * ((size_t)n * W * 8 + (size_t)n * 4 + items_cap * 8) +
            sizeof(RoundCtl) * NCTL + 8 + 8 * STAT_WORDS + (3 * (size_t)ecap + 1) * 4 + 64 * 1024;
 }
 
@@ -619,7 +635,8 @@ static gst_status ensure_slots(gst_ctx* ctx, gst_batch_engine* E, int S, uint32_
     const size_t words = (size_t)n * (((size_t)1 << g) - 1);
     for (int i = 0; ok && i < S; ++i) {
         auto& sl = E->slots[(size_t)i];
-        ok = ok && dalloc(sl.T, words) && dalloc(sl.gm, n) && dalloc(sl.act, (size_t)n * (size_t)(g + 1));
+        ok = ok && dalloc(sl.T, words) && dalloc(sl.gm, n) && dalloc(sl.act, (size_t)n * (size_t)(g + 1)) &&
+             dalloc(sl.live, n);
         for (int j = 0; ok && j < 3; ++j)
             ok = dalloc(sl.chg[j], (size_t)n * W) && dalloc(sl.inq[j], n) && dalloc(sl.lists[j], items_cap);
         ok = ok && dalloc(sl.back, back_bytes(ecap)) && dalloc(sl.spider, 1) &&
@@ -847,6 +864,7 @@ static gst_status fill_slot(gst_ctx* ctx, gst_batch_engine* E, int si, gst_batch
         ma.grp = sl.grp;
         ma.tour_w = tour ? sl.tour + ((size_t)1 << g) * (size_t)(g + 1) : nullptr;
         ma.act = use_act ? sl.act : nullptr;
+        ma.live = (use_act && k >= 3) ? sl.live : nullptr;
         ma.is_full = (k == g) ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], Wk = (uint32_t)((L.m[k] + 63) / 64);
         ma.VB = merge_tile_vertices(Lk, Wk);
@@ -908,6 +926,7 @@ static void enqueue_step_a(gst_ctx* ctx, gst_batch_engine* E, const BatchArgs& b
     const size_t tl = tour_lds_bytes((uint32_t)E->g_cap);
     const bool pw_lds = tl != 0 && tl <= mlds;
     if (ev) hipEventRecord(ev[0], s);
+    k_batch_merge_live<<<mgrid, 256, 0, s>>>(ba);
     if (pw_lds) {
         if (flt) k_batch_merge<true, true><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);
         else k_batch_merge<false, true><<<mgrid + MAX_SLOTS, MERGE_THREADS, mlds, s>>>(ba);

@@ -86,6 +86,8 @@ struct gst_query_bufs {
     uint32_t* rlog = nullptr;    // round log of every level (GST_TRACE_ROUNDS), NCTL * RLOG_ROUNDS * 4 words
     uint8_t* act = nullptr;      // activity bytes, act[k * n + v] (levels 2..g-1)
     uint64_t cap_act = 0;
+    uint32_t* live = nullptr;    // live list of a bounded merge (n entries)
+    uint64_t cap_live = 0;
 };
 
 static void free_bufs(gst_query_bufs* b) {
@@ -95,7 +97,7 @@ static void free_bufs(gst_query_bufs* b) {
     hipFree(b->ctl); hipFree(b->root); hipFree(b->stats); hipFree(b->edges); hipFree(b->grp);
     hipFree(b->pairs); hipFree(b->masks); hipFree(b->pos_of);
     hipFree(b->all_items); hipFree(b->all_cnt);
-    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider); hipFree(b->tour); hipFree(b->act);
+    hipFree(b->ring); hipFree(b->actl); hipFree(b->rlog); hipFree(b->spider); hipFree(b->tour); hipFree(b->act); hipFree(b->live);
     if (b->h_ctl) hipHostFree(b->h_ctl);
     for (auto e : b->evs) hipEventDestroy(e);
     delete b;
@@ -340,6 +342,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
     }
     ok = ok && grow_alloc(B->grp, B->cap_grp, grp_need, 4);
     ok = ok && grow_alloc(B->act, B->cap_act, (uint64_t)n * (uint64_t)(g + 1), 1);
+    ok = ok && grow_alloc(B->live, B->cap_live, (uint64_t)n, 4);
     if (!ok) {
         gst_query_bufs_free(ctx);
         return gst_set_error(ctx, GST_OOM, "solve: device allocation failed (V=%u, g=%d, table %.2f GB)",
@@ -649,6 +652,7 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         ma.grp = nullptr;
         ma.tour_w = nullptr;
         ma.act = (GST_MERGE_SKIP && ub_bits != GST_INF_COST) ? B->act : nullptr;
+        ma.live = k >= 3 ? B->live : nullptr;
         ma.is_full = full ? 1u : 0u;
         const uint32_t Lk = (uint32_t)L.off[k], mk = (uint32_t)L.m[k], Wk = (mk + 63) / 64;
         const size_t pair_bytes = (size_t)L.pair_cnt[k] * 8;
@@ -660,6 +664,8 @@ gst_status gst_dpbf_solve(gst_ctx* ctx, const gst_graph* G, const gst_groups* gr
         const int mgrid = std::max(1, std::min<int>((int)((n + VB - 1) / VB), ctx->cu_count * 8));
         Ev* e = new_ev(1);
         hipEventRecord(e->a, s);
+        if (merge_uses_list(ma))
+            k_merge_live<<<std::max(1, std::min<int>((int)((n + 255) / 256), ctx->cu_count * 8)), 256, 0, s>>>(ma);
         if (flt) k_merge<true><<<mgrid, MERGE_THREADS, lds, s>>>(ma);
         else k_merge<false><<<mgrid, MERGE_THREADS, lds, s>>>(ma);
         hipEventRecord(e->b, s);

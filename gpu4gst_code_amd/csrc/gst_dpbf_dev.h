@@ -32,6 +32,7 @@ struct RoundCtl {
     uint32_t level_done;
     uint32_t err;           // bit0: int cost overflow, bit1: traceback failure
     uint32_t top;           // arrival counter over block groups
+    uint32_t live;          // the level's merge: vertices its pre-pass listed as live
     uint32_t sub[64];       // arrival counters of groups of 32 workgroups
 };
 
@@ -221,6 +222,9 @@ struct MergeArgs {
     const int32_t* grp;    // grp_ptr[0..g] then the members
     uint32_t* tour_w;      // 2^g * g * g words
     uint8_t* act;          // activity bytes of every level, act[j * n + v] (nullptr: no skipping)
+    uint32_t* live;        // with act and k >= 3: the list of live vertices (n entries) that the
+                           // pre-pass (merge_live_body) builds and the merge walks; nullptr: the
+                           // merge walks every vertex in tiles and writes the dead ones itself
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -1962,7 +1966,59 @@ __device__ __forceinline__ uint32_t div_small(uint32_t i, float inv_d) {
     return (uint32_t)(((float)i + 0.5f) * inv_d);
 }
 
-template <bool FLT>
+// a bounded merge of level k >= 3 with activity bytes walks the list of its live vertices
+__host__ __device__ __forceinline__ bool merge_uses_list(const MergeArgs& a) {
+    return a.live != nullptr && a.act != nullptr && a.k >= 3;
+}
+
+// Pre-pass of a merge that walks its live list.  A vertex is live when it belongs to at least k groups
+// or holds a finite state at some level 2..k-1 (no other vertex can hold a finite level-k state: no
+// split pair of it is finite and it is no leaf).  Workgroups of 256 threads take 256-vertex tiles: a
+// dead vertex's level-k words are written inf (coalesced, one thread per word) and its level-k
+// activity byte cleared; a live vertex is appended to the list (one counter atomic per wave).
+__device__ __forceinline__ void merge_live_body(const MergeArgs& a, uint32_t bid, uint32_t nblk) {
+    __shared__ uint32_t s_dead[256];
+    const uint32_t k = a.k, mk = a.m[k], n = a.n;
+    const float inv_mk = 1.0f / (float)mk;
+    uint64_t* Tk = a.T + a.lvl_base[k];
+    uint8_t* act_out = a.is_full ? nullptr : a.act + (size_t)k * n;
+    uint64_t st_bytes = 0;
+    for (uint32_t v0 = bid * 256u; v0 < n; v0 += nblk * 256u) {
+        const uint32_t nv = min(256u, n - v0);
+        const uint32_t v = v0 + threadIdx.x;
+        bool live = false;
+        if (threadIdx.x < nv) {
+            live = (uint32_t)__popc(a.gm[v]) >= k;
+            for (uint32_t j = 2; j < k && !live; ++j) live = a.act[(size_t)j * n + v] != 0;
+            if (!live && act_out) act_out[v] = 0;
+            // this vertex's group mask and activity bytes read, its activity byte cleared
+            st_bytes += 4 + (k - 2) + ((!live && act_out) ? 1 : 0);
+        }
+        const uint64_t bal = __ballot(live);
+        uint32_t base = 0;
+        if (lane_id() == 0 && bal) base = atomicAdd(&a.ctl->live, (uint32_t)__popcll(bal));
+        base = __shfl(base, 0, WAVE);
+        if (live) a.live[base + (uint32_t)__popcll(bal & ((1ull << lane_id()) - 1ull))] = v;
+        s_dead[threadIdx.x] = (threadIdx.x < nv && !live) ? 1u : 0u;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nv * mk; i += 256u) {
+            const uint32_t b = div_small(i, inv_mk);
+            if (s_dead[b]) {
+                Tk[(uint64_t)v0 * mk + i] = GST_INF_WORD;
+                st_bytes += 8;   // a dead vertex's word written
+            }
+        }
+        __syncthreads();
+    }
+    __shared__ unsigned long long s_b;
+    if (threadIdx.x == 0) s_b = 0;
+    __syncthreads();
+    if (st_bytes) atomicAdd(&s_b, (unsigned long long)st_bytes);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_b) atomicAdd(&stat_stripe(a.stats, bid)[ST_MBYTES], s_b);
+}
+
+template <bool FLT, bool LIST>
 __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long long* lds, uint32_t bid, uint32_t nblk) {
     __shared__ uint32_t s_nl;
     const uint32_t k = a.k;
@@ -2034,17 +2090,27 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
         }
     }
 
-    for (uint32_t v0 = bid * VB; v0 < a.n; v0 += nblk * VB) {
-        const uint32_t nv = min(VB, a.n - v0);
+    // LIST: tile t is entries t * VB ... of the live list (absolute vertex ids in lst, the per-vertex
+    // LDS arrays indexed by list position, v0 = 0); the pre-pass has written the dead vertices
+    const uint32_t n_tile = LIST ? a.ctl->live : a.n;
+    for (uint32_t t = bid; t * VB < n_tile; t += nblk) {
+        const uint32_t v0 = LIST ? 0u : t * VB;
+        const uint32_t nv = min(VB, n_tile - t * VB);
         __syncthreads();
-        if (threadIdx.x == 0) s_nl = skip ? 0u : nv;
+        if (threadIdx.x == 0) s_nl = (skip && !LIST) ? 0u : nv;
         for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
-            gms[b] = a.gm[v0 + b];
+            if (LIST) {
+                const uint32_t v = a.live[t * VB + b];
+                lst[b] = v;
+                gms[b] = a.gm[v] | 0x80000000u;
+            } else {
+                gms[b] = a.gm[v0 + b];
+                if (!skip) lst[b] = b;
+            }
             fin[b] = 0u;
-            if (!skip) lst[b] = b;
         }
         __syncthreads();
-        if (skip) {
+        if (skip && !LIST) {
             for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
                 bool live = (uint32_t)__popc(gms[b]) >= k;
                 for (uint32_t j = 2; j < k && !live; ++j) live = a.act[(size_t)j * a.n + v0 + b] != 0;
@@ -2056,7 +2122,7 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
             __syncthreads();
         }
         const uint32_t nl = s_nl;
-        if (skip && nl < nv) {
+        if (skip && !LIST && nl < nv) {
             // the non-live vertices' level-k words: inf, written without reading their rows
             for (uint32_t i = threadIdx.x; i < nv * mk; i += blockDim.x) {
                 const uint32_t b = div_small(i, inv_mk);
@@ -2122,8 +2188,8 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
         const uint32_t items = nl * mk;
         for (uint32_t i = threadIdx.x; threadIdx.x < istride && i < items; i += istride) {
             const uint32_t li = div_small(i, inv_mk), r = i - li * mk;
-            const uint32_t b = lst[li];
-            const uint32_t v = v0 + b;
+            const uint32_t b = LIST ? li : lst[li];
+            const uint32_t v = LIST ? lst[li] : v0 + b;
             const uint32_t X = mk_masks[r];
             const uint32_t gmv = gms[b] & 0x7FFFFFFFu;
             const uint32_t* rv = rows + (size_t)li * Lp;
@@ -2186,15 +2252,16 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
         }
         st_pairs += (uint64_t)items * S;
         st_vert += nl;
-        // bytes: live rows read, every level-k word written, group masks, activity bytes
+        // bytes: live rows read, every level-k word written, group masks, activity bytes (with the
+        // live list: the live vertices' share and the list read; the pre-pass counts the rest)
         st_bytes += (uint64_t)nl * L * 8 + (uint64_t)nv * mk * 8 + (uint64_t)nv * 4 +
-                    (skip ? (uint64_t)nv * (k - 2) : 0) + (act_out ? nv : 0);
+                    ((skip && !LIST) ? (uint64_t)nv * (k - 2) : 0) + (act_out ? nv : 0) + (LIST ? (uint64_t)nv * 4 : 0);
         __syncthreads();
         if (act_out)
-            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) act_out[v0 + b] = (uint8_t)fin[b];
+            for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) act_out[LIST ? lst[b] : v0 + b] = (uint8_t)fin[b];
         if (!a.is_full) {
             for (uint32_t b = threadIdx.x; b < nv; b += blockDim.x) {
-                const uint32_t v = v0 + b;
+                const uint32_t v = LIST ? lst[b] : v0 + b;
                 unsigned long long any = 0;
                 for (uint32_t w = 0; w < W; ++w) any |= bm[b * W + w];
                 if (!any || a.row[v + 1] == a.row[v]) continue;   // nothing to grow, or no arcs
@@ -2220,10 +2287,13 @@ __device__ __forceinline__ void merge_body(const MergeArgs& a, unsigned long lon
     if (lane_id() == 0 && so) atomicOr(&a.ctl->err, 1u);
 }
 
+__global__ void __launch_bounds__(256) k_merge_live(MergeArgs a) { merge_live_body(a, blockIdx.x, gridDim.x); }
+
 template <bool FLT>
 __global__ void __launch_bounds__(MERGE_THREADS) __attribute__((amdgpu_waves_per_eu(GST_MERGE_WPE))) k_merge(MergeArgs a) {
     extern __shared__ unsigned long long lds[];
-    merge_body<FLT>(a, lds, blockIdx.x, gridDim.x);
+    if (merge_uses_list(a)) merge_body<FLT, true>(a, lds, blockIdx.x, gridDim.x);
+    else merge_body<FLT, false>(a, lds, blockIdx.x, gridDim.x);
 }
 
 // cheapest spider: min over v of sum_i T(v,{i}) (the union of v's shortest paths to every group
