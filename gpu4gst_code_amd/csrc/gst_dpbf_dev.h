@@ -800,7 +800,8 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t W = a.W, m = a.m;
     // Level 1 skips the bound check of its masks: its bits are set only by its own rounds (and the
-    // members' leaves), for costs <= gmax, which only fall, and its bound is fixed while it grows
+    // members' leaves), for costs <= the cap of the round that set them, which only fall; a running
+    // spider can lower the cap meanwhile, and such a state is then grown once more, which is harmless
     const bool filter = rub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1);
     // items per wave and step: the round's items spread over every wave taking part
     const uint32_t ipw = min((uint32_t)WAVE, max(1u, (n_cur + nwaves - 1) / nwaves));
@@ -1322,8 +1323,9 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
         for (uint32_t wd = 0; wd < W; ++wd) {
             uint64_t bm = has ? ccur[(uint64_t)v * W + wd] : 0ull;
             // drop masks whose cost (plus lower bound) reaches the bound.  Level 1 skips the check:
-            // its bits are set only by its own rounds (and the members' leaves), for costs <= gmax,
-            // which only fall, and its bound does not change while it grows (no lower bound there)
+            // its bits are set only by its own rounds (and the members' leaves), for costs <= the cap
+            // of the round that set them, which only fall (no lower bound there; a running spider can
+            // lower the cap meanwhile, and such a state is then grown once more, which is harmless)
             if (rub != GST_INF_COST && !(GST_LEVEL1_NOFILTER && a.Tk == a.T1)) {
                 const uint32_t na0 = (uint32_t)__popcll(bm);
                 uint64_t keep = 0;
