@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu
         if (p->grow_start[mid] <= bid) lo = mid; else hi = mid;
     }
     const uint32_t st = p->grow_start[lo];
-    grow_round_body<FLT, GST_PULL_UNR_BATCH>(b.sb[p->grow_slot[lo]].g[p->grow_level[lo]], bid - st, p->grow_start[lo + 1] - st,
+    grow_round_body<FLT>(b.sb[p->grow_slot[lo]].g[p->grow_level[lo]], bid - st, p->grow_start[lo + 1] - st,
                          p->seq);
 }
 

@@ -626,16 +626,6 @@ __global__ void __launch_bounds__(256) k_tour_table(const uint64_t* T1, const in
 #define GST_GROW_WPE 5
 #endif
 constexpr uint32_t UNR = GST_GROW_UNR;   // arc steps issued together (independent gathers in flight per wave)
-// pull rounds: arcs per lane whose neighbour states are gathered together (a row's lane group walks
-// its row in steps of this many arcs), by kernel.  The one-query grow takes 4 (plain BA V=200k g=6
-// solves: 164 -> 173 queries/s); in the batched grow, whose push path holds more registers, 4 spill
-// 14 registers instead of 7 and the learned-bound stream loses 2%.
-#ifndef GST_PULL_UNR_SOLO
-#define GST_PULL_UNR_SOLO 4
-#endif
-#ifndef GST_PULL_UNR_BATCH
-#define GST_PULL_UNR_BATCH 2
-#endif
 // Rows longer than this are "hubs": pull rounds split them into chunk items combined with atomics;
 // shorter rows are pulled whole by one lane group with plain stores.
 constexpr uint32_t HUB_DEG = 128;
@@ -1019,7 +1009,7 @@ __device__ __forceinline__ void push_items_body(const GrowArgs& a, BlockQueue& q
 // ---------------------------------------------------------------------------------------
 // grow: one round of frontier relaxation for all masks of one level
 // ---------------------------------------------------------------------------------------
-template <bool FLT, uint32_t PULL_UNR>
+template <bool FLT>
 __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid, uint32_t nblk, uint32_t seq) {
     __shared__ BlockQueue q;
     __shared__ PushWaveLds s_pw[GROW_WAVES];
@@ -1117,11 +1107,11 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 const uint32_t ocost = (uint32_t)(own >> 32);
                 const uint32_t lb = (prune_lb && lane_ok) ? label_lb<FLT>(a.T1, a.tour, u, a.g, a.masks[rk], a.lbcap) : 0u;
                 unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;
-                for (uint32_t s = 0; s < maxdeg; s += PULL_UNR) {
-                    uint32_t nb[PULL_UNR], wt[PULL_UNR], cn[PULL_UNR];
-                    bool live[PULL_UNR];
+                for (uint32_t s = 0; s < maxdeg; s += UNR) {
+                    uint32_t nb[UNR], wt[UNR], cn[UNR];
+                    bool live[UNR];
 #pragma unroll
-                    for (uint32_t j = 0; j < PULL_UNR; ++j) {
+                    for (uint32_t j = 0; j < UNR; ++j) {
                         const bool ok = lane_ok && s + j < deg;
                         const uint2 r = ok ? a.arc[rs + s + j] : make_uint2(0xFFFFFFFFu, 0u);
                         wt[j] = r.y;
@@ -1133,10 +1123,10 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                     // lane, the rest of every row is dead too
                     if (__ballot(live[0]) == 0) break;
 #pragma unroll
-                    for (uint32_t j = 0; j < PULL_UNR; ++j)
+                    for (uint32_t j = 0; j < UNR; ++j)
                         cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
 #pragma unroll
-                    for (uint32_t j = 0; j < PULL_UNR; ++j) {
+                    for (uint32_t j = 0; j < UNR; ++j) {
                         if (nb[j] == 0xFFFFFFFFu) continue;
                         st_relax++;
                         if (cn[j] < ocost && cn[j] <= rgmax) {
@@ -1216,11 +1206,11 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
             unsigned long long bs = GST_INF_WORD, bz = GST_INF_WORD;  // best via positive / zero-effect edges
             const uint32_t len = e1 - e0;
             const uint32_t steps = (len + A - 1) / A;
-            for (uint32_t s = 0; s < steps; s += PULL_UNR) {
-                uint32_t nb[PULL_UNR], wt[PULL_UNR], cn[PULL_UNR];
-                bool live[PULL_UNR];
+            for (uint32_t s = 0; s < steps; s += UNR) {
+                uint32_t nb[UNR], wt[UNR], cn[UNR];
+                bool live[UNR];
 #pragma unroll
-                for (uint32_t j = 0; j < PULL_UNR; ++j) {
+                for (uint32_t j = 0; j < UNR; ++j) {
                     const uint32_t e = e0 + (s + j) * A + s_arc;
                     const bool ok = lane_ok && e < e1;
                     const uint2 r = ok ? a.arc[e] : make_uint2(0xFFFFFFFFu, 0u);
@@ -1231,10 +1221,10 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
                 }
                 if (__ballot(live[0]) == 0) break;   // each lane's arcs run in increasing weight
 #pragma unroll
-                for (uint32_t j = 0; j < PULL_UNR; ++j)
+                for (uint32_t j = 0; j < UNR; ++j)
                     cn[j] = nb[j] != 0xFFFFFFFFu ? (uint32_t)(a.Tk[(uint64_t)nb[j] * m + rk] >> 32) : GST_INF_COST;
 #pragma unroll
-                for (uint32_t j = 0; j < PULL_UNR; ++j) {
+                for (uint32_t j = 0; j < UNR; ++j) {
                     if (nb[j] == 0xFFFFFFFFu) continue;
                     st_relax++;
                     // only a cheaper neighbour can lower (or tie) our cost, and only one still grown
@@ -1578,7 +1568,7 @@ __device__ __forceinline__ void grow_round_body(const GrowArgs& a, uint32_t bid,
 // 5 workgroups of 4 waves per CU must fit at once (the grid is sized for it): at most 96 VGPRs
 template <bool FLT>
 __global__ void __launch_bounds__(GROW_BLOCK) __attribute__((amdgpu_waves_per_eu(GST_GROW_WPE))) k_grow_round(GrowArgs a) {
-    grow_round_body<FLT, GST_PULL_UNR_SOLO>(a, blockIdx.x, gridDim.x, a.seq);
+    grow_round_body<FLT>(a, blockIdx.x, gridDim.x, a.seq);
 }
 
 // ---------------------------------------------------------------------------------------
